@@ -1,0 +1,211 @@
+/*
+ * ucy.h -- C ABI of the MI355X engine for the quasi-dynamic boundary-integral
+ * hot path of jdpmoore/unicycle (fortran/3d).
+ *
+ * Plain pointers and sizes only.  All pointers are host pointers unless the
+ * name ends in _dev (device pointers, stream-ordered on the context stream).
+ * Every function returns 0 on success and a negative status on failure; the
+ * message is available from ucy_last_error().  The library never aborts the
+ * process (the reference STOPs, e.g. ode45.f90:86-90; a Fortran host maps a
+ * nonzero status back to STOP).
+ *
+ * Layout conventions are the reference's:
+ *   - arrays of 3-vectors are Fortran (3,ns) arrays, i.e. x[3*i+k];
+ *   - the state vector is the reference's interleaved layout, 7 entries per
+ *     patch [ss, sd, ts, td, tn, log10(theta), log10(V)] (macros.f90:57-64)
+ *     followed by 12 per volume [e11..e33, s11..s33] (macros.f90:67-79);
+ *   - the kernel K is the physical (m x n) matrix stored row-major, which is
+ *     byte-for-byte the reference's column-major G(n,m) of greens.f90:116,
+ *     m = 3*Npatch + 6*Nvol receiver rows, n = 2*Npatch + 6*Nvol source
+ *     columns (greens.f90:109-111, 214-216, 266-271).
+ * Threading: one host thread per context; calls are ordered on the context's
+ * stream; a context is not reentrant.
+ */
+#ifndef UCY_H
+#define UCY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UCY_OK 0
+#define UCY_ERR_ARG -1
+#define UCY_ERR_HIP -2
+#define UCY_ERR_STATE -3
+#define UCY_ERR_UNDERFLOW -4   /* ode45.f90:86-90 "stepsize underflow in RKQS" */
+#define UCY_ERR_NONFINITE -5
+#define UCY_ERR_GEOMETRY -6    /* strainvolume.f90:1059-1072 parameter checks */
+#define UCY_ERR_LIMIT -7
+
+#define UCY_PREC_F64 0         /* K stored in binary64 */
+#define UCY_PREC_SPLIT_BF16 1  /* K stored as per-row-scaled bf16 hi+lo pair */
+
+typedef struct ucy_ctx ucy_ctx;
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* One context per GPU, matching one MPI rank of the reference. */
+ucy_ctx *ucy_create(int device, int flags);
+void ucy_destroy(ucy_ctx *ctx);
+const char *ucy_last_error(const ucy_ctx *ctx);
+const char *ucy_version(void);
+
+/* ---- Green's-function routines with the reference argument lists ------- */
+/* Evaluated on the GPU for one receiver against ns sources (parity hooks).
+ * Replaces okada92.f90:368-377 computeTractionKernelsOkada92. */
+int ucy_okada92_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                 const double dv[3], const double nv[3], int ns,
+                                 const double *y, const double *L, const double *W,
+                                 const double *strike, const double *dip, double s, double d,
+                                 double t, double G, double lambda, double *ts, double *td,
+                                 double *tn);
+/* Replaces okada92.f90:424-432 computeStressKernelsOkada92. */
+int ucy_okada92_stress_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                               const double dv[3], const double nv[3], int ns, const double *y,
+                               const double *L, const double *W, const double *strike,
+                               const double *dip, double s, double d, double t, double G,
+                               double lambda, double *s11, double *s12, double *s13,
+                               double *s22, double *s23, double *s33);
+/* Replaces nikkhoo15.f90:187-199 computeTractionKernelsNikkhoo15 (1-based i1..i3). */
+int ucy_nikkhoo15_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                   const double dv[3], const double nv[3], int ns,
+                                   const int *i1, const int *i2, const int *i3, int nVe,
+                                   const double *vertices, double s, double d, double t,
+                                   double G, double lambda, double *ts, double *td, double *tn);
+/* Replaces nikkhoo15.f90:250-262 computeStressKernelsNikkhoo15. */
+int ucy_nikkhoo15_stress_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                 const double dv[3], const double nv[3], int ns, const int *i1,
+                                 const int *i2, const int *i3, int nVe, const double *vertices,
+                                 double s, double d, double t, double G, double lambda,
+                                 double *s11, double *s12, double *s13, double *s22,
+                                 double *s23, double *s33);
+/* Replaces strainvolume.f90:4671-4684 computeTractionKernelsVerticalStrainVolume. */
+int ucy_strainvolume_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                      const double dv[3], const double nv[3], int ns,
+                                      const double *y, const double *L, const double *T,
+                                      const double *W, const double *strike, double e11p,
+                                      double e12p, double e13p, double e22p, double e23p,
+                                      double e33p, double G, double lambda, double *ts,
+                                      double *td, double *tn);
+/* Replaces strainvolume.f90:4743-4756 computeStressKernelsVerticalStrainVolume. */
+int ucy_strainvolume_stress_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                    const double dv[3], const double nv[3], int ns,
+                                    const double *y, const double *L, const double *T,
+                                    const double *W, const double *strike, double e11p,
+                                    double e12p, double e13p, double e22p, double e23p,
+                                    double e33p, double G, double lambda, double *s11,
+                                    double *s12, double *s13, double *s22, double *s23,
+                                    double *s33);
+
+/* Reference frames (host arithmetic, identical to the reference routines).
+ * Replace okada92.f90:59 computeReferenceSystemOkada92,
+ * nikkhoo15.f90:50 / stuart97.f90:44 computeReferenceSystemNikkhoo15/Stuart97,
+ * strainvolume.f90:135 computeReferenceSystemVerticalStrainVolume. */
+int ucy_reference_system_okada92(int ns, const double *x, const double *L, const double *W,
+                                 const double *strike, const double *dip, double *sv,
+                                 double *dv, double *nv, double *xc);
+int ucy_reference_system_nikkhoo15(int nVe, const double *v, int ns, const int *i1,
+                                   const int *i2, const int *i3, double *sv, double *dv,
+                                   double *nv, double *xc);
+int ucy_reference_system_strainvolume(int ns, const double *x, const double *L,
+                                      const double *W, const double *strike, double *sv,
+                                      double *dv, double *nv, double *xc);
+
+/* ---- model definition (the reference's input-deck sections) ------------ */
+/* ratestate.f90:1092 "elastic moduli" lambda, mu (strike/dip in radians below). */
+int ucy_set_elastic(ucy_ctx *ctx, double lambda, double mu);
+/* ratestate.f90:1142-1176 rectangular patches: upper-left corner x(3,ns), length,
+ * width, strike, dip (radians). */
+int ucy_set_rectangles(ucy_ctx *ctx, int ns, const double *x, const double *length,
+                       const double *width, const double *strike, const double *dip);
+/* ratestate.f90:1252-1290 triangles: 1-based vertex indices, vertices v(3,nVe). */
+int ucy_set_triangles(ucy_ctx *ctx, int ns, const int *i1, const int *i2, const int *i3,
+                      int nVe, const double *v);
+/* viscouscycles.f90 strain volumes: upper-left corner x(3,ns), length, thickness,
+ * width, strike (radians). */
+int ucy_set_volumes(ucy_ctx *ctx, int ns, const double *x, const double *length,
+                    const double *thickness, const double *width, const double *strike);
+/* Friction per patch in element order (rectangles then triangles), as the
+ * deck's PATCH_ELEMENT_STRUCT (types.f90:35-40); rake in radians. */
+int ucy_set_friction(ucy_ctx *ctx, int np, const double *Vpl, const double *rake,
+                     const double *mu0, const double *sig, const double *a, const double *b,
+                     const double *L, const double *Vo, const double *damping);
+/* Nonlinear Maxwell rheology per volume (viscouscycles.f90:963-995, 769-803):
+ * prefactor gammadot0, power n, background strain rate e(6,nv), and the
+ * initial-stress law parameters Gm, Q, R, To. */
+int ucy_set_rheology(ucy_ctx *ctx, int nv, const double *gammadot0, const double *npower,
+                     const double *edot_bg, const double *Gm, const double *Q, const double *R,
+                     const double *To);
+
+/* ---- assembly: buildG (greens.f90:73-275) -------------------------------- */
+int ucy_build_kernel(ucy_ctx *ctx, int precision);
+int ucy_kernel_shape(const ucy_ctx *ctx, int64_t *m, int64_t *n);
+/* copy K (m x n row-major, binary64) back to the host */
+int ucy_get_kernel(ucy_ctx *ctx, double *K);
+/* load an externally assembled K (m x n row-major) */
+int ucy_set_kernel(ucy_ctx *ctx, const double *K, int64_t m, int64_t n);
+/* device pointer to K (binary64, row-major) for zero-copy callers */
+const double *ucy_kernel_dev(const ucy_ctx *ctx);
+
+/* ---- RHS, matvec and the Cash-Karp stepper ------------------------------ */
+/* DGEMV("T") of ratestate.f90:719-720: t(m) = K v(n). */
+int ucy_matvec_f64(ucy_ctx *ctx, const double *v, double *t);
+int ucy_matvec_f64_dev(ucy_ctx *ctx, const double *v_dev, double *t_dev);
+/* initStateVector (ratestate.f90:548-610; variant 1 = viscouscycles.f90:710-820) */
+int ucy_init_state(ucy_ctx *ctx, int variant, double *y);
+/* odefun(n,time,y,dydt) of ratestate.f90:624-778 (+ viscouscycles.f90:933-1080);
+ * vmax receives the maximum slip velocity (may be NULL). */
+int ucy_odefun(ucy_ctx *ctx, int n, double time, const double *y, double *dydt, double *vmax);
+/* RKCKm(n,t,yin,dydt,h,yout,yerr) of ode45.f90:116-202 */
+int ucy_rkck(ucy_ctx *ctx, int n, double t, const double *yin, const double *dydt, double h,
+             double *yout, double *yerr);
+/* RKQSm(n,t,y,dydt,yscal,htry,hdid,hnext) of ode45.f90:51-108; epsilon and
+ * maximumTimeStep as the module variables of ode45.f90:40-43. */
+int ucy_rkqs(ucy_ctx *ctx, int n, double *t, double *y, double *dydt, const double *yscal,
+             double htry, double *hdid, double *hnext, double epsilon, double max_step);
+
+/* Main loop of ratestate.f90:307-328 run device-resident: one graph-captured
+ * RK45 attempt per replay, state kept in HBM.  y is read at entry and written
+ * at exit.  Optional logs receive (time, dt_done) of accepted steps. */
+typedef struct {
+    double time;        /* simulated time reached */
+    double dt_next;     /* next step size */
+    int64_t steps;      /* accepted steps */
+    int64_t attempts;   /* RK45 attempts (accepted + rejected) */
+    int64_t rhs_evals;  /* odefun evaluations */
+    double vmax;        /* max slip velocity at exit */
+    double wall_seconds;
+} ucy_run_stats;
+int ucy_integrate(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
+                  double epsilon, double max_step, double *t_log, double *dt_log,
+                  int64_t log_cap, ucy_run_stats *stats);
+
+/* ---- batched ensembles (no counterpart in the reference) --------------- */
+/* Independent fault instances in geometry groups: each group shares one
+ * kernel K; every instance has its own friction parameters, state, time
+ * and step size.  Stage GEMMs run on MFMA with K stored as split bf16. */
+typedef struct ucy_ensemble ucy_ensemble;
+ucy_ensemble *ucy_ensemble_create(ucy_ctx *ctx, int precision);
+void ucy_ensemble_destroy(ucy_ensemble *ens);
+/* add a group: rectangles geometry + n_inst instances of friction (each array
+ * n_inst*np, instance-major) and initial states y0 (n_inst*7*np); returns the
+ * group index or <0 */
+int ucy_ensemble_add_group(ucy_ensemble *ens, int np, const double *x, const double *length,
+                           const double *width, const double *strike, const double *dip,
+                           double lambda, double mu, int n_inst, const double *Vpl,
+                           const double *rake, const double *a, const double *b,
+                           const double *sig, const double *L, const double *damping,
+                           const double *y0);
+int ucy_ensemble_build(ucy_ensemble *ens);
+/* advance every instance until t >= t_end (or max_attempts attempts) */
+int ucy_ensemble_run(ucy_ensemble *ens, double t_end, int64_t max_attempts, double epsilon,
+                     double max_step, ucy_run_stats *stats);
+int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,
+                           double *dt_next, int64_t *steps, int64_t *attempts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,139 @@
+/*
+ * ucy_oracle.h -- CPU oracle for the Unicycle quasi-dynamic hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product is compared against; it is never linked into, loaded by or called
+ * from the product library (unicycle_amd/csrc).  Only tests/, the smoke()
+ * entry point and the cpu_baseline leg of bench.py may load it.
+ *
+ * Every function restates one routine of jdpmoore/unicycle fortran/3d/src
+ * (file:line cited at each definition in ucy_oracle.c) in plain C with the
+ * reference's rounding points: the Okada chain rounds its inputs and outputs
+ * to binary32 exactly where okada92.f90 / dc3d.f do, everything else is IEEE
+ * binary64 with no contraction (build with -ffp-contract=off).
+ *
+ * Conventions follow the reference: arrays of 3-vectors are column-major
+ * (3,ns) i.e. x[3*i+k]; the kernel K is the physical (m x n) matrix stored
+ * row-major, which is the reference's Fortran G(n,m) column-major array.
+ */
+#ifndef UCY_ORACLE_H
+#define UCY_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- rectangles (Okada 1992) ------------------------------------------ */
+void orc_refsys_okada92(int ns, const double *x, const double *L, const double *W,
+                        const double *strike, const double *dip,
+                        double *sv, double *dv, double *nv, double *xc);
+int orc_dc3d(float alpha, float x, float y, float z, float depth, float dip,
+             float al1, float al2, float aw1, float aw2,
+             float disl1, float disl2, float disl3, float u[12]);
+void orc_stress_okada92(double x1, double x2, double x3, double q1, double q2, double q3,
+                        double L, double W, double strike, double dipd,
+                        double s, double d, double t, double G, double lambda,
+                        double sig[6]);
+void orc_traction_kernels_okada92(const double x[3], const double sv[3], const double dv[3],
+                                  const double nv[3], int ns, const double *y,
+                                  const double *L, const double *W, const double *strike,
+                                  const double *dip, double s, double d, double t,
+                                  double G, double lambda, double *ts, double *td, double *tn);
+void orc_stress_kernels_okada92(const double x[3], const double sv[3], const double dv[3],
+                                const double nv[3], int ns, const double *y,
+                                const double *L, const double *W, const double *strike,
+                                const double *dip, double s, double d, double t,
+                                double G, double lambda, double *s11, double *s12,
+                                double *s13, double *s22, double *s23, double *s33);
+
+/* ---- triangles (Nikkhoo & Walter 2015) --------------------------------- */
+void orc_refsys_nikkhoo15(int nVe, const double *v, int ns, const int *i1, const int *i2,
+                          const int *i3, double *sv, double *dv, double *nv, double *xc);
+void orc_strain_in_halfspace(double stress[6], double strain[6], double X, double Y, double Z,
+                             const double P1[3], const double P2[3], const double P3[3],
+                             double Ss, double Ds, double Ts, double mu, double lambda);
+void orc_traction_kernels_nikkhoo15(const double x[3], const double sv[3], const double dv[3],
+                                    const double nv[3], int ns, const int *i1, const int *i2,
+                                    const int *i3, int nVe, const double *vert,
+                                    double s, double d, double t, double G, double lambda,
+                                    double *ts, double *td, double *tn);
+void orc_stress_kernels_nikkhoo15(const double x[3], const double sv[3], const double dv[3],
+                                  const double nv[3], int ns, const int *i1, const int *i2,
+                                  const int *i3, int nVe, const double *vert,
+                                  double s, double d, double t, double G, double lambda,
+                                  double *s11, double *s12, double *s13, double *s22,
+                                  double *s23, double *s33);
+
+/* ---- strain volumes (Barbot, Moore & Lambert 2017) --------------------- */
+void orc_refsys_strainvolume(int ns, const double *x, const double *L, const double *W,
+                             const double *strike, double *sv, double *dv, double *nv,
+                             double *xc);
+int orc_stress_strainvolume(double x1, double x2, double x3, double q1, double q2, double q3,
+                            double L, double T, double W, double theta,
+                            const double epsp[6], double G, double nu, double sig[6]);
+int orc_traction_kernels_strainvolume(const double x[3], const double sv[3], const double dv[3],
+                                      const double nv[3], int ns, const double *y,
+                                      const double *L, const double *T, const double *W,
+                                      const double *strike, const double epsp[6],
+                                      double G, double lambda,
+                                      double *ts, double *td, double *tn);
+int orc_stress_kernels_strainvolume(const double x[3], const double sv[3], const double dv[3],
+                                    const double nv[3], int ns, const double *y,
+                                    const double *L, const double *T, const double *W,
+                                    const double *strike, const double epsp[6],
+                                    double G, double lambda, double *s11, double *s12,
+                                    double *s13, double *s22, double *s23, double *s33);
+
+/* ---- assembly (greens.f90 buildG) -------------------------------------- */
+typedef struct {
+    double lambda, mu;
+    int nrect;  const double *rx, *rL, *rW, *rstrike, *rdip;   /* rectangles */
+    int ntri;   const int *i1, *i2, *i3; int nve; const double *vert; /* triangles */
+    int nvol;   const double *vx, *vL, *vT, *vW, *vstrike;     /* volumes */
+} orc_mesh;
+/* K row-major (m x n); m = 3*(nrect+ntri)+6*nvol, n = 2*(nrect+ntri)+6*nvol.
+ * nthreads > 1 splits receivers across OpenMP threads. Returns 0 or <0. */
+int orc_buildG(const orc_mesh *mesh, double *K, int nthreads);
+void orc_matvec(int m, int n, const double *K, const double *v, double *t, int nthreads);
+
+/* ---- rate-and-state RHS (ratestate.f90 odefun) ------------------------- */
+typedef struct {
+    int n;                 /* number of patches */
+    const double *Vpl, *rake, *mu0, *sig, *a, *b, *L, *Vo, *damping;
+    int nvol;              /* number of strain volumes (viscouscycles) */
+    const double *gammadot0, *npower, *edot_bg; /* per volume: prefactor, exponent,
+                                                   background strain rate (6 each) */
+    const double *Gm, *Qa, *Rgas, *To;          /* per volume: initial-stress law */
+    int ic_variant;        /* 0: ratestate.f90 IC, 1: viscouscycles.f90 IC */
+    const double *K;       /* m x n row-major */
+    int m, ncol;
+    int nthreads;
+} orc_system;
+void orc_init_state(const orc_system *sys, double *y);
+/* y, dydt length 7*n (+12*nvol); returns vMax */
+double orc_odefun(const orc_system *sys, const double *y, double *dydt,
+                  double *vwork /*ncol*/, double *twork /*m*/);
+
+/* ---- Cash-Karp RK45 and step control (ode45.f90) ----------------------- */
+typedef void (*orc_rhs_fn)(void *ctx, const double *y, double *dydt);
+void orc_rkck(int n, const double *yin, const double *dydt, double h, double *yout,
+              double *yerr, double *work /* 7*n */, orc_rhs_fn f, void *ctx);
+/* returns 0 on success, -1 on step-size underflow */
+int orc_rkqs(int n, double *t, double *y, double *dydt, const double *yscal,
+             double htry, double *hdid, double *hnext, double epsilon, double max_step,
+             double *work /* 9*n */, orc_rhs_fn f, void *ctx, int *n_attempts);
+
+/* main loop of ratestate.f90:307-328.  Records accepted (time, dt) pairs.
+ * Returns number of accepted steps or <0 on failure. */
+typedef struct {
+    double time, dt_next;
+    long steps, attempts, rhs_evals;
+} orc_run_stats;
+long orc_integrate(const orc_system *sys, double *y, double interval, double dt0,
+                   long max_steps, double epsilon, double max_step,
+                   double *t_log, double *dt_log, long log_cap, orc_run_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,200 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libucyoracle.so) and of the
+reference checker (oracle/_ref/libucyref.so) -- test infrastructure only.
+
+The oracle is built on demand with ``make -C oracle oracle`` (gcc, seconds).
+The reference checker exists only where the reference tree was available at
+build time; tests that need it skip otherwise and rely on the committed
+golden fixtures under tests/golden/.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libucyoracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libucyref.so")
+
+D = C.c_double
+I = C.c_int
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+
+def p(a):
+    return a.ctypes.data_as(PD)
+
+
+def pi(a):
+    return a.ctypes.data_as(PI)
+
+
+def f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("lambda_", D), ("mu", D),
+                ("nrect", I), ("rx", PD), ("rL", PD), ("rW", PD), ("rstrike", PD), ("rdip", PD),
+                ("ntri", I), ("i1", PI), ("i2", PI), ("i3", PI), ("nve", I), ("vert", PD),
+                ("nvol", I), ("vx", PD), ("vL", PD), ("vT", PD), ("vW", PD), ("vstrike", PD)]
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [("n", I), ("Vpl", PD), ("rake", PD), ("mu0", PD), ("sig", PD), ("a", PD),
+                ("b", PD), ("L", PD), ("Vo", PD), ("damping", PD),
+                ("nvol", I), ("gammadot0", PD), ("npower", PD), ("edot_bg", PD),
+                ("Gm", PD), ("Qa", PD), ("Rgas", PD), ("To", PD), ("ic_variant", I),
+                ("K", PD), ("m", I), ("ncol", I), ("nthreads", I)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("time", D), ("dt_next", D), ("steps", C.c_long), ("attempts", C.c_long),
+                ("rhs_evals", C.c_long)]
+
+
+_ORC = None
+
+
+def oracle() -> C.CDLL:
+    global _ORC
+    if _ORC is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"],
+                           check=True)
+        lib = C.CDLL(ORACLE_SO)
+        lib.orc_dc3d.restype = I
+        lib.orc_buildG.restype = I
+        lib.orc_odefun.restype = D
+        lib.orc_rkqs.restype = I
+        lib.orc_integrate.restype = C.c_long
+        lib.orc_stress_strainvolume.restype = I
+        _ORC = lib
+    return _ORC
+
+
+def ref():
+    if not os.path.exists(REF_SO):
+        return None
+    return C.CDLL(REF_SO)
+
+
+class Mesh:
+    """Keeps the numpy arrays alive behind an OrcMesh."""
+
+    def __init__(self, lambda_, mu, rect=None, tri=None, vol=None):
+        self.keep = []
+        m = OrcMesh()
+        m.lambda_, m.mu = lambda_, mu
+        if rect is not None:
+            x, L, W, st, dp = [f64(a) for a in rect]
+            self.keep += [x, L, W, st, dp]
+            m.nrect = L.size
+            m.rx, m.rL, m.rW, m.rstrike, m.rdip = p(x), p(L), p(W), p(st), p(dp)
+        if tri is not None:
+            i1, i2, i3 = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in tri[:3]]
+            v = f64(tri[3])
+            self.keep += [i1, i2, i3, v]
+            m.ntri = i1.size
+            m.i1, m.i2, m.i3, m.nve, m.vert = pi(i1), pi(i2), pi(i3), v.size // 3, p(v)
+        if vol is not None:
+            x, L, T, W, st = [f64(a) for a in vol]
+            self.keep += [x, L, T, W, st]
+            m.nvol = L.size
+            m.vx, m.vL, m.vT, m.vW, m.vstrike = p(x), p(L), p(T), p(W), p(st)
+        self.c = m
+        self.np = m.nrect + m.ntri
+        self.nvol = m.nvol
+        self.shape = (3 * self.np + 6 * self.nvol, 2 * self.np + 6 * self.nvol)
+
+
+def buildG(mesh: Mesh, nthreads: int = 1) -> np.ndarray:
+    K = np.empty(mesh.shape)
+    rc = oracle().orc_buildG(C.byref(mesh.c), p(K), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_buildG failed {rc}")
+    return K
+
+
+class System:
+    def __init__(self, fric, K, rheo=None, ic_variant=0, nthreads=1):
+        self.keep = []
+        s = OrcSystem()
+        arrs = [f64(a) for a in fric]
+        self.keep += arrs
+        s.n = arrs[0].size
+        (s.Vpl, s.rake, s.mu0, s.sig, s.a, s.b, s.L, s.Vo, s.damping) = [p(a) for a in arrs]
+        if rheo is not None:
+            r = [f64(a) for a in rheo]
+            self.keep += r
+            s.nvol = r[0].size
+            (s.gammadot0, s.npower, s.edot_bg, s.Gm, s.Qa, s.Rgas, s.To) = [p(a) for a in r]
+        s.ic_variant = ic_variant
+        self.K = f64(K)
+        self.keep.append(self.K)
+        s.K = p(self.K)
+        s.m, s.ncol = self.K.shape
+        s.nthreads = nthreads
+        self.c = s
+        self.dim = 7 * s.n + 12 * s.nvol
+
+    def init_state(self):
+        y = np.empty(self.dim)
+        oracle().orc_init_state(C.byref(self.c), p(y))
+        return y
+
+    def odefun(self, y):
+        dydt = np.empty(self.dim)
+        v = np.empty(self.c.ncol)
+        t = np.empty(self.c.m)
+        vmax = oracle().orc_odefun(C.byref(self.c), p(f64(y)), p(dydt), p(v), p(t))
+        return dydt, vmax, v, t
+
+    def integrate(self, y0, interval, dt0=1e-3, max_steps=10**6, epsilon=1e-6,
+                  max_step=1.7976931348623158e308, log_cap=0):
+        y = f64(y0).copy()
+        tl = np.empty(max(log_cap, 1))
+        dl = np.empty(max(log_cap, 1))
+        st = OrcStats()
+        rc = oracle().orc_integrate(C.byref(self.c), p(y), D(interval), D(dt0), C.c_long(max_steps),
+                                    D(epsilon), D(max_step), p(tl), p(dl), C.c_long(log_cap),
+                                    C.byref(st))
+        n = min(st.steps, log_cap)
+        return y, st, tl[:n].copy(), dl[:n].copy(), rc
+
+
+RHS_FN = C.CFUNCTYPE(None, C.c_void_p, PD, PD)
+
+
+def rkqs(n, t, y, dydt, yscal, htry, f, epsilon=1e-6, max_step=1.7976931348623158e308):
+    """Oracle RKQSm with a Python RHS f(y)->dydt (small problems only)."""
+    def cb(ctx, yp, dp):
+        yy = np.ctypeslib.as_array(yp, shape=(n,))
+        dd = np.ctypeslib.as_array(dp, shape=(n,))
+        dd[:] = f(yy.copy())
+    cbf = RHS_FN(cb)
+    w = np.empty(9 * n)
+    tt = D(t)
+    hdid, hnext = D(), D()
+    att = I()
+    y = f64(y).copy()
+    rc = oracle().orc_rkqs(n, C.byref(tt), p(y), p(f64(dydt)), p(f64(yscal)), D(htry),
+                           C.byref(hdid), C.byref(hnext), D(epsilon), D(max_step), p(w), cbf,
+                           None, C.byref(att))
+    return rc, tt.value, y, hdid.value, hnext.value, att.value
+
+
+def rkck(n, y, dydt, h, f):
+    def cb(ctx, yp, dp):
+        yy = np.ctypeslib.as_array(yp, shape=(n,))
+        dd = np.ctypeslib.as_array(dp, shape=(n,))
+        dd[:] = f(yy.copy())
+    cbf = RHS_FN(cb)
+    yout = np.empty(n)
+    yerr = np.empty(n)
+    w = np.empty(7 * n)
+    oracle().orc_rkck(n, p(f64(y)), p(f64(dydt)), D(h), p(yout), p(yerr), p(w), cbf, None)
+    return yout, yerr

@@ -1,0 +1,106 @@
+"""ctypes binding of libucy.so (the C ABI declared in include/ucy.h).
+
+The HIP library is the only compute path: if it is missing, importing this
+module raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("UCY_LIB", os.path.join(_HERE, "libucy.so"))
+
+_d = C.c_double
+_i = C.c_int
+_i64 = C.c_int64
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("dt_next", C.c_double),
+        ("steps", C.c_int64),
+        ("attempts", C.c_int64),
+        ("rhs_evals", C.c_int64),
+        ("vmax", C.c_double),
+        ("wall_seconds", C.c_double),
+    ]
+
+
+_SIGS = {
+    "ucy_create": (_vp, [_i, _i]),
+    "ucy_destroy": (None, [_vp]),
+    "ucy_last_error": (C.c_char_p, [_vp]),
+    "ucy_version": (C.c_char_p, []),
+    "ucy_okada92_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd,
+                                          _d, _d, _d, _d, _d, _pd, _pd, _pd]),
+    "ucy_okada92_stress_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd, _pd,
+                                        _d, _d, _d, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_nikkhoo15_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _i, _pd,
+                                            _d, _d, _d, _d, _d, _pd, _pd, _pd]),
+    "ucy_nikkhoo15_stress_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _i, _pd,
+                                          _d, _d, _d, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_strainvolume_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd,
+                                               _pd, _d, _d, _d, _d, _d, _d, _d, _d, _pd, _pd,
+                                               _pd]),
+    "ucy_strainvolume_stress_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd,
+                                             _pd, _d, _d, _d, _d, _d, _d, _d, _d, _pd, _pd, _pd,
+                                             _pd, _pd, _pd]),
+    "ucy_reference_system_okada92": (_i, [_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_reference_system_nikkhoo15": (_i, [_i, _pd, _i, _pi, _pi, _pi, _pd, _pd, _pd, _pd]),
+    "ucy_reference_system_strainvolume": (_i, [_i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_set_elastic": (_i, [_vp, _d, _d]),
+    "ucy_set_rectangles": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_set_triangles": (_i, [_vp, _i, _pi, _pi, _pi, _i, _pd]),
+    "ucy_set_volumes": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_set_friction": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_set_rheology": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_build_kernel": (_i, [_vp, _i]),
+    "ucy_kernel_shape": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ucy_get_kernel": (_i, [_vp, _pd]),
+    "ucy_set_kernel": (_i, [_vp, _pd, _i64, _i64]),
+    "ucy_kernel_dev": (_vp, [_vp]),
+    "ucy_matvec_f64": (_i, [_vp, _pd, _pd]),
+    "ucy_matvec_f64_dev": (_i, [_vp, _vp, _vp]),
+    "ucy_init_state": (_i, [_vp, _i, _pd]),
+    "ucy_odefun": (_i, [_vp, _i, _d, _pd, _pd, _pd]),
+    "ucy_rkck": (_i, [_vp, _i, _d, _pd, _pd, _d, _pd, _pd]),
+    "ucy_rkqs": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _d, _pd, _pd, _d, _d]),
+    "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _i64, C.POINTER(RunStats)]),
+    "ucy_ensemble_create": (_vp, [_vp, _i]),
+    "ucy_ensemble_destroy": (None, [_vp]),
+    "ucy_ensemble_add_group": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _d, _d, _i, _pd, _pd, _pd,
+                                    _pd, _pd, _pd, _pd, _pd]),
+    "ucy_ensemble_build": (_i, [_vp]),
+    "ucy_ensemble_run": (_i, [_vp, _d, _i64, _d, _d, C.POINTER(RunStats)]),
+    "ucy_ensemble_get_state": (_i, [_vp, _i, _pd, _pd, _pd, C.POINTER(_i64), C.POINTER(_i64)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"libucy.so not found at {path}: build it with "
+            f"'make -C unicycle_amd/csrc' (there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB

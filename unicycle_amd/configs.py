@@ -1,0 +1,119 @@
+"""Synthetic fault models (seeded) in the reference's units and layouts.
+
+Units follow the reference input decks: lengths in m, stresses and moduli in
+MPa, velocities in m/s, time in s.  Angles handed to the engine are radians
+(the deck's degrees are converted at ratestate.f90:1174-1176).
+
+thrust_fault
+    planar dipping thrust of nx (along strike) x nz (down dip) Okada
+    rectangles; aging-law rate-and-state with a velocity-weakening ellipse,
+    radiation damping mu/(2 Vs).  nx=64, nz=32, 2 km patches is the
+    2048-patch single-instance model; the ensemble groups use the same rules
+    at 1024/2048/4096/8192 patches.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+YEAR = 3.15576e7  # s (Julian year)
+
+
+@dataclass
+class FaultModel:
+    lambda_: float
+    mu: float
+    x: np.ndarray          # (N,3) upper-left corners, north/east/down [m]
+    length: np.ndarray
+    width: np.ndarray
+    strike: np.ndarray     # rad
+    dip: np.ndarray        # rad
+    Vpl: np.ndarray
+    rake: np.ndarray       # rad
+    mu0: np.ndarray
+    sig: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    L: np.ndarray
+    Vo: np.ndarray
+    damping: np.ndarray
+    nx: int = 0
+    nz: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+    def friction(self):
+        return (self.Vpl, self.rake, self.mu0, self.sig, self.a, self.b, self.L, self.Vo,
+                self.damping)
+
+    def geometry(self):
+        return (self.x, self.length, self.width, self.strike, self.dip)
+
+
+def thrust_fault(nx: int = 64, nz: int = 32, patch: float = 2000.0, dip_deg: float = 30.0,
+                 top: float = 500.0, mu: float = 30e3, nu: float = 0.25, vs: float = 3000.0,
+                 a: float = 0.010, b_in: float = 0.014, b_out: float = 0.006,
+                 ellipse: tuple | None = (40e3, 20e3), L: float = 0.01, sig: float = 100.0,
+                 Vo: float = 1e-6, mu0: float = 0.6, Vpl: float = 1e-9,
+                 rake_deg: float = 90.0) -> FaultModel:
+    """Planar thrust, strike north, dipping east; patch (i,j) row-major with
+    i along strike fastest."""
+    lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
+    dip = np.deg2rad(dip_deg)
+    i = np.tile(np.arange(nx), nz)
+    j = np.repeat(np.arange(nz), nx)
+    N = nx * nz
+    x = np.empty((N, 3))
+    x[:, 0] = i * patch                                   # north
+    x[:, 1] = j * patch * np.cos(dip)                     # east (down-dip)
+    x[:, 2] = top + j * patch * np.sin(dip)               # depth
+    # velocity-weakening ellipse centred on the fault
+    sa = (i + 0.5) * patch - 0.5 * nx * patch
+    sd = (j + 0.5) * patch - 0.5 * nz * patch
+    if ellipse is None:
+        ax, az = 0.625 * 0.5 * nx * patch, 0.625 * 0.5 * nz * patch
+    else:
+        ax, az = ellipse
+    inside = (sa / ax) ** 2 + (sd / az) ** 2 < 1.0
+    one = np.ones(N)
+    return FaultModel(
+        lambda_=lam, mu=mu, x=x, length=patch * one, width=patch * one,
+        strike=0.0 * one, dip=dip * one, Vpl=Vpl * one, rake=np.deg2rad(rake_deg) * one,
+        mu0=mu0 * one, sig=sig * one, a=a * one, b=np.where(inside, b_in, b_out),
+        L=L * one, Vo=Vo * one, damping=mu / (2.0 * vs) * one, nx=nx, nz=nz,
+        meta={"inside": inside})
+
+
+def perturbed_initial_state(model: FaultModel, y0: np.ndarray, seed: int,
+                            amplitude: float = 1e-3) -> np.ndarray:
+    """Initial slip rate Vpl*(1 + amplitude*N(0,1)) with the state variable at
+    steady state theta = L/V; tractions as the reference initial state."""
+    rng = np.random.default_rng(seed)
+    y = y0.copy()
+    V = model.Vpl * (1.0 + amplitude * rng.standard_normal(model.n))
+    y[6:7 * model.n:7] = np.log10(V)
+    y[5:7 * model.n:7] = np.log10(model.L / V)
+    return y
+
+
+ENSEMBLE_GROUPS = ((32, 32), (64, 32), (64, 64), (128, 64))  # 1024, 2048, 4096, 8192 patches
+
+
+def ensemble_instances(base: FaultModel, n_inst: int, seed: int):
+    """Per-instance friction draws of the uneven-size ensemble: in the
+    weakening zone a-b ~ U(-0.006,-0.002) (a kept, b = a - (a-b)); everywhere
+    L ~ lognormal(ln 0.01 m, 0.4), sigma ~ U(50,150) MPa."""
+    rng = np.random.default_rng(seed)
+    N = base.n
+    inside = base.meta["inside"]
+    amb = rng.uniform(-0.006, -0.002, size=(n_inst, 1))
+    a = np.broadcast_to(base.a, (n_inst, N)).copy()
+    b = np.where(inside[None, :], a - amb, np.broadcast_to(base.b, (n_inst, N)))
+    L = np.exp(np.log(0.01) + 0.4 * rng.standard_normal((n_inst, 1))) * np.ones((1, N))
+    sig = rng.uniform(50.0, 150.0, size=(n_inst, 1)) * np.ones((1, N))
+    seeds = rng.integers(0, 2**31 - 1, size=n_inst)
+    return {"a": a, "b": b, "L": L, "sig": sig, "seeds": seeds}

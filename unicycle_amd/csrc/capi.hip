@@ -1,0 +1,968 @@
+// capi.hip -- host side of the C ABI declared in include/ucy.h.
+//
+// Context = one GPU (one MPI rank of the reference).  Geometry and friction
+// arrive in the reference's layouts; per-element transcendental constants
+// (frames, cos/sin of strike, DCCON0 dip terms, Vpl*cos(rake)) are formed
+// here with the C library exactly as the reference forms them, so the device
+// kernels only see the binary64 values the reference would.
+//
+// Build note: compiled with -fno-builtin-sin -fno-builtin-cos so the host
+// compiler does not fuse sin/cos pairs into sincos, whose results differ in
+// the last bit from separate sin and cos calls.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/ucy.h"
+#include "ucy_internal.h"
+#include "ucy_ext.h"
+
+using namespace ucy;
+
+namespace ucy {
+void launch_commit(hipStream_t st, int dim, const RkBuffers &b, Control *ctl);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ucy_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // elastic moduli
+    double lambda = 0, mu = 0;
+    bool have_elastic = false;
+    // rectangles
+    int nrect = 0;
+    std::vector<double> rx, rL, rW, rstrike, rdip;
+    // triangles
+    int ntri = 0, nve = 0;
+    std::vector<int> i1, i2, i3;
+    std::vector<double> vert;
+    // volumes
+    int nvol = 0;
+    std::vector<double> vx, vL, vT, vW, vstrike;
+    // friction / rheology (element order)
+    std::vector<double> Vpl, rake, mu0, sig, a, b, Lc, Vo, damping;
+    std::vector<double> gd0, npow, ebg, Gm, Qa, Rg, To;
+    bool have_friction = false, have_rheology = false;
+    // device kernel
+    double *K = nullptr;
+    int64_t m = 0, n = 0;
+    // device per-element data
+    Friction *d_fr = nullptr;
+    Rheology *d_rh = nullptr;
+    bool params_dirty = true;
+    // work buffers
+    double *work = nullptr;
+    size_t work_dim = 0;
+    RkBuffers rb{};
+    Control *d_ctl = nullptr;
+    Control *h_ctl = nullptr;  // pinned
+    // captured attempt
+    hipGraphExec_t graph = nullptr;
+    // scratch for the per-kernel hooks
+    DevBuf hook;
+};
+
+static int fail(ucy_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(c, UCY_ERR_HIP, "%s failed: %s (%s:%d)", #call,                   \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                       \
+    } while (0)
+
+static int ensure_buf(ucy_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes) return 0;
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    HIPCHK(hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// host-side reference arithmetic
+// ---------------------------------------------------------------------------
+static const double kRad2Deg = 57.29577951308232;  // okada92.f90:133
+
+static OkadaMedium make_medium(double G, double lambda)
+{
+    OkadaMedium M{};
+    const float alpha = (float)((lambda + G) / (lambda + 2.0 * G));  // okada92.f90:138
+    const double a = (double)alpha;                                 // dc3d.f:45
+    M.alp1 = (1.0 - a) / 2.0;
+    M.alp2 = a / 2.0;
+    M.alp3 = (1.0 - a) / a;
+    M.alp4 = 1.0 - a;
+    M.alp5 = a;
+    M.G = G;
+    M.lambda = lambda;
+    return M;
+}
+
+static OkadaSource make_source(const double *x, double L, double W, double strike, double dipd)
+{
+    OkadaSource S{};
+    S.q1 = x[0];
+    S.q2 = x[1];
+    S.csst = cos(strike);
+    S.ssst = sin(strike);
+    S.depth = (float)x[2];
+    S.al2 = (float)L;
+    S.aw1 = -(float)W;
+    const double ddip = (double)(float)(dipd * kRad2Deg);  // okada92.f90:153, dc3d.f:46
+    const double p18 = 6.283185307179586 / 360.0;          // dc3d.f:531-533
+    double sd = sin(ddip * p18), cd = cos(ddip * p18);
+    if (fabs(cd) < 1.0e-6) {
+        cd = 0.0;
+        if (sd > 0.0) sd = 1.0;
+        if (sd < 0.0) sd = -1.0;
+    }
+    S.sd = sd;
+    S.cd = cd;
+    S.sdsd = sd * sd;
+    S.cdcd = cd * cd;
+    S.sdcd = sd * cd;
+    return S;
+}
+
+// okada92.f90:59-87
+static void refsys_okada(int ns, const double *x, const double *L, const double *W,
+                         const double *strike, const double *dip, double *sv, double *dv,
+                         double *nv, double *xc)
+{
+    for (int i = 0; i < ns; i++) {
+        const double cs = cos(strike[i]), ss = sin(strike[i]);
+        const double cdp = cos(dip[i]), sdp = sin(dip[i]);
+        sv[3 * i] = cs;
+        sv[3 * i + 1] = ss;
+        sv[3 * i + 2] = 0.0;
+        dv[3 * i] = +ss * cdp;
+        dv[3 * i + 1] = -cs * cdp;
+        dv[3 * i + 2] = -sdp;
+        nv[3 * i] = -ss * sdp;
+        nv[3 * i + 1] = +cs * sdp;
+        nv[3 * i + 2] = -cdp;
+        for (int k = 0; k < 3; k++)
+            xc[3 * i + k] = x[3 * i + k] + L[i] / 2 * sv[3 * i + k] - W[i] / 2 * dv[3 * i + k];
+    }
+}
+
+// nikkhoo15.f90:50-101 (identical to stuart97.f90:44-95); 1-based indices
+static void refsys_triangle(int nVe, const double *v, int ns, const int *i1, const int *i2,
+                            const int *i3, double *sv, double *dv, double *nv, double *xc)
+{
+    (void)nVe;
+    for (int j = 0; j < ns; j++) {
+        const double *A = v + 3 * (i1[j] - 1), *B = v + 3 * (i2[j] - 1), *C = v + 3 * (i3[j] - 1);
+        double n0 = (B[1] - A[1]) * (C[2] - A[2]) - (B[2] - A[2]) * (C[1] - A[1]);
+        double n1 = (B[2] - A[2]) * (C[0] - A[0]) - (B[0] - A[0]) * (C[2] - A[2]);
+        double n2 = (B[0] - A[0]) * (C[1] - A[1]) - (B[1] - A[1]) * (C[0] - A[0]);
+        const double norm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        n0 = n0 / norm;
+        n1 = n1 / norm;
+        n2 = n2 / norm;
+        if (0.0 < n2) {
+            n0 = -n0;
+            n1 = -n1;
+            n2 = -n2;
+        }
+        double *N = nv + 3 * j, *S = sv + 3 * j, *D = dv + 3 * j;
+        N[0] = n0;
+        N[1] = n1;
+        N[2] = n2;
+        S[0] = cos(atan2(n0, n1));
+        S[1] = -sin(atan2(n0, n1));
+        S[2] = 0.0;
+        D[0] = N[1] * S[2] - N[2] * S[1];
+        D[1] = N[2] * S[0] - N[0] * S[2];
+        D[2] = N[0] * S[1] - N[1] * S[0];
+        for (int k = 0; k < 3; k++) xc[3 * j + k] = (A[k] + B[k] + C[k]) / 3.0;
+    }
+}
+
+// strainvolume.f90:135-163
+static void refsys_volume(int ns, const double *x, const double *L, const double *W,
+                          const double *strike, double *sv, double *dv, double *nv, double *xc)
+{
+    for (int i = 0; i < ns; i++) {
+        sv[3 * i] = cos(strike[i]);
+        sv[3 * i + 1] = sin(strike[i]);
+        sv[3 * i + 2] = 0.0;
+        dv[3 * i] = 0.0;
+        dv[3 * i + 1] = 0.0;
+        dv[3 * i + 2] = -1.0;
+        nv[3 * i] = -sin(strike[i]);
+        nv[3 * i + 1] = +cos(strike[i]);
+        nv[3 * i + 2] = -0.0;
+        for (int k = 0; k < 3; k++)
+            xc[3 * i + k] = x[3 * i + k] + L[i] / 2 * sv[3 * i + k] - W[i] / 2 * dv[3 * i + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// lifecycle
+// ---------------------------------------------------------------------------
+extern "C" const char *ucy_version(void) { return "ucy-mi355x 0.1 (gfx950)"; }
+
+extern "C" ucy_ctx *ucy_create(int device, int flags)
+{
+    (void)flags;
+    ucy_ctx *c = new ucy_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    if (hipHostMalloc((void **)&c->h_ctl, sizeof(Control), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void **)&c->d_ctl, sizeof(Control)) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+static void free_work(ucy_ctx *c)
+{
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    if (c->work) hipFree(c->work);
+    c->work = nullptr;
+    c->work_dim = 0;
+}
+
+extern "C" void ucy_destroy(ucy_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_work(c);
+    if (c->K) hipFree(c->K);
+    if (c->d_fr) hipFree(c->d_fr);
+    if (c->d_rh) hipFree(c->d_rh);
+    if (c->d_ctl) hipFree(c->d_ctl);
+    if (c->h_ctl) hipHostFree(c->h_ctl);
+    if (c->hook.p) hipFree(c->hook.p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *ucy_last_error(const ucy_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------------------
+// reference frames
+// ---------------------------------------------------------------------------
+extern "C" int ucy_reference_system_okada92(int ns, const double *x, const double *L,
+                                            const double *W, const double *strike,
+                                            const double *dip, double *sv, double *dv,
+                                            double *nv, double *xc)
+{
+    if (ns < 0) return UCY_ERR_ARG;
+    refsys_okada(ns, x, L, W, strike, dip, sv, dv, nv, xc);
+    return UCY_OK;
+}
+
+extern "C" int ucy_reference_system_nikkhoo15(int nVe, const double *v, int ns, const int *i1,
+                                              const int *i2, const int *i3, double *sv,
+                                              double *dv, double *nv, double *xc)
+{
+    if (ns < 0 || nVe < 0) return UCY_ERR_ARG;
+    for (int j = 0; j < ns; j++)
+        if (i1[j] < 1 || i1[j] > nVe || i2[j] < 1 || i2[j] > nVe || i3[j] < 1 || i3[j] > nVe)
+            return UCY_ERR_ARG;
+    refsys_triangle(nVe, v, ns, i1, i2, i3, sv, dv, nv, xc);
+    return UCY_OK;
+}
+
+extern "C" int ucy_reference_system_strainvolume(int ns, const double *x, const double *L,
+                                                 const double *W, const double *strike,
+                                                 double *sv, double *dv, double *nv, double *xc)
+{
+    if (ns < 0) return UCY_ERR_ARG;
+    refsys_volume(ns, x, L, W, strike, sv, dv, nv, xc);
+    return UCY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// model definition
+// ---------------------------------------------------------------------------
+extern "C" int ucy_set_elastic(ucy_ctx *c, double lambda, double mu)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (!(mu > 0)) return fail(c, UCY_ERR_ARG, "shear modulus must be positive");
+    const double nu = lambda / 2.0 / (lambda + mu);  // ratestate.f90:1102
+    if (nu < -1.0 || nu > 0.5) return fail(c, UCY_ERR_ARG, "Poisson's ratio out of [-1,0.5]");
+    c->lambda = lambda;
+    c->mu = mu;
+    c->have_elastic = true;
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_rectangles(ucy_ctx *c, int ns, const double *x, const double *length,
+                                  const double *width, const double *strike, const double *dip)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    for (int i = 0; i < ns; i++)
+        if (!(std::max(length[i], width[i]) > 0.0))  // ratestate.f90:1184
+            return fail(c, UCY_ERR_ARG, "patch %d: length and width must be positive", i + 1);
+    c->nrect = ns;
+    c->rx.assign(x, x + 3 * (size_t)ns);
+    c->rL.assign(length, length + ns);
+    c->rW.assign(width, width + ns);
+    c->rstrike.assign(strike, strike + ns);
+    c->rdip.assign(dip, dip + ns);
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_triangles(ucy_ctx *c, int ns, const int *i1, const int *i2, const int *i3,
+                                 int nVe, const double *v)
+{
+    if (!c || ns < 0 || nVe < 0) return UCY_ERR_ARG;
+    for (int j = 0; j < ns; j++)
+        if (i1[j] < 1 || i1[j] > nVe || i2[j] < 1 || i2[j] > nVe || i3[j] < 1 || i3[j] > nVe)
+            return fail(c, UCY_ERR_ARG, "triangle %d: vertex index out of range", j + 1);
+    c->ntri = ns;
+    c->nve = nVe;
+    c->i1.assign(i1, i1 + ns);
+    c->i2.assign(i2, i2 + ns);
+    c->i3.assign(i3, i3 + ns);
+    c->vert.assign(v, v + 3 * (size_t)nVe);
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_volumes(ucy_ctx *c, int ns, const double *x, const double *length,
+                               const double *thickness, const double *width, const double *strike)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    for (int i = 0; i < ns; i++)
+        if (x[3 * i + 2] < 0)  // strainvolume.f90:1069-1072
+            return fail(c, UCY_ERR_GEOMETRY, "volume %d: source depth (q3) must be positive", i + 1);
+    c->nvol = ns;
+    c->vx.assign(x, x + 3 * (size_t)ns);
+    c->vL.assign(length, length + ns);
+    c->vT.assign(thickness, thickness + ns);
+    c->vW.assign(width, width + ns);
+    c->vstrike.assign(strike, strike + ns);
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_friction(ucy_ctx *c, int np, const double *Vpl, const double *rake,
+                                const double *mu0, const double *sig, const double *a,
+                                const double *b, const double *L, const double *Vo,
+                                const double *damping)
+{
+    if (!c || np < 0) return UCY_ERR_ARG;
+    c->Vpl.assign(Vpl, Vpl + np);
+    c->rake.assign(rake, rake + np);
+    c->mu0.assign(mu0, mu0 + np);
+    c->sig.assign(sig, sig + np);
+    c->a.assign(a, a + np);
+    c->b.assign(b, b + np);
+    c->Lc.assign(L, L + np);
+    c->Vo.assign(Vo, Vo + np);
+    c->damping.assign(damping, damping + np);
+    c->have_friction = true;
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_rheology(ucy_ctx *c, int nv, const double *gammadot0,
+                                const double *npower, const double *edot_bg, const double *Gm,
+                                const double *Q, const double *R, const double *To)
+{
+    if (!c || nv < 0) return UCY_ERR_ARG;
+    c->gd0.assign(gammadot0, gammadot0 + nv);
+    c->npow.assign(npower, npower + nv);
+    c->ebg.assign(edot_bg, edot_bg + 6 * (size_t)nv);
+    c->Gm.assign(Gm, Gm + nv);
+    c->Qa.assign(Q, Q + nv);
+    c->Rg.assign(R, R + nv);
+    c->To.assign(To, To + nv);
+    c->have_rheology = true;
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+static void element_frames(const ucy_ctx *c, std::vector<Receiver> &R)
+{
+    const int ne = c->nrect + c->ntri + c->nvol;
+    std::vector<double> sv(3 * (size_t)ne + 3), dv(3 * (size_t)ne + 3), nv(3 * (size_t)ne + 3),
+        xc(3 * (size_t)ne + 3);
+    if (c->nrect)
+        refsys_okada(c->nrect, c->rx.data(), c->rL.data(), c->rW.data(), c->rstrike.data(),
+                     c->rdip.data(), sv.data(), dv.data(), nv.data(), xc.data());
+    int o = c->nrect;
+    if (c->ntri)
+        refsys_triangle(c->nve, c->vert.data(), c->ntri, c->i1.data(), c->i2.data(), c->i3.data(),
+                        sv.data() + 3 * o, dv.data() + 3 * o, nv.data() + 3 * o, xc.data() + 3 * o);
+    o += c->ntri;
+    if (c->nvol)
+        refsys_volume(c->nvol, c->vx.data(), c->vL.data(), c->vW.data(), c->vstrike.data(),
+                      sv.data() + 3 * o, dv.data() + 3 * o, nv.data() + 3 * o, xc.data() + 3 * o);
+    R.resize(ne);
+    for (int e = 0; e < ne; e++)
+        for (int k = 0; k < 3; k++) {
+            R[e].xc[k] = xc[3 * e + k];
+            R[e].sv[k] = sv[3 * e + k];
+            R[e].dv[k] = dv[3 * e + k];
+            R[e].nv[k] = nv[3 * e + k];
+        }
+}
+
+extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (precision != UCY_PREC_F64)
+        return fail(c, UCY_ERR_ARG, "single-instance kernels are stored in binary64");
+    if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
+    HIPCHK(hipSetDevice(c->device));
+    const int np = c->nrect + c->ntri, ne = np + c->nvol;
+    if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
+    const int64_t m = 3 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t n = 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    free_work(c);
+    if (c->K) HIPCHK(hipFree(c->K));
+    c->K = nullptr;
+    HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
+    c->m = m;
+    c->n = n;
+    std::vector<Receiver> R;
+    element_frames(c, R);
+    Receiver *d_R = nullptr;
+    HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * ne));
+    HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * ne, hipMemcpyHostToDevice, c->stream));
+    int rc = 0;
+    // rectangle sources -> all receivers
+    if (c->nrect) {
+        std::vector<OkadaSource> S(c->nrect);
+        for (int i = 0; i < c->nrect; i++)
+            S[i] = make_source(&c->rx[3 * i], c->rL[i], c->rW[i], c->rstrike[i], c->rdip[i]);
+        OkadaSource *d_S = nullptr;
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(OkadaSource) * c->nrect));
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(OkadaSource) * c->nrect,
+                              hipMemcpyHostToDevice, c->stream));
+        const OkadaMedium M = make_medium(c->mu, c->lambda);
+        for (int r0 = 0; r0 < np; r0 += 65535) {
+            const int nr = std::min(65535, np - r0);
+            launch_okada_rows(c->stream, M, d_S, c->nrect, d_R + r0, nr, 0, c->K, n, 3 * (int64_t)r0,
+                              0, nullptr, nullptr, 0);
+        }
+        for (int r0 = 0; r0 < c->nvol; r0 += 65535) {
+            const int nr = std::min(65535, c->nvol - r0);
+            launch_okada_rows(c->stream, M, d_S, c->nrect, d_R + np + r0, nr, 1, c->K, n,
+                              3 * (int64_t)np + 6 * (int64_t)r0, 0, nullptr, nullptr, 0);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(d_S));
+    }
+    if (c->ntri) rc = ext_build_triangle_columns(c->stream, c->mu, c->lambda, c->ntri, c->i1.data(),
+                                                  c->i2.data(), c->i3.data(), c->nve,
+                                                  c->vert.data(), d_R, np, c->nvol, c->K, n,
+                                                  2 * (int64_t)c->nrect, c->err);
+    if (!rc && c->nvol)
+        rc = ext_build_volume_columns(c->stream, c->mu, c->lambda, c->nvol, c->vx.data(),
+                                      c->vL.data(), c->vT.data(), c->vW.data(),
+                                      c->vstrike.data(), d_R, np, c->K, n, 2 * (int64_t)np, c->err);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(d_R));
+    return rc;
+}
+
+extern "C" int ucy_kernel_shape(const ucy_ctx *c, int64_t *m, int64_t *n)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (m) *m = c->m;
+    if (n) *n = c->n;
+    return UCY_OK;
+}
+
+extern "C" const double *ucy_kernel_dev(const ucy_ctx *c) { return c ? c->K : nullptr; }
+
+extern "C" int ucy_get_kernel(ucy_ctx *c, double *K)
+{
+    if (!c || !K) return UCY_ERR_ARG;
+    if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
+    HIPCHK(hipMemcpyAsync(K, c->K, (size_t)(c->m * c->n) * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
+{
+    if (!c || !K || m <= 0 || n <= 0) return UCY_ERR_ARG;
+    const int np = c->nrect + c->ntri;
+    if (m != 3 * (int64_t)np + 6 * (int64_t)c->nvol || n != 2 * (int64_t)np + 6 * (int64_t)c->nvol)
+        return fail(c, UCY_ERR_ARG, "kernel shape does not match the mesh");
+    HIPCHK(hipSetDevice(c->device));
+    free_work(c);
+    if (c->K) HIPCHK(hipFree(c->K));
+    c->K = nullptr;
+    HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->K, K, (size_t)(m * n) * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->m = m;
+    c->n = n;
+    return UCY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// matvec / RHS / stepper
+// ---------------------------------------------------------------------------
+static int upload_params(ucy_ctx *c)
+{
+    if (!c->params_dirty && c->d_fr) return 0;
+    const int np = c->nrect + c->ntri;
+    if (np && (!c->have_friction || (int)c->Vpl.size() != np))
+        return fail(c, UCY_ERR_STATE, "friction properties not set for all %d patches", np);
+    if (c->nvol && (!c->have_rheology || (int)c->gd0.size() != c->nvol))
+        return fail(c, UCY_ERR_STATE, "rheology not set for all %d volumes", c->nvol);
+    std::vector<Friction> F(np + 1);
+    for (int i = 0; i < np; i++) {
+        F[i].a = c->a[i];
+        F[i].b = c->b[i];
+        F[i].sig = c->sig[i];
+        F[i].L = c->Lc[i];
+        F[i].damping = c->damping[i];
+        F[i].vplc = c->Vpl[i] * cos(c->rake[i]);
+        F[i].vpls = c->Vpl[i] * sin(c->rake[i]);
+    }
+    std::vector<Rheology> H(c->nvol + 1);
+    for (int i = 0; i < c->nvol; i++) {
+        H[i].gammadot0 = c->gd0[i];
+        H[i].npower = c->npow[i];
+        for (int k = 0; k < 6; k++) H[i].ebg[k] = c->ebg[6 * i + k];
+    }
+    if (c->d_fr) HIPCHK(hipFree(c->d_fr));
+    if (c->d_rh) HIPCHK(hipFree(c->d_rh));
+    HIPCHK(hipMalloc((void **)&c->d_fr, sizeof(Friction) * (np + 1)));
+    HIPCHK(hipMalloc((void **)&c->d_rh, sizeof(Rheology) * (c->nvol + 1)));
+    HIPCHK(hipMemcpyAsync(c->d_fr, F.data(), sizeof(Friction) * (np + 1), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_rh, H.data(), sizeof(Rheology) * (c->nvol + 1),
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->params_dirty = false;
+    return 0;
+}
+
+static PlainDims dims(const ucy_ctx *c)
+{
+    PlainDims d;
+    d.np = c->nrect + c->ntri;
+    d.nv = c->nvol;
+    d.dim = 7 * d.np + 12 * d.nv;
+    d.m = (int)c->m;
+    d.n = (int)c->n;
+    return d;
+}
+
+// work arrays: y dydt yscal ytmp ak2..ak6 ycand yerr v t (+ logs separately)
+static int ensure_work(ucy_ctx *c)
+{
+    const PlainDims d = dims(c);
+    if (c->work && c->work_dim == (size_t)d.dim) return 0;
+    free_work(c);
+    const size_t nvec = 11;
+    const size_t bytes = sizeof(double) * (nvec * (size_t)d.dim + (size_t)d.n + (size_t)d.m + 64);
+    HIPCHK(hipMalloc((void **)&c->work, bytes));
+    HIPCHK(hipMemsetAsync(c->work, 0, bytes, c->stream));
+    double *p = c->work;
+    RkBuffers &b = c->rb;
+    b.y = p; p += d.dim;
+    b.dydt = p; p += d.dim;
+    b.yscal = p; p += d.dim;
+    b.ytmp = p; p += d.dim;
+    b.ak[0] = b.ak[1] = b.dydt;
+    for (int k = 2; k <= 6; k++) { b.ak[k] = p; p += d.dim; }
+    b.ycand = p; p += d.dim;
+    b.yerr = p; p += d.dim;
+    b.yr = nullptr;
+    b.v = p; p += d.n;
+    b.t = p; p += d.m;
+    b.tlog = b.dtlog = nullptr;
+    c->work_dim = d.dim;
+    return 0;
+}
+
+static int check_ready(ucy_ctx *c, int n)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
+    const PlainDims d = dims(c);
+    if (n != d.dim) return fail(c, UCY_ERR_ARG, "state length %d, expected %d", n, d.dim);
+    HIPCHK(hipSetDevice(c->device));
+    int rc = upload_params(c);
+    if (rc) return rc;
+    return ensure_work(c);
+}
+
+extern "C" int ucy_matvec_f64_dev(ucy_ctx *c, const double *v_dev, double *t_dev)
+{
+    if (!c || !v_dev || !t_dev) return UCY_ERR_ARG;
+    if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
+    if (c->n % 2) return fail(c, UCY_ERR_STATE, "odd column count");
+    HIPCHK(hipSetDevice(c->device));
+    launch_matvec(c->stream, c->K, (int)c->m, (int)c->n, v_dev, t_dev);
+    HIPCHK(hipGetLastError());
+    return UCY_OK;
+}
+
+extern "C" int ucy_matvec_f64(ucy_ctx *c, const double *v, double *t)
+{
+    if (!c || !v || !t) return UCY_ERR_ARG;
+    if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
+    HIPCHK(hipSetDevice(c->device));
+    if (ensure_buf(c, c->hook, sizeof(double) * (size_t)(c->m + c->n))) return UCY_ERR_HIP;
+    double *dv = (double *)c->hook.p, *dt = dv + c->n;
+    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    int rc = ucy_matvec_f64_dev(c, dv, dt);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(t, dt, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+// initStateVector, ratestate.f90:548-610 (variant 0) / viscouscycles.f90:710-820 (1)
+extern "C" int ucy_init_state(ucy_ctx *c, int variant, double *y)
+{
+    if (!c || !y) return UCY_ERR_ARG;
+    const int np = c->nrect + c->ntri;
+    if (np && (int)c->Vpl.size() != np) return fail(c, UCY_ERR_STATE, "friction not set");
+    if (c->nvol && (int)c->gd0.size() != c->nvol) return fail(c, UCY_ERR_STATE, "rheology not set");
+    const double lg10 = 2.302585092994045684;
+    for (int i = 0; i < np; i++) {
+        double *p = y + 7 * i;
+        const double base = c->sig[i] * (c->mu0[i] + (c->a[i] - c->b[i]) * log(c->Vpl[i] / c->Vo[i]));
+        const double tau = variant ? base + c->damping[i] * c->Vpl[i] : base - c->damping[i] * c->Vpl[i];
+        p[0] = 0.0;
+        p[1] = 0.0;
+        p[2] = tau * cos(c->rake[i]);
+        p[3] = tau * sin(c->rake[i]);
+        p[4] = 0.0;
+        p[5] = log(c->Lc[i] / c->Vpl[i]) / lg10;
+        p[6] = log(c->Vpl[i] * (variant ? 0.98 : 0.99)) / lg10;
+    }
+    for (int i = 0; i < c->nvol; i++) {
+        double *q = y + 7 * np + 12 * i;
+        const double *e = &c->ebg[6 * i];
+        const double eps = sqrt((e[0] * e[0] + 2 * (e[1] * e[1]) + 2 * (e[2] * e[2]) + e[3] * e[3]
+                                 + 2 * (e[4] * e[4]) + e[5] * e[5]) / 2.0);
+        if (0.0 < eps) {
+            const double tau = c->Gm[i] * pow(eps / c->gd0[i] * exp(c->Qa[i] / (c->Rg[i] * c->To[i])),
+                                              1.0 / c->npow[i]);
+            for (int k = 0; k < 6; k++) q[6 + k] = tau * e[k] / eps;
+        } else {
+            for (int k = 0; k < 6; k++) q[6 + k] = 0.0;
+        }
+        for (int k = 0; k < 6; k++) q[k] = 0.0;
+    }
+    return UCY_OK;
+}
+
+extern "C" int ucy_odefun(ucy_ctx *c, int n, double time, const double *y, double *dydt,
+                          double *vmax)
+{
+    (void)time;  // odefun does not depend on time (ratestate.f90:624-778)
+    int rc = check_ready(c, n);
+    if (rc) return rc;
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    double *d_vmax = b.t + d.m;  // spare slot after t
+    HIPCHK(hipMemcpyAsync(b.ytmp, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(d_vmax, 0, sizeof(double), c->stream));
+    launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.ytmp, b.ak[2], b.v, d_vmax);
+    launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], b.t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dydt, b.ak[2], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (vmax) HIPCHK(hipMemcpyAsync(vmax, d_vmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+// one Cash-Karp attempt on device buffers (y, dydt already resident)
+static void rkck_device(ucy_ctx *c, const PlainDims &d, double h)
+{
+    RkBuffers &b = c->rb;
+    for (int s = 2; s <= 6; s++) {
+        launch_rk_combine(c->stream, d.dim, s, h, b.y, b.ak, nullptr, b.ytmp, nullptr, nullptr);
+        launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.ytmp, b.ak[s], b.v, nullptr);
+        launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[s], nullptr);
+    }
+    launch_rk_combine(c->stream, d.dim, 7, h, b.y, b.ak, nullptr, nullptr, b.ycand, b.yerr);
+}
+
+extern "C" int ucy_rkck(ucy_ctx *c, int n, double t, const double *yin, const double *dydt,
+                        double h, double *yout, double *yerr)
+{
+    (void)t;
+    int rc = check_ready(c, n);
+    if (rc) return rc;
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    HIPCHK(hipMemcpyAsync(b.y, yin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.dydt, dydt, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    rkck_device(c, d, h);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(yout, b.ycand, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(yerr, b.yerr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+// RKQSm (ode45.f90:51-108) with the Cash-Karp stages on the device and the
+// scalar step control on the host (the integrator below keeps it on device).
+extern "C" int ucy_rkqs(ucy_ctx *c, int n, double *t, double *y, double *dydt,
+                        const double *yscal, double htry, double *hdid, double *hnext,
+                        double epsilon, double max_step)
+{
+    int rc = check_ready(c, n);
+    if (rc) return rc;
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    HIPCHK(hipMemcpyAsync(b.y, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.dydt, dydt, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> yerr(n), ycand(n);
+    const double SAFETY = 0.7, PGROW = -0.2, PSHRNK = -0.15, ERRCON = 1.89e-4;
+    double h = htry;
+    for (;;) {
+        rkck_device(c, d, h);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(yerr.data(), b.yerr, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double errmax = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double e = fabs(yerr[i] / yscal[i]);
+            if (e > errmax) errmax = e;
+        }
+        errmax = errmax / epsilon;
+        if (errmax > 1.0) {
+            h = h * SAFETY * pow(errmax, PSHRNK);
+            if (*t + h == *t) return fail(c, UCY_ERR_UNDERFLOW, "stepsize underflow in RKQS");
+        } else {
+            *hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
+            if (*hnext > max_step) *hnext = max_step;
+            *hdid = h;
+            *t = *t + h;
+            HIPCHK(hipMemcpyAsync(y, b.ycand, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            return UCY_OK;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device-resident main loop
+// ---------------------------------------------------------------------------
+static int capture_attempt(ucy_ctx *c)
+{
+    if (c->graph) return 0;
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launch_step_start(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl);
+    launch_step_rhs(c->stream, d, c->K, c->d_fr, b, c->d_ctl);
+    for (int s = 2; s <= 6; s++) {
+        launch_stage(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl, s);
+        launch_stage_rhs(c->stream, d, c->K, c->d_fr, b, c->d_ctl, s);
+    }
+    launch_finish(c->stream, d, b, c->d_ctl);
+    launch_controller(c->stream, b, c->d_ctl);
+    launch_commit(c->stream, d.dim, b, c->d_ctl);
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, UCY_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
+                             int64_t max_steps, double epsilon, double max_step, double *t_log,
+                             double *dt_log, int64_t log_cap, ucy_run_stats *stats)
+{
+    if (!c || !y) return UCY_ERR_ARG;
+    const PlainDims d0 = c->K ? dims(c) : PlainDims{};
+    int rc = check_ready(c, d0.dim);
+    if (rc) return rc;
+    if (!(interval > 0) || !(dt0 > 0) || max_steps <= 0 || !(epsilon > 0))
+        return fail(c, UCY_ERR_ARG, "invalid integration parameters");
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    // device log buffers (reallocated per call; logs are parity aids)
+    double *d_log = nullptr;
+    const int64_t cap = (t_log && dt_log) ? log_cap : 0;
+    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * 2 * cap));
+    if (c->graph && (b.tlog != d_log)) {
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    b.tlog = d_log;
+    b.dtlog = d_log ? d_log + cap : nullptr;
+    Control &h = *c->h_ctl;
+    memset(&h, 0, sizeof(Control));
+    h.time = 0.0;
+    h.h = dt0;
+    h.hnext = dt0;
+    h.interval = interval;
+    h.epsilon = epsilon;
+    h.max_step = max_step;
+    h.need_rhs = 1;
+    h.max_steps = max_steps;
+    h.log_cap = cap;
+    HIPCHK(hipMemcpyAsync(c->d_ctl, &h, sizeof(Control), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.y, y, sizeof(double) * d.dim, hipMemcpyHostToDevice, c->stream));
+    rc = capture_attempt(c);
+    if (rc) {
+        if (d_log) hipFree(d_log);
+        return rc;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    int batch = 16;
+    for (;;) {
+        for (int i = 0; i < batch; i++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(Control), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (h.done) break;
+        if (batch < 256) batch *= 2;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(y, b.y, sizeof(double) * d.dim, hipMemcpyDeviceToHost, c->stream));
+    if (cap > 0 && h.log_count > 0) {
+        HIPCHK(hipMemcpyAsync(t_log, b.tlog, sizeof(double) * h.log_count, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(dt_log, b.dtlog, sizeof(double) * h.log_count,
+                              hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (d_log) {
+        // the graph refers to the log buffers; drop it with them
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        HIPCHK(hipFree(d_log));
+        b.tlog = b.dtlog = nullptr;
+    }
+    if (stats) {
+        stats->time = h.time;
+        stats->dt_next = h.h;
+        stats->steps = h.steps;
+        stats->attempts = h.attempts;
+        stats->rhs_evals = h.rhs_evals;
+        stats->vmax = h.vmax;
+        stats->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    if (h.error == 1) return fail(c, UCY_ERR_UNDERFLOW, "stepsize underflow in RKQS");
+    if (h.error == 2) return fail(c, UCY_ERR_NONFINITE, "non-finite error norm");
+    return UCY_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel parity hooks (one receiver, ns sources)
+// ---------------------------------------------------------------------------
+static int okada_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
+                      const double dv[3], const double nv[3], int ns, const double *y,
+                      const double *L, const double *W, const double *strike, const double *dip,
+                      double s, double dd, double t, double G, double lambda, double **out)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    if (ns == 0) return UCY_OK;
+    if (t != 0.0) return fail(c, UCY_ERR_ARG, "tensile opening is not part of the slip kernels");
+    HIPCHK(hipSetDevice(c->device));
+    const int NC = kind == 0 ? 3 : 6;
+    std::vector<OkadaSource> S(ns);
+    for (int i = 0; i < ns; i++) S[i] = make_source(y + 3 * i, L[i], W[i], strike[i], dip[i]);
+    Receiver R;
+    for (int k = 0; k < 3; k++) {
+        R.xc[k] = x[k];
+        R.sv[k] = sv[k];
+        R.dv[k] = dv[k];
+        R.nv[k] = nv[k];
+    }
+    const size_t bytes = sizeof(OkadaSource) * ns + sizeof(Receiver) + sizeof(double) * 2 * NC * (size_t)ns;
+    if (ensure_buf(c, c->hook, bytes + 256)) return UCY_ERR_HIP;
+    char *base = (char *)c->hook.p;
+    OkadaSource *d_S = (OkadaSource *)base;
+    Receiver *d_R = (Receiver *)(base + sizeof(OkadaSource) * ns);
+    double *d_s = (double *)(base + ((sizeof(OkadaSource) * ns + sizeof(Receiver) + 15) & ~(size_t)15));
+    double *d_d = d_s + NC * (size_t)ns;
+    HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(OkadaSource) * ns, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_R, &R, sizeof(Receiver), hipMemcpyHostToDevice, c->stream));
+    launch_okada_rows(c->stream, make_medium(G, lambda), d_S, ns, d_R, 1, kind, nullptr, 0, 0, 0,
+                      d_s, d_d, ns);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hs(NC * (size_t)ns), hd(NC * (size_t)ns);
+    HIPCHK(hipMemcpyAsync(hs.data(), d_s, sizeof(double) * NC * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hd.data(), d_d, sizeof(double) * NC * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // unit slips are exact; other (s, d) combine the two unit responses
+    for (int k = 0; k < NC; k++)
+        for (int i = 0; i < ns; i++) {
+            const double vs = hs[k * (size_t)ns + i], vd = hd[k * (size_t)ns + i];
+            double r;
+            if (s == 1.0 && dd == 0.0) r = vs;
+            else if (s == 0.0 && dd == 1.0) r = vd;
+            else r = s * vs + dd * vd;
+            out[k][i] = r;
+        }
+    return UCY_OK;
+}
+
+extern "C" int ucy_okada92_traction_kernels(ucy_ctx *c, const double x[3], const double sv[3],
+                                            const double dv[3], const double nv[3], int ns,
+                                            const double *y, const double *L, const double *W,
+                                            const double *strike, const double *dip, double s,
+                                            double d, double t, double G, double lambda,
+                                            double *ts, double *td, double *tn)
+{
+    double *out[3] = {ts, td, tn};
+    return okada_hook(c, 0, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t, G, lambda, out);
+}
+
+extern "C" int ucy_okada92_stress_kernels(ucy_ctx *c, const double x[3], const double sv[3],
+                                          const double dv[3], const double nv[3], int ns,
+                                          const double *y, const double *L, const double *W,
+                                          const double *strike, const double *dip, double s,
+                                          double d, double t, double G, double lambda,
+                                          double *s11, double *s12, double *s13, double *s22,
+                                          double *s23, double *s33)
+{
+    double *out[6] = {s11, s12, s13, s22, s23, s33};
+    return okada_hook(c, 1, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t, G, lambda, out);
+}

@@ -1,0 +1,105 @@
+// k_assemble.hip -- assembly of the traction/stress interaction kernel K
+// (buildG, jdpmoore/unicycle fortran/3d/src/greens.f90:73-275) from
+// rectangular (Okada 1992) sources.
+//
+// One (receiver, source) pair per lane, both unit slips per lane: a
+// wavefront covers 64 consecutive sources of one receiver, so each of the
+// receiver's rows is written as one contiguous 1 KiB run (2 doubles per lane,
+// columns 2s and 2s+1 of greens.f90:214-216).
+#include <hip/hip_runtime.h>
+#include "ucy_internal.h"
+
+namespace ucy {
+
+// traction (sigma.n) projected on (s,d,n), okada92.f90:390-398
+__device__ __forceinline__ void proj_traction(const Receiver &R, const double S[6], double t[3])
+{
+    const double t1 = R.nv[0] * S[0] + R.nv[1] * S[1] + R.nv[2] * S[2];
+    const double t2 = R.nv[0] * S[1] + R.nv[1] * S[3] + R.nv[2] * S[4];
+    const double t3 = R.nv[0] * S[2] + R.nv[1] * S[4] + R.nv[2] * S[5];
+    t[0] = t1 * R.sv[0] + t2 * R.sv[1] + t3 * R.sv[2];
+    t[1] = t1 * R.dv[0] + t2 * R.dv[1] + t3 * R.dv[2];
+    t[2] = t1 * R.nv[0] + t2 * R.nv[1] + t3 * R.nv[2];
+}
+
+// stress in the receiver volume frame, okada92.f90:445-462
+__device__ __forceinline__ void proj_volume(const Receiver &R, const double S[6], double o[6])
+{
+    const double *sv = R.sv, *dv = R.dv, *nv = R.nv;
+    const double a1 = sv[0] * S[0] + sv[1] * S[1] + sv[2] * S[2];
+    const double a2 = sv[0] * S[1] + sv[1] * S[3] + sv[2] * S[4];
+    const double a3 = sv[0] * S[2] + sv[1] * S[4] + sv[2] * S[5];
+    const double b1 = nv[0] * S[0] + nv[1] * S[1] + nv[2] * S[2];
+    const double b2 = nv[0] * S[1] + nv[1] * S[3] + nv[2] * S[4];
+    const double b3 = nv[0] * S[2] + nv[1] * S[4] + nv[2] * S[5];
+    const double c1 = -dv[0] * S[0] - dv[1] * S[1] - dv[2] * S[2];
+    const double c2 = -dv[0] * S[1] - dv[1] * S[3] - dv[2] * S[4];
+    const double c3 = -dv[0] * S[2] - dv[1] * S[4] - dv[2] * S[5];
+    o[0] = a1 * sv[0] + a2 * sv[1] + a3 * sv[2];
+    o[1] = a1 * nv[0] + a2 * nv[1] + a3 * nv[2];
+    o[2] = -a1 * dv[0] - a2 * dv[1] - a3 * dv[2];
+    o[3] = b1 * nv[0] + b2 * nv[1] + b3 * nv[2];
+    o[4] = -b1 * dv[0] - b2 * dv[1] - b3 * dv[2];
+    o[5] = -c1 * dv[0] - c2 * dv[1] - c3 * dv[2];
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256)
+okada_rows_kernel(OkadaMedium M, const OkadaSource *__restrict__ src, int ns,
+                  const Receiver *__restrict__ rcv, double *__restrict__ K, int64_t ldk,
+                  int64_t row0, int64_t col0, double *s_out, double *d_out, int out_stride)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (s >= ns) return;
+    const Receiver R = rcv[r];
+    const OkadaSource S = src[s];
+    // okada92.f90:144-153
+    const double dx = R.xc[0] - S.q1, dy = R.xc[1] - S.q2;
+    const float X = (float)(dx * S.csst + dy * S.ssst);
+    const float Y = (float)(dx * S.ssst - dy * S.csst);
+    const float Z = (float)(-R.xc[2]);
+    float gs[9], gd[9];
+    dc3d_pair(M, S, X, Y, Z, gs, gd);
+    double sigs[6], sigd[6];
+    okada_stress_from_grad(M, S.csst, S.ssst, gs, sigs);
+    okada_stress_from_grad(M, S.csst, S.ssst, gd, sigd);
+    constexpr int NC = KIND == 0 ? 3 : 6;
+    double os[NC], od[NC];
+    if constexpr (KIND == 0) {
+        proj_traction(R, sigs, os);
+        proj_traction(R, sigd, od);
+    } else {
+        proj_volume(R, sigs, os);
+        proj_volume(R, sigd, od);
+    }
+    if (K) {
+        const int64_t base = (row0 + (int64_t)NC * r) * ldk + col0 + 2 * (int64_t)s;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+            *reinterpret_cast<double2 *>(K + base + c * ldk) = make_double2(os[c], od[c]);
+    }
+    if (s_out) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            s_out[(int64_t)c * out_stride + s] = os[c];
+            d_out[(int64_t)c * out_stride + s] = od[c];
+        }
+    }
+}
+
+void launch_okada_rows(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
+                       const Receiver *rcv, int nr, int kind, double *K, int64_t ldk,
+                       int64_t row0, int64_t col0, double *s_out, double *d_out, int out_stride)
+{
+    if (ns <= 0 || nr <= 0) return;
+    dim3 grid((ns + 255) / 256, nr);
+    if (kind == 0)
+        hipLaunchKernelGGL(okada_rows_kernel<0>, grid, dim3(256), 0, st, M, src, ns, rcv, K, ldk,
+                           row0, col0, s_out, d_out, out_stride);
+    else
+        hipLaunchKernelGGL(okada_rows_kernel<1>, grid, dim3(256), 0, st, M, src, ns, rcv, K, ldk,
+                           row0, col0, s_out, d_out, out_stride);
+}
+
+}  // namespace ucy

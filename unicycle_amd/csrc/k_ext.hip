@@ -1,0 +1,19 @@
+// k_ext.hip -- placeholder until the triangle and strain-volume kernels land.
+#include "ucy_ext.h"
+#include "../../include/ucy.h"
+namespace ucy {
+int ext_build_triangle_columns(hipStream_t, double, double, int, const int *, const int *,
+                               const int *, int, const double *, const Receiver *, int, int,
+                               double *, int64_t, int64_t, std::string &msg)
+{
+    msg = "triangle sources not available in this build";
+    return UCY_ERR_STATE;
+}
+int ext_build_volume_columns(hipStream_t, double, double, int, const double *, const double *,
+                             const double *, const double *, const double *, const Receiver *,
+                             int, double *, int64_t, int64_t, std::string &msg)
+{
+    msg = "strain-volume sources not available in this build";
+    return UCY_ERR_STATE;
+}
+}  // namespace ucy

@@ -1,0 +1,544 @@
+// k_ode.hip -- rate-and-state right-hand side, the traction matvec and the
+// Cash-Karp RK45 attempt, device-resident for graph capture.
+//
+// Reference: jdpmoore/unicycle fortran/3d/src
+//   ratestate.f90:624-778   odefun (pre-pass, DGEMV, post-pass)
+//   viscouscycles.f90:933-1080  volume branch of odefun
+//   ode45.f90:51-202        RKQSm / RKCKm
+//   ratestate.f90:307-328   main loop (yscal, time, exit test)
+//
+// Every kernel of the attempt reads the Control block first and returns when
+// the run is done, so a captured attempt can be replayed any number of times.
+#include <hip/hip_runtime.h>
+#include "ucy_internal.h"
+
+namespace ucy {
+
+// ratestate.f90:130 lg10=LOG(1.d1)
+__constant__ const double kLg10 = 2.302585092994045684;
+
+// Cash-Karp tableau (ode45.f90:126-134)
+#define B21 0.2
+#define B31 (3.0 / 40.0)
+#define B32 (9.0 / 40.0)
+#define B41 0.3
+#define B42 (-0.9)
+#define B43 1.2
+#define B51 (-11.0 / 54.0)
+#define B52 2.5
+#define B53 (-70.0 / 27.0)
+#define B54 (35.0 / 27.0)
+#define B61 (1631.0 / 55296.0)
+#define B62 (175.0 / 512.0)
+#define B63 (575.0 / 13824.0)
+#define B64 (44275.0 / 110592.0)
+#define B65 (253.0 / 4096.0)
+#define C1 (37.0 / 378.0)
+#define C3 (250.0 / 621.0)
+#define C4 (125.0 / 594.0)
+#define C6 (512.0 / 1771.0)
+#define DC1 (C1 - 2825.0 / 27648.0)
+#define DC3 (C3 - 18575.0 / 48384.0)
+#define DC4 (C4 - 13525.0 / 55296.0)
+#define DC5 (-277.0 / 14336.0)
+#define DC6 (C6 - 0.25)
+
+__device__ __forceinline__ unsigned long long dbits(double x)
+{
+    return (unsigned long long)__double_as_longlong(x);
+}
+
+// ---------------------------------------------------------------------------
+// pre-pass for one patch (ratestate.f90:683-702) and one volume
+// (viscouscycles.f90:940-995)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double patch_prepass(const double *p, const Friction &f, double *dp,
+                                                double *vp)
+{
+    const double rake = atan2(p[3], p[2]);
+    const double velocity = exp(p[6] * kLg10);
+    const double vs = velocity * cos(rake), vd = velocity * sin(rake);
+    dp[0] = vs;
+    dp[1] = vd;
+    vp[0] = vs - f.vplc;
+    vp[1] = vd - f.vpls;
+    return velocity;
+}
+
+__device__ __forceinline__ void volume_prepass(const double *q, const Rheology &r, double *dq,
+                                               double *vq)
+{
+    double s11 = q[6], s12 = q[7], s13 = q[8], s22 = q[9], s23 = q[10], s33 = q[11];
+    const double p = (s11 + s22 + s33) / 3.0;
+    s11 = s11 - p;
+    s22 = s22 - p;
+    s33 = s33 - p;
+    const double sII = sqrt((s11 * s11 + 2.0 * (s12 * s12) + 2.0 * (s13 * s13) + s22 * s22
+                             + 2.0 * (s23 * s23) + s33 * s33) / 2.0);
+    const double eII = r.gammadot0 * pow(sII, r.npower - 1.0);
+    const double e[6] = {0.0 + eII * s11, 0.0 + eII * s12, 0.0 + eII * s13,
+                         0.0 + eII * s22, 0.0 + eII * s23, 0.0 + eII * s33};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        dq[k] = e[k];
+        vq[k] = e[k] - r.ebg[k];
+    }
+}
+
+// post-pass for one patch (ratestate.f90:747-771)
+__device__ __forceinline__ void patch_postpass(const double *p, const Friction &f, const double *t,
+                                               double *dp)
+{
+    const double rake = atan2(p[3], p[2]);
+    const double velocity = exp(p[6] * kLg10);
+    const double cr = cos(rake), sr = sin(rake);
+    dp[5] = (exp(-p[5] * kLg10) - velocity / f.L) / kLg10;
+    const double dtau = t[0] * cr + t[1] * sr;
+    dp[6] = (dtau - f.b * f.sig * dp[5] * kLg10) / (f.a * f.sig + f.damping * velocity) / kLg10;
+    const double correction = f.damping * velocity * dp[6] * kLg10;
+    dp[2] = t[0] - correction * cr;
+    dp[3] = t[1] - correction * sr;
+    dp[4] = t[2];
+}
+
+__device__ __forceinline__ int elem_state_off(const PlainDims &d, int e)
+{
+    return e < d.np ? 7 * e : 7 * d.np + 12 * (e - d.np);
+}
+__device__ __forceinline__ int elem_v_off(const PlainDims &d, int e)
+{
+    return e < d.np ? 2 * e : 2 * d.np + 6 * (e - d.np);
+}
+__device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
+{
+    return e < d.np ? 3 * e : 3 * d.np + 6 * (e - d.np);
+}
+
+// ---------------------------------------------------------------------------
+// wave-level dot products of NR rows of K with v; one wavefront per element
+// ---------------------------------------------------------------------------
+template <int NR>
+__device__ __forceinline__ void wave_rows_dot(const double *__restrict__ K, int64_t row0, int n,
+                                              const double *__restrict__ v, int lane, double acc[NR])
+{
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r] = 0.0;
+    const int n2 = n >> 1;
+    const double2 *v2 = reinterpret_cast<const double2 *>(v);
+    const double2 *kr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) kr[r] = reinterpret_cast<const double2 *>(K + (row0 + r) * (int64_t)n);
+    int j = lane;
+    for (; j + 192 < n2; j += 256) {
+        double2 vv[4];
+        double2 kk[NR][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) vv[u] = v2[j + 64 * u];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) kk[r][u] = kr[r][j + 64 * u];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                acc[r] = fma(kk[r][u].x, vv[u].x, acc[r]);
+                acc[r] = fma(kk[r][u].y, vv[u].y, acc[r]);
+            }
+    }
+    for (; j < n2; j += 64) {
+        const double2 vv = v2[j];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const double2 kk = kr[r][j];
+            acc[r] = fma(kk.x, vv.x, acc[r]);
+            acc[r] = fma(kk.y, vv.y, acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        double a = acc[r];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        acc[r] = a;
+    }
+}
+
+// generic t = K v (m rows), one wavefront per row
+__global__ void __launch_bounds__(256) matvec_kernel(const double *__restrict__ K, int m, int n,
+                                                     const double *__restrict__ v,
+                                                     double *__restrict__ t)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= m) return;
+    double acc[1];
+    wave_rows_dot<1>(K, row, n, v, lane, acc);
+    if (lane == 0) t[row] = acc[0];
+}
+
+void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t)
+{
+    hipLaunchKernelGGL(matvec_kernel, dim3((m + 3) / 4), dim3(256), 0, st, K, m, n, v, t);
+}
+
+// ---------------------------------------------------------------------------
+// standalone odefun pieces (ucy_odefun)
+// ---------------------------------------------------------------------------
+__global__ void prepass_kernel(PlainDims d, const Friction *__restrict__ fr,
+                               const Rheology *__restrict__ rh, const double *__restrict__ y,
+                               double *__restrict__ dydt, double *__restrict__ v, double *vmax)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.np + d.nv) return;
+    const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
+    if (e < d.np) {
+        for (int k = 2; k < 7; k++) dydt[so + k] = 0.0;
+        const double vel = patch_prepass(y + so, fr[e], dydt + so, v + vo);
+        if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
+    } else {
+        volume_prepass(y + so, rh[e - d.np], dydt + so, v + vo);
+    }
+}
+
+void launch_prepass(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                    const double *y, double *dydt, double *v, double *vmax_out)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(prepass_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, y,
+                       dydt, v, vmax_out);
+}
+
+// K v for the rows of element e, then that element's post-pass.  One
+// wavefront per element; 4 elements per 256-thread block.
+__device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const double *__restrict__ K,
+                                               const Friction *__restrict__ fr,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ v,
+                                               double *__restrict__ dydt, double *t_out, int e,
+                                               int lane, const double *yscal_h, double *yscal)
+{
+    const int so = elem_state_off(d, e);
+    const int64_t ro = elem_row_off(d, e);
+    if (e < d.np) {
+        double acc[3];
+        wave_rows_dot<3>(K, ro, d.n, v, lane, acc);
+        if (lane == 0) {
+            if (t_out) { t_out[ro] = acc[0]; t_out[ro + 1] = acc[1]; t_out[ro + 2] = acc[2]; }
+            patch_postpass(y + so, fr[e], acc, dydt + so);
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 7; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    } else {
+        double acc[6];
+        wave_rows_dot<6>(K, ro, d.n, v, lane, acc);
+        if (lane == 0) {
+            for (int k = 0; k < 6; k++) {
+                if (t_out) t_out[ro + k] = acc[k];
+                dydt[so + 6 + k] = acc[k];  // viscouscycles.f90:1075-1080
+            }
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 12; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) gemv_post_kernel(PlainDims d, const double *__restrict__ K,
+                                                        const Friction *__restrict__ fr,
+                                                        const double *__restrict__ y,
+                                                        const double *__restrict__ v,
+                                                        double *__restrict__ dydt, double *t_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= d.np + d.nv) return;
+    elem_gemv_post(d, K, fr, y, v, dydt, t_out, e, lane, nullptr, nullptr);
+}
+
+void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                      const double *y, const double *v, double *dydt, double *t_out)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(gemv_post_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, y, v,
+                       dydt, t_out);
+}
+
+// ---------------------------------------------------------------------------
+// one RK45 attempt, split in graph-capturable kernels
+// ---------------------------------------------------------------------------
+// step start: RHS at y (ratestate.f90:309) -- pre-pass half
+__global__ void step_start_kernel(PlainDims d, const Friction *__restrict__ fr,
+                                  const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
+{
+    if (ctl->done || !ctl->need_rhs) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) {
+        ctl->errmax_bits = 0ull;
+        ctl->vmax = 0.0;
+    }
+    if (e >= d.np + d.nv) return;
+    const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
+    if (e < d.np) patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+    else volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+}
+
+void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                       const RkBuffers &b, Control *ctl)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(step_start_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, b,
+                       ctl);
+}
+
+// step start: matvec + post-pass half, then yscal (ratestate.f90:315)
+__global__ void __launch_bounds__(256) step_rhs_kernel(PlainDims d, const double *__restrict__ K,
+                                                       const Friction *__restrict__ fr,
+                                                       RkBuffers b, Control *ctl)
+{
+    if (ctl->done || !ctl->need_rhs) return;
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= d.np + d.nv) return;
+    elem_gemv_post(d, K, fr, b.y, b.v, b.dydt, nullptr, e, lane, &ctl->h, b.yscal);
+}
+
+void launch_step_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                     const RkBuffers &b, Control *ctl)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(step_rhs_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, b, ctl);
+}
+
+// intermediate state of stage s (ode45.f90:137-180), BLAS-1 order kept:
+// yrkck = B_s1*dydt; yrkck += B_sj*AKj; ytmp = yin + h*yrkck
+__device__ __forceinline__ double stage_value(int stage, double h, double y0, double k1, double k2,
+                                              double k3, double k4, double k5)
+{
+    double yr;
+    switch (stage) {
+    case 2:
+        return y0 + (h * B21) * k1;
+    case 3:
+        yr = B31 * k1;
+        yr = yr + B32 * k2;
+        return y0 + h * yr;
+    case 4:
+        yr = B41 * k1;
+        yr = yr + B42 * k2;
+        yr = yr + B43 * k3;
+        return y0 + h * yr;
+    case 5:
+        yr = B51 * k1;
+        yr = yr + B52 * k2;
+        yr = yr + B53 * k3;
+        yr = yr + B54 * k4;
+        return y0 + h * yr;
+    default:
+        yr = B61 * k1;
+        yr = yr + B62 * k2;
+        yr = yr + B63 * k3;
+        yr = yr + B64 * k4;
+        yr = yr + B65 * k5;
+        return y0 + h * yr;
+    }
+}
+
+__global__ void stage_kernel(PlainDims d, const Friction *__restrict__ fr,
+                             const Rheology *__restrict__ rh, RkBuffers b, Control *ctl, int stage)
+{
+    if (ctl->done) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (stage == 2 && e == 0) ctl->errmax_bits = 0ull;
+    if (e >= d.np + d.nv) return;
+    const double h = ctl->h;
+    const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
+    const int ns = e < d.np ? 7 : 12;
+    for (int k = 0; k < ns; k++) {
+        const int i = so + k;
+        b.ytmp[i] = stage_value(stage, h, b.y[i], b.dydt[i], b.ak[2][i], b.ak[3][i], b.ak[4][i],
+                                b.ak[5][i]);
+    }
+    double *out = b.ak[stage];
+    if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, b.v + vo);
+    else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, b.v + vo);
+}
+
+void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                  const RkBuffers &b, Control *ctl, int stage)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(stage_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, b, ctl,
+                       stage);
+}
+
+__global__ void __launch_bounds__(256) stage_rhs_kernel(PlainDims d, const double *__restrict__ K,
+                                                        const Friction *__restrict__ fr,
+                                                        RkBuffers b, Control *ctl, int stage)
+{
+    if (ctl->done) return;
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= d.np + d.nv) return;
+    elem_gemv_post(d, K, fr, b.ytmp, b.v, b.ak[stage], nullptr, e, lane, nullptr, nullptr);
+}
+
+void launch_stage_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                      const RkBuffers &b, Control *ctl, int stage)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(stage_rhs_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, b, ctl,
+                       stage);
+}
+
+// 5th-order solution, embedded error and max |yerr/yscal| (ode45.f90:185-200, :75)
+__global__ void __launch_bounds__(256) finish_kernel(int dim, RkBuffers b, Control *ctl)
+{
+    if (ctl->done) return;
+    const double h = ctl->h;
+    double emax = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += gridDim.x * blockDim.x) {
+        const double k1 = b.dydt[i], k3 = b.ak[3][i], k4 = b.ak[4][i], k5 = b.ak[5][i],
+                     k6 = b.ak[6][i];
+        double yr = C1 * k1;
+        yr = yr + C3 * k3;
+        yr = yr + C4 * k4;
+        yr = yr + C6 * k6;
+        b.ycand[i] = b.y[i] + h * yr;
+        double e = DC1 * k1;
+        e = e + DC3 * k3;
+        e = e + DC4 * k4;
+        e = e + DC5 * k5;
+        e = e + DC6 * k6;
+        e = h * e;
+        b.yerr[i] = e;
+        const double r = fabs(e / b.yscal[i]);
+        // non-finite ratios propagate as NaN bits, which order above +inf
+        emax = (r > emax || r != r) ? r : emax;
+    }
+    unsigned long long bits = dbits(emax);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&ctl->errmax_bits, bits);
+}
+
+void launch_finish(hipStream_t st, const PlainDims &d, const RkBuffers &b, Control *ctl)
+{
+    int blocks = (d.dim + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(finish_kernel, dim3(blocks), dim3(256), 0, st, d.dim, b, ctl);
+}
+
+// step control of RKQSm (ode45.f90:70-105) and the time update / exit test of
+// the main loop (ratestate.f90:317-326).  RKQSm is called with t0 = 0
+// (ratestate.f90:317), so its underflow test is h == 0.
+__global__ void controller_kernel(RkBuffers b, Control *ctl)
+{
+    if (ctl->done) return;
+    const double SAFETY = 0.7, PGROW = -0.2, PSHRNK = -0.15, ERRCON = 1.89e-4;
+    ctl->attempts += 1;
+    ctl->rhs_evals += ctl->need_rhs ? 6 : 5;
+    const double raw = __longlong_as_double((long long)ctl->errmax_bits);
+    if (!(raw == raw) || raw > 1.7976931348623157e308) {
+        ctl->error = 2;
+        ctl->done = 1;
+        ctl->accepted = 0;
+        return;
+    }
+    const double errmax = raw / ctl->epsilon;
+    double h = ctl->h;
+    if (errmax > 1.0) {
+        h = h * SAFETY * pow(errmax, PSHRNK);
+        ctl->h = h;
+        ctl->need_rhs = 0;
+        ctl->accepted = 0;
+        if (0.0 + h == 0.0) {
+            ctl->error = 1;
+            ctl->done = 1;
+        }
+        return;
+    }
+    double hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
+    if (hnext > ctl->max_step) hnext = ctl->max_step;
+    ctl->hdid = h;
+    ctl->hnext = hnext;
+    ctl->time = ctl->time + h;
+    ctl->steps += 1;
+    if (ctl->log_count < ctl->log_cap) {
+        b.tlog[ctl->log_count] = ctl->time;
+        b.dtlog[ctl->log_count] = h;
+        ctl->log_count += 1;
+    }
+    ctl->accepted = 1;
+    ctl->need_rhs = 1;
+    ctl->h = hnext;
+    if (ctl->interval <= ctl->time || ctl->steps >= ctl->max_steps) ctl->done = 1;
+}
+
+// y <- ycand after an accepted attempt (ode45.f90:103)
+__global__ void commit_kernel(int dim, RkBuffers b, Control *ctl)
+{
+    if (!ctl->accepted) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += gridDim.x * blockDim.x)
+        b.y[i] = b.ycand[i];
+}
+
+void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl)
+{
+    hipLaunchKernelGGL(controller_kernel, dim3(1), dim3(1), 0, st, b, ctl);
+}
+
+void launch_commit(hipStream_t st, int dim, const RkBuffers &b, Control *ctl)
+{
+    int blocks = (dim + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(commit_kernel, dim3(blocks), dim3(256), 0, st, dim, b, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// standalone stage combination for ucy_rkck: stage 2..6 writes ytmp, stage 7
+// writes yout and yerr.
+// ---------------------------------------------------------------------------
+__global__ void rk_combine_kernel(int n, int stage, double h, const double *__restrict__ yin,
+                                  const double *k1, const double *k2, const double *k3,
+                                  const double *k4, const double *k5, const double *k6,
+                                  double *ytmp, double *yout, double *yerr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (stage <= 6) {
+        ytmp[i] = stage_value(stage, h, yin[i], k1[i], stage > 2 ? k2[i] : 0.0,
+                              stage > 3 ? k3[i] : 0.0, stage > 4 ? k4[i] : 0.0,
+                              stage > 5 ? k5[i] : 0.0);
+        return;
+    }
+    double yr = C1 * k1[i];
+    yr = yr + C3 * k3[i];
+    yr = yr + C4 * k4[i];
+    yr = yr + C6 * k6[i];
+    yout[i] = yin[i] + h * yr;
+    double e = DC1 * k1[i];
+    e = e + DC3 * k3[i];
+    e = e + DC4 * k4[i];
+    e = e + DC5 * k5[i];
+    e = e + DC6 * k6[i];
+    yerr[i] = h * e;
+}
+
+void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,
+                       double *const *ak, double *yr, double *ytmp, double *yout, double *yerr)
+{
+    (void)yr;
+    hipLaunchKernelGGL(rk_combine_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, stage, h,
+                       yin, ak[1], ak[2], ak[3], ak[4], ak[5], ak[6], ytmp, yout, yerr);
+}
+
+}  // namespace ucy

@@ -1,0 +1,87 @@
+// ucy_internal.h -- structures shared between the host-side C ABI and the
+// HIP kernels of the plain (single-instance, binary64) path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "okada_dev.h"
+
+namespace ucy {
+
+// Receiver description: centre, frame and first row of K it owns.
+struct Receiver {
+    double xc[3], sv[3], dv[3], nv[3];
+};
+
+// Per-patch friction constants for the RHS (types.f90:35-40).  vplc/vpls are
+// Vpl*cos(rake) and Vpl*sin(rake) formed on the host with the C library, the
+// same product the reference forms at ratestate.f90:701-702.
+struct Friction {
+    double a, b, sig, L, damping, vplc, vpls, pad;
+};
+
+// Per-volume rheology for the RHS (viscouscycles.f90:963-995).
+struct Rheology {
+    double gammadot0, npower;
+    double ebg[6];
+};
+
+// Device-side control block of the graph-captured integrator.
+struct Control {
+    double time;        // simulated time (ratestate.f90:176,321)
+    double h;           // step size of the next attempt (dt_try, or shrunk h)
+    double hnext;       // dt_next of the last accepted step
+    double hdid;        // dt_done of the last accepted step
+    double interval;    // end time (ratestate.f90:324)
+    double epsilon;     // ode45.f90:40
+    double max_step;    // ode45.f90:43
+    unsigned long long errmax_bits;  // max |yerr/yscal| as ordered bits
+    double vmax;
+    int need_rhs;       // 1: next attempt starts a new step
+    int accepted;       // 1: last attempt accepted, y <- ycand pending
+    int done;
+    int error;          // 0 ok, 1 underflow, 2 non-finite error norm
+    long long steps, attempts, max_steps, rhs_evals;
+    long long log_count, log_cap;
+};
+
+struct PlainDims {
+    int np;      // patches
+    int nv;      // volumes
+    int dim;     // 7*np + 12*nv
+    int m, n;    // K rows, columns
+};
+
+// ---- assembly launchers (k_assemble.hip) ----
+// rows for receivers [r0, r0+nr) from all rectangle sources; kind 0 traction
+// (3 rows per receiver), kind 1 volume-frame stress (6 rows per receiver).
+void launch_okada_rows(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
+                       const Receiver *rcv, int nr, int kind, double *K, int64_t ldk,
+                       int64_t row0, int64_t col0, double *s_out, double *d_out, int out_stride);
+
+// ---- GEMV / RHS launchers (k_ode.hip) ----
+void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t);
+// odefun building blocks
+void launch_prepass(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                    const double *y, double *dydt, double *v, double *vmax_out);
+void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                      const double *y, const double *v, double *dydt, double *t_out);
+// integrator graph pieces
+struct RkBuffers {
+    double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr, *yr, *v, *t;
+    double *tlog, *dtlog;
+};
+void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                       const RkBuffers &b, Control *ctl);
+void launch_step_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                     const RkBuffers &b, Control *ctl);
+void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                  const RkBuffers &b, Control *ctl, int stage);
+void launch_stage_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+                      const RkBuffers &b, Control *ctl, int stage);
+void launch_finish(hipStream_t st, const PlainDims &d, const RkBuffers &b, Control *ctl);
+void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl);
+// standalone Cash-Karp combination kernels for the ucy_rkck entry point
+void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,
+                       double *const *ak, double *yr, double *ytmp, double *yout, double *yerr);
+
+}  // namespace ucy

@@ -1,0 +1,320 @@
+"""Host-side mirror of the reference's fortran/3d call surface over libucy.
+
+Names and argument meaning follow jdpmoore/unicycle fortran/3d/src:
+``buildG`` (greens.f90:73), ``odefun`` (ratestate.f90:624), ``RKCKm`` /
+``RKQSm`` (ode45.f90:116 / 51), ``initStateVector`` (ratestate.f90:548), the
+Green's-function routines of okada92.f90 / nikkhoo15.f90 / strainvolume.f90,
+and the main loop of ratestate.f90:307-328 (``integrate``).  Errors that the
+reference turns into ``STOP`` raise :class:`UcyError` here.
+
+Every method calls the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+PREC_F64 = 0
+PREC_SPLIT_BF16 = 1
+
+
+class UcyError(RuntimeError):
+    """A nonzero status from libucy (the reference would STOP)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _pi(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a, shape=None) -> np.ndarray:
+    out = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if shape is not None:
+        out = out.reshape(shape)
+    return out
+
+
+def _vec3(a) -> np.ndarray:
+    return _f64(a).reshape(3)
+
+
+@dataclass
+class IntegrateResult:
+    y: np.ndarray
+    time: float
+    dt_next: float
+    steps: int
+    attempts: int
+    rhs_evals: int
+    vmax: float
+    wall_seconds: float
+    t_log: np.ndarray | None = None
+    dt_log: np.ndarray | None = None
+
+
+class Engine:
+    """One GPU context (one MPI rank of the reference)."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.lib()
+        self._h = self._L.ucy_create(device, 0)
+        if not self._h:
+            raise UcyError(-2, f"cannot create a context on device {device}")
+        self.np = 0
+        self.nvol = 0
+
+    # -- plumbing ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ucy_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, status: int):
+        if status != 0:
+            msg = self._L.ucy_last_error(self._h)
+            raise UcyError(status, msg.decode() if msg else "")
+
+    # -- model definition -------------------------------------------------
+    def set_elastic(self, lambda_: float, mu: float):
+        self.lambda_, self.mu = float(lambda_), float(mu)
+        self._chk(self._L.ucy_set_elastic(self._h, self.lambda_, self.mu))
+
+    def set_rectangles(self, x, length, width, strike, dip):
+        x = _f64(x).reshape(-1, 3)
+        ns = x.shape[0]
+        self._rect = [x, _f64(length), _f64(width), _f64(strike), _f64(dip)]
+        self.nrect = ns
+        self._chk(self._L.ucy_set_rectangles(self._h, ns, *[_p(a) for a in self._rect]))
+        self._update_np()
+
+    def set_triangles(self, i1, i2, i3, vertices):
+        """i1, i2, i3 are 1-based vertex indices (ratestate.f90:1252-1290)."""
+        idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
+        v = _f64(vertices).reshape(-1, 3)
+        self._tri = idx + [v]
+        self.ntri = idx[0].size
+        self._chk(self._L.ucy_set_triangles(self._h, self.ntri, _pi(idx[0]), _pi(idx[1]),
+                                            _pi(idx[2]), v.shape[0], _p(v)))
+        self._update_np()
+
+    def set_volumes(self, x, length, thickness, width, strike):
+        x = _f64(x).reshape(-1, 3)
+        self._vol = [x, _f64(length), _f64(thickness), _f64(width), _f64(strike)]
+        self.nvol = x.shape[0]
+        self._chk(self._L.ucy_set_volumes(self._h, self.nvol, *[_p(a) for a in self._vol]))
+
+    def _update_np(self):
+        self.np = getattr(self, "nrect", 0) + getattr(self, "ntri", 0)
+
+    def set_friction(self, Vpl, rake, mu0, sig, a, b, L, Vo, damping):
+        arrs = [_f64(v) for v in (Vpl, rake, mu0, sig, a, b, L, Vo, damping)]
+        n = arrs[0].size
+        self._fric = arrs
+        self._chk(self._L.ucy_set_friction(self._h, n, *[_p(v) for v in arrs]))
+
+    def set_rheology(self, gammadot0, npower, edot_bg, Gm, Q, R, To):
+        arrs = [_f64(v) for v in (gammadot0, npower)] + [_f64(edot_bg).reshape(-1)] + \
+               [_f64(v) for v in (Gm, Q, R, To)]
+        self._rheo = arrs
+        self._chk(self._L.ucy_set_rheology(self._h, arrs[0].size, *[_p(v) for v in arrs]))
+
+    # -- assembly ---------------------------------------------------------
+    def buildG(self, precision: int = PREC_F64):
+        self._chk(self._L.ucy_build_kernel(self._h, precision))
+
+    @property
+    def shape(self):
+        m, n = C.c_int64(), C.c_int64()
+        self._chk(self._L.ucy_kernel_shape(self._h, C.byref(m), C.byref(n)))
+        return m.value, n.value
+
+    @property
+    def dim(self) -> int:
+        return 7 * self.np + 12 * self.nvol
+
+    def get_kernel(self) -> np.ndarray:
+        m, n = self.shape
+        K = np.empty((m, n))
+        self._chk(self._L.ucy_get_kernel(self._h, _p(K)))
+        return K
+
+    def set_kernel(self, K):
+        K = _f64(K)
+        self._chk(self._L.ucy_set_kernel(self._h, _p(K), K.shape[0], K.shape[1]))
+
+    def matvec(self, v) -> np.ndarray:
+        v = _f64(v)
+        m, _ = self.shape
+        t = np.empty(m)
+        self._chk(self._L.ucy_matvec_f64(self._h, _p(v), _p(t)))
+        return t
+
+    def kernel_dev_ptr(self) -> int:
+        return self._L.ucy_kernel_dev(self._h) or 0
+
+    def matvec_dev(self, v_ptr: int, t_ptr: int):
+        self._chk(self._L.ucy_matvec_f64_dev(self._h, C.c_void_p(v_ptr), C.c_void_p(t_ptr)))
+
+    # -- RHS and stepper --------------------------------------------------
+    def initStateVector(self, variant: int = 0) -> np.ndarray:
+        y = np.empty(self.dim)
+        self._chk(self._L.ucy_init_state(self._h, variant, _p(y)))
+        return y
+
+    def odefun(self, n: int, time: float, y, dydt: np.ndarray) -> float:
+        """Fill ``dydt`` in place (ratestate.f90:624); returns vMax."""
+        y = _f64(y)
+        vmax = C.c_double()
+        self._chk(self._L.ucy_odefun(self._h, int(n), float(time), _p(y), _p(dydt),
+                                     C.byref(vmax)))
+        return vmax.value
+
+    def RKCKm(self, n, t, yin, dydt, h, yout: np.ndarray, yerr: np.ndarray):
+        self._chk(self._L.ucy_rkck(self._h, int(n), float(t), _p(_f64(yin)), _p(_f64(dydt)),
+                                   float(h), _p(yout), _p(yerr)))
+
+    def RKQSm(self, n, t, y: np.ndarray, dydt, yscal, htry, epsilon=1e-6,
+              maximumTimeStep=1.7976931348623158e308):
+        """Adaptive step (ode45.f90:51); updates ``y`` in place, returns (t, hdid, hnext)."""
+        tt, hdid, hnext = C.c_double(t), C.c_double(), C.c_double()
+        self._chk(self._L.ucy_rkqs(self._h, int(n), C.byref(tt), _p(y), _p(_f64(dydt)),
+                                   _p(_f64(yscal)), float(htry), C.byref(hdid), C.byref(hnext),
+                                   float(epsilon), float(maximumTimeStep)))
+        return tt.value, hdid.value, hnext.value
+
+    def integrate(self, y, interval: float, dt0: float = 1.0e-3, max_steps: int = 1000000,
+                  epsilon: float = 1.0e-6, max_step: float = 1.7976931348623158e308,
+                  log_cap: int = 0) -> IntegrateResult:
+        """Main loop of ratestate.f90:307-328, device resident."""
+        y = _f64(y).copy()
+        st = _lib.RunStats()
+        tl = np.empty(max(log_cap, 1))
+        dl = np.empty(max(log_cap, 1))
+        self._chk(self._L.ucy_integrate(self._h, _p(y), float(interval), float(dt0),
+                                        int(max_steps), float(epsilon), float(max_step),
+                                        _p(tl) if log_cap else None, _p(dl) if log_cap else None,
+                                        int(log_cap), C.byref(st)))
+        nlog = min(st.steps, log_cap)
+        return IntegrateResult(y, st.time, st.dt_next, st.steps, st.attempts, st.rhs_evals,
+                               st.vmax, st.wall_seconds,
+                               tl[:nlog].copy() if log_cap else None,
+                               dl[:nlog].copy() if log_cap else None)
+
+    # -- Green's-function routines (one receiver, ns sources) -------------
+    def computeTractionKernelsOkada92(self, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t,
+                                      G, lambda_):
+        out = [np.empty(ns) for _ in range(3)]
+        self._chk(self._L.ucy_okada92_traction_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            _p(_f64(y).reshape(-1)), _p(_f64(L)), _p(_f64(W)), _p(_f64(strike)), _p(_f64(dip)),
+            float(s), float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+    def computeStressKernelsOkada92(self, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t,
+                                    G, lambda_):
+        out = [np.empty(ns) for _ in range(6)]
+        self._chk(self._L.ucy_okada92_stress_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            _p(_f64(y).reshape(-1)), _p(_f64(L)), _p(_f64(W)), _p(_f64(strike)), _p(_f64(dip)),
+            float(s), float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+    def computeTractionKernelsNikkhoo15(self, x, sv, dv, nv, ns, i1, i2, i3, nVe, vertices,
+                                        s, d, t, G, lambda_):
+        idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
+        out = [np.empty(ns) for _ in range(3)]
+        self._chk(self._L.ucy_nikkhoo15_traction_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            *[_pi(a) for a in idx], int(nVe), _p(_f64(vertices).reshape(-1)), float(s),
+            float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+    def computeStressKernelsNikkhoo15(self, x, sv, dv, nv, ns, i1, i2, i3, nVe, vertices,
+                                      s, d, t, G, lambda_):
+        idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
+        out = [np.empty(ns) for _ in range(6)]
+        self._chk(self._L.ucy_nikkhoo15_stress_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            *[_pi(a) for a in idx], int(nVe), _p(_f64(vertices).reshape(-1)), float(s),
+            float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+    def computeTractionKernelsVerticalStrainVolume(self, x, sv, dv, nv, ns, y, L, T, W, strike,
+                                                   e11p, e12p, e13p, e22p, e23p, e33p, G,
+                                                   lambda_):
+        out = [np.empty(ns) for _ in range(3)]
+        self._chk(self._L.ucy_strainvolume_traction_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            _p(_f64(y).reshape(-1)), _p(_f64(L)), _p(_f64(T)), _p(_f64(W)), _p(_f64(strike)),
+            float(e11p), float(e12p), float(e13p), float(e22p), float(e23p), float(e33p),
+            float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+    def computeStressKernelsVerticalStrainVolume(self, x, sv, dv, nv, ns, y, L, T, W, strike,
+                                                 e11p, e12p, e13p, e22p, e23p, e33p, G, lambda_):
+        out = [np.empty(ns) for _ in range(6)]
+        self._chk(self._L.ucy_strainvolume_stress_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            _p(_f64(y).reshape(-1)), _p(_f64(L)), _p(_f64(T)), _p(_f64(W)), _p(_f64(strike)),
+            float(e11p), float(e12p), float(e13p), float(e22p), float(e23p), float(e33p),
+            float(G), float(lambda_), *[_p(o) for o in out]))
+        return tuple(out)
+
+
+# -- reference frames (host arithmetic in the library) ---------------------
+def computeReferenceSystemOkada92(x, L, W, strike, dip):
+    x = _f64(x).reshape(-1, 3)
+    ns = x.shape[0]
+    out = [np.empty((ns, 3)) for _ in range(4)]
+    st = _lib.lib().ucy_reference_system_okada92(ns, _p(x), _p(_f64(L)), _p(_f64(W)),
+                                                 _p(_f64(strike)), _p(_f64(dip)),
+                                                 *[_p(o) for o in out])
+    if st:
+        raise UcyError(st, "computeReferenceSystemOkada92")
+    return tuple(out)  # sv, dv, nv, xc
+
+
+def computeReferenceSystemNikkhoo15(v, i1, i2, i3):
+    v = _f64(v).reshape(-1, 3)
+    idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
+    ns = idx[0].size
+    out = [np.empty((ns, 3)) for _ in range(4)]
+    st = _lib.lib().ucy_reference_system_nikkhoo15(v.shape[0], _p(v), ns, *[_pi(a) for a in idx],
+                                                   *[_p(o) for o in out])
+    if st:
+        raise UcyError(st, "computeReferenceSystemNikkhoo15")
+    return tuple(out)
+
+
+def computeReferenceSystemVerticalStrainVolume(x, L, W, strike):
+    x = _f64(x).reshape(-1, 3)
+    ns = x.shape[0]
+    out = [np.empty((ns, 3)) for _ in range(4)]
+    st = _lib.lib().ucy_reference_system_strainvolume(ns, _p(x), _p(_f64(L)), _p(_f64(W)),
+                                                      _p(_f64(strike)), *[_p(o) for o in out])
+    if st:
+        raise UcyError(st, "computeReferenceSystemVerticalStrainVolume")
+    return tuple(out)
