@@ -168,19 +168,26 @@ int ucy_rkqs(ucy_ctx *ctx, int n, double *t, double *y, double *dydt, const doub
 
 /* Main loop of ratestate.f90:307-328 run device-resident: one graph-captured
  * RK45 attempt per replay, state kept in HBM.  y is read at entry and written
- * at exit.  Optional logs receive (time, dt_done) of accepted steps. */
+ * at exit.  Optional logs receive, per accepted step, the time reached, dt_done
+ * and the vMax of the step-start odefun (ratestate.f90:530-532). */
 typedef struct {
     double time;        /* simulated time reached */
     double dt_next;     /* next step size */
     int64_t steps;      /* accepted steps */
     int64_t attempts;   /* RK45 attempts (accepted + rejected) */
     int64_t rhs_evals;  /* odefun evaluations */
-    double vmax;        /* max slip velocity at exit */
+    double vmax;        /* vMax of the last accepted step's step-start odefun */
     double wall_seconds;
 } ucy_run_stats;
 int ucy_integrate(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
                   double epsilon, double max_step, double *t_log, double *dt_log,
-                  int64_t log_cap, ucy_run_stats *stats);
+                  double *vmax_log, int64_t log_cap, ucy_run_stats *stats);
+
+/* Measurement hook (no counterpart in the reference): average duration in
+ * seconds of the fused matvec + odefun post-pass kernel (the dominant kernel
+ * of every RHS), timed with HIP events on the context stream over iters
+ * launches at the current state. */
+int ucy_time_rhs_kernel(ucy_ctx *ctx, int iters, double *avg_seconds);
 
 /* ---- batched ensembles (no counterpart in the reference) --------------- */
 /* Independent fault instances in geometry groups: each group shares one

@@ -358,27 +358,30 @@ typedef struct {
     const orc_system *S;
     double *v, *t;
     long evals;
+    double vmax;
 } rhs_ctx;
 
 static void rhs_adapter(void *p, const double *y, double *dydt)
 {
     rhs_ctx *c = (rhs_ctx *)p;
     c->evals++;
-    orc_odefun(c->S, y, dydt, c->v, c->t);
+    c->vmax = orc_odefun(c->S, y, dydt, c->v, c->t);
 }
 
 long orc_integrate(const orc_system *S, double *y, double interval, double dt0,
                    long max_steps, double epsilon, double max_step,
-                   double *t_log, double *dt_log, long log_cap, orc_run_stats *st)
+                   double *t_log, double *dt_log, double *vmax_log, long log_cap,
+                   orc_run_stats *st)
 {
     const int dim = 7 * S->n + 12 * S->nvol;
     double *dydt = malloc(sizeof(double) * dim), *yscal = malloc(sizeof(double) * dim);
     double *w = malloc(sizeof(double) * 9 * dim);
-    rhs_ctx c = {S, malloc(sizeof(double) * S->ncol), malloc(sizeof(double) * S->m), 0};
+    rhs_ctx c = {S, malloc(sizeof(double) * S->ncol), malloc(sizeof(double) * S->m), 0, 0.0};
     double time = 0.0, dt_next = dt0, dt_done = 0.0;
     long i, attempts = 0, rc = 0;
     for (i = 1; i <= max_steps; i++) {
         rhs_adapter(&c, y, dydt);
+        double vmax = c.vmax;
         double dt_try = dt_next;
         for (int k = 0; k < dim; k++) yscal[k] = fabs(y[k]) + fabs(dt_try * dydt[k]) + 1.0e-30;
         double t0 = 0.0;
@@ -391,7 +394,11 @@ long orc_integrate(const orc_system *S, double *y, double interval, double dt0,
         }
         attempts += att;
         time = time + dt_done;
-        if (t_log && i - 1 < log_cap) { t_log[i - 1] = time; dt_log[i - 1] = dt_done; }
+        if (t_log && i - 1 < log_cap) {
+            t_log[i - 1] = time;
+            dt_log[i - 1] = dt_done;
+            if (vmax_log) vmax_log[i - 1] = vmax;
+        }
         if (interval <= time) break;
     }
     if (i > max_steps) i = max_steps;

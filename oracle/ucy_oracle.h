@@ -131,7 +131,8 @@ typedef struct {
 } orc_run_stats;
 long orc_integrate(const orc_system *sys, double *y, double interval, double dt0,
                    long max_steps, double epsilon, double max_step,
-                   double *t_log, double *dt_log, long log_cap, orc_run_stats *st);
+                   double *t_log, double *dt_log, double *vmax_log, long log_cap,
+                   orc_run_stats *st);
 
 #ifdef __cplusplus
 }

@@ -158,11 +158,13 @@ class System:
         y = f64(y0).copy()
         tl = np.empty(max(log_cap, 1))
         dl = np.empty(max(log_cap, 1))
+        vl = np.empty(max(log_cap, 1))
         st = OrcStats()
         rc = oracle().orc_integrate(C.byref(self.c), p(y), D(interval), D(dt0), C.c_long(max_steps),
-                                    D(epsilon), D(max_step), p(tl), p(dl), C.c_long(log_cap),
-                                    C.byref(st))
+                                    D(epsilon), D(max_step), p(tl), p(dl), p(vl),
+                                    C.c_long(log_cap), C.byref(st))
         n = min(st.steps, log_cap)
+        self.vmax_log = vl[:n].copy()
         return y, st, tl[:n].copy(), dl[:n].copy(), rc
 
 

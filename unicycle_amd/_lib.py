@@ -70,7 +70,9 @@ _SIGS = {
     "ucy_odefun": (_i, [_vp, _i, _d, _pd, _pd, _pd]),
     "ucy_rkck": (_i, [_vp, _i, _d, _pd, _pd, _d, _pd, _pd]),
     "ucy_rkqs": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _d, _pd, _pd, _d, _d]),
-    "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _i64, C.POINTER(RunStats)]),
+    "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _pd, _i64,
+                           C.POINTER(RunStats)]),
+    "ucy_time_rhs_kernel": (_i, [_vp, _i, _pd]),
     "ucy_ensemble_create": (_vp, [_vp, _i]),
     "ucy_ensemble_destroy": (None, [_vp]),
     "ucy_ensemble_add_group": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _d, _d, _i, _pd, _pd, _pd,

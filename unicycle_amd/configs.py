@@ -59,9 +59,16 @@ def thrust_fault(nx: int = 64, nz: int = 32, patch: float = 2000.0, dip_deg: flo
                  a: float = 0.010, b_in: float = 0.014, b_out: float = 0.006,
                  ellipse: tuple | None = (40e3, 20e3), L: float = 0.01, sig: float = 100.0,
                  Vo: float = 1e-6, mu0: float = 0.6, Vpl: float = 1e-9,
-                 rake_deg: float = 90.0) -> FaultModel:
+                 rake_deg: float = 89.0) -> FaultModel:
     """Planar thrust, strike north, dipping east; patch (i,j) row-major with
-    i along strike fastest."""
+    i along strike fastest.
+
+    The default rake is 89 degrees (a thrust with a 1-degree oblique
+    component).  At exactly 90 degrees the strike-slip and strike-traction
+    state entries are pure rounding noise (cos(pi/2) ~ 6e-17), yet they enter
+    the RK45 error norm with the 1e-30 floor of yscal (ratestate.f90:315), so
+    the step-size sequence -- and the run length -- is set by round-off and
+    differs between any two summation orders of the matvec."""
     lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
     dip = np.deg2rad(dip_deg)
     i = np.tile(np.arange(nx), nz)

@@ -616,7 +616,7 @@ static int ensure_work(ucy_ctx *c)
     b.yr = nullptr;
     b.v = p; p += d.n;
     b.t = p; p += d.m;
-    b.tlog = b.dtlog = nullptr;
+    b.tlog = b.dtlog = b.vlog = nullptr;
     c->work_dim = d.dim;
     return 0;
 }
@@ -816,7 +816,8 @@ static int capture_attempt(ucy_ctx *c)
 
 extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                              int64_t max_steps, double epsilon, double max_step, double *t_log,
-                             double *dt_log, int64_t log_cap, ucy_run_stats *stats)
+                             double *dt_log, double *vmax_log, int64_t log_cap,
+                             ucy_run_stats *stats)
 {
     if (!c || !y) return UCY_ERR_ARG;
     const PlainDims d0 = c->K ? dims(c) : PlainDims{};
@@ -828,14 +829,15 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
     RkBuffers &b = c->rb;
     // device log buffers (reallocated per call; logs are parity aids)
     double *d_log = nullptr;
-    const int64_t cap = (t_log && dt_log) ? log_cap : 0;
-    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * 2 * cap));
+    const int64_t cap = (t_log && dt_log && vmax_log) ? log_cap : 0;
+    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * 3 * cap));
     if (c->graph && (b.tlog != d_log)) {
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }
     b.tlog = d_log;
     b.dtlog = d_log ? d_log + cap : nullptr;
+    b.vlog = d_log ? d_log + 2 * cap : nullptr;
     Control &h = *c->h_ctl;
     memset(&h, 0, sizeof(Control));
     h.time = 0.0;
@@ -870,6 +872,8 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                               c->stream));
         HIPCHK(hipMemcpyAsync(dt_log, b.dtlog, sizeof(double) * h.log_count,
                               hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(vmax_log, b.vlog, sizeof(double) * h.log_count,
+                              hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (d_log) {
@@ -877,7 +881,7 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
         HIPCHK(hipFree(d_log));
-        b.tlog = b.dtlog = nullptr;
+        b.tlog = b.dtlog = b.vlog = nullptr;
     }
     if (stats) {
         stats->time = h.time;
@@ -885,7 +889,7 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
         stats->steps = h.steps;
         stats->attempts = h.attempts;
         stats->rhs_evals = h.rhs_evals;
-        stats->vmax = h.vmax;
+        stats->vmax = h.last_vmax;
         stats->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
     }
     if (h.error == 1) return fail(c, UCY_ERR_UNDERFLOW, "stepsize underflow in RKQS");
@@ -965,4 +969,33 @@ extern "C" int ucy_okada92_stress_kernels(ucy_ctx *c, const double x[3], const d
 {
     double *out[6] = {s11, s12, s13, s22, s23, s33};
     return okada_hook(c, 1, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t, G, lambda, out);
+}
+
+// ---------------------------------------------------------------------------
+// measurement hook: HIP-event timing of the fused matvec + post-pass kernel
+// ---------------------------------------------------------------------------
+extern "C" int ucy_time_rhs_kernel(ucy_ctx *c, int iters, double *avg_seconds)
+{
+    if (!c || iters <= 0 || !avg_seconds) return UCY_ERR_ARG;
+    const PlainDims d0 = c->K ? dims(c) : PlainDims{};
+    int rc = check_ready(c, d0.dim);
+    if (rc) return rc;
+    const PlainDims d = dims(c);
+    RkBuffers &b = c->rb;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.y, b.ak[2], b.v, nullptr);
+    launch_gemv_post(c->stream, d, c->K, c->d_fr, b.y, b.v, b.ak[2], nullptr);  // warm
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; i++)
+        launch_gemv_post(c->stream, d, c->K, c->d_fr, b.y, b.v, b.ak[2], nullptr);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *avg_seconds = 1e-3 * ms / iters;
+    return UCY_OK;
 }

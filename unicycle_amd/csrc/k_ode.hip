@@ -276,16 +276,18 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 __global__ void step_start_kernel(PlainDims d, const Friction *__restrict__ fr,
                                   const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
 {
+    // ctl->vmax was cleared by the controller that accepted the previous step
     if (ctl->done || !ctl->need_rhs) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e == 0) {
-        ctl->errmax_bits = 0ull;
-        ctl->vmax = 0.0;
-    }
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (e < d.np) patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
-    else volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+    if (e < d.np) {
+        // vMax of the step-start odefun (ratestate.f90:693), as exported every step
+        const double vel = patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
+    } else {
+        volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+    }
 }
 
 void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
@@ -475,8 +477,11 @@ __global__ void controller_kernel(RkBuffers b, Control *ctl)
     if (ctl->log_count < ctl->log_cap) {
         b.tlog[ctl->log_count] = ctl->time;
         b.dtlog[ctl->log_count] = h;
+        b.vlog[ctl->log_count] = ctl->vmax;
         ctl->log_count += 1;
     }
+    ctl->last_vmax = ctl->vmax;
+    ctl->vmax = 0.0;
     ctl->accepted = 1;
     ctl->need_rhs = 1;
     ctl->h = hnext;

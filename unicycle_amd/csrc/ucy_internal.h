@@ -35,7 +35,8 @@ struct Control {
     double epsilon;     // ode45.f90:40
     double max_step;    // ode45.f90:43
     unsigned long long errmax_bits;  // max |yerr/yscal| as ordered bits
-    double vmax;
+    double vmax;        // max slip velocity of the current step-start odefun
+    double last_vmax;   // same, for the last accepted step
     int need_rhs;       // 1: next attempt starts a new step
     int accepted;       // 1: last attempt accepted, y <- ycand pending
     int done;
@@ -68,7 +69,7 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 // integrator graph pieces
 struct RkBuffers {
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr, *yr, *v, *t;
-    double *tlog, *dtlog;
+    double *tlog, *dtlog, *vlog;
 };
 void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
                        const RkBuffers &b, Control *ctl);

@@ -61,6 +61,7 @@ class IntegrateResult:
     wall_seconds: float
     t_log: np.ndarray | None = None
     dt_log: np.ndarray | None = None
+    vmax_log: np.ndarray | None = None
 
 
 class Engine:
@@ -213,15 +214,23 @@ class Engine:
         st = _lib.RunStats()
         tl = np.empty(max(log_cap, 1))
         dl = np.empty(max(log_cap, 1))
+        vl = np.empty(max(log_cap, 1))
         self._chk(self._L.ucy_integrate(self._h, _p(y), float(interval), float(dt0),
                                         int(max_steps), float(epsilon), float(max_step),
                                         _p(tl) if log_cap else None, _p(dl) if log_cap else None,
-                                        int(log_cap), C.byref(st)))
+                                        _p(vl) if log_cap else None, int(log_cap), C.byref(st)))
         nlog = min(st.steps, log_cap)
         return IntegrateResult(y, st.time, st.dt_next, st.steps, st.attempts, st.rhs_evals,
                                st.vmax, st.wall_seconds,
                                tl[:nlog].copy() if log_cap else None,
-                               dl[:nlog].copy() if log_cap else None)
+                               dl[:nlog].copy() if log_cap else None,
+                               vl[:nlog].copy() if log_cap else None)
+
+    def time_rhs_kernel(self, iters: int = 50) -> float:
+        """Average seconds of the fused matvec + post-pass kernel (HIP events)."""
+        out = C.c_double()
+        self._chk(self._L.ucy_time_rhs_kernel(self._h, int(iters), C.byref(out)))
+        return out.value
 
     # -- Green's-function routines (one receiver, ns sources) -------------
     def computeTractionKernelsOkada92(self, x, sv, dv, nv, ns, y, L, W, strike, dip, s, d, t,
