@@ -205,12 +205,35 @@ int ucy_ensemble_add_group(ucy_ensemble *ens, int np, const double *x, const dou
                            const double *rake, const double *a, const double *b,
                            const double *sig, const double *L, const double *damping,
                            const double *y0);
+/* assemble every group's K on the device, convert it to the ensemble
+ * precision and tune the stage GEMM tiling on the device */
 int ucy_ensemble_build(ucy_ensemble *ens);
-/* advance every instance until t >= t_end (or max_attempts attempts) */
+/* back to the initial states given to ucy_ensemble_add_group, t = 0 */
+int ucy_ensemble_reset(ucy_ensemble *ens, double dt0);
+typedef struct {
+    double inst_time_sum;   /* sum over instances of simulated time reached */
+    double min_time, max_time;
+    int64_t steps;          /* accepted steps, all instances */
+    int64_t attempts;       /* RK45 attempts, all instances */
+    int64_t replays;        /* graph replays (lock-step attempts) */
+    int64_t n_inst, n_failed;
+    double wall_seconds;
+} ucy_ensemble_stats;
+/* advance every instance until t >= t_end or max_attempts attempts */
 int ucy_ensemble_run(ucy_ensemble *ens, double t_end, int64_t max_attempts, double epsilon,
-                     double max_step, ucy_run_stats *stats);
+                     double max_step, ucy_ensemble_stats *stats);
+/* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
+ * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,
                            double *dt_next, int64_t *steps, int64_t *attempts);
+/* one stage matvec of a group in the ensemble precision: V is n x n_inst
+ * (row-major, column = instance), T is m x n_inst */
+int ucy_ensemble_matvec(ucy_ensemble *ens, int group, const double *V, double *T);
+/* chosen split-K k-blocks per wave and the tuning table (may be NULL) */
+int ucy_ensemble_tuning(const ucy_ensemble *ens, int *kb_per_split, double *gemm_seconds);
+/* HIP-event timing of the all-groups stage GEMM (seconds per launch) and its
+ * algorithmic HBM bytes per launch */
+int ucy_ensemble_time_gemm(ucy_ensemble *ens, int iters, double *avg_seconds, double *bytes);
 
 #ifdef __cplusplus
 }

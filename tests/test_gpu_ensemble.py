@@ -1,0 +1,118 @@
+"""Batched ensembles on the GPU: split-bf16 MFMA stage GEMM and binary64
+per-instance stepping, checked against the plain binary64 path and the oracle.
+
+Tolerances:
+- split-bf16 matvec: K = s (hi + lo) and V = hi + lo carry 16 significant bits
+  each; the three products are accumulated in fp32.  Per row the error is
+  bounded by 2^-15 * sum_c |K_rc||V_c| (two 2^-17 truncations, the dropped
+  lo*lo term and fp32 accumulation over <= 16384 terms, with margin).
+- binary64 ensemble vs the plain path: same per-instance algorithm, only the
+  matvec summation order differs, so the accept/reject sequence is identical
+  and states agree to 1e-6 relative before the first event.
+"""
+import numpy as np
+import pytest
+
+import orc
+from unicycle_amd import configs
+from unicycle_amd.ensemble import Ensemble, PREC_F64, PREC_SPLIT_BF16
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(model, n_inst, seed):
+    draws = configs.ensemble_instances(model, n_inst, seed)
+    y0 = []
+    sysm0 = orc.System(model.friction(), np.zeros((3 * model.n, 2 * model.n)))
+    base = sysm0.init_state()
+    for i in range(n_inst):
+        y = base.copy()
+        # reference initial state with this instance's friction
+        fr = list(model.friction())
+        fr[3], fr[4], fr[5], fr[6] = draws["sig"][i], draws["a"][i], draws["b"][i], draws["L"][i]
+        s = orc.System(fr, np.zeros((3 * model.n, 2 * model.n)))
+        y = configs.perturbed_initial_state(model, s.init_state(), int(draws["seeds"][i]))
+        y0.append(y)
+    return draws, np.array(y0)
+
+
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_F64])
+def test_ensemble_matvec(engine_factory, precision):
+    eng = engine_factory()
+    groups = [configs.thrust_fault(nx=8, nz=8, ellipse=None),
+              configs.thrust_fault(nx=16, nz=8, ellipse=None)]
+    ens = Ensemble(eng, precision)
+    for gi, model in enumerate(groups):
+        n_inst = 64 if gi == 0 else 37
+        draws, y0 = _instances(model, n_inst, seed=gi)
+        ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
+    ens.build()
+    rng = np.random.default_rng(0)
+    for gi, model in enumerate(groups):
+        K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
+        n_inst = ens.groups[gi][1]
+        # slip-rate deviations spanning many decades, both signs
+        V = rng.standard_normal((K.shape[1], n_inst)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], n_inst))
+        T = ens.matvec(gi, V)
+        want = K @ V
+        scale = np.abs(K) @ np.abs(V)
+        err = np.abs(T - want)
+        if precision == PREC_SPLIT_BF16:
+            assert np.all(err <= 2.0 ** -15 * scale), (err / scale).max()
+            assert (err / scale).max() > 1e-9   # really reduced precision
+        else:
+            assert np.all(err <= 1e-13 * scale)
+
+
+def _plain_engine(engine_factory, model, fr):
+    eng = engine_factory()
+    eng.set_elastic(model.lambda_, model.mu)
+    eng.set_rectangles(*model.geometry())
+    eng.set_friction(*fr)
+    eng.buildG()
+    return eng
+
+
+def test_ensemble_f64_matches_plain_path(engine_factory):
+    model = configs.thrust_fault(nx=8, nz=4, ellipse=None)
+    n_inst = 4
+    draws, y0 = _instances(model, n_inst, seed=3)
+    eng = engine_factory()
+    ens = Ensemble(eng, PREC_F64)
+    ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
+    ens.build()
+    T = 3.0 * configs.YEAR
+    stats = ens.run(T)
+    assert stats.n_failed == 0 and stats.n_inst == n_inst
+    y, t, dt, steps, att = ens.state(0)
+    for i in range(n_inst):
+        fr = list(model.friction())
+        fr[3], fr[4], fr[5], fr[6] = draws["sig"][i], draws["a"][i], draws["b"][i], draws["L"][i]
+        p = _plain_engine(engine_factory, model, fr)
+        res = p.integrate(y0[i], T)
+        assert res.steps == steps[i] and res.attempts == att[i]
+        assert t[i] == pytest.approx(res.time, rel=1e-6)
+        np.testing.assert_allclose(y[i], res.y, rtol=1e-6, atol=1e-9 * np.abs(res.y).max())
+
+
+def test_ensemble_split_bf16_tracks_f64(engine_factory):
+    """Split-bf16 K (fp32 accumulation) against the binary64 ensemble over a
+    quiescent window: slip within 1e-6 relative, state within 1e-4."""
+    model = configs.thrust_fault(nx=16, nz=8, ellipse=None)
+    n_inst = 8
+    draws, y0 = _instances(model, n_inst, seed=5)
+    out = {}
+    for prec in (PREC_F64, PREC_SPLIT_BF16):
+        eng = engine_factory()
+        ens = Ensemble(eng, prec)
+        ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
+        ens.build()
+        stats = ens.run(2.0 * configs.YEAR)
+        assert stats.n_failed == 0
+        out[prec] = ens.state(0)
+    y64, y16 = out[PREC_F64][0], out[PREC_SPLIT_BF16][0]
+    slip64 = np.hypot(y64[:, 0::7], y64[:, 1::7])
+    slip16 = np.hypot(y16[:, 0::7], y16[:, 1::7])
+    rel = np.linalg.norm(slip16 - slip64) / np.linalg.norm(slip64)
+    print(f"split-bf16 slip error vs binary64: {rel:.3e}")
+    assert rel <= 1e-6

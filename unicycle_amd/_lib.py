@@ -19,6 +19,20 @@ _pi = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
 
+class EnsembleStats(C.Structure):
+    _fields_ = [
+        ("inst_time_sum", C.c_double),
+        ("min_time", C.c_double),
+        ("max_time", C.c_double),
+        ("steps", C.c_int64),
+        ("attempts", C.c_int64),
+        ("replays", C.c_int64),
+        ("n_inst", C.c_int64),
+        ("n_failed", C.c_int64),
+        ("wall_seconds", C.c_double),
+    ]
+
+
 class RunStats(C.Structure):
     _fields_ = [
         ("time", C.c_double),
@@ -78,8 +92,12 @@ _SIGS = {
     "ucy_ensemble_add_group": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _d, _d, _i, _pd, _pd, _pd,
                                     _pd, _pd, _pd, _pd, _pd]),
     "ucy_ensemble_build": (_i, [_vp]),
-    "ucy_ensemble_run": (_i, [_vp, _d, _i64, _d, _d, C.POINTER(RunStats)]),
+    "ucy_ensemble_reset": (_i, [_vp, _d]),
+    "ucy_ensemble_run": (_i, [_vp, _d, _i64, _d, _d, C.POINTER(EnsembleStats)]),
     "ucy_ensemble_get_state": (_i, [_vp, _i, _pd, _pd, _pd, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
+    "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), _pd]),
+    "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),
 }
 
 EXPORTED = tuple(_SIGS)

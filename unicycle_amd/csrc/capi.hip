@@ -20,57 +20,13 @@
 #include "../../include/ucy.h"
 #include "ucy_internal.h"
 #include "ucy_ext.h"
+#include "ucy_ctx.h"
 
 using namespace ucy;
 
 namespace ucy {
 void launch_commit(hipStream_t st, int dim, const RkBuffers &b, Control *ctl);
 }
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-struct ucy_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // elastic moduli
-    double lambda = 0, mu = 0;
-    bool have_elastic = false;
-    // rectangles
-    int nrect = 0;
-    std::vector<double> rx, rL, rW, rstrike, rdip;
-    // triangles
-    int ntri = 0, nve = 0;
-    std::vector<int> i1, i2, i3;
-    std::vector<double> vert;
-    // volumes
-    int nvol = 0;
-    std::vector<double> vx, vL, vT, vW, vstrike;
-    // friction / rheology (element order)
-    std::vector<double> Vpl, rake, mu0, sig, a, b, Lc, Vo, damping;
-    std::vector<double> gd0, npow, ebg, Gm, Qa, Rg, To;
-    bool have_friction = false, have_rheology = false;
-    // device kernel
-    double *K = nullptr;
-    int64_t m = 0, n = 0;
-    // device per-element data
-    Friction *d_fr = nullptr;
-    Rheology *d_rh = nullptr;
-    bool params_dirty = true;
-    // work buffers
-    double *work = nullptr;
-    size_t work_dim = 0;
-    RkBuffers rb{};
-    Control *d_ctl = nullptr;
-    Control *h_ctl = nullptr;  // pinned
-    // captured attempt
-    hipGraphExec_t graph = nullptr;
-    // scratch for the per-kernel hooks
-    DevBuf hook;
-};
 
 static int fail(ucy_ctx *c, int code, const char *fmt, ...)
 {
@@ -109,7 +65,7 @@ static int ensure_buf(ucy_ctx *c, DevBuf &b, size_t bytes)
 // ---------------------------------------------------------------------------
 static const double kRad2Deg = 57.29577951308232;  // okada92.f90:133
 
-static OkadaMedium make_medium(double G, double lambda)
+OkadaMedium make_medium(double G, double lambda)
 {
     OkadaMedium M{};
     const float alpha = (float)((lambda + G) / (lambda + 2.0 * G));  // okada92.f90:138
@@ -124,7 +80,7 @@ static OkadaMedium make_medium(double G, double lambda)
     return M;
 }
 
-static OkadaSource make_source(const double *x, double L, double W, double strike, double dipd)
+OkadaSource make_source(const double *x, double L, double W, double strike, double dipd)
 {
     OkadaSource S{};
     S.q1 = x[0];
@@ -151,7 +107,7 @@ static OkadaSource make_source(const double *x, double L, double W, double strik
 }
 
 // okada92.f90:59-87
-static void refsys_okada(int ns, const double *x, const double *L, const double *W,
+void refsys_okada(int ns, const double *x, const double *L, const double *W,
                          const double *strike, const double *dip, double *sv, double *dv,
                          double *nv, double *xc)
 {

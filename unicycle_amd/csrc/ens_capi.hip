@@ -1,0 +1,568 @@
+// ens_capi.hip -- host side of the batched-ensemble C ABI (include/ucy.h).
+//
+// A group = one geometry (Okada rectangles) shared by up to 64 instances.
+// build(): K assembled on the device by the same Okada kernel as buildG,
+// converted to the ensemble precision, tiling tuned by timing the stage GEMM
+// on the device.  run(): one graph-captured lock-step RK45 attempt for all
+// instances of all groups per replay.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/ucy.h"
+#include "ens_internal.h"
+#include "ucy_ctx.h"
+
+using namespace ucy;
+
+struct HostGroup {
+    int N = 0, ninst = 0;
+    double lambda = 0, mu = 0;
+    std::vector<double> x, L, W, strike, dip;
+    // per instance-major [inst][e]
+    std::vector<double> Vpl, rake, a, b, sig, Lc, damping, y0;
+};
+
+struct ucy_ensemble {
+    ucy_ctx *ctx = nullptr;
+    int mode = 0;  // 0 split-bf16, 1 binary64
+    std::vector<HostGroup> hg;
+    EnsTable tab{};
+    EnsTable *d_tab = nullptr;
+    std::vector<void *> allocs;
+    hipGraphExec_t graph = nullptr;
+    bool built = false;
+    int kb_per_split = 0;
+    std::vector<double> tune_seconds;
+    double gemm_bytes = 0;
+};
+
+static int efail(ucy_ensemble *E, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (E && E->ctx) E->ctx->err = buf;
+    return code;
+}
+
+#define EHIP(call)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return efail(E, UCY_ERR_HIP, "%s failed: %s (%s:%d)", #call,                   \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                       \
+    } while (0)
+
+template <typename T>
+static int ealloc(ucy_ensemble *E, T **p, size_t count)
+{
+    void *q = nullptr;
+    EHIP(hipMalloc(&q, sizeof(T) * (count ? count : 1)));
+    EHIP(hipMemsetAsync(q, 0, sizeof(T) * (count ? count : 1), E->ctx->stream));
+    E->allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+extern "C" ucy_ensemble *ucy_ensemble_create(ucy_ctx *ctx, int precision)
+{
+    if (!ctx) return nullptr;
+    if (precision != UCY_PREC_SPLIT_BF16 && precision != UCY_PREC_F64) return nullptr;
+    ucy_ensemble *E = new ucy_ensemble();
+    E->ctx = ctx;
+    E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0 : 1;
+    return E;
+}
+
+static void release(ucy_ensemble *E)
+{
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    E->graph = nullptr;
+    for (void *p : E->allocs) (void)hipFree(p);
+    E->allocs.clear();
+    E->d_tab = nullptr;
+    E->built = false;
+}
+
+extern "C" void ucy_ensemble_destroy(ucy_ensemble *E)
+{
+    if (!E) return;
+    (void)hipSetDevice(E->ctx->device);
+    (void)hipStreamSynchronize(E->ctx->stream);
+    release(E);
+    delete E;
+}
+
+extern "C" int ucy_ensemble_add_group(ucy_ensemble *E, int np, const double *x, const double *length,
+                                      const double *width, const double *strike, const double *dip,
+                                      double lambda, double mu, int n_inst, const double *Vpl,
+                                      const double *rake, const double *a, const double *b,
+                                      const double *sig, const double *L, const double *damping,
+                                      const double *y0)
+{
+    if (!E || np <= 0 || n_inst <= 0) return UCY_ERR_ARG;
+    if (n_inst > kEnsCols) return efail(E, UCY_ERR_LIMIT, "at most %d instances per group", kEnsCols);
+    if ((int)E->hg.size() >= kMaxGroups) return efail(E, UCY_ERR_LIMIT, "at most %d groups", kMaxGroups);
+    if (!(mu > 0)) return efail(E, UCY_ERR_ARG, "shear modulus must be positive");
+    HostGroup G;
+    G.N = np;
+    G.ninst = n_inst;
+    G.lambda = lambda;
+    G.mu = mu;
+    G.x.assign(x, x + 3 * (size_t)np);
+    G.L.assign(length, length + np);
+    G.W.assign(width, width + np);
+    G.strike.assign(strike, strike + np);
+    G.dip.assign(dip, dip + np);
+    const size_t ne = (size_t)np * n_inst;
+    G.Vpl.assign(Vpl, Vpl + ne);
+    G.rake.assign(rake, rake + ne);
+    G.a.assign(a, a + ne);
+    G.b.assign(b, b + ne);
+    G.sig.assign(sig, sig + ne);
+    G.Lc.assign(L, L + ne);
+    G.damping.assign(damping, damping + ne);
+    G.y0.assign(y0, y0 + 7 * ne);
+    E->hg.push_back(std::move(G));
+    release(E);
+    return (int)E->hg.size() - 1;
+}
+
+// upload y0 (reference layout per instance) into the SoA state and reset control
+static int load_initial(ucy_ensemble *E, double dt0)
+{
+    ucy_ctx *c = E->ctx;
+    for (size_t gi = 0; gi < E->hg.size(); gi++) {
+        const HostGroup &H = E->hg[gi];
+        EnsGroup &G = E->tab.g[gi];
+        const int N = H.N;
+        std::vector<double> Y((size_t)7 * N * kEnsCols, 0.0);
+        for (int i = 0; i < H.ninst; i++)
+            for (int e = 0; e < N; e++)
+                for (int k = 0; k < 7; k++)
+                    Y[((size_t)k * N + e) * kEnsCols + i] = H.y0[((size_t)i * N + e) * 7 + k];
+        EHIP(hipMemcpyAsync(G.y, Y.data(), sizeof(double) * Y.size(), hipMemcpyHostToDevice, c->stream));
+        std::vector<double> hv(kEnsCols, dt0), zero(kEnsCols, 0.0);
+        std::vector<int> one(kEnsCols, 1), izero(kEnsCols, 0);
+        std::vector<long long> lzero(kEnsCols, 0);
+        EHIP(hipMemcpyAsync(G.h, hv.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.hnext, hv.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.time, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.vmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.errmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.need_rhs, one.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.accepted, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.done, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.status, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.steps, lzero.data(), sizeof(long long) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.attempts, lzero.data(), sizeof(long long) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+    }
+    int z = 0;
+    EHIP(hipMemcpyAsync(E->tab.all_done, &z, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    EHIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static void set_split(EnsTable &T, int kbps)
+{
+    for (int gi = 0; gi < T.ngroups; gi++) {
+        EnsGroup &G = T.g[gi];
+        if (G.mode != 0) continue;
+        int k = std::max(1, std::min(kbps, G.nkb));
+        G.ksplit = (G.nkb + k - 1) / k;
+        G.kb_per_split = (G.nkb + G.ksplit - 1) / G.ksplit;
+    }
+    int acc = 0;
+    for (int gi = 0; gi < T.ngroups; gi++) {
+        const EnsGroup &G = T.g[gi];
+        T.blk_gemm[gi] = acc;
+        const int items = G.mode == 0 ? (G.mp / 64) * G.ksplit : (G.m + 7) / 8;
+        acc += (items + 3) / 4;
+    }
+    T.blk_gemm[T.ngroups] = acc;
+}
+
+static int upload_table(ucy_ensemble *E)
+{
+    EHIP(hipMemcpyAsync(E->d_tab, &E->tab, sizeof(EnsTable), hipMemcpyHostToDevice, E->ctx->stream));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    return 0;
+}
+
+static const int kMaxSplit = 16;
+
+extern "C" int ucy_ensemble_build(ucy_ensemble *E)
+{
+    if (!E) return UCY_ERR_ARG;
+    if (E->hg.empty()) return efail(E, UCY_ERR_STATE, "no groups");
+    ucy_ctx *c = E->ctx;
+    EHIP(hipSetDevice(c->device));
+    release(E);
+    EnsTable &T = E->tab;
+    memset(&T, 0, sizeof(T));
+    T.ngroups = (int)E->hg.size();
+    int eacc = 0, iacc = 0, total = 0;
+    double bytes = 0;
+    for (int gi = 0; gi < T.ngroups; gi++) {
+        const HostGroup &H = E->hg[gi];
+        EnsGroup &G = T.g[gi];
+        G.N = H.N;
+        G.ninst = H.ninst;
+        G.m = 3 * H.N;
+        G.n = 2 * H.N;
+        G.mp = (G.m + 63) / 64 * 64;
+        G.np = (G.n + 15) / 16 * 16;
+        G.nkb = G.np / 16;
+        G.mode = E->mode;
+        total += H.ninst;
+        // ---- K on the device (greens.f90:73-275 for rectangle receivers)
+        double *K64 = nullptr;
+        EHIP(hipMalloc((void **)&K64, sizeof(double) * (size_t)G.m * G.n));
+        {
+            std::vector<double> sv(3 * (size_t)H.N), dv(3 * (size_t)H.N), nv(3 * (size_t)H.N),
+                xc(3 * (size_t)H.N);
+            refsys_okada(H.N, H.x.data(), H.L.data(), H.W.data(), H.strike.data(), H.dip.data(),
+                         sv.data(), dv.data(), nv.data(), xc.data());
+            std::vector<Receiver> R(H.N);
+            std::vector<OkadaSource> S(H.N);
+            for (int e = 0; e < H.N; e++) {
+                for (int k = 0; k < 3; k++) {
+                    R[e].xc[k] = xc[3 * e + k];
+                    R[e].sv[k] = sv[3 * e + k];
+                    R[e].dv[k] = dv[3 * e + k];
+                    R[e].nv[k] = nv[3 * e + k];
+                }
+                S[e] = make_source(&H.x[3 * e], H.L[e], H.W[e], H.strike[e], H.dip[e]);
+            }
+            Receiver *dR = nullptr;
+            OkadaSource *dS = nullptr;
+            EHIP(hipMalloc((void **)&dR, sizeof(Receiver) * H.N));
+            EHIP(hipMalloc((void **)&dS, sizeof(OkadaSource) * H.N));
+            EHIP(hipMemcpyAsync(dR, R.data(), sizeof(Receiver) * H.N, hipMemcpyHostToDevice, c->stream));
+            EHIP(hipMemcpyAsync(dS, S.data(), sizeof(OkadaSource) * H.N, hipMemcpyHostToDevice, c->stream));
+            const OkadaMedium M = make_medium(H.mu, H.lambda);
+            for (int r0 = 0; r0 < H.N; r0 += 65535)
+                launch_okada_rows(c->stream, M, dS, H.N, dR + r0, std::min(65535, H.N - r0), 0, K64,
+                                  G.n, 3 * (int64_t)r0, 0, nullptr, nullptr, 0);
+            EHIP(hipGetLastError());
+            EHIP(hipStreamSynchronize(c->stream));
+            EHIP(hipFree(dR));
+            EHIP(hipFree(dS));
+        }
+        if (G.mode == 0) {
+            uint16_t *tiles = nullptr;
+            double *rs = nullptr;
+            if (ealloc(E, &tiles, (size_t)G.mp * G.np * 2)) return UCY_ERR_HIP;
+            if (ealloc(E, &rs, G.mp)) return UCY_ERR_HIP;
+            ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
+            EHIP(hipGetLastError());
+            EHIP(hipStreamSynchronize(c->stream));
+            EHIP(hipFree(K64));
+            G.Ktiles = tiles;
+            G.rscale = rs;
+            if (ealloc(E, &G.Vtiles, (size_t)G.np * kEnsCols * 2)) return UCY_ERR_HIP;
+            if (ealloc(E, &G.Tpart, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
+            bytes += 4.0 * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
+        } else {
+            E->allocs.push_back(K64);
+            G.K64 = K64;
+            if (ealloc(E, &G.V64, (size_t)G.n * kEnsCols)) return UCY_ERR_HIP;
+            if (ealloc(E, &G.T64, (size_t)G.m * kEnsCols)) return UCY_ERR_HIP;
+            bytes += 8.0 * G.m * G.n + 8.0 * G.n * kEnsCols;
+        }
+        // ---- state and control
+        const size_t S7 = (size_t)7 * G.N * kEnsCols;
+        double **vecs[] = {&G.y, &G.dydt, &G.yscal, &G.ytmp, &G.ak[2], &G.ak[3], &G.ak[4],
+                           &G.ak[5], &G.ak[6], &G.ycand, &G.yerr};
+        for (double **v : vecs)
+            if (ealloc(E, v, S7)) return UCY_ERR_HIP;
+        G.ak[0] = G.ak[1] = G.dydt;
+        // friction SoA [e][inst]
+        const size_t F = (size_t)G.N * kEnsCols;
+        double **fv[] = {&G.fr.a, &G.fr.b, &G.fr.sig, &G.fr.L, &G.fr.damping, &G.fr.vplc, &G.fr.vpls};
+        for (double **v : fv)
+            if (ealloc(E, v, F)) return UCY_ERR_HIP;
+        {
+            std::vector<double> fa(F, 0.0), fb(F, 0.0), fs(F, 1.0), fl(F, 1.0), fd(F, 0.0),
+                fc(F, 0.0), fsn(F, 0.0);
+            for (int i = 0; i < H.ninst; i++)
+                for (int e = 0; e < H.N; e++) {
+                    const size_t src = (size_t)i * H.N + e, dst = (size_t)e * kEnsCols + i;
+                    fa[dst] = H.a[src];
+                    fb[dst] = H.b[src];
+                    fs[dst] = H.sig[src];
+                    fl[dst] = H.Lc[src];
+                    fd[dst] = H.damping[src];
+                    fc[dst] = H.Vpl[src] * cos(H.rake[src]);
+                    fsn[dst] = H.Vpl[src] * sin(H.rake[src]);
+                }
+            const std::vector<double> *hs[] = {&fa, &fb, &fs, &fl, &fd, &fc, &fsn};
+            double *ds[] = {G.fr.a, G.fr.b, G.fr.sig, G.fr.L, G.fr.damping, G.fr.vplc, G.fr.vpls};
+            for (int k = 0; k < 7; k++)
+                EHIP(hipMemcpyAsync(ds[k], hs[k]->data(), sizeof(double) * F, hipMemcpyHostToDevice, c->stream));
+            EHIP(hipStreamSynchronize(c->stream));
+        }
+        double **cv[] = {&G.time, &G.h, &G.hnext, &G.hdid, &G.errmax, &G.vmax};
+        for (double **v : cv)
+            if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
+        int **ci[] = {&G.need_rhs, &G.accepted, &G.done, &G.status};
+        for (int **v : ci)
+            if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.steps, kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.attempts, kEnsCols)) return UCY_ERR_HIP;
+        T.blk_elem[gi] = eacc;
+        eacc += (G.N + 3) / 4;
+        T.blk_inst[gi] = iacc;
+        iacc += 1;
+    }
+    T.blk_elem[T.ngroups] = eacc;
+    T.blk_inst[T.ngroups] = iacc;
+    T.total_inst = total;
+    T.t_end = 0;
+    T.epsilon = 1e-6;
+    T.max_step = 1.7976931348623157e308;
+    T.max_attempts = 1;
+    if (ealloc(E, &T.all_done, 1)) return UCY_ERR_HIP;
+    void *dt = nullptr;
+    EHIP(hipMalloc(&dt, sizeof(EnsTable)));
+    E->allocs.push_back(dt);
+    E->d_tab = (EnsTable *)dt;
+    E->gemm_bytes = bytes;
+    // ---- tune the split-K tiling on the device: time the all-groups GEMM
+    E->tune_seconds.clear();
+    int best = 0;
+    double best_t = 1e30;
+    const int cands[] = {32, 64, 128, 256, 512};
+    if (E->mode == 0) {
+        hipEvent_t e0, e1;
+        EHIP(hipEventCreate(&e0));
+        EHIP(hipEventCreate(&e1));
+        for (int kb : cands) {
+            set_split(T, kb);
+            bool ok = true;
+            for (int gi = 0; gi < T.ngroups; gi++)
+                if (T.g[gi].ksplit > kMaxSplit) ok = false;
+            if (!ok) {
+                E->tune_seconds.push_back(-1.0);
+                continue;
+            }
+            if (upload_table(E)) return UCY_ERR_HIP;
+            ens_launch_gemm(c->stream, E->d_tab, T);
+            EHIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 5; r++) ens_launch_gemm(c->stream, E->d_tab, T);
+            EHIP(hipEventRecord(e1, c->stream));
+            EHIP(hipEventSynchronize(e1));
+            float ms = 0;
+            EHIP(hipEventElapsedTime(&ms, e0, e1));
+            const double t = 1e-3 * ms / 5;
+            E->tune_seconds.push_back(t);
+            if (t < best_t) {
+                best_t = t;
+                best = kb;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (!best) return efail(E, UCY_ERR_STATE, "no admissible GEMM tiling");
+    }
+    E->kb_per_split = best;
+    set_split(T, best ? best : 64);
+    if (upload_table(E)) return UCY_ERR_HIP;
+    E->built = true;
+    return load_initial(E, 1.0e-3);
+}
+
+extern "C" int ucy_ensemble_reset(ucy_ensemble *E, double dt0)
+{
+    if (!E) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    if (!(dt0 > 0)) return efail(E, UCY_ERR_ARG, "dt0 must be positive");
+    EHIP(hipSetDevice(E->ctx->device));
+    return load_initial(E, dt0);
+}
+
+static int capture(ucy_ensemble *E)
+{
+    if (E->graph) return 0;
+    ucy_ctx *c = E->ctx;
+    const EnsTable &T = E->tab;
+    hipGraph_t g = nullptr;
+    EHIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    ens_launch_step_start(c->stream, E->d_tab, T);
+    ens_launch_gemm(c->stream, E->d_tab, T);
+    ens_launch_post(c->stream, E->d_tab, T, 1);
+    for (int s = 2; s <= 6; s++) {
+        ens_launch_stage(c->stream, E->d_tab, T, s);
+        ens_launch_gemm(c->stream, E->d_tab, T);
+        ens_launch_post(c->stream, E->d_tab, T, s);
+    }
+    ens_launch_finish(c->stream, E->d_tab, T);
+    ens_launch_control(c->stream, E->d_tab, T);
+    ens_launch_commit(c->stream, E->d_tab, T);
+    EHIP(hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return efail(E, UCY_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attempts,
+                                double epsilon, double max_step, ucy_ensemble_stats *stats)
+{
+    if (!E) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    if (!(t_end > 0) || max_attempts <= 0 || !(epsilon > 0))
+        return efail(E, UCY_ERR_ARG, "invalid run parameters");
+    ucy_ctx *c = E->ctx;
+    EHIP(hipSetDevice(c->device));
+    E->tab.t_end = t_end;
+    E->tab.epsilon = epsilon;
+    E->tab.max_step = max_step;
+    E->tab.max_attempts = max_attempts;
+    if (upload_table(E)) return UCY_ERR_HIP;
+    if (capture(E)) return UCY_ERR_HIP;
+    int *h_done = nullptr;
+    EHIP(hipHostMalloc((void **)&h_done, sizeof(int), hipHostMallocDefault));
+    auto t0 = std::chrono::steady_clock::now();
+    int batch = 8;
+    int64_t replays = 0;
+    for (;;) {
+        for (int i = 0; i < batch; i++) EHIP(hipGraphLaunch(E->graph, c->stream));
+        replays += batch;
+        EHIP(hipMemcpyAsync(h_done, E->tab.all_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        EHIP(hipStreamSynchronize(c->stream));
+        if (*h_done >= E->tab.total_inst) break;
+        if (batch < 64) batch *= 2;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    (void)hipHostFree(h_done);
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->min_time = 1e300;
+        for (int gi = 0; gi < E->tab.ngroups; gi++) {
+            const EnsGroup &G = E->tab.g[gi];
+            std::vector<double> tm(kEnsCols);
+            std::vector<long long> st(kEnsCols), at(kEnsCols);
+            std::vector<int> status(kEnsCols);
+            EHIP(hipMemcpy(tm.data(), G.time, sizeof(double) * kEnsCols, hipMemcpyDeviceToHost));
+            EHIP(hipMemcpy(st.data(), G.steps, sizeof(long long) * kEnsCols, hipMemcpyDeviceToHost));
+            EHIP(hipMemcpy(at.data(), G.attempts, sizeof(long long) * kEnsCols, hipMemcpyDeviceToHost));
+            EHIP(hipMemcpy(status.data(), G.status, sizeof(int) * kEnsCols, hipMemcpyDeviceToHost));
+            for (int i = 0; i < G.ninst; i++) {
+                stats->inst_time_sum += tm[i];
+                stats->min_time = std::min(stats->min_time, tm[i]);
+                stats->max_time = std::max(stats->max_time, tm[i]);
+                stats->steps += st[i];
+                stats->attempts += at[i];
+                stats->n_failed += status[i] != 0;
+            }
+            stats->n_inst += G.ninst;
+        }
+        stats->replays = replays;
+        stats->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_get_state(ucy_ensemble *E, int group, double *y, double *time,
+                                      double *dt_next, int64_t *steps, int64_t *attempts)
+{
+    if (!E || group < 0 || group >= (int)E->hg.size()) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    const EnsGroup &G = E->tab.g[group];
+    const HostGroup &H = E->hg[group];
+    if (y) {
+        std::vector<double> Y((size_t)7 * G.N * kEnsCols);
+        EHIP(hipMemcpy(Y.data(), G.y, sizeof(double) * Y.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < H.ninst; i++)
+            for (int e = 0; e < H.N; e++)
+                for (int k = 0; k < 7; k++)
+                    y[((size_t)i * H.N + e) * 7 + k] = Y[((size_t)k * G.N + e) * kEnsCols + i];
+    }
+    std::vector<double> tmp(kEnsCols);
+    std::vector<long long> lt(kEnsCols);
+    if (time) {
+        EHIP(hipMemcpy(tmp.data(), G.time, sizeof(double) * kEnsCols, hipMemcpyDeviceToHost));
+        std::copy(tmp.begin(), tmp.begin() + H.ninst, time);
+    }
+    if (dt_next) {
+        EHIP(hipMemcpy(tmp.data(), G.h, sizeof(double) * kEnsCols, hipMemcpyDeviceToHost));
+        std::copy(tmp.begin(), tmp.begin() + H.ninst, dt_next);
+    }
+    if (steps) {
+        EHIP(hipMemcpy(lt.data(), G.steps, sizeof(long long) * kEnsCols, hipMemcpyDeviceToHost));
+        for (int i = 0; i < H.ninst; i++) steps[i] = lt[i];
+    }
+    if (attempts) {
+        EHIP(hipMemcpy(lt.data(), G.attempts, sizeof(long long) * kEnsCols, hipMemcpyDeviceToHost));
+        for (int i = 0; i < H.ninst; i++) attempts[i] = lt[i];
+    }
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_tuning(const ucy_ensemble *E, int *kb_per_split, double *gemm_seconds)
+{
+    if (!E) return UCY_ERR_ARG;
+    if (kb_per_split) *kb_per_split = E->kb_per_split;
+    if (gemm_seconds)
+        for (size_t i = 0; i < E->tune_seconds.size(); i++) gemm_seconds[i] = E->tune_seconds[i];
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_time_gemm(ucy_ensemble *E, int iters, double *avg_seconds, double *bytes)
+{
+    if (!E || iters <= 0) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    ucy_ctx *c = E->ctx;
+    EHIP(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    EHIP(hipEventCreate(&e0));
+    EHIP(hipEventCreate(&e1));
+    ens_launch_gemm(c->stream, E->d_tab, E->tab);
+    EHIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; i++) ens_launch_gemm(c->stream, E->d_tab, E->tab);
+    EHIP(hipEventRecord(e1, c->stream));
+    EHIP(hipEventSynchronize(e1));
+    float ms = 0;
+    EHIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (avg_seconds) *avg_seconds = 1e-3 * ms / iters;
+    if (bytes) *bytes = E->gemm_bytes;
+    return UCY_OK;
+}
+
+namespace ucy {
+void ens_launch_matvec_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g,
+                             const double *V, double *T);
+}
+
+extern "C" int ucy_ensemble_matvec(ucy_ensemble *E, int group, const double *V, double *T)
+{
+    if (!E || !V || !T || group < 0 || group >= (int)E->hg.size()) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    ucy_ctx *c = E->ctx;
+    EHIP(hipSetDevice(c->device));
+    const EnsGroup &G = E->tab.g[group];
+    double *dV = nullptr, *dT = nullptr;
+    EHIP(hipMalloc((void **)&dV, sizeof(double) * (size_t)G.n * G.ninst));
+    EHIP(hipMalloc((void **)&dT, sizeof(double) * (size_t)G.m * G.ninst));
+    EHIP(hipMemcpyAsync(dV, V, sizeof(double) * (size_t)G.n * G.ninst, hipMemcpyHostToDevice, c->stream));
+    ens_launch_matvec_group(c->stream, E->d_tab, E->tab, group, dV, dT);
+    EHIP(hipGetLastError());
+    EHIP(hipMemcpyAsync(T, dT, sizeof(double) * (size_t)G.m * G.ninst, hipMemcpyDeviceToHost, c->stream));
+    EHIP(hipStreamSynchronize(c->stream));
+    EHIP(hipFree(dV));
+    EHIP(hipFree(dT));
+    return UCY_OK;
+}

@@ -1,0 +1,77 @@
+// ens_internal.h -- batched ensembles of independent fault instances.
+//
+// No counterpart in the reference: every instance runs the reference's
+// per-instance algorithm (odefun ratestate.f90:624-778, RKQSm/RKCKm
+// ode45.f90:51-202, main loop ratestate.f90:307-328) with its own friction,
+// state, time and step size; instances of one geometry group share the
+// kernel K, so the traction matvecs of all instances of a group become one
+// GEMM  T(m x B) = K(m x n) V(n x B).
+//
+// Storage (per group, B = 64 instance columns, padded):
+//   state vectors  SoA  X[(comp*N + e)*64 + inst]   (instance fastest)
+//   K split bf16   pre-tiled in MFMA operand order, see k_ensemble.hip
+//   row scales     exact powers of two, double
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ucy {
+
+constexpr int kEnsCols = 64;     // instance columns per group (padded)
+constexpr int kMaxGroups = 8;
+
+struct EnsFriction {             // per (element, instance), SoA arrays [e*64+inst]
+    double *a, *b, *sig, *L, *damping, *vplc, *vpls;
+};
+
+struct EnsGroup {
+    int N;            // patches
+    int ninst;        // live instances (<= 64)
+    int m, n;         // K rows (3N), columns (2N)
+    int mp, np;       // padded to 64 rows / 16 columns
+    int nkb;          // np / 16
+    int ksplit;       // split-K factor of the GEMM
+    int kb_per_split; // k-blocks per split
+    int mode;         // 0 split-bf16 MFMA, 1 binary64
+    const uint16_t *Ktiles;  // split-bf16 tiles (mode 0)
+    const double *rscale;    // row scales (mode 0)
+    const double *K64;       // binary64 K row-major (mode 1)
+    uint16_t *Vtiles;        // V split-bf16 tiles (mode 0)
+    double *V64;             // V binary64 [col][inst] (mode 1)
+    float *Tpart;            // [ksplit][mp][64] partial products (mode 0)
+    double *T64;             // [m][64] (mode 1)
+    // state (SoA, 7 components)
+    double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr;
+    EnsFriction fr;
+    // per-instance control
+    double *time, *h, *hnext, *hdid, *errmax, *vmax;
+    int *need_rhs, *accepted, *done, *status;
+    long long *steps, *attempts;
+};
+
+struct EnsTable {
+    int ngroups;
+    EnsGroup g[kMaxGroups];
+    int blk_gemm[kMaxGroups + 1];   // prefix sums of GEMM blocks per group
+    int blk_elem[kMaxGroups + 1];   // prefix sums of elementwise blocks
+    int blk_inst[kMaxGroups + 1];   // prefix sums of per-instance blocks
+    double t_end, epsilon, max_step;
+    long long max_attempts;
+    int *all_done;                  // count of finished instances
+    int total_inst;
+};
+
+// launchers (k_ensemble.hip)
+void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,
+                        double *rscale);
+void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
+void ens_launch_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
+void ens_launch_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+// standalone GEMM check: T(m x 64) = K V with the group's storage mode
+void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int grp);
+
+}  // namespace ucy

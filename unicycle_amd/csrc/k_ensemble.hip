@@ -1,0 +1,585 @@
+// k_ensemble.hip -- batched ensembles: split-bf16 K on MFMA, binary64 state,
+// per-instance Cash-Karp RK45 step control.
+//
+// Per-instance arithmetic is the reference's (ratestate.f90:624-778 odefun,
+// ode45.f90:51-202 RKQSm/RKCKm, ratestate.f90:307-328 main loop); only the
+// traction matvec changes: for the 64 instance columns of a group,
+//     T = K V   with  K = diag(s) (Khi + Klo),  V = Vhi + Vlo  (bf16 pairs)
+//     T ~= diag(s) (Khi Vhi + Khi Vlo + Klo Vhi)   accumulated in fp32
+// on v_mfma_f32_32x32x16_bf16.  s are exact powers of two (no rounding).
+//
+// Tiled operand layouts (one 1 KiB tile = one wavefront's 16-byte-per-lane
+// load, lane l holds 8 consecutive k of row/column l%32, k-half l/32):
+//   K tile (rb, kb, part)   rows 32*rb.., k 16*kb..,  part 0 hi / 1 lo
+//       offset  (((rb*nkb + kb)*2 + part)*64 + lane)*8 + j
+//   V tile (kb, cb, part)   k 16*kb.., columns 32*cb..
+//       offset  (((kb*2 + cb)*2 + part)*64 + lane)*8 + j
+#include <hip/hip_runtime.h>
+#include "ens_internal.h"
+
+namespace ucy {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__constant__ const double kELg10 = 2.302585092994045684;
+
+// ---------------------------------------------------------------------------
+// binary64 -> bf16 with round-to-nearest-even, directly from the binary64
+// value (no intermediate binary32 rounding).  Returns the bf16 bits and the
+// rounded value as a double.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t bf16_rne(double x, double &xr)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long exp = (u >> 52) & 0x7ff;
+    if (exp == 0 || exp == 0x7ff) {  // zero / subnormal-binary64 / inf / nan
+        xr = exp == 0 ? 0.0 : x;
+        const float f = (float)xr;
+        return (uint16_t)(__float_as_uint(f) >> 16);
+    }
+    const unsigned long long lsb = (u >> 45) & 1ull;
+    u += 0x00001FFFFFFFFFFFull + lsb;
+    u &= ~0x00001FFFFFFFFFFFull;
+    xr = __longlong_as_double((long long)u);
+    const float f = (float)xr;  // exact: 8 significant bits, binary32 range
+    return (uint16_t)(__float_as_uint(f) >> 16);
+}
+
+__device__ __forceinline__ void split2(double x, uint16_t &hi, uint16_t &lo)
+{
+    double h, l;
+    hi = bf16_rne(x, h);
+    lo = bf16_rne(x - h, l);
+}
+
+__device__ __forceinline__ size_t ktile_off(int r, int c, int nkb, int part)
+{
+    const int rb = r >> 5, kb = c >> 4, w = c & 15;
+    const int lane = (r & 31) + 32 * (w >> 3), j = w & 7;
+    return ((((size_t)rb * nkb + kb) * 2 + part) * 64 + lane) * 8 + j;
+}
+
+__device__ __forceinline__ size_t vtile_off(int c, int col, int part)
+{
+    const int kb = c >> 4, w = c & 15, cb = col >> 5;
+    const int lane = (col & 31) + 32 * (w >> 3), j = w & 7;
+    return ((((size_t)kb * 2 + cb) * 2 + part) * 64 + lane) * 8 + j;
+}
+
+// ---------------------------------------------------------------------------
+// K -> split bf16 tiles with exact power-of-two row scales
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) rowscale_kernel(const double *__restrict__ K, int m, int n,
+                                                       int mp, double *rscale)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= mp) return;
+    double mx = 0.0;
+    if (r < m)
+        for (int c = lane; c < n; c += 64) mx = fmax(mx, fabs(K[(size_t)r * n + c]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if (lane == 0) {
+        double s = 1.0;
+        if (mx > 0.0) {
+            int e;
+            frexp(mx, &e);          // mx = f * 2^e, f in [0.5, 1)
+            s = ldexp(1.0, e);      // mx / s in [0.5, 1)
+        }
+        rscale[r] = s;
+    }
+}
+
+__global__ void split_kernel(const double *__restrict__ K, int m, int n, int mp, int np,
+                             const double *__restrict__ rscale, uint16_t *__restrict__ tiles)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)mp * np) return;
+    const int r = (int)(i / np), c = (int)(i % np);
+    const double x = (r < m && c < n) ? K[(size_t)r * n + c] / rscale[r] : 0.0;
+    uint16_t hi, lo;
+    split2(x, hi, lo);
+    const int nkb = np >> 4;
+    tiles[ktile_off(r, c, nkb, 0)] = hi;
+    tiles[ktile_off(r, c, nkb, 1)] = lo;
+}
+
+void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np,
+                        uint16_t *tiles, double *rscale)
+{
+    hipLaunchKernelGGL(rowscale_kernel, dim3((mp + 3) / 4), dim3(256), 0, st, K, m, n, mp, rscale);
+    const long long tot = (long long)mp * np;
+    hipLaunchKernelGGL(split_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, K, m, n,
+                       mp, np, rscale, tiles);
+}
+
+// ---------------------------------------------------------------------------
+// group lookup from a block index and a prefix table
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int group_of(const int *prefix, int ng, int blk)
+{
+    int g = 0;
+    while (g + 1 < ng && blk >= prefix[g + 1]) g++;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// split-bf16 GEMM: one wavefront = 64 rows x 64 columns over one k-split.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
+{
+    const int nrp = G.mp >> 6;               // row pairs of 32-row blocks
+    const int rp = item % nrp, sp = item / nrp;
+    const int kb0 = sp * G.kb_per_split;
+    int kb1 = kb0 + G.kb_per_split;
+    if (kb1 > G.nkb) kb1 = G.nkb;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const u32x4 *A0 = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(2 * rp) * G.nkb * 2) * 64 + lane;
+    const u32x4 *A1 = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(2 * rp + 1) * G.nkb * 2) * 64 + lane;
+    const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
+#pragma unroll 2
+    for (int kb = kb0; kb < kb1; kb++) {
+        const u32x4 a0h = __builtin_nontemporal_load(A0 + (size_t)kb * 128);
+        const u32x4 a0l = __builtin_nontemporal_load(A0 + (size_t)kb * 128 + 64);
+        const u32x4 a1h = __builtin_nontemporal_load(A1 + (size_t)kb * 128);
+        const u32x4 a1l = __builtin_nontemporal_load(A1 + (size_t)kb * 128 + 64);
+        const u32x4 b0h = Bv[(size_t)kb * 256];
+        const u32x4 b0l = Bv[(size_t)kb * 256 + 64];
+        const u32x4 b1h = Bv[(size_t)kb * 256 + 128];
+        const u32x4 b1l = Bv[(size_t)kb * 256 + 192];
+        const bf16x8 A[2][2] = {{__builtin_bit_cast(bf16x8, a0h), __builtin_bit_cast(bf16x8, a0l)},
+                                {__builtin_bit_cast(bf16x8, a1h), __builtin_bit_cast(bf16x8, a1l)}};
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, b0h), __builtin_bit_cast(bf16x8, b0l)},
+                                {__builtin_bit_cast(bf16x8, b1h), __builtin_bit_cast(bf16x8, b1l)}};
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][1], B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][0], B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][0], B[b][0], acc[a][b], 0, 0, 0);
+            }
+    }
+    // 32x32 fp32 accumulator layout: value i of lane l is row 8*(i/4) + 4*(l/32) + i%4,
+    // column l%32 of the tile
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 64 * rp + 32 * a + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+}
+
+// binary64 batched matvec: one wavefront = 8 rows x 64 instance columns
+__device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int lane)
+{
+    const int r0 = item * 8;
+    if (r0 >= G.m) return;
+    double acc[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) acc[r] = 0.0;
+    const int nr = G.m - r0 < 8 ? G.m - r0 : 8;
+    for (int c = 0; c < G.n; c++) {
+        const double v = G.V64[(size_t)c * kEnsCols + lane];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            if (r < nr) acc[r] = fma(G.K64[(size_t)(r0 + r) * G.n + c], v, acc[r]);
+    }
+    for (int r = 0; r < nr; r++) G.T64[(size_t)(r0 + r) * kEnsCols + lane] = acc[r];
+}
+
+__device__ __forceinline__ int gemm_items(const EnsGroup &G)
+{
+    return G.mode == 0 ? (G.mp >> 6) * G.ksplit : (G.m + 7) / 8;
+}
+
+__global__ void __launch_bounds__(256) gemm_kernel(const EnsTable *__restrict__ tab)
+{
+    const int blk = blockIdx.x;
+    const int g = group_of(tab->blk_gemm, tab->ngroups, blk);
+    const EnsGroup &G = tab->g[g];
+    const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
+    if (item >= gemm_items(G)) return;
+    const int lane = threadIdx.x & 63;
+    if (G.mode == 0) gemm_wave_bf16(G, item, lane);
+    else gemm_wave_f64(G, item, lane);
+}
+
+void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab)
+{
+    hipLaunchKernelGGL(gemm_kernel, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0, st, dtab);
+}
+
+__global__ void __launch_bounds__(256) gemm_group_kernel(const EnsTable *__restrict__ tab, int g)
+{
+    const EnsGroup &G = tab->g[g];
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= gemm_items(G)) return;
+    const int lane = threadIdx.x & 63;
+    if (G.mode == 0) gemm_wave_bf16(G, item, lane);
+    else gemm_wave_f64(G, item, lane);
+}
+
+void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g)
+{
+    const EnsGroup &G = htab.g[g];
+    const int items = G.mode == 0 ? (G.mp >> 6) * G.ksplit : (G.m + 7) / 8;
+    hipLaunchKernelGGL(gemm_group_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
+}
+
+// ---------------------------------------------------------------------------
+// per-(element, instance) pieces of odefun
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t sidx(const EnsGroup &G, int comp, int e, int inst)
+{
+    return ((size_t)comp * G.N + e) * kEnsCols + inst;
+}
+
+// pre-pass (ratestate.f90:683-702) of element e of instance inst at state X;
+// writes the slip rates to out[0..1] and the bf16 (or binary64) V entries
+__device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X, double *out,
+                                              int e, int inst)
+{
+    const size_t f = (size_t)e * kEnsCols + inst;
+    const double rake = atan2(X[sidx(G, 3, e, inst)], X[sidx(G, 2, e, inst)]);
+    const double velocity = exp(X[sidx(G, 6, e, inst)] * kELg10);
+    const double vs = velocity * cos(rake), vd = velocity * sin(rake);
+    out[sidx(G, 0, e, inst)] = vs;
+    out[sidx(G, 1, e, inst)] = vd;
+    const double v0 = vs - G.fr.vplc[f], v1 = vd - G.fr.vpls[f];
+    if (G.mode == 0) {
+        uint16_t hi, lo;
+        split2(v0, hi, lo);
+        G.Vtiles[vtile_off(2 * e, inst, 0)] = hi;
+        G.Vtiles[vtile_off(2 * e, inst, 1)] = lo;
+        split2(v1, hi, lo);
+        G.Vtiles[vtile_off(2 * e + 1, inst, 0)] = hi;
+        G.Vtiles[vtile_off(2 * e + 1, inst, 1)] = lo;
+    } else {
+        G.V64[(size_t)(2 * e) * kEnsCols + inst] = v0;
+        G.V64[(size_t)(2 * e + 1) * kEnsCols + inst] = v1;
+    }
+    return velocity;
+}
+
+__device__ __forceinline__ void ens_traction(const EnsGroup &G, int e, int inst, double t[3])
+{
+    for (int c = 0; c < 3; c++) {
+        const int r = 3 * e + c;
+        if (G.mode == 0) {
+            float s = 0.0f;
+            double acc = 0.0;
+            for (int k = 0; k < G.ksplit; k++) {
+                s = G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
+                acc += (double)s;
+            }
+            t[c] = acc * G.rscale[r];
+        } else {
+            t[c] = G.T64[(size_t)r * kEnsCols + inst];
+        }
+    }
+}
+
+// elementwise launch geometry: blocks of 256 threads = 4 elements x 64 instances
+__device__ __forceinline__ bool elem_coords(const EnsTable *tab, int &g, int &e, int &inst)
+{
+    g = group_of(tab->blk_elem, tab->ngroups, blockIdx.x);
+    const EnsGroup &G = tab->g[g];
+    inst = threadIdx.x & 63;
+    e = (blockIdx.x - tab->blk_elem[g]) * 4 + (threadIdx.x >> 6);
+    return e < G.N && inst < G.ninst;
+}
+
+// step start: odefun pre-pass at y for instances that start a new step
+__global__ void __launch_bounds__(256) ens_step_start_kernel(const EnsTable *__restrict__ tab)
+{
+    int g, e, inst;
+    if (!elem_coords(tab, g, e, inst)) return;
+    const EnsGroup &G = tab->g[g];
+    if (G.done[inst] || !G.need_rhs[inst]) return;
+    const double vel = ens_prepass(G, G.y, G.dydt, e, inst);
+    atomicMax(reinterpret_cast<unsigned long long *>(&G.vmax[inst]),
+              (unsigned long long)__double_as_longlong(vel));
+}
+
+// post-pass (ratestate.f90:747-771) writing stage derivative `stage`
+// (1 = step start, then also yscal of ratestate.f90:315)
+__global__ void __launch_bounds__(256) ens_post_kernel(const EnsTable *__restrict__ tab, int stage)
+{
+    int g, e, inst;
+    if (!elem_coords(tab, g, e, inst)) return;
+    const EnsGroup &G = tab->g[g];
+    if (G.done[inst]) return;
+    if (stage == 1 && !G.need_rhs[inst]) return;
+    const double *X = stage == 1 ? G.y : G.ytmp;
+    double *out = G.ak[stage];
+    const size_t f = (size_t)e * kEnsCols + inst;
+    double t[3];
+    ens_traction(G, e, inst, t);
+    const double rake = atan2(X[sidx(G, 3, e, inst)], X[sidx(G, 2, e, inst)]);
+    const double velocity = exp(X[sidx(G, 6, e, inst)] * kELg10);
+    const double cr = cos(rake), sr = sin(rake);
+    const double d5 = (exp(-X[sidx(G, 5, e, inst)] * kELg10) - velocity / G.fr.L[f]) / kELg10;
+    const double dtau = t[0] * cr + t[1] * sr;
+    const double d6 = (dtau - G.fr.b[f] * G.fr.sig[f] * d5 * kELg10)
+                      / (G.fr.a[f] * G.fr.sig[f] + G.fr.damping[f] * velocity) / kELg10;
+    const double correction = G.fr.damping[f] * velocity * d6 * kELg10;
+    out[sidx(G, 2, e, inst)] = t[0] - correction * cr;
+    out[sidx(G, 3, e, inst)] = t[1] - correction * sr;
+    out[sidx(G, 4, e, inst)] = t[2];
+    out[sidx(G, 5, e, inst)] = d5;
+    out[sidx(G, 6, e, inst)] = d6;
+    if (stage == 1) {
+        const double h = G.h[inst];
+        for (int k = 0; k < 7; k++) {
+            const size_t i = sidx(G, k, e, inst);
+            G.yscal[i] = fabs(G.y[i]) + fabs(h * G.dydt[i]) + 1.0e-30;
+        }
+    }
+}
+
+__device__ __forceinline__ double ens_stage_value(int stage, double h, double y0, double k1,
+                                                  double k2, double k3, double k4, double k5)
+{
+    double yr;
+    switch (stage) {
+    case 2:
+        return y0 + (h * 0.2) * k1;
+    case 3:
+        yr = (3.0 / 40.0) * k1;
+        yr = yr + (9.0 / 40.0) * k2;
+        return y0 + h * yr;
+    case 4:
+        yr = 0.3 * k1;
+        yr = yr + (-0.9) * k2;
+        yr = yr + 1.2 * k3;
+        return y0 + h * yr;
+    case 5:
+        yr = (-11.0 / 54.0) * k1;
+        yr = yr + 2.5 * k2;
+        yr = yr + (-70.0 / 27.0) * k3;
+        yr = yr + (35.0 / 27.0) * k4;
+        return y0 + h * yr;
+    default:
+        yr = (1631.0 / 55296.0) * k1;
+        yr = yr + (175.0 / 512.0) * k2;
+        yr = yr + (575.0 / 13824.0) * k3;
+        yr = yr + (44275.0 / 110592.0) * k4;
+        yr = yr + (253.0 / 4096.0) * k5;
+        return y0 + h * yr;
+    }
+}
+
+__global__ void __launch_bounds__(256) ens_stage_kernel(const EnsTable *__restrict__ tab, int stage)
+{
+    int g, e, inst;
+    if (!elem_coords(tab, g, e, inst)) return;
+    const EnsGroup &G = tab->g[g];
+    if (G.done[inst]) return;
+    if (stage == 2 && e == 0) G.errmax[inst] = 0.0;
+    const double h = G.h[inst];
+    for (int k = 0; k < 7; k++) {
+        const size_t i = sidx(G, k, e, inst);
+        const double k1 = G.dydt[i];
+        const double k2 = stage > 2 ? G.ak[2][i] : 0.0;
+        const double k3 = stage > 3 ? G.ak[3][i] : 0.0;
+        const double k4 = stage > 4 ? G.ak[4][i] : 0.0;
+        const double k5 = stage > 5 ? G.ak[5][i] : 0.0;
+        G.ytmp[i] = ens_stage_value(stage, h, G.y[i], k1, k2, k3, k4, k5);
+    }
+    ens_prepass(G, G.ytmp, G.ak[stage], e, inst);
+}
+
+// fifth-order solution, embedded error, per-instance max |yerr/yscal|
+__global__ void __launch_bounds__(256) ens_finish_kernel(const EnsTable *__restrict__ tab)
+{
+    int g, e, inst;
+    const bool live = elem_coords(tab, g, e, inst);
+    const EnsGroup &G = tab->g[g];
+    double emax = 0.0;
+    if (live && !G.done[inst]) {
+        const double h = G.h[inst];
+        const double DC1 = 37.0 / 378.0 - 2825.0 / 27648.0, DC3 = 250.0 / 621.0 - 18575.0 / 48384.0,
+                     DC4 = 125.0 / 594.0 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0,
+                     DC6 = 512.0 / 1771.0 - 0.25;
+        for (int k = 0; k < 7; k++) {
+            const size_t i = sidx(G, k, e, inst);
+            const double k1 = G.dydt[i], k3 = G.ak[3][i], k4 = G.ak[4][i], k5 = G.ak[5][i],
+                         k6 = G.ak[6][i];
+            double yr = (37.0 / 378.0) * k1;
+            yr = yr + (250.0 / 621.0) * k3;
+            yr = yr + (125.0 / 594.0) * k4;
+            yr = yr + (512.0 / 1771.0) * k6;
+            G.ycand[i] = G.y[i] + h * yr;
+            double er = DC1 * k1;
+            er = er + DC3 * k3;
+            er = er + DC4 * k4;
+            er = er + DC5 * k5;
+            er = er + DC6 * k6;
+            er = h * er;
+            G.yerr[i] = er;
+            const double r = fabs(er / G.yscal[i]);
+            emax = (r > emax || r != r) ? r : emax;
+        }
+    }
+    // the 4 elements of this block share instance columns: reduce across waves
+    __shared__ unsigned long long red[256];
+    red[threadIdx.x] = (unsigned long long)__double_as_longlong(emax);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        unsigned long long b = red[threadIdx.x];
+        for (int w = 1; w < 4; w++) b = red[threadIdx.x + 64 * w] > b ? red[threadIdx.x + 64 * w] : b;
+        if (inst < G.ninst && b)
+            atomicMax(reinterpret_cast<unsigned long long *>(&G.errmax[inst]), b);
+    }
+}
+
+// per-instance RKQSm step control + main-loop time update (ode45.f90:70-105,
+// ratestate.f90:317-326); one thread per instance
+__global__ void ens_control_kernel(EnsTable *tab)
+{
+    const int blk = blockIdx.x;
+    const int g = group_of(tab->blk_inst, tab->ngroups, blk);
+    const EnsGroup &G = tab->g[g];
+    const int inst = (blk - tab->blk_inst[g]) * 64 + threadIdx.x;
+    if (inst >= G.ninst || G.done[inst]) return;
+    const double SAFETY = 0.7, PGROW = -0.2, PSHRNK = -0.15, ERRCON = 1.89e-4;
+    G.attempts[inst] += 1;
+    const double raw = G.errmax[inst];
+    G.accepted[inst] = 0;
+    if (!(raw == raw) || raw > 1.7976931348623157e308) {
+        G.status[inst] = 2;
+        G.done[inst] = 1;
+        atomicAdd(tab->all_done, 1);
+        return;
+    }
+    const double errmax = raw / tab->epsilon;
+    double h = G.h[inst];
+    if (errmax > 1.0) {
+        h = h * SAFETY * pow(errmax, PSHRNK);
+        G.h[inst] = h;
+        G.need_rhs[inst] = 0;
+        if (0.0 + h == 0.0) {
+            G.status[inst] = 1;
+            G.done[inst] = 1;
+            atomicAdd(tab->all_done, 1);
+        }
+    } else {
+        double hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
+        if (hnext > tab->max_step) hnext = tab->max_step;
+        G.hdid[inst] = h;
+        G.hnext[inst] = hnext;
+        G.time[inst] = G.time[inst] + h;
+        G.steps[inst] += 1;
+        G.accepted[inst] = 1;
+        G.need_rhs[inst] = 1;
+        G.h[inst] = hnext;
+        G.vmax[inst] = 0.0;
+        if (tab->t_end <= G.time[inst]) {
+            G.done[inst] = 1;
+            atomicAdd(tab->all_done, 1);
+        }
+    }
+    if (!G.done[inst] && G.attempts[inst] >= tab->max_attempts) {
+        G.done[inst] = 1;
+        atomicAdd(tab->all_done, 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) ens_commit_kernel(const EnsTable *__restrict__ tab)
+{
+    int g, e, inst;
+    if (!elem_coords(tab, g, e, inst)) return;
+    const EnsGroup &G = tab->g[g];
+    if (!G.accepted[inst]) return;
+    for (int k = 0; k < 7; k++) {
+        const size_t i = sidx(G, k, e, inst);
+        G.y[i] = G.ycand[i];
+    }
+}
+
+void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+{
+    hipLaunchKernelGGL(ens_step_start_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
+}
+void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &h, int stage)
+{
+    hipLaunchKernelGGL(ens_post_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab, stage);
+}
+void ens_launch_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &h, int stage)
+{
+    hipLaunchKernelGGL(ens_stage_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab, stage);
+}
+void ens_launch_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+{
+    hipLaunchKernelGGL(ens_finish_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
+}
+void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+{
+    hipLaunchKernelGGL(ens_control_kernel, dim3(h.blk_inst[h.ngroups]), dim3(64), 0, st,
+                       const_cast<EnsTable *>(dtab));
+}
+void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+{
+    hipLaunchKernelGGL(ens_commit_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
+}
+
+}  // namespace ucy
+
+namespace ucy {
+// V (n x ninst, column = instance, row-major [c][inst]) -> the group's V storage
+__global__ void ens_fill_v_kernel(const EnsTable *__restrict__ tab, int g, const double *__restrict__ V)
+{
+    const EnsGroup &G = tab->g[g];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n * kEnsCols) return;
+    const int c = i / kEnsCols, inst = i % kEnsCols;
+    const double v = inst < G.ninst ? V[(size_t)c * G.ninst + inst] : 0.0;
+    if (G.mode == 0) {
+        uint16_t hi, lo;
+        split2(v, hi, lo);
+        G.Vtiles[vtile_off(c, inst, 0)] = hi;
+        G.Vtiles[vtile_off(c, inst, 1)] = lo;
+    } else {
+        G.V64[i] = v;
+    }
+}
+
+// T (m x ninst) from the partial products
+__global__ void ens_reduce_t_kernel(const EnsTable *__restrict__ tab, int g, double *__restrict__ T)
+{
+    const EnsGroup &G = tab->g[g];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.m * G.ninst) return;
+    const int r = i / G.ninst, inst = i % G.ninst;
+    if (G.mode == 0) {
+        double acc = 0.0;
+        for (int k = 0; k < G.ksplit; k++) acc += (double)G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
+        T[i] = acc * G.rscale[r];
+    } else {
+        T[i] = G.T64[(size_t)r * kEnsCols + inst];
+    }
+}
+
+void ens_launch_matvec_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g,
+                             const double *V, double *T)
+{
+    const EnsGroup &G = htab.g[g];
+    hipLaunchKernelGGL(ens_fill_v_kernel, dim3((G.n * kEnsCols + 255) / 256), dim3(256), 0, st, dtab, g, V);
+    ens_launch_gemm_group(st, dtab, htab, g);
+    hipLaunchKernelGGL(ens_reduce_t_kernel, dim3((G.m * G.ninst + 255) / 256), dim3(256), 0, st, dtab, g, T);
+}
+}  // namespace ucy

@@ -1,0 +1,64 @@
+// ucy_ctx.h -- the context behind the opaque ucy_ctx handle of include/ucy.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "ucy_internal.h"
+
+using ucy::Control;
+using ucy::Friction;
+using ucy::Rheology;
+using ucy::RkBuffers;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ucy_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // elastic moduli
+    double lambda = 0, mu = 0;
+    bool have_elastic = false;
+    // rectangles
+    int nrect = 0;
+    std::vector<double> rx, rL, rW, rstrike, rdip;
+    // triangles
+    int ntri = 0, nve = 0;
+    std::vector<int> i1, i2, i3;
+    std::vector<double> vert;
+    // volumes
+    int nvol = 0;
+    std::vector<double> vx, vL, vT, vW, vstrike;
+    // friction / rheology (element order)
+    std::vector<double> Vpl, rake, mu0, sig, a, b, Lc, Vo, damping;
+    std::vector<double> gd0, npow, ebg, Gm, Qa, Rg, To;
+    bool have_friction = false, have_rheology = false;
+    // device kernel
+    double *K = nullptr;
+    int64_t m = 0, n = 0;
+    // device per-element data
+    Friction *d_fr = nullptr;
+    Rheology *d_rh = nullptr;
+    bool params_dirty = true;
+    // work buffers
+    double *work = nullptr;
+    size_t work_dim = 0;
+    RkBuffers rb{};
+    Control *d_ctl = nullptr;
+    Control *h_ctl = nullptr;  // pinned
+    // captured attempt
+    hipGraphExec_t graph = nullptr;
+    // scratch for the per-kernel hooks
+    DevBuf hook;
+};
+
+
+// host-side reference arithmetic shared by the single-instance and ensemble
+// paths (capi.hip)
+ucy::OkadaMedium make_medium(double G, double lambda);
+ucy::OkadaSource make_source(const double *x, double L, double W, double strike, double dipd);
+void refsys_okada(int ns, const double *x, const double *L, const double *W, const double *strike,
+                  const double *dip, double *sv, double *dv, double *nv, double *xc);

@@ -1,0 +1,115 @@
+"""Batched ensembles of independent fault instances (no counterpart in the
+reference; each instance runs the reference's per-instance algorithm).
+
+Instances are grouped by geometry; a group shares one kernel K stored in the
+ensemble precision (split bf16 hi+lo with exact row scales on MFMA, or
+binary64).  State, time, step control and error norms stay binary64 per
+instance.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .engine import Engine, UcyError, PREC_F64, PREC_SPLIT_BF16, _p  # noqa: F401
+
+
+@dataclass
+class EnsembleStats:
+    inst_time_sum: float
+    min_time: float
+    max_time: float
+    steps: int
+    attempts: int
+    replays: int
+    n_inst: int
+    n_failed: int
+    wall_seconds: float
+
+
+class Ensemble:
+    def __init__(self, engine: Engine, precision: int = PREC_SPLIT_BF16):
+        self.eng = engine
+        self._L = engine._L
+        self._h = self._L.ucy_ensemble_create(engine._h, precision)
+        if not self._h:
+            raise UcyError(-1, "cannot create ensemble")
+        self.groups = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ucy_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        self.eng._chk(st)
+
+    def add_group(self, model, a, b, sig, L, y0):
+        """model: configs.FaultModel geometry and the shared Vpl/rake/damping;
+        a, b, sig, L: (n_inst, N) per-instance friction; y0: (n_inst, 7N)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n_inst, N = a.shape
+        rep = lambda v: np.ascontiguousarray(np.broadcast_to(v, (n_inst, N)), dtype=np.float64)
+        arrs = [rep(model.Vpl), rep(model.rake), a, rep(b), rep(sig), rep(L), rep(model.damping),
+                np.ascontiguousarray(y0, dtype=np.float64).reshape(n_inst, 7 * N)]
+        geo = [np.ascontiguousarray(g, dtype=np.float64) for g in model.geometry()]
+        st = self._L.ucy_ensemble_add_group(self._h, N, *[_p(g) for g in geo],
+                                            float(model.lambda_), float(model.mu), n_inst,
+                                            *[_p(v) for v in arrs])
+        if st < 0:
+            self._chk(st)
+        self.groups.append((N, n_inst))
+        return st
+
+    def build(self):
+        self._chk(self._L.ucy_ensemble_build(self._h))
+
+    def reset(self, dt0: float = 1e-3):
+        self._chk(self._L.ucy_ensemble_reset(self._h, float(dt0)))
+
+    def run(self, t_end: float, max_attempts: int = 10**9, epsilon: float = 1e-6,
+            max_step: float = 1.7976931348623158e308) -> EnsembleStats:
+        st = _lib.EnsembleStats()
+        self._chk(self._L.ucy_ensemble_run(self._h, float(t_end), int(max_attempts),
+                                           float(epsilon), float(max_step), C.byref(st)))
+        return EnsembleStats(st.inst_time_sum, st.min_time, st.max_time, st.steps, st.attempts,
+                             st.replays, st.n_inst, st.n_failed, st.wall_seconds)
+
+    def state(self, group: int):
+        N, n_inst = self.groups[group]
+        y = np.empty((n_inst, 7 * N))
+        t = np.empty(n_inst)
+        dt = np.empty(n_inst)
+        steps = np.empty(n_inst, dtype=np.int64)
+        att = np.empty(n_inst, dtype=np.int64)
+        self._chk(self._L.ucy_ensemble_get_state(
+            self._h, group, _p(y), _p(t), _p(dt),
+            steps.ctypes.data_as(C.POINTER(C.c_int64)), att.ctypes.data_as(C.POINTER(C.c_int64))))
+        return y, t, dt, steps, att
+
+    def matvec(self, group: int, V) -> np.ndarray:
+        N, n_inst = self.groups[group]
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(2 * N, n_inst)
+        T = np.empty((3 * N, n_inst))
+        self._chk(self._L.ucy_ensemble_matvec(self._h, group, _p(V), _p(T)))
+        return T
+
+    def tuning(self):
+        kb = C.c_int()
+        t = np.full(5, -1.0)
+        self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), _p(t)))
+        return kb.value, t
+
+    def time_gemm(self, iters: int = 20):
+        s, b = C.c_double(), C.c_double()
+        self._chk(self._L.ucy_ensemble_time_gemm(self._h, int(iters), C.byref(s), C.byref(b)))
+        return s.value, b.value
