@@ -1,0 +1,204 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own
+Green's-function routines, compiled from jdpmoore/unicycle fortran/3d/src by
+oracle/Makefile into oracle/_ref/libucyref.so (run `make -C oracle ref`).
+
+The fixtures are inputs and the reference's outputs only:
+  okada_dc3d.npz        DC3D (dc3d.f:1-199) on random binary32 inputs, plus
+                        singular-edge and negative-extension cases
+  okada_traction.npz    computeTractionKernelsOkada92 / computeStressKernels-
+                        Okada92 (okada92.f90:368-473) for random sources
+  nikkhoo_strain.npz    straininhalfspace_ (StrainInHalfSpace.c:42) at random
+                        points, three unit slips, incl. a surface triangle and
+                        a horizontal triangle
+  nikkhoo_traction.npz  computeTractionKernelsNikkhoo15 / computeStressKernels-
+                        Nikkhoo15 (nikkhoo15.f90:187-304)
+  strainvolume.npz      computeTractionKernels / computeStressKernels-
+                        VerticalStrainVolume (strainvolume.f90:4671-4799)
+
+Usage: python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libucyref.so"))
+D, F, I = C.c_double, C.c_float, C.c_int
+
+
+def P(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def PI(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def dc3d_cases(rng, n=400):
+    rows = []
+    for _ in range(n):
+        rows.append([2 / 3, rng.uniform(-5e3, 5e3), rng.uniform(-5e3, 5e3), -rng.uniform(0, 5e3),
+                     rng.uniform(0, 5e3), rng.uniform(1, 90), 0.0, rng.uniform(500, 3000),
+                     -rng.uniform(500, 3000), 0.0])
+    # receiver on the fault edge (singular) and on a negative edge extension
+    rows.append([2 / 3, 0.0, 0.0, -1000.0, 1000.0, 90.0, 0.0, 1000.0, -1000.0, 0.0])
+    rows.append([2 / 3, -500.0, 0.0, -1000.0, 1000.0, 90.0, 0.0, 1000.0, -1000.0, 0.0])
+    rows.append([2 / 3, 500.0, 0.0, -500.0, 1000.0, 90.0, 0.0, 1000.0, -1000.0, 0.0])
+    rows.append([2 / 3, 1500.0, 300.0, 0.0, 1000.0, 45.0, 0.0, 1000.0, -1000.0, 0.0])
+    inp = np.array(rows, dtype=np.float32)
+    out = np.zeros((len(rows), 3, 12), dtype=np.float32)
+    iret = np.zeros((len(rows), 3), dtype=np.int32)
+    for i, r in enumerate(inp):
+        for s in range(3):
+            u = (F * 12)()
+            ir = I()
+            disl = [0.0, 0.0, 0.0]
+            disl[s] = 1.0
+            REF.ref_dc3d(*[F(v) for v in r], *[F(v) for v in disl], u, C.byref(ir))
+            out[i, s] = np.array(u[:], dtype=np.float32)
+            iret[i, s] = ir.value
+    np.savez_compressed(os.path.join(HERE, "okada_dc3d.npz"), inputs=inp, u=out, iret=iret)
+
+
+def rect_sources(rng, ns):
+    y = np.zeros((ns, 3))
+    y[:, 0] = rng.uniform(-20e3, 20e3, ns)
+    y[:, 1] = rng.uniform(-20e3, 20e3, ns)
+    y[:, 2] = rng.uniform(0, 20e3, ns)
+    return (y, rng.uniform(500, 5000, ns), rng.uniform(500, 5000, ns),
+            rng.uniform(0, 2 * np.pi, ns), rng.uniform(0.05, np.pi / 2, ns))
+
+
+def receiver(rng):
+    x = np.array([rng.uniform(-5e3, 5e3), rng.uniform(-5e3, 5e3), rng.uniform(100, 10e3)])
+    st, dp = rng.uniform(0, 2 * np.pi), rng.uniform(0.1, np.pi / 2)
+    sv = np.array([np.cos(st), np.sin(st), 0.0])
+    dv = np.array([np.sin(st) * np.cos(dp), -np.cos(st) * np.cos(dp), -np.sin(dp)])
+    nv = np.array([-np.sin(st) * np.sin(dp), np.cos(st) * np.sin(dp), -np.cos(dp)])
+    return x, sv, dv, nv
+
+
+def okada_kernels(rng, ns=300, nrcv=4):
+    y, L, W, st, dp = rect_sources(rng, ns)
+    rcv, trac, stress = [], [], []
+    for _ in range(nrcv):
+        x, sv, dv, nv = receiver(rng)
+        rcv.append(np.concatenate([x, sv, dv, nv]))
+        for s, d in [(1.0, 0.0), (0.0, 1.0)]:
+            t = [np.zeros(ns) for _ in range(3)]
+            REF.ref_traction_okada92(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)), P(L),
+                                     P(W), P(st), P(dp), D(s), D(d), D(0.0), D(30e3), D(30e3),
+                                     *[P(a) for a in t])
+            trac.append(np.stack(t))
+            q = [np.zeros(ns) for _ in range(6)]
+            REF.ref_stress_okada92(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)), P(L),
+                                   P(W), P(st), P(dp), D(s), D(d), D(0.0), D(30e3), D(30e3),
+                                   *[P(a) for a in q])
+            stress.append(np.stack(q))
+    np.savez_compressed(os.path.join(HERE, "okada_traction.npz"), y=y, L=L, W=W, strike=st, dip=dp,
+                        receivers=np.array(rcv), traction=np.array(trac), stress=np.array(stress))
+
+
+def triangles(rng, n):
+    P1 = np.stack([rng.uniform(-5, 5, n), rng.uniform(-5, 5, n), -rng.uniform(0.5, 10, n)], 1)
+    P2 = P1 + rng.uniform(-3, 3, (n, 3))
+    P3 = P1 + rng.uniform(-3, 3, (n, 3))
+    P2[:, 2] = np.minimum(P2[:, 2], -0.1)
+    P3[:, 2] = np.minimum(P3[:, 2], -0.1)
+    # a triangle on the surface and a horizontal buried triangle
+    P1[0], P2[0], P3[0] = [0, 0, 0], [2, 0, 0], [0, 2, 0]
+    P1[1], P2[1], P3[1] = [0, 0, -3], [2, 0, -3], [0, 2, -3]
+    return P1, P2, P3
+
+
+def nikkhoo_strain(rng, n=300):
+    P1, P2, P3 = triangles(rng, n)
+    pts = np.stack([rng.uniform(-10, 10, n), rng.uniform(-10, 10, n), -rng.uniform(0, 12, n)], 1)
+    stress = np.zeros((n, 3, 6))
+    strain = np.zeros((n, 3, 6))
+    for i in range(n):
+        for s in range(3):
+            b = [0.0, 0.0, 0.0]
+            b[s] = 1.0
+            sg, e = np.zeros(6), np.zeros(6)
+            REF.straininhalfspace_(P(sg), P(e), D(pts[i, 0]), D(pts[i, 1]), D(pts[i, 2]),
+                                   P(np.ascontiguousarray(P1[i])), P(np.ascontiguousarray(P2[i])),
+                                   P(np.ascontiguousarray(P3[i])), *[D(v) for v in b],
+                                   D(30e3), D(30e3))
+            stress[i, s], strain[i, s] = sg, e
+    np.savez_compressed(os.path.join(HERE, "nikkhoo_strain.npz"), P1=P1, P2=P2, P3=P3, points=pts,
+                        stress=stress, strain=strain)
+
+
+def nikkhoo_kernels(rng, ntri=200, nrcv=4):
+    # vertices in NED metres, sources below the receivers' depth range
+    nve = 3 * ntri
+    v = np.zeros((nve, 3))
+    c = np.stack([rng.uniform(-20e3, 20e3, ntri), rng.uniform(-20e3, 20e3, ntri),
+                  rng.uniform(1e3, 20e3, ntri)], 1)
+    for k in range(3):
+        v[k::3] = c + rng.uniform(-2e3, 2e3, (ntri, 3))
+    v[:, 2] = np.abs(v[:, 2]) + 100.0
+    i1 = np.arange(1, nve, 3, dtype=np.int32)
+    i2 = i1 + 1
+    i3 = i1 + 2
+    rcv, trac, stress = [], [], []
+    for _ in range(nrcv):
+        x, sv, dv, nv = receiver(rng)
+        rcv.append(np.concatenate([x, sv, dv, nv]))
+        for s, d in [(1.0, 0.0), (0.0, 1.0)]:
+            t = [np.zeros(ntri) for _ in range(3)]
+            REF.ref_traction_nikkhoo15(P(x), P(sv), P(dv), P(nv), ntri, PI(i1), PI(i2), PI(i3), nve,
+                                       P(np.ascontiguousarray(v)), D(s), D(d), D(0.0), D(30e3),
+                                       D(30e3), *[P(a) for a in t])
+            trac.append(np.stack(t))
+            q = [np.zeros(ntri) for _ in range(6)]
+            REF.ref_stress_nikkhoo15(P(x), P(sv), P(dv), P(nv), ntri, PI(i1), PI(i2), PI(i3), nve,
+                                     P(np.ascontiguousarray(v)), D(s), D(d), D(0.0), D(30e3),
+                                     D(30e3), *[P(a) for a in q])
+            stress.append(np.stack(q))
+    np.savez_compressed(os.path.join(HERE, "nikkhoo_traction.npz"), vertices=v, i1=i1, i2=i2,
+                        i3=i3, receivers=np.array(rcv), traction=np.array(trac),
+                        stress=np.array(stress))
+
+
+def strainvolume_kernels(rng, ns=60, nrcv=4):
+    y = np.zeros((ns, 3))
+    y[:, 0] = rng.uniform(-20e3, 20e3, ns)
+    y[:, 1] = rng.uniform(-20e3, 20e3, ns)
+    y[:, 2] = rng.uniform(1e3, 20e3, ns)
+    L, T, W = rng.uniform(1e3, 5e3, ns), rng.uniform(1e3, 5e3, ns), rng.uniform(1e3, 5e3, ns)
+    st = rng.uniform(0, 2 * np.pi, ns)
+    rcv, trac, stress = [], [], []
+    for _ in range(nrcv):
+        x, sv, dv, nv = receiver(rng)
+        rcv.append(np.concatenate([x, sv, dv, nv]))
+        for j in range(6):
+            e = [0.0] * 6
+            e[j] = 1.0
+            t = [np.zeros(ns) for _ in range(3)]
+            REF.ref_traction_strainvolume(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)),
+                                          P(L), P(T), P(W), P(st), *[D(v) for v in e], D(30e3),
+                                          D(30e3), *[P(a) for a in t])
+            trac.append(np.stack(t))
+            q = [np.zeros(ns) for _ in range(6)]
+            REF.ref_stress_strainvolume(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)),
+                                        P(L), P(T), P(W), P(st), *[D(v) for v in e], D(30e3),
+                                        D(30e3), *[P(a) for a in q])
+            stress.append(np.stack(q))
+    np.savez_compressed(os.path.join(HERE, "strainvolume.npz"), y=y, L=L, T=T, W=W, strike=st,
+                        receivers=np.array(rcv), traction=np.array(trac), stress=np.array(stress))
+
+
+if __name__ == "__main__":
+    rng = np.random.default_rng(20260101)
+    dc3d_cases(rng)
+    okada_kernels(rng)
+    nikkhoo_strain(rng)
+    nikkhoo_kernels(rng)
+    strainvolume_kernels(rng)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))

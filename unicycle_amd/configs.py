@@ -95,6 +95,52 @@ def thrust_fault(nx: int = 64, nz: int = 32, patch: float = 2000.0, dip_deg: flo
         meta={"inside": inside})
 
 
+@dataclass
+class TriangleModel:
+    """Triangular mesh in the layout of the reference's .tri/.ned decks
+    (ratestate.f90:1252-1290): 1-based vertex indices, vertices north/east/down."""
+    lambda_: float
+    mu: float
+    i1: np.ndarray
+    i2: np.ndarray
+    i3: np.ndarray
+    vertices: np.ndarray   # (nVe,3)
+    fric: tuple            # (Vpl, rake, mu0, sig, a, b, L, Vo, damping), per triangle
+
+    @property
+    def n(self) -> int:
+        return self.i1.size
+
+    def geometry(self):
+        return (self.i1, self.i2, self.i3, self.vertices)
+
+    def friction(self):
+        return self.fric
+
+
+def triangulate(model: FaultModel) -> TriangleModel:
+    """Split every rectangle of ``model`` along its diagonal into the
+    triangles (A,B,C) and (A,C,D), A the upper corner at the start of strike,
+    B along strike, C,D one width down dip.  Vertices shared between patches
+    are merged; friction is inherited by both halves."""
+    st, dp = model.strike, model.dip
+    sv = np.stack([np.cos(st), np.sin(st), 0.0 * st], 1)
+    dd = np.stack([-np.sin(st) * np.cos(dp), np.cos(st) * np.cos(dp), np.sin(dp)], 1)
+    A = model.x
+    B = A + model.length[:, None] * sv
+    C = B + model.width[:, None] * dd
+    Dc = A + model.width[:, None] * dd
+    corners = np.stack([A, B, C, Dc], 1).reshape(-1, 3)
+    key = np.round(corners, 6)
+    verts, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.reshape(-1, 4) + 1
+    i1 = np.stack([inv[:, 0], inv[:, 0]], 1).reshape(-1).astype(np.int32)
+    i2 = np.stack([inv[:, 1], inv[:, 2]], 1).reshape(-1).astype(np.int32)
+    i3 = np.stack([inv[:, 2], inv[:, 3]], 1).reshape(-1).astype(np.int32)
+    fric = tuple(np.repeat(np.asarray(f, dtype=np.float64), 2) for f in model.friction())
+    return TriangleModel(model.lambda_, model.mu, i1, i2, i3, verts.astype(np.float64), fric)
+
+
 def perturbed_initial_state(model: FaultModel, y0: np.ndarray, seed: int,
                             amplitude: float = 1e-3) -> np.ndarray:
     """Initial slip rate Vpl*(1 + amplitude*N(0,1)) with the state variable at

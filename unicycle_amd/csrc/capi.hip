@@ -220,6 +220,7 @@ extern "C" void ucy_destroy(ucy_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    while (!c->ensembles.empty()) ucy_ensemble_destroy(c->ensembles.back());
     free_work(c);
     if (c->K) hipFree(c->K);
     if (c->d_fr) hipFree(c->d_fr);

@@ -79,6 +79,7 @@ extern "C" ucy_ensemble *ucy_ensemble_create(ucy_ctx *ctx, int precision)
     ucy_ensemble *E = new ucy_ensemble();
     E->ctx = ctx;
     E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0 : 1;
+    ctx->ensembles.push_back(E);
     return E;
 }
 
@@ -98,6 +99,8 @@ extern "C" void ucy_ensemble_destroy(ucy_ensemble *E)
     (void)hipSetDevice(E->ctx->device);
     (void)hipStreamSynchronize(E->ctx->stream);
     release(E);
+    auto &reg = E->ctx->ensembles;
+    reg.erase(std::remove(reg.begin(), reg.end(), E), reg.end());
     delete E;
 }
 
@@ -175,7 +178,6 @@ static void set_split(EnsTable &T, int kbps)
 {
     for (int gi = 0; gi < T.ngroups; gi++) {
         EnsGroup &G = T.g[gi];
-        if (G.mode != 0) continue;
         int k = std::max(1, std::min(kbps, G.nkb));
         G.ksplit = (G.nkb + k - 1) / k;
         G.kb_per_split = (G.nkb + G.ksplit - 1) / G.ksplit;
@@ -184,7 +186,7 @@ static void set_split(EnsTable &T, int kbps)
     for (int gi = 0; gi < T.ngroups; gi++) {
         const EnsGroup &G = T.g[gi];
         T.blk_gemm[gi] = acc;
-        const int items = G.mode == 0 ? (G.mp / 64) * G.ksplit : (G.m + 7) / 8;
+        const int items = (G.mp / 64) * G.ksplit;
         acc += (items + 3) / 4;
     }
     T.blk_gemm[T.ngroups] = acc;
@@ -272,11 +274,16 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             if (ealloc(E, &G.Tpart, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
             bytes += 4.0 * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
         } else {
-            E->allocs.push_back(K64);
-            G.K64 = K64;
-            if (ealloc(E, &G.V64, (size_t)G.n * kEnsCols)) return UCY_ERR_HIP;
-            if (ealloc(E, &G.T64, (size_t)G.m * kEnsCols)) return UCY_ERR_HIP;
-            bytes += 8.0 * G.m * G.n + 8.0 * G.n * kEnsCols;
+            double *tiles = nullptr;
+            if (ealloc(E, &tiles, (size_t)G.mp * G.np)) return UCY_ERR_HIP;
+            ens_launch_tile_K64(c->stream, K64, G.m, G.n, G.mp, G.np, tiles);
+            EHIP(hipGetLastError());
+            EHIP(hipStreamSynchronize(c->stream));
+            EHIP(hipFree(K64));
+            G.K64 = tiles;
+            if (ealloc(E, &G.V64, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
+            if (ealloc(E, &G.T64, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
+            bytes += 8.0 * G.mp * G.np + 8.0 * G.np * kEnsCols;
         }
         // ---- state and control
         const size_t S7 = (size_t)7 * G.N * kEnsCols;
@@ -341,7 +348,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     int best = 0;
     double best_t = 1e30;
     const int cands[] = {32, 64, 128, 256, 512};
-    if (E->mode == 0) {
+    {
         hipEvent_t e0, e1;
         EHIP(hipEventCreate(&e0));
         EHIP(hipEventCreate(&e1));

@@ -35,11 +35,11 @@ struct EnsGroup {
     int mode;         // 0 split-bf16 MFMA, 1 binary64
     const uint16_t *Ktiles;  // split-bf16 tiles (mode 0)
     const double *rscale;    // row scales (mode 0)
-    const double *K64;       // binary64 K row-major (mode 1)
+    const double *K64;       // binary64 K tiles (mode 1)
     uint16_t *Vtiles;        // V split-bf16 tiles (mode 0)
-    double *V64;             // V binary64 [col][inst] (mode 1)
+    double *V64;             // V binary64 tiles (mode 1)
     float *Tpart;            // [ksplit][mp][64] partial products (mode 0)
-    double *T64;             // [m][64] (mode 1)
+    double *T64;             // [ksplit][mp][64] partial products (mode 1)
     // state (SoA, 7 components)
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr;
     EnsFriction fr;
@@ -64,6 +64,8 @@ struct EnsTable {
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,
                         double *rscale);
+void ens_launch_tile_K64(hipStream_t st, const double *K, int m, int n, int mp, int np,
+                         double *tiles);
 void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);

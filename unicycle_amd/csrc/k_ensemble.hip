@@ -68,6 +68,36 @@ __device__ __forceinline__ size_t vtile_off(int c, int col, int part)
     return ((((size_t)kb * 2 + cb) * 2 + part) * 64 + lane) * 8 + j;
 }
 
+// binary64 tiles for v_mfma_f64_16x16x4f64: one 512-byte tile = 16 rows (or
+// columns) x 4 k, lane l holds row/column l%16, k l/16
+//   K tile (rt, kt)  offset (rt*nkt + kt)*64 + lane
+//   V tile (kt, ct)  offset (kt*4 + ct)*64 + lane
+__device__ __forceinline__ size_t k64_off(int r, int c, int nkt)
+{
+    return ((size_t)(r >> 4) * nkt + (c >> 2)) * 64 + (r & 15) + 16 * (c & 3);
+}
+__device__ __forceinline__ size_t v64_off(int c, int col)
+{
+    return ((size_t)(c >> 2) * 4 + (col >> 4)) * 64 + (col & 15) + 16 * (c & 3);
+}
+
+__global__ void tile64_kernel(const double *__restrict__ K, int m, int n, int mp, int np,
+                              double *__restrict__ tiles)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)mp * np) return;
+    const int r = (int)(i / np), c = (int)(i % np);
+    tiles[k64_off(r, c, np >> 2)] = (r < m && c < n) ? K[(size_t)r * n + c] : 0.0;
+}
+
+void ens_launch_tile_K64(hipStream_t st, const double *K, int m, int n, int mp, int np,
+                         double *tiles)
+{
+    const long long tot = (long long)mp * np;
+    hipLaunchKernelGGL(tile64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, K, m,
+                       n, mp, np, tiles);
+}
+
 // ---------------------------------------------------------------------------
 // K -> split bf16 tiles with exact power-of-two row scales
 // ---------------------------------------------------------------------------
@@ -183,27 +213,53 @@ __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int 
             }
 }
 
-// binary64 batched matvec: one wavefront = 8 rows x 64 instance columns
+// binary64 batched matvec on v_mfma_f64_16x16x4f64: one wavefront = 64 rows
+// x 64 columns over one k-split (4 x 4 tiles of 16 x 16)
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int lane)
 {
-    const int r0 = item * 8;
-    if (r0 >= G.m) return;
-    double acc[8];
+    const int nrq = G.mp >> 6;
+    const int rq = item % nrq, sp = item / nrq;
+    const int nkt = G.np >> 2;
+    const int kt0 = sp * G.kb_per_split * 4;
+    int kt1 = kt0 + G.kb_per_split * 4;
+    if (kt1 > nkt) kt1 = nkt;
+    doublex4 acc[4][4];
 #pragma unroll
-    for (int r = 0; r < 8; r++) acc[r] = 0.0;
-    const int nr = G.m - r0 < 8 ? G.m - r0 : 8;
-    for (int c = 0; c < G.n; c++) {
-        const double v = G.V64[(size_t)c * kEnsCols + lane];
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int r = 0; r < 8; r++)
-            if (r < nr) acc[r] = fma(G.K64[(size_t)(r0 + r) * G.n + c], v, acc[r]);
+        for (int c = 0; c < 4; c++) acc[a][c] = (doublex4){0.0, 0.0, 0.0, 0.0};
+    const double *A = G.K64 + (size_t)(4 * rq) * nkt * 64 + lane;
+    const double *B = G.V64 + lane;
+    for (int kt = kt0; kt < kt1; kt++) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = __builtin_nontemporal_load(A + ((size_t)a * nkt + kt) * 64);
+#pragma unroll
+        for (int c = 0; c < 4; c++) bv[c] = B[((size_t)kt * 4 + c) * 64];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
     }
-    for (int r = 0; r < nr; r++) G.T64[(size_t)(r0 + r) * kEnsCols + lane] = acc[r];
+    // 16x16 f64 accumulator layout: value i of lane l is row 4*(l/16) + i, column l%16
+    double *T = G.T64 + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 64 * rq + 16 * a + 4 * (lane >> 4) + i;
+                const int col = 16 * c + (lane & 15);
+                T[(size_t)row * kEnsCols + col] = acc[a][c][i];
+            }
 }
 
 __device__ __forceinline__ int gemm_items(const EnsGroup &G)
 {
-    return G.mode == 0 ? (G.mp >> 6) * G.ksplit : (G.m + 7) / 8;
+    return (G.mp >> 6) * G.ksplit;
 }
 
 __global__ void __launch_bounds__(256) gemm_kernel(const EnsTable *__restrict__ tab)
@@ -236,7 +292,7 @@ __global__ void __launch_bounds__(256) gemm_group_kernel(const EnsTable *__restr
 void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g)
 {
     const EnsGroup &G = htab.g[g];
-    const int items = G.mode == 0 ? (G.mp >> 6) * G.ksplit : (G.m + 7) / 8;
+    const int items = (G.mp >> 6) * G.ksplit;
     hipLaunchKernelGGL(gemm_group_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
 }
 
@@ -269,8 +325,8 @@ __device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X
         G.Vtiles[vtile_off(2 * e + 1, inst, 0)] = hi;
         G.Vtiles[vtile_off(2 * e + 1, inst, 1)] = lo;
     } else {
-        G.V64[(size_t)(2 * e) * kEnsCols + inst] = v0;
-        G.V64[(size_t)(2 * e + 1) * kEnsCols + inst] = v1;
+        G.V64[v64_off(2 * e, inst)] = v0;
+        G.V64[v64_off(2 * e + 1, inst)] = v1;
     }
     return velocity;
 }
@@ -288,7 +344,9 @@ __device__ __forceinline__ void ens_traction(const EnsGroup &G, int e, int inst,
             }
             t[c] = acc * G.rscale[r];
         } else {
-            t[c] = G.T64[(size_t)r * kEnsCols + inst];
+            double acc = 0.0;
+            for (int k = 0; k < G.ksplit; k++) acc += G.T64[((size_t)k * G.mp + r) * kEnsCols + inst];
+            t[c] = acc;
         }
     }
 }
@@ -554,7 +612,7 @@ __global__ void ens_fill_v_kernel(const EnsTable *__restrict__ tab, int g, const
         G.Vtiles[vtile_off(c, inst, 0)] = hi;
         G.Vtiles[vtile_off(c, inst, 1)] = lo;
     } else {
-        G.V64[i] = v;
+        G.V64[v64_off(c, inst)] = v;
     }
 }
 
@@ -570,7 +628,9 @@ __global__ void ens_reduce_t_kernel(const EnsTable *__restrict__ tab, int g, dou
         for (int k = 0; k < G.ksplit; k++) acc += (double)G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
         T[i] = acc * G.rscale[r];
     } else {
-        T[i] = G.T64[(size_t)r * kEnsCols + inst];
+        double acc = 0.0;
+        for (int k = 0; k < G.ksplit; k++) acc += G.T64[((size_t)k * G.mp + r) * kEnsCols + inst];
+        T[i] = acc;
     }
 }
 

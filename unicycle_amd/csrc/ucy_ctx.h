@@ -53,6 +53,8 @@ struct ucy_ctx {
     hipGraphExec_t graph = nullptr;
     // scratch for the per-kernel hooks
     DevBuf hook;
+    // ensembles created on this context (destroyed with it)
+    std::vector<struct ucy_ensemble *> ensembles;
 };
 
 

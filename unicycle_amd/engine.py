@@ -74,10 +74,13 @@ class Engine:
             raise UcyError(-2, f"cannot create a context on device {device}")
         self.np = 0
         self.nvol = 0
+        self._ensembles = []
 
     # -- plumbing ---------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
+            for e in getattr(self, "_ensembles", []):
+                e.close()
             self._L.ucy_destroy(self._h)
             self._h = None
 

@@ -38,10 +38,14 @@ class Ensemble:
         if not self._h:
             raise UcyError(-1, "cannot create ensemble")
         self.groups = []
+        engine._ensembles.append(self)
 
     def close(self):
+        # the context destroys its ensembles with it; only a live context's
+        # ensemble handle is still valid
         if getattr(self, "_h", None):
-            self._L.ucy_ensemble_destroy(self._h)
+            if getattr(self.eng, "_h", None):
+                self._L.ucy_ensemble_destroy(self._h)
             self._h = None
 
     def __del__(self):
