@@ -42,6 +42,9 @@ extern "C" {
 
 #define UCY_PREC_F64 0         /* K stored in binary64 */
 #define UCY_PREC_SPLIT_BF16 1  /* K stored as per-row-scaled bf16 hi+lo pair */
+/* ensembles only: binary64 K for the step-start matvec t_0 = K v_0 and the
+ * split-bf16 pair for the stage increments t_i = t_0 + K (v_i - v_0) */
+#define UCY_PREC_MIXED 2
 
 typedef struct ucy_ctx ucy_ctx;
 
@@ -219,7 +222,12 @@ typedef struct {
     int64_t n_inst, n_failed;
     double wall_seconds;
 } ucy_ensemble_stats;
-/* advance every instance until t >= t_end or max_attempts attempts */
+/* advance every instance from its current state until t >= t_end or its
+ * attempt count (cumulative since build/reset) reaches max_attempts; a later
+ * call with a larger t_end or max_attempts continues the same run, so a
+ * caller can advance in chunks.  Failed instances (step-size underflow,
+ * non-finite error norm) stay stopped.  stats: cumulative counters, wall
+ * time of this call. */
 int ucy_ensemble_run(ucy_ensemble *ens, double t_end, int64_t max_attempts, double epsilon,
                      double max_step, ucy_ensemble_stats *stats);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,

@@ -16,9 +16,12 @@ import pytest
 
 import orc
 from unicycle_amd import configs
-from unicycle_amd.ensemble import Ensemble, PREC_F64, PREC_SPLIT_BF16
+from unicycle_amd.ensemble import Ensemble, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16
 
 pytestmark = pytest.mark.gpu
+# bound on RK45 attempts per instance: the windows below need a few thousand;
+# a wrong stage matvec shows up as a failed assertion, not a long run
+MAX_ATTEMPTS = 200000
 
 
 def _instances(model, n_inst, seed):
@@ -37,8 +40,10 @@ def _instances(model, n_inst, seed):
     return draws, np.array(y0)
 
 
-@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_F64])
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_F64, PREC_MIXED])
 def test_ensemble_matvec(engine_factory, precision):
+    """The standalone matvec uses the stage precision: split bf16 for
+    PREC_SPLIT_BF16 and PREC_MIXED, binary64 for PREC_F64."""
     eng = engine_factory()
     groups = [configs.thrust_fault(nx=8, nz=8, ellipse=None),
               configs.thrust_fault(nx=16, nz=8, ellipse=None)]
@@ -58,7 +63,7 @@ def test_ensemble_matvec(engine_factory, precision):
         want = K @ V
         scale = np.abs(K) @ np.abs(V)
         err = np.abs(T - want)
-        if precision == PREC_SPLIT_BF16:
+        if precision != PREC_F64:
             assert np.all(err <= 2.0 ** -13 * scale), (err / scale).max()
             assert (err / scale).max() > 1e-9   # really reduced precision
         else:
@@ -83,8 +88,9 @@ def test_ensemble_f64_matches_plain_path(engine_factory):
     ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
     ens.build()
     T = 3.0 * configs.YEAR
-    stats = ens.run(T)
+    stats = ens.run(T, max_attempts=MAX_ATTEMPTS)
     assert stats.n_failed == 0 and stats.n_inst == n_inst
+    assert stats.min_time >= T, "attempt cap reached before the end of the window"
     y, t, dt, steps, att = ens.state(0)
     for i in range(n_inst):
         fr = list(model.friction())
@@ -96,31 +102,62 @@ def test_ensemble_f64_matches_plain_path(engine_factory):
         np.testing.assert_allclose(y[i], res.y, rtol=1e-6, atol=1e-9 * np.abs(res.y).max())
 
 
-def test_ensemble_split_bf16_tracks_f64(engine_factory):
-    """Split-bf16 K (fp32 accumulation) against the binary64 ensemble over a
-    window ending before any instance's first seismic event (vMax < 1e-3 m/s
-    throughout, checked on the binary64 run).
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_MIXED])
+def test_ensemble_reduced_precision_tracks_f64(engine_factory, precision):
+    """Split-bf16 stage increments (with a split-bf16 or a binary64 step-start
+    matvec) against the binary64 ensemble over a window ending before any
+    instance's first seismic event (vMax < 1e-3 m/s at the end, checked on
+    the binary64 run).
 
-    The bound is the measured behaviour of the 16-bit stage matvec, not the
-    1e-6 target: V depends exponentially on the shear stress (ratestate.f90:
-    688, 762), so relative stress-rate errors of ~1e-5 reach the slip at the
-    1e-4 level within a loading window."""
+    The bounds are the measured behaviour, not a binary64-level agreement:
+    V depends exponentially on the shear stress (ratestate.f90:688, 762) and
+    nucleating instances amplify stress-rate errors by orders of magnitude
+    within the window.  A broken stage (wrong tile, scale, split or
+    increment) moves the slip by O(1)."""
     model = configs.thrust_fault(nx=16, nz=8, ellipse=None)
     n_inst = 8
     draws, y0 = _instances(model, n_inst, seed=5)
     out = {}
-    for prec in (PREC_F64, PREC_SPLIT_BF16):
+    for prec in (PREC_F64, precision):
         eng = engine_factory()
         ens = Ensemble(eng, prec)
         ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
         ens.build()
-        stats = ens.run(0.5 * configs.YEAR)
-        assert stats.n_failed == 0
+        stats = ens.run(0.5 * configs.YEAR, max_attempts=MAX_ATTEMPTS)
+        assert stats.n_failed == 0 and stats.min_time >= 0.5 * configs.YEAR
         out[prec] = ens.state(0)
-    y64, y16 = out[PREC_F64][0], out[PREC_SPLIT_BF16][0]
+    y64, yr = out[PREC_F64][0], out[precision][0]
     assert np.max(y64[:, 6::7]) < -3.0, "window contains an event"
     slip64 = np.hypot(y64[:, 0::7], y64[:, 1::7])
-    slip16 = np.hypot(y16[:, 0::7], y16[:, 1::7])
-    rel = np.linalg.norm(slip16 - slip64) / np.linalg.norm(slip64)
-    print(f"split-bf16 slip error vs binary64 over 0.5 yr: {rel:.3e}")
-    assert rel <= 1e-3
+    slipr = np.hypot(yr[:, 0::7], yr[:, 1::7])
+    rel = np.linalg.norm(slipr - slip64) / np.linalg.norm(slip64)
+    att64, attr = out[PREC_F64][4], out[precision][4]
+    print(f"precision {precision}: slip error vs binary64 over 0.5 yr {rel:.3e}, "
+          f"attempts {attr.sum()} vs {att64.sum()}")
+    assert rel <= 1e-2
+    # the step controller is not driven by the matvec rounding
+    assert attr.sum() <= 1.5 * att64.sum() + 50 * n_inst
+
+
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_MIXED])
+def test_reduced_precision_start_does_not_stall(engine_factory, precision):
+    """From the perturbed steady state the step size grows from dt0 = 1e-3 s
+    as in binary64: absolute split-bf16 stage matvecs put an h-independent
+    rounding floor into the error estimate of components that start at zero
+    (slip, normal traction) and stall the controller at small steps."""
+    model = configs.thrust_fault(nx=32, nz=16, ellipse=None)
+    n_inst = 64
+    draws, y0 = _instances(model, n_inst, seed=11)
+    att = {}
+    for prec in (PREC_F64, precision):
+        eng = engine_factory()
+        ens = Ensemble(eng, prec)
+        ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
+        ens.build()
+        T = 0.01 * configs.YEAR
+        stats = ens.run(T, max_attempts=5000)
+        assert stats.n_failed == 0
+        assert stats.min_time >= T, f"precision {prec}: stalled at t = {stats.min_time:.3g} s"
+        att[prec] = stats.attempts
+    print(f"precision {precision}: attempts {att[precision]} vs binary64 {att[PREC_F64]}")
+    assert att[precision] <= 1.5 * att[PREC_F64] + 20 * n_inst

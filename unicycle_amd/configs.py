@@ -170,3 +170,44 @@ def ensemble_instances(base: FaultModel, n_inst: int, seed: int):
     sig = rng.uniform(50.0, 150.0, size=(n_inst, 1)) * np.ones((1, N))
     seeds = rng.integers(0, 2**31 - 1, size=n_inst)
     return {"a": a, "b": b, "L": L, "sig": sig, "seeds": seeds}
+
+
+def ensemble_setup(engine, precision: int, groups=ENSEMBLE_GROUPS, n_inst: int = 64,
+                   seed: int = 100):
+    """The uneven-size ensemble on ``engine``: one group per (nx, nz) of
+    ``groups`` (2 km patches, the thrust_fault layout rules), n_inst
+    instances each with the draws of ensemble_instances and the reference
+    initial state (ratestate.f90:548-610) of its own friction, perturbed by
+    perturbed_initial_state.  Returns (ensemble, [(model, draws, y0), ...])
+    with the ensemble built."""
+    from .engine import Engine
+    from .ensemble import Ensemble
+    ens = Ensemble(engine, precision)
+    info = []
+    for gi, (nx, nz) in enumerate(groups):
+        m = thrust_fault(nx=nx, nz=nz, ellipse=None)
+        d = ensemble_instances(m, n_inst, seed=seed + gi)
+        ic = Engine(engine.device)
+        try:
+            ic.set_elastic(m.lambda_, m.mu)
+            ic.set_rectangles(*m.geometry())
+            y0 = []
+            for i in range(n_inst):
+                fr = list(m.friction())
+                fr[3], fr[4], fr[5], fr[6] = d["sig"][i], d["a"][i], d["b"][i], d["L"][i]
+                ic.set_friction(*fr)
+                y0.append(perturbed_initial_state(m, ic.initStateVector(), int(d["seeds"][i])))
+        finally:
+            ic.close()
+        y0 = np.array(y0)
+        ens.add_group(m, d["a"], d["b"], d["sig"], d["L"], y0)
+        info.append((m, d, y0))
+    ens.build()
+    return ens, info
+
+
+def instance_friction(model: FaultModel, draws, i: int):
+    """Friction tuple of instance i of an ensemble group."""
+    fr = list(model.friction())
+    fr[3], fr[4], fr[5], fr[6] = draws["sig"][i], draws["a"][i], draws["b"][i], draws["L"][i]
+    return tuple(fr)

@@ -75,10 +75,12 @@ static int ealloc(ucy_ensemble *E, T **p, size_t count)
 extern "C" ucy_ensemble *ucy_ensemble_create(ucy_ctx *ctx, int precision)
 {
     if (!ctx) return nullptr;
-    if (precision != UCY_PREC_SPLIT_BF16 && precision != UCY_PREC_F64) return nullptr;
+    if (precision != UCY_PREC_SPLIT_BF16 && precision != UCY_PREC_F64 &&
+        precision != UCY_PREC_MIXED)
+        return nullptr;
     ucy_ensemble *E = new ucy_ensemble();
     E->ctx = ctx;
-    E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0 : 1;
+    E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0 : precision == UCY_PREC_F64 ? 1 : 2;
     ctx->ensembles.push_back(E);
     return E;
 }
@@ -259,32 +261,35 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             EHIP(hipFree(dR));
             EHIP(hipFree(dS));
         }
-        if (G.mode == 0) {
+        // stage-GEMM bytes per launch: split bf16 for modes 0 and 2
+        if (G.mode != 1) {
             uint16_t *tiles = nullptr;
             double *rs = nullptr;
             if (ealloc(E, &tiles, (size_t)G.mp * G.np * 2)) return UCY_ERR_HIP;
             if (ealloc(E, &rs, G.mp)) return UCY_ERR_HIP;
             ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
             EHIP(hipGetLastError());
-            EHIP(hipStreamSynchronize(c->stream));
-            EHIP(hipFree(K64));
             G.Ktiles = tiles;
             G.rscale = rs;
             if (ealloc(E, &G.Vtiles, (size_t)G.np * kEnsCols * 2)) return UCY_ERR_HIP;
             if (ealloc(E, &G.Tpart, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
+            if (ealloc(E, &G.v0, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
+            if (ealloc(E, &G.t0, (size_t)G.mp * kEnsCols)) return UCY_ERR_HIP;
             bytes += 4.0 * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
-        } else {
+        }
+        if (G.mode != 0) {
             double *tiles = nullptr;
             if (ealloc(E, &tiles, (size_t)G.mp * G.np)) return UCY_ERR_HIP;
             ens_launch_tile_K64(c->stream, K64, G.m, G.n, G.mp, G.np, tiles);
             EHIP(hipGetLastError());
-            EHIP(hipStreamSynchronize(c->stream));
-            EHIP(hipFree(K64));
             G.K64 = tiles;
             if (ealloc(E, &G.V64, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.T64, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
-            bytes += 8.0 * G.mp * G.np + 8.0 * G.np * kEnsCols;
+            if (G.mode == 1) bytes += 8.0 * G.mp * G.np + 8.0 * G.np * kEnsCols;
         }
+        EHIP(hipStreamSynchronize(c->stream));
+        EHIP(hipFree(K64));
+        if (ealloc(E, &G.any_rhs, 1)) return UCY_ERR_HIP;
         // ---- state and control
         const size_t S7 = (size_t)7 * G.N * kEnsCols;
         double **vecs[] = {&G.y, &G.dydt, &G.yscal, &G.ytmp, &G.ak[2], &G.ak[3], &G.ak[4],
@@ -362,9 +367,9 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
                 continue;
             }
             if (upload_table(E)) return UCY_ERR_HIP;
-            ens_launch_gemm(c->stream, E->d_tab, T);
+            ens_launch_gemm(c->stream, E->d_tab, T, 2);
             EHIP(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 5; r++) ens_launch_gemm(c->stream, E->d_tab, T);
+            for (int r = 0; r < 5; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
             EHIP(hipEventRecord(e1, c->stream));
             EHIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -404,11 +409,11 @@ static int capture(ucy_ensemble *E)
     hipGraph_t g = nullptr;
     EHIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     ens_launch_step_start(c->stream, E->d_tab, T);
-    ens_launch_gemm(c->stream, E->d_tab, T);
+    ens_launch_gemm(c->stream, E->d_tab, T, 1);
     ens_launch_post(c->stream, E->d_tab, T, 1);
     for (int s = 2; s <= 6; s++) {
         ens_launch_stage(c->stream, E->d_tab, T, s);
-        ens_launch_gemm(c->stream, E->d_tab, T);
+        ens_launch_gemm(c->stream, E->d_tab, T, s);
         ens_launch_post(c->stream, E->d_tab, T, s);
     }
     ens_launch_finish(c->stream, E->d_tab, T);
@@ -436,12 +441,18 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     E->tab.max_attempts = max_attempts;
     if (upload_table(E)) return UCY_ERR_HIP;
     if (capture(E)) return UCY_ERR_HIP;
+    // continue from the current state: re-arm every instance short of t_end
+    EHIP(hipMemsetAsync(E->tab.all_done, 0, sizeof(int), c->stream));
+    ens_launch_rearm(c->stream, E->d_tab, E->tab);
+    EHIP(hipGetLastError());
     int *h_done = nullptr;
     EHIP(hipHostMalloc((void **)&h_done, sizeof(int), hipHostMallocDefault));
+    EHIP(hipMemcpyAsync(h_done, E->tab.all_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EHIP(hipStreamSynchronize(c->stream));
     auto t0 = std::chrono::steady_clock::now();
     int batch = 8;
     int64_t replays = 0;
-    for (;;) {
+    for (; *h_done < E->tab.total_inst;) {
         for (int i = 0; i < batch; i++) EHIP(hipGraphLaunch(E->graph, c->stream));
         replays += batch;
         EHIP(hipMemcpyAsync(h_done, E->tab.all_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -535,9 +546,9 @@ extern "C" int ucy_ensemble_time_gemm(ucy_ensemble *E, int iters, double *avg_se
     hipEvent_t e0, e1;
     EHIP(hipEventCreate(&e0));
     EHIP(hipEventCreate(&e1));
-    ens_launch_gemm(c->stream, E->d_tab, E->tab);
+    ens_launch_gemm(c->stream, E->d_tab, E->tab, 2);
     EHIP(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < iters; i++) ens_launch_gemm(c->stream, E->d_tab, E->tab);
+    for (int i = 0; i < iters; i++) ens_launch_gemm(c->stream, E->d_tab, E->tab, 2);
     EHIP(hipEventRecord(e1, c->stream));
     EHIP(hipEventSynchronize(e1));
     float ms = 0;

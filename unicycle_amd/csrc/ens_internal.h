@@ -32,14 +32,20 @@ struct EnsGroup {
     int nkb;          // np / 16
     int ksplit;       // split-K factor of the GEMM
     int kb_per_split; // k-blocks per split
-    int mode;         // 0 split-bf16 MFMA, 1 binary64
-    const uint16_t *Ktiles;  // split-bf16 tiles (mode 0)
-    const double *rscale;    // row scales (mode 0)
-    const double *K64;       // binary64 K tiles (mode 1)
-    uint16_t *Vtiles;        // V split-bf16 tiles (mode 0)
-    double *V64;             // V binary64 tiles (mode 1)
-    float *Tpart;            // [ksplit][mp][64] partial products (mode 0)
-    double *T64;             // [ksplit][mp][64] partial products (mode 1)
+    int mode;         // 0 split-bf16, 1 binary64, 2 mixed (see below)
+    const uint16_t *Ktiles;  // split-bf16 tiles (modes 0, 2)
+    const double *rscale;    // row scales (modes 0, 2)
+    const double *K64;       // binary64 K tiles (modes 1, 2)
+    uint16_t *Vtiles;        // V split-bf16 tiles (modes 0, 2)
+    double *V64;             // V binary64 tiles (modes 1, 2)
+    float *Tpart;            // [ksplit][mp][64] partial products (modes 0, 2)
+    double *T64;             // [ksplit][mp][64] partial products (modes 1, 2)
+    // modes 0 and 2 evaluate the stage matvecs as increments over the step
+    // start:  t_i = t_0 + K (v_i - v_0),  t_0 = K v_0 (split bf16 in mode 0,
+    // binary64 in mode 2), the increment always split bf16
+    double *v0;              // [2N][64] slip-rate deviations at the step start
+    double *t0;              // [mp][64] tractions at the step start
+    int *any_rhs;            // some instance of the group starts a step
     // state (SoA, 7 components)
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr;
     EnsFriction fr;
@@ -66,14 +72,17 @@ void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, i
                         double *rscale);
 void ens_launch_tile_K64(hipStream_t st, const double *K, int m, int n, int mp, int np,
                          double *tiles);
-void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+// stage 1 = the step-start matvec (skipped when no instance of a group starts
+// a step), 2..6 the stage matvecs, 0 a standalone matvec
+void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
 void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
 void ens_launch_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
 void ens_launch_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
-// standalone GEMM check: T(m x 64) = K V with the group's storage mode
+void ens_launch_rearm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+// standalone GEMM check: T(m x 64) = K V with the group's stage precision
 void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int grp);
 
 }  // namespace ucy

@@ -243,7 +243,8 @@ __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int l
             for (int c = 0; c < 4; c++)
                 acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
     }
-    // 16x16 f64 accumulator layout: value i of lane l is row 4*(l/16) + i, column l%16
+    // 16x16 f64 accumulator layout (unlike the f32 16x16 forms, rows are
+    // interleaved): value i of lane l is row 4*i + l/16, column l%16
     double *T = G.T64 + (size_t)sp * G.mp * kEnsCols;
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -251,7 +252,7 @@ __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int l
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int row = 64 * rq + 16 * a + 4 * (lane >> 4) + i;
+                const int row = 64 * rq + 16 * a + 4 * i + (lane >> 4);
                 const int col = 16 * c + (lane & 15);
                 T[(size_t)row * kEnsCols + col] = acc[a][c][i];
             }
@@ -262,21 +263,28 @@ __device__ __forceinline__ int gemm_items(const EnsGroup &G)
     return (G.mp >> 6) * G.ksplit;
 }
 
-__global__ void __launch_bounds__(256) gemm_kernel(const EnsTable *__restrict__ tab)
+// binary64 path: mode 1 always, mode 2 for the step-start matvec
+__device__ __forceinline__ bool gemm_f64_path(const EnsGroup &G, int stage)
+{
+    return G.mode == 1 || (G.mode == 2 && stage == 1);
+}
+
+__global__ void __launch_bounds__(256) gemm_kernel(const EnsTable *__restrict__ tab, int stage)
 {
     const int blk = blockIdx.x;
     const int g = group_of(tab->blk_gemm, tab->ngroups, blk);
     const EnsGroup &G = tab->g[g];
     const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
     if (item >= gemm_items(G)) return;
+    if (stage == 1 && !*G.any_rhs) return;   // every instance retries: t_0 stands
     const int lane = threadIdx.x & 63;
-    if (G.mode == 0) gemm_wave_bf16(G, item, lane);
-    else gemm_wave_f64(G, item, lane);
+    if (gemm_f64_path(G, stage)) gemm_wave_f64(G, item, lane);
+    else gemm_wave_bf16(G, item, lane);
 }
 
-void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab)
+void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
 {
-    hipLaunchKernelGGL(gemm_kernel, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0, st, dtab);
+    hipLaunchKernelGGL(gemm_kernel, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0, st, dtab, stage);
 }
 
 __global__ void __launch_bounds__(256) gemm_group_kernel(const EnsTable *__restrict__ tab, int g)
@@ -285,8 +293,8 @@ __global__ void __launch_bounds__(256) gemm_group_kernel(const EnsTable *__restr
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (item >= gemm_items(G)) return;
     const int lane = threadIdx.x & 63;
-    if (G.mode == 0) gemm_wave_bf16(G, item, lane);
-    else gemm_wave_f64(G, item, lane);
+    if (gemm_f64_path(G, 0)) gemm_wave_f64(G, item, lane);
+    else gemm_wave_bf16(G, item, lane);
 }
 
 void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g)
@@ -304,10 +312,20 @@ __device__ __forceinline__ size_t sidx(const EnsGroup &G, int comp, int e, int i
     return ((size_t)comp * G.N + e) * kEnsCols + inst;
 }
 
+__device__ __forceinline__ void put_split(const EnsGroup &G, int c, int inst, double v)
+{
+    uint16_t hi, lo;
+    split2(v, hi, lo);
+    G.Vtiles[vtile_off(c, inst, 0)] = hi;
+    G.Vtiles[vtile_off(c, inst, 1)] = lo;
+}
+
 // pre-pass (ratestate.f90:683-702) of element e of instance inst at state X;
-// writes the slip rates to out[0..1] and the bf16 (or binary64) V entries
+// writes the slip rates to out[0..1] and the GEMM operand entries: binary64
+// v (mode 1); at the step start (base) v_0, kept, as split bf16 (mode 0) or
+// binary64 (mode 2); at a stage the split-bf16 increment v - v_0
 __device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X, double *out,
-                                              int e, int inst)
+                                              int e, int inst, bool base)
 {
     const size_t f = (size_t)e * kEnsCols + inst;
     const double rake = atan2(X[sidx(G, 3, e, inst)], X[sidx(G, 2, e, inst)]);
@@ -316,37 +334,55 @@ __device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X
     out[sidx(G, 0, e, inst)] = vs;
     out[sidx(G, 1, e, inst)] = vd;
     const double v0 = vs - G.fr.vplc[f], v1 = vd - G.fr.vpls[f];
-    if (G.mode == 0) {
-        uint16_t hi, lo;
-        split2(v0, hi, lo);
-        G.Vtiles[vtile_off(2 * e, inst, 0)] = hi;
-        G.Vtiles[vtile_off(2 * e, inst, 1)] = lo;
-        split2(v1, hi, lo);
-        G.Vtiles[vtile_off(2 * e + 1, inst, 0)] = hi;
-        G.Vtiles[vtile_off(2 * e + 1, inst, 1)] = lo;
-    } else {
+    double *b0 = G.v0 + (size_t)(2 * e) * kEnsCols + inst, *b1 = b0 + kEnsCols;
+    if (G.mode == 1 || (G.mode == 2 && base)) {
         G.V64[v64_off(2 * e, inst)] = v0;
         G.V64[v64_off(2 * e + 1, inst)] = v1;
+        if (G.mode == 2) {
+            *b0 = v0;
+            *b1 = v1;
+        }
+    } else if (base) {
+        *b0 = v0;
+        *b1 = v1;
+        put_split(G, 2 * e, inst, v0);
+        put_split(G, 2 * e + 1, inst, v1);
+    } else {
+        put_split(G, 2 * e, inst, v0 - *b0);
+        put_split(G, 2 * e + 1, inst, v1 - *b1);
     }
     return velocity;
 }
 
-__device__ __forceinline__ void ens_traction(const EnsGroup &G, int e, int inst, double t[3])
+__device__ __forceinline__ double sum_bf16(const EnsGroup &G, int r, int inst)
+{
+    double acc = 0.0;
+    for (int k = 0; k < G.ksplit; k++) acc += (double)G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
+    return acc * G.rscale[r];
+}
+
+__device__ __forceinline__ double sum_f64(const EnsGroup &G, int r, int inst)
+{
+    double acc = 0.0;
+    for (int k = 0; k < G.ksplit; k++) acc += G.T64[((size_t)k * G.mp + r) * kEnsCols + inst];
+    return acc;
+}
+
+// traction rates of element e at the current stage (stage 1: step start,
+// whose value is kept as t_0 in modes 0 and 2)
+__device__ __forceinline__ void ens_traction(const EnsGroup &G, int e, int inst, int stage,
+                                             double t[3])
 {
     for (int c = 0; c < 3; c++) {
         const int r = 3 * e + c;
-        if (G.mode == 0) {
-            float s = 0.0f;
-            double acc = 0.0;
-            for (int k = 0; k < G.ksplit; k++) {
-                s = G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
-                acc += (double)s;
-            }
-            t[c] = acc * G.rscale[r];
+        double *t0 = G.t0 + (size_t)r * kEnsCols + inst;
+        if (G.mode == 1) {
+            t[c] = sum_f64(G, r, inst);
+        } else if (stage == 1) {
+            t[c] = G.mode == 2 ? sum_f64(G, r, inst) : sum_bf16(G, r, inst);
+            *t0 = t[c];
         } else {
-            double acc = 0.0;
-            for (int k = 0; k < G.ksplit; k++) acc += G.T64[((size_t)k * G.mp + r) * kEnsCols + inst];
-            t[c] = acc;
+            t[c] = *t0 + sum_bf16(G, r, inst);
         }
     }
 }
@@ -368,7 +404,8 @@ __global__ void __launch_bounds__(256) ens_step_start_kernel(const EnsTable *__r
     if (!elem_coords(tab, g, e, inst)) return;
     const EnsGroup &G = tab->g[g];
     if (G.done[inst] || !G.need_rhs[inst]) return;
-    const double vel = ens_prepass(G, G.y, G.dydt, e, inst);
+    *G.any_rhs = 1;
+    const double vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
     atomicMax(reinterpret_cast<unsigned long long *>(&G.vmax[inst]),
               (unsigned long long)__double_as_longlong(vel));
 }
@@ -386,7 +423,7 @@ __global__ void __launch_bounds__(256) ens_post_kernel(const EnsTable *__restric
     double *out = G.ak[stage];
     const size_t f = (size_t)e * kEnsCols + inst;
     double t[3];
-    ens_traction(G, e, inst, t);
+    ens_traction(G, e, inst, stage, t);
     const double rake = atan2(X[sidx(G, 3, e, inst)], X[sidx(G, 2, e, inst)]);
     const double velocity = exp(X[sidx(G, 6, e, inst)] * kELg10);
     const double cr = cos(rake), sr = sin(rake);
@@ -458,7 +495,7 @@ __global__ void __launch_bounds__(256) ens_stage_kernel(const EnsTable *__restri
         const double k5 = stage > 5 ? G.ak[5][i] : 0.0;
         G.ytmp[i] = ens_stage_value(stage, h, G.y[i], k1, k2, k3, k4, k5);
     }
-    ens_prepass(G, G.ytmp, G.ak[stage], e, inst);
+    ens_prepass(G, G.ytmp, G.ak[stage], e, inst, false);
 }
 
 // fifth-order solution, embedded error, per-instance max |yerr/yscal|
@@ -467,6 +504,8 @@ __global__ void __launch_bounds__(256) ens_finish_kernel(const EnsTable *__restr
     int g, e, inst;
     const bool live = elem_coords(tab, g, e, inst);
     const EnsGroup &G = tab->g[g];
+    // all GEMMs of this attempt have run: clear the step-start flag
+    if (blockIdx.x == tab->blk_elem[g] && threadIdx.x == 0) *G.any_rhs = 0;
     double emax = 0.0;
     if (live && !G.done[inst]) {
         const double h = G.h[inst];
@@ -557,6 +596,28 @@ __global__ void ens_control_kernel(EnsTable *tab)
     }
 }
 
+// start of a run call: an instance that has not failed, is short of the
+// call's end time and still has attempts left steps again; the finished
+// count is rebuilt from the flags (all_done is zeroed by the caller)
+__global__ void ens_rearm_kernel(EnsTable *tab)
+{
+    const int blk = blockIdx.x;
+    const int g = group_of(tab->blk_inst, tab->ngroups, blk);
+    const EnsGroup &G = tab->g[g];
+    const int inst = (blk - tab->blk_inst[g]) * 64 + threadIdx.x;
+    if (inst >= G.ninst) return;
+    const int done = !(G.status[inst] == 0 && G.time[inst] < tab->t_end &&
+                       G.attempts[inst] < tab->max_attempts);
+    G.done[inst] = done;
+    if (done) atomicAdd(tab->all_done, 1);
+}
+
+void ens_launch_rearm(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+{
+    hipLaunchKernelGGL(ens_rearm_kernel, dim3(h.blk_inst[h.ngroups]), dim3(64), 0, st,
+                       const_cast<EnsTable *>(dtab));
+}
+
 __global__ void __launch_bounds__(256) ens_commit_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
@@ -606,14 +667,8 @@ __global__ void ens_fill_v_kernel(const EnsTable *__restrict__ tab, int g, const
     if (i >= G.n * kEnsCols) return;
     const int c = i / kEnsCols, inst = i % kEnsCols;
     const double v = inst < G.ninst ? V[(size_t)c * G.ninst + inst] : 0.0;
-    if (G.mode == 0) {
-        uint16_t hi, lo;
-        split2(v, hi, lo);
-        G.Vtiles[vtile_off(c, inst, 0)] = hi;
-        G.Vtiles[vtile_off(c, inst, 1)] = lo;
-    } else {
-        G.V64[v64_off(c, inst)] = v;
-    }
+    if (G.mode == 1) G.V64[v64_off(c, inst)] = v;
+    else put_split(G, c, inst, v);
 }
 
 // T (m x ninst) from the partial products
@@ -623,15 +678,7 @@ __global__ void ens_reduce_t_kernel(const EnsTable *__restrict__ tab, int g, dou
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G.m * G.ninst) return;
     const int r = i / G.ninst, inst = i % G.ninst;
-    if (G.mode == 0) {
-        double acc = 0.0;
-        for (int k = 0; k < G.ksplit; k++) acc += (double)G.Tpart[((size_t)k * G.mp + r) * kEnsCols + inst];
-        T[i] = acc * G.rscale[r];
-    } else {
-        double acc = 0.0;
-        for (int k = 0; k < G.ksplit; k++) acc += G.T64[((size_t)k * G.mp + r) * kEnsCols + inst];
-        T[i] = acc;
-    }
+    T[i] = G.mode == 1 ? sum_f64(G, r, inst) : sum_bf16(G, r, inst);
 }
 
 void ens_launch_matvec_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g,

@@ -141,8 +141,11 @@ static TdSide host_side(const double PA[3], const double PB[3])
     return T;
 }
 
-// source in NED vertices -> TriSource in the ENU frame of straininhalfspace_
-static TriSource host_tri(const double *P1, const double *P2, const double *P3)
+// source in NED vertices -> TriSource in the ENU frame of straininhalfspace_;
+// the two directions are unit strike and unit dip slip, or, with gen set,
+// the single slip (s, d, t) of nikkhoo15.f90:120-153 in direction 0
+static TriSource host_tri(const double *P1, const double *P2, const double *P3,
+                          bool gen = false, double s = 0.0, double d = 0.0, double t = 0.0)
 {
     const double Q1[3] = {P1[1], P1[0], -P1[2]}, Q2[3] = {P2[1], P2[0], -P2[2]},
                  Q3[3] = {P3[1], P3[0], -P3[2]};
@@ -156,10 +159,15 @@ static TriSource host_tri(const double *P1, const double *P2, const double *P3)
     S.side[2] = host_side(Q3, Q1);
     double nv[3], sv[3], dv[3];
     host_td_frame(Q1, Q2, Q3, false, nv, sv, dv);
-    // TDCS slips (tensile, strike, dip): strike (0,1,0), dip (0,0,-1)
+    // TDCS slips (tensile, strike, dip): strike (0,1,0), dip (0,0,-1); the
+    // dip slip enters negated (nikkhoo15.f90:142)
+    const double b0[3] = {gen ? t : 0.0, gen ? s : 1.0, gen ? -d : 0.0};
+    const double b1[3] = {0.0, 0.0, gen ? 0.0 : -1.0};
     for (int i = 0; i < 3; i++) {
-        S.bfs[0][i] = nv[i] * 0.0 + sv[i] * 1.0 + dv[i] * 0.0;
-        S.bfs[1][i] = nv[i] * 0.0 + sv[i] * 0.0 + dv[i] * -1.0;
+        S.btd[0][i] = b0[i];
+        S.btd[1][i] = b1[i];
+        S.bfs[0][i] = nv[i] * b0[0] + sv[i] * b0[1] + dv[i] * b0[2];
+        S.bfs[1][i] = nv[i] * b1[0] + sv[i] * b1[1] + dv[i] * b1[2];
     }
     S.surface = (M1[2] == 0.0 && M2[2] == 0.0 && M3[2] == 0.0) ? 1 : 0;
     return S;
@@ -195,7 +203,7 @@ __device__ __forceinline__ void tri_proj_volume(const Receiver &R, const double 
     o[5] = -c1 * dv[0] - c2 * dv[1] - c3 * dv[2];
 }
 
-template <int KIND>
+template <int KIND, bool GEN>
 __global__ void __launch_bounds__(128)
 tri_rows_kernel(TriMedium M, const TriSource *__restrict__ src, int ns,
                 const Receiver *__restrict__ rcv, double *__restrict__ K, int64_t ldk,
@@ -206,7 +214,7 @@ tri_rows_kernel(TriMedium M, const TriSource *__restrict__ src, int ns,
     if (s >= ns) return;
     const Receiver R = rcv[r];
     double Ss[6], Sd[6];
-    tri_stress2(M, src[s], R.xc[0], R.xc[1], R.xc[2], Ss, Sd);
+    tri_stress2<GEN>(M, src[s], R.xc[0], R.xc[1], R.xc[2], Ss, Sd);
     constexpr int NC = KIND == 0 ? 3 : 6;
     double os[NC], od[NC];
     if constexpr (KIND == 0) {
@@ -231,18 +239,27 @@ tri_rows_kernel(TriMedium M, const TriSource *__restrict__ src, int ns,
     }
 }
 
+// kind 0 traction rows, 1 volume-frame stress rows; gen: general slip
+// vectors (per-pair hooks) instead of unit strike / dip
 static void launch_tri_rows(hipStream_t st, const TriMedium &M, const TriSource *src, int ns,
                             const Receiver *rcv, int nr, int kind, double *K, int64_t ldk,
-                            int64_t row0, int64_t col0, double *s_out, double *d_out, int stride)
+                            int64_t row0, int64_t col0, double *s_out, double *d_out, int stride,
+                            bool gen = false)
 {
     if (ns <= 0 || nr <= 0) return;
     dim3 grid((ns + 127) / 128, nr);
-    if (kind == 0)
-        hipLaunchKernelGGL(tri_rows_kernel<0>, grid, dim3(128), 0, st, M, src, ns, rcv, K, ldk, row0,
-                           col0, s_out, d_out, stride);
+    if (kind == 0 && !gen)
+        hipLaunchKernelGGL((tri_rows_kernel<0, false>), grid, dim3(128), 0, st, M, src, ns, rcv, K,
+                           ldk, row0, col0, s_out, d_out, stride);
+    else if (kind == 0)
+        hipLaunchKernelGGL((tri_rows_kernel<0, true>), grid, dim3(128), 0, st, M, src, ns, rcv, K,
+                           ldk, row0, col0, s_out, d_out, stride);
+    else if (!gen)
+        hipLaunchKernelGGL((tri_rows_kernel<1, false>), grid, dim3(128), 0, st, M, src, ns, rcv, K,
+                           ldk, row0, col0, s_out, d_out, stride);
     else
-        hipLaunchKernelGGL(tri_rows_kernel<1>, grid, dim3(128), 0, st, M, src, ns, rcv, K, ldk, row0,
-                           col0, s_out, d_out, stride);
+        hipLaunchKernelGGL((tri_rows_kernel<1, true>), grid, dim3(128), 0, st, M, src, ns, rcv, K,
+                           ldk, row0, col0, s_out, d_out, stride);
 }
 
 static TriMedium tri_medium(double G, double lambda)
@@ -298,10 +315,6 @@ static int tri_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
 {
     if (!c || ns < 0) return UCY_ERR_ARG;
     if (ns == 0) return UCY_OK;
-    if (t != 0.0) {
-        c->err = "tensile opening is not part of the slip kernels";
-        return UCY_ERR_ARG;
-    }
     for (int j = 0; j < ns; j++)
         if (i1[j] < 1 || i1[j] > nVe || i2[j] < 1 || i2[j] > nVe || i3[j] < 1 || i3[j] > nVe) {
             c->err = "vertex index out of range";
@@ -311,7 +324,8 @@ static int tri_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
     const int NC = kind == 0 ? 3 : 6;
     std::vector<TriSource> S(ns);
     for (int j = 0; j < ns; j++)
-        S[j] = host_tri(vert + 3 * (i1[j] - 1), vert + 3 * (i2[j] - 1), vert + 3 * (i3[j] - 1));
+        S[j] = host_tri(vert + 3 * (i1[j] - 1), vert + 3 * (i2[j] - 1), vert + 3 * (i3[j] - 1),
+                        true, s, d, t);
     Receiver R;
     for (int k = 0; k < 3; k++) {
         R.xc[k] = x[k];
@@ -330,7 +344,7 @@ static int tri_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
     std::vector<double> h(2 * NC * (size_t)ns);
     if (e == hipSuccess) {
         launch_tri_rows(c->stream, tri_medium(G, lambda), dS, ns, dR, 1, kind, nullptr, 0, 0, 0, dout,
-                        dout + NC * (size_t)ns, ns);
+                        dout + NC * (size_t)ns, ns, true);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream);
@@ -343,14 +357,7 @@ static int tri_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
         return UCY_ERR_HIP;
     }
     for (int k = 0; k < NC; k++)
-        for (int i = 0; i < ns; i++) {
-            const double vs = h[k * (size_t)ns + i], vd = h[(NC + k) * (size_t)ns + i];
-            double r;
-            if (s == 1.0 && d == 0.0) r = vs;
-            else if (s == 0.0 && d == 1.0) r = vd;
-            else r = s * vs + d * vd;
-            out[k][i] = r;
-        }
+        for (int i = 0; i < ns; i++) out[k][i] = h[k * (size_t)ns + i];
     return UCY_OK;
 }
 

@@ -88,7 +88,9 @@ struct TdSide {
 struct TriSource {
     TdFull full[2];       // main, image
     TdSide side[3];
-    double bfs[2][3];     // EFCS slip of unit strike / dip (for the harmonic part)
+    double bfs[2][3];     // EFCS slip of the two computed directions (harmonic part)
+    double btd[2][3];     // TDCS Burgers vectors (tensile, strike, dip) of the two
+                          // directions; used by the general-slip form only
     int surface;          // all vertices at z = 0 (image yz/zz sign quirk)
     int pad;
 };
@@ -317,16 +319,20 @@ __device__ __forceinline__ void hooke6(const double e[6], double mu, double lamb
     s[4] = 2.0 * mu * e[4];
 }
 
-// NED stress (s11,s12,s13,s22,s23,s33) at NED point (x1,x2,x3) for unit strike
-// (Ss) and unit dip (Ds) slip of the source: straininhalfspace_ in ENU with
-// the conversions of nikkhoo15.f90:130-161
+// NED stress (s11,s12,s13,s22,s23,s33) at NED point (x1,x2,x3) for two slip
+// vectors of the source: straininhalfspace_ in ENU with the conversions of
+// nikkhoo15.f90:130-161.  GEN = false: unit strike (Ss) and unit dip (Sd),
+// with the Burgers vectors as compile-time constants; GEN = true: the two
+// TDCS Burgers vectors stored in the source (any strike / dip / opening).
+template <bool GEN = false>
 __device__ __forceinline__ void tri_stress2(const TriMedium &M, const TriSource &S, double x1,
                                             double x2, double x3, double Ss[6], double Sd[6])
 {
     const G3 X = gmk(x2, 1, 0, 0), Y = gmk(x1, 0, 1, 0), Z = gmk(-x3, 0, 0, 1);
     // TDCS Burgers vectors (tensile, strike, dip): strike unit -> (0,1,0);
     // dip unit d=1 enters as Ds=-d (nikkhoo15.f90:142) -> (0,0,-1)
-    const double bs[3] = {0.0, 1.0, 0.0}, bd[3] = {0.0, 0.0, -1.0};
+    const double cs[3] = {0.0, 1.0, 0.0}, cd[3] = {0.0, 0.0, -1.0};
+    const double *bs = GEN ? S.btd[0] : cs, *bd = GEN ? S.btd[1] : cd;
     double stress[2][6];
     for (int k = 0; k < 2; k++)
         for (int i = 0; i < 6; i++) stress[k][i] = 0.0;

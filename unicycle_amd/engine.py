@@ -20,6 +20,7 @@ from . import _lib
 
 PREC_F64 = 0
 PREC_SPLIT_BF16 = 1
+PREC_MIXED = 2        # ensembles: binary64 step-start matvec, split-bf16 increments
 
 
 class UcyError(RuntimeError):
@@ -69,6 +70,7 @@ class Engine:
 
     def __init__(self, device: int = 0):
         self._L = _lib.lib()
+        self.device = int(device)
         self._h = self._L.ucy_create(device, 0)
         if not self._h:
             raise UcyError(-2, f"cannot create a context on device {device}")

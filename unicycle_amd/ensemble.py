@@ -1,20 +1,26 @@
 """Batched ensembles of independent fault instances (no counterpart in the
 reference; each instance runs the reference's per-instance algorithm).
 
-Instances are grouped by geometry; a group shares one kernel K stored in the
-ensemble precision (split bf16 hi+lo with exact row scales on MFMA, or
-binary64).  State, time, step control and error norms stay binary64 per
-instance.
+Instances are grouped by geometry; a group shares one kernel K.  Precisions:
+  PREC_F64         binary64 K on v_mfma_f64_16x16x4f64 for every matvec;
+  PREC_SPLIT_BF16  split bf16 hi+lo K with exact row scales on
+                   v_mfma_f32_32x32x16_bf16; the stage matvecs are increments
+                   t_i = t_0 + K (v_i - v_0) over the step start, so the
+                   embedded error estimate (whose weights sum to zero) does
+                   not see the constant part of the rounding;
+  PREC_MIXED       as PREC_SPLIT_BF16 with t_0 = K v_0 in binary64.
+State, time, step control and error norms stay binary64 per instance.
 """
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
 
 from . import _lib
-from .engine import Engine, UcyError, PREC_F64, PREC_SPLIT_BF16, _p  # noqa: F401
+from .engine import Engine, UcyError, PREC_F64, PREC_SPLIT_BF16, PREC_MIXED, _p  # noqa: F401
 
 
 @dataclass
@@ -81,7 +87,27 @@ class Ensemble:
         self._chk(self._L.ucy_ensemble_reset(self._h, float(dt0)))
 
     def run(self, t_end: float, max_attempts: int = 10**9, epsilon: float = 1e-6,
-            max_step: float = 1.7976931348623158e308) -> EnsembleStats:
+            max_step: float = 1.7976931348623158e308, chunk_attempts: int | None = None,
+            progress=None) -> EnsembleStats:
+        """Advance to t_end (ratestate.f90:307-328 per instance).  With
+        chunk_attempts the run is issued in slices of that many attempts per
+        instance, calling progress(stats) after each slice; wall_seconds then
+        sums the slices."""
+        if not chunk_attempts:
+            return self._run(t_end, max_attempts, epsilon, max_step)
+        wall, cap = 0.0, 0
+        while True:
+            cap = min(max_attempts, cap + int(chunk_attempts))
+            st = self._run(t_end, cap, epsilon, max_step)
+            wall += st.wall_seconds
+            st = dataclasses.replace(st, wall_seconds=wall)
+            if progress is not None:
+                progress(st)
+            # replays == 0: every instance is at t_end or has failed
+            if st.min_time >= t_end or cap >= max_attempts or st.replays == 0:
+                return st
+
+    def _run(self, t_end, max_attempts, epsilon, max_step) -> EnsembleStats:
         st = _lib.EnsembleStats()
         self._chk(self._L.ucy_ensemble_run(self._h, float(t_end), int(max_attempts),
                                            float(epsilon), float(max_step), C.byref(st)))
