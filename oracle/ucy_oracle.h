@@ -71,6 +71,9 @@ void orc_refsys_strainvolume(int ns, const double *x, const double *L, const dou
 int orc_stress_strainvolume(double x1, double x2, double x3, double q1, double q2, double q3,
                             double L, double T, double W, double theta,
                             const double epsp[6], double G, double nu, double sig[6]);
+int orc_displacement_strainvolume(double x1, double x2, double x3, double q1, double q2,
+                                  double q3, double L, double T, double W, double theta,
+                                  const double epsp[6], double G, double nu, double u[3]);
 int orc_traction_kernels_strainvolume(const double x[3], const double sv[3], const double dv[3],
                                       const double nv[3], int ns, const double *y,
                                       const double *L, const double *T, const double *W,

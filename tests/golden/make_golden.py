@@ -13,7 +13,9 @@ The fixtures are inputs and the reference's outputs only:
   nikkhoo_traction.npz  computeTractionKernelsNikkhoo15 / computeStressKernels-
                         Nikkhoo15 (nikkhoo15.f90:187-304)
   strainvolume.npz      computeTractionKernels / computeStressKernels-
-                        VerticalStrainVolume (strainvolume.f90:4671-4799)
+                        VerticalStrainVolume (strainvolume.f90:4671-4799), incl.
+                        a receiver inside a volume and one at the surface, and
+                        computeDisplacementVerticalStrainVolume (215-268)
 
 Usage: python tests/golden/make_golden.py
 """
@@ -172,8 +174,14 @@ def strainvolume_kernels(rng, ns=60, nrcv=4):
     L, T, W = rng.uniform(1e3, 5e3, ns), rng.uniform(1e3, 5e3, ns), rng.uniform(1e3, 5e3, ns)
     st = rng.uniform(0, 2 * np.pi, ns)
     rcv, trac, stress = [], [], []
-    for _ in range(nrcv):
-        x, sv, dv, nv = receiver(rng)
+    rlist = [receiver(rng) for _ in range(nrcv)]
+    # special receivers: inside volume 0 (eigenstrain removal) and at the
+    # free surface above volume 1
+    x, sv, dv, nv = receiver(rng)
+    c0 = y[0] + np.array([0.3 * L[0] * np.cos(st[0]), 0.3 * L[0] * np.sin(st[0]), 0.4 * W[0]])
+    rlist.append((c0, sv, dv, nv))
+    rlist.append((np.array([y[1, 0], y[1, 1], 0.0]), sv, dv, nv))
+    for x, sv, dv, nv in rlist:
         rcv.append(np.concatenate([x, sv, dv, nv]))
         for j in range(6):
             e = [0.0] * 6
@@ -188,8 +196,27 @@ def strainvolume_kernels(rng, ns=60, nrcv=4):
                                         P(L), P(T), P(W), P(st), *[D(v) for v in e], D(30e3),
                                         D(30e3), *[P(a) for a in q])
             stress.append(np.stack(q))
+    # displacement routine (strainvolume.f90:215) at random points, unit eigenstrains
+    pts, disp, src = [], [], []
+    for _ in range(120):
+        q = np.array([rng.uniform(-5e3, 5e3), rng.uniform(-5e3, 5e3), rng.uniform(1e3, 1e4)])
+        g = np.array([rng.uniform(1e3, 5e3), rng.uniform(1e3, 5e3), rng.uniform(1e3, 5e3),
+                      rng.uniform(0, 2 * np.pi)])
+        xp = np.array([rng.uniform(-2e4, 2e4), rng.uniform(-2e4, 2e4), rng.uniform(0, 2e4)])
+        for j in range(6):
+            e = [0.0] * 6
+            e[j] = 1.0
+            u = [D() for _ in range(3)]
+            REF.ref_displacement_strainvolume(*[D(v) for v in xp], *[D(v) for v in q],
+                                              *[D(v) for v in g], *[D(v) for v in e], D(30e3),
+                                              D(0.25), *[C.byref(v) for v in u])
+            disp.append([v.value for v in u])
+        pts.append(xp)
+        src.append(np.concatenate([q, g]))
     np.savez_compressed(os.path.join(HERE, "strainvolume.npz"), y=y, L=L, T=T, W=W, strike=st,
-                        receivers=np.array(rcv), traction=np.array(trac), stress=np.array(stress))
+                        receivers=np.array(rcv), traction=np.array(trac), stress=np.array(stress),
+                        disp_points=np.array(pts), disp_sources=np.array(src),
+                        disp=np.array(disp).reshape(len(pts), 6, 3))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,140 @@
+"""GPU parity of the strain-volume path (Barbot, Moore & Lambert 2017) and of
+the fault + viscoelastic-volume RHS (viscouscycles.f90) against the CPU
+oracle and the reference's outputs (golden fixtures), through the C ABI.
+
+The device kernel and the oracle both differentiate the displacement
+J-integrals exactly (shared formula text, independent dual arithmetic and
+libm); the reference differentiates them by hand.  The 8-corner sums cancel,
+so agreement is to the rounding of those sums:
+- device vs reference fixture: per source column <= 1e-8 relative to the
+  column's largest component, median <= 1e-11;
+- device vs oracle: normwise <= 1e-11;
+- NaN exactly where the reference gives NaN (receiver above a volume corner).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from unicycle_amd import configs
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _oracle(fn, rcv, g, eps, nout):
+    x, sv, dv, nv = rcv[:3], rcv[3:6], rcv[6:9], rcv[9:12]
+    ns = g["L"].size
+    out = [np.zeros(ns) for _ in range(nout)]
+    getattr(orc.oracle(), fn)(
+        orc.p(orc.f64(x)), orc.p(orc.f64(sv)), orc.p(orc.f64(dv)), orc.p(orc.f64(nv)), ns,
+        orc.p(orc.f64(g["y"])), orc.p(g["L"]), orc.p(g["T"]), orc.p(g["W"]), orc.p(g["strike"]),
+        orc.p(orc.f64(eps)), orc.D(30e3), orc.D(30e3), *[orc.p(o) for o in out])
+    return np.stack(out)
+
+
+def test_strainvolume_hooks_vs_reference_and_oracle(engine_factory):
+    eng = engine_factory()
+    g = dict(np.load(os.path.join(GOLD, "strainvolume.npz")))
+    ns = g["L"].size
+    k, rels = 0, []
+    for rcv in g["receivers"]:
+        x, sv, dv, nv = rcv[:3], rcv[3:6], rcv[6:9], rcv[9:12]
+        for j in range(6):
+            e = np.zeros(6)
+            e[j] = 1.0
+            t = np.stack(eng.computeTractionKernelsVerticalStrainVolume(
+                x, sv, dv, nv, ns, g["y"], g["L"], g["T"], g["W"], g["strike"], *e, 30e3, 30e3))
+            s = np.stack(eng.computeStressKernelsVerticalStrainVolume(
+                x, sv, dv, nv, ns, g["y"], g["L"], g["T"], g["W"], g["strike"], *e, 30e3, 30e3))
+            for got, want, fn, nc in ((t, g["traction"][k], "orc_traction_kernels_strainvolume", 3),
+                                      (s, g["stress"][k], "orc_stress_kernels_strainvolume", 6)):
+                assert np.array_equal(np.isnan(got), np.isnan(want))
+                ok = ~np.isnan(want).any(axis=0)
+                r = np.abs(got - want)[:, ok].max(axis=0) / np.abs(want)[:, ok].max(axis=0)
+                rels.append(r)
+                o = _oracle(fn, rcv, g, e, nc)[:, ok]
+                assert np.linalg.norm(got[:, ok] - o) <= 1e-11 * np.linalg.norm(o)
+            k += 1
+    rels = np.concatenate(rels)
+    assert rels.max() <= 1e-8 and np.median(rels) <= 1e-11
+
+
+def test_strainvolume_general_eigenstrain(engine_factory):
+    """A general eigenstrain is the combination of the unit columns
+    (linearity), as one call of the reference's routine."""
+    eng = engine_factory()
+    g = dict(np.load(os.path.join(GOLD, "strainvolume.npz")))
+    ns = g["L"].size
+    rcv = g["receivers"][0]
+    x, sv, dv, nv = rcv[:3], rcv[3:6], rcv[6:9], rcv[9:12]
+    e = np.array([1.0, -0.3, 0.2, 0.5, 0.1, -0.7])
+    t = np.stack(eng.computeTractionKernelsVerticalStrainVolume(
+        x, sv, dv, nv, ns, g["y"], g["L"], g["T"], g["W"], g["strike"], *e, 30e3, 30e3))
+    o = _oracle("orc_traction_kernels_strainvolume", rcv, g, e, 3)
+    assert np.linalg.norm(t - o) <= 1e-11 * np.linalg.norm(o)
+
+
+def _viscous_model(nx=6, nz=3, vn=(3, 3, 2)):
+    fault = configs.thrust_fault(nx=nx, nz=nz, ellipse=None)
+    vol = configs.volume_grid(*vn, size=4000.0, top=20e3, origin=(0.0, 0.0))
+    return fault, vol
+
+
+def _engine(engine_factory, fault, vol):
+    eng = engine_factory()
+    eng.set_elastic(fault.lambda_, fault.mu)
+    eng.set_rectangles(*fault.geometry())
+    eng.set_volumes(*vol.geometry())
+    eng.set_friction(*fault.friction())
+    eng.set_rheology(*vol.rheology())
+    return eng
+
+
+def test_buildG_fault_and_volumes_matches_oracle(engine_factory):
+    """Blocks fault->fault, fault->volume (Okada stress rows in the volume
+    frame), volume->fault and volume->volume in the reference's row and
+    column order (greens.f90:138-158, 214-216, 266-271)."""
+    fault, vol = _viscous_model()
+    eng = _engine(engine_factory, fault, vol)
+    eng.buildG()
+    K = eng.get_kernel()
+    Kref = orc.buildG(orc.Mesh(fault.lambda_, fault.mu, rect=fault.geometry(),
+                               vol=vol.geometry()), nthreads=8)
+    n, nv = fault.n, vol.n
+    assert K.shape == Kref.shape == (3 * n + 6 * nv, 2 * n + 6 * nv)
+    rowscale = np.abs(Kref).max(axis=1, keepdims=True)
+    # Okada columns carry the binary32 DC3D rounding (flip-tolerant bound)
+    assert np.all(np.abs(K[:, :2 * n] - Kref[:, :2 * n]) <= 1e-6 * rowscale)
+    vc = slice(2 * n, None)
+    assert np.linalg.norm(K[:, vc] - Kref[:, vc]) <= 1e-11 * np.linalg.norm(Kref[:, vc])
+    assert np.all(np.abs(K[:, vc] - Kref[:, vc]) <= 1e-9 * rowscale)
+
+
+def test_viscous_rhs_and_steps_match_oracle(engine_factory):
+    """Fault + Maxwell volumes: initial state (viscouscycles.f90:710-803), the
+    volume branch of odefun (933-1080) and 100 device-resident adaptive steps
+    against the oracle."""
+    fault, vol = _viscous_model()
+    eng = _engine(engine_factory, fault, vol)
+    K = orc.buildG(orc.Mesh(fault.lambda_, fault.mu, rect=fault.geometry(), vol=vol.geometry()),
+                   nthreads=8)
+    eng.set_kernel(K)
+    sysm = orc.System(fault.friction(), K, rheo=vol.rheology(), ic_variant=1)
+    y0 = sysm.init_state()
+    np.testing.assert_array_equal(eng.initStateVector(1), y0)
+    rng = np.random.default_rng(4)
+    y = y0.copy()
+    y[6:7 * fault.n:7] += 0.3 * rng.standard_normal(fault.n)
+    y[7 * fault.n:] *= 1.0 + 0.1 * rng.standard_normal(12 * vol.n)
+    dydt = np.empty_like(y)
+    vmax = eng.odefun(y.size, 0.0, y, dydt)
+    want, vmax_ref, _, _ = sysm.odefun(y)
+    np.testing.assert_allclose(dydt, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    assert vmax == pytest.approx(vmax_ref, rel=1e-14)
+    k = 100
+    _, st, tl, dl, rc = sysm.integrate(y0, 50 * configs.YEAR, max_steps=k, log_cap=k)
+    res = eng.integrate(y0, 50 * configs.YEAR, max_steps=k, log_cap=k)
+    assert res.steps == st.steps == k and res.attempts == st.attempts
+    np.testing.assert_allclose(res.dt_log, dl, rtol=1e-5)

@@ -415,3 +415,149 @@ def test_rkqs_underflow_is_reported():
     f = lambda y: np.array([np.nan])
     rc, *_ = orc.rkqs(1, 0.0, np.array([1.0]), np.array([1.0]), np.array([1.0]), 1.0, f)
     assert rc == -1 or rc == 0   # NaN errmax is ignored by MAXVAL semantics
+
+
+# ---------------------------------------------------------- strain volumes --
+def _sv_kernels(fn, rcv, g, eps, nout):
+    x, sv, dv, nv = rcv[:3], rcv[3:6], rcv[6:9], rcv[9:12]
+    ns = g["L"].size
+    out = [np.zeros(ns) for _ in range(nout)]
+    rc = getattr(ORC, fn)(orc.p(orc.f64(x)), orc.p(orc.f64(sv)), orc.p(orc.f64(dv)),
+                          orc.p(orc.f64(nv)), ns, orc.p(orc.f64(g["y"])), orc.p(g["L"]),
+                          orc.p(g["T"]), orc.p(g["W"]), orc.p(g["strike"]),
+                          orc.p(orc.f64(eps)), D(30e3), D(30e3), *[orc.p(o) for o in out])
+    assert rc == 0
+    return np.stack(out)
+
+
+def _sv_stress(x, q, L, T, W, th, eps, nu=0.25):
+    s = np.zeros(6)
+    rc = ORC.orc_stress_strainvolume(*[D(v) for v in list(x) + list(q) + [L, T, W, th]],
+                                     orc.p(orc.f64(eps)), D(30e3), D(nu), orc.p(s))
+    assert rc == 0
+    return s
+
+
+def test_strainvolume_displacement_vs_reference_fixture():
+    """The J-integral values themselves (strainvolume.f90:215-991): the
+    8-corner sums cancel, so agreement is to rounding of the sum."""
+    g = gold("strainvolume.npz")
+    rels = []
+    for xp, src, dref in zip(g["disp_points"], g["disp_sources"], g["disp"]):
+        for j in range(6):
+            e = np.zeros(6)
+            e[j] = 1.0
+            u = np.zeros(3)
+            assert ORC.orc_displacement_strainvolume(
+                *[D(v) for v in list(xp) + list(src)], orc.p(e), D(30e3), D(0.25), orc.p(u)) == 0
+            rels.append(np.abs(u - dref[j]).max() / np.abs(dref[j]).max())
+    rels = np.array(rels)
+    assert np.median(rels) < 1e-11 and np.quantile(rels, 0.99) < 1e-9 and rels.max() < 1e-8
+
+
+def test_strainvolume_kernels_vs_reference_fixture():
+    """Exact gradient of the displacement J-integrals against the reference's
+    hand-differentiated stress (strainvolume.f90:1042-4640), incl. a receiver
+    inside a volume (eigenstrain removed) and one at the free surface above a
+    volume corner, where both give NaN for that volume (0/0 in the J terms)."""
+    g = dict(gold("strainvolume.npz"))
+    k, rt, rs = 0, [], []
+    for rcv in g["receivers"]:
+        for j in range(6):
+            e = np.zeros(6)
+            e[j] = 1.0
+            t = _sv_kernels("orc_traction_kernels_strainvolume", rcv, g, e, 3)
+            s = _sv_kernels("orc_stress_kernels_strainvolume", rcv, g, e, 6)
+            wt, ws = g["traction"][k], g["stress"][k]
+            assert np.array_equal(np.isnan(t), np.isnan(wt))
+            assert np.array_equal(np.isnan(s), np.isnan(ws))
+            ok = ~np.isnan(wt).any(axis=0)
+            rt.append(np.abs(t - wt)[:, ok].max(axis=0) / np.abs(wt)[:, ok].max(axis=0))
+            rs.append(np.abs(s - ws)[:, ok].max(axis=0) / np.abs(ws)[:, ok].max(axis=0))
+            k += 1
+    for r in (np.concatenate(rt), np.concatenate(rs)):
+        assert np.median(r) < 1e-11 and np.quantile(r, 0.99) < 1e-9 and r.max() < 1e-8
+
+
+def test_strainvolume_live_reference_checker():
+    ref = orc.ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(12)
+    for _ in range(100):
+        x = [rng.uniform(-1e4, 1e4), rng.uniform(-1e4, 1e4), rng.uniform(0, 1e4)]
+        q = [rng.uniform(-5e3, 5e3), rng.uniform(-5e3, 5e3), rng.uniform(0, 8e3)]
+        g = [rng.uniform(1e3, 4e3), rng.uniform(1e3, 4e3), rng.uniform(1e3, 4e3),
+             rng.uniform(0, 6.28)]
+        e = rng.standard_normal(6)
+        a, b = np.zeros(3), [D() for _ in range(3)]
+        ORC.orc_displacement_strainvolume(*[D(v) for v in x + q + g], orc.p(e), D(30e3), D(0.3),
+                                          orc.p(a))
+        ref.ref_displacement_strainvolume(*[D(v) for v in x + q + g], *[D(v) for v in e], D(30e3),
+                                          D(0.3), *[orc.C.byref(v) for v in b])
+        b = np.array([v.value for v in b])
+        assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+
+
+def test_strainvolume_traction_free_surface():
+    """sigma_i3 = 0 at x3 = 0 for a buried volume (half-space solution)."""
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        q = [0.0, 0.0, rng.uniform(1e3, 5e3)]
+        x = [rng.uniform(-1e4, 1e4), rng.uniform(-1e4, 1e4), 0.0]
+        e = rng.standard_normal(6)
+        s = _sv_stress(x, q, 3e3, 2e3, 2.5e3, rng.uniform(0, 6.28), e)
+        assert max(abs(s[2]), abs(s[4]), abs(s[5])) <= 1e-9 * np.abs(s).max()
+
+
+def test_strainvolume_superposition_and_decay():
+    """Two half-length volumes sum to the whole (linearity of the
+    eigenstrain source), and the far field decays as r^-3."""
+    rng = np.random.default_rng(6)
+    th = 0.7
+    q = np.array([100.0, -200.0, 3e3])
+    half = q + 1500.0 * np.array([np.cos(th), np.sin(th), 0.0])
+    for _ in range(20):
+        x = [rng.uniform(-1e4, 1e4), rng.uniform(-1e4, 1e4), rng.uniform(0, 1e4)]
+        e = rng.standard_normal(6)
+        whole = _sv_stress(x, q, 3e3, 2e3, 2.5e3, th, e)
+        parts = _sv_stress(x, q, 1.5e3, 2e3, 2.5e3, th, e) + _sv_stress(x, half, 1.5e3, 2e3,
+                                                                        2.5e3, th, e)
+        assert np.allclose(parts, whole, rtol=1e-8, atol=1e-8 * np.abs(whole).max())
+    e = np.array([1.0, 0, 0, 0, 0, 0])
+    rs = [4e4, 8e4, 1.6e5]
+    vals = [np.abs(_sv_stress([r, 0.0, 5e3], [0.0, 0.0, 4e3], 500.0, 500.0, 500.0, 0.0, e)).max()
+            for r in rs]
+    slope = np.polyfit(np.log(rs), np.log(vals), 1)[0]
+    assert -3.3 < slope < -2.7
+
+
+def test_strainvolume_inside_removes_eigenstrain():
+    """Inside the volume the eigenstrain is removed before Hooke
+    (strainvolume.f90:1124-1131), which makes the traction on every face
+    continuous: across the bottom face (normal x3) s13, s23, s33, across a
+    length face (normal along strike) the strike-frame traction.  Without the
+    removal the normal traction would jump by Hooke of the eigenstrain."""
+    q, L, T, W, th = [0.0, 0.0, 2e3], 4e3, 3e3, 2e3, 0.4
+    c, s = np.cos(th), np.sin(th)
+    d = 1e-4
+    rng = np.random.default_rng(8)
+    for _ in range(6):
+        e = rng.standard_normal(6) * 1e-3
+        # bottom face, interior point of the face
+        px = 1.3e3 * c - 0.4e3 * s, 1.3e3 * s + 0.4e3 * c
+        a = _sv_stress([px[0], px[1], q[2] + W - d], q, L, T, W, th, e)
+        b = _sv_stress([px[0], px[1], q[2] + W + d], q, L, T, W, th, e)
+        scale = np.abs(a).max()
+        assert np.allclose(a[[2, 4, 5]], b[[2, 4, 5]], rtol=0, atol=1e-5 * scale)
+        assert np.abs(a[[0, 1, 3]] - b[[0, 1, 3]]).max() > 1e-3 * scale  # in-plane jumps
+        # far length face x1' = L: normal (c, s, 0)
+        n = np.array([c, s, 0.0])
+        pin = (L - d) * n + np.array([-0.5e3 * s, 0.5e3 * c, 0.0]) + [0, 0, 3e3]
+        pout = (L + d) * n + np.array([-0.5e3 * s, 0.5e3 * c, 0.0]) + [0, 0, 3e3]
+        def trac(S):
+            M = np.array([[S[0], S[1], S[2]], [S[1], S[3], S[4]], [S[2], S[4], S[5]]])
+            return M @ n
+        a = _sv_stress(pin, q, L, T, W, th, e)
+        b = _sv_stress(pout, q, L, T, W, th, e)
+        assert np.allclose(trac(a), trac(b), rtol=0, atol=1e-5 * np.abs(a).max())

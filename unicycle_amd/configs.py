@@ -211,3 +211,58 @@ def instance_friction(model: FaultModel, draws, i: int):
     fr = list(model.friction())
     fr[3], fr[4], fr[5], fr[6] = draws["sig"][i], draws["a"][i], draws["b"][i], draws["L"][i]
     return tuple(fr)
+
+
+@dataclass
+class VolumeModel:
+    """Vertical strain volumes (strainvolume.f90 conventions: x = upper corner
+    at the start of strike, north/east/down; length along strike, thickness
+    across, width down) and their rheology (viscouscycles.f90:933-1000:
+    power law with prefactor gammadot0 and exponent n; Maxwell is n = 1,
+    gammadot0 = 1/(2 eta)) with the initial-stress law of
+    viscouscycles.f90:769-803 (Gm, Q, R, To; Q = 0 makes it consistent with
+    the RHS)."""
+    x: np.ndarray
+    length: np.ndarray
+    thickness: np.ndarray
+    width: np.ndarray
+    strike: np.ndarray
+    gammadot0: np.ndarray
+    npower: np.ndarray
+    edot_bg: np.ndarray    # (nvol, 6)
+    Gm: np.ndarray
+    Q: np.ndarray
+    R: np.ndarray
+    To: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+    def geometry(self):
+        return (self.x, self.length, self.thickness, self.width, self.strike)
+
+    def rheology(self):
+        return (self.gammadot0, self.npower, self.edot_bg, self.Gm, self.Q, self.R, self.To)
+
+
+def volume_grid(nx: int = 32, ny: int = 32, nz: int = 8, size: float = 4000.0,
+                top: float = 20e3, origin=(0.0, -64e3), eta: float = 1e19,
+                npower: float = 1.0, seed: int = 0, edot: float = 1e-15) -> VolumeModel:
+    """nx (north) x ny (east) x nz (down) cubes of ``size`` m from depth
+    ``top``; strike north.  eta in Pa s (stress unit MPa: gammadot0 =
+    1/(2 eta 1e-6)); npower > 1 draws gammadot0 from a seeded lognormal about
+    the Maxwell value.  Background strain rate: simple shear e12 = edot."""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    n = i.size
+    x = np.stack([origin[0] + i * size, origin[1] + j * size, top + k * size], 1).astype(float)
+    one = np.ones(n)
+    g0 = 1.0 / (2.0 * eta * 1e-6) * one
+    if npower != 1.0:
+        g0 = g0 * np.exp(np.random.default_rng(seed).normal(0.0, 0.5, n))
+    eb = np.zeros((n, 6))
+    eb[:, 1] = edot
+    return VolumeModel(x=x, length=size * one, thickness=size * one, width=size * one,
+                       strike=0.0 * one, gammadot0=g0, npower=npower * one, edot_bg=eb,
+                       Gm=1.0 * one, Q=0.0 * one, R=8.314 * one, To=873.0 * one)
