@@ -6,8 +6,9 @@
  * name ends in _dev (device pointers, stream-ordered on the context stream).
  * Every function returns 0 on success and a negative status on failure; the
  * message is available from ucy_last_error().  The library never aborts the
- * process (the reference STOPs, e.g. ode45.f90:86-90; a Fortran host maps a
- * nonzero status back to STOP).
+ * process (the reference STOPs, e.g. ode45.f90:86-90; the Fortran module
+ * ucy_fortran (unicycle_amd/fortran/ucy_fortran.f90) maps a nonzero status
+ * back to STOP).
  *
  * Layout conventions are the reference's:
  *   - arrays of 3-vectors are Fortran (3,ns) arrays, i.e. x[3*i+k];
@@ -230,6 +231,11 @@ typedef struct {
  * time of this call. */
 int ucy_ensemble_run(ucy_ensemble *ens, double t_end, int64_t max_attempts, double epsilon,
                      double max_step, ucy_ensemble_stats *stats);
+/* Measurement option (off by default; the reference's loop steps past the
+ * interval, ratestate.f90:321-326): clamp each instance's last step so that
+ * it lands exactly on t_end, so that runs in different precisions can be
+ * compared at the same time. */
+int ucy_ensemble_set_stop_at_end(ucy_ensemble *ens, int on);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
  * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,

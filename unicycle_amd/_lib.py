@@ -95,6 +95,7 @@ _SIGS = {
     "ucy_ensemble_reset": (_i, [_vp, _d]),
     "ucy_ensemble_run": (_i, [_vp, _d, _i64, _d, _d, C.POINTER(EnsembleStats)]),
     "ucy_ensemble_get_state": (_i, [_vp, _i, _pd, _pd, _pd, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ucy_ensemble_set_stop_at_end": (_i, [_vp, _i]),
     "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
     "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), _pd]),
     "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),

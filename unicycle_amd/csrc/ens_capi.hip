@@ -38,6 +38,7 @@ struct ucy_ensemble {
     hipGraphExec_t graph = nullptr;
     bool built = false;
     int kb_per_split = 0;
+    int stop_at_end = 0;
     std::vector<double> tune_seconds;
     double gemm_bytes = 0;
 };
@@ -439,6 +440,7 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     E->tab.epsilon = epsilon;
     E->tab.max_step = max_step;
     E->tab.max_attempts = max_attempts;
+    E->tab.stop_at_end = E->stop_at_end;
     if (upload_table(E)) return UCY_ERR_HIP;
     if (capture(E)) return UCY_ERR_HIP;
     // continue from the current state: re-arm every instance short of t_end
@@ -487,6 +489,13 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
         stats->replays = replays;
         stats->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
     }
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_set_stop_at_end(ucy_ensemble *E, int on)
+{
+    if (!E) return UCY_ERR_ARG;
+    E->stop_at_end = on ? 1 : 0;
     return UCY_OK;
 }
 

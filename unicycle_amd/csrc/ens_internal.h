@@ -63,6 +63,7 @@ struct EnsTable {
     int blk_inst[kMaxGroups + 1];   // prefix sums of per-instance blocks
     double t_end, epsilon, max_step;
     long long max_attempts;
+    int stop_at_end;                // clamp the last step to land on t_end
     int *all_done;                  // count of finished instances
     int total_inst;
 };

@@ -578,8 +578,13 @@ __global__ void ens_control_kernel(EnsTable *tab)
         double hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
         if (hnext > tab->max_step) hnext = tab->max_step;
         G.hdid[inst] = h;
+        // the step that was clamped to the end lands on t_end exactly
+        const double tnew = G.time[inst] + h;
+        const bool last = tab->stop_at_end && tnew >= tab->t_end * (1.0 - 1e-15);
+        G.time[inst] = last ? tab->t_end : tnew;
+        if (tab->stop_at_end && !last && G.time[inst] + hnext > tab->t_end)
+            hnext = tab->t_end - G.time[inst];
         G.hnext[inst] = hnext;
-        G.time[inst] = G.time[inst] + h;
         G.steps[inst] += 1;
         G.accepted[inst] = 1;
         G.need_rhs[inst] = 1;

@@ -114,6 +114,11 @@ class Ensemble:
         return EnsembleStats(st.inst_time_sum, st.min_time, st.max_time, st.steps, st.attempts,
                              st.replays, st.n_inst, st.n_failed, st.wall_seconds)
 
+    def set_stop_at_end(self, on: bool = True):
+        """Clamp each instance's last step to land exactly on t_end (a
+        measurement option; the reference's loop steps past the interval)."""
+        self._chk(self._L.ucy_ensemble_set_stop_at_end(self._h, int(bool(on))))
+
     def state(self, group: int):
         N, n_inst = self.groups[group]
         y = np.empty((n_inst, 7 * N))
