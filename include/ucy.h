@@ -190,7 +190,8 @@ int ucy_integrate(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t 
 /* Measurement hook (no counterpart in the reference): average duration in
  * seconds of the fused matvec + odefun post-pass kernel (the dominant kernel
  * of every RHS), timed with HIP events on the context stream over iters
- * launches at the current state. */
+ * launches at the state of the last ucy_odefun call (or the last stage of
+ * ucy_integrate). */
 int ucy_time_rhs_kernel(ucy_ctx *ctx, int iters, double *avg_seconds);
 
 /* ---- batched ensembles (no counterpart in the reference) --------------- */
@@ -236,6 +237,12 @@ int ucy_ensemble_run(ucy_ensemble *ens, double t_end, int64_t max_attempts, doub
  * it lands exactly on t_end, so that runs in different precisions can be
  * compared at the same time. */
 int ucy_ensemble_set_stop_at_end(ucy_ensemble *ens, int on);
+/* Event recording: per instance, the start time of the first accepted step
+ * whose step-start vMax exceeds vmax_threshold (default 1e-3 m/s; the vMax of
+ * ratestate.f90:530-532).  With stop_at_event an instance stops there. */
+int ucy_ensemble_set_events(ucy_ensemble *ens, double vmax_threshold, int stop_at_event);
+/* first-event times of a group's instances (-1: no event yet) */
+int ucy_ensemble_first_events(ucy_ensemble *ens, int group, double *t_event);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
  * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,

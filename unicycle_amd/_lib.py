@@ -96,6 +96,8 @@ _SIGS = {
     "ucy_ensemble_run": (_i, [_vp, _d, _i64, _d, _d, C.POINTER(EnsembleStats)]),
     "ucy_ensemble_get_state": (_i, [_vp, _i, _pd, _pd, _pd, C.POINTER(_i64), C.POINTER(_i64)]),
     "ucy_ensemble_set_stop_at_end": (_i, [_vp, _i]),
+    "ucy_ensemble_set_events": (_i, [_vp, C.c_double, _i]),
+    "ucy_ensemble_first_events": (_i, [_vp, _i, _pd]),
     "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
     "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), _pd]),
     "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),

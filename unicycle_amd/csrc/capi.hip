@@ -942,11 +942,12 @@ extern "C" int ucy_time_rhs_kernel(ucy_ctx *c, int iters, double *avg_seconds)
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.y, b.ak[2], b.v, nullptr);
-    launch_gemv_post(c->stream, d, c->K, c->d_fr, b.y, b.v, b.ak[2], nullptr);  // warm
+    // at the state of the last ucy_odefun call (or the last stage of ucy_integrate)
+    launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.ytmp, b.ak[2], b.v, nullptr);
+    launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], nullptr);  // warm
     HIPCHK(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; i++)
-        launch_gemv_post(c->stream, d, c->K, c->d_fr, b.y, b.v, b.ak[2], nullptr);
+        launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], nullptr);
     HIPCHK(hipEventRecord(e1, c->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

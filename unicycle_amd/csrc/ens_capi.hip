@@ -39,6 +39,8 @@ struct ucy_ensemble {
     bool built = false;
     int kb_per_split = 0;
     int stop_at_end = 0;
+    int stop_at_event = 0;
+    double event_vmax = 1e-3;
     std::vector<double> tune_seconds;
     double gemm_bytes = 0;
 };
@@ -163,6 +165,8 @@ static int load_initial(ucy_ensemble *E, double dt0)
         EHIP(hipMemcpyAsync(G.hnext, hv.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.time, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.vmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> none(kEnsCols, -1.0);
+        EHIP(hipMemcpyAsync(G.t_event, none.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.errmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.need_rhs, one.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.accepted, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
@@ -323,7 +327,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
                 EHIP(hipMemcpyAsync(ds[k], hs[k]->data(), sizeof(double) * F, hipMemcpyHostToDevice, c->stream));
             EHIP(hipStreamSynchronize(c->stream));
         }
-        double **cv[] = {&G.time, &G.h, &G.hnext, &G.hdid, &G.errmax, &G.vmax};
+        double **cv[] = {&G.time, &G.h, &G.hnext, &G.hdid, &G.errmax, &G.vmax, &G.t_event};
         for (double **v : cv)
             if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
         int **ci[] = {&G.need_rhs, &G.accepted, &G.done, &G.status};
@@ -441,6 +445,8 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     E->tab.max_step = max_step;
     E->tab.max_attempts = max_attempts;
     E->tab.stop_at_end = E->stop_at_end;
+    E->tab.stop_at_event = E->stop_at_event;
+    E->tab.event_vmax = E->event_vmax;
     if (upload_table(E)) return UCY_ERR_HIP;
     if (capture(E)) return UCY_ERR_HIP;
     // continue from the current state: re-arm every instance short of t_end
@@ -496,6 +502,26 @@ extern "C" int ucy_ensemble_set_stop_at_end(ucy_ensemble *E, int on)
 {
     if (!E) return UCY_ERR_ARG;
     E->stop_at_end = on ? 1 : 0;
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_set_events(ucy_ensemble *E, double vmax_threshold, int stop_at_event)
+{
+    if (!E || !(vmax_threshold > 0)) return UCY_ERR_ARG;
+    E->event_vmax = vmax_threshold;
+    E->stop_at_event = stop_at_event ? 1 : 0;
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_first_events(ucy_ensemble *E, int group, double *t_event)
+{
+    if (!E || group < 0 || group >= (int)E->hg.size() || !t_event) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    std::vector<double> tmp(kEnsCols);
+    EHIP(hipMemcpy(tmp.data(), E->tab.g[group].t_event, sizeof(double) * kEnsCols, hipMemcpyDeviceToHost));
+    std::copy(tmp.begin(), tmp.begin() + E->hg[group].ninst, t_event);
     return UCY_OK;
 }
 

@@ -51,6 +51,7 @@ struct EnsGroup {
     EnsFriction fr;
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;
+    double *t_event;         // time of the first step starting with vMax > event_vmax (-1: none)
     int *need_rhs, *accepted, *done, *status;
     long long *steps, *attempts;
 };
@@ -64,6 +65,8 @@ struct EnsTable {
     double t_end, epsilon, max_step;
     long long max_attempts;
     int stop_at_end;                // clamp the last step to land on t_end
+    int stop_at_event;              // an instance stops at its first event
+    double event_vmax;              // slip-velocity threshold of an event (m/s)
     int *all_done;                  // count of finished instances
     int total_inst;
 };

@@ -589,8 +589,11 @@ __global__ void ens_control_kernel(EnsTable *tab)
         G.accepted[inst] = 1;
         G.need_rhs[inst] = 1;
         G.h[inst] = hnext;
+        // vMax of this step's start (ratestate.f90:530-532 logs it per step)
+        const bool event = G.t_event[inst] < 0.0 && G.vmax[inst] > tab->event_vmax;
+        if (event) G.t_event[inst] = G.time[inst] - h;
         G.vmax[inst] = 0.0;
-        if (tab->t_end <= G.time[inst]) {
+        if (tab->t_end <= G.time[inst] || (event && tab->stop_at_event)) {
             G.done[inst] = 1;
             atomicAdd(tab->all_done, 1);
         }
@@ -612,7 +615,8 @@ __global__ void ens_rearm_kernel(EnsTable *tab)
     const int inst = (blk - tab->blk_inst[g]) * 64 + threadIdx.x;
     if (inst >= G.ninst) return;
     const int done = !(G.status[inst] == 0 && G.time[inst] < tab->t_end &&
-                       G.attempts[inst] < tab->max_attempts);
+                       G.attempts[inst] < tab->max_attempts &&
+                       !(tab->stop_at_event && G.t_event[inst] >= 0.0));
     G.done[inst] = done;
     if (done) atomicAdd(tab->all_done, 1);
 }

@@ -119,6 +119,18 @@ class Ensemble:
         measurement option; the reference's loop steps past the interval)."""
         self._chk(self._L.ucy_ensemble_set_stop_at_end(self._h, int(bool(on))))
 
+    def set_events(self, vmax_threshold: float = 1e-3, stop_at_event: bool = False):
+        """Record each instance's first event (step-start vMax above the
+        threshold); optionally stop the instance there."""
+        self._chk(self._L.ucy_ensemble_set_events(self._h, float(vmax_threshold),
+                                                  int(bool(stop_at_event))))
+
+    def first_events(self, group: int) -> np.ndarray:
+        _, n_inst = self.groups[group]
+        t = np.empty(n_inst)
+        self._chk(self._L.ucy_ensemble_first_events(self._h, group, _p(t)))
+        return t
+
     def state(self, group: int):
         N, n_inst = self.groups[group]
         y = np.empty((n_inst, 7 * N))
