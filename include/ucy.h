@@ -186,6 +186,17 @@ typedef struct {
 int ucy_integrate(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
                   double epsilon, double max_step, double *t_log, double *dt_log,
                   double *vmax_log, int64_t log_cap, ucy_run_stats *stats);
+/* The same main loop with the step history the learned step-size proposal is
+ * trained on: per accepted step also errmax/epsilon of the accepted attempt
+ * and the number of rejected attempts before it (errmax and rejects may be
+ * NULL).  No counterpart in the reference. */
+typedef struct {
+    double *t, *dt, *vmax, *errmax, *rejects;
+    int64_t cap;
+} ucy_step_log;
+int ucy_integrate_ex(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
+                     double epsilon, double max_step, const ucy_step_log *log,
+                     ucy_run_stats *stats);
 
 /* Measurement hook (no counterpart in the reference): average duration in
  * seconds of the fused matvec + odefun post-pass kernel (the dominant kernel
@@ -255,6 +266,36 @@ int ucy_ensemble_tuning(const ucy_ensemble *ens, int *kb_per_split, double *gemm
 /* HIP-event timing of the all-groups stage GEMM (seconds per launch) and its
  * algorithmic HBM bytes per launch */
 int ucy_ensemble_time_gemm(ucy_ensemble *ens, int iters, double *avg_seconds, double *bytes);
+
+/* ---- learned RK45 step-size proposal (no counterpart in the reference) --- */
+/* An 8 -> 64 -> 64 -> 1 tanh perceptron (binary32, 4801 parameters) that
+ * proposes log10 of the next step from per-instance step history; trained
+ * on the plain integrator's step logs (ucy_integrate_ex) by hand-written
+ * forward/backward kernels with Adam.  In an ensemble it replaces the growth
+ * rule of RKQSm on acceptance; the embedded error estimate still accepts or
+ * rejects, and a rejection shrinks with the classic rule (ode45.f90:82). */
+#define UCY_MLP_INPUTS 8
+#define UCY_MLP_PARAMS 4801
+typedef struct ucy_mlp ucy_mlp;
+ucy_mlp *ucy_mlp_create(ucy_ctx *ctx, uint64_t seed);   /* Glorot-uniform init */
+void ucy_mlp_destroy(ucy_mlp *mlp);
+int ucy_mlp_set_params(ucy_mlp *mlp, const float *theta);
+int ucy_mlp_get_params(ucy_mlp *mlp, float *theta);
+/* feature and target normalisation: x_n = (x - mean) / scale */
+int ucy_mlp_set_normalization(ucy_mlp *mlp, const float *x_mean, const float *x_scale,
+                              float y_mean, float y_scale);
+/* raw features x (n x 8) -> raw outputs y (n) */
+int ucy_mlp_forward(ucy_mlp *mlp, int64_t n, const float *x, float *y);
+/* mean squared error of the normalised outputs over n samples and its
+ * gradient with respect to the 4801 parameters */
+int ucy_mlp_loss_grad(ucy_mlp *mlp, int64_t n, const float *x, const float *t, double *loss,
+                      float *grad);
+/* Adam over shuffled minibatches; loss_log (epochs entries, may be NULL) */
+int ucy_mlp_train(ucy_mlp *mlp, int64_t n, const float *x, const float *t, int epochs,
+                  int batch, float lr, uint64_t seed, double *loss_log);
+/* use (a copy of) mlp for the step proposals of an ensemble; NULL restores
+ * the classic controller */
+int ucy_ensemble_set_mlp(ucy_ensemble *ens, const ucy_mlp *mlp);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,124 @@
+"""The step-size network on the GPU: forward pass and gradient of the
+training loss against the binary64 numpy oracle (oracle/mlp_oracle.py), and
+training by the hand-written kernels.
+
+Tolerances: the device computes in binary32 (tanhf, fmaf) and sums the
+per-sample gradient terms in blocks of 64; against binary64 the relative
+error of the loss and the normwise error of the gradient are <= 1e-4, the
+outputs within 1e-4 of their scale."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from unicycle_amd import stepnet
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed=0, n=3000):
+    rng = np.random.default_rng(seed)
+    th = (rng.normal(0, 0.25, M.NP)).astype(np.float32)
+    X = rng.normal(0, 1, (n, 8)).astype(np.float32) * 3 + 1
+    T = rng.normal(0, 1, n).astype(np.float32) * 2 - 5
+    norm = (np.full(8, 1.0, np.float32), np.full(8, 3.0, np.float32), -5.0, 2.0)
+    return th, X, T, norm
+
+
+def test_forward_and_gradient_match_oracle(engine_factory):
+    eng = engine_factory()
+    net = stepnet.StepNet(eng, seed=1)
+    th, X, T, norm = _case()
+    net.params = th
+    net.set_normalization(*norm)
+    np.testing.assert_array_equal(net.params, th)
+    y = net.forward(X)
+    yo = M.predict(th.astype(np.float64), X, norm)
+    assert np.abs(y - yo).max() <= 1e-4 * np.abs(yo).max()
+    loss, g = net.loss_grad(X, T)
+    lo, go = M.loss_grad(th.astype(np.float64), X, T, norm)
+    assert abs(loss - lo) <= 1e-4 * lo
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    # every parameter block carries gradient (a dropped term shows here)
+    for a, b in ((0, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                 (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.NP)):
+        assert np.linalg.norm(g[a:b] - go[a:b]) <= 1e-3 * np.linalg.norm(go[a:b]) + 1e-12
+    net.close()
+
+
+def test_training_fits_a_smooth_map(engine_factory):
+    eng = engine_factory()
+    net = stepnet.StepNet(eng, seed=2)
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-2, 2, (20000, 8)).astype(np.float32)
+    T = (np.sin(X[:, 0]) + 0.5 * X[:, 2] * X[:, 3] - 0.3 * X[:, 6]).astype(np.float32)
+    net.fit_normalization(X, T)
+    l0, _ = net.loss_grad(X, T)
+    log = net.train(X, T, epochs=25, batch=2048, lr=3e-3, seed=4)
+    l1, _ = net.loss_grad(X, T)
+    assert np.all(np.isfinite(log))
+    assert l1 < 0.05 * l0, (l0, l1, log[-5:])
+    # the trained parameters reproduce the oracle's prediction
+    yo = M.predict(net.params.astype(np.float64), X[:500], net.norm)
+    assert np.abs(net.forward(X[:500]) - yo).max() <= 1e-4 * np.abs(yo).max()
+    net.close()
+
+
+def test_learned_proposal_gain_on_held_out_instances(engine_factory):
+    """Train on the plain binary64 integrator's step logs of 16 seeded
+    instances of a 256-patch fault, then run 48 held-out instances as an
+    ensemble over a fixed simulated window (every instance clamped to land
+    on it) with the classic RKQSm growth rule and with the learned proposal.
+
+    Measured gain (what this test records): fewer RK45 attempts (RHS
+    evaluations) for the same simulated window; first-event times of the
+    instances with an event agree with the classic run to < 0.1 %."""
+    from unicycle_amd import PREC_F64, configs
+    from unicycle_amd.ensemble import Ensemble
+    eng = engine_factory()
+    m = configs.thrust_fault(nx=16, nz=16, ellipse=None)
+    d = configs.ensemble_instances(m, 64, seed=7)
+    ic = engine_factory()
+    ic.set_elastic(m.lambda_, m.mu)
+    ic.set_rectangles(*m.geometry())
+    y0 = []
+    for i in range(64):
+        ic.set_friction(*configs.instance_friction(m, d, i))
+        y0.append(configs.perturbed_initial_state(m, ic.initStateVector(), int(d["seeds"][i])))
+    y0 = np.array(y0)
+    X, T = stepnet.training_data(ic, m, d, range(16), 3000, y0)
+    net = stepnet.StepNet(eng, seed=0)
+    net.fit_normalization(X, T)
+    log = net.train(X, T, epochs=40, batch=4096, lr=2e-3, seed=1)
+    assert log[-1] < 0.05 * log[0]
+    rmse = float(np.sqrt(np.mean((net.forward(X) - T) ** 2)))
+    rmse_classic = float(np.sqrt(np.mean((X[:, 6] - T) ** 2)))
+    held = list(range(16, 64))
+    out = {}
+    for use in (False, True):
+        ens = Ensemble(eng, PREC_F64)
+        ens.add_group(m, d["a"][held], d["b"][held], d["sig"][held], d["L"][held], y0[held])
+        ens.build()
+        if use:
+            net.attach(ens)
+        ens.set_stop_at_end(True)
+        st = ens.run(0.5 * configs.YEAR, max_attempts=10**6, chunk_attempts=5000)
+        assert st.n_failed == 0 and st.min_time >= 0.5 * configs.YEAR
+        y, t, dt, s, a = ens.state(0)
+        out[use] = (a.sum(), s.sum(), ens.first_events(0))
+        ens.close()
+    (a0, s0, e0), (a1, s1, e1) = out[False], out[True]
+    both = (e0 > 0) & (e1 > 0)
+    rel = np.abs(e1[both] - e0[both]) / e0[both]
+    print(f"learned proposal: rmse {rmse:.3f} (classic rule {rmse_classic:.3f}) in log10 h; "
+          f"attempts {a1} vs {a0} (gain {a0 / a1:.3f}); rejected {1 - s1 / a1:.3f} vs "
+          f"{1 - s0 / a0:.3f}; events {both.sum()}, max rel time diff {rel.max():.2e}")
+    assert rmse < rmse_classic
+    assert a1 < a0                       # fewer RHS evaluations for the same window
+    assert ((e0 > 0) ^ (e1 > 0)).sum() == 0 and rel.max() < 1e-3
+    net.close()

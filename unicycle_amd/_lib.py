@@ -17,6 +17,7 @@ _i64 = C.c_int64
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
 _vp = C.c_void_p
+_pf = C.POINTER(C.c_float)
 
 
 class EnsembleStats(C.Structure):
@@ -42,6 +43,17 @@ class RunStats(C.Structure):
         ("rhs_evals", C.c_int64),
         ("vmax", C.c_double),
         ("wall_seconds", C.c_double),
+    ]
+
+
+class StepLog(C.Structure):
+    _fields_ = [
+        ("t", C.POINTER(C.c_double)),
+        ("dt", C.POINTER(C.c_double)),
+        ("vmax", C.POINTER(C.c_double)),
+        ("errmax", C.POINTER(C.c_double)),
+        ("rejects", C.POINTER(C.c_double)),
+        ("cap", C.c_int64),
     ]
 
 
@@ -86,7 +98,18 @@ _SIGS = {
     "ucy_rkqs": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _d, _pd, _pd, _d, _d]),
     "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _pd, _i64,
                            C.POINTER(RunStats)]),
+    "ucy_integrate_ex": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, C.POINTER(StepLog),
+                              C.POINTER(RunStats)]),
     "ucy_time_rhs_kernel": (_i, [_vp, _i, _pd]),
+    "ucy_mlp_create": (_vp, [_vp, C.c_uint64]),
+    "ucy_mlp_destroy": (None, [_vp]),
+    "ucy_mlp_set_params": (_i, [_vp, _pf]),
+    "ucy_mlp_get_params": (_i, [_vp, _pf]),
+    "ucy_mlp_set_normalization": (_i, [_vp, _pf, _pf, C.c_float, C.c_float]),
+    "ucy_mlp_forward": (_i, [_vp, _i64, _pf, _pf]),
+    "ucy_mlp_loss_grad": (_i, [_vp, _i64, _pf, _pf, _pd, _pf]),
+    "ucy_mlp_train": (_i, [_vp, _i64, _pf, _pf, _i, _i, C.c_float, C.c_uint64, _pd]),
+    "ucy_ensemble_set_mlp": (_i, [_vp, _vp]),
     "ucy_ensemble_create": (_vp, [_vp, _i]),
     "ucy_ensemble_destroy": (None, [_vp]),
     "ucy_ensemble_add_group": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _d, _d, _i, _pd, _pd, _pd,

@@ -573,7 +573,7 @@ static int ensure_work(ucy_ctx *c)
     b.yr = nullptr;
     b.v = p; p += d.n;
     b.t = p; p += d.m;
-    b.tlog = b.dtlog = b.vlog = nullptr;
+    b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
     c->work_dim = d.dim;
     return 0;
 }
@@ -776,6 +776,18 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                              double *dt_log, double *vmax_log, int64_t log_cap,
                              ucy_run_stats *stats)
 {
+    ucy_step_log log = {t_log, dt_log, vmax_log, nullptr, nullptr, log_cap};
+    return ucy_integrate_ex(c, y, interval, dt0, max_steps, epsilon, max_step, &log, stats);
+}
+
+extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double dt0,
+                                int64_t max_steps, double epsilon, double max_step,
+                                const ucy_step_log *log, ucy_run_stats *stats)
+{
+    double *t_log = log ? log->t : nullptr, *dt_log = log ? log->dt : nullptr;
+    double *vmax_log = log ? log->vmax : nullptr;
+    double *err_log = log ? log->errmax : nullptr, *rej_log = log ? log->rejects : nullptr;
+    const int64_t log_cap = log ? log->cap : 0;
     if (!c || !y) return UCY_ERR_ARG;
     const PlainDims d0 = c->K ? dims(c) : PlainDims{};
     int rc = check_ready(c, d0.dim);
@@ -787,7 +799,8 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
     // device log buffers (reallocated per call; logs are parity aids)
     double *d_log = nullptr;
     const int64_t cap = (t_log && dt_log && vmax_log) ? log_cap : 0;
-    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * 3 * cap));
+    const bool ext = cap > 0 && err_log && rej_log;
+    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * (ext ? 5 : 3) * cap));
     if (c->graph && (b.tlog != d_log)) {
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -795,6 +808,8 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
     b.tlog = d_log;
     b.dtlog = d_log ? d_log + cap : nullptr;
     b.vlog = d_log ? d_log + 2 * cap : nullptr;
+    b.elog = ext ? d_log + 3 * cap : nullptr;
+    b.rlog = ext ? d_log + 4 * cap : nullptr;
     Control &h = *c->h_ctl;
     memset(&h, 0, sizeof(Control));
     h.time = 0.0;
@@ -831,6 +846,12 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                               hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(vmax_log, b.vlog, sizeof(double) * h.log_count,
                               hipMemcpyDeviceToHost, c->stream));
+        if (ext) {
+            HIPCHK(hipMemcpyAsync(err_log, b.elog, sizeof(double) * h.log_count,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(rej_log, b.rlog, sizeof(double) * h.log_count,
+                                  hipMemcpyDeviceToHost, c->stream));
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (d_log) {
@@ -838,7 +859,7 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
         HIPCHK(hipFree(d_log));
-        b.tlog = b.dtlog = b.vlog = nullptr;
+        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
     }
     if (stats) {
         stats->time = h.time;

@@ -41,6 +41,9 @@ struct ucy_ensemble {
     int stop_at_end = 0;
     int stop_at_event = 0;
     double event_vmax = 1e-3;
+    float *d_mlp = nullptr;  // copy of the proposal network's parameters
+    MlpNorm mlp_norm{};
+    bool use_mlp = false;
     std::vector<double> tune_seconds;
     double gemm_bytes = 0;
 };
@@ -104,6 +107,7 @@ extern "C" void ucy_ensemble_destroy(ucy_ensemble *E)
     (void)hipSetDevice(E->ctx->device);
     (void)hipStreamSynchronize(E->ctx->stream);
     release(E);
+    if (E->d_mlp) (void)hipFree(E->d_mlp);
     auto &reg = E->ctx->ensembles;
     reg.erase(std::remove(reg.begin(), reg.end(), E), reg.end());
     delete E;
@@ -167,6 +171,8 @@ static int load_initial(ucy_ensemble *E, double dt0)
         EHIP(hipMemcpyAsync(G.vmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         std::vector<double> none(kEnsCols, -1.0);
         EHIP(hipMemcpyAsync(G.t_event, none.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> hz(4 * kEnsCols, 0.0);
+        EHIP(hipMemcpyAsync(G.hist, hz.data(), sizeof(double) * hz.size(), hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.errmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.need_rhs, one.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.accepted, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
@@ -333,6 +339,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         int **ci[] = {&G.need_rhs, &G.accepted, &G.done, &G.status};
         for (int **v : ci)
             if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.hist, 4 * kEnsCols)) return UCY_ERR_HIP;
         if (ealloc(E, &G.steps, kEnsCols)) return UCY_ERR_HIP;
         if (ealloc(E, &G.attempts, kEnsCols)) return UCY_ERR_HIP;
         T.blk_elem[gi] = eacc;
@@ -447,6 +454,8 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     E->tab.stop_at_end = E->stop_at_end;
     E->tab.stop_at_event = E->stop_at_event;
     E->tab.event_vmax = E->event_vmax;
+    E->tab.mlp = E->use_mlp ? E->d_mlp : nullptr;
+    E->tab.mlp_norm = E->mlp_norm;
     if (upload_table(E)) return UCY_ERR_HIP;
     if (capture(E)) return UCY_ERR_HIP;
     // continue from the current state: re-arm every instance short of t_end
@@ -502,6 +511,26 @@ extern "C" int ucy_ensemble_set_stop_at_end(ucy_ensemble *E, int on)
 {
     if (!E) return UCY_ERR_ARG;
     E->stop_at_end = on ? 1 : 0;
+    return UCY_OK;
+}
+
+namespace ucy {
+const float *mlp_device_params(const ucy_mlp *M);
+MlpNorm mlp_norm(const ucy_mlp *M);
+}
+
+extern "C" int ucy_ensemble_set_mlp(ucy_ensemble *E, const ucy_mlp *M)
+{
+    if (!E) return UCY_ERR_ARG;
+    EHIP(hipSetDevice(E->ctx->device));
+    if (!M) {
+        E->use_mlp = false;
+        return UCY_OK;
+    }
+    if (!E->d_mlp) EHIP(hipMalloc((void **)&E->d_mlp, sizeof(float) * kMlpParams));
+    EHIP(hipMemcpy(E->d_mlp, mlp_device_params(M), sizeof(float) * kMlpParams, hipMemcpyDeviceToDevice));
+    E->mlp_norm = mlp_norm(M);
+    E->use_mlp = true;
     return UCY_OK;
 }
 

@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "mlp_dev.h"
 
 namespace ucy {
 
@@ -52,6 +53,8 @@ struct EnsGroup {
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;
     double *t_event;         // time of the first step starting with vMax > event_vmax (-1: none)
+    double *hist;            // [4][64] proposal history: log10 vMax, h, log10 errmax of the
+                             // last accepted step (h = 0: none yet), rejections of this step
     int *need_rhs, *accepted, *done, *status;
     long long *steps, *attempts;
 };
@@ -67,6 +70,8 @@ struct EnsTable {
     int stop_at_end;                // clamp the last step to land on t_end
     int stop_at_event;              // an instance stops at its first event
     double event_vmax;              // slip-velocity threshold of an event (m/s)
+    const float *mlp;               // step-size proposal parameters (null: classic control)
+    MlpNorm mlp_norm;
     int *all_done;                  // count of finished instances
     int total_inst;
 };

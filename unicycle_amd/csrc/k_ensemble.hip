@@ -565,10 +565,12 @@ __global__ void ens_control_kernel(EnsTable *tab)
     }
     const double errmax = raw / tab->epsilon;
     double h = G.h[inst];
+    double *hist = G.hist;
     if (errmax > 1.0) {
         h = h * SAFETY * pow(errmax, PSHRNK);
         G.h[inst] = h;
         G.need_rhs[inst] = 0;
+        hist[3 * kEnsCols + inst] += 1.0;
         if (0.0 + h == 0.0) {
             G.status[inst] = 1;
             G.done[inst] = 1;
@@ -577,6 +579,27 @@ __global__ void ens_control_kernel(EnsTable *tab)
     } else {
         double hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
         if (hnext > tab->max_step) hnext = tab->max_step;
+        if (tab->mlp && errmax > ERRCON) {
+            // learned proposal of the next step where the error estimate is
+            // informative (below ERRCON the classic rule already takes the
+            // largest growth, x10); the embedded error estimate still judges
+            // the proposal and a rejection falls back to the classic shrink
+            const bool first = hist[1 * kEnsCols + inst] == 0.0;
+            float f[kMlpIn];
+            mlp_features(G.vmax[inst], hist[inst], h, hist[1 * kEnsCols + inst], errmax,
+                         hist[2 * kEnsCols + inst], hist[3 * kEnsCols + inst], hnext, first, f);
+            const MlpNorm &nm = tab->mlp_norm;
+#pragma unroll
+            for (int i = 0; i < kMlpIn; i++) f[i] = (f[i] - nm.xmean[i]) / nm.xscale[i];
+            const double lp = (double)(mlp_eval(tab->mlp, f) * nm.yscale + nm.ymean);
+            double hp = pow(10.0, lp);
+            hp = fmin(fmax(hp, 0.1 * h), 10.0 * h);
+            hnext = (hp == hp) ? fmin(hp, tab->max_step) : hnext;
+        }
+        hist[inst] = log10(fmax(G.vmax[inst], 1e-30));
+        hist[1 * kEnsCols + inst] = h;
+        hist[2 * kEnsCols + inst] = log10(fmax(errmax, 1e-10));
+        hist[3 * kEnsCols + inst] = 0.0;
         G.hdid[inst] = h;
         // the step that was clamped to the end lands on t_end exactly
         const double tnew = G.time[inst] + h;

@@ -462,6 +462,7 @@ __global__ void controller_kernel(RkBuffers b, Control *ctl)
         ctl->h = h;
         ctl->need_rhs = 0;
         ctl->accepted = 0;
+        ctl->step_rejects += 1;
         if (0.0 + h == 0.0) {
             ctl->error = 1;
             ctl->done = 1;
@@ -478,8 +479,13 @@ __global__ void controller_kernel(RkBuffers b, Control *ctl)
         b.tlog[ctl->log_count] = ctl->time;
         b.dtlog[ctl->log_count] = h;
         b.vlog[ctl->log_count] = ctl->vmax;
+        if (b.elog) {
+            b.elog[ctl->log_count] = errmax;
+            b.rlog[ctl->log_count] = (double)ctl->step_rejects;
+        }
         ctl->log_count += 1;
     }
+    ctl->step_rejects = 0;
     ctl->last_vmax = ctl->vmax;
     ctl->vmax = 0.0;
     ctl->accepted = 1;

@@ -43,6 +43,7 @@ struct Control {
     int error;          // 0 ok, 1 underflow, 2 non-finite error norm
     long long steps, attempts, max_steps, rhs_evals;
     long long log_count, log_cap;
+    int step_rejects;   // rejected attempts of the current step
 };
 
 struct PlainDims {
@@ -70,6 +71,7 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 struct RkBuffers {
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr, *yr, *v, *t;
     double *tlog, *dtlog, *vlog;
+    double *elog, *rlog;   // errmax/epsilon of the accepted attempt, rejections (optional)
 };
 void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
                        const RkBuffers &b, Control *ctl);

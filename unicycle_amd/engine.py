@@ -63,6 +63,8 @@ class IntegrateResult:
     t_log: np.ndarray | None = None
     dt_log: np.ndarray | None = None
     vmax_log: np.ndarray | None = None
+    errmax_log: np.ndarray | None = None    # errmax/epsilon of each accepted attempt
+    reject_log: np.ndarray | None = None    # rejected attempts before each accepted one
 
 
 class Engine:
@@ -213,23 +215,27 @@ class Engine:
 
     def integrate(self, y, interval: float, dt0: float = 1.0e-3, max_steps: int = 1000000,
                   epsilon: float = 1.0e-6, max_step: float = 1.7976931348623158e308,
-                  log_cap: int = 0) -> IntegrateResult:
-        """Main loop of ratestate.f90:307-328, device resident."""
+                  log_cap: int = 0, history: bool = False) -> IntegrateResult:
+        """Main loop of ratestate.f90:307-328, device resident.  log_cap > 0
+        logs time, dt and vMax per accepted step; history adds errmax/epsilon
+        and the rejections of each step (the step-size proposal's data)."""
         y = _f64(y).copy()
         st = _lib.RunStats()
-        tl = np.empty(max(log_cap, 1))
-        dl = np.empty(max(log_cap, 1))
-        vl = np.empty(max(log_cap, 1))
-        self._chk(self._L.ucy_integrate(self._h, _p(y), float(interval), float(dt0),
-                                        int(max_steps), float(epsilon), float(max_step),
-                                        _p(tl) if log_cap else None, _p(dl) if log_cap else None,
-                                        _p(vl) if log_cap else None, int(log_cap), C.byref(st)))
+        logs = [np.empty(max(log_cap, 1)) for _ in range(5)]
+        lg = _lib.StepLog()
+        if log_cap:
+            lg.t, lg.dt, lg.vmax = _p(logs[0]), _p(logs[1]), _p(logs[2])
+            if history:
+                lg.errmax, lg.rejects = _p(logs[3]), _p(logs[4])
+        lg.cap = int(log_cap)
+        self._chk(self._L.ucy_integrate_ex(self._h, _p(y), float(interval), float(dt0),
+                                           int(max_steps), float(epsilon), float(max_step),
+                                           C.byref(lg), C.byref(st)))
         nlog = min(st.steps, log_cap)
+        cut = lambda a, on=True: a[:nlog].copy() if (log_cap and on) else None
         return IntegrateResult(y, st.time, st.dt_next, st.steps, st.attempts, st.rhs_evals,
-                               st.vmax, st.wall_seconds,
-                               tl[:nlog].copy() if log_cap else None,
-                               dl[:nlog].copy() if log_cap else None,
-                               vl[:nlog].copy() if log_cap else None)
+                               st.vmax, st.wall_seconds, cut(logs[0]), cut(logs[1]),
+                               cut(logs[2]), cut(logs[3], history), cut(logs[4], history))
 
     def time_rhs_kernel(self, iters: int = 50) -> float:
         """Average seconds of the fused matvec + post-pass kernel (HIP events)."""
