@@ -36,6 +36,13 @@ Reported:
   roofline      the stage GEMM of the headline variant (dominant kernel),
                 timed with HIP events on the library stream: HBM-bound for
                 split bf16, MFMA-bound for binary64.
+  learned_step_proposal
+                the 8-64-64-1 step-size network trained on the plain binary64
+                integrator's step histories of 16 instances of the 1024-patch
+                group, against the classic controller on the 48 held-out
+                instances over a fixed simulated window: RHS evaluations per
+                simulated year, rejected fraction, slip and first-event-time
+                differences, and sim-yr/s against the plain binary64 path.
   cpu_baseline  the oracle's C restatement (OpenMP over the host cores): RK45
                 attempts/s on instance 0 of every group over a bounded sample,
                 costed over every instance's attempt count of the GPU run.
@@ -285,6 +292,81 @@ def bench_ensemble(args):
     return out, (cpu_inputs, att64, sim_years)
 
 
+def bench_learned(args, plain_us_per_attempt):
+    """The learned step-size proposal on a fixed simulated window: the
+    1024-patch group's instances 0..15 give the plain binary64 integrator's
+    step histories (training data); the 48 held-out instances run as an
+    ensemble to the window's end (clamped) with the classic controller and
+    with the learned proposal, in binary64 and in split bf16."""
+    from unicycle_amd import Engine, PREC_F64, PREC_SPLIT_BF16, configs, stepnet
+    from unicycle_amd.ensemble import Ensemble
+    T = args.learned_years * configs.YEAR
+    nx, nz = configs.ENSEMBLE_GROUPS[0]
+    m = configs.thrust_fault(nx=nx, nz=nz, ellipse=None)
+    d = configs.ensemble_instances(m, 64, seed=100)   # the bench's group 0 draws
+    eng = Engine(0)
+    ic = Engine(0)
+    ic.set_elastic(m.lambda_, m.mu)
+    ic.set_rectangles(*m.geometry())
+    y0 = []
+    for i in range(64):
+        ic.set_friction(*configs.instance_friction(m, d, i))
+        y0.append(configs.perturbed_initial_state(m, ic.initStateVector(), int(d["seeds"][i])))
+    y0 = np.array(y0)
+    t0 = time.perf_counter()
+    X, Tt = stepnet.training_data(ic, m, d, range(16), args.learned_steps, y0)
+    t_data = time.perf_counter() - t0
+    ic.close()
+    net = stepnet.StepNet(eng, seed=0)
+    net.fit_normalization(X, Tt)
+    t0 = time.perf_counter()
+    loss = net.train(X, Tt, epochs=40, batch=4096, lr=2e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    rmse = float(np.sqrt(np.mean((net.forward(X) - Tt) ** 2)))
+    rmse_classic = float(np.sqrt(np.mean((X[:, 6] - Tt) ** 2)))
+    log(f"learned: {X.shape[0]} samples in {t_data:.1f} s, trained in {t_train:.2f} s, "
+        f"rmse {rmse:.3f} (classic rule {rmse_classic:.3f})")
+    held = list(range(16, 64))
+    runs = {}
+    for name, prec, use in (("f64_classic", PREC_F64, False), ("f64_learned", PREC_F64, True),
+                            ("split_bf16_learned", PREC_SPLIT_BF16, True)):
+        ens = Ensemble(eng, prec)
+        ens.add_group(m, d["a"][held], d["b"][held], d["sig"][held], d["L"][held], y0[held])
+        ens.build()
+        if use:
+            net.attach(ens)
+        ens.set_stop_at_end(True)
+        ens.set_events(1e-3, False)
+        t0 = time.perf_counter()
+        st = ens.run(T, max_attempts=10**6, chunk_attempts=2000, progress=_progress(name))
+        wall = time.perf_counter() - t0
+        y, t, dt, steps, att = ens.state(0)
+        runs[name] = {"y": y, "att": att, "steps": steps, "te": ens.first_events(0),
+                      "wall": wall, "failed": st.n_failed}
+        ens.close()
+    net.close()
+    ref = runs["f64_classic"]
+    n_h = len(held)
+    plain_wall = float(np.sum(ref["att"])) * plain_us_per_attempt * 1e-6
+    out = {"window_years": args.learned_years, "patches": m.n, "held_out_instances": n_h,
+           "training": {"samples": int(X.shape[0]), "instances": 16,
+                        "steps_per_instance": args.learned_steps, "epochs": 40,
+                        "loss_first_last": [float(loss[0]), float(loss[-1])],
+                        "rmse_log10_h": rmse, "rmse_classic_rule": rmse_classic,
+                        "train_s": t_train},
+           "plain_fp64_classic_sim_yr_per_s": n_h * args.learned_years / plain_wall}
+    for name, r in runs.items():
+        a, s_ = float(np.sum(r["att"])), float(np.sum(r["steps"]))
+        out[name] = {"sim_yr_per_s": n_h * args.learned_years / r["wall"],
+                     "vs_plain_fp64": plain_wall / r["wall"],
+                     "rhs_per_sim_yr_per_instance": 6.0 * a / (n_h * args.learned_years),
+                     "rejected_fraction": 1.0 - s_ / a, "failed": int(r["failed"]),
+                     "attempt_gain_vs_classic": float(np.sum(ref["att"])) / a,
+                     "slip_error_vs_f64_classic": slip_error([ref["y"]], [r["y"]]),
+                     "first_event_time_vs_f64_classic": event_error(ref["te"], r["te"])}
+    return out
+
+
 def cpu_baseline_ensemble(cpu_inputs, att64, sim_years, budget_s=20.0):
     """Oracle (C restatement, OpenMP) on the host cores: attempts/s of
     instance 0 of each group over a bounded sample of its run, costed over
@@ -407,6 +489,11 @@ def main():
     ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16"],
                     help="stage-GEMM precision of --workload gemm")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-years", type=float, default=0.25,
+                    help="simulated window of the learned-proposal comparison")
+    ap.add_argument("--learned-steps", type=int, default=3000,
+                    help="plain-integrator steps per training instance")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X and has no "
@@ -418,6 +505,9 @@ def main():
         print(json.dumps(fn(args)), flush=True)
         return
     out, (cpu_inputs, att64, sim_years) = bench_ensemble(args)
+    if not args.no_learned:
+        out["learned_step_proposal"] = bench_learned(
+            args, out["plain_fp64"]["groups"][0]["us_per_attempt"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ensemble(cpu_inputs, att64, sim_years)
     print(json.dumps(out), flush=True)

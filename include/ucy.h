@@ -254,6 +254,9 @@ int ucy_ensemble_set_stop_at_end(ucy_ensemble *ens, int on);
 int ucy_ensemble_set_events(ucy_ensemble *ens, double vmax_threshold, int stop_at_event);
 /* first-event times of a group's instances (-1: no event yet) */
 int ucy_ensemble_first_events(ucy_ensemble *ens, int group, double *t_event);
+/* per-instance status: 0 running or finished, 1 step-size underflow
+ * (ode45.f90:86-90), 2 non-finite error norm */
+int ucy_ensemble_status(ucy_ensemble *ens, int group, int *status);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
  * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,

@@ -108,7 +108,8 @@ def test_learned_proposal_gain_on_held_out_instances(engine_factory):
             net.attach(ens)
         ens.set_stop_at_end(True)
         st = ens.run(0.5 * configs.YEAR, max_attempts=10**6, chunk_attempts=5000)
-        assert st.n_failed == 0 and st.min_time >= 0.5 * configs.YEAR
+        assert st.n_failed == 0, f"learned={use}: status {ens.status(0)}"
+        assert st.min_time >= 0.5 * configs.YEAR
         y, t, dt, s, a = ens.state(0)
         out[use] = (a.sum(), s.sum(), ens.first_events(0))
         ens.close()

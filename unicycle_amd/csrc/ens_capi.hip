@@ -554,6 +554,18 @@ extern "C" int ucy_ensemble_first_events(ucy_ensemble *E, int group, double *t_e
     return UCY_OK;
 }
 
+extern "C" int ucy_ensemble_status(ucy_ensemble *E, int group, int *status)
+{
+    if (!E || group < 0 || group >= (int)E->hg.size() || !status) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    std::vector<int> tmp(kEnsCols);
+    EHIP(hipMemcpy(tmp.data(), E->tab.g[group].status, sizeof(int) * kEnsCols, hipMemcpyDeviceToHost));
+    std::copy(tmp.begin(), tmp.begin() + E->hg[group].ninst, status);
+    return UCY_OK;
+}
+
 extern "C" int ucy_ensemble_get_state(ucy_ensemble *E, int group, double *y, double *time,
                                       double *dt_next, int64_t *steps, int64_t *attempts)
 {

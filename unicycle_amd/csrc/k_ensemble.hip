@@ -592,8 +592,11 @@ __global__ void ens_control_kernel(EnsTable *tab)
 #pragma unroll
             for (int i = 0; i < kMlpIn; i++) f[i] = (f[i] - nm.xmean[i]) / nm.xscale[i];
             const double lp = (double)(mlp_eval(tab->mlp, f) * nm.yscale + nm.ymean);
+            // within a factor 4 of the classic proposal and the classic
+            // growth limit: the network corrects the rule, it does not
+            // extrapolate past it
             double hp = pow(10.0, lp);
-            hp = fmin(fmax(hp, 0.1 * h), 10.0 * h);
+            hp = fmin(fmax(hp, 0.25 * hnext), fmin(4.0 * hnext, 10.0 * h));
             hnext = (hp == hp) ? fmin(hp, tab->max_step) : hnext;
         }
         hist[inst] = log10(fmax(G.vmax[inst], 1e-30));

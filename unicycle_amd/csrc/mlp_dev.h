@@ -22,7 +22,8 @@
 //   f5 rejected attempts of step k
 //   f6 log10 of the classic proposal (ode45.f90:92-99)
 //   f7 f3 of step k-1                            (f3 at the first step)
-// The output is the normalised log10 of the proposed step.
+// The output is the normalised log10 of the proposed step; the controller
+// keeps it within a factor 4 of the classic proposal and below 10 h_k.
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -131,6 +131,14 @@ class Ensemble:
         self._chk(self._L.ucy_ensemble_first_events(self._h, group, _p(t)))
         return t
 
+    def status(self, group: int) -> np.ndarray:
+        """0 running/finished, 1 step-size underflow, 2 non-finite error norm."""
+        _, n_inst = self.groups[group]
+        st = np.empty(n_inst, dtype=np.int32)
+        self._chk(self._L.ucy_ensemble_status(self._h, group,
+                                              st.ctypes.data_as(C.POINTER(C.c_int))))
+        return st
+
     def state(self, group: int):
         N, n_inst = self.groups[group]
         y = np.empty((n_inst, 7 * N))
