@@ -295,9 +295,11 @@ def bench_ensemble(args):
 def bench_learned(args, plain_us_per_attempt):
     """The learned step-size proposal on a fixed simulated window: the
     1024-patch group's instances 0..15 give the plain binary64 integrator's
-    step histories (training data); the 48 held-out instances run as an
-    ensemble to the window's end (clamped) with the classic controller and
-    with the learned proposal, in binary64 and in split bf16."""
+    step histories (training data), refined by two closed-loop rounds on the
+    same instances and an output offset picked on half the window; the 48
+    held-out instances run as an ensemble to the window's end (clamped) with
+    the classic controller and with the learned proposal, in binary64 and in
+    split bf16."""
     from unicycle_amd import Engine, PREC_F64, PREC_SPLIT_BF16, configs, stepnet
     from unicycle_amd.ensemble import Ensemble
     T = args.learned_years * configs.YEAR
@@ -326,6 +328,19 @@ def bench_learned(args, plain_us_per_attempt):
     rmse_classic = float(np.sqrt(np.mean((X[:, 6] - Tt) ** 2)))
     log(f"learned: {X.shape[0]} samples in {t_data:.1f} s, trained in {t_train:.2f} s, "
         f"rmse {rmse:.3f} (classic rule {rmse_classic:.3f})")
+    # closed-loop refinement and output offset on the training instances
+    t0 = time.perf_counter()
+    tr = list(range(16))
+    ens = Ensemble(eng, PREC_F64)
+    ens.add_group(m, d["a"][tr], d["b"][tr], d["sig"][tr], d["L"][tr], y0[tr])
+    ens.build()
+    X, Tt = stepnet.fit_closed_loop(net, X, Tt, ens, args.learned_steps, rounds=2,
+                                    progress=_progress("closed-loop"))
+    offset, tried = stepnet.calibrate_offset(net, ens, 0.5 * T, progress=_progress("offset"))
+    ens.close()
+    t_refine = time.perf_counter() - t0
+    log(f"learned: closed-loop data {X.shape[0]} samples, offset {offset} from {tried}, "
+        f"{t_refine:.1f} s")
     held = list(range(16, 64))
     runs = {}
     for name, prec, use in (("f64_classic", PREC_F64, False), ("f64_learned", PREC_F64, True),
@@ -343,6 +358,8 @@ def bench_learned(args, plain_us_per_attempt):
         y, t, dt, steps, att = ens.state(0)
         runs[name] = {"y": y, "att": att, "steps": steps, "te": ens.first_events(0),
                       "wall": wall, "failed": st.n_failed}
+        if st.n_failed:
+            log(f"[{name}] failed instances, status {ens.status(0).tolist()}")
         ens.close()
     net.close()
     ref = runs["f64_classic"]
@@ -353,7 +370,12 @@ def bench_learned(args, plain_us_per_attempt):
                         "steps_per_instance": args.learned_steps, "epochs": 40,
                         "loss_first_last": [float(loss[0]), float(loss[-1])],
                         "rmse_log10_h": rmse, "rmse_classic_rule": rmse_classic,
-                        "train_s": t_train},
+                        "train_s": t_train, "closed_loop_rounds": 2,
+                        "samples_after_closed_loop": int(X.shape[0]),
+                        "output_offset_log10": offset,
+                        "offset_attempts": {str(k): (v if np.isfinite(v) else None)
+                                            for k, v in tried.items()},
+                        "refine_s": t_refine},
            "plain_fp64_classic_sim_yr_per_s": n_h * args.learned_years / plain_wall}
     for name, r in runs.items():
         a, s_ = float(np.sum(r["att"])), float(np.sum(r["steps"]))

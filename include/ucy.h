@@ -255,8 +255,18 @@ int ucy_ensemble_set_events(ucy_ensemble *ens, double vmax_threshold, int stop_a
 /* first-event times of a group's instances (-1: no event yet) */
 int ucy_ensemble_first_events(ucy_ensemble *ens, int group, double *t_event);
 /* per-instance status: 0 running or finished, 1 step-size underflow
- * (ode45.f90:86-90), 2 non-finite error norm */
+ * (ode45.f90:86-90), 2 non-finite error norm on four attempts in a row (an
+ * attempt with a non-finite norm is rejected with a x0.1 shrink) */
 int ucy_ensemble_status(ucy_ensemble *ens, int group, int *status);
+/* per-step log of a group's accepted steps (the ensemble form of the step
+ * logs of ucy_integrate_ex; no counterpart in the reference): cap steps per
+ * instance from the last reset, cap = 0 turns it off.  get_step_log writes
+ * [ninst][cap] arrays (any may be null): time after the step, accepted h,
+ * step-start vMax, errmax / epsilon, rejected attempts of the step; count[i]
+ * = logged steps of instance i */
+int ucy_ensemble_set_step_log(ucy_ensemble *ens, int group, long long cap);
+int ucy_ensemble_get_step_log(ucy_ensemble *ens, int group, double *t, double *dt,
+                              double *vmax, double *errmax, double *rejects, long long *count);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
  * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,

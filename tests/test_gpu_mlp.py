@@ -69,20 +69,10 @@ def test_training_fits_a_smooth_map(engine_factory):
     net.close()
 
 
-def test_learned_proposal_gain_on_held_out_instances(engine_factory):
-    """Train on the plain binary64 integrator's step logs of 16 seeded
-    instances of a 256-patch fault, then run 48 held-out instances as an
-    ensemble over a fixed simulated window (every instance clamped to land
-    on it) with the classic RKQSm growth rule and with the learned proposal.
-
-    Measured gain (what this test records): fewer RK45 attempts (RHS
-    evaluations) for the same simulated window; first-event times of the
-    instances with an event agree with the classic run to < 0.1 %."""
-    from unicycle_amd import PREC_F64, configs
-    from unicycle_amd.ensemble import Ensemble
-    eng = engine_factory()
-    m = configs.thrust_fault(nx=16, nz=16, ellipse=None)
-    d = configs.ensemble_instances(m, 64, seed=7)
+def _instances(engine_factory, nx, nz, seed):
+    from unicycle_amd import configs
+    m = configs.thrust_fault(nx=nx, nz=nz, ellipse=None)
+    d = configs.ensemble_instances(m, 64, seed=seed)
     ic = engine_factory()
     ic.set_elastic(m.lambda_, m.mu)
     ic.set_rectangles(*m.geometry())
@@ -90,7 +80,67 @@ def test_learned_proposal_gain_on_held_out_instances(engine_factory):
     for i in range(64):
         ic.set_friction(*configs.instance_friction(m, d, i))
         y0.append(configs.perturbed_initial_state(m, ic.initStateVector(), int(d["seeds"][i])))
-    y0 = np.array(y0)
+    return m, d, ic, np.array(y0)
+
+
+def _group(eng, m, d, y0, idx, prec):
+    from unicycle_amd.ensemble import Ensemble
+    ens = Ensemble(eng, prec)
+    ens.add_group(m, d["a"][idx], d["b"][idx], d["sig"][idx], d["L"][idx], y0[idx])
+    ens.build()
+    return ens
+
+
+def test_ensemble_step_log(engine_factory):
+    """The ensemble's per-step log: times are the running sums of the
+    accepted steps, rejections add up to attempts - steps, the first steps
+    of a binary64 instance follow the plain integrator's log (same RKQSm
+    arithmetic, ode45.f90:51-108), and the log's features match the plain
+    log's feature definitions."""
+    from unicycle_amd import PREC_F64, configs
+    m, d, ic, y0 = _instances(engine_factory, 8, 8, 3)
+    eng = engine_factory()
+    ens = _group(eng, m, d, y0, list(range(4)), PREC_F64)
+    ens.set_step_log(300, 0)
+    ens.reset()
+    ens.run(1e30, max_attempts=400)
+    logs = ens.step_log(0)
+    y, t, dt, steps, att = ens.state(0)
+    ic.set_friction(*configs.instance_friction(m, d, 0))
+    ic.buildG()
+    r = ic.integrate(y0[0], 1e30, max_steps=40, log_cap=40, history=True)
+    for i, lg in enumerate(logs):
+        n = len(lg["dt"])
+        assert n == min(steps[i], 300) and n > 50
+        assert np.allclose(np.cumsum(lg["dt"]), lg["t"], rtol=1e-12, atol=0)
+        if steps[i] <= 300:
+            assert lg["rejects"].sum() == att[i] - steps[i]
+    # the error norm is a difference of stage rates that are themselves small
+    # differences (K (v - Vpl) near steady state), so the GEMM's and the
+    # GEMV's summation orders show in it at ~1e-5 after ten steps
+    k = 20
+    assert np.array_equal(logs[0]["dt"][:9], r.dt_log[:9])     # the x10 growth phase
+    assert np.allclose(logs[0]["dt"][:k], r.dt_log[:k], rtol=1e-4)
+    assert np.allclose(logs[0]["vmax"][:k], r.vmax_log[:k], rtol=1e-6)
+    assert np.allclose(logs[0]["errmax"][:k], r.errmax_log[:k], rtol=1e-3)
+    ens.close()
+
+
+def test_learned_proposal_gain_on_held_out_instances(engine_factory):
+    """Train on the plain binary64 integrator's step logs of 16 seeded
+    instances of a 256-patch fault, refine on the histories of the same
+    instances run under the network (two closed-loop rounds), pick the
+    output offset on a window of those instances, then run 48 held-out
+    instances as an ensemble over a fixed simulated window (every instance
+    clamped to land on it) with the classic RKQSm growth rule, with the
+    network trained on the plain logs only, and with the refined network.
+
+    Measured gain (what this test records): fewer RK45 attempts (RHS
+    evaluations) for the same simulated window; first-event times of the
+    instances with an event agree with the classic run to < 0.1 %."""
+    from unicycle_amd import PREC_F64, configs
+    m, d, ic, y0 = _instances(engine_factory, 16, 16, 7)
+    eng = engine_factory()
     X, T = stepnet.training_data(ic, m, d, range(16), 3000, y0)
     net = stepnet.StepNet(eng, seed=0)
     net.fit_normalization(X, T)
@@ -99,27 +149,36 @@ def test_learned_proposal_gain_on_held_out_instances(engine_factory):
     rmse = float(np.sqrt(np.mean((net.forward(X) - T) ** 2)))
     rmse_classic = float(np.sqrt(np.mean((X[:, 6] - T) ** 2)))
     held = list(range(16, 64))
+    W = 0.5 * configs.YEAR
+
     out = {}
-    for use in (False, True):
-        ens = Ensemble(eng, PREC_F64)
-        ens.add_group(m, d["a"][held], d["b"][held], d["sig"][held], d["L"][held], y0[held])
-        ens.build()
-        if use:
+    for name in ("classic", "open_loop", "closed_loop"):
+        if name == "closed_loop":
+            train = _group(eng, m, d, y0, list(range(16)), PREC_F64)
+            X, T = stepnet.fit_closed_loop(net, X, T, train, 3000, rounds=2)
+            off, tried = stepnet.calibrate_offset(net, train, 0.25 * configs.YEAR)
+            train.close()
+        ens = _group(eng, m, d, y0, held, PREC_F64)
+        if name != "classic":
             net.attach(ens)
         ens.set_stop_at_end(True)
-        st = ens.run(0.5 * configs.YEAR, max_attempts=10**6, chunk_attempts=5000)
-        assert st.n_failed == 0, f"learned={use}: status {ens.status(0)}"
-        assert st.min_time >= 0.5 * configs.YEAR
+        st = ens.run(W, max_attempts=10**6, chunk_attempts=5000)
+        assert st.n_failed == 0, f"{name}: status {ens.status(0)}"
+        assert st.min_time >= W
         y, t, dt, s, a = ens.state(0)
-        out[use] = (a.sum(), s.sum(), ens.first_events(0))
+        out[name] = (a.sum(), s.sum(), ens.first_events(0))
         ens.close()
-    (a0, s0, e0), (a1, s1, e1) = out[False], out[True]
-    both = (e0 > 0) & (e1 > 0)
-    rel = np.abs(e1[both] - e0[both]) / e0[both]
-    print(f"learned proposal: rmse {rmse:.3f} (classic rule {rmse_classic:.3f}) in log10 h; "
-          f"attempts {a1} vs {a0} (gain {a0 / a1:.3f}); rejected {1 - s1 / a1:.3f} vs "
-          f"{1 - s0 / a0:.3f}; events {both.sum()}, max rel time diff {rel.max():.2e}")
+    a0, s0, e0 = out["classic"]
+    for name in ("open_loop", "closed_loop"):
+        a1, s1, e1 = out[name]
+        both = (e0 > 0) & (e1 > 0)
+        rel = np.abs(e1[both] - e0[both]) / e0[both]
+        print(f"{name}: attempts {a1} vs {a0} (gain {a0 / a1:.3f}); rejected "
+              f"{1 - s1 / a1:.3f} vs {1 - s0 / a0:.3f}; events {both.sum()}, max rel time "
+              f"diff {rel.max():.2e}")
+        assert ((e0 > 0) ^ (e1 > 0)).sum() == 0 and rel.max() < 1e-3
+    print(f"rmse {rmse:.3f} (classic rule {rmse_classic:.3f}) in log10 h; offset {off} "
+          f"from {tried}")
     assert rmse < rmse_classic
-    assert a1 < a0                       # fewer RHS evaluations for the same window
-    assert ((e0 > 0) ^ (e1 > 0)).sum() == 0 and rel.max() < 1e-3
+    assert out["closed_loop"][0] < a0     # fewer RHS evaluations for the same window
     net.close()

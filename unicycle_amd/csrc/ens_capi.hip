@@ -42,6 +42,7 @@ struct ucy_ensemble {
     int stop_at_event = 0;
     double event_vmax = 1e-3;
     float *d_mlp = nullptr;  // copy of the proposal network's parameters
+    std::vector<double *> slogs;  // per-group step logs (null: off)
     MlpNorm mlp_norm{};
     bool use_mlp = false;
     std::vector<double> tune_seconds;
@@ -97,6 +98,9 @@ static void release(ucy_ensemble *E)
     E->graph = nullptr;
     for (void *p : E->allocs) (void)hipFree(p);
     E->allocs.clear();
+    for (double *p : E->slogs)
+        if (p) (void)hipFree(p);
+    E->slogs.clear();
     E->d_tab = nullptr;
     E->built = false;
 }
@@ -171,7 +175,7 @@ static int load_initial(ucy_ensemble *E, double dt0)
         EHIP(hipMemcpyAsync(G.vmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         std::vector<double> none(kEnsCols, -1.0);
         EHIP(hipMemcpyAsync(G.t_event, none.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
-        std::vector<double> hz(4 * kEnsCols, 0.0);
+        std::vector<double> hz(5 * kEnsCols, 0.0);
         EHIP(hipMemcpyAsync(G.hist, hz.data(), sizeof(double) * hz.size(), hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.errmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.need_rhs, one.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
@@ -339,7 +343,9 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         int **ci[] = {&G.need_rhs, &G.accepted, &G.done, &G.status};
         for (int **v : ci)
             if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
-        if (ealloc(E, &G.hist, 4 * kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.hist, 5 * kEnsCols)) return UCY_ERR_HIP;
+        G.slog = nullptr;
+        G.slog_cap = 0;
         if (ealloc(E, &G.steps, kEnsCols)) return UCY_ERR_HIP;
         if (ealloc(E, &G.attempts, kEnsCols)) return UCY_ERR_HIP;
         T.blk_elem[gi] = eacc;
@@ -551,6 +557,58 @@ extern "C" int ucy_ensemble_first_events(ucy_ensemble *E, int group, double *t_e
     std::vector<double> tmp(kEnsCols);
     EHIP(hipMemcpy(tmp.data(), E->tab.g[group].t_event, sizeof(double) * kEnsCols, hipMemcpyDeviceToHost));
     std::copy(tmp.begin(), tmp.begin() + E->hg[group].ninst, t_event);
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_set_step_log(ucy_ensemble *E, int group, long long cap)
+{
+    if (!E || group < 0 || group >= (int)E->hg.size() || cap < 0) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    if (E->slogs.size() < E->hg.size()) E->slogs.resize(E->hg.size(), nullptr);
+    EnsGroup &G = E->tab.g[group];
+    if (E->slogs[group]) EHIP(hipFree(E->slogs[group]));
+    E->slogs[group] = nullptr;
+    G.slog = nullptr;
+    G.slog_cap = 0;
+    if (cap > 0) {
+        double *p = nullptr;
+        const size_t bytes = sizeof(double) * 5 * kEnsCols * (size_t)cap;
+        if (hipMalloc((void **)&p, bytes) != hipSuccess)
+            return efail(E, UCY_ERR_HIP, "step log of %lld steps does not fit", cap);
+        E->slogs[group] = p;
+        G.slog = p;
+        G.slog_cap = cap;
+    }
+    return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_get_step_log(ucy_ensemble *E, int group, double *t, double *dt,
+                                         double *vmax, double *errmax, double *rejects,
+                                         long long *count)
+{
+    if (!E || group < 0 || group >= (int)E->hg.size() || !count) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    const EnsGroup &G = E->tab.g[group];
+    if (!G.slog) return efail(E, UCY_ERR_STATE, "no step log on group %d", group);
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    const int ni = E->hg[group].ninst;
+    std::vector<long long> steps(kEnsCols);
+    EHIP(hipMemcpy(steps.data(), G.steps, sizeof(long long) * kEnsCols, hipMemcpyDeviceToHost));
+    const size_t cap = (size_t)G.slog_cap;
+    std::vector<double> buf(5 * kEnsCols * cap);
+    EHIP(hipMemcpy(buf.data(), G.slog, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+    double *out[5] = {t, dt, vmax, errmax, rejects};
+    for (int i = 0; i < ni; i++) {
+        count[i] = std::min<long long>(steps[i], (long long)cap);
+        for (int f = 0; f < 5; f++) {
+            if (!out[f]) continue;
+            for (size_t k = 0; k < (size_t)count[i]; k++)
+                out[f][(size_t)i * cap + k] = buf[(k * 5 + f) * kEnsCols + i];
+        }
+    }
     return UCY_OK;
 }
 

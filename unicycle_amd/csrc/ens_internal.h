@@ -53,8 +53,12 @@ struct EnsGroup {
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;
     double *t_event;         // time of the first step starting with vMax > event_vmax (-1: none)
-    double *hist;            // [4][64] proposal history: log10 vMax, h, log10 errmax of the
-                             // last accepted step (h = 0: none yet), rejections of this step
+    double *hist;            // [5][64] proposal history: log10 vMax, h, log10 errmax of the
+                             // last accepted step (h = 0: none yet), rejections of this step,
+                             // consecutive attempts with a non-finite error norm
+    double *slog;            // optional per-step log [cap][5][64]: time after the step, h,
+                             // step-start vMax, errmax / epsilon, rejections (null: off)
+    long long slog_cap;
     int *need_rhs, *accepted, *done, *status;
     long long *steps, *attempts;
 };

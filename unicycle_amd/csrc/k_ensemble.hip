@@ -557,15 +557,28 @@ __global__ void ens_control_kernel(EnsTable *tab)
     G.attempts[inst] += 1;
     const double raw = G.errmax[inst];
     G.accepted[inst] = 0;
-    if (!(raw == raw) || raw > 1.7976931348623157e308) {
-        G.status[inst] = 2;
-        G.done[inst] = 1;
-        atomicAdd(tab->all_done, 1);
-        return;
-    }
-    const double errmax = raw / tab->epsilon;
     double h = G.h[inst];
     double *hist = G.hist;
+    if (!(raw == raw) || raw > 1.7976931348623157e308) {
+        // an attempt whose error norm is not finite (an overflow inside the
+        // stages) is rejected with the largest shrink of the rule, x0.1
+        // (the reference would accept it: NaN > 1 is false); the fourth such
+        // attempt in a row fails the instance
+        const double nf = hist[4 * kEnsCols + inst] + 1.0;
+        hist[4 * kEnsCols + inst] = nf;
+        hist[3 * kEnsCols + inst] += 1.0;
+        h *= 0.1;
+        G.h[inst] = h;
+        G.need_rhs[inst] = 0;
+        if (nf >= 4.0 || 0.0 + h == 0.0) {
+            G.status[inst] = 2;
+            G.done[inst] = 1;
+            atomicAdd(tab->all_done, 1);
+        }
+        return;
+    }
+    hist[4 * kEnsCols + inst] = 0.0;
+    const double errmax = raw / tab->epsilon;
     if (errmax > 1.0) {
         h = h * SAFETY * pow(errmax, PSHRNK);
         G.h[inst] = h;
@@ -598,6 +611,14 @@ __global__ void ens_control_kernel(EnsTable *tab)
             double hp = pow(10.0, lp);
             hp = fmin(fmax(hp, 0.25 * hnext), fmin(4.0 * hnext, 10.0 * h));
             hnext = (hp == hp) ? fmin(hp, tab->max_step) : hnext;
+        }
+        if (G.slog && G.steps[inst] < G.slog_cap) {
+            double *r = G.slog + (size_t)G.steps[inst] * 5 * kEnsCols + inst;
+            r[0] = G.time[inst] + h;
+            r[kEnsCols] = h;
+            r[2 * kEnsCols] = G.vmax[inst];
+            r[3 * kEnsCols] = errmax;
+            r[4 * kEnsCols] = hist[3 * kEnsCols + inst];
         }
         hist[inst] = log10(fmax(G.vmax[inst], 1e-30));
         hist[1 * kEnsCols + inst] = h;

@@ -44,6 +44,7 @@ class Ensemble:
         if not self._h:
             raise UcyError(-1, "cannot create ensemble")
         self.groups = []
+        self._log_cap = 0
         engine._ensembles.append(self)
 
     def close(self):
@@ -138,6 +139,26 @@ class Ensemble:
         self._chk(self._L.ucy_ensemble_status(self._h, group,
                                               st.ctypes.data_as(C.POINTER(C.c_int))))
         return st
+
+    def set_step_log(self, cap: int, group: int | None = None):
+        """Log up to cap accepted steps per instance from the last reset
+        (cap = 0: off), on one group or all."""
+        for g in ([group] if group is not None else range(len(self.groups))):
+            self._chk(self._L.ucy_ensemble_set_step_log(self._h, g, int(cap)))
+        self._log_cap = int(cap)
+
+    def step_log(self, group: int) -> list[dict]:
+        """Per instance: arrays t (after the step), dt, vmax (step start),
+        errmax (/ epsilon) and rejects of the logged steps."""
+        _, n_inst = self.groups[group]
+        cap = self._log_cap
+        arrs = [np.zeros((n_inst, cap)) for _ in range(5)]
+        cnt = np.zeros(n_inst, dtype=np.int64)
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        self._chk(self._L.ucy_ensemble_get_step_log(
+            self._h, group, *[P(a) for a in arrs], cnt.ctypes.data_as(C.POINTER(C.c_longlong))))
+        names = ("t", "dt", "vmax", "errmax", "rejects")
+        return [{k: a[i, :cnt[i]].copy() for k, a in zip(names, arrs)} for i in range(n_inst)]
 
     def state(self, group: int):
         N, n_inst = self.groups[group]

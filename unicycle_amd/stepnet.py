@@ -14,6 +14,13 @@ the network's prediction error (~0.08 in log10, a factor 1.2, i.e. 2.5 in
 errmax).  The controller uses the proposal only where the classic rule's
 error ratio exceeds ERRCON (ode45.f90:92-96); below it the classic x10
 growth is already the largest allowed.
+
+Trained on the classic controller's histories alone, the network meets
+states of its own making that those histories never show (a step the classic
+rule would not have taken); fit_closed_loop adds the histories of runs under
+the network's own proposals, labelled the same way (DAgger, Ross, Gordon and
+Bagnell 2011), and calibrate_offset picks the output offset that needs the
+fewest attempts over a window of the training instances.
 """
 from __future__ import annotations
 
@@ -164,3 +171,60 @@ def training_data(engine: Engine, model, draws, instances, steps, y0s):
         Xs.append(X)
         Ts.append(T)
     return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def features_from_logs(logs):
+    """Features and targets of several instances' step logs (dicts with
+    dt, vmax, errmax, rejects, as Ensemble.step_log returns)."""
+    Xs, Ts = [], []
+    for r in logs:
+        X, T = features_from_history(r["dt"], r["vmax"], r["errmax"], r["rejects"])
+        Xs.append(X)
+        Ts.append(T)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def closed_loop_data(net, ens, steps, group=0, progress=None):
+    """Step histories of an ensemble group over `steps` accepted steps per
+    instance from its initial states, with the network's proposals."""
+    ens.set_step_log(steps, group)
+    ens.reset()
+    net.attach(ens)
+    ens.run(1e30, max_attempts=int(2 * steps), chunk_attempts=2000, progress=progress)
+    logs = ens.step_log(group)
+    ens.set_step_log(0, group)
+    return features_from_logs(logs)
+
+
+def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1e-3,
+                    seed=1, group=0, progress=None):
+    """Rounds of: run the training instances (ensemble `ens`, binary64)
+    under the current network, add their histories to (X, T), train on the
+    union.  Returns the grown data set."""
+    for r in range(rounds):
+        Xr, Tr = closed_loop_data(net, ens, steps, group, progress)
+        X, T = np.concatenate([X, Xr]), np.concatenate([T, Tr])
+        net.train(X, T, epochs=epochs, batch=batch, lr=lr, seed=seed + r + 1)
+    return X, T
+
+
+def calibrate_offset(net, ens, window, offsets=(-0.15, -0.1, -0.05, 0.0, 0.05), group=0,
+                     progress=None):
+    """Shift of the network's output (log10 h) with the fewest attempts for
+    the instances of `ens` to reach t = window (last step clamped to land
+    on it); an offset that fails an instance is not eligible.  Leaves the
+    chosen shift in the network's normalisation and returns
+    (offset, {offset: attempts})."""
+    xm, xs, ym, ys = net.norm
+    res = {}
+    for d in offsets:
+        net.set_normalization(xm, xs, ym + d, ys)
+        ens.reset()
+        net.attach(ens)
+        ens.set_stop_at_end(True)
+        st = ens.run(window, max_attempts=10**7, chunk_attempts=5000, progress=progress)
+        ok = st.n_failed == 0 and st.min_time >= window
+        res[d] = float(st.attempts) if ok else float("inf")
+    best = min(res, key=res.get)
+    net.set_normalization(xm, xs, ym + best, ys)
+    return best, res
