@@ -164,7 +164,8 @@ def bench_ensemble(args):
     # dominant kernel: the stage GEMM over all groups, HIP events on its stream
     t_gemm, gemm_bytes = ens.time_gemm(20)
     flops = sum(2.0 * m * n * 64 for m, n, _ in shapes)
-    kb, _ = ens.tuning()
+    kb, _, _ = ens.tuning()
+    tiling = {"f64": {"kb_per_split": kb}}
     # precision references: slip at a fixed pre-event time, first-event times
     Tslip = args.slip_years * configs.YEAR
     Tevt = args.event_years * configs.YEAR
@@ -181,6 +182,9 @@ def bench_ensemble(args):
         str_ = run_attempts(ensr, R, name)
         wall_r = time.perf_counter() - t0
         t_g, b_g = ensr.time_gemm(20)
+        kb_r, var_r, tt = ensr.tuning()
+        tiling[name] = {"kb_per_split": kb_r, "variant": var_r,
+                        "us_per_launch": [round(1e6 * x, 1) for x in tt]}
         _, ysr = slip_at(ensr, Tslip, f"{name} slip window")
         ter = first_events(ensr, Tevt, f"{name} first events")
         ensr.close()
@@ -224,7 +228,7 @@ def bench_ensemble(args):
             "dtype": "f64", "desc": "binary64 K on MFMA"}
     roof = {"bound": "mfma", "achieved": flops / t_gemm / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
             "unit": "TFLOP/s", "frac": flops / t_gemm / FP64_MFMA_PEAK, "traffic": None,
-            "kernel": "ucy::gemm_kernel (binary64 MFMA stage GEMM, all groups)",
+            "kernel": "ucy::gemm_f64_kernel (binary64 MFMA stage GEMM, all groups)",
             "avg_launch_us": 1e6 * t_gemm, "flops_per_launch": flops,
             "bytes_per_launch": gemm_bytes, "hbm_GBps": gemm_bytes / t_gemm / 1e9}
     for name in ("mixed", "split_bf16"):
@@ -242,7 +246,7 @@ def bench_ensemble(args):
             roof = {"bound": "hbm", "achieved": gb, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": gb * 1e9 / HBM_PEAK,
                     "traffic": MEASURED_TRAFFIC["split_bf16_stage_gemm"],
-                    "kernel": "ucy::gemm_kernel (split-bf16 MFMA stage GEMM, all groups)",
+                    "kernel": "ucy::gemm_bf16_kernel<variant> (split-bf16 MFMA stage GEMM, all groups)",
                     "avg_launch_us": r["stage_gemm_us"],
                     "bytes_per_launch": r["stage_gemm_GBps"] * 1e9 * r["stage_gemm_us"] * 1e-6,
                     "useful_TFLOPs": r["stage_gemm_useful_TFLOPs"]}
@@ -266,7 +270,7 @@ def bench_ensemble(args):
                                f"{head['desc']}, eps 1e-6",
                    "headline_variant": head["name"],
                    "instances": n_inst, "groups": [[m, n] for m, n, _ in shapes],
-                   "attempts_per_instance": R, "gemm_kb_per_split": kb},
+                   "attempts_per_instance": R, "gemm_tiling": tiling},
         "vs_plain_fp64": head["value"] / value_plain,
         "batched_f64": {"value": value, "unit": "sim-yr/s", "vs_plain_fp64": value / value_plain,
                         "ms_per_step": 1e3 * t_total / args.steps},
@@ -458,10 +462,13 @@ def bench_gemm(args):
     eng = Engine(0)
     ens, info = configs.ensemble_setup(eng, prec)
     t, b = ens.time_gemm(20)
+    kb, var, tt = ens.tuning()
     flops = sum(2.0 * 3 * m.n * 2 * m.n * 64 for m, d, y0 in info)
     ens.close()
     return {"workload": f"stage GEMM {args.precision}", "avg_launch_us": 1e6 * t,
-            "bytes_per_launch": b, "GBps": b / t / 1e9, "useful_TFLOPs": flops / t / 1e12}
+            "bytes_per_launch": b, "GBps": b / t / 1e9, "useful_TFLOPs": flops / t / 1e12,
+            "tuning": {"kb_per_split": kb, "variant": var,
+                       "us_per_launch": [round(1e6 * x, 1) for x in tt]}}
 
 
 def bench_rhs(args):

@@ -274,8 +274,13 @@ int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time
 /* one stage matvec of a group in the ensemble precision: V is n x n_inst
  * (row-major, column = instance), T is m x n_inst */
 int ucy_ensemble_matvec(ucy_ensemble *ens, int group, const double *V, double *T);
-/* chosen split-K k-blocks per wave and the tuning table (may be NULL) */
-int ucy_ensemble_tuning(const ucy_ensemble *ens, int *kb_per_split, double *gemm_seconds);
+/* the autotuner's choice for the stage GEMM: k-blocks (of 16) per split and
+ * the split-bf16 kernel variant (0: 64 K rows per wave; 1: 64 rows with the
+ * next k-block's loads issued ahead; 2: 128 rows, ahead), and its table of
+ * seconds per launch (15 entries: k-blocks 32, 64, 128, 256, 512 of variant
+ * 0, then of 1, then of 2; -1 = not admissible; pointers may be NULL) */
+int ucy_ensemble_tuning(const ucy_ensemble *ens, int *kb_per_split, int *variant,
+                        double *gemm_seconds);
 /* HIP-event timing of the all-groups stage GEMM (seconds per launch) and its
  * algorithmic HBM bytes per launch */
 int ucy_ensemble_time_gemm(ucy_ensemble *ens, int iters, double *avg_seconds, double *bytes);

@@ -125,7 +125,7 @@ _SIGS = {
     "ucy_ensemble_set_step_log": (_i, [_vp, _i, C.c_longlong]),
     "ucy_ensemble_get_step_log": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, C.POINTER(C.c_longlong)]),
     "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
-    "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), _pd]),
+    "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), _pd]),
     "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),
 }
 

@@ -38,6 +38,7 @@ struct ucy_ensemble {
     hipGraphExec_t graph = nullptr;
     bool built = false;
     int kb_per_split = 0;
+    int variant = 0;  // split-bf16 GEMM variant chosen by the autotuner
     int stop_at_end = 0;
     int stop_at_event = 0;
     double event_vmax = 1e-3;
@@ -191,22 +192,26 @@ static int load_initial(ucy_ensemble *E, double dt0)
     return 0;
 }
 
-static void set_split(EnsTable &T, int kbps)
+static void set_split(EnsTable &T, int kbps, int variant)
 {
+    T.variant = variant;
+    const int rb = ens_gemm_variant_rows(variant) / 32;
     for (int gi = 0; gi < T.ngroups; gi++) {
         EnsGroup &G = T.g[gi];
         int k = std::max(1, std::min(kbps, G.nkb));
         G.ksplit = (G.nkb + k - 1) / k;
         G.kb_per_split = (G.nkb + G.ksplit - 1) / G.ksplit;
     }
-    int acc = 0;
+    int acc = 0, acc64 = 0;
     for (int gi = 0; gi < T.ngroups; gi++) {
         const EnsGroup &G = T.g[gi];
         T.blk_gemm[gi] = acc;
-        const int items = (G.mp / 64) * G.ksplit;
-        acc += (items + 3) / 4;
+        T.blk_gemm64[gi] = acc64;
+        acc += (((G.mp / 32 + rb - 1) / rb) * G.ksplit + 3) / 4;
+        acc64 += ((G.mp / 64) * G.ksplit + 3) / 4;
     }
     T.blk_gemm[T.ngroups] = acc;
+    T.blk_gemm64[T.ngroups] = acc64;
 }
 
 static int upload_table(ucy_ensemble *E)
@@ -368,15 +373,22 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     E->gemm_bytes = bytes;
     // ---- tune the split-K tiling on the device: time the all-groups GEMM
     E->tune_seconds.clear();
-    int best = 0;
+    int best = 0, best_v = 0;
     double best_t = 1e30;
     const int cands[] = {32, 64, 128, 256, 512};
     {
         hipEvent_t e0, e1;
         EHIP(hipEventCreate(&e0));
         EHIP(hipEventCreate(&e1));
-        for (int kb : cands) {
-            set_split(T, kb);
+        // (split-bf16 variant, k-blocks per split); the binary64 GEMM has one
+        // variant
+        for (int ci = 0; ci < kGemmVariants * 5; ci++) {
+            const int kb = cands[ci % 5], v = ci / 5;
+            if (v > 0 && E->mode == 1) {
+                E->tune_seconds.push_back(-1.0);
+                continue;
+            }
+            set_split(T, kb, v);
             bool ok = true;
             for (int gi = 0; gi < T.ngroups; gi++)
                 if (T.g[gi].ksplit > kMaxSplit) ok = false;
@@ -397,6 +409,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             if (t < best_t) {
                 best_t = t;
                 best = kb;
+                best_v = v;
             }
         }
         (void)hipEventDestroy(e0);
@@ -404,7 +417,8 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         if (!best) return efail(E, UCY_ERR_STATE, "no admissible GEMM tiling");
     }
     E->kb_per_split = best;
-    set_split(T, best ? best : 64);
+    E->variant = best_v;
+    set_split(T, best, best_v);
     if (upload_table(E)) return UCY_ERR_HIP;
     E->built = true;
     return load_initial(E, 1.0e-3);
@@ -662,10 +676,12 @@ extern "C" int ucy_ensemble_get_state(ucy_ensemble *E, int group, double *y, dou
     return UCY_OK;
 }
 
-extern "C" int ucy_ensemble_tuning(const ucy_ensemble *E, int *kb_per_split, double *gemm_seconds)
+extern "C" int ucy_ensemble_tuning(const ucy_ensemble *E, int *kb_per_split, int *variant,
+                                   double *gemm_seconds)
 {
     if (!E) return UCY_ERR_ARG;
     if (kb_per_split) *kb_per_split = E->kb_per_split;
+    if (variant) *variant = E->variant;
     if (gemm_seconds)
         for (size_t i = 0; i < E->tune_seconds.size(); i++) gemm_seconds[i] = E->tune_seconds[i];
     return UCY_OK;

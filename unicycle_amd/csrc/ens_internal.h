@@ -66,7 +66,9 @@ struct EnsGroup {
 struct EnsTable {
     int ngroups;
     EnsGroup g[kMaxGroups];
-    int blk_gemm[kMaxGroups + 1];   // prefix sums of GEMM blocks per group
+    int blk_gemm[kMaxGroups + 1];   // prefix sums of split-bf16 GEMM blocks per group
+    int blk_gemm64[kMaxGroups + 1]; // prefix sums of binary64 GEMM blocks per group
+    int variant;                    // split-bf16 GEMM variant (rows per wave, pipelining)
     int blk_elem[kMaxGroups + 1];   // prefix sums of elementwise blocks
     int blk_inst[kMaxGroups + 1];   // prefix sums of per-instance blocks
     double t_end, epsilon, max_step;
@@ -79,6 +81,11 @@ struct EnsTable {
     int *all_done;                  // count of finished instances
     int total_inst;
 };
+
+// split-bf16 stage-GEMM variants of the autotuner: 0 = 64 rows per wave,
+// 1 = 64 rows with the next k-block's loads issued ahead, 2 = 128 rows, ahead
+constexpr int kGemmVariants = 3;
+int ens_gemm_variant_rows(int variant);
 
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,

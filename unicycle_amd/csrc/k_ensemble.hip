@@ -156,62 +156,106 @@ __device__ __forceinline__ int group_of(const int *prefix, int ng, int blk)
 }
 
 // ---------------------------------------------------------------------------
-// split-bf16 GEMM: one wavefront = 64 rows x 64 columns over one k-split.
+// split-bf16 GEMM: one wavefront = RB 32-row blocks x 64 columns over one
+// k-split.  RB = 4 halves the V-tile reads per K byte against RB = 2 at
+// twice the accumulators; the autotuner picks.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
+template <int RB, bool PIPE>
+__device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int sp, int lane)
 {
-    const int nrp = G.mp >> 6;               // row pairs of 32-row blocks
-    const int rp = item % nrp, sp = item / nrp;
     const int kb0 = sp * G.kb_per_split;
     int kb1 = kb0 + G.kb_per_split;
     if (kb1 > G.nkb) kb1 = G.nkb;
-    floatx16 acc[2][2];
+    floatx16 acc[RB][2];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RB; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
-    const u32x4 *A0 = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(2 * rp) * G.nkb * 2) * 64 + lane;
-    const u32x4 *A1 = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(2 * rp + 1) * G.nkb * 2) * 64 + lane;
+    // tile (rb, kb): hi then lo, 64 lanes x 16 bytes each.  Pipelining is
+    // by hand: loops holding MFMA builtins are not runtime-unrolled by the
+    // compiler.
+    const u32x4 *A = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)rb0 * G.nkb * 2) * 64 + lane;
+    const size_t rstride = (size_t)G.nkb * 128;
     const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
-#pragma unroll 2
-    for (int kb = kb0; kb < kb1; kb++) {
-        const u32x4 a0h = __builtin_nontemporal_load(A0 + (size_t)kb * 128);
-        const u32x4 a0l = __builtin_nontemporal_load(A0 + (size_t)kb * 128 + 64);
-        const u32x4 a1h = __builtin_nontemporal_load(A1 + (size_t)kb * 128);
-        const u32x4 a1l = __builtin_nontemporal_load(A1 + (size_t)kb * 128 + 64);
-        const u32x4 b0h = Bv[(size_t)kb * 256];
-        const u32x4 b0l = Bv[(size_t)kb * 256 + 64];
-        const u32x4 b1h = Bv[(size_t)kb * 256 + 128];
-        const u32x4 b1l = Bv[(size_t)kb * 256 + 192];
-        const bf16x8 A[2][2] = {{__builtin_bit_cast(bf16x8, a0h), __builtin_bit_cast(bf16x8, a0l)},
-                                {__builtin_bit_cast(bf16x8, a1h), __builtin_bit_cast(bf16x8, a1l)}};
-        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, b0h), __builtin_bit_cast(bf16x8, b0l)},
-                                {__builtin_bit_cast(bf16x8, b1h), __builtin_bit_cast(bf16x8, b1l)}};
+    u32x4 ca[RB][2], cb[4], na[RB][2], nb[4];
+    auto load = [&](int kb, u32x4 (&ra)[RB][2], u32x4 (&rbv)[4]) {
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int a = 0; a < RB; a++) {
+            ra[a][0] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128);
+            ra[a][1] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128 + 64);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) rbv[q] = Bv[(size_t)kb * 256 + 64 * q];
+    };
+    if (kb0 < kb1) load(kb0, ca, cb);
+    for (int kb = kb0; kb < kb1; kb++) {
+        // PIPE: the next k-block's loads go out before this one's MFMAs
+        // (more bytes in flight per wave, more registers: fewer waves)
+        if (!PIPE && kb > kb0) load(kb, ca, cb);
+        if (PIPE && kb + 1 < kb1) load(kb + 1, na, nb);
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, cb[0]), __builtin_bit_cast(bf16x8, cb[1])},
+                                {__builtin_bit_cast(bf16x8, cb[2]), __builtin_bit_cast(bf16x8, cb[3])}};
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, ca[a][0]);
+            const bf16x8 Al = __builtin_bit_cast(bf16x8, ca[a][1]);
 #pragma unroll
             for (int b = 0; b < 2; b++) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][1], B[b][0], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][0], B[b][1], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[a][0], B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
             }
+        }
+        if (PIPE) {
+#pragma unroll
+            for (int a = 0; a < RB; a++) {
+                ca[a][0] = na[a][0];
+                ca[a][1] = na[a][1];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) cb[q] = nb[q];
+        }
     }
     // 32x32 fp32 accumulator layout: value i of lane l is row 8*(i/4) + 4*(l/32) + i%4,
     // column l%32 of the tile
     float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < RB; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const int row = 64 * rp + 32 * a + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
                 const int col = 32 * b + (lane & 31);
                 T[(size_t)row * kEnsCols + col] = acc[a][b][i];
             }
 }
+
+// row items of RB 32-row blocks: mp is a multiple of 64, so the last item of
+// RB = 4 may hold two blocks
+__host__ __device__ __forceinline__ int bf16_row_items(int mp, int rb)
+{
+    return ((mp >> 5) + rb - 1) / rb;
+}
+
+template <int RB, bool PIPE>
+__device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
+{
+    const int nrb = G.mp >> 5;
+    const int nri = bf16_row_items(G.mp, RB);
+    const int rb0 = (item % nri) * RB, sp = item / nri;
+    if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE>(G, rb0, sp, lane);
+    else gemm_tile_bf16<2, PIPE>(G, rb0, sp, lane);
+}
+
+// split-bf16 GEMM variants of the autotuner: K rows per wave, pipelined loads
+struct Bf16Variant {
+    int rb;
+    bool pipe;
+};
+constexpr Bf16Variant kBf16Variants[kGemmVariants] = {{2, false}, {2, true}, {4, true}};
 
 // binary64 batched matvec on v_mfma_f64_16x16x4f64: one wavefront = 64 rows
 // x 64 columns over one k-split (4 x 4 tiles of 16 x 16)
@@ -258,50 +302,86 @@ __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int l
             }
 }
 
-__device__ __forceinline__ int gemm_items(const EnsGroup &G)
-{
-    return (G.mp >> 6) * G.ksplit;
-}
-
 // binary64 path: mode 1 always, mode 2 for the step-start matvec
-__device__ __forceinline__ bool gemm_f64_path(const EnsGroup &G, int stage)
+__host__ __device__ __forceinline__ bool gemm_f64_path(int mode, int stage)
 {
-    return G.mode == 1 || (G.mode == 2 && stage == 1);
+    return mode == 1 || (mode == 2 && stage == 1);
 }
 
-__global__ void __launch_bounds__(256) gemm_kernel(const EnsTable *__restrict__ tab, int stage)
+// stage 1 = the step-start matvec (skipped when every instance of the group
+// retries: t_0 stands), 2..6 the stage matvecs, 0 a standalone matvec
+template <int V>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restrict__ tab, int stage)
 {
+    constexpr int RB = kBf16Variants[V].rb;
     const int blk = blockIdx.x;
     const int g = group_of(tab->blk_gemm, tab->ngroups, blk);
     const EnsGroup &G = tab->g[g];
     const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
-    if (item >= gemm_items(G)) return;
-    if (stage == 1 && !*G.any_rhs) return;   // every instance retries: t_0 stands
-    const int lane = threadIdx.x & 63;
-    if (gemm_f64_path(G, stage)) gemm_wave_f64(G, item, lane);
-    else gemm_wave_bf16(G, item, lane);
+    if (item >= bf16_row_items(G.mp, RB) * G.ksplit) return;
+    if (stage == 1 && !*G.any_rhs) return;
+    gemm_wave_bf16<RB, kBf16Variants[V].pipe>(G, item, threadIdx.x & 63);
+}
+
+int ens_gemm_variant_rows(int v)
+{
+    return 32 * kBf16Variants[v].rb;
+}
+
+__global__ void __launch_bounds__(256) gemm_f64_kernel(const EnsTable *__restrict__ tab, int stage)
+{
+    const int blk = blockIdx.x;
+    const int g = group_of(tab->blk_gemm64, tab->ngroups, blk);
+    const EnsGroup &G = tab->g[g];
+    const int item = (blk - tab->blk_gemm64[g]) * 4 + (threadIdx.x >> 6);
+    if (item >= (G.mp >> 6) * G.ksplit) return;
+    if (stage == 1 && !*G.any_rhs) return;
+    gemm_wave_f64(G, item, threadIdx.x & 63);
 }
 
 void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
 {
-    hipLaunchKernelGGL(gemm_kernel, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0, st, dtab, stage);
+    if (gemm_f64_path(htab.g[0].mode, stage))
+        hipLaunchKernelGGL(gemm_f64_kernel, dim3(htab.blk_gemm64[htab.ngroups]), dim3(256), 0, st,
+                           dtab, stage);
+    else if (htab.variant == 0)
+        hipLaunchKernelGGL(gemm_bf16_kernel<0>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
+                           st, dtab, stage);
+    else if (htab.variant == 1)
+        hipLaunchKernelGGL(gemm_bf16_kernel<1>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
+                           st, dtab, stage);
+    else
+        hipLaunchKernelGGL(gemm_bf16_kernel<2>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
+                           st, dtab, stage);
 }
 
-__global__ void __launch_bounds__(256) gemm_group_kernel(const EnsTable *__restrict__ tab, int g)
+// one group's standalone matvec (stage 0)
+__global__ void __launch_bounds__(256) gemm_group_bf16_kernel(const EnsTable *__restrict__ tab, int g)
 {
     const EnsGroup &G = tab->g[g];
     const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= gemm_items(G)) return;
-    const int lane = threadIdx.x & 63;
-    if (gemm_f64_path(G, 0)) gemm_wave_f64(G, item, lane);
-    else gemm_wave_bf16(G, item, lane);
+    if (item >= bf16_row_items(G.mp, 2) * G.ksplit) return;
+    gemm_wave_bf16<2, false>(G, item, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(256) gemm_group_f64_kernel(const EnsTable *__restrict__ tab, int g)
+{
+    const EnsGroup &G = tab->g[g];
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= (G.mp >> 6) * G.ksplit) return;
+    gemm_wave_f64(G, item, threadIdx.x & 63);
 }
 
 void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g)
 {
     const EnsGroup &G = htab.g[g];
-    const int items = (G.mp >> 6) * G.ksplit;
-    hipLaunchKernelGGL(gemm_group_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
+    if (gemm_f64_path(G.mode, 0)) {
+        const int items = (G.mp >> 6) * G.ksplit;
+        hipLaunchKernelGGL(gemm_group_f64_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
+    } else {
+        const int items = bf16_row_items(G.mp, 2) * G.ksplit;
+        hipLaunchKernelGGL(gemm_group_bf16_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -180,10 +180,14 @@ class Ensemble:
         return T
 
     def tuning(self):
-        kb = C.c_int()
-        t = np.full(5, -1.0)
-        self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), _p(t)))
-        return kb.value, t
+        """(k-blocks per split, split-bf16 GEMM variant, seconds per launch
+        of the 15 candidates: k-blocks 32..512 of variant 0, 1, 2).  Variants:
+        0 = 64 K rows per wave, 1 = 64 rows with loads issued a k-block
+        ahead, 2 = 128 rows, ahead."""
+        kb, var = C.c_int(), C.c_int()
+        t = np.full(15, -1.0)
+        self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
+        return kb.value, var.value, t
 
     def time_gemm(self, iters: int = 20):
         s, b = C.c_double(), C.c_double()
