@@ -39,10 +39,12 @@ Reported:
   learned_step_proposal
                 the 8-64-64-1 step-size network trained on the plain binary64
                 integrator's step histories of 16 instances of the 1024-patch
-                group, against the classic controller on the 48 held-out
-                instances over a fixed simulated window: RHS evaluations per
-                simulated year, rejected fraction, slip and first-event-time
-                differences, and sim-yr/s against the plain binary64 path.
+                group, refined by two closed-loop rounds on the same
+                instances and an output offset, against the classic
+                controller on the 48 held-out instances over a fixed
+                simulated window: RHS evaluations per simulated year,
+                rejected fraction, slip and first-event-time differences, and
+                the (lockstep-bound) sim-yr/s against the plain binary64 path.
   cpu_baseline  the oracle's C restatement (OpenMP over the host cores): RK45
                 attempts/s on instance 0 of every group over a bounded sample,
                 costed over every instance's attempt count of the GPU run.
@@ -70,7 +72,7 @@ INF_T = 1e30               # no end time: runs are bounded by attempts
 # HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE runs, scaled by a 2 GiB copy (profiles/copy_calibration_2GiB.md,
 # profiles/stage_gemm_split_bf16_ensemble256.md); same workload as this bench
-MEASURED_TRAFFIC = {"split_bf16_stage_gemm": 2.273e9}
+MEASURED_TRAFFIC = {"split_bf16_stage_gemm": 2.269e9}
 
 
 def log(*a):

@@ -440,17 +440,14 @@ static int capture(ucy_ensemble *E)
     const EnsTable &T = E->tab;
     hipGraph_t g = nullptr;
     EHIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    ens_launch_step_start(c->stream, E->d_tab, T);
-    ens_launch_gemm(c->stream, E->d_tab, T, 1);
-    ens_launch_post(c->stream, E->d_tab, T, 1);
-    for (int s = 2; s <= 6; s++) {
-        ens_launch_stage(c->stream, E->d_tab, T, s);
+    ens_launch_start(c->stream, E->d_tab, T);
+    for (int s = 1; s <= 5; s++) {
         ens_launch_gemm(c->stream, E->d_tab, T, s);
-        ens_launch_post(c->stream, E->d_tab, T, s);
+        ens_launch_post_stage(c->stream, E->d_tab, T, s);
     }
-    ens_launch_finish(c->stream, E->d_tab, T);
+    ens_launch_gemm(c->stream, E->d_tab, T, 6);
+    ens_launch_post_finish(c->stream, E->d_tab, T);
     ens_launch_control(c->stream, E->d_tab, T);
-    ens_launch_commit(c->stream, E->d_tab, T);
     EHIP(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -497,6 +494,9 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
         if (*h_done >= E->tab.total_inst) break;
         if (batch < 64) batch *= 2;
     }
+    // the last attempt's accepted steps (later attempts commit at their start)
+    ens_launch_commit(c->stream, E->d_tab, E->tab);
+    EHIP(hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
     (void)hipHostFree(h_done);
     if (stats) {

@@ -95,10 +95,12 @@ void ens_launch_tile_K64(hipStream_t st, const double *K, int m, int n, int mp, 
 // stage 1 = the step-start matvec (skipped when no instance of a group starts
 // a step), 2..6 the stage matvecs, 0 a standalone matvec
 void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
-void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
-void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
-void ens_launch_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
-void ens_launch_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+// one attempt = start (commit of the previous accepted step + step-start
+// pre-pass), gemm 1, post_stage 1, gemm 2, ..., post_stage 5, gemm 6,
+// post_finish, control; a run ends with one commit
+void ens_launch_start(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
+void ens_launch_post_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage);
+void ens_launch_post_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_rearm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);

@@ -477,12 +477,25 @@ __device__ __forceinline__ bool elem_coords(const EnsTable *tab, int &g, int &e,
     return e < G.N && inst < G.ninst;
 }
 
-// step start: odefun pre-pass at y for instances that start a new step
-__global__ void __launch_bounds__(256) ens_step_start_kernel(const EnsTable *__restrict__ tab)
+// attempt start: commit of the previous attempt's accepted step (y = ycand,
+// ode45.f90:100-101), then the odefun pre-pass at y for instances that start
+// a new step.  The commit is idempotent (a finished instance keeps its
+// accepted flag and its ycand), and the run ends with one commit launch.
+__device__ __forceinline__ void commit_elem(const EnsGroup &G, int e, int inst)
+{
+    if (!G.accepted[inst]) return;
+    for (int k = 0; k < 7; k++) {
+        const size_t i = sidx(G, k, e, inst);
+        G.y[i] = G.ycand[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
     if (!elem_coords(tab, g, e, inst)) return;
     const EnsGroup &G = tab->g[g];
+    commit_elem(G, e, inst);
     if (G.done[inst] || !G.need_rhs[inst]) return;
     *G.any_rhs = 1;
     const double vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
@@ -492,13 +505,8 @@ __global__ void __launch_bounds__(256) ens_step_start_kernel(const EnsTable *__r
 
 // post-pass (ratestate.f90:747-771) writing stage derivative `stage`
 // (1 = step start, then also yscal of ratestate.f90:315)
-__global__ void __launch_bounds__(256) ens_post_kernel(const EnsTable *__restrict__ tab, int stage)
+__device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, int stage)
 {
-    int g, e, inst;
-    if (!elem_coords(tab, g, e, inst)) return;
-    const EnsGroup &G = tab->g[g];
-    if (G.done[inst]) return;
-    if (stage == 1 && !G.need_rhs[inst]) return;
     const double *X = stage == 1 ? G.y : G.ytmp;
     double *out = G.ak[stage];
     const size_t f = (size_t)e * kEnsCols + inst;
@@ -558,12 +566,10 @@ __device__ __forceinline__ double ens_stage_value(int stage, double h, double y0
     }
 }
 
-__global__ void __launch_bounds__(256) ens_stage_kernel(const EnsTable *__restrict__ tab, int stage)
+// stage input `stage` (2..6) of element e: y + h sum_j a_ij k_j
+// (ode45.f90:133-161) and its odefun pre-pass
+__device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, int stage)
 {
-    int g, e, inst;
-    if (!elem_coords(tab, g, e, inst)) return;
-    const EnsGroup &G = tab->g[g];
-    if (G.done[inst]) return;
     if (stage == 2 && e == 0) G.errmax[inst] = 0.0;
     const double h = G.h[inst];
     for (int k = 0; k < 7; k++) {
@@ -578,8 +584,53 @@ __global__ void __launch_bounds__(256) ens_stage_kernel(const EnsTable *__restri
     ens_prepass(G, G.ytmp, G.ak[stage], e, inst, false);
 }
 
-// fifth-order solution, embedded error, per-instance max |yerr/yscal|
-__global__ void __launch_bounds__(256) ens_finish_kernel(const EnsTable *__restrict__ tab)
+// after the GEMM of stage s (1..5): the post-pass of s, then the input and
+// pre-pass of stage s+1 (one launch per stage instead of two)
+__global__ void __launch_bounds__(256) ens_post_stage_kernel(const EnsTable *__restrict__ tab, int stage)
+{
+    int g, e, inst;
+    if (!elem_coords(tab, g, e, inst)) return;
+    const EnsGroup &G = tab->g[g];
+    if (G.done[inst]) return;
+    // a retried attempt keeps the step-start derivative (ratestate.f90:313)
+    if (stage != 1 || G.need_rhs[inst]) post_elem(G, e, inst, stage);
+    stage_elem(G, e, inst, stage + 1);
+}
+
+// fifth-order solution and embedded error of element e (ode45.f90:162-172);
+// returns max_k |yerr/yscal| (NaN propagates)
+__device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst)
+{
+    double emax = 0.0;
+    const double h = G.h[inst];
+    const double DC1 = 37.0 / 378.0 - 2825.0 / 27648.0, DC3 = 250.0 / 621.0 - 18575.0 / 48384.0,
+                 DC4 = 125.0 / 594.0 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0,
+                 DC6 = 512.0 / 1771.0 - 0.25;
+    for (int k = 0; k < 7; k++) {
+        const size_t i = sidx(G, k, e, inst);
+        const double k1 = G.dydt[i], k3 = G.ak[3][i], k4 = G.ak[4][i], k5 = G.ak[5][i],
+                     k6 = G.ak[6][i];
+        double yr = (37.0 / 378.0) * k1;
+        yr = yr + (250.0 / 621.0) * k3;
+        yr = yr + (125.0 / 594.0) * k4;
+        yr = yr + (512.0 / 1771.0) * k6;
+        G.ycand[i] = G.y[i] + h * yr;
+        double er = DC1 * k1;
+        er = er + DC3 * k3;
+        er = er + DC4 * k4;
+        er = er + DC5 * k5;
+        er = er + DC6 * k6;
+        er = h * er;
+        G.yerr[i] = er;
+        const double r = fabs(er / G.yscal[i]);
+        emax = (r > emax || r != r) ? r : emax;
+    }
+    return emax;
+}
+
+// after the GEMM of stage 6: its post-pass, the fifth-order solution, the
+// embedded error and the per-instance max |yerr/yscal|
+__global__ void __launch_bounds__(256) ens_post_finish_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
     const bool live = elem_coords(tab, g, e, inst);
@@ -588,29 +639,8 @@ __global__ void __launch_bounds__(256) ens_finish_kernel(const EnsTable *__restr
     if (blockIdx.x == tab->blk_elem[g] && threadIdx.x == 0) *G.any_rhs = 0;
     double emax = 0.0;
     if (live && !G.done[inst]) {
-        const double h = G.h[inst];
-        const double DC1 = 37.0 / 378.0 - 2825.0 / 27648.0, DC3 = 250.0 / 621.0 - 18575.0 / 48384.0,
-                     DC4 = 125.0 / 594.0 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0,
-                     DC6 = 512.0 / 1771.0 - 0.25;
-        for (int k = 0; k < 7; k++) {
-            const size_t i = sidx(G, k, e, inst);
-            const double k1 = G.dydt[i], k3 = G.ak[3][i], k4 = G.ak[4][i], k5 = G.ak[5][i],
-                         k6 = G.ak[6][i];
-            double yr = (37.0 / 378.0) * k1;
-            yr = yr + (250.0 / 621.0) * k3;
-            yr = yr + (125.0 / 594.0) * k4;
-            yr = yr + (512.0 / 1771.0) * k6;
-            G.ycand[i] = G.y[i] + h * yr;
-            double er = DC1 * k1;
-            er = er + DC3 * k3;
-            er = er + DC4 * k4;
-            er = er + DC5 * k5;
-            er = er + DC6 * k6;
-            er = h * er;
-            G.yerr[i] = er;
-            const double r = fabs(er / G.yscal[i]);
-            emax = (r > emax || r != r) ? r : emax;
-        }
+        post_elem(G, e, inst, 6);
+        emax = finish_elem(G, e, inst);
     }
     // the 4 elements of this block share instance columns: reduce across waves
     __shared__ unsigned long long red[256];
@@ -642,7 +672,7 @@ __global__ void ens_control_kernel(EnsTable *tab)
     if (!(raw == raw) || raw > 1.7976931348623157e308) {
         // an attempt whose error norm is not finite (an overflow inside the
         // stages) is rejected with the largest shrink of the rule, x0.1
-        // (the reference would accept it: NaN > 1 is false); the fourth such
+        // (ode45.f90:81 would accept it: NaN > 1 is false); the fourth such
         // attempt in a row fails the instance
         const double nf = hist[4 * kEnsCols + inst] + 1.0;
         hist[4 * kEnsCols + inst] = nf;
@@ -758,29 +788,21 @@ __global__ void __launch_bounds__(256) ens_commit_kernel(const EnsTable *__restr
 {
     int g, e, inst;
     if (!elem_coords(tab, g, e, inst)) return;
-    const EnsGroup &G = tab->g[g];
-    if (!G.accepted[inst]) return;
-    for (int k = 0; k < 7; k++) {
-        const size_t i = sidx(G, k, e, inst);
-        G.y[i] = G.ycand[i];
-    }
+    commit_elem(tab->g[g], e, inst);
 }
 
-void ens_launch_step_start(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
+void ens_launch_start(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
 {
-    hipLaunchKernelGGL(ens_step_start_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
+    hipLaunchKernelGGL(ens_start_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
 }
-void ens_launch_post(hipStream_t st, const EnsTable *dtab, const EnsTable &h, int stage)
+void ens_launch_post_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &h, int stage)
 {
-    hipLaunchKernelGGL(ens_post_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab, stage);
+    hipLaunchKernelGGL(ens_post_stage_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab,
+                       stage);
 }
-void ens_launch_stage(hipStream_t st, const EnsTable *dtab, const EnsTable &h, int stage)
+void ens_launch_post_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
 {
-    hipLaunchKernelGGL(ens_stage_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab, stage);
-}
-void ens_launch_finish(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
-{
-    hipLaunchKernelGGL(ens_finish_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
+    hipLaunchKernelGGL(ens_post_finish_kernel, dim3(h.blk_elem[h.ngroups]), dim3(256), 0, st, dtab);
 }
 void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
 {
