@@ -48,6 +48,12 @@ Reported:
   cpu_baseline  the oracle's C restatement (OpenMP over the host cores): RK45
                 attempts/s on instance 0 of every group over a bounded sample,
                 costed over every instance's attempt count of the GPU run.
+  assembly      Green's-function assembly (buildG) on the device: pairs/s of
+                the Okada rectangles (2048-patch thrust), the Nikkhoo-Walter
+                triangles (20,000-triangle curved megathrust, 19.2 GB K) and
+                the strain volumes (4096 patches + 8192 cuboids, 28.2 GB K),
+                against the FP64 vector peak, with the oracle's buildG on the
+                host cores beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--attempts R]
 Prints exactly one JSON line on stdout; progress goes to stderr.
@@ -73,6 +79,12 @@ INF_T = 1e30               # no end time: runs are bounded by attempts
 # WRITE_SIZE runs, scaled by a 2 GiB copy (profiles/copy_calibration_2GiB.md,
 # profiles/stage_gemm_split_bf16_ensemble256.md); same workload as this bench
 MEASURED_TRAFFIC = {"split_bf16_stage_gemm": 2.269e9}
+# FP64 flop per (receiver, source) pair of the assembly kernels from
+# rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / pairs on the same
+# models (profiles/assembly_greens_functions.md)
+ASSEMBLY_FLOP_PER_PAIR = {"okada": 7108.0, "okada_stress": 7199.0, "nikkhoo": 21680.0,
+                          "strain_volume": 61800.0}
+FP64_VALU_PEAK = 78.6e12   # FLOP/s, dense binary64 vector
 
 
 def log(*a):
@@ -455,6 +467,103 @@ def bench_plain(args):
     }
 
 
+def bench_assembly(args, cpu=True):
+    """Green's-function assembly (buildG, greens.f90:73-275) on the device
+    for the three source types at the model sizes of the single-instance
+    configurations, with the oracle's OpenMP buildG on the host cores for
+    per-pair rates on smaller meshes of the same kind.  A pair is one
+    (receiver, source) element pair with all its slip directions or
+    eigenstrains: 2 x 3 traction entries (rectangles, triangles), 6 x 3 or
+    6 x 6 (volumes)."""
+    from unicycle_amd import Engine, configs
+    out = {}
+
+    def gpu(tag, setup, warm):
+        eng = Engine(0)
+        warm(eng)                         # the same kernels on a small model
+        eng.buildG()
+        setup(eng)
+        t0 = time.perf_counter()
+        eng.buildG()
+        wall = time.perf_counter() - t0
+        secs = eng.build_seconds()
+        m, n = eng.shape
+        eng.close()
+        log(f"assembly {tag}: {wall:.3f} s, K {8.0 * m * n / 1e9:.1f} GB")
+        return wall, secs, 8.0 * m * n
+
+    # rectangles: the 2048-patch thrust (Okada traction pairs)
+    rm = configs.thrust_fault()
+    N = rm.n
+
+    def s_rect(e, m=rm):
+        e.set_elastic(m.lambda_, m.mu)
+        e.set_rectangles(*m.geometry())
+    wall, secs, kb = gpu("rectangles", s_rect, lambda e: s_rect(e, configs.thrust_fault(8, 8)))
+    def roof(pairs_per_s, key):
+        a = pairs_per_s * ASSEMBLY_FLOP_PER_PAIR[key]
+        return {"bound": "fp64 valu", "achieved": a / 1e12, "peak": FP64_VALU_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": a / FP64_VALU_PEAK,
+                "flop_per_pair": ASSEMBLY_FLOP_PER_PAIR[key]}
+
+    out["okada_rectangles"] = {"elements": N, "pairs": N * N, "K_bytes": kb, "gpu_s": secs[0],
+                               "gpu_pairs_per_s": N * N / secs[0],
+                               "roofline": roof(N * N / secs[0], "okada")}
+    # triangles: the 20,000-triangle curved megathrust (19.2 GB K)
+    tm = configs.megathrust_mesh()
+    T = tm.n
+
+    def s_tri(e, m=tm):
+        e.set_elastic(m.lambda_, m.mu)
+        e.set_triangles(*m.geometry())
+    wall, secs, kb = gpu("triangles", s_tri, lambda e: s_tri(e, configs.megathrust_mesh(8, 8)))
+    out["nikkhoo_triangles"] = {"elements": T, "pairs": T * T, "K_bytes": kb, "gpu_s": secs[1],
+                                "gpu_pairs_per_s": T * T / secs[1],
+                                "roofline": roof(T * T / secs[1], "nikkhoo")}
+    # fault + strain volumes: 4096 1.5-km patches and 32 x 32 x 8 cuboids (28.2 GB K)
+    fm = configs.thrust_fault(nx=64, nz=64, patch=1500.0)
+    vm = configs.volume_grid()
+    Nf, Nv = fm.n, vm.n
+
+    def s_vol(e, f=fm, v=vm):
+        e.set_elastic(f.lambda_, f.mu)
+        e.set_rectangles(*f.geometry())
+        e.set_volumes(*v.geometry())
+    wall, secs, kb = gpu("fault + volumes", s_vol,
+                         lambda e: s_vol(e, configs.thrust_fault(8, 8), configs.volume_grid(4, 4, 2)))
+    out["fault_and_volumes"] = {
+        "patches": Nf, "volumes": Nv, "K_bytes": kb, "gpu_s_total": wall,
+        "okada_pairs": Nf * (Nf + Nv), "okada_gpu_s": secs[0],
+        "okada_gpu_pairs_per_s": Nf * (Nf + Nv) / secs[0],
+        "volume_pairs": Nv * (Nf + Nv), "volume_gpu_s": secs[2],
+        "volume_gpu_pairs_per_s": Nv * (Nf + Nv) / secs[2],
+        "volume_roofline": roof(Nv * (Nf + Nv) / secs[2], "strain_volume")}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import orc
+        cores = host_cores()
+        samples = {
+            "okada_rectangles": (lambda: orc.Mesh(rm.lambda_, rm.mu,
+                                                  rect=configs.thrust_fault(64, 32).geometry())),
+            "nikkhoo_triangles": (lambda: orc.Mesh(tm.lambda_, tm.mu, tri=configs.megathrust_mesh(
+                nx=24, ny=24).geometry())),
+            "volumes": (lambda: orc.Mesh(fm.lambda_, fm.mu, vol=configs.volume_grid(8, 8, 4).geometry())),
+        }
+        for key, mk in samples.items():
+            mesh = mk()
+            ne = mesh.np + mesh.nvol
+            t0 = time.perf_counter()
+            orc.buildG(mesh, nthreads=cores)
+            w = time.perf_counter() - t0
+            rate = ne * ne / w
+            tgt = out[key] if key in out else out["fault_and_volumes"]
+            tgt["cpu_pairs_per_s" if key in out else "volume_cpu_pairs_per_s"] = rate
+            tgt["cpu_sample" if key in out else "volume_cpu_sample"] = (
+                f"oracle buildG, {ne} elements ({ne * ne} pairs) in {w:.2f} s on {cores} cores")
+            log(f"assembly cpu {key}: {rate:.3g} pairs/s")
+    return out
+
+
 def bench_gemm(args):
     """The all-groups stage GEMM of one precision alone, launched directly
     (no graph): autotuning launches, then 20 timed launches.  For profiler
@@ -516,11 +625,12 @@ def main():
     ap.add_argument("--event-years", type=float, default=0.5,
                     help="horizon of the first-event comparison")
     ap.add_argument("--workload", default="ensemble",
-                    choices=["ensemble", "plain", "gemm", "rhs", "copy"])
+                    choices=["ensemble", "plain", "gemm", "rhs", "copy", "assembly"])
     ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16"],
                     help="stage-GEMM precision of --workload gemm")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--learned-years", type=float, default=0.25,
                     help="simulated window of the learned-proposal comparison")
     ap.add_argument("--learned-steps", type=int, default=3000,
@@ -532,7 +642,8 @@ def main():
         sys.exit(2)
     if args.workload != "ensemble":
         fn = {"plain": bench_plain, "gemm": bench_gemm, "rhs": bench_rhs,
-              "copy": bench_copy}[args.workload]
+              "copy": bench_copy,
+              "assembly": lambda a: bench_assembly(a, cpu=not a.no_cpu)}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     out, (cpu_inputs, att64, sim_years) = bench_ensemble(args)
@@ -541,6 +652,8 @@ def main():
             args, out["plain_fp64"]["groups"][0]["us_per_attempt"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_ensemble(cpu_inputs, att64, sim_years)
+    if not args.no_assembly:
+        out["assembly"] = bench_assembly(args, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -145,6 +145,10 @@ int ucy_set_rheology(ucy_ctx *ctx, int nv, const double *gammadot0, const double
 
 /* ---- assembly: buildG (greens.f90:73-275) -------------------------------- */
 int ucy_build_kernel(ucy_ctx *ctx, int precision);
+/* wall seconds of the last ucy_build_kernel by source type: rectangle,
+ * triangle and strain-volume columns (each ends with a stream synchronize;
+ * no counterpart in the reference: for measuring assembly) */
+int ucy_build_seconds(const ucy_ctx *ctx, double *rect, double *tri, double *vol);
 int ucy_kernel_shape(const ucy_ctx *ctx, int64_t *m, int64_t *n);
 /* copy K (m x n row-major, binary64) back to the host */
 int ucy_get_kernel(ucy_ctx *ctx, double *K);

@@ -118,6 +118,22 @@ def test_buildG_triangles_matches_oracle(engine_factory):
     assert np.all(np.abs(K - Kref) <= 1e-10 * rowscale)
 
 
+def test_buildG_megathrust_mesh_matches_oracle(engine_factory):
+    """The curved, jittered megathrust mesh (non-planar, irregular
+    triangles of both orientations) at 12 x 8 quads: K and the triangle
+    frames against the oracle."""
+    tm = configs.megathrust_mesh(nx=12, ny=8, seed=5)
+    eng = _tri_engine(engine_factory, tm)
+    eng.buildG()
+    K = eng.get_kernel()
+    Kref = orc.buildG(orc.Mesh(tm.lambda_, tm.mu, tri=tm.geometry()), nthreads=8)
+    assert K.shape == Kref.shape == (3 * tm.n, 2 * tm.n)
+    assert not np.isnan(K).any()
+    assert _rel(K, Kref) < 1e-12
+    rowscale = np.abs(Kref).max(axis=1, keepdims=True)
+    assert np.all(np.abs(K - Kref) <= 1e-10 * rowscale)
+
+
 def test_buildG_mixed_mesh_layout(engine_factory):
     """Rectangles first, then triangles (ratestate.f90:1174-1290 element
     order) in both the rows and the columns of K."""

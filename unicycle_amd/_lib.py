@@ -87,6 +87,7 @@ _SIGS = {
     "ucy_set_rheology": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
     "ucy_build_kernel": (_i, [_vp, _i]),
     "ucy_kernel_shape": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ucy_build_seconds": (_i, [_vp, _pd, _pd, _pd]),
     "ucy_get_kernel": (_i, [_vp, _pd]),
     "ucy_set_kernel": (_i, [_vp, _pd, _i64, _i64]),
     "ucy_kernel_dev": (_vp, [_vp]),

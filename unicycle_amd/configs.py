@@ -10,6 +10,12 @@ thrust_fault
     radiation damping mu/(2 Vs).  nx=64, nz=32, 2 km patches is the
     2048-patch single-instance model; the ensemble groups use the same rules
     at 1024/2048/4096/8192 patches.
+megathrust_mesh
+    curved megathrust of ~20,000 jittered triangles with random-field
+    friction (the triangle-mesh model).
+volume_grid
+    vertical strain-volume cuboids with Maxwell or power-law rheology (with
+    the 4096-patch, 1.5 km thrust: the fault + viscoelastic model).
 """
 from __future__ import annotations
 
@@ -139,6 +145,60 @@ def triangulate(model: FaultModel) -> TriangleModel:
     i3 = np.stack([inv[:, 2], inv[:, 3]], 1).reshape(-1).astype(np.int32)
     fric = tuple(np.repeat(np.asarray(f, dtype=np.float64), 2) for f in model.friction())
     return TriangleModel(model.lambda_, model.mu, i1, i2, i3, verts.astype(np.float64), fric)
+
+
+def gaussian_field(points: np.ndarray, mean: float, sd: float, corr: float, seed: int,
+                   modes: int = 512) -> np.ndarray:
+    """Stationary Gaussian random field with covariance sd^2 exp(-r^2 / (2 corr^2))
+    at ``points`` (n, 2) [m], by random Fourier features (Rahimi and Recht
+    2007): sqrt(2/M) sum_m cos(k_m . x + phi_m), k_m ~ N(0, I/corr^2),
+    phi_m ~ U(0, 2 pi).  Seeded."""
+    rng = np.random.default_rng(seed)
+    k = rng.normal(0.0, 1.0 / corr, (modes, 2))
+    ph = rng.uniform(0.0, 2.0 * np.pi, modes)
+    f = np.sqrt(2.0 / modes) * np.cos(points @ k.T + ph).sum(axis=1)
+    return mean + sd * f
+
+
+def megathrust_mesh(nx: int = 100, ny: int = 100, length: float = 200e3, width: float = 100e3,
+                    jitter: float = 0.1, seed: int = 11, mu: float = 30e3, nu: float = 0.25,
+                    vs: float = 3000.0, a: float = 0.010, sig: float = 100.0, Vo: float = 1e-6,
+                    mu0: float = 0.6, Vpl: float = 1e-9, rake_deg: float = 89.0) -> TriangleModel:
+    """Curved megathrust z = 5 + 0.1 x + 0.0004 x^2 km (x across strike, km)
+    over ``length`` along strike (north) x ``width`` across (east): an
+    nx (strike) x ny (dip) grid of quadrilaterals split along diagonals into
+    2 nx ny triangles (20,000 at the defaults); interior vertices jittered by
+    U(-jitter, jitter) x edge length in plan view, depths then taken on the
+    surface.  Friction: a - b from a seeded Gaussian random field (mean
+    -0.002, sd 0.003, correlation 20 km) at the centroids, L ~ lognormal(ln
+    0.02 m, 0.5), the rest as thrust_fault."""
+    rng = np.random.default_rng(seed)
+    dn, de = length / nx, width / ny
+    gi, gj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    north = gi * dn
+    east = gj * de
+    interior = (gi > 0) & (gi < nx) & (gj > 0) & (gj < ny)
+    north = north + interior * rng.uniform(-jitter, jitter, north.shape) * dn
+    east = east + interior * rng.uniform(-jitter, jitter, east.shape) * de
+    xk = east / 1e3
+    depth = (5.0 + 0.1 * xk + 0.0004 * xk * xk) * 1e3
+    verts = np.stack([north.ravel(), east.ravel(), depth.ravel()], 1)
+    vid = lambda i, j: i * (ny + 1) + j + 1  # noqa: E731  (1-based, as the decks)
+    qi, qj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    qi, qj = qi.ravel(), qj.ravel()
+    A, B, Cc, D = vid(qi, qj), vid(qi + 1, qj), vid(qi + 1, qj + 1), vid(qi, qj + 1)
+    i1 = np.stack([A, A], 1).reshape(-1).astype(np.int32)
+    i2 = np.stack([B, Cc], 1).reshape(-1).astype(np.int32)
+    i3 = np.stack([Cc, D], 1).reshape(-1).astype(np.int32)
+    n = i1.size
+    cen = (verts[i1 - 1] + verts[i2 - 1] + verts[i3 - 1]) / 3.0
+    amb = gaussian_field(cen[:, :2], -0.002, 0.003, 20e3, seed + 1)
+    Lc = np.exp(np.log(0.02) + 0.5 * np.random.default_rng(seed + 2).normal(size=n))
+    lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
+    one = np.ones(n)
+    fric = (Vpl * one, np.deg2rad(rake_deg) * one, mu0 * one, sig * one, a * one, a - amb,
+            Lc, Vo * one, mu / (2.0 * vs) * one)
+    return TriangleModel(lam, mu, i1, i2, i3, verts, fric)
 
 
 def perturbed_initial_state(model: FaultModel, y0: np.ndarray, seed: int,

@@ -423,6 +423,13 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * ne));
     HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * ne, hipMemcpyHostToDevice, c->stream));
     int rc = 0;
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point t0) {
+        return std::chrono::duration<double>(clk::now() - t0).count();
+    };
+    c->build_seconds[0] = c->build_seconds[1] = c->build_seconds[2] = 0.0;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto t0 = clk::now();
     // rectangle sources -> all receivers
     if (c->nrect) {
         std::vector<OkadaSource> S(c->nrect);
@@ -447,17 +454,32 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(d_S));
     }
+    c->build_seconds[0] = since(t0);
+    t0 = clk::now();
     if (c->ntri) rc = ext_build_triangle_columns(c->stream, c->mu, c->lambda, c->ntri, c->i1.data(),
                                                   c->i2.data(), c->i3.data(), c->nve,
                                                   c->vert.data(), d_R, np, c->nvol, c->K, n,
                                                   2 * (int64_t)c->nrect, c->err);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->build_seconds[1] = since(t0);
+    t0 = clk::now();
     if (!rc && c->nvol)
         rc = ext_build_volume_columns(c->stream, c->mu, c->lambda, c->nvol, c->vx.data(),
                                       c->vL.data(), c->vT.data(), c->vW.data(),
                                       c->vstrike.data(), d_R, np, c->K, n, 2 * (int64_t)np, c->err);
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->build_seconds[2] = since(t0);
     HIPCHK(hipFree(d_R));
     return rc;
+}
+
+extern "C" int ucy_build_seconds(const ucy_ctx *c, double *rect, double *tri, double *vol)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (rect) *rect = c->build_seconds[0];
+    if (tri) *tri = c->build_seconds[1];
+    if (vol) *vol = c->build_seconds[2];
+    return UCY_OK;
 }
 
 extern "C" int ucy_kernel_shape(const ucy_ctx *c, int64_t *m, int64_t *n)

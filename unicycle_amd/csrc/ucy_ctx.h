@@ -39,6 +39,9 @@ struct ucy_ctx {
     // device kernel
     double *K = nullptr;
     int64_t m = 0, n = 0;
+    // wall seconds of the last ucy_build_kernel by source type (rectangle,
+    // triangle, volume columns; each phase ends with a stream synchronize)
+    double build_seconds[3] = {0, 0, 0};
     // device per-element data
     Friction *d_fr = nullptr;
     Rheology *d_rh = nullptr;

@@ -153,6 +153,13 @@ class Engine:
     def buildG(self, precision: int = PREC_F64):
         self._chk(self._L.ucy_build_kernel(self._h, precision))
 
+    def build_seconds(self):
+        """Wall seconds of the last buildG by source type: (rectangles,
+        triangles, volumes)."""
+        r, t, v = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self._L.ucy_build_seconds(self._h, C.byref(r), C.byref(t), C.byref(v)))
+        return r.value, t.value, v.value
+
     @property
     def shape(self):
         m, n = C.c_int64(), C.c_int64()
