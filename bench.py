@@ -474,11 +474,21 @@ def bench_assembly(args, cpu=True):
     per-pair rates on smaller meshes of the same kind.  A pair is one
     (receiver, source) element pair with all its slip directions or
     eigenstrains: 2 x 3 traction entries (rectangles, triangles), 6 x 3 or
-    6 x 6 (volumes)."""
-    from unicycle_amd import Engine, configs
+    6 x 6 (volumes).  On each model it also times one RHS with the binary64
+    K and with the split-bf16 K (plain_rhs)."""
+    from unicycle_amd import Engine, PREC_SPLIT_BF16, configs
     out = {}
 
-    def gpu(tag, setup, warm):
+    rhs = {}
+
+    def rhs_time(eng, variant):
+        y0 = eng.initStateVector(variant)
+        eng.odefun(y0.size, 0.0, y0, np.empty_like(y0))
+        return eng.time_rhs_kernel(20)
+
+    def gpu(tag, setup, warm, fric, variant=0):
+        """buildG timing, then one RHS (K v + odefun post-pass, the kernel of
+        every RK45 stage) with the binary64 K and with the split-bf16 K."""
         eng = Engine(0)
         warm(eng)                         # the same kernels on a small model
         eng.buildG()
@@ -488,8 +498,20 @@ def bench_assembly(args, cpu=True):
         wall = time.perf_counter() - t0
         secs = eng.build_seconds()
         m, n = eng.shape
+        fric(eng)
+        t64 = rhs_time(eng, variant)
+        eng.buildG(PREC_SPLIT_BF16)
+        tsp = rhs_time(eng, variant)
         eng.close()
-        log(f"assembly {tag}: {wall:.3f} s, K {8.0 * m * n / 1e9:.1f} GB")
+        rhs[tag] = {"K_rows": m, "K_cols": n,
+                    "f64": {"us": 1e6 * t64, "rhs_per_s": 1.0 / t64,
+                            "GBps": 8.0 * m * n / t64 / 1e9, "hbm_frac": 8.0 * m * n / t64 / HBM_PEAK},
+                    "split_bf16": {"us": 1e6 * tsp, "rhs_per_s": 1.0 / tsp,
+                                   "GBps": 4.0 * m * n / tsp / 1e9,
+                                   "hbm_frac": 4.0 * m * n / tsp / HBM_PEAK},
+                    "speedup": t64 / tsp}
+        log(f"assembly {tag}: {wall:.3f} s, K {8.0 * m * n / 1e9:.1f} GB; RHS {1e6 * t64:.0f} us "
+            f"binary64, {1e6 * tsp:.0f} us split bf16")
         return wall, secs, 8.0 * m * n
 
     # rectangles: the 2048-patch thrust (Okada traction pairs)
@@ -499,7 +521,8 @@ def bench_assembly(args, cpu=True):
     def s_rect(e, m=rm):
         e.set_elastic(m.lambda_, m.mu)
         e.set_rectangles(*m.geometry())
-    wall, secs, kb = gpu("rectangles", s_rect, lambda e: s_rect(e, configs.thrust_fault(8, 8)))
+    wall, secs, kb = gpu("rectangles", s_rect, lambda e: s_rect(e, configs.thrust_fault(8, 8)),
+                         lambda e: e.set_friction(*rm.friction()))
     def roof(pairs_per_s, key):
         a = pairs_per_s * ASSEMBLY_FLOP_PER_PAIR[key]
         return {"bound": "fp64 valu", "achieved": a / 1e12, "peak": FP64_VALU_PEAK / 1e12,
@@ -516,7 +539,8 @@ def bench_assembly(args, cpu=True):
     def s_tri(e, m=tm):
         e.set_elastic(m.lambda_, m.mu)
         e.set_triangles(*m.geometry())
-    wall, secs, kb = gpu("triangles", s_tri, lambda e: s_tri(e, configs.megathrust_mesh(8, 8)))
+    wall, secs, kb = gpu("triangles", s_tri, lambda e: s_tri(e, configs.megathrust_mesh(8, 8)),
+                         lambda e: e.set_friction(*tm.friction()))
     out["nikkhoo_triangles"] = {"elements": T, "pairs": T * T, "K_bytes": kb, "gpu_s": secs[1],
                                 "gpu_pairs_per_s": T * T / secs[1],
                                 "roofline": roof(T * T / secs[1], "nikkhoo")}
@@ -529,8 +553,12 @@ def bench_assembly(args, cpu=True):
         e.set_elastic(f.lambda_, f.mu)
         e.set_rectangles(*f.geometry())
         e.set_volumes(*v.geometry())
+    def f_vol(e):
+        e.set_friction(*fm.friction())
+        e.set_rheology(*vm.rheology())
     wall, secs, kb = gpu("fault + volumes", s_vol,
-                         lambda e: s_vol(e, configs.thrust_fault(8, 8), configs.volume_grid(4, 4, 2)))
+                         lambda e: s_vol(e, configs.thrust_fault(8, 8), configs.volume_grid(4, 4, 2)),
+                         f_vol, variant=1)
     out["fault_and_volumes"] = {
         "patches": Nf, "volumes": Nv, "K_bytes": kb, "gpu_s_total": wall,
         "okada_pairs": Nf * (Nf + Nv), "okada_gpu_s": secs[0],
@@ -538,6 +566,7 @@ def bench_assembly(args, cpu=True):
         "volume_pairs": Nv * (Nf + Nv), "volume_gpu_s": secs[2],
         "volume_gpu_pairs_per_s": Nv * (Nf + Nv) / secs[2],
         "volume_roofline": roof(Nv * (Nf + Nv) / secs[2], "strain_volume")}
+    out["plain_rhs"] = rhs
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import orc

@@ -144,7 +144,16 @@ int ucy_set_rheology(ucy_ctx *ctx, int nv, const double *gammadot0, const double
                      const double *To);
 
 /* ---- assembly: buildG (greens.f90:73-275) -------------------------------- */
+/* K is always built and kept in binary64.  UCY_PREC_SPLIT_BF16 and
+ * UCY_PREC_MIXED also store K as packed split-bf16 pairs (4 bytes per entry,
+ * power-of-two row scales; no counterpart in the reference), which
+ * ucy_integrate then streams: t_i = t_0 + K~ (v_i - v_0) at every stage, with
+ * t_0 = K~ v_0 (split) or K v_0 in binary64 (mixed) at the step start.  The
+ * per-routine entry points (ucy_odefun, ucy_rkck, ucy_rkqs, ucy_matvec_f64)
+ * always use the binary64 K. */
 int ucy_build_kernel(ucy_ctx *ctx, int precision);
+/* t = K~ v with the stored split-bf16 K (host vectors; reduced builds only) */
+int ucy_matvec_reduced(ucy_ctx *ctx, const double *v, double *t);
 /* wall seconds of the last ucy_build_kernel by source type: rectangle,
  * triangle and strain-volume columns (each ends with a stream synchronize;
  * no counterpart in the reference: for measuring assembly) */

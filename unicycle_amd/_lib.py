@@ -93,6 +93,7 @@ _SIGS = {
     "ucy_kernel_dev": (_vp, [_vp]),
     "ucy_matvec_f64": (_i, [_vp, _pd, _pd]),
     "ucy_matvec_f64_dev": (_i, [_vp, _vp, _vp]),
+    "ucy_matvec_reduced": (_i, [_vp, _pd, _pd]),
     "ucy_init_state": (_i, [_vp, _i, _pd]),
     "ucy_odefun": (_i, [_vp, _i, _d, _pd, _pd, _pd]),
     "ucy_rkck": (_i, [_vp, _i, _d, _pd, _pd, _d, _pd, _pd]),

@@ -223,6 +223,8 @@ extern "C" void ucy_destroy(ucy_ctx *c)
     while (!c->ensembles.empty()) ucy_ensemble_destroy(c->ensembles.back());
     free_work(c);
     if (c->K) hipFree(c->K);
+    if (c->Kp) hipFree(c->Kp);
+    if (c->Krs) hipFree(c->Krs);
     if (c->d_fr) hipFree(c->d_fr);
     if (c->d_rh) hipFree(c->d_rh);
     if (c->d_ctl) hipFree(c->d_ctl);
@@ -403,8 +405,9 @@ static void element_frames(const ucy_ctx *c, std::vector<Receiver> &R)
 extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
 {
     if (!c) return UCY_ERR_ARG;
-    if (precision != UCY_PREC_F64)
-        return fail(c, UCY_ERR_ARG, "single-instance kernels are stored in binary64");
+    if (precision != UCY_PREC_F64 && precision != UCY_PREC_SPLIT_BF16 &&
+        precision != UCY_PREC_MIXED)
+        return fail(c, UCY_ERR_ARG, "unknown precision %d", precision);
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
     HIPCHK(hipSetDevice(c->device));
     const int np = c->nrect + c->ntri, ne = np + c->nvol;
@@ -414,6 +417,12 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
     c->K = nullptr;
+    if (c->Kp) HIPCHK(hipFree(c->Kp));
+    c->Kp = nullptr;
+    if (c->Krs) HIPCHK(hipFree(c->Krs));
+    c->Krs = nullptr;
+    c->kprec = UCY_PREC_F64;
+    if (n % 2) return fail(c, UCY_ERR_STATE, "odd column count");
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     c->m = m;
     c->n = n;
@@ -470,6 +479,15 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipStreamSynchronize(c->stream));
     c->build_seconds[2] = since(t0);
     HIPCHK(hipFree(d_R));
+    if (!rc && precision != UCY_PREC_F64) {
+        // the split pairs of K (4 bytes per entry) beside the binary64 K
+        HIPCHK(hipMalloc((void **)&c->Kp, (size_t)(m * n) * sizeof(uint32_t)));
+        HIPCHK(hipMalloc((void **)&c->Krs, (size_t)m * sizeof(double)));
+        launch_split_rowmajor(c->stream, c->K, m, n, c->Krs, c->Kp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->kprec = precision;
+    }
     return rc;
 }
 
@@ -512,6 +530,12 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
     c->K = nullptr;
+    if (c->Kp) HIPCHK(hipFree(c->Kp));
+    c->Kp = nullptr;
+    if (c->Krs) HIPCHK(hipFree(c->Krs));
+    c->Krs = nullptr;
+    c->kprec = UCY_PREC_F64;
+    if (n % 2) return fail(c, UCY_ERR_STATE, "odd column count");
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     HIPCHK(hipMemcpyAsync(c->K, K, (size_t)(m * n) * sizeof(double), hipMemcpyHostToDevice,
                           c->stream));
@@ -579,7 +603,8 @@ static int ensure_work(ucy_ctx *c)
     if (c->work && c->work_dim == (size_t)d.dim) return 0;
     free_work(c);
     const size_t nvec = 11;
-    const size_t bytes = sizeof(double) * (nvec * (size_t)d.dim + (size_t)d.n + (size_t)d.m + 64);
+    const size_t bytes =
+        sizeof(double) * (nvec * (size_t)d.dim + 2 * ((size_t)d.n + (size_t)d.m) + 64);
     HIPCHK(hipMalloc((void **)&c->work, bytes));
     HIPCHK(hipMemsetAsync(c->work, 0, bytes, c->stream));
     double *p = c->work;
@@ -595,6 +620,9 @@ static int ensure_work(ucy_ctx *c)
     b.yr = nullptr;
     b.v = p; p += d.n;
     b.t = p; p += d.m;
+    b.v0 = p; p += d.n;
+    b.t0 = p; p += d.m;
+    if (c->kprec == UCY_PREC_F64) b.v0 = b.t0 = nullptr;
     b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
     c->work_dim = d.dim;
     return 0;
@@ -610,6 +638,24 @@ static int check_ready(ucy_ctx *c, int n)
     int rc = upload_params(c);
     if (rc) return rc;
     return ensure_work(c);
+}
+
+extern "C" int ucy_matvec_reduced(ucy_ctx *c, const double *v, double *t)
+{
+    if (!c || !v || !t) return UCY_ERR_ARG;
+    if (!c->Kp) return fail(c, UCY_ERR_STATE, "kernel not built in reduced precision");
+    HIPCHK(hipSetDevice(c->device));
+    double *dv = nullptr, *dt = nullptr;
+    HIPCHK(hipMalloc((void **)&dv, sizeof(double) * c->n));
+    HIPCHK(hipMalloc((void **)&dt, sizeof(double) * c->m));
+    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    launch_matvec_split(c->stream, c->Kp, c->Krs, (int)c->m, (int)c->n, dv, dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(t, dt, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(dv));
+    HIPCHK(hipFree(dt));
+    return UCY_OK;
 }
 
 extern "C" int ucy_matvec_f64_dev(ucy_ctx *c, const double *v_dev, double *t_dev)
@@ -777,11 +823,12 @@ static int capture_attempt(ucy_ctx *c)
     RkBuffers &b = c->rb;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const KOperand K{c->K, c->Kp, c->Krs, c->kprec};
     launch_step_start(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl);
-    launch_step_rhs(c->stream, d, c->K, c->d_fr, b, c->d_ctl);
+    launch_step_rhs(c->stream, d, K, c->d_fr, b, c->d_ctl);
     for (int s = 2; s <= 6; s++) {
         launch_stage(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl, s);
-        launch_stage_rhs(c->stream, d, c->K, c->d_fr, b, c->d_ctl, s);
+        launch_stage_rhs(c->stream, d, K, c->d_fr, b, c->d_ctl, s);
     }
     launch_finish(c->stream, d, b, c->d_ctl);
     launch_controller(c->stream, b, c->d_ctl);
@@ -985,12 +1032,19 @@ extern "C" int ucy_time_rhs_kernel(ucy_ctx *c, int iters, double *avg_seconds)
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // at the state of the last ucy_odefun call (or the last stage of ucy_integrate)
+    // at the state of the last ucy_odefun call (or the last stage of ucy_integrate);
+    // in reduced precision a stage RHS: t = t0 + split(K) v over the b.t buffer
     launch_prepass(c->stream, d, c->d_fr, c->d_rh, b.ytmp, b.ak[2], b.v, nullptr);
-    launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], nullptr);  // warm
+    const KOperand K{c->K, c->Kp, c->Krs, c->kprec};
+    auto one = [&]() {
+        if (c->kprec == UCY_PREC_F64)
+            launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], nullptr);
+        else
+            launch_gemv_post_inc(c->stream, d, K, c->d_fr, b.ytmp, b.v, b.ak[2], b.t);
+    };
+    one();  // warm
     HIPCHK(hipEventRecord(e0, c->stream));
-    for (int i = 0; i < iters; i++)
-        launch_gemv_post(c->stream, d, c->K, c->d_fr, b.ytmp, b.v, b.ak[2], nullptr);
+    for (int i = 0; i < iters; i++) one();
     HIPCHK(hipEventRecord(e1, c->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;

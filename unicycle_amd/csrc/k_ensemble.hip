@@ -145,6 +145,30 @@ void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, i
                        mp, np, rscale, tiles);
 }
 
+// single-instance K -> packed split-bf16 pairs, row-major with the row
+// stride of K: word = hi << 16 | lo of K / rs[row]
+__global__ void __launch_bounds__(256) split_rowmajor_kernel(const double *__restrict__ K,
+                                                             int64_t m, int64_t n,
+                                                             const double *__restrict__ rs,
+                                                             uint32_t *__restrict__ Kp)
+{
+    const int64_t tot = m * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint16_t hi, lo;
+        split2(K[i] / rs[i / n], hi, lo);
+        Kp[i] = ((uint32_t)hi << 16) | lo;
+    }
+}
+
+void launch_split_rowmajor(hipStream_t st, const double *K, int64_t m, int64_t n, double *rs,
+                           uint32_t *Kp)
+{
+    hipLaunchKernelGGL(rowscale_kernel, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, st, K,
+                       (int)m, (int)n, (int)m, rs);
+    hipLaunchKernelGGL(split_rowmajor_kernel, dim3(256 * 64), dim3(256), 0, st, K, m, n, rs, Kp);
+}
+
 // ---------------------------------------------------------------------------
 // group lookup from a block index and a prefix table
 // ---------------------------------------------------------------------------

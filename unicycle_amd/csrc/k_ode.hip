@@ -10,6 +10,7 @@
 // Every kernel of the attempt reads the Control block first and returns when
 // the run is done, so a captured attempt can be replayed any number of times.
 #include <hip/hip_runtime.h>
+#include "../../include/ucy.h"
 #include "ucy_internal.h"
 
 namespace ucy {
@@ -117,6 +118,10 @@ __device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
 // ---------------------------------------------------------------------------
 // wave-level dot products of NR rows of K with v; one wavefront per element
 // ---------------------------------------------------------------------------
+// K is streamed once per RHS: non-temporal loads keep it from evicting v and
+// the state from the caches
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 template <int NR>
 __device__ __forceinline__ void wave_rows_dot(const double *__restrict__ K, int64_t row0, int n,
                                               const double *__restrict__ v, int lane, double acc[NR])
@@ -125,19 +130,19 @@ __device__ __forceinline__ void wave_rows_dot(const double *__restrict__ K, int6
     for (int r = 0; r < NR; r++) acc[r] = 0.0;
     const int n2 = n >> 1;
     const double2 *v2 = reinterpret_cast<const double2 *>(v);
-    const double2 *kr[NR];
+    const f64x2 *kr[NR];
 #pragma unroll
-    for (int r = 0; r < NR; r++) kr[r] = reinterpret_cast<const double2 *>(K + (row0 + r) * (int64_t)n);
+    for (int r = 0; r < NR; r++) kr[r] = reinterpret_cast<const f64x2 *>(K + (row0 + r) * (int64_t)n);
     int j = lane;
     for (; j + 192 < n2; j += 256) {
         double2 vv[4];
-        double2 kk[NR][4];
+        f64x2 kk[NR][4];
 #pragma unroll
         for (int u = 0; u < 4; u++) vv[u] = v2[j + 64 * u];
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
-            for (int u = 0; u < 4; u++) kk[r][u] = kr[r][j + 64 * u];
+            for (int u = 0; u < 4; u++) kk[r][u] = __builtin_nontemporal_load(kr[r] + j + 64 * u);
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -150,7 +155,7 @@ __device__ __forceinline__ void wave_rows_dot(const double *__restrict__ K, int6
         const double2 vv = v2[j];
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            const double2 kk = kr[r][j];
+            const f64x2 kk = kr[r][j];
             acc[r] = fma(kk.x, vv.x, acc[r]);
             acc[r] = fma(kk.y, vv.y, acc[r]);
         }
@@ -162,6 +167,84 @@ __device__ __forceinline__ void wave_rows_dot(const double *__restrict__ K, int6
         for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
         acc[r] = a;
     }
+}
+
+// the same with the packed split-bf16 K: entry = (hi + lo) * rs[row], each
+// pair decoded exactly to binary64 and accumulated in binary64 (the GEMV is
+// bound by HBM at 4 bytes per entry, so the wider arithmetic is free)
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double split_entry(uint32_t w)
+{
+    return (double)__uint_as_float(w & 0xFFFF0000u) + (double)__uint_as_float(w << 16);
+}
+
+template <int NR>
+__device__ __forceinline__ void wave_rows_dot_split(const uint32_t *__restrict__ Kp,
+                                                    const double *__restrict__ rs, int64_t row0,
+                                                    int n, const double *__restrict__ v, int lane,
+                                                    double acc[NR])
+{
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r] = 0.0;
+    const int n2 = n >> 1;
+    const double2 *v2 = reinterpret_cast<const double2 *>(v);
+    const u32x2 *kr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) kr[r] = reinterpret_cast<const u32x2 *>(Kp + (row0 + r) * (int64_t)n);
+    int j = lane;
+    for (; j + 192 < n2; j += 256) {
+        double2 vv[4];
+        u32x2 kk[NR][4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) vv[u] = v2[j + 64 * u];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) kk[r][u] = __builtin_nontemporal_load(kr[r] + j + 64 * u);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                acc[r] = fma(split_entry(kk[r][u].x), vv[u].x, acc[r]);
+                acc[r] = fma(split_entry(kk[r][u].y), vv[u].y, acc[r]);
+            }
+    }
+    for (; j < n2; j += 64) {
+        const double2 vv = v2[j];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const u32x2 kk = kr[r][j];
+            acc[r] = fma(split_entry(kk.x), vv.x, acc[r]);
+            acc[r] = fma(split_entry(kk.y), vv.y, acc[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        double a = acc[r];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        acc[r] = a * rs[row0 + r];
+    }
+}
+
+__global__ void __launch_bounds__(256) matvec_split_kernel(const uint32_t *__restrict__ Kp,
+                                                           const double *__restrict__ rs, int m,
+                                                           int n, const double *__restrict__ v,
+                                                           double *__restrict__ t)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= m) return;
+    double acc[1];
+    wave_rows_dot_split<1>(Kp, rs, row, n, v, lane, acc);
+    if (lane == 0) t[row] = acc[0];
+}
+
+void launch_matvec_split(hipStream_t st, const uint32_t *Kp, const double *rs, int m, int n,
+                         const double *v, double *t)
+{
+    hipLaunchKernelGGL(matvec_split_kernel, dim3((m + 3) / 4), dim3(256), 0, st, Kp, rs, m, n, v, t);
 }
 
 // generic t = K v (m rows), one wavefront per row
@@ -209,20 +292,43 @@ void launch_prepass(hipStream_t st, const PlainDims &d, const Friction *fr, cons
                        dydt, v, vmax_out);
 }
 
+// matvec modes of the RHS: binary64 K v; binary64 K v kept as t0 (step start
+// of the mixed path); split K v kept as t0 (step start of the split path);
+// t0 + split K dv (stages of both reduced paths, v holding v - v0)
+enum { DOT_F64 = 0, DOT_F64_T0 = 1, DOT_SPLIT_T0 = 2, DOT_SPLIT_INC = 3 };
+
+template <int NR, int MODE>
+__device__ __forceinline__ void elem_rows(const KOperand &K, int64_t ro, int n,
+                                          const double *__restrict__ v, double *t0, int lane,
+                                          double acc[NR])
+{
+    if (MODE <= DOT_F64_T0) wave_rows_dot<NR>(K.K, ro, n, v, lane, acc);
+    else wave_rows_dot_split<NR>(K.Kp, K.rs, ro, n, v, lane, acc);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (MODE == DOT_F64_T0 || MODE == DOT_SPLIT_T0) t0[ro + r] = acc[r];
+            if (MODE == DOT_SPLIT_INC) acc[r] += t0[ro + r];
+        }
+    }
+}
+
 // K v for the rows of element e, then that element's post-pass.  One
 // wavefront per element; 4 elements per 256-thread block.
-__device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const double *__restrict__ K,
+template <int MODE>
+__device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const KOperand &K,
                                                const Friction *__restrict__ fr,
                                                const double *__restrict__ y,
                                                const double *__restrict__ v,
                                                double *__restrict__ dydt, double *t_out, int e,
-                                               int lane, const double *yscal_h, double *yscal)
+                                               int lane, const double *yscal_h, double *yscal,
+                                               double *t0)
 {
     const int so = elem_state_off(d, e);
     const int64_t ro = elem_row_off(d, e);
     if (e < d.np) {
         double acc[3];
-        wave_rows_dot<3>(K, ro, d.n, v, lane, acc);
+        elem_rows<3, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
             if (t_out) { t_out[ro] = acc[0]; t_out[ro + 1] = acc[1]; t_out[ro + 2] = acc[2]; }
             patch_postpass(y + so, fr[e], acc, dydt + so);
@@ -234,7 +340,7 @@ __device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const double 
         }
     } else {
         double acc[6];
-        wave_rows_dot<6>(K, ro, d.n, v, lane, acc);
+        elem_rows<6, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
             for (int k = 0; k < 6; k++) {
                 if (t_out) t_out[ro + k] = acc[k];
@@ -258,7 +364,29 @@ __global__ void __launch_bounds__(256) gemv_post_kernel(PlainDims d, const doubl
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= d.np + d.nv) return;
-    elem_gemv_post(d, K, fr, y, v, dydt, t_out, e, lane, nullptr, nullptr);
+    const KOperand k{K, nullptr, nullptr, 0};
+    elem_gemv_post<DOT_F64>(d, k, fr, y, v, dydt, t_out, e, lane, nullptr, nullptr, nullptr);
+}
+
+// a stage RHS of the reduced paths alone (t0 + split K dv, post-pass), for timing
+__global__ void __launch_bounds__(256) gemv_post_inc_kernel(PlainDims d, KOperand K,
+                                                            const Friction *__restrict__ fr,
+                                                            const double *__restrict__ y,
+                                                            const double *__restrict__ v,
+                                                            double *__restrict__ dydt, double *t0)
+{
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= d.np + d.nv) return;
+    elem_gemv_post<DOT_SPLIT_INC>(d, K, fr, y, v, dydt, nullptr, e, lane, nullptr, nullptr, t0);
+}
+
+void launch_gemv_post_inc(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
+                          const double *y, const double *v, double *dydt, double *t0)
+{
+    const int ne = d.np + d.nv;
+    hipLaunchKernelGGL(gemv_post_inc_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, y, v,
+                       dydt, t0);
 }
 
 void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
@@ -288,6 +416,10 @@ __global__ void step_start_kernel(PlainDims d, const Friction *__restrict__ fr,
     } else {
         volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
     }
+    if (b.v0) {
+        const int nvk = e < d.np ? 2 : 6;
+        for (int k = 0; k < nvk; k++) b.v0[vo + k] = b.v[vo + k];
+    }
 }
 
 void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
@@ -299,7 +431,8 @@ void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, c
 }
 
 // step start: matvec + post-pass half, then yscal (ratestate.f90:315)
-__global__ void __launch_bounds__(256) step_rhs_kernel(PlainDims d, const double *__restrict__ K,
+template <int MODE>
+__global__ void __launch_bounds__(256) step_rhs_kernel(PlainDims d, KOperand K,
                                                        const Friction *__restrict__ fr,
                                                        RkBuffers b, Control *ctl)
 {
@@ -307,14 +440,19 @@ __global__ void __launch_bounds__(256) step_rhs_kernel(PlainDims d, const double
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= d.np + d.nv) return;
-    elem_gemv_post(d, K, fr, b.y, b.v, b.dydt, nullptr, e, lane, &ctl->h, b.yscal);
+    elem_gemv_post<MODE>(d, K, fr, b.y, b.v, b.dydt, nullptr, e, lane, &ctl->h, b.yscal, b.t0);
 }
 
-void launch_step_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+void launch_step_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
                      const RkBuffers &b, Control *ctl)
 {
-    const int ne = d.np + d.nv;
-    hipLaunchKernelGGL(step_rhs_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, b, ctl);
+    const dim3 g((d.np + d.nv + 3) / 4), t(256);
+    if (K.prec == UCY_PREC_SPLIT_BF16)
+        hipLaunchKernelGGL(step_rhs_kernel<DOT_SPLIT_T0>, g, t, 0, st, d, K, fr, b, ctl);
+    else if (K.prec == UCY_PREC_MIXED)
+        hipLaunchKernelGGL(step_rhs_kernel<DOT_F64_T0>, g, t, 0, st, d, K, fr, b, ctl);
+    else
+        hipLaunchKernelGGL(step_rhs_kernel<DOT_F64>, g, t, 0, st, d, K, fr, b, ctl);
 }
 
 // intermediate state of stage s (ode45.f90:137-180), BLAS-1 order kept:
@@ -369,6 +507,10 @@ __global__ void stage_kernel(PlainDims d, const Friction *__restrict__ fr,
     double *out = b.ak[stage];
     if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, b.v + vo);
     else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, b.v + vo);
+    if (b.v0) {
+        const int nvk = e < d.np ? 2 : 6;
+        for (int k = 0; k < nvk; k++) b.v[vo + k] -= b.v0[vo + k];
+    }
 }
 
 void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
@@ -379,7 +521,8 @@ void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const 
                        stage);
 }
 
-__global__ void __launch_bounds__(256) stage_rhs_kernel(PlainDims d, const double *__restrict__ K,
+template <int MODE>
+__global__ void __launch_bounds__(256) stage_rhs_kernel(PlainDims d, KOperand K,
                                                         const Friction *__restrict__ fr,
                                                         RkBuffers b, Control *ctl, int stage)
 {
@@ -387,15 +530,18 @@ __global__ void __launch_bounds__(256) stage_rhs_kernel(PlainDims d, const doubl
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= d.np + d.nv) return;
-    elem_gemv_post(d, K, fr, b.ytmp, b.v, b.ak[stage], nullptr, e, lane, nullptr, nullptr);
+    elem_gemv_post<MODE>(d, K, fr, b.ytmp, b.v, b.ak[stage], nullptr, e, lane, nullptr, nullptr,
+                         b.t0);
 }
 
-void launch_stage_rhs(hipStream_t st, const PlainDims &d, const double *K, const Friction *fr,
+void launch_stage_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
                       const RkBuffers &b, Control *ctl, int stage)
 {
-    const int ne = d.np + d.nv;
-    hipLaunchKernelGGL(stage_rhs_kernel, dim3((ne + 3) / 4), dim3(256), 0, st, d, K, fr, b, ctl,
-                       stage);
+    const dim3 g((d.np + d.nv + 3) / 4), t(256);
+    if (K.prec == UCY_PREC_F64)
+        hipLaunchKernelGGL(stage_rhs_kernel<DOT_F64>, g, t, 0, st, d, K, fr, b, ctl, stage);
+    else
+        hipLaunchKernelGGL(stage_rhs_kernel<DOT_SPLIT_INC>, g, t, 0, st, d, K, fr, b, ctl, stage);
 }
 
 // 5th-order solution, embedded error and max |yerr/yscal| (ode45.f90:185-200, :75)

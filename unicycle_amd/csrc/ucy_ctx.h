@@ -36,9 +36,13 @@ struct ucy_ctx {
     std::vector<double> Vpl, rake, mu0, sig, a, b, Lc, Vo, damping;
     std::vector<double> gd0, npow, ebg, Gm, Qa, Rg, To;
     bool have_friction = false, have_rheology = false;
-    // device kernel
+    // device kernel: binary64 always; packed split-bf16 pairs and row scales
+    // when built in reduced precision (kprec)
     double *K = nullptr;
     int64_t m = 0, n = 0;
+    uint32_t *Kp = nullptr;
+    double *Krs = nullptr;
+    int kprec = 0;
     // wall seconds of the last ucy_build_kernel by source type (rectangle,
     // triangle, volume columns; each phase ends with a stream synchronize)
     double build_seconds[3] = {0, 0, 0};

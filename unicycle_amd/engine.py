@@ -151,7 +151,17 @@ class Engine:
 
     # -- assembly ---------------------------------------------------------
     def buildG(self, precision: int = PREC_F64):
+        """Assemble K (greens.f90:73-275) on the device.  PREC_SPLIT_BF16 /
+        PREC_MIXED also keep K as split-bf16 pairs, which integrate() then
+        streams (stage matvecs as increments over the step start)."""
         self._chk(self._L.ucy_build_kernel(self._h, precision))
+
+    def matvec_reduced(self, v):
+        """K~ v with the stored split-bf16 K (reduced-precision builds)."""
+        v = _f64(v)
+        t = np.empty(self.shape[0])
+        self._chk(self._L.ucy_matvec_reduced(self._h, _p(v), _p(t)))
+        return t
 
     def build_seconds(self):
         """Wall seconds of the last buildG by source type: (rectangles,
