@@ -19,20 +19,24 @@ Reported:
   value         sum over instances of the simulated time reached per wall
                 second, for the fastest batched variant that meets the
                 binary64 bar on this run (slip error <= 1e-6 at a fixed
-                pre-event time, first-event times within 0.1 %): split bf16,
-                mixed, or binary64 (which meets it by construction).
+                pre-event time, first-event times within 0.1 %): split bf16
+                with learned step-size proposals, split bf16, mixed, or
+                binary64 (which meets it by construction).
   batched_f64   binary64 K on v_mfma_f64_16x16x4f64; per-instance arithmetic
                 and step decisions are those of the plain path.
   plain_fp64    the single-instance device path (streaming binary64 GEMV per
                 instance), measured per attempt on instance 0 of every group
                 and costed over every instance's attempt count of the run.
   vs_plain_fp64 value / plain_fp64 on the same work.
-  mixed, split_bf16
+  mixed, split_bf16, split_bf16_learned
                 binary64 step-start matvec (mixed) or split bf16 throughout,
-                split-bf16 stage increments: throughput on the same attempt
-                budget, slip error against binary64 at a fixed pre-event time
-                (runs clamped to land on it), first-event times against
-                binary64 (runs stopped at each instance's first event).
+                split-bf16 stage increments; the last with the step-size
+                network (trained on instances drawn apart from the bench's:
+                plain binary64 histories, closed-loop rounds, output offset):
+                throughput on the same attempt budget, slip error against
+                binary64 at a fixed pre-event time (runs clamped to land on
+                it), first-event times against binary64 (runs stopped at each
+                instance's first event).
   roofline      the stage GEMM of the headline variant (dominant kernel),
                 timed with HIP events on the library stream: HBM-bound for
                 split bf16, MFMA-bound for binary64.
@@ -154,6 +158,50 @@ def event_error(ta, tb):
             "within_0.1pct": float(np.mean(r <= 1e-3))}
 
 
+TRAIN_SEED = 200   # draws of the network's training instances (the bench's are 100 + group)
+
+
+def train_step_net(eng, args):
+    """The step-size network for the headline ensemble, trained apart from
+    the bench's instances: the plain binary64 integrator's step histories of
+    16 instances of the 1024-patch geometry (draws seeded TRAIN_SEED), then
+    two closed-loop rounds and the output offset on an ensemble of 16
+    training instances of each of the four geometries in split bf16 (the
+    precision the network then serves)."""
+    from unicycle_amd import Engine, PREC_SPLIT_BF16, configs, stepnet
+    t0 = time.perf_counter()
+    nx, nz = configs.ENSEMBLE_GROUPS[0]
+    m = configs.thrust_fault(nx=nx, nz=nz, ellipse=None)
+    d = configs.ensemble_instances(m, 16, seed=TRAIN_SEED)
+    ic = Engine(0)
+    ic.set_elastic(m.lambda_, m.mu)
+    ic.set_rectangles(*m.geometry())
+    y0 = []
+    for i in range(16):
+        ic.set_friction(*configs.instance_friction(m, d, i))
+        y0.append(configs.perturbed_initial_state(m, ic.initStateVector(), int(d["seeds"][i])))
+    X, T = stepnet.training_data(ic, m, d, range(16), args.learned_steps, np.array(y0))
+    ic.close()
+    net = stepnet.StepNet(eng, seed=0)
+    net.fit_normalization(X, T)
+    loss = net.train(X, T, epochs=40, batch=4096, lr=2e-3, seed=1)
+    n_plain = X.shape[0]
+    tr, _ = configs.ensemble_setup(eng, PREC_SPLIT_BF16, n_inst=16, seed=TRAIN_SEED)
+    X, T = stepnet.fit_closed_loop(net, X, T, tr, args.learned_steps, rounds=2,
+                                   progress=_progress("train closed-loop"))
+    offset, tried = stepnet.calibrate_offset(net, tr, args.slip_years * configs.YEAR,
+                                             progress=_progress("train offset"))
+    tr.close()
+    info = {"plain_f64_samples": n_plain, "samples": int(X.shape[0]),
+            "training_instances": "16 per geometry, draws seeded apart from the bench's",
+            "loss_first_last": [float(loss[0]), float(loss[-1])], "closed_loop_rounds": 2,
+            "output_offset_log10": offset,
+            "offset_attempts": {str(k): (v if np.isfinite(v) else None) for k, v in tried.items()},
+            "train_s": time.perf_counter() - t0}
+    log(f"step-size network: {info}")
+    return net, info
+
+
 def bench_ensemble(args):
     from unicycle_amd import Engine, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, configs
     R = args.attempts
@@ -188,8 +236,15 @@ def bench_ensemble(args):
     ens.close()
 
     reduced = {}
-    for name, prec in (("mixed", PREC_MIXED), ("split_bf16", PREC_SPLIT_BF16)):
+    variants = [("mixed", PREC_MIXED, None), ("split_bf16", PREC_SPLIT_BF16, None)]
+    net = None
+    if not args.no_learned:
+        net, net_info = train_step_net(eng, args)
+        variants.append(("split_bf16_learned", PREC_SPLIT_BF16, net))
+    for name, prec, mlp in variants:
         ensr, _ = configs.ensemble_setup(eng, prec)
+        if mlp is not None:
+            mlp.attach(ensr)
         if args.warmup:
             run_attempts(ensr, min(R, 200), f"{name} warmup")
         t0 = time.perf_counter()
@@ -209,9 +264,15 @@ def bench_ensemble(args):
             "first_event_time_vs_f64": event_error(te64, ter),
             "stage_gemm_us": 1e6 * t_g, "stage_gemm_GBps": b_g / t_g / 1e9,
             "stage_gemm_hbm_frac": b_g / t_g / HBM_PEAK,
-            "stage_gemm_useful_TFLOPs": flops / t_g / 1e12}
+            "stage_gemm_useful_TFLOPs": flops / t_g / 1e12,
+            "rejected_fraction": 1.0 - str_.steps / max(1, str_.attempts)}
     reduced["mixed"]["dtype"] = "binary64 K for t_0 = K v_0, split-bf16 increments, f64 state"
     reduced["split_bf16"]["dtype"] = "split-bf16 K (t_0 and increments), f64 state"
+    if net is not None:
+        reduced["split_bf16_learned"]["dtype"] = ("split-bf16 K (t_0 and increments), f64 state, "
+                                                  "learned step-size proposals")
+        reduced["split_bf16_learned"]["training"] = net_info
+        net.close()
 
     # plain binary64 device path: instance 0 of each group, R attempts
     plain_time, plain_detail, cpu_inputs = 0.0, [], []
@@ -245,7 +306,7 @@ def bench_ensemble(args):
             "kernel": "ucy::gemm_f64_kernel (binary64 MFMA stage GEMM, all groups)",
             "avg_launch_us": 1e6 * t_gemm, "flops_per_launch": flops,
             "bytes_per_launch": gemm_bytes, "hbm_GBps": gemm_bytes / t_gemm / 1e9}
-    for name in ("mixed", "split_bf16"):
+    for name in reduced:
         r = reduced[name]
         ev = r["first_event_time_vs_f64"]
         ok = r["slip_error_vs_f64"] <= 1e-6 and ev.get("event_only_in_one", 0) == 0 and \
@@ -254,7 +315,9 @@ def bench_ensemble(args):
         if ok and r["value"] > head["value"]:
             head = {"name": name, "value": r["value"], "ms_per_step": 1e3 * r["wall_s"],
                     "dtype": "f64 state, split-bf16 K (fp32 MFMA accumulate)"
-                             + (" + binary64 step-start matvec" if name == "mixed" else ""),
+                             + (" + binary64 step-start matvec" if name == "mixed" else "")
+                             + (" + learned step proposals (fp32 MLP)" if "learned" in name
+                                else ""),
                     "desc": r["dtype"]}
             gb = r["stage_gemm_GBps"]
             roof = {"bound": "hbm", "achieved": gb, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -292,8 +355,7 @@ def bench_ensemble(args):
                        "method": "single-instance device path, per-attempt cost of instance 0 "
                                  "of each group times every instance's attempts",
                        "groups": plain_detail},
-        "mixed": reduced["mixed"],
-        "split_bf16": reduced["split_bf16"],
+        **reduced,
         "roofline": roof,
         "f64_gemm_roofline": {"bound": "mfma", "achieved": flops / t_gemm / 1e12,
                               "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s",

@@ -42,8 +42,9 @@ def _instances(model, n_inst, seed):
 
 @pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_F64, PREC_MIXED])
 def test_ensemble_matvec(engine_factory, precision):
-    """The standalone matvec uses the stage precision: split bf16 for
-    PREC_SPLIT_BF16 and PREC_MIXED, binary64 for PREC_F64."""
+    """The standalone matvec uses the stage precision (split bf16 for
+    PREC_SPLIT_BF16 and PREC_MIXED, binary64 for PREC_F64) and the stage GEMM
+    with the ensemble's tiling; every tiling variant is checked."""
     eng = engine_factory()
     groups = [configs.thrust_fault(nx=8, nz=8, ellipse=None),
               configs.thrust_fault(nx=16, nz=8, ellipse=None)]
@@ -54,20 +55,26 @@ def test_ensemble_matvec(engine_factory, precision):
         ens.add_group(model, draws["a"], draws["b"], draws["sig"], draws["L"], y0)
     ens.build()
     rng = np.random.default_rng(0)
-    for gi, model in enumerate(groups):
-        K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
-        n_inst = ens.groups[gi][1]
-        # slip-rate deviations spanning many decades, both signs
-        V = rng.standard_normal((K.shape[1], n_inst)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], n_inst))
-        T = ens.matvec(gi, V)
-        want = K @ V
-        scale = np.abs(K) @ np.abs(V)
-        err = np.abs(T - want)
-        if precision != PREC_F64:
-            assert np.all(err <= 2.0 ** -13 * scale), (err / scale).max()
-            assert (err / scale).max() > 1e-9   # really reduced precision
-        else:
-            assert np.all(err <= 1e-13 * scale)
+    Ks = [orc.buildG(orc.Mesh(m.lambda_, m.mu, rect=m.geometry()), nthreads=8) for m in groups]
+    # every GEMM variant of the autotuner, at split depths that give one and
+    # several k-splits (the groups have 8 and 16 k-blocks)
+    tilings = [(kb, 0) for kb in (4, 64)] if precision == PREC_F64 else \
+        [(kb, v) for v in range(5) for kb in (4, 64)]
+    for kb, v in tilings:
+        ens.set_tiling(kb, v)
+        for gi, K in enumerate(Ks):
+            n_inst = ens.groups[gi][1]
+            # slip-rate deviations spanning many decades, both signs
+            V = rng.standard_normal((K.shape[1], n_inst)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], n_inst))
+            T = ens.matvec(gi, V)
+            want = K @ V
+            scale = np.abs(K) @ np.abs(V)
+            err = np.abs(T - want)
+            if precision != PREC_F64:
+                assert np.all(err <= 2.0 ** -13 * scale), (kb, v, (err / scale).max())
+                assert (err / scale).max() > 1e-9   # really reduced precision
+            else:
+                assert np.all(err <= 1e-13 * scale), (kb, v)
 
 
 def _plain_engine(engine_factory, model, fr):

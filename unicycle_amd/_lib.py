@@ -128,10 +128,12 @@ _SIGS = {
     "ucy_ensemble_get_step_log": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, C.POINTER(C.c_longlong)]),
     "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
     "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), _pd]),
+    "ucy_ensemble_set_tiling": (_i, [_vp, _i, _i]),
     "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),
 }
 
 EXPORTED = tuple(_SIGS)
+GEMM_TUNE_ENTRIES = 25   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:

@@ -207,7 +207,10 @@ static void set_split(EnsTable &T, int kbps, int variant)
         const EnsGroup &G = T.g[gi];
         T.blk_gemm[gi] = acc;
         T.blk_gemm64[gi] = acc64;
-        acc += (((G.mp / 32 + rb - 1) / rb) * G.ksplit + 3) / 4;
+        const int nri = (G.mp / 32 + rb - 1) / rb;
+        // LDS variants: one block per (4 consecutive row items, k-split)
+        acc += ens_gemm_variant_lds(variant) ? ((nri + 3) / 4) * G.ksplit
+                                             : (nri * G.ksplit + 3) / 4;
         acc64 += ((G.mp / 64) * G.ksplit + 3) / 4;
     }
     T.blk_gemm[T.ngroups] = acc;
@@ -676,6 +679,28 @@ extern "C" int ucy_ensemble_get_state(ucy_ensemble *E, int group, double *y, dou
     return UCY_OK;
 }
 
+extern "C" int ucy_ensemble_set_tiling(ucy_ensemble *E, int kb_per_split, int variant)
+{
+    if (!E || kb_per_split <= 0 || variant < 0 || variant >= kGemmVariants) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    if (E->mode == 1 && variant != 0)
+        return efail(E, UCY_ERR_ARG, "the binary64 GEMM has one variant");
+    EnsTable T = E->tab;
+    set_split(T, kb_per_split, variant);
+    for (int gi = 0; gi < T.ngroups; gi++)
+        if (T.g[gi].ksplit > kMaxSplit)
+            return efail(E, UCY_ERR_LIMIT, "%d k-blocks per split needs more than %d splits",
+                         kb_per_split, kMaxSplit);
+    EHIP(hipSetDevice(E->ctx->device));
+    EHIP(hipStreamSynchronize(E->ctx->stream));
+    E->tab = T;
+    E->kb_per_split = kb_per_split;
+    E->variant = variant;
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);   // grid sizes change
+    E->graph = nullptr;
+    return upload_table(E);
+}
+
 extern "C" int ucy_ensemble_tuning(const ucy_ensemble *E, int *kb_per_split, int *variant,
                                    double *gemm_seconds)
 {
@@ -683,7 +708,8 @@ extern "C" int ucy_ensemble_tuning(const ucy_ensemble *E, int *kb_per_split, int
     if (kb_per_split) *kb_per_split = E->kb_per_split;
     if (variant) *variant = E->variant;
     if (gemm_seconds)
-        for (size_t i = 0; i < E->tune_seconds.size(); i++) gemm_seconds[i] = E->tune_seconds[i];
+        for (size_t i = 0; i < UCY_GEMM_TUNE_ENTRIES; i++)
+            gemm_seconds[i] = i < E->tune_seconds.size() ? E->tune_seconds[i] : -1.0;
     return UCY_OK;
 }
 

@@ -83,9 +83,12 @@ struct EnsTable {
 };
 
 // split-bf16 stage-GEMM variants of the autotuner: 0 = 64 rows per wave,
-// 1 = 64 rows with the next k-block's loads issued ahead, 2 = 128 rows, ahead
-constexpr int kGemmVariants = 3;
+// 1 = 64 rows with the next k-block's loads issued ahead, 2 = 128 rows, ahead,
+// 3 = 128 rows with V staged in LDS per block in chunks of 4 k-blocks,
+// 4 = 64 rows, ahead, V staged in chunks of 8 k-blocks
+constexpr int kGemmVariants = 5;
 int ens_gemm_variant_rows(int variant);
+int ens_gemm_variant_lds(int variant);
 
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,
@@ -105,6 +108,5 @@ void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &ht
 void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 void ens_launch_rearm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab);
 // standalone GEMM check: T(m x 64) = K V with the group's stage precision
-void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int grp);
 
 }  // namespace ucy

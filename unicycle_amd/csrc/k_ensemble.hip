@@ -274,12 +274,124 @@ __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int 
     else gemm_tile_bf16<2, PIPE>(G, rb0, sp, lane);
 }
 
-// split-bf16 GEMM variants of the autotuner: K rows per wave, pipelined loads
+// split-bf16 GEMM variants of the autotuner: K rows per wave, pipelined
+// loads, V staged in LDS (chunk of k-blocks, 0 = V read by every wave)
 struct Bf16Variant {
     int rb;
     bool pipe;
+    int lds_chunk;
 };
-constexpr Bf16Variant kBf16Variants[kGemmVariants] = {{2, false}, {2, true}, {4, true}};
+constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
+    {2, false, 0}, {2, true, 0}, {4, true, 0}, {4, false, 4}, {2, true, 8}};
+
+// LDS variants: the 4 waves of a block take 4 consecutive row items of one
+// k-split; the block stages the split's V tiles through LDS in chunks of CK
+// k-blocks (4 KiB each), double-buffered, one barrier per chunk, so V is read
+// from L2 once per block instead of once per wave.  Waves past the last row
+// item only help with the staging.  The last row item of a group may hold
+// two 32-row blocks (mp is a multiple of 64).
+template <int RB, bool PIPE, int CK>
+__device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, int sp, int wave,
+                                                    int lane, u32x4 *lds)
+{
+    const int nrb = G.mp >> 5;
+    const int nri = bf16_row_items(G.mp, RB);
+    const bool valid = ri < nri;
+    const int rb0 = ri * RB;
+    const int nvr = valid ? min(RB, nrb - rb0) : 0;   // RB, or 2 on the tail item
+    const int kb0 = sp * G.kb_per_split;
+    const int kb1 = min(kb0 + G.kb_per_split, G.nkb);
+    const int tid = wave * 64 + lane;
+    floatx16 acc[RB][2];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const u32x4 *A = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(valid ? rb0 : 0) * G.nkb * 2) * 64 + lane;
+    const size_t rstride = (size_t)G.nkb * 128;
+    const u32x4 *Vg = reinterpret_cast<const u32x4 *>(G.Vtiles);
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    // chunk 0 of V into buffer 0
+#pragma unroll
+    for (int u = 0; u < CK; u++) {
+        const int i = tid + 256 * u, kb = kb0 + (i >> 8);
+        lds[i] = kb < kb1 ? Vg[(size_t)kb * 256 + (i & 255)] : zero;
+    }
+    __syncthreads();
+    u32x4 ca[RB][2], na[RB][2], stg[CK];
+    auto loadK = [&](int kb, u32x4 (&ra)[RB][2]) {
+#pragma unroll
+        for (int a = 0; a < RB; a++)
+            if (a < nvr) {
+                ra[a][0] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128);
+                ra[a][1] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128 + 64);
+            }
+    };
+    if (valid) loadK(kb0, ca);
+    int buf = 0;
+    for (int kb = kb0; kb < kb1; kb++) {
+        const int k = (kb - kb0) % CK;
+        const int next = kb - k + CK;          // first k-block of the next chunk
+        if (k == 0 && next < kb1) {
+#pragma unroll
+            for (int u = 0; u < CK; u++) {
+                const int i = tid + 256 * u, kbn = next + (i >> 8);
+                stg[u] = kbn < kb1 ? Vg[(size_t)kbn * 256 + (i & 255)] : zero;
+            }
+        }
+        if (PIPE && valid && kb + 1 < kb1) loadK(kb + 1, na);
+        if (!PIPE && valid && kb > kb0) loadK(kb, ca);
+        if (valid) {
+            const u32x4 *Bl = lds + buf * (CK * 256) + k * 256 + lane;
+            const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, Bl[0]), __builtin_bit_cast(bf16x8, Bl[64])},
+                                    {__builtin_bit_cast(bf16x8, Bl[128]), __builtin_bit_cast(bf16x8, Bl[192])}};
+#pragma unroll
+            for (int a = 0; a < RB; a++) {
+                if (a >= nvr) continue;
+                const bf16x8 Ah = __builtin_bit_cast(bf16x8, ca[a][0]);
+                const bf16x8 Al = __builtin_bit_cast(bf16x8, ca[a][1]);
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
+                }
+            }
+            if (PIPE) {
+#pragma unroll
+                for (int a = 0; a < RB; a++) {
+                    ca[a][0] = na[a][0];
+                    ca[a][1] = na[a][1];
+                }
+            }
+        }
+        // end of a chunk with another to come: every wave has read this
+        // chunk's buffer; the other buffer (read two chunks ago, before the
+        // last barrier) takes the next chunk
+        if (k == CK - 1 && next < kb1) {
+#pragma unroll
+            for (int u = 0; u < CK; u++) lds[(buf ^ 1) * (CK * 256) + tid + 256 * u] = stg[u];
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    if (!valid) return;
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < RB; a++) {
+        if (a >= nvr) continue;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+    }
+}
 
 // binary64 batched matvec on v_mfma_f64_16x16x4f64: one wavefront = 64 rows
 // x 64 columns over one k-split (4 x 4 tiles of 16 x 16)
@@ -338,13 +450,28 @@ template <int V>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restrict__ tab, int stage)
 {
     constexpr int RB = kBf16Variants[V].rb;
+    constexpr int CK = kBf16Variants[V].lds_chunk;
     const int blk = blockIdx.x;
     const int g = group_of(tab->blk_gemm, tab->ngroups, blk);
     const EnsGroup &G = tab->g[g];
-    const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
-    if (item >= bf16_row_items(G.mp, RB) * G.ksplit) return;
-    if (stage == 1 && !*G.any_rhs) return;
-    gemm_wave_bf16<RB, kBf16Variants[V].pipe>(G, item, threadIdx.x & 63);
+    if (stage == 1 && !*G.any_rhs) return;   // block-uniform
+    if constexpr (CK > 0) {
+        __shared__ u32x4 lds[2 * (CK > 0 ? CK : 1) * 256];
+        const int nrg = (bf16_row_items(G.mp, RB) + 3) / 4;
+        const int local = blk - tab->blk_gemm[g];
+        const int wave = threadIdx.x >> 6;
+        gemm_block_bf16_lds<RB, kBf16Variants[V].pipe, CK>(G, 4 * (local % nrg) + wave, local / nrg,
+                                                           wave, threadIdx.x & 63, lds);
+    } else {
+        const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
+        if (item >= bf16_row_items(G.mp, RB) * G.ksplit) return;
+        gemm_wave_bf16<RB, kBf16Variants[V].pipe>(G, item, threadIdx.x & 63);
+    }
+}
+
+int ens_gemm_variant_lds(int v)
+{
+    return kBf16Variants[v].lds_chunk;
 }
 
 int ens_gemm_variant_rows(int v)
@@ -374,38 +501,15 @@ void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab,
     else if (htab.variant == 1)
         hipLaunchKernelGGL(gemm_bf16_kernel<1>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
                            st, dtab, stage);
-    else
+    else if (htab.variant == 2)
         hipLaunchKernelGGL(gemm_bf16_kernel<2>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
                            st, dtab, stage);
-}
-
-// one group's standalone matvec (stage 0)
-__global__ void __launch_bounds__(256) gemm_group_bf16_kernel(const EnsTable *__restrict__ tab, int g)
-{
-    const EnsGroup &G = tab->g[g];
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= bf16_row_items(G.mp, 2) * G.ksplit) return;
-    gemm_wave_bf16<2, false>(G, item, threadIdx.x & 63);
-}
-
-__global__ void __launch_bounds__(256) gemm_group_f64_kernel(const EnsTable *__restrict__ tab, int g)
-{
-    const EnsGroup &G = tab->g[g];
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (item >= (G.mp >> 6) * G.ksplit) return;
-    gemm_wave_f64(G, item, threadIdx.x & 63);
-}
-
-void ens_launch_gemm_group(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int g)
-{
-    const EnsGroup &G = htab.g[g];
-    if (gemm_f64_path(G.mode, 0)) {
-        const int items = (G.mp >> 6) * G.ksplit;
-        hipLaunchKernelGGL(gemm_group_f64_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
-    } else {
-        const int items = bf16_row_items(G.mp, 2) * G.ksplit;
-        hipLaunchKernelGGL(gemm_group_bf16_kernel, dim3((items + 3) / 4), dim3(256), 0, st, dtab, g);
-    }
+    else if (htab.variant == 3)
+        hipLaunchKernelGGL(gemm_bf16_kernel<3>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
+                           st, dtab, stage);
+    else
+        hipLaunchKernelGGL(gemm_bf16_kernel<4>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
+                           st, dtab, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -868,7 +972,9 @@ void ens_launch_matvec_group(hipStream_t st, const EnsTable *dtab, const EnsTabl
 {
     const EnsGroup &G = htab.g[g];
     hipLaunchKernelGGL(ens_fill_v_kernel, dim3((G.n * kEnsCols + 255) / 256), dim3(256), 0, st, dtab, g, V);
-    ens_launch_gemm_group(st, dtab, htab, g);
+    // the all-groups stage GEMM with the current tiling (the one the runs
+    // use); the other groups' partial products are scratch
+    ens_launch_gemm(st, dtab, htab, 0);
     hipLaunchKernelGGL(ens_reduce_t_kernel, dim3((G.m * G.ninst + 255) / 256), dim3(256), 0, st, dtab, g, T);
 }
 }  // namespace ucy

@@ -181,13 +181,18 @@ class Ensemble:
 
     def tuning(self):
         """(k-blocks per split, split-bf16 GEMM variant, seconds per launch
-        of the 15 candidates: k-blocks 32..512 of variant 0, 1, 2).  Variants:
+        of the candidates: k-blocks 32..512 of variant 0, 1, ...).  Variants:
         0 = 64 K rows per wave, 1 = 64 rows with loads issued a k-block
-        ahead, 2 = 128 rows, ahead."""
+        ahead, 2 = 128 rows, ahead, 3 = 128 rows with V staged in LDS in
+        chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged in chunks of 8."""
         kb, var = C.c_int(), C.c_int()
-        t = np.full(15, -1.0)
+        t = np.full(_lib.GEMM_TUNE_ENTRIES, -1.0)
         self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
         return kb.value, var.value, t
+
+    def set_tiling(self, kb_per_split: int, variant: int = 0):
+        """Override the autotuner's GEMM tiling."""
+        self._chk(self._L.ucy_ensemble_set_tiling(self._h, int(kb_per_split), int(variant)))
 
     def time_gemm(self, iters: int = 20):
         s, b = C.c_double(), C.c_double()

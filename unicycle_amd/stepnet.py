@@ -184,23 +184,40 @@ def features_from_logs(logs):
     return np.concatenate(Xs), np.concatenate(Ts)
 
 
-def closed_loop_data(net, ens, steps, group=0, progress=None):
-    """Step histories of an ensemble group over `steps` accepted steps per
-    instance from its initial states, with the network's proposals."""
-    ens.set_step_log(steps, group)
+def ensemble_histories(ens, steps, net=None, groups=None, progress=None):
+    """Step histories of the instances of an ensemble's groups (all by
+    default) over `steps` accepted steps per instance from their initial
+    states: under the network's proposals when `net` is given, else under the
+    classic rule."""
+    groups = list(range(len(ens.groups))) if groups is None else list(groups)
+    for g in groups:
+        ens.set_step_log(steps, g)
     ens.reset()
-    net.attach(ens)
+    if net is not None:
+        net.attach(ens)
+    else:
+        detach(ens)
+    ens.set_stop_at_end(False)
     ens.run(1e30, max_attempts=int(2 * steps), chunk_attempts=2000, progress=progress)
-    logs = ens.step_log(group)
-    ens.set_step_log(0, group)
-    return features_from_logs(logs)
+    Xs, Ts = [], []
+    for g in groups:
+        X, T = features_from_logs(ens.step_log(g))
+        Xs.append(X)
+        Ts.append(T)
+        ens.set_step_log(0, g)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def closed_loop_data(net, ens, steps, group=None, progress=None):
+    """Histories with the network's proposals (one group, or all)."""
+    return ensemble_histories(ens, steps, net, None if group is None else [group], progress)
 
 
 def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1e-3,
-                    seed=1, group=0, progress=None):
-    """Rounds of: run the training instances (ensemble `ens`, binary64)
-    under the current network, add their histories to (X, T), train on the
-    union.  Returns the grown data set."""
+                    seed=1, group=None, progress=None):
+    """Rounds of: run the training instances (ensemble `ens`) under the
+    current network, add their histories to (X, T), train on the union.
+    Returns the grown data set."""
     for r in range(rounds):
         Xr, Tr = closed_loop_data(net, ens, steps, group, progress)
         X, T = np.concatenate([X, Xr]), np.concatenate([T, Tr])
@@ -208,8 +225,7 @@ def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1
     return X, T
 
 
-def calibrate_offset(net, ens, window, offsets=(-0.15, -0.1, -0.05, 0.0, 0.05), group=0,
-                     progress=None):
+def calibrate_offset(net, ens, window, offsets=(-0.15, -0.1, -0.05, 0.0, 0.05), progress=None):
     """Shift of the network's output (log10 h) with the fewest attempts for
     the instances of `ens` to reach t = window (last step clamped to land
     on it); an offset that fails an instance is not eligible.  Leaves the
