@@ -274,12 +274,14 @@ int ucy_ensemble_status(ucy_ensemble *ens, int group, int *status);
 /* per-step log of a group's accepted steps (the ensemble form of the step
  * logs of ucy_integrate_ex; no counterpart in the reference): cap steps per
  * instance from the last reset, cap = 0 turns it off.  get_step_log writes
- * [ninst][cap] arrays (any may be null): time after the step, accepted h,
- * step-start vMax, errmax / epsilon, rejected attempts of the step; count[i]
- * = logged steps of instance i */
+ * [ninst][cap_out] arrays (any may be null; at most cap_out steps per
+ * instance): time after the step, accepted h, step-start vMax,
+ * errmax / epsilon, rejected attempts of the step; count[i] = steps written
+ * for instance i */
 int ucy_ensemble_set_step_log(ucy_ensemble *ens, int group, long long cap);
-int ucy_ensemble_get_step_log(ucy_ensemble *ens, int group, double *t, double *dt,
-                              double *vmax, double *errmax, double *rejects, long long *count);
+int ucy_ensemble_get_step_log(ucy_ensemble *ens, int group, long long cap_out, double *t,
+                              double *dt, double *vmax, double *errmax, double *rejects,
+                              long long *count);
 /* y: n_inst x (7*np) in the reference layout; per-instance time, next step,
  * accepted steps and attempts (any output may be NULL) */
 int ucy_ensemble_get_state(ucy_ensemble *ens, int group, double *y, double *time,

@@ -126,6 +126,29 @@ def test_ensemble_step_log(engine_factory):
     ens.close()
 
 
+def test_ensemble_step_log_per_group_capacity(engine_factory):
+    """Logs of two groups read after the first group's log is switched off:
+    each group's arrays follow its own capacity."""
+    from unicycle_amd import PREC_SPLIT_BF16
+    from unicycle_amd.ensemble import Ensemble
+    eng = engine_factory()
+    ens = Ensemble(eng, PREC_SPLIT_BF16)
+    for nx, nz, seed in ((8, 8, 3), (8, 4, 4)):
+        m, d, ic, y0 = _instances(engine_factory, nx, nz, seed)
+        ens.add_group(m, d["a"][:8], d["b"][:8], d["sig"][:8], d["L"][:8], y0[:8])
+    ens.build()
+    ens.set_step_log(200, 0)
+    ens.set_step_log(120, 1)
+    ens.reset()
+    ens.run(1e30, max_attempts=300)
+    l0 = ens.step_log(0)
+    ens.set_step_log(0, 0)
+    l1 = ens.step_log(1)
+    assert all(len(r["dt"]) == 200 for r in l0)
+    assert all(len(r["dt"]) == 120 for r in l1)
+    ens.close()
+
+
 def test_learned_proposal_gain_on_held_out_instances(engine_factory):
     """Train on the plain binary64 integrator's step logs of 16 seeded
     instances of a 256-patch fault, refine on the histories of the same

@@ -125,7 +125,8 @@ _SIGS = {
     "ucy_ensemble_first_events": (_i, [_vp, _i, _pd]),
     "ucy_ensemble_status": (_i, [_vp, _i, _pi]),
     "ucy_ensemble_set_step_log": (_i, [_vp, _i, C.c_longlong]),
-    "ucy_ensemble_get_step_log": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, C.POINTER(C.c_longlong)]),
+    "ucy_ensemble_get_step_log": (_i, [_vp, _i, C.c_longlong, _pd, _pd, _pd, _pd, _pd,
+                                       C.POINTER(C.c_longlong)]),
     "ucy_ensemble_matvec": (_i, [_vp, _i, _pd, _pd]),
     "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), _pd]),
     "ucy_ensemble_set_tiling": (_i, [_vp, _i, _i]),

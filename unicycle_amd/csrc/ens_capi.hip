@@ -601,11 +601,12 @@ extern "C" int ucy_ensemble_set_step_log(ucy_ensemble *E, int group, long long c
     return UCY_OK;
 }
 
-extern "C" int ucy_ensemble_get_step_log(ucy_ensemble *E, int group, double *t, double *dt,
-                                         double *vmax, double *errmax, double *rejects,
-                                         long long *count)
+extern "C" int ucy_ensemble_get_step_log(ucy_ensemble *E, int group, long long cap_out,
+                                         double *t, double *dt, double *vmax, double *errmax,
+                                         double *rejects, long long *count)
 {
-    if (!E || group < 0 || group >= (int)E->hg.size() || !count) return UCY_ERR_ARG;
+    if (!E || group < 0 || group >= (int)E->hg.size() || !count || cap_out < 0)
+        return UCY_ERR_ARG;
     if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
     const EnsGroup &G = E->tab.g[group];
     if (!G.slog) return efail(E, UCY_ERR_STATE, "no step log on group %d", group);
@@ -619,11 +620,11 @@ extern "C" int ucy_ensemble_get_step_log(ucy_ensemble *E, int group, double *t, 
     EHIP(hipMemcpy(buf.data(), G.slog, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
     double *out[5] = {t, dt, vmax, errmax, rejects};
     for (int i = 0; i < ni; i++) {
-        count[i] = std::min<long long>(steps[i], (long long)cap);
+        count[i] = std::min<long long>(std::min<long long>(steps[i], (long long)cap), cap_out);
         for (int f = 0; f < 5; f++) {
             if (!out[f]) continue;
             for (size_t k = 0; k < (size_t)count[i]; k++)
-                out[f][(size_t)i * cap + k] = buf[(k * 5 + f) * kEnsCols + i];
+                out[f][(size_t)i * cap_out + k] = buf[(k * 5 + f) * kEnsCols + i];
         }
     }
     return UCY_OK;

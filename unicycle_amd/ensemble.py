@@ -44,7 +44,7 @@ class Ensemble:
         if not self._h:
             raise UcyError(-1, "cannot create ensemble")
         self.groups = []
-        self._log_cap = 0
+        self._log_cap = {}
         engine._ensembles.append(self)
 
     def close(self):
@@ -145,18 +145,19 @@ class Ensemble:
         (cap = 0: off), on one group or all."""
         for g in ([group] if group is not None else range(len(self.groups))):
             self._chk(self._L.ucy_ensemble_set_step_log(self._h, g, int(cap)))
-        self._log_cap = int(cap)
+            self._log_cap[g] = int(cap)
 
     def step_log(self, group: int) -> list[dict]:
         """Per instance: arrays t (after the step), dt, vmax (step start),
         errmax (/ epsilon) and rejects of the logged steps."""
         _, n_inst = self.groups[group]
-        cap = self._log_cap
+        cap = self._log_cap.get(group, 0)
         arrs = [np.zeros((n_inst, cap)) for _ in range(5)]
         cnt = np.zeros(n_inst, dtype=np.int64)
         P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
         self._chk(self._L.ucy_ensemble_get_step_log(
-            self._h, group, *[P(a) for a in arrs], cnt.ctypes.data_as(C.POINTER(C.c_longlong))))
+            self._h, group, cap, *[P(a) for a in arrs],
+            cnt.ctypes.data_as(C.POINTER(C.c_longlong))))
         names = ("t", "dt", "vmax", "errmax", "rejects")
         return [{k: a[i, :cnt[i]].copy() for k, a in zip(names, arrs)} for i in range(n_inst)]
 

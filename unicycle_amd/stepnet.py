@@ -225,7 +225,8 @@ def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1
     return X, T
 
 
-def calibrate_offset(net, ens, window, offsets=(-0.15, -0.1, -0.05, 0.0, 0.05), progress=None):
+def calibrate_offset(net, ens, window, offsets=(-0.1, -0.05, 0.0, 0.05, 0.1, 0.15, 0.2, 0.3),
+                     progress=None):
     """Shift of the network's output (log10 h) with the fewest attempts for
     the instances of `ens` to reach t = window (last step clamped to land
     on it); an offset that fails an instance is not eligible.  Leaves the
