@@ -85,6 +85,16 @@ int ucy_nikkhoo15_stress_kernels(ucy_ctx *ctx, const double x[3], const double s
                                  double s, double d, double t, double G, double lambda,
                                  double *s11, double *s12, double *s13, double *s22,
                                  double *s23, double *s33);
+/* straininhalfspace_ (StrainInHalfSpace.c:42-173), batched over n
+ * (observation point, triangle) pairs: point i (X[i], Y[i], Z[i]) and
+ * triangle i (P1, P2, P3 as n x 3, row i = one vertex), all east/north/up
+ * with Z <= 0; TDCS slip (Ss, Ds, Ts) and moduli as the reference takes them
+ * by value.  stress and strain are n x 6 in the reference's order (xx, xy,
+ * xz, yy, yz, zz); NaN where the reference yields NaN (point on a side). */
+int ucy_straininhalfspace(ucy_ctx *ctx, int n, double *stress, double *strain, const double *X,
+                          const double *Y, const double *Z, const double *P1, const double *P2,
+                          const double *P3, double Ss, double Ds, double Ts, double mu,
+                          double lambda);
 /* Replaces strainvolume.f90:4671-4684 computeTractionKernelsVerticalStrainVolume. */
 int ucy_strainvolume_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
                                       const double dv[3], const double nv[3], int ns,

@@ -63,6 +63,41 @@ def test_nikkhoo_hooks_vs_reference_and_oracle(engine_factory):
             k += 1
 
 
+def test_straininhalfspace_vs_reference(engine_factory):
+    """The straininhalfspace_ entry point against the reference's own
+    outputs (golden fixture: 300 random triangles and points, unit strike,
+    dip and opening slip) with the oracle's bounds (strain by exact
+    differentiation vs the reference's hand-derived strain: relative to the
+    largest component of each tensor), and against the oracle itself."""
+    eng = engine_factory()
+    g = dict(np.load(os.path.join(GOLD, "nikkhoo_strain.npz")))
+    rels, rels_orc = [], []
+    for s in range(3):
+        b = [0.0, 0.0, 0.0]
+        b[s] = 1.0
+        stress, strain = eng.straininhalfspace(g["points"], g["P1"], g["P2"], g["P3"], *b,
+                                               30e3, 30e3)
+        for i in range(stress.shape[0]):
+            X, Y, Z = g["points"][i]
+            sg, e = np.zeros(6), np.zeros(6)
+            orc.oracle().orc_strain_in_halfspace(
+                orc.p(sg), orc.p(e), orc.D(X), orc.D(Y), orc.D(Z), orc.p(orc.f64(g["P1"][i])),
+                orc.p(orc.f64(g["P2"][i])), orc.p(orc.f64(g["P3"][i])), *[orc.D(v) for v in b],
+                orc.D(30e3), orc.D(30e3))
+            for got, want, mine in ((stress[i], g["stress"][i, s], sg),
+                                    (strain[i], g["strain"][i, s], e)):
+                assert np.array_equal(np.isnan(got), np.isnan(want)), (i, s)
+                if np.isnan(want).any():
+                    continue
+                rels.append(np.abs(got - want).max() / np.abs(want).max())
+                rels_orc.append(np.abs(got - mine).max() / np.abs(mine).max())
+    rels, rels_orc = np.array(rels), np.array(rels_orc)
+    assert np.median(rels) < 1e-12
+    assert np.quantile(rels, 0.99) < 1e-10
+    assert rels.max() < 1e-8
+    assert rels_orc.max() < 1e-10
+
+
 def test_nikkhoo_edge_cases(engine_factory):
     """Surface triangle (image flip of the reference), horizontal buried
     triangle (strike fallback), receiver on a side (NaN), receiver on the

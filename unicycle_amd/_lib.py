@@ -70,6 +70,8 @@ _SIGS = {
                                             _d, _d, _d, _d, _d, _pd, _pd, _pd]),
     "ucy_nikkhoo15_stress_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pi, _pi, _pi, _i, _pd,
                                           _d, _d, _d, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "ucy_straininhalfspace": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _d, _d,
+                                   _d, _d]),
     "ucy_strainvolume_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _pd,
                                                _pd, _d, _d, _d, _d, _d, _d, _d, _d, _pd, _pd,
                                                _pd]),

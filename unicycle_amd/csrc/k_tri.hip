@@ -304,9 +304,98 @@ int ext_build_triangle_columns(hipStream_t st, double G, double lambda, int ntri
     return 0;
 }
 
+// straininhalfspace_ (StrainInHalfSpace.c:42-173) over n (point, triangle)
+// pairs, in the reference's east/north/up frame: main + image + harmonic
+// parts, each as stress and strain, with the image's yz and zz components of
+// both negated for a triangle with every vertex at the surface (:162-165; this
+// makes the stress of such a triangle differ from Hooke's law applied to its
+// strain, as in the reference).  Direction 0 of the source carries the slip.
+__global__ void __launch_bounds__(128) strain_pairs_kernel(TriMedium M, const TriSource *__restrict__ S,
+                                                           const double *__restrict__ X,
+                                                           const double *__restrict__ Y,
+                                                           const double *__restrict__ Z, int n,
+                                                           double *__restrict__ stress,
+                                                           double *__restrict__ strain)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const TriSource &T = S[i];
+    const G3 Xg = gmk(X[i], 1, 0, 0), Yg = gmk(Y[i], 0, 1, 0), Zg = gmk(Z[i], 0, 0, 1);
+    G3 us[3], ud[3];
+    double e[3][6], sg[3][6];
+    td_full2(T.full[0], Xg, Yg, Zg, M.nu, T.btd[0], T.btd[1], us, ud);   // main
+    sym_strain(us, e[0]);
+    hooke6(e[0], M.mu, M.lambda, sg[0]);
+    td_full2(T.full[1], Xg, Yg, Zg, M.nu, T.btd[0], T.btd[1], us, ud);   // image
+    sym_strain(us, e[1]);
+    hooke6(e[1], M.mu, M.lambda, sg[1]);
+    if (T.surface) {
+        e[1][4] = -e[1][4];
+        e[1][5] = -e[1][5];
+        sg[1][4] = -sg[1][4];
+        sg[1][5] = -sg[1][5];
+    }
+    td_fsc2(T, Xg, Yg, Zg, M.nu, us, ud);                                 // harmonic
+    sym_strain(us, e[2]);
+    hooke6(e[2], M.mu, M.lambda, sg[2]);
+    for (int k = 0; k < 6; k++) {
+        stress[6 * (size_t)i + k] = sg[0][k] + sg[1][k] + sg[2][k];
+        strain[6 * (size_t)i + k] = e[0][k] + e[1][k] + e[2][k];
+    }
+}
+
 }  // namespace ucy
 
 using namespace ucy;
+
+extern "C" int ucy_straininhalfspace(ucy_ctx *c, int n, double *stress, double *strain,
+                                     const double *X, const double *Y, const double *Z,
+                                     const double *P1, const double *P2, const double *P3,
+                                     double Ss, double Ds, double Ts, double mu, double lambda)
+{
+    if (!c || n < 0 || (n > 0 && (!stress || !strain || !X || !Y || !Z || !P1 || !P2 || !P3)))
+        return UCY_ERR_ARG;
+    if (n == 0) return UCY_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return UCY_ERR_HIP;
+    // the reference takes ENU vertices; host_tri takes NED ones and converts
+    // them back, and its general slip (s, d, t) enters as TDCS (t, s, -d)
+    std::vector<TriSource> S(n);
+    for (int i = 0; i < n; i++) {
+        const double *p[3] = {P1 + 3 * (size_t)i, P2 + 3 * (size_t)i, P3 + 3 * (size_t)i};
+        double q[3][3];
+        for (int k = 0; k < 3; k++) {
+            q[k][0] = p[k][1];
+            q[k][1] = p[k][0];
+            q[k][2] = -p[k][2];
+        }
+        S[i] = host_tri(q[0], q[1], q[2], true, Ss, -Ds, Ts);
+    }
+    TriSource *dS = nullptr;
+    double *dbuf = nullptr;
+    const size_t nn = (size_t)n;
+    hipError_t e = hipMalloc((void **)&dS, sizeof(TriSource) * nn);
+    if (e == hipSuccess) e = hipMalloc((void **)&dbuf, sizeof(double) * 15 * nn);
+    if (e == hipSuccess) e = hipMemcpyAsync(dS, S.data(), sizeof(TriSource) * nn, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbuf, X, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbuf + nn, Y, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbuf + 2 * nn, Z, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(strain_pairs_kernel, dim3((n + 127) / 128), dim3(128), 0, c->stream,
+                           tri_medium(mu, lambda), dS, dbuf, dbuf + nn, dbuf + 2 * nn, n,
+                           dbuf + 3 * nn, dbuf + 9 * nn);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(stress, dbuf + 3 * nn, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(strain, dbuf + 9 * nn, sizeof(double) * 6 * nn, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dS);
+    (void)hipFree(dbuf);
+    if (e != hipSuccess) {
+        c->err = std::string("straininhalfspace: ") + hipGetErrorString(e);
+        return UCY_ERR_HIP;
+    }
+    return UCY_OK;
+}
 
 static int tri_hook(ucy_ctx *c, int kind, const double x[3], const double sv[3],
                     const double dv[3], const double nv[3], int ns, const int *i1, const int *i2,

@@ -289,6 +289,20 @@ class Engine:
             float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
         return tuple(out)
 
+    def straininhalfspace(self, points, P1, P2, P3, Ss, Ds, Ts, mu, lambda_):
+        """straininhalfspace_ (StrainInHalfSpace.c:42) for n (point, triangle)
+        pairs, east/north/up: returns (stress, strain), each n x 6
+        (xx, xy, xz, yy, yz, zz)."""
+        pts = _f64(points).reshape(-1, 3)
+        n = pts.shape[0]
+        X, Y, Z = (np.ascontiguousarray(pts[:, k]) for k in range(3))
+        Ps = [_f64(P).reshape(n, 3) for P in (P1, P2, P3)]
+        stress, strain = np.empty((n, 6)), np.empty((n, 6))
+        self._chk(self._L.ucy_straininhalfspace(self._h, n, _p(stress), _p(strain), _p(X), _p(Y),
+                                                _p(Z), *[_p(P) for P in Ps], float(Ss), float(Ds),
+                                                float(Ts), float(mu), float(lambda_)))
+        return stress, strain
+
     def computeStressKernelsNikkhoo15(self, x, sv, dv, nv, ns, i1, i2, i3, nVe, vertices,
                                       s, d, t, G, lambda_):
         idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
