@@ -604,7 +604,7 @@ static int ensure_work(ucy_ctx *c)
     free_work(c);
     const size_t nvec = 11;
     const size_t bytes =
-        sizeof(double) * (nvec * (size_t)d.dim + 2 * ((size_t)d.n + (size_t)d.m) + 64);
+        sizeof(double) * (nvec * (size_t)d.dim + 3 * (size_t)d.n + 2 * (size_t)d.m + 64);
     HIPCHK(hipMalloc((void **)&c->work, bytes));
     HIPCHK(hipMemsetAsync(c->work, 0, bytes, c->stream));
     double *p = c->work;
@@ -622,6 +622,7 @@ static int ensure_work(ucy_ctx *c)
     b.t = p; p += d.m;
     b.v0 = p; p += d.n;
     b.t0 = p; p += d.m;
+    b.v2 = p; p += d.n;
     if (c->kprec == UCY_PREC_F64) b.v0 = b.t0 = nullptr;
     b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
     c->work_dim = d.dim;
@@ -824,15 +825,10 @@ static int capture_attempt(ucy_ctx *c)
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const KOperand K{c->K, c->Kp, c->Krs, c->kprec};
-    launch_step_start(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl);
-    launch_step_rhs(c->stream, d, K, c->d_fr, b, c->d_ctl);
-    for (int s = 2; s <= 6; s++) {
-        launch_stage(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl, s);
-        launch_stage_rhs(c->stream, d, K, c->d_fr, b, c->d_ctl, s);
-    }
-    launch_finish(c->stream, d, b, c->d_ctl);
+    launch_start(c->stream, d, c->d_fr, c->d_rh, b, c->d_ctl);
+    for (int s = 1; s <= 5; s++) launch_rhs_stage(c->stream, d, K, c->d_fr, c->d_rh, b, c->d_ctl, s);
+    launch_rhs_finish(c->stream, d, K, c->d_fr, b, c->d_ctl);
     launch_controller(c->stream, b, c->d_ctl);
-    launch_commit(c->stream, d.dim, b, c->d_ctl);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -906,6 +902,10 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
         if (h.done) break;
         if (batch < 256) batch *= 2;
     }
+    // the last accepted step (later attempts commit at their start)
+    launch_commit(c->stream, d.dim, b, c->d_ctl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
     HIPCHK(hipMemcpyAsync(y, b.y, sizeof(double) * d.dim, hipMemcpyDeviceToHost, c->stream));
     if (cap > 0 && h.log_count > 0) {

@@ -316,7 +316,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         // ---- state and control
         const size_t S7 = (size_t)7 * G.N * kEnsCols;
         double **vecs[] = {&G.y, &G.dydt, &G.yscal, &G.ytmp, &G.ak[2], &G.ak[3], &G.ak[4],
-                           &G.ak[5], &G.ak[6], &G.ycand, &G.yerr};
+                           &G.ak[5], &G.ak[6], &G.ycand};
         for (double **v : vecs)
             if (ealloc(E, v, S7)) return UCY_ERR_HIP;
         G.ak[0] = G.ak[1] = G.dydt;

@@ -48,7 +48,7 @@ struct EnsGroup {
     double *t0;              // [mp][64] tractions at the step start
     int *any_rhs;            // some instance of the group starts a step
     // state (SoA, 7 components)
-    double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr;
+    double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand;   // the embedded error is only reduced
     EnsFriction fr;
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;

@@ -621,14 +621,27 @@ __device__ __forceinline__ void commit_elem(const EnsGroup &G, int e, int inst)
 __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
-    if (!elem_coords(tab, g, e, inst)) return;
+    const bool live = elem_coords(tab, g, e, inst);
     const EnsGroup &G = tab->g[g];
-    commit_elem(G, e, inst);
-    if (G.done[inst] || !G.need_rhs[inst]) return;
-    *G.any_rhs = 1;
-    const double vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
-    atomicMax(reinterpret_cast<unsigned long long *>(&G.vmax[inst]),
-              (unsigned long long)__double_as_longlong(vel));
+    double vel = 0.0;
+    if (live) {
+        commit_elem(G, e, inst);
+        if (!G.done[inst] && G.need_rhs[inst]) {
+            *G.any_rhs = 1;
+            vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
+        }
+    }
+    // vMax per instance: the 4 elements of this block share instance
+    // columns, so reduce across the waves first (one atomic per block and
+    // instance; slip rates are positive, so their bit patterns order as they do)
+    __shared__ unsigned long long red[256];
+    red[threadIdx.x] = (unsigned long long)__double_as_longlong(vel);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        unsigned long long b = red[threadIdx.x];
+        for (int w = 1; w < 4; w++) b = red[threadIdx.x + 64 * w] > b ? red[threadIdx.x + 64 * w] : b;
+        if (inst < G.ninst && b) atomicMax(reinterpret_cast<unsigned long long *>(&G.vmax[inst]), b);
+    }
 }
 
 // post-pass (ratestate.f90:747-771) writing stage derivative `stage`
@@ -749,7 +762,6 @@ __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst
         er = er + DC5 * k5;
         er = er + DC6 * k6;
         er = h * er;
-        G.yerr[i] = er;
         const double r = fabs(er / G.yscal[i]);
         emax = (r > emax || r != r) ? r : emax;
     }

@@ -398,17 +398,32 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 }
 
 // ---------------------------------------------------------------------------
-// one RK45 attempt, split in graph-capturable kernels
+// one RK45 attempt as 8 graph-capturable kernels:
+//   start        commit of the previous accepted step (y = ycand,
+//                ode45.f90:103) and the step-start pre-pass at y
+//                (ratestate.f90:309, odefun :667-707), thread per element
+//   rhs_stage 1  K v + post-pass at y (dydt, yscal of ratestate.f90:315),
+//                then the input of stage 2 and its pre-pass, wave per element
+//   rhs_stage s  K v + post-pass of stage s (2..5), then stage s+1's input
+//   rhs_finish   K v + post-pass of stage 6, the 5th-order solution and the
+//                embedded error (ode45.f90:185-200), max |yerr/yscal| (:75)
+//   controller   RKQSm's decision and the main loop's time update
+// The slip-rate vector is double-buffered (odd stages read v, even stages
+// v2) because a wave writes the next stage's entries of its element while
+// other waves still read the current ones.  A run ends with one commit.
 // ---------------------------------------------------------------------------
-// step start: RHS at y (ratestate.f90:309) -- pre-pass half
-__global__ void step_start_kernel(PlainDims d, const Friction *__restrict__ fr,
-                                  const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
+__global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
+                             const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
 {
-    // ctl->vmax was cleared by the controller that accepted the previous step
-    if (ctl->done || !ctl->need_rhs) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
+    if (ctl->accepted) {
+        const int ns = e < d.np ? 7 : 12;
+        for (int k = 0; k < ns; k++) b.y[so + k] = b.ycand[so + k];
+    }
+    // ctl->vmax was cleared by the controller that accepted the previous step
+    if (ctl->done || !ctl->need_rhs) return;
     if (e < d.np) {
         // vMax of the step-start odefun (ratestate.f90:693), as exported every step
         const double vel = patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
@@ -422,37 +437,11 @@ __global__ void step_start_kernel(PlainDims d, const Friction *__restrict__ fr,
     }
 }
 
-void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
-                       const RkBuffers &b, Control *ctl)
+void launch_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                  const RkBuffers &b, Control *ctl)
 {
     const int ne = d.np + d.nv;
-    hipLaunchKernelGGL(step_start_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, b,
-                       ctl);
-}
-
-// step start: matvec + post-pass half, then yscal (ratestate.f90:315)
-template <int MODE>
-__global__ void __launch_bounds__(256) step_rhs_kernel(PlainDims d, KOperand K,
-                                                       const Friction *__restrict__ fr,
-                                                       RkBuffers b, Control *ctl)
-{
-    if (ctl->done || !ctl->need_rhs) return;
-    const int lane = threadIdx.x & 63;
-    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (e >= d.np + d.nv) return;
-    elem_gemv_post<MODE>(d, K, fr, b.y, b.v, b.dydt, nullptr, e, lane, &ctl->h, b.yscal, b.t0);
-}
-
-void launch_step_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
-                     const RkBuffers &b, Control *ctl)
-{
-    const dim3 g((d.np + d.nv + 3) / 4), t(256);
-    if (K.prec == UCY_PREC_SPLIT_BF16)
-        hipLaunchKernelGGL(step_rhs_kernel<DOT_SPLIT_T0>, g, t, 0, st, d, K, fr, b, ctl);
-    else if (K.prec == UCY_PREC_MIXED)
-        hipLaunchKernelGGL(step_rhs_kernel<DOT_F64_T0>, g, t, 0, st, d, K, fr, b, ctl);
-    else
-        hipLaunchKernelGGL(step_rhs_kernel<DOT_F64>, g, t, 0, st, d, K, fr, b, ctl);
+    hipLaunchKernelGGL(start_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, b, ctl);
 }
 
 // intermediate state of stage s (ode45.f90:137-180), BLAS-1 order kept:
@@ -489,14 +478,12 @@ __device__ __forceinline__ double stage_value(int stage, double h, double y0, do
     }
 }
 
-__global__ void stage_kernel(PlainDims d, const Friction *__restrict__ fr,
-                             const Rheology *__restrict__ rh, RkBuffers b, Control *ctl, int stage)
+// the input of stage `stage` (2..6) for element e and its pre-pass; the
+// slip-rate entries go to vout (less v0 in the reduced paths)
+__device__ __forceinline__ void stage_elem(const PlainDims &d, const Friction *__restrict__ fr,
+                                           const Rheology *__restrict__ rh, const RkBuffers &b,
+                                           double h, int e, int stage, double *vout)
 {
-    if (ctl->done) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (stage == 2 && e == 0) ctl->errmax_bits = 0ull;
-    if (e >= d.np + d.nv) return;
-    const double h = ctl->h;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
     const int ns = e < d.np ? 7 : 12;
     for (int k = 0; k < ns; k++) {
@@ -505,84 +492,108 @@ __global__ void stage_kernel(PlainDims d, const Friction *__restrict__ fr,
                                 b.ak[5][i]);
     }
     double *out = b.ak[stage];
-    if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, b.v + vo);
-    else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, b.v + vo);
+    if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
     if (b.v0) {
         const int nvk = e < d.np ? 2 : 6;
-        for (int k = 0; k < nvk; k++) b.v[vo + k] -= b.v0[vo + k];
+        for (int k = 0; k < nvk; k++) vout[vo + k] -= b.v0[vo + k];
     }
 }
 
-void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
-                  const RkBuffers &b, Control *ctl, int stage)
-{
-    const int ne = d.np + d.nv;
-    hipLaunchKernelGGL(stage_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d, fr, rh, b, ctl,
-                       stage);
-}
-
 template <int MODE>
-__global__ void __launch_bounds__(256) stage_rhs_kernel(PlainDims d, KOperand K,
+__global__ void __launch_bounds__(256) rhs_stage_kernel(PlainDims d, KOperand K,
                                                         const Friction *__restrict__ fr,
+                                                        const Rheology *__restrict__ rh,
                                                         RkBuffers b, Control *ctl, int stage)
 {
     if (ctl->done) return;
     const int lane = threadIdx.x & 63;
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= d.np + d.nv) return;
-    elem_gemv_post<MODE>(d, K, fr, b.ytmp, b.v, b.ak[stage], nullptr, e, lane, nullptr, nullptr,
-                         b.t0);
+    const double *vin = (stage & 1) ? b.v : b.v2;
+    double *vout = (stage & 1) ? b.v2 : b.v;
+    if (stage == 1) {
+        // a retried attempt keeps the step-start derivative (the reference's
+        // RKQSm loop reuses dydt)
+        if (ctl->need_rhs)
+            elem_gemv_post<MODE>(d, K, fr, b.y, vin, b.dydt, nullptr, e, lane, &ctl->h, b.yscal,
+                                 b.t0);
+        if (lane == 0 && e == 0) ctl->errmax_bits = 0ull;
+    } else {
+        elem_gemv_post<MODE>(d, K, fr, b.ytmp, vin, b.ak[stage], nullptr, e, lane, nullptr,
+                             nullptr, b.t0);
+    }
+    if (lane == 0) stage_elem(d, fr, rh, b, ctl->h, e, stage + 1, vout);
 }
 
-void launch_stage_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
-                      const RkBuffers &b, Control *ctl, int stage)
+template <int MODE>
+__global__ void __launch_bounds__(256) rhs_finish_kernel(PlainDims d, KOperand K,
+                                                         const Friction *__restrict__ fr,
+                                                         RkBuffers b, Control *ctl)
+{
+    if (ctl->done) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int e = blockIdx.x * 4 + w;
+    const bool live = e < d.np + d.nv;
+    if (live)
+        elem_gemv_post<MODE>(d, K, fr, b.ytmp, b.v2, b.ak[6], nullptr, e, lane, nullptr, nullptr,
+                             b.t0);
+    double emax = 0.0;
+    if (live && lane == 0) {
+        const double h = ctl->h;
+        const int so = elem_state_off(d, e), ns = e < d.np ? 7 : 12;
+        for (int k = 0; k < ns; k++) {
+            const int i = so + k;
+            const double k1 = b.dydt[i], k3 = b.ak[3][i], k4 = b.ak[4][i], k5 = b.ak[5][i],
+                         k6 = b.ak[6][i];
+            double yr = C1 * k1;
+            yr = yr + C3 * k3;
+            yr = yr + C4 * k4;
+            yr = yr + C6 * k6;
+            b.ycand[i] = b.y[i] + h * yr;
+            double er = DC1 * k1;
+            er = er + DC3 * k3;
+            er = er + DC4 * k4;
+            er = er + DC5 * k5;
+            er = er + DC6 * k6;
+            er = h * er;
+            const double r = fabs(er / b.yscal[i]);
+            // non-finite ratios propagate as NaN bits, which order above +inf
+            emax = (r > emax || r != r) ? r : emax;
+        }
+    }
+    __shared__ unsigned long long red[4];
+    if (lane == 0) red[w] = dbits(emax);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+        for (int k = 1; k < 4; k++) m = red[k] > m ? red[k] : m;
+        atomicMax(&ctl->errmax_bits, m);
+    }
+}
+
+void launch_rhs_stage(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
+                      const Rheology *rh, const RkBuffers &b, Control *ctl, int stage)
+{
+    const dim3 g((d.np + d.nv + 3) / 4), t(256);
+    if (stage == 1 && K.prec == UCY_PREC_SPLIT_BF16)
+        hipLaunchKernelGGL(rhs_stage_kernel<DOT_SPLIT_T0>, g, t, 0, st, d, K, fr, rh, b, ctl, stage);
+    else if (stage == 1 && K.prec == UCY_PREC_MIXED)
+        hipLaunchKernelGGL(rhs_stage_kernel<DOT_F64_T0>, g, t, 0, st, d, K, fr, rh, b, ctl, stage);
+    else if (K.prec == UCY_PREC_F64)
+        hipLaunchKernelGGL(rhs_stage_kernel<DOT_F64>, g, t, 0, st, d, K, fr, rh, b, ctl, stage);
+    else
+        hipLaunchKernelGGL(rhs_stage_kernel<DOT_SPLIT_INC>, g, t, 0, st, d, K, fr, rh, b, ctl, stage);
+}
+
+void launch_rhs_finish(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
+                       const RkBuffers &b, Control *ctl)
 {
     const dim3 g((d.np + d.nv + 3) / 4), t(256);
     if (K.prec == UCY_PREC_F64)
-        hipLaunchKernelGGL(stage_rhs_kernel<DOT_F64>, g, t, 0, st, d, K, fr, b, ctl, stage);
+        hipLaunchKernelGGL(rhs_finish_kernel<DOT_F64>, g, t, 0, st, d, K, fr, b, ctl);
     else
-        hipLaunchKernelGGL(stage_rhs_kernel<DOT_SPLIT_INC>, g, t, 0, st, d, K, fr, b, ctl, stage);
-}
-
-// 5th-order solution, embedded error and max |yerr/yscal| (ode45.f90:185-200, :75)
-__global__ void __launch_bounds__(256) finish_kernel(int dim, RkBuffers b, Control *ctl)
-{
-    if (ctl->done) return;
-    const double h = ctl->h;
-    double emax = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += gridDim.x * blockDim.x) {
-        const double k1 = b.dydt[i], k3 = b.ak[3][i], k4 = b.ak[4][i], k5 = b.ak[5][i],
-                     k6 = b.ak[6][i];
-        double yr = C1 * k1;
-        yr = yr + C3 * k3;
-        yr = yr + C4 * k4;
-        yr = yr + C6 * k6;
-        b.ycand[i] = b.y[i] + h * yr;
-        double e = DC1 * k1;
-        e = e + DC3 * k3;
-        e = e + DC4 * k4;
-        e = e + DC5 * k5;
-        e = e + DC6 * k6;
-        e = h * e;
-        b.yerr[i] = e;
-        const double r = fabs(e / b.yscal[i]);
-        // non-finite ratios propagate as NaN bits, which order above +inf
-        emax = (r > emax || r != r) ? r : emax;
-    }
-    unsigned long long bits = dbits(emax);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(bits, off, 64);
-        bits = o > bits ? o : bits;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(&ctl->errmax_bits, bits);
-}
-
-void launch_finish(hipStream_t st, const PlainDims &d, const RkBuffers &b, Control *ctl)
-{
-    int blocks = (d.dim + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(finish_kernel, dim3(blocks), dim3(256), 0, st, d.dim, b, ctl);
+        hipLaunchKernelGGL(rhs_finish_kernel<DOT_SPLIT_INC>, g, t, 0, st, d, K, fr, b, ctl);
 }
 
 // step control of RKQSm (ode45.f90:70-105) and the time update / exit test of

@@ -70,14 +70,13 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 // integrator graph pieces
 struct RkBuffers {
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr, *yr, *v, *t;
+    double *v2;            // slip rates of the even stages (the attempt graph double-buffers v)
     // reduced-precision K (null in binary64): slip rates and tractions of the
     // step start; the stage matvecs then take v - v0 and add t0
     double *v0, *t0;
     double *tlog, *dtlog, *vlog;
     double *elog, *rlog;   // errmax/epsilon of the accepted attempt, rejections (optional)
 };
-void launch_step_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
-                       const RkBuffers &b, Control *ctl);
 // the stored K of a context: binary64 always; packed split-bf16 pairs
 // (hi << 16 | lo, row-major, row stride n) with power-of-two row scales when
 // built in reduced precision
@@ -93,13 +92,13 @@ void launch_matvec_split(hipStream_t st, const uint32_t *Kp, const double *rs, i
                          const double *v, double *t);
 void launch_gemv_post_inc(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
                           const double *y, const double *v, double *dydt, double *t0);
-void launch_step_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
-                     const RkBuffers &b, Control *ctl);
-void launch_stage(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
-                  const RkBuffers &b, Control *ctl, int stage);
-void launch_stage_rhs(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
-                      const RkBuffers &b, Control *ctl, int stage);
-void launch_finish(hipStream_t st, const PlainDims &d, const RkBuffers &b, Control *ctl);
+// the attempt graph (k_ode.hip): start, rhs_stage 1..5, rhs_finish, controller
+void launch_start(hipStream_t st, const PlainDims &d, const Friction *fr, const Rheology *rh,
+                  const RkBuffers &b, Control *ctl);
+void launch_rhs_stage(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
+                      const Rheology *rh, const RkBuffers &b, Control *ctl, int stage);
+void launch_rhs_finish(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
+                       const RkBuffers &b, Control *ctl);
 void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl);
 // standalone Cash-Karp combination kernels for the ucy_rkck entry point
 void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,

@@ -19,8 +19,8 @@ Trained on the classic controller's histories alone, the network meets
 states of its own making that those histories never show (a step the classic
 rule would not have taken); fit_closed_loop adds the histories of runs under
 the network's own proposals, labelled the same way (DAgger, Ross, Gordon and
-Bagnell 2011), and calibrate_offset picks the output offset that needs the
-fewest attempts over a window of the training instances.
+Bagnell 2011), and calibrate_offset picks the output offset that takes the
+training instances furthest in simulated time for a fixed attempt budget.
 """
 from __future__ import annotations
 
@@ -225,23 +225,24 @@ def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1
     return X, T
 
 
-def calibrate_offset(net, ens, window, offsets=(-0.1, -0.05, 0.0, 0.05, 0.1, 0.15, 0.2, 0.3),
+def calibrate_offset(net, ens, attempts, offsets=(-0.1, -0.05, 0.0, 0.05, 0.1, 0.15, 0.2, 0.3),
                      progress=None):
-    """Shift of the network's output (log10 h) with the fewest attempts for
-    the instances of `ens` to reach t = window (last step clamped to land
-    on it); an offset that fails an instance is not eligible.  Leaves the
-    chosen shift in the network's normalisation and returns
-    (offset, {offset: attempts})."""
+    """Shift of the network's output (log10 h) that takes the instances of
+    `ens` furthest in simulated time (summed over instances) within
+    `attempts` RK45 attempts each from their initial states, the cost of
+    every candidate being the same fixed budget; an offset that fails an
+    instance is not eligible.  Leaves the chosen shift in the network's
+    normalisation and returns (offset, {offset: simulated seconds})."""
     xm, xs, ym, ys = net.norm
     res = {}
     for d in offsets:
         net.set_normalization(xm, xs, ym + d, ys)
         ens.reset()
         net.attach(ens)
-        ens.set_stop_at_end(True)
-        st = ens.run(window, max_attempts=10**7, chunk_attempts=5000, progress=progress)
-        ok = st.n_failed == 0 and st.min_time >= window
-        res[d] = float(st.attempts) if ok else float("inf")
-    best = min(res, key=res.get)
+        ens.set_stop_at_end(False)
+        st = ens.run(1e30, max_attempts=int(attempts), chunk_attempts=max(1, int(attempts) // 2),
+                     progress=progress)
+        res[d] = float(st.inst_time_sum) if st.n_failed == 0 else -float("inf")
+    best = max(res, key=res.get)
     net.set_normalization(xm, xs, ym + best, ys)
     return best, res
