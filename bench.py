@@ -189,7 +189,7 @@ def train_step_net(eng, args):
     tr, _ = configs.ensemble_setup(eng, PREC_SPLIT_BF16, n_inst=16, seed=TRAIN_SEED)
     X, T = stepnet.fit_closed_loop(net, X, T, tr, args.learned_steps, rounds=2,
                                    progress=_progress("train closed-loop"))
-    offset, tried = stepnet.calibrate_offset(net, tr, max(200, args.attempts // 2),
+    offset, tried = stepnet.calibrate_offset(net, tr, attempts=max(200, args.attempts // 2),
                                              progress=_progress("train offset"))
     tr.close()
     info = {"plain_f64_samples": n_plain, "samples": int(X.shape[0]),
@@ -416,7 +416,7 @@ def bench_learned(args, plain_us_per_attempt):
     ens.build()
     X, Tt = stepnet.fit_closed_loop(net, X, Tt, ens, args.learned_steps, rounds=2,
                                     progress=_progress("closed-loop"))
-    offset, tried = stepnet.calibrate_offset(net, ens, 1500, progress=_progress("offset"))
+    offset, tried = stepnet.calibrate_offset(net, ens, window=0.5 * T, progress=_progress("offset"))
     ens.close()
     t_refine = time.perf_counter() - t0
     log(f"learned: closed-loop data {X.shape[0]} samples, offset {offset} from {tried}, "
@@ -453,8 +453,8 @@ def bench_learned(args, plain_us_per_attempt):
                         "train_s": t_train, "closed_loop_rounds": 2,
                         "samples_after_closed_loop": int(X.shape[0]),
                         "output_offset_log10": offset,
-                        "offset_sim_seconds": {str(k): (v if np.isfinite(v) else None)
-                                               for k, v in tried.items()},
+                        "offset_attempts_to_half_window": {
+                            str(k): (v if np.isfinite(v) else None) for k, v in tried.items()},
                         "refine_s": t_refine},
            "plain_fp64_classic_sim_yr_per_s": n_h * args.learned_years / plain_wall}
     for name, r in runs.items():

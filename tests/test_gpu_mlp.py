@@ -153,8 +153,8 @@ def test_learned_proposal_gain_on_held_out_instances(engine_factory):
     """Train on the plain binary64 integrator's step logs of 16 seeded
     instances of a 256-patch fault, refine on the histories of the same
     instances run under the network (two closed-loop rounds), pick the
-    output offset on those instances (simulated time reached in 1500
-    attempts), then run 48 held-out
+    output offset on those instances (fewest attempts to a 0.15-year
+    window), then run 48 held-out
     instances as an ensemble over a fixed simulated window (every instance
     clamped to land on it) with the classic RKQSm growth rule, with the
     network trained on the plain logs only, and with the refined network.
@@ -185,7 +185,7 @@ def test_learned_proposal_gain_on_held_out_instances(engine_factory):
             train = _group(eng, m, d, y0, list(range(16)), PREC_F64)
             X, T = stepnet.fit_closed_loop(net, X, T, train, 3000, rounds=2)
             lap("closed loop")
-            off, tried = stepnet.calibrate_offset(net, train, 1500)
+            off, tried = stepnet.calibrate_offset(net, train, window=0.15 * configs.YEAR)
             lap("offset")
             train.close()
         ens = _group(eng, m, d, y0, held, PREC_F64)

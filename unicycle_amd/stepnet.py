@@ -19,8 +19,9 @@ Trained on the classic controller's histories alone, the network meets
 states of its own making that those histories never show (a step the classic
 rule would not have taken); fit_closed_loop adds the histories of runs under
 the network's own proposals, labelled the same way (DAgger, Ross, Gordon and
-Bagnell 2011), and calibrate_offset picks the output offset that takes the
-training instances furthest in simulated time for a fixed attempt budget.
+Bagnell 2011), and calibrate_offset picks the output offset on the training
+instances: the furthest simulated time for a fixed attempt budget, or the
+fewest attempts to a fixed window.
 """
 from __future__ import annotations
 
@@ -225,24 +226,59 @@ def fit_closed_loop(net, X, T, ens, steps, rounds=2, epochs=40, batch=4096, lr=1
     return X, T
 
 
-def calibrate_offset(net, ens, attempts, offsets=(-0.1, -0.05, 0.0, 0.05, 0.1, 0.15, 0.2, 0.3),
-                     progress=None):
-    """Shift of the network's output (log10 h) that takes the instances of
-    `ens` furthest in simulated time (summed over instances) within
-    `attempts` RK45 attempts each from their initial states, the cost of
-    every candidate being the same fixed budget; an offset that fails an
-    instance is not eligible.  Leaves the chosen shift in the network's
-    normalisation and returns (offset, {offset: simulated seconds})."""
+def calibrate_offset(net, ens, attempts=None, window=None,
+                     offsets=(-0.1, -0.05, 0.0, 0.05, 0.1, 0.15, 0.2, 0.3), progress=None):
+    """Shift of the network's output (log10 h) picked on the instances of
+    `ens`, with one of two objectives:
+
+    attempts=R   the furthest summed simulated time within R RK45 attempts
+                 per instance from the initial states (every candidate costs
+                 the same fixed budget);
+    window=W     the fewest attempts for every instance to reach t = W (last
+                 step clamped to land on it).  Offsets are tried from 0
+                 outwards and a direction stops at the first worse one; every
+                 run is capped at twice the per-instance attempts of offset
+                 0, and a capped run is not eligible.
+
+    An offset that fails an instance is not eligible.  Leaves the chosen
+    shift in the network's normalisation and returns (offset, {offset:
+    score}) with score = simulated seconds (attempts=R) or attempts
+    (window=W)."""
+    assert (attempts is None) != (window is None), "one objective"
     xm, xs, ym, ys = net.norm
-    res = {}
-    for d in offsets:
+
+    def run(d, cap):
         net.set_normalization(xm, xs, ym + d, ys)
         ens.reset()
         net.attach(ens)
-        ens.set_stop_at_end(False)
-        st = ens.run(1e30, max_attempts=int(attempts), chunk_attempts=max(1, int(attempts) // 2),
-                     progress=progress)
-        res[d] = float(st.inst_time_sum) if st.n_failed == 0 else -float("inf")
+        if attempts is not None:
+            ens.set_stop_at_end(False)
+            st = ens.run(1e30, max_attempts=int(attempts), chunk_attempts=max(1, int(attempts) // 2),
+                         progress=progress)
+            return (float(st.inst_time_sum) if st.n_failed == 0 else -float("inf")), st
+        ens.set_stop_at_end(True)
+        st = ens.run(window, max_attempts=int(cap), chunk_attempts=5000, progress=progress)
+        ok = st.n_failed == 0 and st.min_time >= window
+        return (-float(st.attempts) if ok else -float("inf")), st
+
+    res = {}
+    if attempts is not None:
+        for d in offsets:
+            res[d] = run(d, None)[0]
+    else:
+        base, st0 = run(0.0, 10**7)
+        res[0.0] = base
+        most = max(int(np.max(ens.state(g)[4])) for g in range(len(ens.groups)))
+        cap = 2 * max(1, most)
+        for side in (sorted(d for d in offsets if d > 0), sorted((d for d in offsets if d < 0),
+                                                                 reverse=True)):
+            prev = base
+            for d in side:
+                res[d] = run(d, cap)[0]
+                if res[d] < prev:
+                    break
+                prev = res[d]
     best = max(res, key=res.get)
     net.set_normalization(xm, xs, ym + best, ys)
-    return best, res
+    score = {d: (v if attempts is not None else -v) for d, v in res.items()}
+    return best, score
