@@ -209,17 +209,27 @@ typedef struct {
 int ucy_integrate(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
                   double epsilon, double max_step, double *t_log, double *dt_log,
                   double *vmax_log, int64_t log_cap, ucy_run_stats *stats);
-/* The same main loop with the step history the learned step-size proposal is
- * trained on: per accepted step also errmax/epsilon of the accepted attempt
- * and the number of rejected attempts before it (errmax and rejects may be
- * NULL).  No counterpart in the reference. */
+/* The same main loop from time t0 (the run continues the reference's running
+ * sum time = time + dt_done, so a run split into calls accumulates time as one
+ * run does), with the step history the learned step-size proposal is trained
+ * on: per accepted step also errmax/epsilon of the accepted attempt and the
+ * number of rejected attempts before it (errmax and rejects may be NULL), and
+ * optionally observed state entries.  Logged times are absolute.  No
+ * counterpart in the reference. */
 typedef struct {
     double *t, *dt, *vmax, *errmax, *rejects;
     int64_t cap;
+    /* observed state entries, the per-iteration export of ratestate.f90:
+     * 488-536 (needs the step log above): n_obs indices into the state
+     * vector; per logged step their values at the step start (obs_y) and the
+     * step-start derivatives (obs_dydt), [cap][n_obs] each.  n_obs = 0: off */
+    int64_t n_obs;
+    const int64_t *obs_index;
+    double *obs_y, *obs_dydt;
 } ucy_step_log;
-int ucy_integrate_ex(ucy_ctx *ctx, double *y, double interval, double dt0, int64_t max_steps,
-                     double epsilon, double max_step, const ucy_step_log *log,
-                     ucy_run_stats *stats);
+int ucy_integrate_ex(ucy_ctx *ctx, double *y, double t0, double interval, double dt0,
+                     int64_t max_steps, double epsilon, double max_step,
+                     const ucy_step_log *log, ucy_run_stats *stats);
 
 /* Measurement hook (no counterpart in the reference): average duration in
  * seconds of the fused matvec + odefun post-pass kernel (the dominant kernel

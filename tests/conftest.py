@@ -15,12 +15,24 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libucy.so)")
 
 
+_GPU = None
+
+
 def gpu_available() -> bool:
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    """A context on device 0 through libucy itself (torch is not needed by
+    the library, and a second HIP runtime in the process is avoided)."""
+    global _GPU
+    if _GPU is None:
+        try:
+            from unicycle_amd import _lib
+            L = _lib.load()
+            h = L.ucy_create(0, 0)
+            _GPU = bool(h)
+            if h:
+                L.ucy_destroy(h)
+        except Exception:
+            _GPU = False
+    return _GPU
 
 
 @pytest.fixture(scope="session")

@@ -1,0 +1,79 @@
+"""The drop-in driver end to end: an input deck in the reference's layout
+(8 x 4 thrust, two observed patches) through unicycle_amd.ratestate on the
+GPU, its time.txt and patch-*.dat against the oracle's run of the same
+model (ratestate.f90:307-328 restated in C), and the outputs independent of
+how the device run is split into calls (the start time continues the
+reference's running sum)."""
+import io
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from unicycle_amd import configs, deck, ratestate
+
+pytestmark = pytest.mark.gpu
+
+
+def _read(path, widths):
+    """Fixed-width fields, as the reference's formats write them (a negative
+    ES20.12E3 value fills its field and touches the previous one)."""
+    out = []
+    for line in open(path).read().splitlines():
+        if line.startswith("#"):
+            continue
+        row, p = [], 0
+        for w in widths:
+            row.append(float(line[p:p + w]))
+            p += w
+        assert p == len(line)
+        out.append(row)
+    return np.array(out)
+
+
+def test_deck_run_matches_oracle(tmp_path, engine_factory):
+    m = configs.thrust_fault(nx=8, nz=4)
+    wdir = str(tmp_path / "out")
+    text = deck.format_rectangles(wdir, m.lambda_, m.mu, 1.0e9, m.x, m.length, m.width,
+                                  m.strike, m.dip, *m.friction(), observe=[1, 17])
+    d = deck.parse(text)
+    steps = 300
+    log = io.StringIO()
+    it, time, y = ratestate.run(d, max_iterations=steps, chunk=steps, out=log)
+    assert it == steps
+    tt = _read(os.path.join(wdir, "time.txt"), [19, 19])
+    p1 = _read(os.path.join(wdir, "patch-00000001.dat"), [19] + [20] * 14)
+    p17 = _read(os.path.join(wdir, "patch-00000017.dat"), [19] + [20] * 14)
+    assert tt.shape == (steps + 1, 2) and p1.shape == (steps, 15) and p17.shape == (steps, 15)
+    assert log.getvalue().splitlines()[-1] == f"{steps + 1:09d} time steps."
+    # the oracle on the deck's own model (its K from the oracle's buildG)
+    fr = d.friction_arrays()
+    K = orc.buildG(orc.Mesh(d.lambda_, d.mu, rect=(d.rect_x, d.rect_length, d.rect_width,
+                                                   d.rect_strike, d.rect_dip)), nthreads=8)
+    sysm = orc.System(fr, K)
+    y0 = sysm.init_state()
+    yref, st, tl, dl, rc = sysm.integrate(y0, d.interval, max_steps=steps, log_cap=steps)
+    assert st.steps == steps
+    # time.txt: (0, 1e-3), then per iteration (time at its start, previous step)
+    assert tt[0].tolist() == [0.0, 1.0e-3]
+    t_start = np.concatenate([[0.0], tl[:-1]])
+    prev = np.concatenate([[0.0], dl[:-1]])
+    np.testing.assert_allclose(tt[1:, 0], t_start, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(tt[1:, 1], prev, rtol=1e-6, atol=0)
+    # patch files: first row = initial state and its derivative
+    dydt0 = sysm.odefun(y0)[0]
+    for p, e in ((p1, 0), (p17, 16)):
+        assert p[0, 0] == 0.0
+        np.testing.assert_allclose(p[0, 1:8], y0[7 * e:7 * e + 7], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(p[0, 8:15], dydt0[7 * e:7 * e + 7], rtol=1e-9,
+                                   atol=1e-12 * np.abs(dydt0[7 * e:7 * e + 7]).max())
+        # the time column of the patch file is the iteration start time
+        np.testing.assert_array_equal(p[:, 0], tt[1:, 0])
+    np.testing.assert_allclose(y, yref, rtol=1e-6, atol=1e-9 * np.abs(yref).max())
+    # split into many calls, the run writes the same files
+    wdir2 = str(tmp_path / "out2")
+    d2 = deck.parse(text.replace(wdir, wdir2))
+    ratestate.run(d2, max_iterations=steps, chunk=37, out=io.StringIO())
+    for name in ("time.txt", "patch-00000001.dat", "patch-00000017.dat"):
+        assert open(os.path.join(wdir, name)).read() == open(os.path.join(wdir2, name)).read()

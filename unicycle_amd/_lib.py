@@ -54,6 +54,10 @@ class StepLog(C.Structure):
         ("errmax", C.POINTER(C.c_double)),
         ("rejects", C.POINTER(C.c_double)),
         ("cap", C.c_int64),
+        ("n_obs", C.c_int64),
+        ("obs_index", C.POINTER(C.c_int64)),
+        ("obs_y", C.POINTER(C.c_double)),
+        ("obs_dydt", C.POINTER(C.c_double)),
     ]
 
 
@@ -102,7 +106,7 @@ _SIGS = {
     "ucy_rkqs": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _d, _pd, _pd, _d, _d]),
     "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _pd, _i64,
                            C.POINTER(RunStats)]),
-    "ucy_integrate_ex": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, C.POINTER(StepLog),
+    "ucy_integrate_ex": (_i, [_vp, _pd, _d, _d, _d, _i64, _d, _d, C.POINTER(StepLog),
                               C.POINTER(RunStats)]),
     "ucy_time_rhs_kernel": (_i, [_vp, _i, _pd]),
     "ucy_mlp_create": (_vp, [_vp, C.c_uint64]),

@@ -829,6 +829,7 @@ static int capture_attempt(ucy_ctx *c)
     for (int s = 1; s <= 5; s++) launch_rhs_stage(c->stream, d, K, c->d_fr, c->d_rh, b, c->d_ctl, s);
     launch_rhs_finish(c->stream, d, K, c->d_fr, b, c->d_ctl);
     launch_controller(c->stream, b, c->d_ctl);
+    if (b.n_obs > 0) launch_obs_log(c->stream, b, c->d_ctl);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -841,11 +842,12 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                              double *dt_log, double *vmax_log, int64_t log_cap,
                              ucy_run_stats *stats)
 {
-    ucy_step_log log = {t_log, dt_log, vmax_log, nullptr, nullptr, log_cap};
-    return ucy_integrate_ex(c, y, interval, dt0, max_steps, epsilon, max_step, &log, stats);
+    ucy_step_log log = {t_log, dt_log, vmax_log, nullptr, nullptr, log_cap, 0, nullptr, nullptr,
+                        nullptr};
+    return ucy_integrate_ex(c, y, 0.0, interval, dt0, max_steps, epsilon, max_step, &log, stats);
 }
 
-extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double dt0,
+extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interval, double dt0,
                                 int64_t max_steps, double epsilon, double max_step,
                                 const ucy_step_log *log, ucy_run_stats *stats)
 {
@@ -853,11 +855,12 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
     double *vmax_log = log ? log->vmax : nullptr;
     double *err_log = log ? log->errmax : nullptr, *rej_log = log ? log->rejects : nullptr;
     const int64_t log_cap = log ? log->cap : 0;
-    if (!c || !y) return UCY_ERR_ARG;
+    const int64_t n_obs = log ? log->n_obs : 0;
+    if (!c || !y || n_obs < 0) return UCY_ERR_ARG;
     const PlainDims d0 = c->K ? dims(c) : PlainDims{};
     int rc = check_ready(c, d0.dim);
     if (rc) return rc;
-    if (!(interval > 0) || !(dt0 > 0) || max_steps <= 0 || !(epsilon > 0))
+    if (!(interval > 0) || !(dt0 > 0) || max_steps <= 0 || !(epsilon > 0) || !(t0 == t0))
         return fail(c, UCY_ERR_ARG, "invalid integration parameters");
     const PlainDims d = dims(c);
     RkBuffers &b = c->rb;
@@ -875,9 +878,35 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
     b.vlog = d_log ? d_log + 2 * cap : nullptr;
     b.elog = ext ? d_log + 3 * cap : nullptr;
     b.rlog = ext ? d_log + 4 * cap : nullptr;
+    // observed entries (needs the step log for the row index)
+    int64_t *d_oidx = nullptr;
+    double *d_obs = nullptr;
+    b.obs_idx = nullptr;
+    b.n_obs = 0;
+    b.obs_y = b.obs_dydt = nullptr;
+    if (n_obs > 0) {
+        if (cap <= 0 || !log->obs_index || !log->obs_y || !log->obs_dydt) {
+            if (d_log) hipFree(d_log);
+            return fail(c, UCY_ERR_ARG, "observed entries need the step log and their buffers");
+        }
+        for (int64_t j = 0; j < n_obs; j++)
+            if (log->obs_index[j] < 0 || log->obs_index[j] >= d.dim) {
+                if (d_log) hipFree(d_log);
+                return fail(c, UCY_ERR_ARG, "observed index %lld outside the state",
+                            (long long)log->obs_index[j]);
+            }
+        HIPCHK(hipMalloc((void **)&d_oidx, sizeof(int64_t) * n_obs));
+        HIPCHK(hipMalloc((void **)&d_obs, sizeof(double) * 2 * n_obs * cap));
+        HIPCHK(hipMemcpyAsync(d_oidx, log->obs_index, sizeof(int64_t) * n_obs,
+                              hipMemcpyHostToDevice, c->stream));
+        b.obs_idx = d_oidx;
+        b.n_obs = n_obs;
+        b.obs_y = d_obs;
+        b.obs_dydt = d_obs + n_obs * cap;
+    }
     Control &h = *c->h_ctl;
     memset(&h, 0, sizeof(Control));
-    h.time = 0.0;
+    h.time = t0;
     h.h = dt0;
     h.hnext = dt0;
     h.interval = interval;
@@ -891,9 +920,17 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
     rc = capture_attempt(c);
     if (rc) {
         if (d_log) hipFree(d_log);
+        if (d_oidx) {
+            hipFree(d_oidx);
+            hipFree(d_obs);
+        }
+        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+        b.obs_idx = nullptr;
+        b.n_obs = 0;
+        b.obs_y = b.obs_dydt = nullptr;
         return rc;
     }
-    auto t0 = std::chrono::steady_clock::now();
+    auto w0 = std::chrono::steady_clock::now();
     int batch = 16;
     for (;;) {
         for (int i = 0; i < batch; i++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
@@ -921,6 +958,12 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
             HIPCHK(hipMemcpyAsync(rej_log, b.rlog, sizeof(double) * h.log_count,
                                   hipMemcpyDeviceToHost, c->stream));
         }
+        if (n_obs > 0) {
+            HIPCHK(hipMemcpyAsync(log->obs_y, b.obs_y, sizeof(double) * n_obs * h.log_count,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(log->obs_dydt, b.obs_dydt, sizeof(double) * n_obs * h.log_count,
+                                  hipMemcpyDeviceToHost, c->stream));
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (d_log) {
@@ -930,6 +973,13 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
         HIPCHK(hipFree(d_log));
         b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
     }
+    if (d_oidx) {
+        HIPCHK(hipFree(d_oidx));
+        HIPCHK(hipFree(d_obs));
+        b.obs_idx = nullptr;
+        b.n_obs = 0;
+        b.obs_y = b.obs_dydt = nullptr;
+    }
     if (stats) {
         stats->time = h.time;
         stats->dt_next = h.h;
@@ -937,7 +987,7 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double interval, double d
         stats->attempts = h.attempts;
         stats->rhs_evals = h.rhs_evals;
         stats->vmax = h.last_vmax;
-        stats->wall_seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->wall_seconds = std::chrono::duration<double>(t1 - w0).count();
     }
     if (h.error == 1) return fail(c, UCY_ERR_UNDERFLOW, "stepsize underflow in RKQS");
     if (h.error == 2) return fail(c, UCY_ERR_NONFINITE, "non-finite error norm");

@@ -416,6 +416,7 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
                              const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) ctl->obs_pending = 0;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
     if (ctl->accepted) {
@@ -646,9 +647,34 @@ __global__ void controller_kernel(RkBuffers b, Control *ctl)
     ctl->last_vmax = ctl->vmax;
     ctl->vmax = 0.0;
     ctl->accepted = 1;
+    ctl->obs_pending = 1;
     ctl->need_rhs = 1;
     ctl->h = hnext;
     if (ctl->interval <= ctl->time || ctl->steps >= ctl->max_steps) ctl->done = 1;
+}
+
+// the step-start state and derivative of the observed entries after an
+// accepted step, at the step's log row (the controller has advanced
+// log_count; y is committed at the next attempt's start)
+__global__ void obs_log_kernel(RkBuffers b, Control *ctl)
+{
+    // only in the attempt that accepted the step: replays issued after the
+    // run ended keep `accepted` but have committed y already
+    if (!ctl->obs_pending) return;
+    const int64_t row = ctl->log_count - 1;
+    if (row < 0 || row >= ctl->log_cap) return;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b.n_obs;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = b.obs_idx[j];
+        b.obs_y[row * b.n_obs + j] = b.y[i];
+        b.obs_dydt[row * b.n_obs + j] = b.dydt[i];
+    }
+}
+
+void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl)
+{
+    const int64_t blocks = b.n_obs < 256 * 64 ? (b.n_obs + 255) / 256 : 64;
+    hipLaunchKernelGGL(obs_log_kernel, dim3((unsigned)blocks), dim3(256), 0, st, b, ctl);
 }
 
 // y <- ycand after an accepted attempt (ode45.f90:103)

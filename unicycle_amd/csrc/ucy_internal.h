@@ -44,6 +44,8 @@ struct Control {
     long long steps, attempts, max_steps, rhs_evals;
     long long log_count, log_cap;
     int step_rejects;   // rejected attempts of the current step
+    int obs_pending;    // set by the controller on acceptance, cleared at the next
+                        // attempt's start: the observed entries are logged once per step
 };
 
 struct PlainDims {
@@ -71,6 +73,10 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 struct RkBuffers {
     double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand, *yerr, *yr, *v, *t;
     double *v2;            // slip rates of the even stages (the attempt graph double-buffers v)
+    // observed state entries logged per accepted step (null: off)
+    const int64_t *obs_idx;
+    int64_t n_obs;
+    double *obs_y, *obs_dydt;
     // reduced-precision K (null in binary64): slip rates and tractions of the
     // step start; the stage matvecs then take v - v0 and add t0
     double *v0, *t0;
@@ -100,6 +106,7 @@ void launch_rhs_stage(hipStream_t st, const PlainDims &d, const KOperand &K, con
 void launch_rhs_finish(hipStream_t st, const PlainDims &d, const KOperand &K, const Friction *fr,
                        const RkBuffers &b, Control *ctl);
 void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl);
+void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl);
 // standalone Cash-Karp combination kernels for the ucy_rkck entry point
 void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,
                        double *const *ak, double *yr, double *ytmp, double *yout, double *yerr);

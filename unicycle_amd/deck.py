@@ -1,0 +1,299 @@
+"""The input deck of the reference's 3-D rate-and-state program.
+
+Reads the format that ratestate.f90:958-1739 (`init`) reads, section by
+section, with its checks: comment lines start with '#' (getdata.f), values
+are Fortran list-directed (blanks or commas, D or E exponents, quoted or bare
+names), angles are in degrees and converted with the reference's DEG2RAD.
+
+Sections, in order:
+  output directory
+  elastic moduli                       lambda mu
+  time interval                        (s)
+  number of rectangular patches        ns
+    i Vpl x1 x2 x3 length width strike dip rake        (ns lines)
+    number of frictional rectangular patches           (= ns)
+    i mu0 sig a b L Vo G/(2Vs)                         (ns lines)
+  number of triangular patches         nt
+    i Vpl i1 i2 i3 rake                                (nt lines)
+    number of vertices                                 nVe
+    i x1 x2 x3                                         (nVe lines)
+    number of frictional triangular patches            (= nt)
+    i mu0 sig a b L Vo G/(2Vs)                         (nt lines)
+  number of observation patches        no
+    i patch                                            (no lines)
+  number of observation points         np
+    i name x1 x2 x3                                    (np lines)
+  number of events                     ne
+    time                                               (ne - 1 lines: the first
+                                                        event is at t = 0)
+The patch and friction sections of a source type are present only when its
+count is positive.
+"""
+from __future__ import annotations
+
+import shlex
+from dataclasses import dataclass, field
+
+import numpy as np
+
+DEG2RAD = 0.01745329251994329547437168059786927  # input of ratestate.f90, as the reference
+
+
+class DeckError(ValueError):
+    """An input deck that the reference would reject (its STOPs)."""
+
+
+@dataclass
+class Deck:
+    wdir: str
+    lambda_: float
+    mu: float
+    interval: float
+    # rectangles: upper corners (ns, 3) north/east/down, sizes, radians
+    rect_x: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    rect_length: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    rect_width: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    rect_strike: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    rect_dip: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    # triangles: 1-based vertex indices, vertices (nVe, 3)
+    tri_i1: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    tri_i2: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    tri_i3: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    vertices: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    # per element (rectangles then triangles): Vpl, rake (rad), mu0, sig, a, b, L, Vo, damping
+    friction: dict = field(default_factory=dict)
+    observation_states: list = field(default_factory=list)   # 1-based element indices
+    observation_points: list = field(default_factory=list)   # (name, x1, x2, x3)
+    events: list = field(default_factory=list)               # times, the first at 0
+
+    @property
+    def n_rect(self) -> int:
+        return self.rect_length.size
+
+    @property
+    def n_tri(self) -> int:
+        return self.tri_i1.size
+
+    @property
+    def nu(self) -> float:
+        return self.lambda_ / 2.0 / (self.lambda_ + self.mu)
+
+    def friction_arrays(self):
+        """(Vpl, rake, mu0, sig, a, b, L, Vo, damping) in element order, the
+        argument order of Engine.set_friction."""
+        keys = ("Vpl", "rake", "mu0", "sig", "a", "b", "L", "Vo", "damping")
+        return tuple(np.asarray(self.friction[k], dtype=np.float64) for k in keys)
+
+
+def _fortran_tokens(line: str):
+    """Values of one list-directed record: blanks and commas separate, a
+    slash ends the record, quotes delimit strings."""
+    cut = line
+    # a slash outside quotes ends the record
+    quote = None
+    for i, ch in enumerate(line):
+        if ch in "'\"":
+            quote = None if quote == ch else (ch if quote is None else quote)
+        elif ch == "/" and quote is None:
+            cut = line[:i]
+            break
+    lex = shlex.shlex(cut.replace(",", " "), posix=True)
+    lex.whitespace_split = True
+    lex.commenters = ""
+    return list(lex)
+
+
+def _real(tok: str) -> float:
+    try:
+        return float(tok.replace("d", "e").replace("D", "e"))
+    except ValueError as exc:
+        raise DeckError(f"not a number: {tok!r}") from exc
+
+
+def _int(tok: str) -> int:
+    try:
+        return int(tok)
+    except ValueError as exc:
+        raise DeckError(f"not an integer: {tok!r}") from exc
+
+
+class _Lines:
+    """getdata.f: skips lines whose first non-blank character is '#'
+    (and blank lines)."""
+
+    def __init__(self, text: str):
+        self.lines = text.splitlines()
+        self.k = 0
+
+    def next(self, what: str) -> str:
+        while self.k < len(self.lines):
+            line = self.lines[self.k]
+            self.k += 1
+            s = line.strip()
+            if s and not s.startswith("#"):
+                return line
+        raise DeckError(f"input file ends before the {what}")
+
+    def values(self, what: str, n: int):
+        toks = _fortran_tokens(self.next(what))
+        if len(toks) < n:
+            raise DeckError(f"{what}: {n} values expected, got {len(toks)}")
+        return toks
+
+
+def parse(text: str) -> Deck:
+    """Parse an input deck; raises DeckError where the reference stops."""
+    L = _Lines(text)
+    wdir = L.next("output directory").strip()
+    t = L.values("elastic moduli", 2)
+    lam, mu = _real(t[0]), _real(t[1])
+    if 0 > mu:
+        raise DeckError("shear modulus must be positive")
+    nu = lam / 2.0 / (lam + mu)
+    if -1.0 > nu:
+        raise DeckError("Poisson's ratio must be greater than -1")
+    if 0.5 < nu:
+        raise DeckError("Poisson's ratio must be lower than 0.5")
+    interval = _real(L.values("time interval", 1)[0])
+    if interval <= 0.0:
+        raise DeckError("simulation time must be positive")
+    d = Deck(wdir=wdir, lambda_=lam, mu=mu, interval=interval)
+    fr = {k: [] for k in ("Vpl", "rake", "mu0", "sig", "a", "b", "L", "Vo", "damping")}
+
+    def friction_rows(n, kind):
+        dummy = _int(L.values(f"number of frictional {kind} patches", 1)[0])
+        if dummy != n:
+            raise DeckError(f"all {kind} patches require frictional properties")
+        for k in range(1, n + 1):
+            t = L.values(f"{kind} friction row {k}", 8)
+            if _int(t[0]) != k:
+                raise DeckError(f"invalid friction property definition for {kind} patch: "
+                                "unexpected index")
+            for key, tok in zip(("mu0", "sig", "a", "b", "L", "Vo", "damping"), t[1:8]):
+                fr[key].append(_real(tok))
+
+    # rectangular patches (ratestate.f90:1130-1237)
+    ns = _int(L.values("number of rectangular patches", 1)[0])
+    if ns > 0:
+        rows = []
+        for k in range(1, ns + 1):
+            t = L.values(f"rectangular patch {k}", 10)
+            if _int(t[0]) != k:
+                raise DeckError("invalid rectangular patch definition: unexpected index")
+            v = [_real(x) for x in t[1:10]]
+            if max(v[4], v[5]) <= 0.0:
+                raise DeckError("patch length and width must be positive")
+            rows.append(v)
+        r = np.array(rows)
+        d.rect_x = r[:, 1:4].copy()
+        d.rect_length, d.rect_width = r[:, 4].copy(), r[:, 5].copy()
+        d.rect_strike, d.rect_dip = r[:, 6] * DEG2RAD, r[:, 7] * DEG2RAD
+        fr["Vpl"] += list(r[:, 0])
+        fr["rake"] += list(r[:, 8] * DEG2RAD)
+        friction_rows(ns, "rectangular")
+    # triangular patches (ratestate.f90:1238-1380)
+    nt = _int(L.values("number of triangular patches", 1)[0])
+    if nt > 0:
+        i1, i2, i3, vpl, rake = [], [], [], [], []
+        for k in range(1, nt + 1):
+            t = L.values(f"triangular patch {k}", 6)
+            idx = [_int(x) for x in t[2:5]]
+            if min(idx) < 0:
+                raise DeckError("negative vertex index")
+            if _int(t[0]) != k:
+                raise DeckError("invalid triangular patch definition: unexpected index")
+            vpl.append(_real(t[1]))
+            i1.append(idx[0])
+            i2.append(idx[1])
+            i3.append(idx[2])
+            rake.append(_real(t[5]) * DEG2RAD)
+        nve = _int(L.values("number of vertices", 1)[0])
+        if max(max(i1), max(i2), max(i3)) > nve:
+            raise DeckError("not enough vertices")
+        verts = []
+        for k in range(1, nve + 1):
+            t = L.values(f"vertex {k}", 4)
+            if _int(t[0]) != k:
+                raise DeckError("invalid vertex definition: unexpected index")
+            verts.append([_real(x) for x in t[1:4]])
+        d.tri_i1, d.tri_i2, d.tri_i3 = (np.array(a, np.int32) for a in (i1, i2, i3))
+        d.vertices = np.array(verts, dtype=np.float64).reshape(-1, 3)
+        fr["Vpl"] += vpl
+        fr["rake"] += rake
+        friction_rows(nt, "triangular")
+    # observation patches (ratestate.f90:1381-1400)
+    no = _int(L.values("number of observation patches", 1)[0])
+    for k in range(1, no + 1):
+        t = L.values(f"observation patch {k}", 2)
+        if _int(t[0]) != k:
+            raise DeckError("observation patches: unexpected index")
+        d.observation_states.append(_int(t[1]))
+    # observation points (ratestate.f90:1420-1455)
+    npnt = _int(L.values("number of observation points", 1)[0])
+    for k in range(1, npnt + 1):
+        t = L.values(f"observation point {k}", 5)
+        if _int(t[0]) != k:
+            raise DeckError("observation points: unexpected index")
+        d.observation_points.append((t[1], _real(t[2]), _real(t[3]), _real(t[4])))
+    # events (ratestate.f90:1457-1484): the first is at t = 0 and has no line
+    ne = _int(L.values("number of events", 1)[0])
+    for k in range(1, ne + 1):
+        if k == 1:
+            d.events.append(0.0)
+            continue
+        tk = _real(L.values(f"event {k}", 1)[0])
+        if tk <= d.events[-1]:
+            raise DeckError("timing of perturbations must increase")
+        d.events.append(tk)
+    if (d.n_rect == 0 and d.n_tri == 0) or d.interval <= 0.0:
+        raise DeckError("nothing to do")
+    d.friction = fr
+    nel = d.n_rect + d.n_tri
+    for j in d.observation_states:
+        if not 1 <= j <= nel:
+            raise DeckError(f"observation patch {j} is not an element")
+    return d
+
+
+def read(path: str) -> Deck:
+    with open(path) as f:
+        return parse(f.read())
+
+
+def format_rectangles(wdir, lambda_, mu, interval, x, length, width, strike, dip, Vpl, rake,
+                      mu0, sig, a, b, L, Vo, damping, observe=(), points=(), events=1) -> str:
+    """An input deck of rectangular patches (angles in radians here, written in
+    degrees), in the layout parse() reads."""
+    R2D = 1.0 / DEG2RAD
+    out = ["# output directory", str(wdir), "# elastic moduli", f"{lambda_!r} {mu!r}",
+           "# time interval", f"{interval!r}", "# number of rectangular patches", str(len(length)),
+           "# n Vpl x1 x2 x3 length width strike dip rake"]
+    for k in range(len(length)):
+        out.append(" ".join([str(k + 1)] + [repr(float(v)) for v in (
+            Vpl[k], x[k][0], x[k][1], x[k][2], length[k], width[k], strike[k] * R2D,
+            dip[k] * R2D, rake[k] * R2D)]))
+    out += ["# number of frictional rectangular patches", str(len(length)),
+            "# n mu0 sig a b L Vo G/(2Vs)"]
+    for k in range(len(length)):
+        out.append(" ".join([str(k + 1)] + [repr(float(v)) for v in (
+            mu0[k], sig[k], a[k], b[k], L[k], Vo[k], damping[k])]))
+    out += ["# number of triangular patches", "0", "# number of observation patches",
+            str(len(observe))]
+    out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
+    out += ["# number of observation points", str(len(points))]
+    out += [f"{k + 1} {n} {x1!r} {x2!r} {x3!r}" for k, (n, x1, x2, x3) in enumerate(points)]
+    out += ["# number of events", str(events)]
+    out += [f"{float(k)!r}" for k in range(1, events)]
+    return "\n".join(out) + "\n"
+
+
+def _es(x: float, digits: int, ewidth: int) -> str:
+    """Fortran ES(w).(digits)E(ewidth) of one value, without padding."""
+    m = f"{x:.{digits}E}"
+    mant, exp = m.split("E")
+    e = int(exp)
+    return f"{mant}E{'-' if e < 0 else '+'}{abs(e):0{ewidth}d}"
+
+
+def es_field(x: float, width: int, digits: int, ewidth: int) -> str:
+    return _es(float(x), digits, ewidth).rjust(width)

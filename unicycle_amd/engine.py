@@ -65,6 +65,8 @@ class IntegrateResult:
     vmax_log: np.ndarray | None = None
     errmax_log: np.ndarray | None = None    # errmax/epsilon of each accepted attempt
     reject_log: np.ndarray | None = None    # rejected attempts before each accepted one
+    obs_y: np.ndarray | None = None         # (steps, n_obs) step-start values of observed entries
+    obs_dydt: np.ndarray | None = None      # (steps, n_obs) their step-start derivatives
 
 
 class Engine:
@@ -232,10 +234,15 @@ class Engine:
 
     def integrate(self, y, interval: float, dt0: float = 1.0e-3, max_steps: int = 1000000,
                   epsilon: float = 1.0e-6, max_step: float = 1.7976931348623158e308,
-                  log_cap: int = 0, history: bool = False) -> IntegrateResult:
+                  log_cap: int = 0, history: bool = False, observe=None,
+                  t0: float = 0.0) -> IntegrateResult:
         """Main loop of ratestate.f90:307-328, device resident.  log_cap > 0
         logs time, dt and vMax per accepted step; history adds errmax/epsilon
-        and the rejections of each step (the step-size proposal's data)."""
+        and the rejections of each step (the step-size proposal's data);
+        observe (state indices, needs log_cap) adds the step-start values and
+        derivatives of those entries per step (the export of
+        ratestate.f90:488-536).  t0 continues a run: the interval and the
+        logged times are absolute."""
         y = _f64(y).copy()
         st = _lib.RunStats()
         logs = [np.empty(max(log_cap, 1)) for _ in range(5)]
@@ -245,14 +252,24 @@ class Engine:
             if history:
                 lg.errmax, lg.rejects = _p(logs[3]), _p(logs[4])
         lg.cap = int(log_cap)
-        self._chk(self._L.ucy_integrate_ex(self._h, _p(y), float(interval), float(dt0),
+        obs = None
+        if observe is not None and len(observe):
+            idx = np.ascontiguousarray(np.asarray(observe, dtype=np.int64))
+            obs = (idx, np.empty((max(log_cap, 1), idx.size)), np.empty((max(log_cap, 1), idx.size)))
+            lg.n_obs = idx.size
+            lg.obs_index = idx.ctypes.data_as(C.POINTER(C.c_int64))
+            lg.obs_y, lg.obs_dydt = _p(obs[1]), _p(obs[2])
+        self._chk(self._L.ucy_integrate_ex(self._h, _p(y), float(t0), float(interval), float(dt0),
                                            int(max_steps), float(epsilon), float(max_step),
                                            C.byref(lg), C.byref(st)))
         nlog = min(st.steps, log_cap)
-        cut = lambda a, on=True: a[:nlog].copy() if (log_cap and on) else None
-        return IntegrateResult(y, st.time, st.dt_next, st.steps, st.attempts, st.rhs_evals,
-                               st.vmax, st.wall_seconds, cut(logs[0]), cut(logs[1]),
-                               cut(logs[2]), cut(logs[3], history), cut(logs[4], history))
+        cut = lambda a, on=True: a[:nlog].copy() if (log_cap and on) else None  # noqa: E731
+        res = IntegrateResult(y, st.time, st.dt_next, st.steps, st.attempts, st.rhs_evals,
+                              st.vmax, st.wall_seconds, cut(logs[0]), cut(logs[1]),
+                              cut(logs[2]), cut(logs[3], history), cut(logs[4], history))
+        if obs is not None:
+            res.obs_y, res.obs_dydt = obs[1][:nlog].copy(), obs[2][:nlog].copy()
+        return res
 
     def time_rhs_kernel(self, iters: int = 50) -> float:
         """Average seconds of the fused matvec + post-pass kernel (HIP events)."""
