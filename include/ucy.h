@@ -176,6 +176,30 @@ int ucy_set_kernel(ucy_ctx *ctx, const double *K, int64_t m, int64_t n);
 /* device pointer to K (binary64, row-major) for zero-copy callers */
 const double *ucy_kernel_dev(const ucy_ctx *ctx);
 
+/* ---- observation points: buildO (greens.f90:302-462) and exportPoints ---- */
+/* Displacement kernels at npts observation points x(3,npts) (north, east,
+ * down), built on the device and kept there: O is the physical (3 npts x n)
+ * matrix, row 3k+c = NED component c at point k, columns as K's (the
+ * reference's O(n, 3 npts)).  Rectangles: computeDisplacementOkada92
+ * (okada92.f90:230-298, zero at a DC3D-singular fault edge); triangles:
+ * computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367); volumes:
+ * computeDisplacementVerticalStrainVolume (strainvolume.f90:215-268, its
+ * STOPs as UCY_ERR_GEOMETRY).  npts = 0 drops the kernels. */
+int ucy_build_displacement(ucy_ctx *ctx, int npts, const double *x);
+/* shape of O (rows = 3 npts, n); 0 x 0 when not built */
+int ucy_displacement_shape(const ucy_ctx *ctx, int64_t *rows, int64_t *n);
+/* copy O back to the host, and form buildO's Of and Ol from it (each may be
+ * NULL): Of = the fault columns plus the volume columns' third component
+ * (greens.f90:445), Ol = the volume columns' first two components */
+int ucy_get_displacement_kernel(ucy_ctx *ctx, double *O, double *Of, double *Ol);
+/* exportPoints (ratestate.f90:414-486, viscouscycles.f90:523-625): d = O u
+ * with u the strike and dip slip of each patch and e11..e33 of each volume,
+ * gathered from the state y; df = Of u and dl = Ol u (may be NULL).  Host
+ * vectors; _dev takes device pointers. */
+int ucy_displacement(ucy_ctx *ctx, const double *y, double *d, double *df, double *dl);
+int ucy_displacement_dev(ucy_ctx *ctx, const double *y_dev, double *d_dev, double *df_dev,
+                         double *dl_dev);
+
 /* ---- RHS, matvec and the Cash-Karp stepper ------------------------------ */
 /* DGEMV("T") of ratestate.f90:719-720: t(m) = K v(n). */
 int ucy_matvec_f64(ucy_ctx *ctx, const double *v, double *t);
@@ -226,6 +250,10 @@ typedef struct {
     int64_t n_obs;
     const int64_t *obs_index;
     double *obs_y, *obs_dydt;
+    /* observation-point displacement at each logged step's start (needs the
+     * step log and ucy_build_displacement): d, df, dl of ucy_displacement,
+     * [cap][3 npts] each; any may be NULL (all NULL: off) */
+    double *disp, *disp_f, *disp_l;
 } ucy_step_log;
 int ucy_integrate_ex(ucy_ctx *ctx, double *y, double t0, double interval, double dt0,
                      int64_t max_steps, double epsilon, double max_step,

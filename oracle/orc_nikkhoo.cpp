@@ -430,6 +430,46 @@ extern "C" void orc_strain_in_halfspace(double stress[6], double strain[6], doub
     }
 }
 
+/* dispinhalfspace_ (DispInHalfSpace.c:42-152): main + harmonic + image parts
+ * summed in that order; for a triangle with all vertices at z = 0 the image
+ * part's vertical component and then the total change sign. ENU. */
+extern "C" void orc_disp_in_halfspace(double u[3], double X, double Y, double Z,
+                                      const double P1[3], const double P2[3],
+                                      const double P3[3], double Ss, double Ds, double Ts,
+                                      double nu)
+{
+    const D3 x(X), y(Y), z(Z);
+    const double M1[3] = {P1[0], P1[1], -P1[2]}, M2[3] = {P2[0], P2[1], -P2[2]},
+                 M3[3] = {P3[0], P3[1], -P3[2]};
+    D3 ums[3], ufs[3], uis[3];
+    td_disp_fs(x, y, z, P1, P2, P3, Ss, Ds, Ts, nu, ums);
+    td_disp_harfunc(x, y, z, P1, P2, P3, Ss, Ds, Ts, nu, ufs);
+    td_disp_fs(x, y, z, M1, M2, M3, Ss, Ds, Ts, nu, uis);
+    const bool surface = M1[2] == 0.0 && M2[2] == 0.0 && M3[2] == 0.0;
+    if (surface) uis[2].v = -uis[2].v;
+    for (int i = 0; i < 3; i++) {
+        u[i] = ums[i].v + ufs[i].v + uis[i].v;
+        if (surface) u[i] = -u[i];
+    }
+}
+
+/* computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367): NED point and
+ * vertices to ENU, dip slip negated, result back to NED */
+extern "C" void orc_displacement_nikkhoo15(double x1, double x2, double x3, const double P1[3],
+                                           const double P2[3], const double P3[3], double ss,
+                                           double d, double t, double G, double lambda,
+                                           double u[3])
+{
+    const double nu = lambda / (lambda + G) / 2.0;
+    const double Q1[3] = {P1[1], P1[0], -P1[2]}, Q2[3] = {P2[1], P2[0], -P2[2]},
+                 Q3[3] = {P3[1], P3[0], -P3[2]};
+    double w[3];
+    orc_disp_in_halfspace(w, x2, x1, -x3, Q1, Q2, Q3, ss, -d, t, nu);
+    u[0] = w[1];
+    u[1] = w[0];
+    u[2] = -w[2];
+}
+
 /* computeReferenceSystemNikkhoo15 (nikkhoo15.f90:50-101); 1-based indices */
 extern "C" void orc_refsys_nikkhoo15(int nVe, const double *v, int ns, const int *i1,
                                      const int *i2, const int *i3, double *sv, double *dv,

@@ -128,6 +128,76 @@ int orc_buildG(const orc_mesh *M, double *K, int nthreads)
     return rc ? -2 : 0;
 }
 
+/* buildO (greens.f90:302-462).  O, Of, Ol row-major (3 nobs x n): row 3k+c is
+ * NED component c at observation point k (x[3k..3k+2]), columns as in buildG.
+ * Of holds the fault columns and, of the volume columns, only the third
+ * component (greens.f90:445 writes the volume's third component into Of);
+ * Ol holds the volume columns' first two components.  Of, Ol may be NULL.
+ * Returns 0, or -2 when a strain-volume parameter check fails. */
+int orc_buildO(const orc_mesh *M, int nobs, const double *x, double *O, double *Of, double *Ol,
+               int nthreads)
+{
+    const int np = M->nrect + M->ntri;
+    const int n = 2 * np + 6 * M->nvol;
+    const double nu = M->lambda / (M->lambda + M->mu) / 2.0;
+    const double eye[6][6] = {{1, 0, 0, 0, 0, 0}, {0, 1, 0, 0, 0, 0}, {0, 0, 1, 0, 0, 0},
+                              {0, 0, 0, 1, 0, 0}, {0, 0, 0, 0, 1, 0}, {0, 0, 0, 0, 0, 1}};
+    int rc = 0;
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) reduction(|:rc)
+    for (int k = 0; k < nobs; k++) {
+        const double *p = x + 3 * k;
+        double *row[3], *rf[3], *rl[3];
+        for (int c = 0; c < 3; c++) {
+            row[c] = O + (size_t)(3 * k + c) * n;
+            rf[c] = Of ? Of + (size_t)(3 * k + c) * n : NULL;
+            rl[c] = Ol ? Ol + (size_t)(3 * k + c) * n : NULL;
+            for (int j = 0; j < n; j++) {
+                if (rf[c]) rf[c][j] = 0.0;
+                if (rl[c]) rl[c][j] = 0.0;
+            }
+        }
+        double u[3];
+        for (int s = 0; s < M->nrect; s++)
+            for (int j = 0; j < 2; j++) {
+                const double *q = M->rx + 3 * s;
+                orc_displacement_okada92(p[0], p[1], p[2], q[0], q[1], q[2], M->rL[s], M->rW[s],
+                                         M->rstrike[s], M->rdip[s], eye[j][0], eye[j][1],
+                                         eye[j][2], M->mu, M->lambda, u);
+                for (int c = 0; c < 3; c++) {
+                    row[c][2 * s + j] = u[c];
+                    if (rf[c]) rf[c][2 * s + j] = u[c];
+                }
+            }
+        for (int s = 0; s < M->ntri; s++)
+            for (int j = 0; j < 2; j++) {
+                orc_displacement_nikkhoo15(p[0], p[1], p[2], M->vert + 3 * (M->i1[s] - 1),
+                                           M->vert + 3 * (M->i2[s] - 1),
+                                           M->vert + 3 * (M->i3[s] - 1), eye[j][0], eye[j][1],
+                                           eye[j][2], M->mu, M->lambda, u);
+                const int col = 2 * M->nrect + 2 * s + j;
+                for (int c = 0; c < 3; c++) {
+                    row[c][col] = u[c];
+                    if (rf[c]) rf[c][col] = u[c];
+                }
+            }
+        for (int s = 0; s < M->nvol; s++)
+            for (int j = 0; j < 6; j++) {
+                const double *q = M->vx + 3 * s;
+                if (orc_displacement_strainvolume(p[0], p[1], p[2], q[0], q[1], q[2], M->vL[s],
+                                                  M->vT[s], M->vW[s], M->vstrike[s], eye[j],
+                                                  M->mu, nu, u))
+                    rc |= 1;
+                const int col = 2 * np + 6 * s + j;
+                for (int c = 0; c < 3; c++) row[c][col] = u[c];
+                if (rl[0]) rl[0][col] = u[0];
+                if (rl[1]) rl[1][col] = u[1];
+                if (rf[2]) rf[2][col] = u[2];
+            }
+    }
+    return rc ? -2 : 0;
+}
+
 /* t = K v, rows summed left to right (DGEMV("T") on G(n,m), ratestate.f90:719) */
 void orc_matvec(int m, int n, const double *K, const double *v, double *t, int nthreads)
 {

@@ -486,6 +486,33 @@ void orc_stress_okada92(double x1, double x2, double x3, double q1, double q2, d
     sig[5] = lambda * ekk + 2.0 * G * e33;
 }
 
+/* computeDisplacementOkada92 (okada92.f90:230-298): the same binary32
+ * interface as the stress routine; DC3D's iret is honoured here (a receiver
+ * on a fault edge gets zero displacement).  Returns iret. */
+int orc_displacement_okada92(double x1, double x2, double x3, double q1, double q2, double q3,
+                             double L, double W, double strike, double dipd,
+                             double s, double d, double t, double G, double lambda,
+                             double u[3])
+{
+    const double RAD2DEG = 57.29577951308232;
+    float alpha = (float)((lambda + G) / (lambda + 2.0 * G));
+    float z = (float)(-x3);
+    double csst = cos(strike), ssst = sin(strike);
+    float x = (float)((x1 - q1) * csst + (x2 - q2) * ssst);
+    float y = (float)((x1 - q1) * ssst - (x2 - q2) * csst);
+    float uu[12];
+    int iret = orc_dc3d(alpha, x, y, z, (float)q3, (float)(dipd * RAD2DEG), 0.0f, (float)L,
+                        -(float)W, 0.0f, (float)s, (float)d, (float)t, uu);
+    if (iret == 0) {
+        u[0] = (double)uu[0] * csst + (double)uu[1] * ssst;
+        u[1] = (double)uu[0] * ssst - (double)uu[1] * csst;
+        u[2] = -(double)uu[2];
+    } else {
+        u[0] = u[1] = u[2] = 0.0;
+    }
+    return iret;
+}
+
 /* traction (sigma.n) projected on (s,d,n); okada92.f90:390-398 */
 static void project_traction(const double nv[3], const double sv[3], const double dv[3],
                              const double S[6], double *ts, double *td, double *tn)

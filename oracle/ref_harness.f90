@@ -58,6 +58,16 @@ CONTAINS
          sig(1),sig(2),sig(3),sig(4),sig(5),sig(6))
   END SUBROUTINE
 
+  ! okada92.f90:230 computeDisplacementOkada92 (one point, one source)
+  SUBROUTINE ref_displacement_okada92(x1,x2,x3,q1,q2,q3,L,W,strike,dipd,s,d,t,G,lambda,u) &
+       BIND(C,NAME="ref_displacement_okada92")
+    USE okada92
+    REAL(C_DOUBLE), VALUE :: x1,x2,x3,q1,q2,q3,L,W,strike,dipd,s,d,t,G,lambda
+    REAL(C_DOUBLE), INTENT(OUT) :: u(3)
+    CALL computeDisplacementOkada92(x1,x2,x3,q1,q2,q3,L,W,strike,dipd,s,d,t,G,lambda, &
+         u(1),u(2),u(3))
+  END SUBROUTINE
+
   ! dc3d.f:1 DC3D (REAL*4 interface)
   SUBROUTINE ref_dc3d(alpha,x,y,z,depth,dip,al1,al2,aw1,aw2,d1,d2,d3,u,iret) BIND(C,NAME="ref_dc3d")
     REAL(C_FLOAT), VALUE :: alpha,x,y,z,depth,dip,al1,al2,aw1,aw2,d1,d2,d3
@@ -112,6 +122,16 @@ CONTAINS
     j1=i1; j2=i2; j3=i3
     CALL computeStressKernelsNikkhoo15(x,sv,dv,nv,ns,j1,j2,j3,nVe,vert,s,d,t,G,lambda, &
          s11,s12,s13,s22,s23,s33)
+  END SUBROUTINE
+
+  ! nikkhoo15.f90:324 computeDisplacementNikkhoo15 (one point, one triangle)
+  SUBROUTINE ref_displacement_nikkhoo15(x1,x2,x3,P1,P2,P3,s,d,t,G,lambda,u) &
+       BIND(C,NAME="ref_displacement_nikkhoo15")
+    USE nikkhoo15
+    REAL(C_DOUBLE), VALUE :: x1,x2,x3,s,d,t,G,lambda
+    REAL(C_DOUBLE), INTENT(IN) :: P1(3),P2(3),P3(3)
+    REAL(C_DOUBLE), INTENT(OUT) :: u(3)
+    CALL computeDisplacementNikkhoo15(x1,x2,x3,P1,P2,P3,s,d,t,G,lambda,u(1),u(2),u(3))
   END SUBROUTINE
 
   ! strainvolume.f90:135 computeReferenceSystemVerticalStrainVolume

@@ -34,6 +34,11 @@ void orc_stress_okada92(double x1, double x2, double x3, double q1, double q2, d
                         double L, double W, double strike, double dipd,
                         double s, double d, double t, double G, double lambda,
                         double sig[6]);
+/* computeDisplacementOkada92; returns DC3D's iret (u = 0 when nonzero) */
+int orc_displacement_okada92(double x1, double x2, double x3, double q1, double q2, double q3,
+                             double L, double W, double strike, double dipd,
+                             double s, double d, double t, double G, double lambda,
+                             double u[3]);
 void orc_traction_kernels_okada92(const double x[3], const double sv[3], const double dv[3],
                                   const double nv[3], int ns, const double *y,
                                   const double *L, const double *W, const double *strike,
@@ -52,6 +57,13 @@ void orc_refsys_nikkhoo15(int nVe, const double *v, int ns, const int *i1, const
 void orc_strain_in_halfspace(double stress[6], double strain[6], double X, double Y, double Z,
                              const double P1[3], const double P2[3], const double P3[3],
                              double Ss, double Ds, double Ts, double mu, double lambda);
+/* dispinhalfspace_ (ENU) and computeDisplacementNikkhoo15 (NED) */
+void orc_disp_in_halfspace(double u[3], double X, double Y, double Z, const double P1[3],
+                           const double P2[3], const double P3[3], double Ss, double Ds,
+                           double Ts, double nu);
+void orc_displacement_nikkhoo15(double x1, double x2, double x3, const double P1[3],
+                                const double P2[3], const double P3[3], double ss, double d,
+                                double t, double G, double lambda, double u[3]);
 void orc_traction_kernels_nikkhoo15(const double x[3], const double sv[3], const double dv[3],
                                     const double nv[3], int ns, const int *i1, const int *i2,
                                     const int *i3, int nVe, const double *vert,
@@ -98,6 +110,10 @@ typedef struct {
  * nthreads > 1 splits receivers across OpenMP threads. Returns 0 or <0. */
 int orc_buildG(const orc_mesh *mesh, double *K, int nthreads);
 void orc_matvec(int m, int n, const double *K, const double *v, double *t, int nthreads);
+/* buildO (greens.f90:302-462): O, Of, Ol row-major (3 nobs x n), x (3,nobs).
+ * Of, Ol may be NULL.  Returns 0 or -2 (strain-volume parameter check). */
+int orc_buildO(const orc_mesh *mesh, int nobs, const double *x, double *O, double *Of,
+               double *Ol, int nthreads);
 
 /* ---- rate-and-state RHS (ratestate.f90 odefun) ------------------------- */
 typedef struct {

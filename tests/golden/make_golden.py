@@ -16,8 +16,15 @@ The fixtures are inputs and the reference's outputs only:
                         VerticalStrainVolume (strainvolume.f90:4671-4799), incl.
                         a receiver inside a volume and one at the surface, and
                         computeDisplacementVerticalStrainVolume (215-268)
+  displacement.npz      computeDisplacementOkada92 (okada92.f90:230-298) and
+                        computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367,
+                        DispInHalfSpace.c:42-152) at random points, at the
+                        surface and at depth, three unit slips, incl. a fault
+                        edge (DC3D singular), a surface triangle and a
+                        horizontal triangle
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [name ...]   (default: all)
+
 """
 import ctypes as C
 import os
@@ -219,13 +226,74 @@ def strainvolume_kernels(rng, ns=60, nrcv=4):
                         disp=np.array(disp).reshape(len(pts), 6, 3))
 
 
+def displacement(rng, nrect=400, ntri=400):
+    # rectangles: (x1, x2, x3, q1, q2, q3, L, W, strike, dip), every third
+    # point at the surface
+    rin = np.zeros((nrect, 10))
+    for i in range(nrect):
+        rin[i] = [rng.uniform(-2e4, 2e4), rng.uniform(-2e4, 2e4),
+                  0.0 if i % 3 == 0 else rng.uniform(0, 2e4), rng.uniform(-2e4, 2e4),
+                  rng.uniform(-2e4, 2e4), rng.uniform(0, 2e4), rng.uniform(500, 5e3),
+                  rng.uniform(500, 5e3), rng.uniform(0, 2 * np.pi), rng.uniform(0.05, np.pi / 2)]
+    # a fault reaching the surface seen from its top corner (singular edge)
+    # and from the surface trace's extension
+    rin[0] = [1e3, 2e3, 0.0, 1e3, 2e3, 0.0, 4e3, 3e3, 0.3, 1.2]
+    rin[1] = [1e3 - 2e3 * np.cos(0.3), 2e3 - 2e3 * np.sin(0.3), 0.0, 1e3, 2e3, 0.0, 4e3, 3e3,
+              0.3, 1.2]
+    ru = np.zeros((nrect, 3, 3))
+    for i, r in enumerate(rin):
+        for s in range(3):
+            b = [0.0, 0.0, 0.0]
+            b[s] = 1.0
+            u = np.zeros(3)
+            REF.ref_displacement_okada92(*[D(v) for v in r], *[D(v) for v in b], D(30e3),
+                                         D(30e3), P(u))
+            ru[i, s] = u
+    # triangles in NED (positive depth), points at the surface and at depth
+    P1 = np.stack([rng.uniform(-5e3, 5e3, ntri), rng.uniform(-5e3, 5e3, ntri),
+                   rng.uniform(500, 1e4, ntri)], 1)
+    P2 = P1 + rng.uniform(-3e3, 3e3, (ntri, 3))
+    P3 = P1 + rng.uniform(-3e3, 3e3, (ntri, 3))
+    P2[:, 2] = np.maximum(P2[:, 2], 100.0)
+    P3[:, 2] = np.maximum(P3[:, 2], 100.0)
+    P1[0], P2[0], P3[0] = [0, 0, 0], [2e3, 0, 0], [0, 2e3, 0]
+    P1[1], P2[1], P3[1] = [0, 0, 3e3], [2e3, 0, 3e3], [0, 2e3, 3e3]
+    pts = np.stack([rng.uniform(-1e4, 1e4, ntri), rng.uniform(-1e4, 1e4, ntri),
+                    rng.uniform(0, 1.2e4, ntri)], 1)
+    pts[::3, 2] = 0.0
+    tu = np.zeros((ntri, 3, 3))
+    for i in range(ntri):
+        for s in range(3):
+            b = [0.0, 0.0, 0.0]
+            b[s] = 1.0
+            u = np.zeros(3)
+            REF.ref_displacement_nikkhoo15(*[D(v) for v in pts[i]],
+                                           P(np.ascontiguousarray(P1[i])),
+                                           P(np.ascontiguousarray(P2[i])),
+                                           P(np.ascontiguousarray(P3[i])), *[D(v) for v in b],
+                                           D(30e3), D(30e3), P(u))
+            tu[i, s] = u
+    np.savez_compressed(os.path.join(HERE, "displacement.npz"), rect_inputs=rin, rect_u=ru,
+                        P1=P1, P2=P2, P3=P3, tri_points=pts, tri_u=tu)
+
+
+FIXTURES = {
+    "okada_dc3d": (20260101, [dc3d_cases, okada_kernels, nikkhoo_strain, nikkhoo_kernels,
+                              strainvolume_kernels]),
+    "displacement": (20261202, [displacement]),
+}
+
 if __name__ == "__main__":
-    rng = np.random.default_rng(20260101)
-    dc3d_cases(rng)
-    okada_kernels(rng)
-    nikkhoo_strain(rng)
-    nikkhoo_kernels(rng)
-    strainvolume_kernels(rng)
+    import sys
+    # the first five fixtures share one random stream (one seed, in order)
+    names = sys.argv[1:] or list(FIXTURES)
+    for name in names:
+        seed, fns = FIXTURES["okada_dc3d" if name in ("okada_traction", "nikkhoo_strain",
+                                                       "nikkhoo_traction", "strainvolume")
+                             else name]
+        rng = np.random.default_rng(seed)
+        for fn in fns:
+            fn(rng)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -72,6 +72,8 @@ def oracle() -> C.CDLL:
         lib.orc_rkqs.restype = I
         lib.orc_integrate.restype = C.c_long
         lib.orc_stress_strainvolume.restype = I
+        lib.orc_displacement_okada92.restype = I
+        lib.orc_buildO.restype = I
         _ORC = lib
     return _ORC
 
@@ -117,6 +119,28 @@ def buildG(mesh: Mesh, nthreads: int = 1) -> np.ndarray:
     if rc:
         raise RuntimeError(f"orc_buildG failed {rc}")
     return K
+
+
+def buildO(mesh: Mesh, points, nthreads: int = 1):
+    """(O, Of, Ol) of greens.f90:302-462, each (3 npts x n) row-major."""
+    x = f64(points)
+    nobs = x.size // 3
+    shp = (3 * nobs, mesh.shape[1])
+    O, Of, Ol = np.empty(shp), np.empty(shp), np.empty(shp)
+    rc = oracle().orc_buildO(C.byref(mesh.c), nobs, p(x), p(O), p(Of), p(Ol), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_buildO failed {rc}")
+    return O, Of, Ol
+
+
+def slip_vector(y, np_, nvol):
+    """u of exportPoints (ratestate.f90:431-447, viscouscycles.f90:539-580):
+    strike and dip slip per patch, then the six strain entries per volume."""
+    y = np.asarray(y)
+    u = [y[:7 * np_].reshape(np_, 7)[:, :2].ravel()]
+    if nvol:
+        u.append(y[7 * np_:].reshape(nvol, 12)[:, :6].ravel())
+    return np.concatenate(u)
 
 
 class System:

@@ -77,3 +77,45 @@ def test_deck_run_matches_oracle(tmp_path, engine_factory):
     ratestate.run(d2, max_iterations=steps, chunk=37, out=io.StringIO())
     for name in ("time.txt", "patch-00000001.dat", "patch-00000017.dat"):
         assert open(os.path.join(wdir, name)).read() == open(os.path.join(wdir2, name)).read()
+
+
+def test_deck_run_writes_observation_points(tmp_path, engine_factory):
+    """opts-<name>.dat (ratestate.f90:291-304, 414-486): per iteration the time
+    and the displacement of each observation point at the iteration's start.
+    With every patch observed, each row equals the oracle's O (buildO) times
+    the slip written in the patch files for the same iteration."""
+    m = configs.thrust_fault(nx=8, nz=4)
+    wdir = str(tmp_path / "pts")
+    pts = [("GPS1", 3000.0, -2000.0, 0.0), ("STATION_LONGNAME", 8000.0, 20000.0, 0.0),
+           ("deep", 12000.0, 4000.0, 1500.0)]
+    text = deck.format_rectangles(wdir, m.lambda_, m.mu, 1.0e9, m.x, m.length, m.width,
+                                  m.strike, m.dip, *m.friction(),
+                                  observe=list(range(1, m.n + 1)), points=pts)
+    d = deck.parse(text)
+    assert [p[0] for p in d.observation_points] == ["GPS1", "STATION_LO", "deep"]
+    steps = 150
+    ratestate.run(d, max_iterations=steps, chunk=steps, out=io.StringIO())
+    tt = _read(os.path.join(wdir, "time.txt"), [19, 19])
+    O, _, _ = orc.buildO(orc.Mesh(d.lambda_, d.mu, rect=(d.rect_x, d.rect_length, d.rect_width,
+                                                         d.rect_strike, d.rect_dip)),
+                         [p[1:] for p in d.observation_points])
+    slip = np.zeros((steps, 2 * m.n))
+    for e in range(m.n):
+        p = _read(os.path.join(wdir, f"patch-{e + 1:08d}.dat"), [19] + [20] * 14)
+        slip[:, 2 * e:2 * e + 2] = p[:, 1:3]
+    want = slip @ O.T
+    scale = np.abs(want).max()
+    assert scale > 0
+    for j, name in enumerate(["GPS1", "STATION_LO", "deep"]):
+        q = _read(os.path.join(wdir, f"opts-{name}.dat"), [19] * 4)
+        assert q.shape == (steps, 4)
+        np.testing.assert_array_equal(q[:, 0], tt[1:, 0])
+        np.testing.assert_allclose(q[:, 1:], want[:, 3 * j:3 * j + 3], rtol=1e-6,
+                                   atol=1e-9 * scale)
+    # the run split into calls writes the same files
+    wdir2 = str(tmp_path / "pts2")
+    ratestate.run(deck.parse(text.replace(wdir, wdir2)), max_iterations=steps, chunk=41,
+                  out=io.StringIO())
+    for name in ("GPS1", "STATION_LO", "deep"):
+        f = f"opts-{name}.dat"
+        assert open(os.path.join(wdir, f)).read() == open(os.path.join(wdir2, f)).read()

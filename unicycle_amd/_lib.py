@@ -58,6 +58,9 @@ class StepLog(C.Structure):
         ("obs_index", C.POINTER(C.c_int64)),
         ("obs_y", C.POINTER(C.c_double)),
         ("obs_dydt", C.POINTER(C.c_double)),
+        ("disp", C.POINTER(C.c_double)),
+        ("disp_f", C.POINTER(C.c_double)),
+        ("disp_l", C.POINTER(C.c_double)),
     ]
 
 
@@ -97,6 +100,11 @@ _SIGS = {
     "ucy_get_kernel": (_i, [_vp, _pd]),
     "ucy_set_kernel": (_i, [_vp, _pd, _i64, _i64]),
     "ucy_kernel_dev": (_vp, [_vp]),
+    "ucy_build_displacement": (_i, [_vp, _i, _pd]),
+    "ucy_displacement_shape": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ucy_get_displacement_kernel": (_i, [_vp, _pd, _pd, _pd]),
+    "ucy_displacement": (_i, [_vp, _pd, _pd, _pd, _pd]),
+    "ucy_displacement_dev": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ucy_matvec_f64": (_i, [_vp, _pd, _pd]),
     "ucy_matvec_f64_dev": (_i, [_vp, _vp, _vp]),
     "ucy_matvec_reduced": (_i, [_vp, _pd, _pd]),

@@ -225,6 +225,7 @@ extern "C" void ucy_destroy(ucy_ctx *c)
     if (c->K) hipFree(c->K);
     if (c->Kp) hipFree(c->Kp);
     if (c->Krs) hipFree(c->Krs);
+    if (c->O) hipFree(c->O);
     if (c->d_fr) hipFree(c->d_fr);
     if (c->d_rh) hipFree(c->d_rh);
     if (c->d_ctl) hipFree(c->d_ctl);
@@ -546,6 +547,147 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
 }
 
 // ---------------------------------------------------------------------------
+// observation points: buildO (greens.f90:302-462) and exportPoints
+// ---------------------------------------------------------------------------
+extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
+{
+    if (!c || npts < 0 || (npts > 0 && !x)) return UCY_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if (c->graph) {  // a captured attempt may log through the old O
+        hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (c->O) HIPCHK(hipFree(c->O));
+    c->O = nullptr;
+    c->o_rows = c->o_n = 0;
+    if (npts == 0) return UCY_OK;
+    if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
+    const int np = c->nrect + c->ntri;
+    if (np + c->nvol == 0) return fail(c, UCY_ERR_STATE, "no elements");
+    const int64_t rows = 3 * (int64_t)npts, n = 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    double *d_pts = nullptr, *O = nullptr;
+    HIPCHK(hipMalloc((void **)&d_pts, sizeof(double) * 3 * (size_t)npts));
+    HIPCHK(hipMalloc((void **)&O, sizeof(double) * (size_t)(rows * n)));
+    HIPCHK(hipMemcpyAsync(d_pts, x, sizeof(double) * 3 * (size_t)npts, hipMemcpyHostToDevice,
+                          c->stream));
+    int rc = 0;
+    if (c->nrect) {
+        std::vector<OkadaSource> S(c->nrect);
+        for (int i = 0; i < c->nrect; i++)
+            S[i] = make_source(&c->rx[3 * i], c->rL[i], c->rW[i], c->rstrike[i], c->rdip[i]);
+        OkadaSource *d_S = nullptr;
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(OkadaSource) * c->nrect));
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(OkadaSource) * c->nrect,
+                              hipMemcpyHostToDevice, c->stream));
+        launch_okada_disp(c->stream, make_medium(c->mu, c->lambda), d_S, c->nrect, d_pts, npts, O,
+                          n, 0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(d_S));
+    }
+    if (c->ntri)
+        rc = ext_build_triangle_disp(c->stream, c->mu, c->lambda, c->ntri, c->i1.data(),
+                                     c->i2.data(), c->i3.data(), c->vert.data(), d_pts, npts, O, n,
+                                     2 * (int64_t)c->nrect, c->err);
+    if (!rc && c->nvol)
+        rc = ext_build_volume_disp(c->stream, c->mu, c->lambda, c->nvol, c->vx.data(),
+                                   c->vL.data(), c->vT.data(), c->vW.data(), c->vstrike.data(), x,
+                                   d_pts, npts, O, n, 2 * (int64_t)np, c->err);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(d_pts));
+    if (rc) {
+        HIPCHK(hipFree(O));
+        return rc;
+    }
+    c->O = O;
+    c->o_rows = rows;
+    c->o_n = n;
+    return UCY_OK;
+}
+
+extern "C" int ucy_displacement_shape(const ucy_ctx *c, int64_t *rows, int64_t *n)
+{
+    if (!c) return UCY_ERR_ARG;
+    if (rows) *rows = c->O ? c->o_rows : 0;
+    if (n) *n = c->O ? c->o_n : 0;
+    return UCY_OK;
+}
+
+// O built, and for the current mesh
+static int check_disp(ucy_ctx *c)
+{
+    if (!c->O) return fail(c, UCY_ERR_STATE, "displacement kernels not built");
+    if (c->o_n != 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol)
+        return fail(c, UCY_ERR_STATE, "displacement kernels built for another mesh");
+    return 0;
+}
+
+extern "C" int ucy_get_displacement_kernel(ucy_ctx *c, double *O, double *Of, double *Ol)
+{
+    if (!c) return UCY_ERR_ARG;
+    int rc = check_disp(c);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t cnt = (size_t)(c->o_rows * c->o_n);
+    std::vector<double> tmp;
+    double *h = O;
+    if (!h) {
+        tmp.resize(cnt);
+        h = tmp.data();
+    }
+    HIPCHK(hipMemcpyAsync(h, c->O, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t nf = 2 * (int64_t)(c->nrect + c->ntri);
+    for (int64_t r = 0; r < c->o_rows; r++) {
+        const bool vertical = r % 3 == 2;
+        for (int64_t j = 0; j < c->o_n; j++) {
+            const double v = h[r * c->o_n + j];
+            const bool f = j < nf || vertical;
+            if (Of) Of[r * c->o_n + j] = f ? v : 0.0;
+            if (Ol) Ol[r * c->o_n + j] = f ? 0.0 : v;
+        }
+    }
+    return UCY_OK;
+}
+
+extern "C" int ucy_displacement_dev(ucy_ctx *c, const double *y, double *d, double *df,
+                                    double *dl)
+{
+    if (!c || !y || !d) return UCY_ERR_ARG;
+    int rc = check_disp(c);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    launch_disp_gemv(c->stream, c->O, c->o_rows, c->o_n, c->nrect + c->ntri, y, d, df, dl,
+                     nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+extern "C" int ucy_displacement(ucy_ctx *c, const double *y, double *d, double *df, double *dl)
+{
+    if (!c || !y || !d) return UCY_ERR_ARG;
+    int rc = check_disp(c);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const int64_t dim = 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
+    const int64_t R = c->o_rows;
+    double *buf = nullptr;
+    HIPCHK(hipMalloc((void **)&buf, sizeof(double) * (size_t)(dim + 3 * R)));
+    double *dy = buf, *dd = buf + dim, *ddf = dd + R, *ddl = ddf + R;
+    HIPCHK(hipMemcpyAsync(dy, y, sizeof(double) * dim, hipMemcpyHostToDevice, c->stream));
+    launch_disp_gemv(c->stream, c->O, R, c->o_n, c->nrect + c->ntri, dy, dd, df ? ddf : nullptr,
+                     dl ? ddl : nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(d, dd, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    if (df) HIPCHK(hipMemcpyAsync(df, ddf, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    if (dl) HIPCHK(hipMemcpyAsync(dl, ddl, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(buf));
+    return UCY_OK;
+}
+
+// ---------------------------------------------------------------------------
 // matvec / RHS / stepper
 // ---------------------------------------------------------------------------
 static int upload_params(ucy_ctx *c)
@@ -830,6 +972,9 @@ static int capture_attempt(ucy_ctx *c)
     launch_rhs_finish(c->stream, d, K, c->d_fr, b, c->d_ctl);
     launch_controller(c->stream, b, c->d_ctl);
     if (b.n_obs > 0) launch_obs_log(c->stream, b, c->d_ctl);
+    if (b.disp)
+        launch_disp_gemv(c->stream, b.O, b.o_rows, b.o_n, d.np, b.y, b.disp, b.disp_f, b.disp_l,
+                         c->d_ctl);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -843,7 +988,7 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                              ucy_run_stats *stats)
 {
     ucy_step_log log = {t_log, dt_log, vmax_log, nullptr, nullptr, log_cap, 0, nullptr, nullptr,
-                        nullptr};
+                        nullptr, nullptr, nullptr, nullptr};
     return ucy_integrate_ex(c, y, 0.0, interval, dt0, max_steps, epsilon, max_step, &log, stats);
 }
 
@@ -904,6 +1049,36 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
         b.obs_y = d_obs;
         b.obs_dydt = d_obs + n_obs * cap;
     }
+    // observation-point displacement (needs the step log and O)
+    const bool want_disp = log && (log->disp || log->disp_f || log->disp_l);
+    double *d_disp = nullptr;
+    b.O = nullptr;
+    b.o_rows = b.o_n = 0;
+    b.disp = b.disp_f = b.disp_l = nullptr;
+    if (want_disp) {
+        rc = cap > 0 ? check_disp(c)
+                     : fail(c, UCY_ERR_ARG, "displacement logging needs the step log");
+        if (rc) {
+            if (d_log) hipFree(d_log);
+            if (d_oidx) {
+                hipFree(d_oidx);
+                hipFree(d_obs);
+            }
+            b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+            b.obs_idx = nullptr;
+            b.n_obs = 0;
+            b.obs_y = b.obs_dydt = nullptr;
+            return rc;
+        }
+        const int64_t R = c->o_rows;
+        HIPCHK(hipMalloc((void **)&d_disp, sizeof(double) * 3 * R * cap));
+        b.O = c->O;
+        b.o_rows = R;
+        b.o_n = c->o_n;
+        b.disp = d_disp;
+        b.disp_f = log->disp_f ? d_disp + R * cap : nullptr;
+        b.disp_l = log->disp_l ? d_disp + 2 * R * cap : nullptr;
+    }
     Control &h = *c->h_ctl;
     memset(&h, 0, sizeof(Control));
     h.time = t0;
@@ -924,10 +1099,13 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
             hipFree(d_oidx);
             hipFree(d_obs);
         }
+        if (d_disp) hipFree(d_disp);
         b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
         b.obs_idx = nullptr;
         b.n_obs = 0;
         b.obs_y = b.obs_dydt = nullptr;
+        b.O = nullptr;
+        b.disp = b.disp_f = b.disp_l = nullptr;
         return rc;
     }
     auto w0 = std::chrono::steady_clock::now();
@@ -964,6 +1142,13 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
             HIPCHK(hipMemcpyAsync(log->obs_dydt, b.obs_dydt, sizeof(double) * n_obs * h.log_count,
                                   hipMemcpyDeviceToHost, c->stream));
         }
+        const size_t dbytes = sizeof(double) * (size_t)(b.o_rows * h.log_count);
+        if (want_disp && log->disp)
+            HIPCHK(hipMemcpyAsync(log->disp, b.disp, dbytes, hipMemcpyDeviceToHost, c->stream));
+        if (want_disp && log->disp_f)
+            HIPCHK(hipMemcpyAsync(log->disp_f, b.disp_f, dbytes, hipMemcpyDeviceToHost, c->stream));
+        if (want_disp && log->disp_l)
+            HIPCHK(hipMemcpyAsync(log->disp_l, b.disp_l, dbytes, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (d_log) {
@@ -979,6 +1164,11 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
         b.obs_idx = nullptr;
         b.n_obs = 0;
         b.obs_y = b.obs_dydt = nullptr;
+    }
+    if (d_disp) {
+        HIPCHK(hipFree(d_disp));
+        b.O = nullptr;
+        b.disp = b.disp_f = b.disp_l = nullptr;
     }
     if (stats) {
         stats->time = h.time;

@@ -7,6 +7,7 @@
 // receiver's rows is written as one contiguous 1 KiB run (2 doubles per lane,
 // columns 2s and 2s+1 of greens.f90:214-216).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "ucy_internal.h"
 
 namespace ucy {
@@ -100,6 +101,51 @@ void launch_okada_rows(hipStream_t st, const OkadaMedium &M, const OkadaSource *
     else
         hipLaunchKernelGGL(okada_rows_kernel<1>, grid, dim3(256), 0, st, M, src, ns, rcv, K, ldk,
                            row0, col0, s_out, d_out, out_stride);
+}
+
+// Displacement at observation points from rectangle sources, buildO
+// (greens.f90:302-462) with computeDisplacementOkada92 (okada92.f90:230-298):
+// one (point, source) pair per lane, both unit slips; rows 3k..3k+2 of O
+// (NED components at point k), columns col0 + 2s, col0 + 2s + 1.  A point on
+// a fault edge (DC3D singular) gets zero displacement.
+__global__ void __launch_bounds__(256)
+okada_disp_kernel(OkadaMedium M, const OkadaSource *__restrict__ src, int ns,
+                  const double *__restrict__ pts, double *__restrict__ O, int64_t ldo, int64_t col0)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (s >= ns) return;
+    const OkadaSource S = src[s];
+    // okada92.f90:262-268
+    const double dx = pts[3 * k] - S.q1, dy = pts[3 * k + 1] - S.q2;
+    const float X = (float)(dx * S.csst + dy * S.ssst);
+    const float Y = (float)(dx * S.ssst - dy * S.csst);
+    const float Z = (float)(-pts[3 * k + 2]);
+    float us[3], ud[3];
+    const int iret = dc3d_pair_out<0, 3>(M, S, X, Y, Z, us, ud);
+    double os[3] = {0.0, 0.0, 0.0}, od[3] = {0.0, 0.0, 0.0};
+    if (iret == 0) {  // okada92.f90:286-295
+        os[0] = (double)us[0] * S.csst + (double)us[1] * S.ssst;
+        os[1] = (double)us[0] * S.ssst - (double)us[1] * S.csst;
+        os[2] = -(double)us[2];
+        od[0] = (double)ud[0] * S.csst + (double)ud[1] * S.ssst;
+        od[1] = (double)ud[0] * S.ssst - (double)ud[1] * S.csst;
+        od[2] = -(double)ud[2];
+    }
+    const int64_t base = 3 * (int64_t)k * ldo + col0 + 2 * (int64_t)s;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        *reinterpret_cast<double2 *>(O + base + c * ldo) = make_double2(os[c], od[c]);
+}
+
+void launch_okada_disp(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
+                       const double *pts, int npts, double *O, int64_t ldo, int64_t col0)
+{
+    for (int k0 = 0; k0 < npts && ns > 0; k0 += 65535) {
+        const int nk = std::min(65535, npts - k0);
+        hipLaunchKernelGGL(okada_disp_kernel, dim3((ns + 255) / 256, nk), dim3(256), 0, st, M, src,
+                           ns, pts + 3 * (int64_t)k0, O + 3 * (int64_t)k0 * ldo, ldo, col0);
+    }
 }
 
 }  // namespace ucy

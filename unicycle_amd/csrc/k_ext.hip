@@ -86,9 +86,48 @@ VD_FN VD sv_acoth(const VD &X)
     return VD(v, d * X.g[0], d * X.g[1], d * X.g[2]);
 }
 
+// value only: the displacement kernels need no derivatives
+struct VV {
+    double v;
+    __host__ __device__ VV() : v(0.0) {}
+    __host__ __device__ VV(double c) : v(c) {}
+    __host__ __device__ double d(int) const { return 0.0; }
+    __host__ __device__ double val() const { return v; }
+};
+VD_FN VV operator+(const VV &a, const VV &b) { return VV(a.v + b.v); }
+VD_FN VV operator-(const VV &a, const VV &b) { return VV(a.v - b.v); }
+VD_FN VV operator-(const VV &a) { return VV(-a.v); }
+VD_FN VV operator*(const VV &a, const VV &b) { return VV(a.v * b.v); }
+VD_FN VV operator/(const VV &a, const VV &b) { return VV(a.v / b.v); }
+VD_FN VV operator+(const VV &a, double b) { return VV(a.v + b); }
+VD_FN VV operator+(double b, const VV &a) { return VV(a.v + b); }
+VD_FN VV operator-(const VV &a, double b) { return VV(a.v - b); }
+VD_FN VV operator-(double b, const VV &a) { return VV(b - a.v); }
+VD_FN VV operator*(const VV &a, double b) { return VV(a.v * b); }
+VD_FN VV operator*(double b, const VV &a) { return VV(a.v * b); }
+VD_FN VV operator/(const VV &a, double b) { return VV(a.v / b); }
+VD_FN VV sv_sqrt(const VV &a) { return VV(sqrt(a.v)); }
+VD_FN VV sv_atan2(const VV &y, const VV &x) { return VV(atan2(y.v, x.v)); }
+VD_FN VV sv_atan3(const VV &y, const VV &x)
+{
+    return VV(x.v == 0.0 ? copysign(1.5707963267948966, y.v) : atan(y.v / x.v));
+}
+VD_FN VV sv_xlogy(const VV &x, const VV &y) { return VV(x.v == 0.0 ? 0.0 : x.v * log(y.v)); }
+VD_FN VV sv_acoth(const VV &X)
+{
+    return VV(0.5 * (log(fabs(X.v + 1.0)) - log(fabs(X.v - 1.0))));
+}
+
 }  // namespace ucy
 
 #include "sv_jint.h"
+
+namespace svj {
+template <>
+struct has_derivative<ucy::VV> {
+    static constexpr bool value = false;
+};
+}  // namespace svj
 
 namespace ucy {
 
@@ -164,6 +203,34 @@ sv_rows_kernel(double G, double nu, const SvSource *__restrict__ src, int ns,
             for (int c = 0; c < NC; c++) out[(int64_t)(e * NC + c) * out_stride + s] = o[e][c];
 }
 
+struct SeedVV {
+    __host__ __device__ VV operator()(double v, int) const { return VV(v); }
+};
+
+// displacement at observation points from strain-volume sources (buildO,
+// greens.f90:423-448 with computeDisplacementVerticalStrainVolume,
+// strainvolume.f90:215-268): one (point, volume) pair per lane, the six unit
+// eigenstrains from one 8-corner sum; rows 3k..3k+2, columns col0 + 6s + e
+__global__ void __launch_bounds__(128)
+sv_disp_kernel(double G, double nu, const SvSource *__restrict__ src, int ns,
+               const double *__restrict__ pts, double *__restrict__ O, int64_t ldo, int64_t col0)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (s >= ns) return;
+    const SvSource S = src[s];
+    double u[6][3];
+    svj::displacement6<VV>(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2], S.q[0], S.q[1], S.q[2], S.L,
+                           S.T, S.W, S.strike, G, nu, SeedVV(), u);
+    const int64_t base = 3 * (int64_t)k * ldo + col0 + 6 * (int64_t)s;
+    for (int c = 0; c < 3; c++) {
+        double2 *p = reinterpret_cast<double2 *>(O + base + c * ldo);
+        p[0] = make_double2(u[0][c], u[1][c]);
+        p[1] = make_double2(u[2][c], u[3][c]);
+        p[2] = make_double2(u[4][c], u[5][c]);
+    }
+}
+
 static void launch_sv_rows(hipStream_t st, double G, double nu, const SvSource *src, int ns,
                            const Receiver *rcv, int nr, int kind, double *K, int64_t ldk,
                            int64_t row0, int64_t col0, double *out, int stride)
@@ -220,6 +287,46 @@ int ext_build_volume_columns(hipStream_t st, double G, double lambda, int nvol, 
     if (dS) (void)hipFree(dS);
     if (e != hipSuccess) {
         msg = std::string("strain-volume assembly: ") + hipGetErrorString(e);
+        return UCY_ERR_HIP;
+    }
+    return 0;
+}
+
+int ext_build_volume_disp(hipStream_t st, double G, double lambda, int nvol, const double *x,
+                          const double *L, const double *T, const double *W, const double *strike,
+                          const double *pts, const double *d_pts, int npts, double *O,
+                          int64_t ldo, int64_t col0, std::string &msg)
+{
+    // buildO passes nu = lambda/(lambda+mu)/2 (greens.f90:337)
+    const double nu = lambda / (lambda + G) / 2.0;
+    std::vector<SvSource> S(nvol);
+    for (int i = 0; i < nvol; i++) {
+        S[i] = {{x[3 * i], x[3 * i + 1], x[3 * i + 2]}, L[i], T[i], W[i], strike[i]};
+        if (sv_bad(nu, 0.0, S[i].q[2])) {
+            msg = "strain volume: -1<=nu<=0.5 and source depth >= 0 required";
+            return UCY_ERR_GEOMETRY;
+        }
+    }
+    for (int k = 0; k < npts; k++)
+        if (pts[3 * k + 2] < 0.0) {  // strainvolume.f90:236-239
+            msg = "strain volume: observation depth must be positive";
+            return UCY_ERR_GEOMETRY;
+        }
+    SvSource *dS = nullptr;
+    hipError_t e = hipMalloc((void **)&dS, sizeof(SvSource) * nvol);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dS, S.data(), sizeof(SvSource) * nvol, hipMemcpyHostToDevice, st);
+    for (int k0 = 0; k0 < npts && e == hipSuccess; k0 += 65535) {
+        const int nk = std::min(65535, npts - k0);
+        hipLaunchKernelGGL(sv_disp_kernel, dim3((nvol + 127) / 128, nk), dim3(128), 0, st, G, nu,
+                           dS, nvol, d_pts + 3 * (int64_t)k0, O + 3 * (int64_t)k0 * ldo, ldo,
+                           col0);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (dS) (void)hipFree(dS);
+    if (e != hipSuccess) {
+        msg = std::string("strain-volume displacement kernels: ") + hipGetErrorString(e);
         return UCY_ERR_HIP;
     }
     return 0;

@@ -671,6 +671,77 @@ __global__ void obs_log_kernel(RkBuffers b, Control *ctl)
     }
 }
 
+// exportPoints (ratestate.f90:414-486, viscouscycles.f90:523-625): d = O u,
+// the reference's DGEMV("T") on O(n, 3 npts), with u gathered from the state
+// (strike and dip slip of each patch, e11..e33 of each volume).  One block per
+// row of O, four waves across its columns.  df and dl split the same products
+// as buildO splits O into Of and Ol (greens.f90:388-445): fault columns and
+// the volumes' third component go to df, the volumes' first two to dl.
+__global__ void __launch_bounds__(256)
+disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np,
+                 const double *__restrict__ y, double *d, double *df, double *dl,
+                 const Control *ctl)
+{
+    int64_t off = 0;
+    if (ctl) {
+        if (!ctl->obs_pending) return;
+        const int64_t row = ctl->log_count - 1;
+        if (row < 0 || row >= ctl->log_cap) return;
+        off = row * rows;
+    }
+    const int64_t r = blockIdx.x;
+    const bool vertical = (r % 3) == 2;
+    const double *Or = O + r * n;
+    const int64_t nf = 2 * (int64_t)np, vbase = 7 * (int64_t)np;
+    double a = 0.0, af = 0.0, al = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+        double u;
+        if (j < nf) {
+            u = y[7 * (j >> 1) + (j & 1)];
+        } else {
+            const int64_t q = (j - nf) / 6;
+            u = y[vbase + 12 * q + (j - nf - 6 * q)];
+        }
+        const double p = Or[j] * u;
+        a += p;
+        if (j < nf || vertical) af += p;
+        else al += p;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        af += __shfl_xor(af, o, 64);
+        al += __shfl_xor(al, o, 64);
+    }
+    __shared__ double part[3][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = a;
+        part[1][w] = af;
+        part[2][w] = al;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < nw; k++) {
+            s0 += part[0][k];
+            s1 += part[1][k];
+            s2 += part[2][k];
+        }
+        d[off + r] = s0;
+        if (df) df[off + r] = s1;
+        if (dl) dl[off + r] = s2;
+    }
+}
+
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np,
+                      const double *y, double *d, double *df, double *dl, const Control *ctl)
+{
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(disp_gemv_kernel, dim3((unsigned)rows), dim3(256), 0, st, O, rows, n, np, y,
+                       d, df, dl, ctl);
+}
+
 void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl)
 {
     const int64_t blocks = b.n_obs < 256 * 64 ? (b.n_obs + 255) / 256 : 64;

@@ -304,6 +304,52 @@ int ext_build_triangle_columns(hipStream_t st, double G, double lambda, int ntri
     return 0;
 }
 
+// displacement at observation points from triangle sources (buildO,
+// greens.f90:302-462): one (point, triangle) pair per lane, both unit slips;
+// rows 3k..3k+2 of O, columns col0 + 2s + {0,1}
+__global__ void __launch_bounds__(128)
+tri_disp_kernel(TriMedium M, const TriSource *__restrict__ src, int ns,
+                const double *__restrict__ pts, double *__restrict__ O, int64_t ldo, int64_t col0)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (s >= ns) return;
+    double us[3], ud[3];
+    tri_disp2(M, src[s], pts[3 * k], pts[3 * k + 1], pts[3 * k + 2], us, ud);
+    const int64_t base = 3 * (int64_t)k * ldo + col0 + 2 * (int64_t)s;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        *reinterpret_cast<double2 *>(O + base + c * ldo) = make_double2(us[c], ud[c]);
+}
+
+int ext_build_triangle_disp(hipStream_t st, double G, double lambda, int ntri, const int *i1,
+                            const int *i2, const int *i3, const double *vert, const double *d_pts,
+                            int npts, double *O, int64_t ldo, int64_t col0, std::string &msg)
+{
+    std::vector<TriSource> S(ntri);
+    for (int j = 0; j < ntri; j++)
+        S[j] = host_tri(vert + 3 * (i1[j] - 1), vert + 3 * (i2[j] - 1), vert + 3 * (i3[j] - 1));
+    TriSource *dS = nullptr;
+    hipError_t e = hipMalloc((void **)&dS, sizeof(TriSource) * ntri);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dS, S.data(), sizeof(TriSource) * ntri, hipMemcpyHostToDevice, st);
+    // computeDisplacementNikkhoo15: nu from lambda (nikkhoo15.f90:339)
+    const TriMedium M = tri_medium(G, lambda);
+    for (int k0 = 0; k0 < npts && e == hipSuccess; k0 += 65535) {
+        const int nk = std::min(65535, npts - k0);
+        hipLaunchKernelGGL(tri_disp_kernel, dim3((ntri + 127) / 128, nk), dim3(128), 0, st, M, dS,
+                           ntri, d_pts + 3 * (int64_t)k0, O + 3 * (int64_t)k0 * ldo, ldo, col0);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (dS) (void)hipFree(dS);
+    if (e != hipSuccess) {
+        msg = std::string("triangle displacement kernels: ") + hipGetErrorString(e);
+        return UCY_ERR_HIP;
+    }
+    return 0;
+}
+
 // straininhalfspace_ (StrainInHalfSpace.c:42-173) over n (point, triangle)
 // pairs, in the reference's east/north/up frame: main + image + harmonic
 // parts, each as stress and strain, with the image's yz and zz components of

@@ -281,11 +281,13 @@ __device__ __forceinline__ void dc_accum_image(double U[12], const double a[12],
 }
 
 // DC3D for unit strike slip and unit dip slip.  Returns 1 on the singular
-// edge (both outputs zero), 0 otherwise.  Displacement-gradient outputs only
-// (u[3..11] of the reference's 12; the displacement itself is not needed by
-// the stress path), rounded to binary32 as dc3d.f:168-179.
-__device__ __forceinline__ int dc3d_pair(const OkadaMedium &M, const OkadaSource &S, float X,
-                                         float Y, float Z, float gs[9], float gd[9])
+// edge (all outputs zero), 0 otherwise.  Writes outputs u[OUT0 .. OUT0+N) of
+// the reference's 12, rounded to binary32 as dc3d.f:168-179: the stress path
+// takes the displacement gradient (OUT0 = 3, N = 9), the observation-point
+// path the displacement (OUT0 = 0, N = 3).
+template <int OUT0, int N>
+__device__ __forceinline__ int dc3d_pair_out(const OkadaMedium &M, const OkadaSource &S, float X,
+                                             float Y, float Z, float gs[N], float gd[N])
 {
     const float epsf = 1.0e-6f;
     double Us[12], Ud[12];
@@ -311,7 +313,7 @@ __device__ __forceinline__ int dc3d_pair(const OkadaMedium &M, const OkadaSource
         if (Q == 0.0 && ((xi[0] * xi[1] <= 0.0 && et[0] * et[1] == 0.0) ||
                          (et[0] * et[1] <= 0.0 && xi[0] * xi[1] == 0.0))) {
 #pragma unroll
-            for (int i = 0; i < 9; i++) { gs[i] = 0.0f; gd[i] = 0.0f; }
+            for (int i = 0; i < N; i++) { gs[i] = 0.0f; gd[i] = 0.0f; }
             return 1;
         }
         int kxi[2] = {0, 0}, ket[2] = {0, 0};
@@ -340,8 +342,14 @@ __device__ __forceinline__ int dc3d_pair(const OkadaMedium &M, const OkadaSource
         }
     }
 #pragma unroll
-    for (int i = 0; i < 9; i++) { gs[i] = (float)Us[i + 3]; gd[i] = (float)Ud[i + 3]; }
+    for (int i = 0; i < N; i++) { gs[i] = (float)Us[i + OUT0]; gd[i] = (float)Ud[i + OUT0]; }
     return 0;
+}
+
+__device__ __forceinline__ int dc3d_pair(const OkadaMedium &M, const OkadaSource &S, float X,
+                                         float Y, float Z, float gs[9], float gd[9])
+{
+    return dc3d_pair_out<3, 9>(M, S, X, Y, Z, gs, gd);
 }
 
 // computeStressOkada92 (okada92.f90:174-205) from the binary32 gradients.

@@ -313,12 +313,21 @@ struct Grad {
     double u[3][6];
 };
 
+// number types that carry no derivatives (value-only evaluation for the
+// displacement) specialise this to false; their .d(j) is never called
+template <class T>
+struct has_derivative {
+    static constexpr bool value = true;
+};
+
 template <class T>
 SV_FN void add(Grad &G, int i, int k, double s, const T &v)
 {
-    G.g[i][k][0] += s * v.d(0);
-    G.g[i][k][1] += s * v.d(1);
-    G.g[i][k][2] += s * v.d(2);
+    if constexpr (has_derivative<T>::value) {
+        G.g[i][k][0] += s * v.d(0);
+        G.g[i][k][1] += s * v.d(1);
+        G.g[i][k][2] += s * v.d(2);
+    }
     G.u[i][k] += s * v.val();
 }
 
@@ -383,19 +392,12 @@ SV_FN void gradients(Grad &G, double x1p, double x2p, double x3, double q3, doub
     }
 }
 
-// Displacement (NED u1 u2 u3) at (x1, x2, x3) for the primed eigenstrain
-// epsp (11 12 13 22 23 33); strainvolume.f90:215-268
-template <class T, class Seed>
-SV_FN void displacement(double x1, double x2, double x3, double q1, double q2, double q3,
-                        double L, double Th, double W, double theta, const double epsp[6],
-                        double G, double nu, Seed seed, double u[3])
+// displacement of the primed eigenstrain epsp from the accumulated J sums,
+// rotated back from the primed frame (strainvolume.f90:262-268)
+SV_FN void disp_from_sums(const Grad &Gr, const double epsp[6], double G, double nu, double ct,
+                          double st, double u[3])
 {
     const double lambda = G * 2.0 * nu / (1.0 - 2.0 * nu);
-    const double ct = cos(theta), st = sin(theta);
-    const double x1p = (x1 - q1) * ct + (x2 - q2) * st;
-    const double x2p = -(x1 - q1) * st + (x2 - q2) * ct;
-    Grad Gr;
-    gradients<T>(Gr, x1p, x2p, x3, q3, L, Th, W, nu, seed);
     const double K = 1.0 / (16.0 * 3.141592653589793115997963468544185161 * (1.0 - nu) * G);
     const double tr = epsp[0] + epsp[3] + epsp[5];
     const double cf[6] = {lambda * tr + 2.0 * G * epsp[0], lambda * tr + 2.0 * G * epsp[3],
@@ -410,6 +412,41 @@ SV_FN void displacement(double x1, double x2, double x3, double q1, double q2, d
     u[0] = up[0] * ct - up[1] * st;
     u[1] = up[0] * st + up[1] * ct;
     u[2] = up[2];
+}
+
+// Displacement (NED u1 u2 u3) at (x1, x2, x3) for the primed eigenstrain
+// epsp (11 12 13 22 23 33); strainvolume.f90:215-268
+template <class T, class Seed>
+SV_FN void displacement(double x1, double x2, double x3, double q1, double q2, double q3,
+                        double L, double Th, double W, double theta, const double epsp[6],
+                        double G, double nu, Seed seed, double u[3])
+{
+    const double ct = cos(theta), st = sin(theta);
+    const double x1p = (x1 - q1) * ct + (x2 - q2) * st;
+    const double x2p = -(x1 - q1) * st + (x2 - q2) * ct;
+    Grad Gr;
+    gradients<T>(Gr, x1p, x2p, x3, q3, L, Th, W, nu, seed);
+    disp_from_sums(Gr, epsp, G, nu, ct, st, u);
+}
+
+// The same for the six unit eigenstrains at once (one 8-corner sum):
+// out[e] = displacement of unit eigenstrain e (buildO, greens.f90:423-448)
+template <class T, class Seed>
+SV_FN void displacement6(double x1, double x2, double x3, double q1, double q2, double q3,
+                         double L, double Th, double W, double theta, double G, double nu,
+                         Seed seed, double out[6][3])
+{
+    const double ct = cos(theta), st = sin(theta);
+    const double x1p = (x1 - q1) * ct + (x2 - q2) * st;
+    const double x2p = -(x1 - q1) * st + (x2 - q2) * ct;
+    Grad Gr;
+    gradients<T>(Gr, x1p, x2p, x3, q3, L, Th, W, nu, seed);
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        double ep[6] = {0, 0, 0, 0, 0, 0};
+        ep[e] = 1.0;
+        disp_from_sums(Gr, ep, G, nu, ct, st, out[e]);
+    }
 }
 
 // boxcar functions of strainvolume.f90:53-84

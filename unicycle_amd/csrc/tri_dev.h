@@ -382,4 +382,38 @@ __device__ __forceinline__ void tri_stress2(const TriMedium &M, const TriSource 
     }
 }
 
+// NED displacement at NED point (x1, x2, x3) for unit strike slip (us) and
+// unit dip slip (ud): computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367)
+// over dispinhalfspace_ (DispInHalfSpace.c:42-152).  The main, harmonic and
+// image parts are summed in that order; with every vertex at the surface the
+// image part's vertical component and then the total change sign.  Constant
+// seeds: only the values of the G3 arithmetic are used.
+__device__ __forceinline__ void tri_disp2(const TriMedium &M, const TriSource &S, double x1,
+                                          double x2, double x3, double us[3], double ud[3])
+{
+    const G3 X = gc(x2), Y = gc(x1), Z = gc(-x3);
+    const double cs[3] = {0.0, 1.0, 0.0}, cd[3] = {0.0, 0.0, -1.0};
+    G3 ms[3], md[3], fs[3], fd[3], is_[3], id[3];
+    td_full2(S.full[0], X, Y, Z, M.nu, cs, cd, ms, md);
+    td_fsc2(S, X, Y, Z, M.nu, fs, fd);
+    td_full2(S.full[1], X, Y, Z, M.nu, cs, cd, is_, id);
+    double ws[3], wd[3];
+    for (int i = 0; i < 3; i++) {
+        const bool flip = S.surface && i == 2;
+        ws[i] = ms[i].v + fs[i].v + (flip ? -is_[i].v : is_[i].v);
+        wd[i] = md[i].v + fd[i].v + (flip ? -id[i].v : id[i].v);
+        if (S.surface) {
+            ws[i] = -ws[i];
+            wd[i] = -wd[i];
+        }
+    }
+    // east, north, up -> north, east, down
+    us[0] = ws[1];
+    us[1] = ws[0];
+    us[2] = -ws[2];
+    ud[0] = wd[1];
+    ud[1] = wd[0];
+    ud[2] = -wd[2];
+}
+
 }  // namespace ucy

@@ -43,6 +43,10 @@ struct ucy_ctx {
     uint32_t *Kp = nullptr;
     double *Krs = nullptr;
     int kprec = 0;
+    // displacement kernels at observation points (buildO): o_rows x o_n
+    // row-major on the device, built for a mesh with o_n columns
+    double *O = nullptr;
+    int64_t o_rows = 0, o_n = 0;
     // wall seconds of the last ucy_build_kernel by source type (rectangle,
     // triangle, volume columns; each phase ends with a stream synchronize)
     double build_seconds[3] = {0, 0, 0};

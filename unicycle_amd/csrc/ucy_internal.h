@@ -62,6 +62,11 @@ void launch_okada_rows(hipStream_t st, const OkadaMedium &M, const OkadaSource *
                        const Receiver *rcv, int nr, int kind, double *K, int64_t ldk,
                        int64_t row0, int64_t col0, double *s_out, double *d_out, int out_stride);
 
+// displacement rows 3k..3k+2 of O for points k in [0, npts) (x (3,npts)) from all
+// rectangle sources, columns col0 + 2s + {0,1}
+void launch_okada_disp(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
+                       const double *pts, int npts, double *O, int64_t ldo, int64_t col0);
+
 // ---- GEMV / RHS launchers (k_ode.hip) ----
 void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t);
 // odefun building blocks
@@ -77,6 +82,11 @@ struct RkBuffers {
     const int64_t *obs_idx;
     int64_t n_obs;
     double *obs_y, *obs_dydt;
+    // observation-point displacement logged per accepted step (disp null: off):
+    // O (o_rows x o_n, row-major) times the slip / strain entries of y
+    const double *O;
+    int64_t o_rows, o_n;
+    double *disp, *disp_f, *disp_l;
     // reduced-precision K (null in binary64): slip rates and tractions of the
     // step start; the stage matvecs then take v - v0 and add t0
     double *v0, *t0;
@@ -107,6 +117,12 @@ void launch_rhs_finish(hipStream_t st, const PlainDims &d, const KOperand &K, co
                        const RkBuffers &b, Control *ctl);
 void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl);
 void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl);
+// observation-point displacement d = O u with u the slip (2 per patch) and
+// strain (6 per volume) entries of y; df, dl (may be null): the parts of buildO's
+// Of and Ol.  With ctl: written at log row log_count - 1 of the arrays, only in
+// the attempt that accepted a step.
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np,
+                      const double *y, double *d, double *df, double *dl, const Control *ctl);
 // standalone Cash-Karp combination kernels for the ucy_rkck entry point
 void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,
                        double *const *ak, double *yr, double *ytmp, double *yout, double *yerr);

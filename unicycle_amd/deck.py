@@ -234,7 +234,9 @@ def parse(text: str) -> Deck:
         t = L.values(f"observation point {k}", 5)
         if _int(t[0]) != k:
             raise DeckError("observation points: unexpected index")
-        d.observation_points.append((t[1], _real(t[2]), _real(t[3]), _real(t[4])))
+        # the name is CHARACTER(LEN=10) (types.f90:88): a list-directed read
+        # keeps its leftmost 10 characters
+        d.observation_points.append((t[1][:10], _real(t[2]), _real(t[3]), _real(t[4])))
     # events (ratestate.f90:1457-1484): the first is at t = 0 and has no line
     ne = _int(L.values("number of events", 1)[0])
     for k in range(1, ne + 1):

@@ -67,6 +67,9 @@ class IntegrateResult:
     reject_log: np.ndarray | None = None    # rejected attempts before each accepted one
     obs_y: np.ndarray | None = None         # (steps, n_obs) step-start values of observed entries
     obs_dydt: np.ndarray | None = None      # (steps, n_obs) their step-start derivatives
+    disp: np.ndarray | None = None          # (steps, 3 npts) observation-point displacement
+    disp_f: np.ndarray | None = None        # its fault part (Of u) and ...
+    disp_l: np.ndarray | None = None        # ... volume part (Ol u), at each step start
 
 
 class Engine:
@@ -205,6 +208,34 @@ class Engine:
     def matvec_dev(self, v_ptr: int, t_ptr: int):
         self._chk(self._L.ucy_matvec_f64_dev(self._h, C.c_void_p(v_ptr), C.c_void_p(t_ptr)))
 
+    # -- observation points (buildO, exportPoints) --------------------------
+    def buildO(self, points):
+        """Displacement kernels at observation points (npts, 3) north/east/down
+        (greens.f90:302-462), built and kept on the device."""
+        x = _f64(points).reshape(-1, 3)
+        self._chk(self._L.ucy_build_displacement(self._h, x.shape[0], _p(x)))
+
+    @property
+    def displacement_shape(self):
+        r, n = C.c_int64(), C.c_int64()
+        self._chk(self._L.ucy_displacement_shape(self._h, C.byref(r), C.byref(n)))
+        return r.value, n.value
+
+    def get_displacement_kernels(self):
+        """(O, Of, Ol) of buildO, each (3 npts, n) row-major."""
+        r, n = self.displacement_shape
+        O, Of, Ol = np.empty((r, n)), np.empty((r, n)), np.empty((r, n))
+        self._chk(self._L.ucy_get_displacement_kernel(self._h, _p(O), _p(Of), _p(Ol)))
+        return O, Of, Ol
+
+    def displacement(self, y):
+        """(d, df, dl) of exportPoints (ratestate.f90:414-486,
+        viscouscycles.f90:523-625) for the state y."""
+        r, _ = self.displacement_shape
+        d, df, dl = np.empty(r), np.empty(r), np.empty(r)
+        self._chk(self._L.ucy_displacement(self._h, _p(_f64(y)), _p(d), _p(df), _p(dl)))
+        return d, df, dl
+
     # -- RHS and stepper --------------------------------------------------
     def initStateVector(self, variant: int = 0) -> np.ndarray:
         y = np.empty(self.dim)
@@ -235,14 +266,15 @@ class Engine:
     def integrate(self, y, interval: float, dt0: float = 1.0e-3, max_steps: int = 1000000,
                   epsilon: float = 1.0e-6, max_step: float = 1.7976931348623158e308,
                   log_cap: int = 0, history: bool = False, observe=None,
-                  t0: float = 0.0) -> IntegrateResult:
+                  t0: float = 0.0, displacement: bool = False) -> IntegrateResult:
         """Main loop of ratestate.f90:307-328, device resident.  log_cap > 0
         logs time, dt and vMax per accepted step; history adds errmax/epsilon
         and the rejections of each step (the step-size proposal's data);
         observe (state indices, needs log_cap) adds the step-start values and
         derivatives of those entries per step (the export of
-        ratestate.f90:488-536).  t0 continues a run: the interval and the
-        logged times are absolute."""
+        ratestate.f90:488-536); displacement (needs log_cap and buildO) adds
+        the observation-point displacement at each step start (exportPoints).
+        t0 continues a run: the interval and the logged times are absolute."""
         y = _f64(y).copy()
         st = _lib.RunStats()
         logs = [np.empty(max(log_cap, 1)) for _ in range(5)]
@@ -259,6 +291,11 @@ class Engine:
             lg.n_obs = idx.size
             lg.obs_index = idx.ctypes.data_as(C.POINTER(C.c_int64))
             lg.obs_y, lg.obs_dydt = _p(obs[1]), _p(obs[2])
+        dsp = None
+        if displacement:
+            r, _ = self.displacement_shape
+            dsp = [np.empty((max(log_cap, 1), r)) for _ in range(3)]
+            lg.disp, lg.disp_f, lg.disp_l = (_p(a) for a in dsp)
         self._chk(self._L.ucy_integrate_ex(self._h, _p(y), float(t0), float(interval), float(dt0),
                                            int(max_steps), float(epsilon), float(max_step),
                                            C.byref(lg), C.byref(st)))
@@ -269,6 +306,8 @@ class Engine:
                               cut(logs[2]), cut(logs[3], history), cut(logs[4], history))
         if obs is not None:
             res.obs_y, res.obs_dydt = obs[1][:nlog].copy(), obs[2][:nlog].copy()
+        if dsp is not None:
+            res.disp, res.disp_f, res.disp_l = (a[:nlog].copy() for a in dsp)
         return res
 
     def time_rhs_kernel(self, iters: int = 50) -> float:

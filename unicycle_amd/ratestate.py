@@ -14,10 +14,13 @@ of iterations.  It writes what the reference writes each iteration
                                 patch's 7 state entries, then their
                                 derivatives, at the iteration's start
                                 (ES19.12E2, then ES20.12E3);
+  <wdir>/opts-<name>.dat        per observation point and iteration: the time
+                                and the displacement (north, east, down) at
+                                the iteration's start, d = O u of exportPoints
+                                (ratestate.f90:414-486), 4 x ES19.12E2;
   stdout                        every 50 iterations: iteration, time, step, vMax.
 
-Observation points (displacement at the surface) and the VTP geometry files
-are not written.
+The VTP geometry files are not written.
 """
 from __future__ import annotations
 
@@ -42,6 +45,8 @@ def setup(d: deckmod.Deck, device: int = 0, precision: int = PREC_F64) -> Engine
         eng.set_triangles(d.tri_i1, d.tri_i2, d.tri_i3, d.vertices)
     eng.set_friction(*d.friction_arrays())
     eng.buildG(precision)
+    if d.observation_points:   # buildO (greens.f90:302-462, ratestate.f90:208)
+        eng.buildO([p[1:] for p in d.observation_points])
     return eng
 
 
@@ -55,13 +60,16 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     time, dt_next, dt_done = 0.0, 1.0e-3, 0.0     # ratestate.f90:166-167
     obs = [j - 1 for j in d.observation_states]   # 0-based patches
     idx = np.array([DPATCH * e + k for e in obs for k in range(DPATCH)], dtype=np.int64)
-    files = []
+    files, pfiles = [], []
     if write_files:
         os.makedirs(d.wdir, exist_ok=True)
         ftime = open(os.path.join(d.wdir, "time.txt"), "w")
         ftime.write("#               time                 dt\n")
         ftime.write(fmt(0.0, 19, 12, 2) + fmt(dt_next, 19, 12, 2) + "\n")
         files = [open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w") for j in d.observation_states]
+        # ratestate.f90:291-304
+        pfiles = [open(os.path.join(d.wdir, f"opts-{p[0].strip()}.dat"), "w")
+                  for p in d.observation_points]
     out.write("#       n               time                 dt       vMax\n")
     out.write(f"{0:09d}" + fmt(time, 19, 12, 2) + fmt(dt_next, 19, 12, 2) + "\n")
     it = 0
@@ -70,7 +78,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
             n = min(chunk, max_iterations - it)
             res = eng.integrate(y, d.interval, dt0=dt_next, max_steps=n, epsilon=epsilon,
                                 max_step=max_step, log_cap=n, observe=idx if idx.size else None,
-                                t0=time)
+                                t0=time, displacement=bool(d.observation_points))
             for k in range(res.steps):
                 i = it + k + 1
                 t_start = res.t_log[k - 1] if k else time
@@ -83,6 +91,10 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
                         f.write(fmt(t_start, 19, 12, 2)
                                 + "".join(fmt(v, 20, 12, 3) for v in ys)
                                 + "".join(fmt(v, 20, 12, 3) for v in ds) + "\n")
+                    for j, f in enumerate(pfiles):
+                        f.write(fmt(t_start, 19, 12, 2)
+                                + "".join(fmt(v, 19, 12, 2) for v in res.disp[k, 3 * j:3 * j + 3])
+                                + "\n")
                 if i % 50 == 0:
                     out.write(f"{i:09d}" + fmt(t_start, 19, 12, 2) + fmt(prev, 19, 12, 2)
                               + fmt(res.vmax_log[k], 11, 4, 2) + "\n")
@@ -96,7 +108,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     finally:
         if write_files:
             ftime.close()
-            for f in files:
+            for f in files + pfiles:
                 f.close()
         eng.close()
     # the reference's loop index after EXIT (or past the last iteration)
@@ -133,8 +145,7 @@ def main(argv=None) -> int:
     print(f"# output directory\n  {d.wdir}")
     print(f"# {d.n_rect} rectangular and {d.n_tri} triangular patches, "
           f"{len(d.observation_states)} observed")
-    if d.observation_points:
-        print("# observation points are not written by this driver", file=sys.stderr)
+    print(f"# {len(d.observation_points)} observation points")
     if a.dry_run:
         print("dry run: abort calculation")
         return 0
