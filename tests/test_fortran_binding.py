@@ -2,7 +2,8 @@
 compiles with the image's Fortran compiler and links against libucy.so; on a
 GPU, a host program in the shape of ratestate.f90:307-328 (tests/
 fortran_driver.f90) steps through the library and matches the oracle's
-main loop."""
+main loop, and its buildO / exportPoints calls match the oracle's
+displacement kernels."""
 import os
 import shutil
 import subprocess
@@ -33,13 +34,18 @@ def _build(tmp):
     return exe
 
 
-def _write_mesh(path, model, nsteps):
+POINTS = np.array([[3e3, -2e3, 0.0], [8e3, 2e4, 0.0], [1.2e4, 4e3, 0.0]])
+
+
+def _write_mesh(path, model, nsteps, points=POINTS):
     with open(path, "wb") as f:
         np.array([model.n, nsteps], dtype=np.int32).tofile(f)
         np.array([model.lambda_, model.mu], dtype=np.float64).tofile(f)
         np.ascontiguousarray(model.x, dtype=np.float64).tofile(f)  # x(3,ns) column-major
         for arr in (model.length, model.width, model.strike, model.dip) + tuple(model.friction()):
             np.asarray(arr, dtype=np.float64).tofile(f)
+        np.array([len(points)], dtype=np.int32).tofile(f)
+        np.ascontiguousarray(points, dtype=np.float64).tofile(f)   # x(3,nobs)
 
 
 def test_fortran_binding_links(tmp_path):
@@ -70,7 +76,9 @@ def test_fortran_host_matches_oracle(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     out = np.fromfile(tmp_path / "out.bin", dtype=np.float64)
-    tl, dl, y = out[:nsteps], out[nsteps:2 * nsteps], out[2 * nsteps:]
+    nd = 3 * len(POINTS)
+    tl, dl, y = out[:nsteps], out[nsteps:2 * nsteps], out[2 * nsteps:2 * nsteps + 7 * model.n]
+    disp = out[2 * nsteps + 7 * model.n:].reshape(3, nd)
     K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
     sysm = orc.System(model.friction(), K)
     y0 = sysm.init_state()
@@ -78,3 +86,10 @@ def test_fortran_host_matches_oracle(tmp_path):
     np.testing.assert_allclose(dl, dlo, rtol=1e-5)
     np.testing.assert_allclose(tl, tlo, rtol=1e-5)
     np.testing.assert_allclose(y, yo, rtol=1e-6, atol=1e-9 * np.abs(yo).max())
+    # exportPoints through the binding: d = O u of the final state; no volumes,
+    # so the fault part is all of it
+    O, _, _ = orc.buildO(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), POINTS)
+    want = O @ orc.slip_vector(y, model.n, 0)
+    np.testing.assert_allclose(disp[0], want, rtol=1e-6, atol=1e-9 * np.abs(want).max())
+    np.testing.assert_array_equal(disp[1], disp[0])
+    assert not disp[2].any() and np.abs(want).max() > 0

@@ -12,6 +12,8 @@
 !!                                      ode45.f90:116-124
 !!   computeTractionKernelsOkada92(...) okada92.f90:368-377 (and the other
 !!                                      Green's-function routines)
+!!   buildO(in,layout,O,Of,Ol)          greens.f90:302 (O stays on the GPU)
+!!   exportPoints()                     ratestate.f90:414-486 (d = O u)
 !! A nonzero status from the library becomes the reference's STOP with the
 !! library's message (the library itself never aborts).
 !-----------------------------------------------------------------------
@@ -164,6 +166,22 @@ MODULE ucy_fortran
        INTEGER(C_INT) :: ucy_okada92_traction_kernels
      END FUNCTION ucy_okada92_traction_kernels
 
+     FUNCTION ucy_build_displacement(ctx, npts, x) BIND(C, NAME="ucy_build_displacement")
+       IMPORT :: C_PTR, C_INT, C_DOUBLE
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT), VALUE :: npts
+       REAL(C_DOUBLE), INTENT(IN) :: x(3,*)
+       INTEGER(C_INT) :: ucy_build_displacement
+     END FUNCTION ucy_build_displacement
+
+     FUNCTION ucy_displacement(ctx, y, d, df, dl) BIND(C, NAME="ucy_displacement")
+       IMPORT :: C_PTR, C_INT, C_DOUBLE
+       TYPE(C_PTR), VALUE :: ctx
+       REAL(C_DOUBLE), INTENT(IN) :: y(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: d(*), df(*), dl(*)
+       INTEGER(C_INT) :: ucy_displacement
+     END FUNCTION ucy_displacement
+
      FUNCTION ucy_strlen(s) BIND(C, NAME="strlen")
        IMPORT :: C_PTR, C_SIZE_T
        TYPE(C_PTR), VALUE :: s
@@ -276,5 +294,22 @@ CONTAINS
                    W, strike, dip, s, d, t, G, lambda, ts, td, tn), &
                    "computeTractionKernelsOkada92")
   END SUBROUTINE computeTractionKernelsOkada92_gpu
+
+  !> buildO of greens.f90:302 for nObs observation points x(3,nObs) (north,
+  !! east, down): the displacement kernels are assembled and kept on the GPU
+  SUBROUTINE buildO_gpu(nObs, x)
+    INTEGER, INTENT(IN) :: nObs
+    REAL*8, DIMENSION(3,nObs), INTENT(IN) :: x
+    CALL ucy_check(ucy_build_displacement(ucy_ctx, INT(nObs, C_INT), x), "buildO")
+  END SUBROUTINE buildO_gpu
+
+  !> the product of exportPoints (ratestate.f90:414-486; viscouscycles.f90:
+  !! 523-625): d = O u, df = Of u, dl = Ol u, u gathered from the state y
+  SUBROUTINE exportPoints_gpu(n, y, nd, d, df, dl)
+    INTEGER, INTENT(IN) :: n, nd
+    REAL*8, DIMENSION(n), INTENT(IN) :: y
+    REAL*8, DIMENSION(nd), INTENT(OUT) :: d, df, dl
+    CALL ucy_check(ucy_displacement(ucy_ctx, y, d, df, dl), "exportPoints")
+  END SUBROUTINE exportPoints_gpu
 
 END MODULE ucy_fortran
