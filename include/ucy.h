@@ -254,6 +254,10 @@ typedef struct {
      * step log and ucy_build_displacement): d, df, dl of ucy_displacement,
      * [cap][3 npts] each; any may be NULL (all NULL: off) */
     double *disp, *disp_f, *disp_l;
+    /* eMax of each logged step's start: the largest strain-rate norm
+     * sqrt(e:e/2) over the strain volumes (viscouscycles.f90:986; 0 without
+     * volumes), [cap]; NULL: off */
+    double *emax;
 } ucy_step_log;
 int ucy_integrate_ex(ucy_ctx *ctx, double *y, double t0, double interval, double dt0,
                      int64_t max_steps, double epsilon, double max_step,

@@ -109,3 +109,125 @@ def test_fortran_es_formats():
     assert f(6.02e23, 20, 12, 3) == " 6.020000000000E+023"
     assert f(-1.5e-10, 20, 12, 3) == "-1.500000000000E-010"
     assert f(2.5e-7, 11, 4, 2) == " 2.5000E-07"
+
+
+# ------------------------------------------------------------ viscouscycles --
+VDECK = """# viscouscycles deck: one rectangle, two volumes
+out/v1
+# elastic moduli
+30e3 30e3
+# time interval
+1e10
+# number of rectangular patches
+1
+1 1e-9 0 0 500 2000 2000 0 30 90
+# number of frictional rectangular patches
+1
+1 0.6 100 0.01 0.014 0.01 1e-6 5
+# number of triangular patches
+0
+# number of cuboid strain volumes
+2
+# n e11 e12 e13 e22 e23 e33 x1 x2 x3 length width thickness strike dip
+1 0 1d-15 0 0 0 0   0 0 20e3 4000 3000 2000 0 90
+2 0 1e-15 0 0 0 0 4000 0 20e3 4000 3000 2000 10 90
+# number of nonlinear Maxwell strain volumes
+2
+# n Gm gammadot0m n Q R
+1 1 5e-14 1 0 8.314
+2 1 5e-14 3 0 8.314
+# number of thermal strain volumes
+2
+1 1 873
+2 1 900
+# number of observation patches
+1
+1 1
+# number of observation volumes
+1
+1 2
+# number of observation points
+1
+1 GPS 1e3 2e3 0
+# number of perturbations
+1
+"""
+
+
+def test_parse_viscous_deck():
+    d = deck.parse_viscous(VDECK)
+    assert d.n_rect == 1 and d.n_tri == 0 and d.n_vol == 2
+    np.testing.assert_array_equal(d.vol_x, [[0, 0, 20e3], [4000, 0, 20e3]])
+    # the deck's order is length, width, thickness
+    np.testing.assert_array_equal(d.vol_length, [4000, 4000])
+    np.testing.assert_array_equal(d.vol_width, [3000, 3000])
+    np.testing.assert_array_equal(d.vol_thickness, [2000, 2000])
+    assert d.vol_strike[1] == 10 * deck.DEG2RAD and d.vol_dip[0] == 90 * deck.DEG2RAD
+    np.testing.assert_array_equal(d.vol_ebg[:, 1], [1e-15, 1e-15])
+    assert d.n_maxwell == 2 and d.maxwell["npower"] == [1.0, 3.0]
+    assert d.thermal["To"] == [873.0, 900.0]
+    assert d.observation_states == [1] and d.observation_volumes == [2]
+    assert d.observation_points == [("GPS", 1e3, 2e3, 0.0)] and d.events == [0.0]
+    from unicycle_amd import ratestate
+    g0, npow, ebg, Gm, Q, R, To = ratestate.rheology(d)
+    np.testing.assert_array_equal(npow, [1.0, 3.0])
+    np.testing.assert_array_equal(To, [873.0, 900.0])
+
+
+@pytest.mark.parametrize("old,new,needle", [
+    ("2 0 1e-15 0 0 0 0 4000 0 20e3 4000 3000 2000 10 90",
+     "2 0 1e-15 0 0 0 0 4000 0 20e3 4000 3000 2000 10 80", "vertical"),
+    ("1 0 1d-15 0 0 0 0   0 0 20e3", "1 0 1d-15 0 0 0 0   0 0 -2e3", "depth must be positive"),
+    ("2 0 1e-15 0 0 0 0 4000", "3 0 1e-15 0 0 0 0 4000", "unexpected index"),
+    ("1 0 1d-15 0 0 0 0   0 0 20e3 4000 3000 2000 0 90",
+     "1 0 1d-15 0 0 0 0   0 0 20e3 0 0 0 0 90", "dimension must be positive"),
+    ("# number of nonlinear Maxwell strain volumes\n2",
+     "# number of nonlinear Maxwell strain volumes\n1", "none or all"),
+    ("2 1 5e-14 3 0 8.314", "2 1 5e-14 3 0 0", "R must be positive"),
+    ("1 1 5e-14 1 0 8.314", "1 0 5e-14 1 0 8.314", "sig0 must be positive"),
+    ("2 1 900", "2 1 0", "To must be positive"),
+    ("# number of thermal strain volumes\n2", "# number of thermal strain volumes\n1",
+     "none or all"),
+    ("# number of observation volumes\n1\n1 2", "# number of observation volumes\n1\n1 3",
+     "not a volume"),
+])
+def test_viscous_deck_errors(old, new, needle):
+    assert old in VDECK
+    with pytest.raises(deck.DeckError, match=needle):
+        deck.parse_viscous(VDECK.replace(old, new, 1))
+
+
+def test_viscous_deck_without_thermal_rows():
+    """No thermal rows: allowed with Q = 0 (To does not enter), refused with
+    Q != 0 (the reference would read an unset temperature)."""
+    from unicycle_amd import ratestate
+    text = VDECK.replace("# number of thermal strain volumes\n2\n1 1 873\n2 1 900",
+                         "# number of thermal strain volumes\n0")
+    d = deck.parse_viscous(text)
+    assert not d.thermal
+    np.testing.assert_array_equal(ratestate.rheology(d)[6], [1.0, 1.0])
+    d = deck.parse_viscous(text.replace("2 1 5e-14 3 0 8.314", "2 1 5e-14 3 1e5 8.314"))
+    with pytest.raises(deck.DeckError, match="thermal"):
+        ratestate.rheology(d)
+
+
+def test_viscous_deck_round_trip():
+    fault = configs.thrust_fault(nx=3, nz=2, ellipse=None)
+    vol = configs.volume_grid(2, 2, 1, size=4000.0, top=20e3, origin=(0.0, 0.0))
+    text = deck.format_viscous("w", fault.lambda_, fault.mu, 1e9, fault, vol, observe=[1, 6],
+                               observe_vol=[4], points=[("A", 1.0, 2.0, 0.0)], events=3)
+    d = deck.parse_viscous(text)
+    assert d.n_rect == fault.n and d.n_vol == vol.n
+    np.testing.assert_array_equal(d.vol_x, vol.x)
+    np.testing.assert_array_equal(d.vol_thickness, vol.thickness)
+    np.testing.assert_array_equal(d.vol_ebg, vol.edot_bg)
+    np.testing.assert_allclose(d.vol_strike, vol.strike, rtol=1e-15)
+    from unicycle_amd import ratestate
+    for got, want in zip(ratestate.rheology(d), vol.rheology()):
+        np.testing.assert_allclose(got, want, rtol=1e-15)
+    for got, want in zip(d.friction_arrays(), fault.friction()):
+        np.testing.assert_allclose(got, want, rtol=1e-15)
+    assert d.observation_volumes == [4] and d.events == [0.0, 1.0, 2.0]
+    # the volume-free deck with no elements at all is refused
+    with pytest.raises(deck.DeckError, match="nothing to do"):
+        deck.parse_viscous(deck.format_viscous("w", 30e3, 30e3, 1e9))

@@ -119,3 +119,72 @@ def test_deck_run_writes_observation_points(tmp_path, engine_factory):
     for name in ("GPS1", "STATION_LO", "deep"):
         f = f"opts-{name}.dat"
         assert open(os.path.join(wdir, f)).read() == open(os.path.join(wdir2, f)).read()
+
+
+def test_viscous_deck_run_matches_oracle(tmp_path, engine_factory):
+    """viscouscycles deck (faults + Maxwell volumes, every element observed,
+    two observation points) through the driver against the oracle's run of the
+    same model: time.txt, the first rows of the patch and volume files, eMax
+    of the progress lines against the volume files' strain rates, and the
+    10-column opts rows against the oracle's O, Of, Ol times the slips and
+    strains the same run wrote."""
+    fault = configs.thrust_fault(nx=6, nz=3, ellipse=None)
+    vol = configs.volume_grid(3, 3, 2, size=4000.0, top=20e3, origin=(0.0, 0.0))
+    wdir = str(tmp_path / "visc")
+    pts = [("GPS1", 5e3, 2e3, 0.0), ("GPS2", 2e4, 8e3, 0.0)]
+    text = deck.format_viscous(wdir, fault.lambda_, fault.mu, 50 * configs.YEAR, fault, vol,
+                               observe=list(range(1, fault.n + 1)),
+                               observe_vol=list(range(1, vol.n + 1)), points=pts)
+    d = deck.parse_viscous(text)
+    steps = 100
+    log = io.StringIO()
+    it, _, y = ratestate.run(d, max_iterations=steps, chunk=steps, out=log)
+    assert it == steps
+    mesh = orc.Mesh(fault.lambda_, fault.mu, rect=fault.geometry(), vol=vol.geometry())
+    K = orc.buildG(mesh, nthreads=8)
+    sysm = orc.System(fault.friction(), K, rheo=vol.rheology(), ic_variant=1)
+    y0 = sysm.init_state()
+    yref, st, tl, dl, rc = sysm.integrate(y0, d.interval, max_steps=steps, log_cap=steps)
+    tt = _read(os.path.join(wdir, "time.txt"), [19, 19])
+    np.testing.assert_allclose(tt[1:, 0], np.concatenate([[0.0], tl[:-1]]), rtol=1e-5, atol=0)
+    np.testing.assert_allclose(tt[2:, 1], dl[:-1], rtol=1e-5)
+    dydt0 = sysm.odefun(y0)[0]
+    npch = fault.n
+    slip = np.zeros((steps, 2 * npch))
+    for e in range(npch):
+        p = _read(os.path.join(wdir, f"patch-{e + 1:08d}.dat"), [19] + [20] * 14)
+        np.testing.assert_allclose(p[0, 1:8], y0[7 * e:7 * e + 7], rtol=1e-12, atol=0)
+        slip[:, 2 * e:2 * e + 2] = p[:, 1:3]
+    strain = np.zeros((steps, 6 * vol.n))
+    erate = np.zeros((steps, vol.n))
+    for v in range(vol.n):
+        q = _read(os.path.join(wdir, f"volume-{v + 1:08d}.dat"), [19] + [20] * 24)
+        o = 7 * npch + 12 * v
+        np.testing.assert_allclose(q[0, 1:13], y0[o:o + 12], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(q[0, 13:25], dydt0[o:o + 12], rtol=1e-9,
+                                   atol=1e-12 * np.abs(dydt0[o:o + 12]).max())
+        np.testing.assert_array_equal(q[:, 0], tt[1:, 0])
+        strain[:, 6 * v:6 * v + 6] = q[:, 1:7]
+        r = q[:, 13:19]
+        erate[:, v] = np.sqrt((r[:, 0] ** 2 + 2 * r[:, 1] ** 2 + 2 * r[:, 2] ** 2 + r[:, 3] ** 2
+                               + 2 * r[:, 4] ** 2 + r[:, 5] ** 2) / 2)
+    # progress lines: iteration, time, dt, vMax, eMax (viscouscycles.f90:690-695)
+    lines = [ln for ln in log.getvalue().splitlines() if not ln.startswith("#")]
+    prog = [ln for ln in lines[1:-1]]
+    assert [int(ln[:9]) for ln in prog] == [50, 100]
+    for ln in prog:
+        i = int(ln[:9])
+        emax = float(ln[9 + 38 + 11:9 + 38 + 22])
+        assert emax == pytest.approx(erate[i - 1].max(), rel=1e-4)
+    # opts: time, d, df, dl (viscouscycles.f90:612-629)
+    O, Of, Ol = orc.buildO(mesh, [p[1:] for p in pts])
+    u = np.concatenate([slip, strain], axis=1)
+    for j, (name, *_x) in enumerate(pts):
+        q = _read(os.path.join(wdir, f"opts-{name}.dat"), [19] * 10)
+        assert q.shape == (steps, 10)
+        for c, A in enumerate((O, Of, Ol)):
+            want = u @ A[3 * j:3 * j + 3].T
+            np.testing.assert_allclose(q[:, 1 + 3 * c:4 + 3 * c], want, rtol=1e-6,
+                                       atol=1e-9 * np.abs(want).max() + 1e-300)
+    assert os.path.getsize(os.path.join(wdir, "volume.vtp")) > 0
+    assert np.abs(y - yref).max() <= 1e-5 * np.abs(yref).max()

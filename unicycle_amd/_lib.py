@@ -61,6 +61,7 @@ class StepLog(C.Structure):
         ("disp", C.POINTER(C.c_double)),
         ("disp_f", C.POINTER(C.c_double)),
         ("disp_l", C.POINTER(C.c_double)),
+        ("emax", C.POINTER(C.c_double)),
     ]
 
 

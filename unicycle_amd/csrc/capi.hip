@@ -766,7 +766,7 @@ static int ensure_work(ucy_ctx *c)
     b.t0 = p; p += d.m;
     b.v2 = p; p += d.n;
     if (c->kprec == UCY_PREC_F64) b.v0 = b.t0 = nullptr;
-    b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+    b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = b.emlog = nullptr;
     c->work_dim = d.dim;
     return 0;
 }
@@ -988,7 +988,7 @@ extern "C" int ucy_integrate(ucy_ctx *c, double *y, double interval, double dt0,
                              ucy_run_stats *stats)
 {
     ucy_step_log log = {t_log, dt_log, vmax_log, nullptr, nullptr, log_cap, 0, nullptr, nullptr,
-                        nullptr, nullptr, nullptr, nullptr};
+                        nullptr, nullptr, nullptr, nullptr, nullptr};
     return ucy_integrate_ex(c, y, 0.0, interval, dt0, max_steps, epsilon, max_step, &log, stats);
 }
 
@@ -1013,7 +1013,10 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
     double *d_log = nullptr;
     const int64_t cap = (t_log && dt_log && vmax_log) ? log_cap : 0;
     const bool ext = cap > 0 && err_log && rej_log;
-    if (cap > 0) HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * (ext ? 5 : 3) * cap));
+    double *em_log = log ? log->emax : nullptr;
+    const bool with_em = cap > 0 && em_log;
+    if (cap > 0)
+        HIPCHK(hipMalloc((void **)&d_log, sizeof(double) * (6 * cap)));
     if (c->graph && (b.tlog != d_log)) {
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -1023,6 +1026,7 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
     b.vlog = d_log ? d_log + 2 * cap : nullptr;
     b.elog = ext ? d_log + 3 * cap : nullptr;
     b.rlog = ext ? d_log + 4 * cap : nullptr;
+    b.emlog = with_em ? d_log + 5 * cap : nullptr;
     // observed entries (needs the step log for the row index)
     int64_t *d_oidx = nullptr;
     double *d_obs = nullptr;
@@ -1064,7 +1068,7 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
                 hipFree(d_oidx);
                 hipFree(d_obs);
             }
-            b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+            b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = b.emlog = nullptr;
             b.obs_idx = nullptr;
             b.n_obs = 0;
             b.obs_y = b.obs_dydt = nullptr;
@@ -1100,7 +1104,7 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
             hipFree(d_obs);
         }
         if (d_disp) hipFree(d_disp);
-        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = b.emlog = nullptr;
         b.obs_idx = nullptr;
         b.n_obs = 0;
         b.obs_y = b.obs_dydt = nullptr;
@@ -1136,6 +1140,9 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
             HIPCHK(hipMemcpyAsync(rej_log, b.rlog, sizeof(double) * h.log_count,
                                   hipMemcpyDeviceToHost, c->stream));
         }
+        if (with_em)
+            HIPCHK(hipMemcpyAsync(em_log, b.emlog, sizeof(double) * h.log_count,
+                                  hipMemcpyDeviceToHost, c->stream));
         if (n_obs > 0) {
             HIPCHK(hipMemcpyAsync(log->obs_y, b.obs_y, sizeof(double) * n_obs * h.log_count,
                                   hipMemcpyDeviceToHost, c->stream));
@@ -1156,7 +1163,7 @@ extern "C" int ucy_integrate_ex(ucy_ctx *c, double *y, double t0, double interva
         hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
         HIPCHK(hipFree(d_log));
-        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = nullptr;
+        b.tlog = b.dtlog = b.vlog = b.elog = b.rlog = b.emlog = nullptr;
     }
     if (d_oidx) {
         HIPCHK(hipFree(d_oidx));

@@ -431,6 +431,11 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
     } else {
         volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+        // eMax of the step-start odefun (viscouscycles.f90:986), printed by export
+        const double *r = b.dydt + so;
+        const double en = sqrt((r[0] * r[0] + 2.0 * (r[1] * r[1]) + 2.0 * (r[2] * r[2]) +
+                                r[3] * r[3] + 2.0 * (r[4] * r[4]) + r[5] * r[5]) / 2.0);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
     }
     if (b.v0) {
         const int nvk = e < d.np ? 2 : 6;
@@ -641,11 +646,13 @@ __global__ void controller_kernel(RkBuffers b, Control *ctl)
             b.elog[ctl->log_count] = errmax;
             b.rlog[ctl->log_count] = (double)ctl->step_rejects;
         }
+        if (b.emlog) b.emlog[ctl->log_count] = ctl->emax;
         ctl->log_count += 1;
     }
     ctl->step_rejects = 0;
     ctl->last_vmax = ctl->vmax;
     ctl->vmax = 0.0;
+    ctl->emax = 0.0;
     ctl->accepted = 1;
     ctl->obs_pending = 1;
     ctl->need_rhs = 1;

@@ -37,6 +37,8 @@ struct Control {
     unsigned long long errmax_bits;  // max |yerr/yscal| as ordered bits
     double vmax;        // max slip velocity of the current step-start odefun
     double last_vmax;   // same, for the last accepted step
+    double emax;        // max strain-rate norm of the volumes, same odefun
+                        // (viscouscycles.f90:986)
     int need_rhs;       // 1: next attempt starts a new step
     int accepted;       // 1: last attempt accepted, y <- ycand pending
     int done;
@@ -92,6 +94,7 @@ struct RkBuffers {
     double *v0, *t0;
     double *tlog, *dtlog, *vlog;
     double *elog, *rlog;   // errmax/epsilon of the accepted attempt, rejections (optional)
+    double *emlog;         // eMax of the step-start odefun (optional)
 };
 // the stored K of a context: binary64 always; packed split-bf16 pairs
 // (hi << 16 | lo, row-major, row stride n) with power-of-two row scales when

@@ -1,6 +1,9 @@
-"""The input deck of the reference's 3-D rate-and-state program.
+"""The input decks of the reference's 3-D programs.
 
-Reads the format that ratestate.f90:958-1739 (`init`) reads, section by
+parse() reads the deck of ratestate.f90:958-1739 (`init`); parse_viscous()
+the deck of viscouscycles.f90:1291-2428, which adds cuboid strain volumes
+(after the triangles: the volumes, their nonlinear Maxwell rows and thermal
+rows) and observed volumes (after the observed patches).  Both read section by
 section, with its checks: comment lines start with '#' (getdata.f), values
 are Fortran list-directed (blanks or commas, D or E exponents, quoted or bare
 names), angles are in degrees and converted with the reference's DEG2RAD.
@@ -23,7 +26,7 @@ Sections, in order:
     i patch                                            (no lines)
   number of observation points         np
     i name x1 x2 x3                                    (np lines)
-  number of events                     ne
+  number of perturbations              ne
     time                                               (ne - 1 lines: the first
                                                         event is at t = 0)
 The patch and friction sections of a source type are present only when its
@@ -141,9 +144,9 @@ class _Lines:
         return toks
 
 
-def parse(text: str) -> Deck:
-    """Parse an input deck; raises DeckError where the reference stops."""
-    L = _Lines(text)
+def _header(L: _Lines, cls):
+    """output directory, elastic moduli and interval (ratestate.f90:1073-1127,
+    viscouscycles.f90:1410-1465: the same checks)"""
     wdir = L.next("output directory").strip()
     t = L.values("elastic moduli", 2)
     lam, mu = _real(t[0]), _real(t[1])
@@ -157,7 +160,12 @@ def parse(text: str) -> Deck:
     interval = _real(L.values("time interval", 1)[0])
     if interval <= 0.0:
         raise DeckError("simulation time must be positive")
-    d = Deck(wdir=wdir, lambda_=lam, mu=mu, interval=interval)
+    return cls(wdir=wdir, lambda_=lam, mu=mu, interval=interval)
+
+
+def _patches(L: _Lines, d: Deck):
+    """rectangular and triangular patches with their friction
+    (ratestate.f90:1128-1393; viscouscycles.f90:1466-1729 reads the same)"""
     fr = {k: [] for k in ("Vpl", "rake", "mu0", "sig", "a", "b", "L", "Vo", "damping")}
 
     def friction_rows(n, kind):
@@ -172,7 +180,6 @@ def parse(text: str) -> Deck:
             for key, tok in zip(("mu0", "sig", "a", "b", "L", "Vo", "damping"), t[1:8]):
                 fr[key].append(_real(tok))
 
-    # rectangular patches (ratestate.f90:1130-1237)
     ns = _int(L.values("number of rectangular patches", 1)[0])
     if ns > 0:
         rows = []
@@ -191,7 +198,6 @@ def parse(text: str) -> Deck:
         fr["Vpl"] += list(r[:, 0])
         fr["rake"] += list(r[:, 8] * DEG2RAD)
         friction_rows(ns, "rectangular")
-    # triangular patches (ratestate.f90:1238-1380)
     nt = _int(L.values("number of triangular patches", 1)[0])
     if nt > 0:
         i1, i2, i3, vpl, rake = [], [], [], [], []
@@ -221,14 +227,24 @@ def parse(text: str) -> Deck:
         fr["Vpl"] += vpl
         fr["rake"] += rake
         friction_rows(nt, "triangular")
-    # observation patches (ratestate.f90:1381-1400)
-    no = _int(L.values("number of observation patches", 1)[0])
-    for k in range(1, no + 1):
-        t = L.values(f"observation patch {k}", 2)
+    d.friction = fr
+
+
+def _indices(L: _Lines, what: str):
+    """a count, then (k, index) rows"""
+    n = _int(L.values(f"number of {what}s", 1)[0])
+    out = []
+    for k in range(1, n + 1):
+        t = L.values(f"{what} {k}", 2)
         if _int(t[0]) != k:
-            raise DeckError("observation patches: unexpected index")
-        d.observation_states.append(_int(t[1]))
-    # observation points (ratestate.f90:1420-1455)
+            raise DeckError(f"{what}s: unexpected index")
+        out.append(_int(t[1]))
+    return out
+
+
+def _points_and_events(L: _Lines, d: Deck):
+    """observation points (ratestate.f90:1421-1456, viscouscycles.f90:1986-2021)
+    and perturbations (ratestate.f90:1457-1490, viscouscycles.f90:2022-2055)"""
     npnt = _int(L.values("number of observation points", 1)[0])
     for k in range(1, npnt + 1):
         t = L.values(f"observation point {k}", 5)
@@ -237,8 +253,8 @@ def parse(text: str) -> Deck:
         # the name is CHARACTER(LEN=10) (types.f90:88): a list-directed read
         # keeps its leftmost 10 characters
         d.observation_points.append((t[1][:10], _real(t[2]), _real(t[3]), _real(t[4])))
-    # events (ratestate.f90:1457-1484): the first is at t = 0 and has no line
-    ne = _int(L.values("number of events", 1)[0])
+    # the first event is at t = 0 and has no line
+    ne = _int(L.values("number of perturbations", 1)[0])
     for k in range(1, ne + 1):
         if k == 1:
             d.events.append(0.0)
@@ -247,13 +263,122 @@ def parse(text: str) -> Deck:
         if tk <= d.events[-1]:
             raise DeckError("timing of perturbations must increase")
         d.events.append(tk)
+
+
+def parse(text: str) -> Deck:
+    """Parse a ratestate input deck; raises DeckError where the reference stops."""
+    L = _Lines(text)
+    d = _header(L, Deck)
+    _patches(L, d)
+    # observation patches (ratestate.f90:1394-1420)
+    d.observation_states = _indices(L, "observation patch")
+    _points_and_events(L, d)
     if (d.n_rect == 0 and d.n_tri == 0) or d.interval <= 0.0:
         raise DeckError("nothing to do")
-    d.friction = fr
     nel = d.n_rect + d.n_tri
     for j in d.observation_states:
         if not 1 <= j <= nel:
             raise DeckError(f"observation patch {j} is not an element")
+    return d
+
+
+@dataclass
+class ViscousDeck(Deck):
+    """The input deck of viscouscycles.f90 (1291-2428): a ratestate deck with
+    cuboid strain volumes, their nonlinear Maxwell and thermal properties and
+    observed volumes between the patches and the observation points."""
+    vol_x: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    vol_length: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_width: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_thickness: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_strike: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_dip: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_ebg: np.ndarray = field(default_factory=lambda: np.zeros((0, 6)))  # e11..e33 loading
+    n_maxwell: int = 0
+    maxwell: dict = field(default_factory=dict)   # Gm, gammadot0, npower, Q, R (per volume)
+    thermal: dict = field(default_factory=dict)   # rhoc, To (per volume), or empty
+    observation_volumes: list = field(default_factory=list)   # 1-based volume indices
+
+    @property
+    def n_vol(self) -> int:
+        return self.vol_length.size
+
+
+def parse_viscous(text: str) -> ViscousDeck:
+    """Parse a viscouscycles input deck; raises DeckError where the reference
+    stops (viscouscycles.f90:1410-2065)."""
+    L = _Lines(text)
+    d = _header(L, ViscousDeck)
+    _patches(L, d)
+    # cuboid strain volumes (viscouscycles.f90:1730-1820): loading strain rate,
+    # upper corner, length, width, thickness, strike, dip (must be 90)
+    nv = _int(L.values("number of cuboid strain volumes", 1)[0])
+    if nv > 0:
+        rows = []
+        for k in range(1, nv + 1):
+            t = L.values(f"strain volume {k}", 15)
+            v = [_real(x) for x in t[1:15]]
+            if 90.0 != v[13]:
+                raise DeckError("strain volume must be vertical (dip=90)")
+            if 0 > v[8]:
+                raise DeckError("strain volume depth must be positive")
+            if _int(t[0]) != k:
+                raise DeckError("invalid strain volume definition: unexpected index")
+            if max(v[9], v[10], v[11]) <= 0.0:
+                raise DeckError("strain volume dimension must be positive")
+            rows.append(v)
+        r = np.array(rows)
+        d.vol_ebg = r[:, 0:6].copy()
+        d.vol_x = r[:, 6:9].copy()
+        d.vol_length, d.vol_width, d.vol_thickness = r[:, 9].copy(), r[:, 10].copy(), r[:, 11].copy()
+        d.vol_strike, d.vol_dip = r[:, 12] * DEG2RAD, r[:, 13] * DEG2RAD
+        # nonlinear Maxwell properties (:1821-1881), for none or all volumes
+        d.n_maxwell = _int(L.values("number of nonlinear Maxwell strain volumes", 1)[0])
+        if d.n_maxwell != 0:
+            if d.n_maxwell != nv:
+                raise DeckError("nonlinear Maxwell properties are for none or all strain volumes")
+            keys = ("Gm", "gammadot0", "npower", "Q", "R")
+            d.maxwell = {k: [] for k in keys}
+            for k in range(1, nv + 1):
+                t = L.values(f"Maxwell row {k}", 6)
+                v = [_real(x) for x in t[1:6]]
+                if 0 >= v[0]:
+                    raise DeckError("sig0 must be positive")
+                if 0 >= v[4]:
+                    raise DeckError("R must be positive")
+                if _int(t[0]) != k:
+                    raise DeckError("invalid property definition for strain volume: "
+                                    "unexpected index")
+                for key, x in zip(keys, v):
+                    d.maxwell[key].append(x)
+        # thermal properties (:1882-1930), for none or all volumes
+        nth = _int(L.values("number of thermal strain volumes", 1)[0])
+        if nth != 0:
+            if nth != nv:
+                raise DeckError("thermal properties are for none or all strain volumes")
+            d.thermal = {"rhoc": [], "To": []}
+            for k in range(1, nv + 1):
+                t = L.values(f"thermal row {k}", 3)
+                rhoc, To = _real(t[1]), _real(t[2])
+                if 0 >= To:
+                    raise DeckError("To must be positive")
+                if _int(t[0]) != k:
+                    raise DeckError("invalid property definition for strain volume: "
+                                    "unexpected index")
+                d.thermal["rhoc"].append(rhoc)
+                d.thermal["To"].append(To)
+    d.observation_states = _indices(L, "observation patch")
+    d.observation_volumes = _indices(L, "observation volume")
+    _points_and_events(L, d)
+    if (d.n_rect == 0 and d.n_tri == 0 and d.n_vol == 0) or d.interval <= 0.0:
+        raise DeckError("nothing to do")
+    nel = d.n_rect + d.n_tri
+    for j in d.observation_states:
+        if not 1 <= j <= nel:
+            raise DeckError(f"observation patch {j} is not an element")
+    for j in d.observation_volumes:
+        if not 1 <= j <= d.n_vol:
+            raise DeckError(f"observation volume {j} is not a volume")
     return d
 
 
@@ -284,8 +409,59 @@ def format_rectangles(wdir, lambda_, mu, interval, x, length, width, strike, dip
     out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
     out += ["# number of observation points", str(len(points))]
     out += [f"{k + 1} {n} {x1!r} {x2!r} {x3!r}" for k, (n, x1, x2, x3) in enumerate(points)]
-    out += ["# number of events", str(events)]
+    out += ["# number of perturbations", str(events)]
     out += [f"{float(k)!r}" for k in range(1, events)]
+    return "\n".join(out) + "\n"
+
+
+def format_viscous(wdir, lambda_, mu, interval, fault=None, vol=None, maxwell=True,
+                   thermal=True, observe=(), observe_vol=(), points=(), events=1) -> str:
+    """An input deck of viscouscycles.f90 in the layout parse_viscous() reads:
+    rectangles of a FaultModel (configs) and the cuboids of a VolumeModel with
+    its Maxwell (and thermal) rows; angles written in degrees."""
+    R2D = 1.0 / DEG2RAD
+    f = lambda v: repr(float(v))  # noqa: E731
+    out = ["# output directory", str(wdir), "# elastic moduli", f"{f(lambda_)} {f(mu)}",
+           "# time interval", f(interval), "# number of rectangular patches"]
+    if fault is not None and fault.n:
+        Vpl, rake, mu0, sig, a, b, L, Vo, damping = fault.friction()
+        out.append(str(fault.n))
+        for k in range(fault.n):
+            out.append(" ".join([str(k + 1)] + [f(v) for v in (
+                Vpl[k], *fault.x[k], fault.length[k], fault.width[k], fault.strike[k] * R2D,
+                fault.dip[k] * R2D, rake[k] * R2D)]))
+        out += ["# number of frictional rectangular patches", str(fault.n)]
+        for k in range(fault.n):
+            out.append(" ".join([str(k + 1)] + [f(v) for v in (
+                mu0[k], sig[k], a[k], b[k], L[k], Vo[k], damping[k])]))
+    else:
+        out.append("0")
+    out += ["# number of triangular patches", "0", "# number of cuboid strain volumes"]
+    if vol is not None and vol.n:
+        out.append(str(vol.n))
+        out.append("# n e11 e12 e13 e22 e23 e33 x1 x2 x3 length width thickness strike dip")
+        for k in range(vol.n):
+            out.append(" ".join([str(k + 1)] + [f(v) for v in (
+                *vol.edot_bg[k], *vol.x[k], vol.length[k], vol.width[k], vol.thickness[k],
+                vol.strike[k] * R2D, 90.0)]))
+        out += ["# number of nonlinear Maxwell strain volumes", str(vol.n if maxwell else 0)]
+        if maxwell:
+            for k in range(vol.n):
+                out.append(" ".join([str(k + 1)] + [f(v) for v in (
+                    vol.Gm[k], vol.gammadot0[k], vol.npower[k], vol.Q[k], vol.R[k])]))
+        out += ["# number of thermal strain volumes", str(vol.n if thermal else 0)]
+        if thermal:
+            out += [f"{k + 1} 1.0 {f(vol.To[k])}" for k in range(vol.n)]
+    else:
+        out.append("0")
+    out += ["# number of observation patches", str(len(observe))]
+    out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
+    out += ["# number of observation volumes", str(len(observe_vol))]
+    out += [f"{k + 1} {j}" for k, j in enumerate(observe_vol)]
+    out += ["# number of observation points", str(len(points))]
+    out += [f"{k + 1} {n} {f(x1)} {f(x2)} {f(x3)}" for k, (n, x1, x2, x3) in enumerate(points)]
+    out += ["# number of perturbations", str(events)]
+    out += [f(k) for k in range(1, events)]
     return "\n".join(out) + "\n"
 
 

@@ -70,6 +70,7 @@ class IntegrateResult:
     disp: np.ndarray | None = None          # (steps, 3 npts) observation-point displacement
     disp_f: np.ndarray | None = None        # its fault part (Of u) and ...
     disp_l: np.ndarray | None = None        # ... volume part (Ol u), at each step start
+    emax_log: np.ndarray | None = None      # step-start eMax (largest volume strain-rate norm)
 
 
 class Engine:
@@ -277,10 +278,11 @@ class Engine:
         t0 continues a run: the interval and the logged times are absolute."""
         y = _f64(y).copy()
         st = _lib.RunStats()
-        logs = [np.empty(max(log_cap, 1)) for _ in range(5)]
+        logs = [np.empty(max(log_cap, 1)) for _ in range(6)]
         lg = _lib.StepLog()
         if log_cap:
             lg.t, lg.dt, lg.vmax = _p(logs[0]), _p(logs[1]), _p(logs[2])
+            lg.emax = _p(logs[5])
             if history:
                 lg.errmax, lg.rejects = _p(logs[3]), _p(logs[4])
         lg.cap = int(log_cap)
@@ -308,6 +310,7 @@ class Engine:
             res.obs_y, res.obs_dydt = obs[1][:nlog].copy(), obs[2][:nlog].copy()
         if dsp is not None:
             res.disp, res.disp_f, res.disp_l = (a[:nlog].copy() for a in dsp)
+        res.emax_log = cut(logs[5])
         return res
 
     def time_rhs_kernel(self, iters: int = 50) -> float:

@@ -18,9 +18,15 @@ of iterations.  It writes what the reference writes each iteration
                                 and the displacement (north, east, down) at
                                 the iteration's start, d = O u of exportPoints
                                 (ratestate.f90:414-486), 4 x ES19.12E2;
+  <wdir>/patch-rectangular.vtp the rectangles' geometry and friction
+                                (exportGeometry, ratestate.f90:374-399);
   stdout                        every 50 iterations: iteration, time, step, vMax.
 
-The VTP geometry files are not written.
+A viscouscycles deck (unicycle_amd.viscouscycles, viscouscycles.f90:413-434)
+runs the same loop with strain volumes and adds: volume-%08d.dat per observed
+volume (12 state entries and their derivatives), the fault and flow parts of
+the displacement in opts-<name>.dat (10 columns, viscouscycles.f90:612-629),
+eMax in the progress line and <wdir>/volume.vtp.
 """
 from __future__ import annotations
 
@@ -31,46 +37,107 @@ import sys
 import numpy as np
 
 from . import deck as deckmod
+from . import vtp
 from .engine import PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, Engine
 
-DPATCH = 7   # state entries per patch (STATE_VECTOR_DGF of macros.f90)
+DPATCH = 7    # state entries per patch (STATE_VECTOR_DGF_PATCH of macros.f90)
+DVOLUME = 12  # state entries per strain volume (STATE_VECTOR_DGF_VOLUME)
 
 
 def setup(d: deckmod.Deck, device: int = 0, precision: int = PREC_F64) -> Engine:
+    """Mesh, friction (and for a viscouscycles deck the volumes and their
+    rheology), buildG and buildO on the GPU."""
     eng = Engine(device)
     eng.set_elastic(d.lambda_, d.mu)
     if d.n_rect:
         eng.set_rectangles(d.rect_x, d.rect_length, d.rect_width, d.rect_strike, d.rect_dip)
     if d.n_tri:
         eng.set_triangles(d.tri_i1, d.tri_i2, d.tri_i3, d.vertices)
-    eng.set_friction(*d.friction_arrays())
+    if isinstance(d, deckmod.ViscousDeck) and d.n_vol:
+        eng.set_volumes(d.vol_x, d.vol_length, d.vol_thickness, d.vol_width, d.vol_strike)
+        eng.set_rheology(*rheology(d))
+    if d.n_rect + d.n_tri:
+        eng.set_friction(*d.friction_arrays())
     eng.buildG(precision)
     if d.observation_points:   # buildO (greens.f90:302-462, ratestate.f90:208)
         eng.buildO([p[1:] for p in d.observation_points])
     return eng
 
 
+def rheology(d: deckmod.ViscousDeck):
+    """Engine.set_rheology arguments of a viscouscycles deck: the Maxwell rows
+    (gammadot0, n, Gm, Q, R), the volumes' loading strain rates and the thermal
+    To.  Without Maxwell rows the volumes do not flow (viscouscycles.f90:963:
+    the strain rate stays zero); without thermal rows the Arrhenius factor
+    needs Q = 0 (the reference would read an unset temperature)."""
+    nv = d.n_vol
+    if d.n_maxwell:
+        g0, npow, Gm, Q, R = (np.asarray(d.maxwell[k], dtype=np.float64)
+                              for k in ("gammadot0", "npower", "Gm", "Q", "R"))
+    else:
+        g0, npow, Gm, Q, R = np.zeros(nv), np.ones(nv), np.ones(nv), np.zeros(nv), np.ones(nv)
+    if d.thermal:
+        To = np.asarray(d.thermal["To"], dtype=np.float64)
+    else:
+        if np.any(Q != 0.0):
+            raise deckmod.DeckError("thermal properties are needed when Q is not zero")
+        To = np.ones(nv)
+    return g0, npow, d.vol_ebg, Gm, Q, R, To
+
+
+def rect_friction(d: deckmod.Deck):
+    """(Vpl, rake, mu0, sig, a, b, L) of the rectangles, deck order."""
+    fr = d.friction_arrays()
+    return tuple(np.asarray(a)[:d.n_rect] for a in fr[:7])
+
+
 def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348623158e308,
         max_iterations: int = 1000000, device: int = 0, precision: int = PREC_F64,
         chunk: int = 20000, out=sys.stdout, write_files: bool = True):
-    """Run the deck; returns (iterations, final time, final state)."""
+    """Run the deck (ratestate, or viscouscycles for a ViscousDeck); returns
+    (iterations, final time, final state)."""
     fmt = deckmod.es_field
+    viscous = isinstance(d, deckmod.ViscousDeck)
     eng = setup(d, device, precision)
-    y = eng.initStateVector()
+    # initStateVector: ratestate.f90:548-610, viscouscycles.f90:710-820
+    y = eng.initStateVector(1 if viscous else 0)
+    npatch = d.n_rect + d.n_tri
+    nvol = d.n_vol if viscous else 0
+    if nvol and not d.n_maxwell:
+        # no Maxwell rows: the reference's initial stress reads unset
+        # properties; here the volumes start unstressed
+        for v in range(nvol):
+            y[DPATCH * npatch + DVOLUME * v + 6:DPATCH * npatch + DVOLUME * (v + 1)] = 0.0
     time, dt_next, dt_done = 0.0, 1.0e-3, 0.0     # ratestate.f90:166-167
-    obs = [j - 1 for j in d.observation_states]   # 0-based patches
-    idx = np.array([DPATCH * e + k for e in obs for k in range(DPATCH)], dtype=np.int64)
+    obs_vol = d.observation_volumes if viscous else []
+    idx = [DPATCH * (j - 1) + k for j in d.observation_states for k in range(DPATCH)]
+    idx += [DPATCH * npatch + DVOLUME * (j - 1) + k for j in obs_vol for k in range(DVOLUME)]
+    idx = np.array(idx, dtype=np.int64)
     files, pfiles = [], []
     if write_files:
         os.makedirs(d.wdir, exist_ok=True)
+        # exportGeometry (ratestate.f90:374-399, viscouscycles.f90:473-515)
+        with open(os.path.join(d.wdir, "patch-rectangular.vtp"), "w") as f:
+            f.write(vtp.rectangles_vtp(d.rect_x, d.rect_length, d.rect_width, d.rect_strike,
+                                       d.rect_dip, rect_friction(d)))
+        if viscous:
+            with open(os.path.join(d.wdir, "volume.vtp"), "w") as f:
+                f.write(vtp.volumes_vtp(d.vol_x, d.vol_length, d.vol_thickness, d.vol_width,
+                                        d.vol_strike, d.vol_dip))
         ftime = open(os.path.join(d.wdir, "time.txt"), "w")
         ftime.write("#               time                 dt\n")
         ftime.write(fmt(0.0, 19, 12, 2) + fmt(dt_next, 19, 12, 2) + "\n")
-        files = [open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w") for j in d.observation_states]
+        # observed elements, patches then volumes (ratestate.f90:273-288,
+        # viscouscycles.f90:370-392)
+        files = [(open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w"), DPATCH)
+                 for j in d.observation_states]
+        files += [(open(os.path.join(d.wdir, f"volume-{j:08d}.dat"), "w"), DVOLUME)
+                  for j in obs_vol]
         # ratestate.f90:291-304
         pfiles = [open(os.path.join(d.wdir, f"opts-{p[0].strip()}.dat"), "w")
                   for p in d.observation_points]
-    out.write("#       n               time                 dt       vMax\n")
+    out.write("#       n               time                 dt       vMax"
+              + ("       eMax" if viscous else "") + "\n")
     out.write(f"{0:09d}" + fmt(time, 19, 12, 2) + fmt(dt_next, 19, 12, 2) + "\n")
     it = 0
     try:
@@ -85,19 +152,26 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
                 prev = res.dt_log[k - 1] if k else dt_done
                 if write_files:
                     ftime.write(fmt(t_start, 19, 12, 2) + fmt(prev, 19, 12, 2) + "\n")
-                    for j, f in enumerate(files):
-                        ys = res.obs_y[k, DPATCH * j:DPATCH * (j + 1)]
-                        ds = res.obs_dydt[k, DPATCH * j:DPATCH * (j + 1)]
+                    o = 0
+                    for f, dgf in files:
+                        ys = res.obs_y[k, o:o + dgf]
+                        ds = res.obs_dydt[k, o:o + dgf]
+                        o += dgf
                         f.write(fmt(t_start, 19, 12, 2)
                                 + "".join(fmt(v, 20, 12, 3) for v in ys)
                                 + "".join(fmt(v, 20, 12, 3) for v in ds) + "\n")
                     for j, f in enumerate(pfiles):
+                        cols = [res.disp[k, 3 * j:3 * j + 3]]
+                        if viscous:
+                            cols += [res.disp_f[k, 3 * j:3 * j + 3], res.disp_l[k, 3 * j:3 * j + 3]]
                         f.write(fmt(t_start, 19, 12, 2)
-                                + "".join(fmt(v, 19, 12, 2) for v in res.disp[k, 3 * j:3 * j + 3])
-                                + "\n")
+                                + "".join(fmt(v, 19, 12, 2) for c in cols for v in c) + "\n")
                 if i % 50 == 0:
-                    out.write(f"{i:09d}" + fmt(t_start, 19, 12, 2) + fmt(prev, 19, 12, 2)
-                              + fmt(res.vmax_log[k], 11, 4, 2) + "\n")
+                    line = (f"{i:09d}" + fmt(t_start, 19, 12, 2) + fmt(prev, 19, 12, 2)
+                            + fmt(res.vmax_log[k], 11, 4, 2))
+                    if viscous:
+                        line += fmt(res.emax_log[k], 11, 4, 2)
+                    out.write(line + "\n")
             if res.steps:
                 dt_done = res.dt_log[res.steps - 1]
             it += res.steps
@@ -108,7 +182,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     finally:
         if write_files:
             ftime.close()
-            for f in files + pfiles:
+            for f in [f for f, _ in files] + pfiles:
                 f.close()
         eng.close()
     # the reference's loop index after EXIT (or past the last iteration)
@@ -116,10 +190,12 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     return it, time, y
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser(prog="unicycle_amd.ratestate",
+def main(argv=None, viscous: bool = False) -> int:
+    name = "viscouscycles" if viscous else "ratestate"
+    media = "viscoelastic media" if viscous else "media"
+    ap = argparse.ArgumentParser(prog=f"unicycle_amd.{name}",
                                  description="quasi-dynamic earthquake simulation in three-"
-                                             "dimensional media (GPU)")
+                                             f"dimensional {media} (GPU)")
     ap.add_argument("-e", "--epsilon", type=float, default=1.0e-6)
     ap.add_argument("-m", "--maximum-step", type=float, default=1.7976931348623158e308)
     ap.add_argument("-i", "--maximum-iterations", type=int, default=1000000)
@@ -130,21 +206,25 @@ def main(argv=None) -> int:
     ap.add_argument("deck", nargs="?", help="input deck (standard input if omitted)")
     a = ap.parse_args(argv)
     if a.version:
-        print("unicycle_amd ratestate (gfx950)")
+        print(f"unicycle_amd {name} (gfx950)")
         return 0
     text = open(a.deck).read() if a.deck else sys.stdin.read()
     try:
-        d = deckmod.parse(text)
+        d = deckmod.parse_viscous(text) if viscous else deckmod.parse(text)
+        if viscous and d.n_vol:
+            rheology(d)
     except deckmod.DeckError as exc:
         print(f"input error: {exc}", file=sys.stderr)
         return 1
-    print("# RATESTATE")
-    print("# quasi-dynamic earthquake simulation in three-dimensional media")
+    print(f"# {name.upper()}")
+    print(f"# quasi-dynamic earthquake simulation in three-dimensional {media}")
     print(f"# numerical accuracy: {deckmod.es_field(a.epsilon, 11, 4, 2)}")
     print(f"# maximum iterations: {a.maximum_iterations:11d}")
     print(f"# output directory\n  {d.wdir}")
     print(f"# {d.n_rect} rectangular and {d.n_tri} triangular patches, "
           f"{len(d.observation_states)} observed")
+    if viscous:
+        print(f"# {d.n_vol} strain volumes, {len(d.observation_volumes)} observed")
     print(f"# {len(d.observation_points)} observation points")
     if a.dry_run:
         print("dry run: abort calculation")
