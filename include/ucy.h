@@ -141,6 +141,23 @@ int ucy_set_triangles(ucy_ctx *ctx, int ns, const int *i1, const int *i2, const 
  * width, strike (radians). */
 int ucy_set_volumes(ucy_ctx *ctx, int ns, const double *x, const double *length,
                     const double *thickness, const double *width, const double *strike);
+/* The 2-D antiplane model of fortran/2d/antiplane (ratestate.f90 there):
+ * ns vertical segments (upper corner x(3,ns), north/east/down, x(2) and x(3)
+ * used; width; dip in radians, which the Green's function ignores as the
+ * reference does, antiplane.f90:126-127) in a half space of shear modulus mu.
+ * Replaces any 3-D elements.  The state has 4 entries per patch (slip,
+ * traction, log10 theta, log10 V); K is ns x ns (traction row of each
+ * receiver, slip column of each source); ucy_set_friction takes the same
+ * arrays (rake unused), ucy_build_kernel builds K (greens.f90:93-248), and
+ * ucy_init_state (ratestate.f90:568-605), ucy_odefun (:619-724), ucy_rkck,
+ * ucy_rkqs and ucy_integrate(_ex) run this model. */
+int ucy_set_antiplane(ucy_ctx *ctx, double mu, int ns, const double *x, const double *width,
+                      const double *dip);
+/* computeTractionKernelsAntiplane (antiplane.f90:352-379) */
+int ucy_antiplane_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                   const double dv[3], const double nv[3], int ns,
+                                   const double *y, const double *W, const double *dip,
+                                   double G, double *ts);
 /* Friction per patch in element order (rectangles then triangles), as the
  * deck's PATCH_ELEMENT_STRUCT (types.f90:35-40); rake in radians. */
 int ucy_set_friction(ucy_ctx *ctx, int np, const double *Vpl, const double *rake,

@@ -4,7 +4,8 @@
 !
 ! TEST INFRASTRUCTURE ONLY.  Compiled together with the reference sources
 ! (jdpmoore/unicycle fortran/3d/src: dc3d.f, okada92.f90, nikkhoo15.f90,
-! StrainInHalfSpace.c, strainvolume.f90), read where they lie, into
+! StrainInHalfSpace.c, DispInHalfSpace.c, strainvolume.f90; and
+! fortran/2d/antiplane/src/antiplane.f90), read where they lie, into
 ! oracle/_ref/libucyref.so.  Nothing here is product code.
 !
 ! Each wrapper forwards its arguments unchanged to the reference routine
@@ -176,6 +177,33 @@ CONTAINS
     REAL(C_DOUBLE), INTENT(OUT) :: u1,u2,u3
     CALL computeDisplacementVerticalStrainVolume(x1,x2,x3,q1,q2,q3,L,T,W,theta, &
          e11,e12,e13,e22,e23,e33,G,nu,u1,u2,u3)
+  END SUBROUTINE
+
+  ! fortran/2d/antiplane/src/antiplane.f90:131 computeStressAntiplane
+  SUBROUTINE ref_stress_antiplane(x2,x3,q2,q3,W,dip,G,s12,s13) BIND(C,NAME="ref_stress_antiplane")
+    USE antiplane
+    REAL(C_DOUBLE), VALUE :: x2,x3,q2,q3,W,dip,G
+    REAL(C_DOUBLE), INTENT(OUT) :: s12,s13
+    CALL computeStressAntiplane(x2,x3,q2,q3,W,dip,G,s12,s13)
+  END SUBROUTINE
+
+  ! antiplane.f90:352 computeTractionKernelsAntiplane
+  SUBROUTINE ref_traction_antiplane(x,sv,dv,nv,ns,y,W,dip,G,ts) BIND(C,NAME="ref_traction_antiplane")
+    USE antiplane
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: G
+    REAL(C_DOUBLE), INTENT(OUT) :: ts(ns)
+    CALL computeTractionKernelsAntiplane(x,sv,dv,nv,ns,y,W,dip,G,ts)
+  END SUBROUTINE
+
+  ! antiplane.f90:81 computeReferenceSystemAntiplane
+  SUBROUTINE ref_refsys_antiplane(ns,x,W,dip,sv,dv,nv,xc) BIND(C,NAME="ref_refsys_antiplane")
+    USE antiplane
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), INTENT(OUT) :: sv(3,ns),dv(3,ns),nv(3,ns),xc(3,ns)
+    CALL computeReferenceSystemAntiplane(ns,x,W,dip,sv,dv,nv,xc)
   END SUBROUTINE
 
 END MODULE ucy_ref_harness

@@ -153,6 +153,31 @@ long orc_integrate(const orc_system *sys, double *y, double interval, double dt0
                    double *t_log, double *dt_log, double *vmax_log, long log_cap,
                    orc_run_stats *st);
 
+/* ---- 2-D antiplane (fortran/2d/antiplane/src) ------------------------- */
+void orc_refsys_antiplane(int ns, const double *x, const double *W, const double *dip,
+                          double *sv, double *dv, double *nv, double *xc);
+void orc_stress_antiplane(double x2, double x3, double q2, double q3, double W, double dip,
+                          double G, double *s12, double *s13);
+double orc_displacement_antiplane(double x2, double x3, double q2, double q3, double W,
+                                  double dip, double s);
+void orc_traction_kernels_antiplane(const double x[3], const double sv[3], const double dv[3],
+                                    const double nv[3], int ns, const double *y,
+                                    const double *W, const double *dip, double G, double *ts);
+/* K (ns x ns) row-major: row = receiver traction, column = source slip */
+void orc_buildG_antiplane(int ns, const double *x, const double *W, const double *dip, double mu,
+                          double *K, int nthreads);
+typedef struct {
+    int n;                 /* patches; state 4 per patch: slip, traction, log10 theta, log10 V */
+    const double *Vpl, *mu0, *sig, *a, *b, *L, *Vo, *damping;
+    const double *K;       /* n x n row-major */
+} orc_ap_system;
+void orc_ap_init_state(const orc_ap_system *s, double *y);
+double orc_ap_odefun(const orc_ap_system *s, const double *y, double *dydt, double *vwork,
+                     double *twork);
+long orc_ap_integrate(const orc_ap_system *s, double *y, double interval, double dt0,
+                      long max_steps, double epsilon, double max_step, double *t_log,
+                      double *dt_log, double *vmax_log, long log_cap, orc_run_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

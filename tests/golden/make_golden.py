@@ -16,6 +16,8 @@ The fixtures are inputs and the reference's outputs only:
                         VerticalStrainVolume (strainvolume.f90:4671-4799), incl.
                         a receiver inside a volume and one at the surface, and
                         computeDisplacementVerticalStrainVolume (215-268)
+  antiplane.npz         computeStressAntiplane / computeTractionKernels-
+                        Antiplane (fortran/2d/antiplane/src/antiplane.f90)
   displacement.npz      computeDisplacementOkada92 (okada92.f90:230-298) and
                         computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367,
                         DispInHalfSpace.c:42-152) at random points, at the
@@ -277,10 +279,42 @@ def displacement(rng, nrect=400, ntri=400):
                         P1=P1, P2=P2, P3=P3, tri_points=pts, tri_u=tu)
 
 
+def antiplane(rng, n=400, ns=120):
+    """computeStressAntiplane at random points (some at the surface) and
+    computeTractionKernelsAntiplane of dipping receivers on random segments
+    (fortran/2d/antiplane/src/antiplane.f90:131-156, 352-379)."""
+    inp = np.stack([rng.uniform(-2e4, 2e4, n), rng.uniform(0, 2e4, n), rng.uniform(-2e4, 2e4, n),
+                    rng.uniform(0, 1e4, n), rng.uniform(100, 5e3, n), rng.uniform(0.1, 1.5, n)],
+                   1)
+    inp[::4, 1] = 0.0
+    s = np.zeros((n, 2))
+    for i, r in enumerate(inp):
+        a, b = D(), D()
+        REF.ref_stress_antiplane(*[D(v) for v in r], D(30e3), C.byref(a), C.byref(b))
+        s[i] = a.value, b.value
+    y = np.stack([np.zeros(ns), rng.uniform(-2e4, 2e4, ns), rng.uniform(0, 1e4, ns)], 1)
+    W, dip = rng.uniform(100, 5e3, ns), rng.uniform(0.1, 1.5, ns)
+    rcv, ts = [], []
+    for _ in range(4):
+        d = rng.uniform(0.1, 1.5)
+        x = np.array([0.0, rng.uniform(-1e4, 1e4), rng.uniform(100, 1e4)])
+        sv = np.array([1.0, 0.0, 0.0])
+        dv = np.array([0.0, -np.cos(d), -np.sin(d)])
+        nv = np.array([-0.0, np.sin(d), -np.cos(d)])
+        out = np.zeros(ns)
+        REF.ref_traction_antiplane(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)),
+                                   P(W), P(dip), D(30e3), P(out))
+        rcv.append(np.concatenate([x, sv, dv, nv]))
+        ts.append(out)
+    np.savez_compressed(os.path.join(HERE, "antiplane.npz"), stress_inputs=inp, stress=s,
+                        y=y, W=W, dip=dip, receivers=np.array(rcv), traction=np.array(ts))
+
+
 FIXTURES = {
     "okada_dc3d": (20260101, [dc3d_cases, okada_kernels, nikkhoo_strain, nikkhoo_kernels,
                               strainvolume_kernels]),
     "displacement": (20261202, [displacement]),
+    "antiplane": (20261203, [antiplane]),
 }
 
 if __name__ == "__main__":

@@ -143,6 +143,59 @@ def slip_vector(y, np_, nvol):
     return np.concatenate(u)
 
 
+class OrcAPSystem(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in (
+        "Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")]
+
+
+def buildG_antiplane(x, W, dip, mu, nthreads=1):
+    x, W, dip = f64(x), f64(W), f64(dip)
+    ns = W.size
+    K = np.empty((ns, ns))
+    oracle().orc_buildG_antiplane(ns, p(x), p(W), p(dip), C.c_double(mu), p(K), nthreads)
+    return K
+
+
+class APSystem:
+    """2-D antiplane rate-and-state (fortran/2d/antiplane/src/ratestate.f90)."""
+
+    def __init__(self, fric, K):
+        self.keep = [f64(a) for a in fric] + [f64(K)]
+        s = OrcAPSystem()
+        s.n = self.keep[0].size
+        for k, a in zip(("Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K"), self.keep):
+            setattr(s, k, p(a))
+        self.c = s
+        self.n = s.n
+
+    def init_state(self):
+        y = np.empty(4 * self.n)
+        oracle().orc_ap_init_state(C.byref(self.c), p(y))
+        return y
+
+    def odefun(self, y):
+        y = f64(y)
+        dydt, v, t = np.empty(4 * self.n), np.empty(self.n), np.empty(self.n)
+        lib = oracle()
+        lib.orc_ap_odefun.restype = C.c_double
+        vmax = lib.orc_ap_odefun(C.byref(self.c), p(y), p(dydt), p(v), p(t))
+        return dydt, vmax
+
+    def integrate(self, y0, interval, dt0=1e-3, max_steps=10**6, epsilon=1e-6,
+                  max_step=1.7976931348623158e308, log_cap=0):
+        y = f64(y0).copy()
+        logs = [np.empty(max(log_cap, 1)) for _ in range(3)]
+        st = OrcStats()
+        lib = oracle()
+        lib.orc_ap_integrate.restype = C.c_long
+        rc = lib.orc_ap_integrate(C.byref(self.c), p(y), C.c_double(interval), C.c_double(dt0),
+                                  C.c_long(max_steps), C.c_double(epsilon),
+                                  C.c_double(max_step), p(logs[0]), p(logs[1]), p(logs[2]),
+                                  C.c_long(log_cap), C.byref(st))
+        n = min(st.steps, log_cap)
+        return y, st, logs[0][:n], logs[1][:n], logs[2][:n], rc
+
+
 class System:
     def __init__(self, fric, K, rheo=None, ic_variant=0, nthreads=1):
         self.keep = []

@@ -326,3 +326,49 @@ def volume_grid(nx: int = 32, ny: int = 32, nz: int = 8, size: float = 4000.0,
     return VolumeModel(x=x, length=size * one, thickness=size * one, width=size * one,
                        strike=0.0 * one, gammadot0=g0, npower=npower * one, edot_bg=eb,
                        Gm=1.0 * one, Q=0.0 * one, R=8.314 * one, To=873.0 * one)
+
+
+@dataclass
+class AntiplaneModel:
+    """Vertical strike-slip fault of the 2-D antiplane model
+    (fortran/2d/antiplane): n segments of width W from the surface down,
+    velocity-weakening between depths vw_top and vw_bottom."""
+    mu: float
+    x: np.ndarray          # (n, 3) upper corners; x2 = 0, x3 = depth
+    width: np.ndarray
+    dip: np.ndarray        # radians (the Green's function ignores it)
+    Vpl: np.ndarray
+    mu0: np.ndarray
+    sig: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    L: np.ndarray
+    Vo: np.ndarray
+    damping: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.width.size
+
+    def friction(self):
+        """Engine.set_friction order (the rake is unused in 2-D)."""
+        return (self.Vpl, np.zeros(self.n), self.mu0, self.sig, self.a, self.b, self.L, self.Vo,
+                self.damping)
+
+    def oracle_friction(self):
+        return (self.Vpl, self.mu0, self.sig, self.a, self.b, self.L, self.Vo, self.damping)
+
+
+def antiplane_fault(n: int = 400, depth: float = 40e3, vw_top: float = 2e3,
+                    vw_bottom: float = 15e3, mu: float = 30e3, vs: float = 3000.0,
+                    L: float = 0.01, sig: float = 100.0, Vpl: float = 1e-9) -> AntiplaneModel:
+    W = depth / n
+    z = np.arange(n) * W
+    x = np.zeros((n, 3))
+    x[:, 2] = z
+    one = np.ones(n)
+    mid = z + W / 2
+    b = np.where((mid > vw_top) & (mid < vw_bottom), 0.014, 0.006)
+    return AntiplaneModel(mu=mu, x=x, width=W * one, dip=np.pi / 2 * one, Vpl=Vpl * one,
+                          mu0=0.6 * one, sig=sig * one, a=0.010 * one, b=b, L=L * one,
+                          Vo=1e-6 * one, damping=mu / (2.0 * vs) * one)

@@ -293,6 +293,7 @@ extern "C" int ucy_set_rectangles(ucy_ctx *c, int ns, const double *x, const dou
     for (int i = 0; i < ns; i++)
         if (!(std::max(length[i], width[i]) > 0.0))  // ratestate.f90:1184
             return fail(c, UCY_ERR_ARG, "patch %d: length and width must be positive", i + 1);
+    c->model = 0;
     c->nrect = ns;
     c->rx.assign(x, x + 3 * (size_t)ns);
     c->rL.assign(length, length + ns);
@@ -303,6 +304,52 @@ extern "C" int ucy_set_rectangles(ucy_ctx *c, int ns, const double *x, const dou
     return UCY_OK;
 }
 
+// the 2-D antiplane model (fortran/2d/antiplane): ns segments with upper
+// corner x(3,ns) (x(2), x(3) used), width and dip (radians); replaces any
+// 3-D elements
+extern "C" int ucy_set_antiplane(ucy_ctx *c, double mu, int ns, const double *x,
+                                 const double *width, const double *dip)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    if (!(mu > 0.0)) return fail(c, UCY_ERR_ARG, "shear modulus must be positive");
+    for (int i = 0; i < ns; i++)
+        if (!(width[i] > 0.0))  // ratestate.f90 (2-D antiplane):1090-1094
+            return fail(c, UCY_ERR_ARG, "patch %d: width must be positive", i + 1);
+    c->model = 1;
+    c->mu = mu;
+    c->lambda = 0.0;
+    c->have_elastic = true;
+    c->nrect = ns;
+    c->rx.assign(x, x + 3 * (size_t)ns);
+    c->rW.assign(width, width + ns);
+    c->rL.assign(width, width + ns);
+    c->rstrike.assign(ns, 0.0);
+    c->rdip.assign(dip, dip + ns);
+    c->ntri = c->nvol = c->nve = 0;
+    c->params_dirty = true;
+    return UCY_OK;
+}
+
+// antiplane.f90:81-109
+static void refsys_antiplane(int ns, const double *x, const double *W, const double *dip,
+                             std::vector<Receiver> &R)
+{
+    R.resize(ns);
+    for (int i = 0; i < ns; i++) {
+        Receiver &r = R[i];
+        r.sv[0] = 1.0;
+        r.sv[1] = 0.0;
+        r.sv[2] = 0.0;
+        r.dv[0] = +0.0;
+        r.dv[1] = -cos(dip[i]);
+        r.dv[2] = -sin(dip[i]);
+        r.nv[0] = -0.0;
+        r.nv[1] = +sin(dip[i]);
+        r.nv[2] = -cos(dip[i]);
+        for (int k = 0; k < 3; k++) r.xc[k] = x[3 * i + k] - W[i] / 2 * r.dv[k];
+    }
+}
+
 extern "C" int ucy_set_triangles(ucy_ctx *c, int ns, const int *i1, const int *i2, const int *i3,
                                  int nVe, const double *v)
 {
@@ -310,6 +357,8 @@ extern "C" int ucy_set_triangles(ucy_ctx *c, int ns, const int *i1, const int *i
     for (int j = 0; j < ns; j++)
         if (i1[j] < 1 || i1[j] > nVe || i2[j] < 1 || i2[j] > nVe || i3[j] < 1 || i3[j] > nVe)
             return fail(c, UCY_ERR_ARG, "triangle %d: vertex index out of range", j + 1);
+    if (c->model) c->nrect = 0;
+    c->model = 0;
     c->ntri = ns;
     c->nve = nVe;
     c->i1.assign(i1, i1 + ns);
@@ -327,6 +376,8 @@ extern "C" int ucy_set_volumes(ucy_ctx *c, int ns, const double *x, const double
     for (int i = 0; i < ns; i++)
         if (x[3 * i + 2] < 0)  // strainvolume.f90:1069-1072
             return fail(c, UCY_ERR_GEOMETRY, "volume %d: source depth (q3) must be positive", i + 1);
+    if (c->model) c->nrect = 0;
+    c->model = 0;
     c->nvol = ns;
     c->vx.assign(x, x + 3 * (size_t)ns);
     c->vL.assign(length, length + ns);
@@ -413,8 +464,9 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipSetDevice(c->device));
     const int np = c->nrect + c->ntri, ne = np + c->nvol;
     if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
-    const int64_t m = 3 * (int64_t)np + 6 * (int64_t)c->nvol;
-    const int64_t n = 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const bool ap = c->model == 1;
+    const int64_t m = ap ? np : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t n = ap ? np + (np & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
     c->K = nullptr;
@@ -427,7 +479,44 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     c->m = m;
     c->n = n;
+    c->nl = ap ? np : n;
     std::vector<Receiver> R;
+    if (ap) {  // buildG of fortran/2d/antiplane (greens.f90:93-248)
+        refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R);
+        std::vector<double> S(3 * (size_t)np);
+        for (int i = 0; i < np; i++) {
+            S[3 * i] = c->rx[3 * i + 1];
+            S[3 * i + 1] = c->rx[3 * i + 2];
+            S[3 * i + 2] = c->rW[i];
+        }
+        Receiver *d_R = nullptr;
+        double *d_S = nullptr;
+        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * np));
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(double) * 3 * np));
+        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * np, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice,
+                              c->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        if (n != np) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
+        launch_ap_rows(c->stream, d_S, np, d_R, np, c->mu, c->K, n, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->build_seconds[0] =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->build_seconds[1] = c->build_seconds[2] = 0.0;
+        HIPCHK(hipFree(d_R));
+        HIPCHK(hipFree(d_S));
+        if (precision != UCY_PREC_F64) {
+            HIPCHK(hipMalloc((void **)&c->Kp, (size_t)(m * n) * sizeof(uint32_t)));
+            HIPCHK(hipMalloc((void **)&c->Krs, (size_t)m * sizeof(double)));
+            launch_split_rowmajor(c->stream, c->K, m, n, c->Krs, c->Kp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->kprec = precision;
+        }
+        return UCY_OK;
+    }
     element_frames(c, R);
     Receiver *d_R = nullptr;
     HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * ne));
@@ -505,7 +594,7 @@ extern "C" int ucy_kernel_shape(const ucy_ctx *c, int64_t *m, int64_t *n)
 {
     if (!c) return UCY_ERR_ARG;
     if (m) *m = c->m;
-    if (n) *n = c->n;
+    if (n) *n = c->nl;
     return UCY_OK;
 }
 
@@ -515,8 +604,8 @@ extern "C" int ucy_get_kernel(ucy_ctx *c, double *K)
 {
     if (!c || !K) return UCY_ERR_ARG;
     if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
-    HIPCHK(hipMemcpyAsync(K, c->K, (size_t)(c->m * c->n) * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
+    HIPCHK(hipMemcpy2DAsync(K, sizeof(double) * c->nl, c->K, sizeof(double) * c->n,
+                            sizeof(double) * c->nl, c->m, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return UCY_OK;
 }
@@ -525,8 +614,10 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
 {
     if (!c || !K || m <= 0 || n <= 0) return UCY_ERR_ARG;
     const int np = c->nrect + c->ntri;
-    if (m != 3 * (int64_t)np + 6 * (int64_t)c->nvol || n != 2 * (int64_t)np + 6 * (int64_t)c->nvol)
-        return fail(c, UCY_ERR_ARG, "kernel shape does not match the mesh");
+    const bool shape_ok = c->model == 1
+        ? (m == np && n == np)
+        : (m == 3 * (int64_t)np + 6 * (int64_t)c->nvol && n == 2 * (int64_t)np + 6 * (int64_t)c->nvol);
+    if (!shape_ok) return fail(c, UCY_ERR_ARG, "kernel shape does not match the mesh");
     HIPCHK(hipSetDevice(c->device));
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
@@ -536,13 +627,56 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
     if (c->Krs) HIPCHK(hipFree(c->Krs));
     c->Krs = nullptr;
     c->kprec = UCY_PREC_F64;
-    if (n % 2) return fail(c, UCY_ERR_STATE, "odd column count");
-    HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
-    HIPCHK(hipMemcpyAsync(c->K, K, (size_t)(m * n) * sizeof(double), hipMemcpyHostToDevice,
-                          c->stream));
+    const int64_t ld = c->model == 1 ? n + (n & 1) : n;
+    if (ld % 2) return fail(c, UCY_ERR_STATE, "odd column count");
+    HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * ld) * sizeof(double)));
+    if (ld != n) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * ld) * sizeof(double), c->stream));
+    HIPCHK(hipMemcpy2DAsync(c->K, sizeof(double) * ld, K, sizeof(double) * n, sizeof(double) * n,
+                            m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->m = m;
-    c->n = n;
+    c->n = ld;
+    c->nl = n;
+    return UCY_OK;
+}
+
+// computeTractionKernelsAntiplane (fortran/2d/antiplane/src/antiplane.f90:352-379):
+// one receiver (x, sv, dv, nv), ns segment sources y(3,ns), W, dip
+extern "C" int ucy_antiplane_traction_kernels(ucy_ctx *c, const double x[3], const double sv[3],
+                                              const double dv[3], const double nv[3], int ns,
+                                              const double *y, const double *W,
+                                              const double *dip, double G, double *ts)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    (void)dip;  // the dip of a source does not enter (antiplane.f90:126-127)
+    if (ns == 0) return UCY_OK;
+    HIPCHK(hipSetDevice(c->device));
+    Receiver R;
+    for (int k = 0; k < 3; k++) {
+        R.xc[k] = x[k];
+        R.sv[k] = sv[k];
+        R.dv[k] = dv[k];
+        R.nv[k] = nv[k];
+    }
+    std::vector<double> S(3 * (size_t)ns);
+    for (int i = 0; i < ns; i++) {
+        S[3 * i] = y[3 * i + 1];
+        S[3 * i + 1] = y[3 * i + 2];
+        S[3 * i + 2] = W[i];
+    }
+    char *buf = nullptr;
+    const size_t bytes = sizeof(Receiver) + sizeof(double) * 4 * (size_t)ns;
+    HIPCHK(hipMalloc((void **)&buf, bytes));
+    Receiver *d_R = (Receiver *)buf;
+    double *d_S = (double *)(buf + sizeof(Receiver)), *d_o = d_S + 3 * ns;
+    HIPCHK(hipMemcpyAsync(d_R, &R, sizeof(Receiver), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
+                          c->stream));
+    launch_ap_rows(c->stream, d_S, ns, d_R, 1, G, nullptr, 0, d_o);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ts, d_o, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(buf));
     return UCY_OK;
 }
 
@@ -561,6 +695,8 @@ extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
     c->O = nullptr;
     c->o_rows = c->o_n = 0;
     if (npts == 0) return UCY_OK;
+    if (c->model == 1)
+        return fail(c, UCY_ERR_STATE, "observation points of the antiplane model are not built");
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
     const int np = c->nrect + c->ntri;
     if (np + c->nvol == 0) return fail(c, UCY_ERR_STATE, "no elements");
@@ -705,8 +841,9 @@ static int upload_params(ucy_ctx *c)
         F[i].sig = c->sig[i];
         F[i].L = c->Lc[i];
         F[i].damping = c->damping[i];
-        F[i].vplc = c->Vpl[i] * cos(c->rake[i]);
-        F[i].vpls = c->Vpl[i] * sin(c->rake[i]);
+        // antiplane: v = V - Vpl (ratestate.f90 2-D:676); 3-D: ratestate.f90:701-702
+        F[i].vplc = c->model == 1 ? c->Vpl[i] : c->Vpl[i] * cos(c->rake[i]);
+        F[i].vpls = c->model == 1 ? 0.0 : c->Vpl[i] * sin(c->rake[i]);
     }
     std::vector<Rheology> H(c->nvol + 1);
     for (int i = 0; i < c->nvol; i++) {
@@ -732,7 +869,8 @@ static PlainDims dims(const ucy_ctx *c)
     PlainDims d;
     d.np = c->nrect + c->ntri;
     d.nv = c->nvol;
-    d.dim = 7 * d.np + 12 * d.nv;
+    d.ap = c->model == 1;
+    d.dim = d.ap ? 4 * d.np : 7 * d.np + 12 * d.nv;
     d.m = (int)c->m;
     d.n = (int)c->n;
     return d;
@@ -791,7 +929,9 @@ extern "C" int ucy_matvec_reduced(ucy_ctx *c, const double *v, double *t)
     double *dv = nullptr, *dt = nullptr;
     HIPCHK(hipMalloc((void **)&dv, sizeof(double) * c->n));
     HIPCHK(hipMalloc((void **)&dt, sizeof(double) * c->m));
-    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    if (c->nl != c->n)
+        HIPCHK(hipMemsetAsync(dv + c->nl, 0, sizeof(double) * (c->n - c->nl), c->stream));
+    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->nl, hipMemcpyHostToDevice, c->stream));
     launch_matvec_split(c->stream, c->Kp, c->Krs, (int)c->m, (int)c->n, dv, dt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(t, dt, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
@@ -807,6 +947,16 @@ extern "C" int ucy_matvec_f64_dev(ucy_ctx *c, const double *v_dev, double *t_dev
     if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
     if (c->n % 2) return fail(c, UCY_ERR_STATE, "odd column count");
     HIPCHK(hipSetDevice(c->device));
+    if (c->nl != c->n) {  // the caller's v has nl entries: pad it with zeros
+        if (ensure_buf(c, c->hook, sizeof(double) * (size_t)c->n)) return UCY_ERR_HIP;
+        double *pv = (double *)c->hook.p;
+        if (pv != v_dev) {
+            HIPCHK(hipMemsetAsync(pv + c->nl, 0, sizeof(double) * (c->n - c->nl), c->stream));
+            HIPCHK(hipMemcpyAsync(pv, v_dev, sizeof(double) * c->nl, hipMemcpyDeviceToDevice,
+                                  c->stream));
+        }
+        v_dev = pv;
+    }
     launch_matvec(c->stream, c->K, (int)c->m, (int)c->n, v_dev, t_dev);
     HIPCHK(hipGetLastError());
     return UCY_OK;
@@ -819,7 +969,9 @@ extern "C" int ucy_matvec_f64(ucy_ctx *c, const double *v, double *t)
     HIPCHK(hipSetDevice(c->device));
     if (ensure_buf(c, c->hook, sizeof(double) * (size_t)(c->m + c->n))) return UCY_ERR_HIP;
     double *dv = (double *)c->hook.p, *dt = dv + c->n;
-    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+    if (c->nl != c->n)
+        HIPCHK(hipMemsetAsync(dv + c->nl, 0, sizeof(double) * (c->n - c->nl), c->stream));
+    HIPCHK(hipMemcpyAsync(dv, v, sizeof(double) * c->nl, hipMemcpyHostToDevice, c->stream));
     int rc = ucy_matvec_f64_dev(c, dv, dt);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(t, dt, sizeof(double) * c->m, hipMemcpyDeviceToHost, c->stream));
@@ -835,6 +987,17 @@ extern "C" int ucy_init_state(ucy_ctx *c, int variant, double *y)
     if (np && (int)c->Vpl.size() != np) return fail(c, UCY_ERR_STATE, "friction not set");
     if (c->nvol && (int)c->gd0.size() != c->nvol) return fail(c, UCY_ERR_STATE, "rheology not set");
     const double lg10 = 2.302585092994045684;
+    if (c->model == 1) {  // ratestate.f90 (2-D antiplane):568-605
+        for (int i = 0; i < np; i++) {
+            double *p = y + 4 * i;
+            p[0] = 0.0;
+            p[1] = c->sig[i] * (c->mu0[i] + (c->a[i] - c->b[i]) * log(c->Vpl[i] / c->Vo[i]))
+                   - c->damping[i] * c->Vpl[i];
+            p[2] = log(c->Lc[i] / c->Vpl[i]) / lg10;
+            p[3] = log(c->Vpl[i] * 0.98) / lg10;
+        }
+        return UCY_OK;
+    }
     for (int i = 0; i < np; i++) {
         double *p = y + 7 * i;
         const double base = c->sig[i] * (c->mu0[i] + (c->a[i] - c->b[i]) * log(c->Vpl[i] / c->Vo[i]));

@@ -138,6 +138,44 @@ okada_disp_kernel(OkadaMedium M, const OkadaSource *__restrict__ src, int ns,
         *reinterpret_cast<double2 *>(O + base + c * ldo) = make_double2(os[c], od[c]);
 }
 
+// buildG of the 2-D antiplane model (fortran/2d/antiplane/src/greens.f90:93-248):
+// traction at receiver r from unit slip on source segment s,
+// computeStressAntiplane (antiplane.f90:131-156; vertical segment from depth
+// q3 to q3 + W at q2 with its image, the dip ignored as in the reference)
+// projected by computeTractionKernelsAntiplane (antiplane.f90:371-373).  One
+// pair per lane; row r is written contiguously.
+__global__ void __launch_bounds__(256)
+ap_rows_kernel(const double *__restrict__ src, int ns, const Receiver *__restrict__ rcv, double G,
+               double *__restrict__ K, int64_t ldk, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (s >= ns) return;
+    const Receiver R = rcv[r];
+    const double q2 = src[3 * s], q3 = src[3 * s + 1], W = src[3 * s + 2];
+    const double pi = 3.141592653589793;  // antiplane.f90:35
+    const double r2 = R.xc[1] - q2;
+    const double a = R.xc[2] - q3, b = R.xc[2] + q3, c = R.xc[2] - q3 - W, d = R.xc[2] + q3 + W;
+    const double s12 = G * (-a / (r2 * r2 + a * a) + b / (r2 * r2 + b * b) + c / (r2 * r2 + c * c)
+                            - d / (r2 * r2 + d * d)) / 2.0 / pi;
+    const double s13 = G * (r2 / (r2 * r2 + a * a) - r2 / (r2 * r2 + b * b)
+                            - r2 / (r2 * r2 + c * c) + r2 / (r2 * r2 + d * d)) / 2.0 / pi;
+    const double ts = (R.nv[1] * s12 + R.nv[2] * s13) * R.sv[0] + (R.nv[0] * s12) * R.sv[1]
+                      + (R.nv[0] * s13) * R.sv[2];
+    if (K) K[(int64_t)r * ldk + s] = ts;
+    if (out) out[s] = ts;
+}
+
+void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *rcv, int nr,
+                    double G, double *K, int64_t ldk, double *out)
+{
+    for (int r0 = 0; r0 < nr && ns > 0; r0 += 65535) {
+        const int nk = std::min(65535, nr - r0);
+        hipLaunchKernelGGL(ap_rows_kernel, dim3((ns + 255) / 256, nk), dim3(256), 0, st, src, ns,
+                           rcv + r0, G, K ? K + (int64_t)r0 * ldk : nullptr, ldk, out);
+    }
+}
+
 void launch_okada_disp(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
                        const double *pts, int npts, double *O, int64_t ldo, int64_t col0)
 {

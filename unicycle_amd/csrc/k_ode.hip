@@ -66,6 +66,28 @@ __device__ __forceinline__ double patch_prepass(const double *p, const Friction 
     return velocity;
 }
 
+// 2-D antiplane patch (fortran/2d/antiplane/src/ratestate.f90:660-677): slip
+// rate V, v = V - Vpl (vplc holds Vpl)
+__device__ __forceinline__ double ap_prepass(const double *p, const Friction &f, double *dp,
+                                             double *vp)
+{
+    const double velocity = exp(p[3] * kLg10);
+    dp[0] = velocity;
+    vp[0] = velocity - f.vplc;
+    return velocity;
+}
+
+// ratestate.f90 (2-D antiplane):698-722
+__device__ __forceinline__ void ap_postpass(const double *p, const Friction &f, double t,
+                                            double *dp)
+{
+    const double velocity = exp(p[3] * kLg10);
+    dp[2] = (exp(-p[2] * kLg10) - velocity / f.L) / kLg10;
+    const double dtau = t;
+    dp[3] = (dtau - f.b * f.sig * dp[2] * kLg10) / (f.a * f.sig + f.damping * velocity) / kLg10;
+    dp[1] = dtau - f.damping * velocity * dp[3] * kLg10;
+}
+
 __device__ __forceinline__ void volume_prepass(const double *q, const Rheology &r, double *dq,
                                                double *vq)
 {
@@ -104,15 +126,27 @@ __device__ __forceinline__ void patch_postpass(const double *p, const Friction &
 
 __device__ __forceinline__ int elem_state_off(const PlainDims &d, int e)
 {
+    if (d.ap) return 4 * e;
     return e < d.np ? 7 * e : 7 * d.np + 12 * (e - d.np);
 }
 __device__ __forceinline__ int elem_v_off(const PlainDims &d, int e)
 {
+    if (d.ap) return e;
     return e < d.np ? 2 * e : 2 * d.np + 6 * (e - d.np);
 }
 __device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
 {
+    if (d.ap) return e;
     return e < d.np ? 3 * e : 3 * d.np + 6 * (e - d.np);
+}
+// state entries and slip-rate entries of element e
+__device__ __forceinline__ int elem_nstate(const PlainDims &d, int e)
+{
+    return d.ap ? 4 : (e < d.np ? 7 : 12);
+}
+__device__ __forceinline__ int elem_nv(const PlainDims &d, int e)
+{
+    return d.ap ? 1 : (e < d.np ? 2 : 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,7 +309,11 @@ __global__ void prepass_kernel(PlainDims d, const Friction *__restrict__ fr,
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (e < d.np) {
+    if (d.ap) {
+        for (int k = 1; k < 4; k++) dydt[so + k] = 0.0;
+        const double vel = ap_prepass(y + so, fr[e], dydt + so, v + vo);
+        if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
+    } else if (e < d.np) {
         for (int k = 2; k < 7; k++) dydt[so + k] = 0.0;
         const double vel = patch_prepass(y + so, fr[e], dydt + so, v + vo);
         if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
@@ -326,7 +364,19 @@ __device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const KOperan
 {
     const int so = elem_state_off(d, e);
     const int64_t ro = elem_row_off(d, e);
-    if (e < d.np) {
+    if (d.ap) {
+        double acc[1];
+        elem_rows<1, MODE>(K, ro, d.n, v, t0, lane, acc);
+        if (lane == 0) {
+            if (t_out) t_out[ro] = acc[0];
+            ap_postpass(y + so, fr[e], acc[0], dydt + so);
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 4; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    } else if (e < d.np) {
         double acc[3];
         elem_rows<3, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
@@ -420,12 +470,15 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
     if (ctl->accepted) {
-        const int ns = e < d.np ? 7 : 12;
+        const int ns = elem_nstate(d, e);
         for (int k = 0; k < ns; k++) b.y[so + k] = b.ycand[so + k];
     }
     // ctl->vmax was cleared by the controller that accepted the previous step
     if (ctl->done || !ctl->need_rhs) return;
-    if (e < d.np) {
+    if (d.ap) {
+        const double vel = ap_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
+    } else if (e < d.np) {
         // vMax of the step-start odefun (ratestate.f90:693), as exported every step
         const double vel = patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
@@ -438,7 +491,7 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
     }
     if (b.v0) {
-        const int nvk = e < d.np ? 2 : 6;
+        const int nvk = elem_nv(d, e);
         for (int k = 0; k < nvk; k++) b.v0[vo + k] = b.v[vo + k];
     }
 }
@@ -491,17 +544,18 @@ __device__ __forceinline__ void stage_elem(const PlainDims &d, const Friction *_
                                            double h, int e, int stage, double *vout)
 {
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    const int ns = e < d.np ? 7 : 12;
+    const int ns = elem_nstate(d, e);
     for (int k = 0; k < ns; k++) {
         const int i = so + k;
         b.ytmp[i] = stage_value(stage, h, b.y[i], b.dydt[i], b.ak[2][i], b.ak[3][i], b.ak[4][i],
                                 b.ak[5][i]);
     }
     double *out = b.ak[stage];
-    if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    if (d.ap) ap_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    else if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
     if (b.v0) {
-        const int nvk = e < d.np ? 2 : 6;
+        const int nvk = elem_nv(d, e);
         for (int k = 0; k < nvk; k++) vout[vo + k] -= b.v0[vo + k];
     }
 }
@@ -547,7 +601,7 @@ __global__ void __launch_bounds__(256) rhs_finish_kernel(PlainDims d, KOperand K
     double emax = 0.0;
     if (live && lane == 0) {
         const double h = ctl->h;
-        const int so = elem_state_off(d, e), ns = e < d.np ? 7 : 12;
+        const int so = elem_state_off(d, e), ns = elem_nstate(d, e);
         for (int k = 0; k < ns; k++) {
             const int i = so + k;
             const double k1 = b.dydt[i], k3 = b.ak[3][i], k4 = b.ak[4][i], k5 = b.ak[5][i],

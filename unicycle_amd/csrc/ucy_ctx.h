@@ -22,6 +22,9 @@ struct ucy_ctx {
     // elastic moduli
     double lambda = 0, mu = 0;
     bool have_elastic = false;
+    // 0: the 3-D model; 1: the 2-D antiplane model (its segments use the
+    // rectangle arrays: x, width, dip)
+    int model = 0;
     // rectangles
     int nrect = 0;
     std::vector<double> rx, rL, rW, rstrike, rdip;
@@ -40,6 +43,10 @@ struct ucy_ctx {
     // when built in reduced precision (kprec)
     double *K = nullptr;
     int64_t m = 0, n = 0;
+    // columns of the model (n is that, padded to even for the pairwise row
+    // reads of the GEMVs; the antiplane model with an odd segment count pads
+    // one zero column)
+    int64_t nl = 0;
     uint32_t *Kp = nullptr;
     double *Krs = nullptr;
     int kprec = 0;

@@ -53,8 +53,10 @@ struct Control {
 struct PlainDims {
     int np;      // patches
     int nv;      // volumes
-    int dim;     // 7*np + 12*nv
+    int dim;     // 7*np + 12*nv (4*np for the antiplane model)
     int m, n;    // K rows, columns
+    int ap;      // 1: 2-D antiplane patches (fortran/2d/antiplane: 4 state entries,
+                 // one slip rate and one traction row per patch, no volumes)
 };
 
 // ---- assembly launchers (k_assemble.hip) ----
@@ -68,6 +70,11 @@ void launch_okada_rows(hipStream_t st, const OkadaMedium &M, const OkadaSource *
 // rectangle sources, columns col0 + 2s + {0,1}
 void launch_okada_disp(hipStream_t st, const OkadaMedium &M, const OkadaSource *src, int ns,
                        const double *pts, int npts, double *O, int64_t ldo, int64_t col0);
+
+// antiplane buildG rows for receivers [0, nr) from ns segment sources
+// src[3s..3s+2] = (q2, q3, W); out (one receiver) may replace K
+void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *rcv, int nr,
+                    double G, double *K, int64_t ldk, double *out);
 
 // ---- GEMV / RHS launchers (k_ode.hip) ----
 void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t);

@@ -84,6 +84,7 @@ class Engine:
             raise UcyError(-2, f"cannot create a context on device {device}")
         self.np = 0
         self.nvol = 0
+        self.antiplane = False
         self._ensembles = []
 
     # -- plumbing ---------------------------------------------------------
@@ -116,9 +117,23 @@ class Engine:
         self.lambda_, self.mu = float(lambda_), float(mu)
         self._chk(self._L.ucy_set_elastic(self._h, self.lambda_, self.mu))
 
+    def set_antiplane(self, mu, x, width, dip):
+        """The 2-D antiplane model (fortran/2d/antiplane): vertical segments
+        with upper corners x (ns, 3; x2, x3 used), width, dip (radians),
+        shear modulus mu.  State: 4 entries per patch."""
+        x = _f64(x).reshape(-1, 3)
+        self._ap = [x, _f64(width), _f64(dip)]
+        self.mu = float(mu)
+        self._chk(self._L.ucy_set_antiplane(self._h, self.mu, x.shape[0],
+                                            *[_p(a) for a in self._ap]))
+        self.antiplane = True
+        self.nrect, self.ntri, self.nvol = x.shape[0], 0, 0
+        self._update_np()
+
     def set_rectangles(self, x, length, width, strike, dip):
         x = _f64(x).reshape(-1, 3)
         ns = x.shape[0]
+        self.antiplane = False
         self._rect = [x, _f64(length), _f64(width), _f64(strike), _f64(dip)]
         self.nrect = ns
         self._chk(self._L.ucy_set_rectangles(self._h, ns, *[_p(a) for a in self._rect]))
@@ -129,6 +144,8 @@ class Engine:
         idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
         v = _f64(vertices).reshape(-1, 3)
         self._tri = idx + [v]
+        if self.antiplane:
+            self.nrect, self.antiplane = 0, False
         self.ntri = idx[0].size
         self._chk(self._L.ucy_set_triangles(self._h, self.ntri, _pi(idx[0]), _pi(idx[1]),
                                             _pi(idx[2]), v.shape[0], _p(v)))
@@ -137,6 +154,9 @@ class Engine:
     def set_volumes(self, x, length, thickness, width, strike):
         x = _f64(x).reshape(-1, 3)
         self._vol = [x, _f64(length), _f64(thickness), _f64(width), _f64(strike)]
+        if self.antiplane:
+            self.nrect, self.antiplane = 0, False
+            self._update_np()
         self.nvol = x.shape[0]
         self._chk(self._L.ucy_set_volumes(self._h, self.nvol, *[_p(a) for a in self._vol]))
 
@@ -184,7 +204,7 @@ class Engine:
 
     @property
     def dim(self) -> int:
-        return 7 * self.np + 12 * self.nvol
+        return 4 * self.np if self.antiplane else 7 * self.np + 12 * self.nvol
 
     def get_kernel(self) -> np.ndarray:
         m, n = self.shape
@@ -347,6 +367,13 @@ class Engine:
             *[_pi(a) for a in idx], int(nVe), _p(_f64(vertices).reshape(-1)), float(s),
             float(d), float(t), float(G), float(lambda_), *[_p(o) for o in out]))
         return tuple(out)
+
+    def computeTractionKernelsAntiplane(self, x, sv, dv, nv, ns, y, W, dip, G):
+        ts = np.empty(ns)
+        self._chk(self._L.ucy_antiplane_traction_kernels(
+            self._h, _p(_vec3(x)), _p(_vec3(sv)), _p(_vec3(dv)), _p(_vec3(nv)), int(ns),
+            _p(_f64(y)), _p(_f64(W)), _p(_f64(dip)), float(G), _p(ts)))
+        return ts
 
     def straininhalfspace(self, points, P1, P2, P3, Ss, Ds, Ts, mu, lambda_):
         """straininhalfspace_ (StrainInHalfSpace.c:42) for n (point, triangle)
