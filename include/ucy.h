@@ -201,7 +201,11 @@ const double *ucy_kernel_dev(const ucy_ctx *ctx);
  * (okada92.f90:230-298, zero at a DC3D-singular fault edge); triangles:
  * computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367); volumes:
  * computeDisplacementVerticalStrainVolume (strainvolume.f90:215-268, its
- * STOPs as UCY_ERR_GEOMETRY).  npts = 0 drops the kernels. */
+ * STOPs as UCY_ERR_GEOMETRY).  npts = 0 drops the kernels.  The antiplane
+ * model (ucy_set_antiplane) has one displacement component per point and one
+ * column per segment: computeDisplacementAntiplane (fortran/2d/antiplane/src/
+ * antiplane.f90:178-191, greens.f90:275-383); rows = npts; there d = O u with
+ * u the segments' slip, df = d, dl = 0. */
 int ucy_build_displacement(ucy_ctx *ctx, int npts, const double *x);
 /* shape of O (rows = 3 npts, n); 0 x 0 when not built */
 int ucy_displacement_shape(const ucy_ctx *ctx, int64_t *rows, int64_t *n);

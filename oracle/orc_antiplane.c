@@ -89,6 +89,18 @@ void orc_buildG_antiplane(int ns, const double *x, const double *W, const double
     free(sv);
 }
 
+/* buildO (greens.f90:275-383) for patches: O (npts x ns) row-major, unit
+ * slip of each segment at each point x(3,npts) (x2, x3 used) */
+void orc_buildO_antiplane(int ns, const double *x, const double *W, const double *dip, int npts,
+                          const double *pts, double *O)
+{
+    for (int k = 0; k < npts; k++)
+        for (int s = 0; s < ns; s++)
+            O[(size_t)k * ns + s] = orc_displacement_antiplane(pts[3 * k + 1], pts[3 * k + 2],
+                                                               x[3 * s + 1], x[3 * s + 2], W[s],
+                                                               dip[s], 1.0);
+}
+
 /* initStateVector (ratestate.f90:568-605): slip, traction, log10 theta,
  * log10 V = log10(0.98 Vpl) */
 void orc_ap_init_state(const orc_ap_system *s, double *y)

@@ -166,6 +166,8 @@ void orc_traction_kernels_antiplane(const double x[3], const double sv[3], const
 /* K (ns x ns) row-major: row = receiver traction, column = source slip */
 void orc_buildG_antiplane(int ns, const double *x, const double *W, const double *dip, double mu,
                           double *K, int nthreads);
+void orc_buildO_antiplane(int ns, const double *x, const double *W, const double *dip, int npts,
+                          const double *pts, double *O);
 typedef struct {
     int n;                 /* patches; state 4 per patch: slip, traction, log10 theta, log10 V */
     const double *Vpl, *mu0, *sig, *a, *b, *L, *Vo, *damping;

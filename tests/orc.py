@@ -156,6 +156,14 @@ def buildG_antiplane(x, W, dip, mu, nthreads=1):
     return K
 
 
+def buildO_antiplane(x, W, dip, points):
+    x, W, dip, pts = f64(x), f64(W), f64(dip), f64(points)
+    ns, npts = W.size, pts.size // 3
+    O = np.empty((npts, ns))
+    oracle().orc_buildO_antiplane(ns, p(x), p(W), p(dip), npts, p(pts), p(O))
+    return O
+
+
 class APSystem:
     """2-D antiplane rate-and-state (fortran/2d/antiplane/src/ratestate.f90)."""
 

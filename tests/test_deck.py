@@ -231,3 +231,69 @@ def test_viscous_deck_round_trip():
     # the volume-free deck with no elements at all is refused
     with pytest.raises(deck.DeckError, match="nothing to do"):
         deck.parse_viscous(deck.format_viscous("w", 30e3, 30e3, 1e9))
+
+
+# --------------------------------------------------------------- 2-D antiplane --
+APDECK = """# fortran/2d/antiplane deck
+out/ap
+# shear modulus
+30e3
+# time interval
+3.15e9
+# number of patches
+2
+# n Vpl x2 x3 width dip
+1 1e-9 0 0 1000 90
+2 1d-9, 0, 1000, 1000, 90
+# number of frictional patches
+2
+1 0.6 100 0.01 0.014 0.01 1e-6 5
+2 0.6 100 0.01 0.006 0.01 1e-6 5
+# number of observation patches
+1
+1 2
+# number of observation points
+1
+1 GPS01 5000 0
+"""
+
+
+def test_parse_antiplane_deck():
+    d = deck.parse_antiplane(APDECK)
+    assert d.n == 2 and d.mu == 30e3 and d.interval == 3.15e9
+    np.testing.assert_array_equal(d.x, [[0, 0, 0], [0, 0, 1000]])
+    assert d.dip[0] == 90 * deck.DEG2RAD
+    assert d.observation_states == [2]
+    assert d.observation_points == [("GPS01", 0.0, 5000.0, 0.0)]
+    Vpl, rake, mu0, sig, a, b, L, Vo, eta = d.friction_arrays()
+    np.testing.assert_array_equal(b, [0.014, 0.006])
+    assert not rake.any()
+
+
+@pytest.mark.parametrize("old,new,needle", [
+    ("30e3\n# time", "-1\n# time", "shear modulus"),
+    ("3.15e9", "0", "simulation time"),
+    ("2 1d-9, 0, 1000, 1000, 90", "3 1d-9, 0, 1000, 1000, 90", "unexpected index"),
+    ("1 1e-9 0 0 1000 90", "1 1e-9 0 0 0 90", "width must be positive"),
+    ("# number of frictional patches\n2", "# number of frictional patches\n1",
+     "frictional properties"),
+    ("1 GPS01 5000 0", "2 GPS01 5000 0", "unexpected index"),
+    ("# number of observation patches\n1\n1 2", "# number of observation patches\n1\n1 3",
+     "not an element"),
+])
+def test_antiplane_deck_errors(old, new, needle):
+    assert old in APDECK
+    with pytest.raises(deck.DeckError, match=needle):
+        deck.parse_antiplane(APDECK.replace(old, new, 1))
+
+
+def test_antiplane_deck_round_trip():
+    m = configs.antiplane_fault(n=40)
+    d = deck.parse_antiplane(deck.format_antiplane("w", m, 1e9, observe=[1, 40],
+                                                   points=[("A", 1e3, 0.0)]))
+    np.testing.assert_array_equal(d.x, m.x)
+    np.testing.assert_array_equal(d.width, m.width)
+    np.testing.assert_allclose(d.dip, m.dip, rtol=1e-15)
+    for got, want in zip(d.friction_arrays(), m.friction()):
+        np.testing.assert_allclose(got, want, rtol=1e-15)
+    assert d.observation_states == [1, 40] and d.observation_points == [("A", 0.0, 1e3, 0.0)]

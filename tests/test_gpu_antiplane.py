@@ -99,3 +99,29 @@ def test_reduced_precision_tracks_f64(engine_factory, prec, n):
           f"end times {a.time:.6e} / {b.time:.6e}", flush=True)
     assert b.attempts <= 1.2 * a.attempts + 20
     assert err < (1e-6 if prec == PREC_MIXED else 1e-4)
+
+
+def test_buildO_and_displacement_match_oracle(engine_factory):
+    """buildO of the 2-D model (greens.f90:275-383): one displacement
+    component per point; d = O u with u the segments' slip, df = d, dl = 0."""
+    m = configs.antiplane_fault(n=151)
+    eng = _engine(engine_factory, m)
+    rng = np.random.default_rng(4)
+    pts = np.stack([np.zeros(30), rng.uniform(-5e4, 5e4, 30),
+                    np.where(np.arange(30) % 2 == 0, 0.0, rng.uniform(0, 3e4, 30))], 1)
+    eng.buildO(pts)
+    O, Of, Ol = eng.get_displacement_kernels()
+    Oref = orc.buildO_antiplane(m.x, m.width, m.dip, pts)
+    assert O.shape == Oref.shape == (30, m.n)
+    # four atan terms cancel to the far field: rounding-level differences of
+    # the device's atan, relative to each row's size
+    err = np.abs(O - Oref).max(axis=1) / np.abs(Oref).max(axis=1)
+    assert err.max() < 1e-11, err.max()
+    np.testing.assert_array_equal(Of, O)
+    assert not Ol.any()
+    y = eng.initStateVector()
+    y[0::4] = rng.standard_normal(m.n)
+    d, df, dl = eng.displacement(y)
+    np.testing.assert_allclose(d, O @ y[0::4], rtol=1e-12, atol=1e-12 * np.abs(O).sum(1).max())
+    np.testing.assert_array_equal(df, d)
+    assert not dl.any()

@@ -188,3 +188,40 @@ def test_viscous_deck_run_matches_oracle(tmp_path, engine_factory):
                                        atol=1e-9 * np.abs(want).max() + 1e-300)
     assert os.path.getsize(os.path.join(wdir, "volume.vtp")) > 0
     assert np.abs(y - yref).max() <= 1e-5 * np.abs(yref).max()
+
+
+def test_antiplane_deck_run_matches_oracle(tmp_path, engine_factory):
+    """fortran/2d/antiplane deck through the driver: time.txt and the patch
+    files (4 state entries) against the oracle's run of the same model, and
+    the one-column opts rows against the oracle's buildO times the slips the
+    run wrote (every patch observed)."""
+    m = configs.antiplane_fault(n=100)
+    wdir = str(tmp_path / "ap")
+    pts = [("GPS1", 2e3, 0.0), ("GPS2", 2e4, 0.0), ("BH", 500.0, 3e3)]
+    d = deck.parse_antiplane(deck.format_antiplane(wdir, m, 100 * configs.YEAR,
+                                                   observe=list(range(1, m.n + 1)), points=pts))
+    steps = 120
+    log = io.StringIO()
+    it, _, y = ratestate.run(d, max_iterations=steps, chunk=50, out=log)
+    assert it == steps
+    K = orc.buildG_antiplane(d.x, d.width, d.dip, d.mu, nthreads=8)
+    sysm = orc.APSystem(m.oracle_friction(), K)
+    y0 = sysm.init_state()
+    yref, st, tl, dl, vl, rc = sysm.integrate(y0, d.interval, max_steps=steps, log_cap=steps)
+    tt = _read(os.path.join(wdir, "time.txt"), [19, 19])
+    np.testing.assert_allclose(tt[1:, 0], np.concatenate([[0.0], tl[:-1]]), rtol=1e-6, atol=0)
+    slip = np.zeros((steps, m.n))
+    for e in range(m.n):
+        p = _read(os.path.join(wdir, f"patch-{e + 1:08d}.dat"), [19] + [20] * 8)
+        np.testing.assert_allclose(p[0, 1:5], y0[4 * e:4 * e + 4], rtol=1e-12, atol=0)
+        slip[:, e] = p[:, 1]
+    O = orc.buildO_antiplane(d.x, d.width, d.dip, [p[1:] for p in d.observation_points])
+    want = slip @ O.T
+    for j, (name, *_x) in enumerate(pts):
+        q = _read(os.path.join(wdir, f"opts-{name}.dat"), [19] * 2)
+        assert q.shape == (steps, 2)
+        np.testing.assert_array_equal(q[:, 0], tt[1:, 0])
+        np.testing.assert_allclose(q[:, 1], want[:, j], rtol=1e-6,
+                                   atol=1e-9 * np.abs(want).max() + 1e-300)
+    assert not os.path.exists(os.path.join(wdir, "patch-rectangular.vtp"))
+    assert np.abs(y - yref).max() <= 1e-6 * np.abs(yref).max()

@@ -695,9 +695,32 @@ extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
     c->O = nullptr;
     c->o_rows = c->o_n = 0;
     if (npts == 0) return UCY_OK;
-    if (c->model == 1)
-        return fail(c, UCY_ERR_STATE, "observation points of the antiplane model are not built");
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
+    if (c->model == 1) {  // fortran/2d/antiplane/src/greens.f90:275-383
+        const int ns = c->nrect;
+        if (ns == 0) return fail(c, UCY_ERR_STATE, "no elements");
+        std::vector<double> S(3 * (size_t)ns);
+        for (int i = 0; i < ns; i++) {
+            S[3 * i] = c->rx[3 * i + 1];
+            S[3 * i + 1] = c->rx[3 * i + 2];
+            S[3 * i + 2] = c->rW[i];
+        }
+        double *buf = nullptr, *O = nullptr;
+        HIPCHK(hipMalloc((void **)&buf, sizeof(double) * 3 * ((size_t)ns + npts)));
+        HIPCHK(hipMalloc((void **)&O, sizeof(double) * (size_t)npts * ns));
+        HIPCHK(hipMemcpyAsync(buf, S.data(), sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(buf + 3 * ns, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice,
+                              c->stream));
+        launch_ap_disp(c->stream, buf, ns, buf + 3 * ns, npts, O, ns);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(buf));
+        c->O = O;
+        c->o_rows = npts;
+        c->o_n = ns;
+        return UCY_OK;
+    }
     const int np = c->nrect + c->ntri;
     if (np + c->nvol == 0) return fail(c, UCY_ERR_STATE, "no elements");
     const int64_t rows = 3 * (int64_t)npts, n = 2 * (int64_t)np + 6 * (int64_t)c->nvol;
@@ -753,7 +776,9 @@ extern "C" int ucy_displacement_shape(const ucy_ctx *c, int64_t *rows, int64_t *
 static int check_disp(ucy_ctx *c)
 {
     if (!c->O) return fail(c, UCY_ERR_STATE, "displacement kernels not built");
-    if (c->o_n != 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol)
+    const int64_t want = c->model == 1 ? (int64_t)c->nrect
+                                       : 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol;
+    if (c->o_n != want)
         return fail(c, UCY_ERR_STATE, "displacement kernels built for another mesh");
     return 0;
 }
@@ -773,9 +798,10 @@ extern "C" int ucy_get_displacement_kernel(ucy_ctx *c, double *O, double *Of, do
     }
     HIPCHK(hipMemcpyAsync(h, c->O, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const int64_t nf = 2 * (int64_t)(c->nrect + c->ntri);
+    const bool ap = c->model == 1;
+    const int64_t nf = ap ? c->o_n : 2 * (int64_t)(c->nrect + c->ntri);
     for (int64_t r = 0; r < c->o_rows; r++) {
-        const bool vertical = r % 3 == 2;
+        const bool vertical = !ap && r % 3 == 2;
         for (int64_t j = 0; j < c->o_n; j++) {
             const double v = h[r * c->o_n + j];
             const bool f = j < nf || vertical;
@@ -793,8 +819,8 @@ extern "C" int ucy_displacement_dev(ucy_ctx *c, const double *y, double *d, doub
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    launch_disp_gemv(c->stream, c->O, c->o_rows, c->o_n, c->nrect + c->ntri, y, d, df, dl,
-                     nullptr);
+    launch_disp_gemv(c->stream, c->O, c->o_rows, c->o_n, c->nrect + c->ntri, c->model == 1, y, d,
+                     df, dl, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     return UCY_OK;
@@ -806,14 +832,15 @@ extern "C" int ucy_displacement(ucy_ctx *c, const double *y, double *d, double *
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    const int64_t dim = 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
+    const int64_t dim = c->model == 1 ? 4 * (int64_t)c->nrect
+                                      : 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
     const int64_t R = c->o_rows;
     double *buf = nullptr;
     HIPCHK(hipMalloc((void **)&buf, sizeof(double) * (size_t)(dim + 3 * R)));
     double *dy = buf, *dd = buf + dim, *ddf = dd + R, *ddl = ddf + R;
     HIPCHK(hipMemcpyAsync(dy, y, sizeof(double) * dim, hipMemcpyHostToDevice, c->stream));
-    launch_disp_gemv(c->stream, c->O, R, c->o_n, c->nrect + c->ntri, dy, dd, df ? ddf : nullptr,
-                     dl ? ddl : nullptr, nullptr);
+    launch_disp_gemv(c->stream, c->O, R, c->o_n, c->nrect + c->ntri, c->model == 1, dy, dd,
+                     df ? ddf : nullptr, dl ? ddl : nullptr, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(d, dd, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
     if (df) HIPCHK(hipMemcpyAsync(df, ddf, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
@@ -1136,8 +1163,8 @@ static int capture_attempt(ucy_ctx *c)
     launch_controller(c->stream, b, c->d_ctl);
     if (b.n_obs > 0) launch_obs_log(c->stream, b, c->d_ctl);
     if (b.disp)
-        launch_disp_gemv(c->stream, b.O, b.o_rows, b.o_n, d.np, b.y, b.disp, b.disp_f, b.disp_l,
-                         c->d_ctl);
+        launch_disp_gemv(c->stream, b.O, b.o_rows, b.o_n, d.np, d.ap, b.y, b.disp, b.disp_f,
+                         b.disp_l, c->d_ctl);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);

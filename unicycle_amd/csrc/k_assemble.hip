@@ -166,6 +166,35 @@ ap_rows_kernel(const double *__restrict__ src, int ns, const Receiver *__restric
     if (out) out[s] = ts;
 }
 
+// buildO of the 2-D antiplane model (fortran/2d/antiplane/src/greens.f90:275-383):
+// displacement at point k (x2, x3) from unit slip on segment s,
+// computeDisplacementAntiplane (antiplane.f90:178-191); row k, column s
+__global__ void __launch_bounds__(256)
+ap_disp_kernel(const double *__restrict__ src, int ns, const double *__restrict__ pts,
+               double *__restrict__ O, int64_t ldo)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (s >= ns) return;
+    const double x2 = pts[3 * k + 1], x3 = pts[3 * k + 2];
+    const double q2 = src[3 * s], q3 = src[3 * s + 1], W = src[3 * s + 2];
+    const double pi = 3.141592653589793;
+    const double r = x2 - q2;
+    O[(int64_t)k * ldo + s] = 1.0 * (+atan((x3 - q3) / r) - atan((x3 + q3) / r)
+                                     - atan((x3 - q3 - W) / r) + atan((x3 + q3 + W) / r))
+                              / 2.0 / pi;
+}
+
+void launch_ap_disp(hipStream_t st, const double *src, int ns, const double *pts, int npts,
+                    double *O, int64_t ldo)
+{
+    for (int k0 = 0; k0 < npts && ns > 0; k0 += 65535) {
+        const int nk = std::min(65535, npts - k0);
+        hipLaunchKernelGGL(ap_disp_kernel, dim3((ns + 255) / 256, nk), dim3(256), 0, st, src, ns,
+                           pts + 3 * (int64_t)k0, O + (int64_t)k0 * ldo, ldo);
+    }
+}
+
 void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *rcv, int nr,
                     double G, double *K, int64_t ldk, double *out)
 {

@@ -739,7 +739,7 @@ __global__ void obs_log_kernel(RkBuffers b, Control *ctl)
 // as buildO splits O into Of and Ol (greens.f90:388-445): fault columns and
 // the volumes' third component go to df, the volumes' first two to dl.
 __global__ void __launch_bounds__(256)
-disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np,
+disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, int ap,
                  const double *__restrict__ y, double *d, double *df, double *dl,
                  const Control *ctl)
 {
@@ -751,13 +751,17 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np,
         off = row * rows;
     }
     const int64_t r = blockIdx.x;
-    const bool vertical = (r % 3) == 2;
+    const bool vertical = !ap && (r % 3) == 2;
     const double *Or = O + r * n;
-    const int64_t nf = 2 * (int64_t)np, vbase = 7 * (int64_t)np;
+    // the 2-D antiplane model: one slip per patch (state entry 0 of 4), one
+    // displacement component per point, no volumes
+    const int64_t nf = ap ? n : 2 * (int64_t)np, vbase = 7 * (int64_t)np;
     double a = 0.0, af = 0.0, al = 0.0;
     for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         double u;
-        if (j < nf) {
+        if (ap) {
+            u = y[4 * j];
+        } else if (j < nf) {
             u = y[7 * (j >> 1) + (j & 1)];
         } else {
             const int64_t q = (j - nf) / 6;
@@ -795,12 +799,12 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np,
     }
 }
 
-void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np,
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int ap,
                       const double *y, double *d, double *df, double *dl, const Control *ctl)
 {
     if (rows <= 0) return;
-    hipLaunchKernelGGL(disp_gemv_kernel, dim3((unsigned)rows), dim3(256), 0, st, O, rows, n, np, y,
-                       d, df, dl, ctl);
+    hipLaunchKernelGGL(disp_gemv_kernel, dim3((unsigned)rows), dim3(256), 0, st, O, rows, n, np,
+                       ap, y, d, df, dl, ctl);
 }
 
 void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl)

@@ -75,6 +75,9 @@ void launch_okada_disp(hipStream_t st, const OkadaMedium &M, const OkadaSource *
 // src[3s..3s+2] = (q2, q3, W); out (one receiver) may replace K
 void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *rcv, int nr,
                     double G, double *K, int64_t ldk, double *out);
+// antiplane buildO: row k = point k (x(3,npts), x2 and x3 used), column s
+void launch_ap_disp(hipStream_t st, const double *src, int ns, const double *pts, int npts,
+                    double *O, int64_t ldo);
 
 // ---- GEMV / RHS launchers (k_ode.hip) ----
 void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t);
@@ -128,10 +131,10 @@ void launch_rhs_finish(hipStream_t st, const PlainDims &d, const KOperand &K, co
 void launch_controller(hipStream_t st, const RkBuffers &b, Control *ctl);
 void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl);
 // observation-point displacement d = O u with u the slip (2 per patch) and
-// strain (6 per volume) entries of y; df, dl (may be null): the parts of buildO's
+// strain (6 per volume) entries of y (ap: the slip, entry 0 of 4 per patch); df, dl (may be null): the parts of buildO's
 // Of and Ol.  With ctl: written at log row log_count - 1 of the arrays, only in
 // the attempt that accepted a step.
-void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np,
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int ap,
                       const double *y, double *d, double *df, double *dl, const Control *ctl);
 // standalone Cash-Karp combination kernels for the ucy_rkck entry point
 void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,

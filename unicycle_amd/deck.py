@@ -414,6 +414,111 @@ def format_rectangles(wdir, lambda_, mu, interval, x, length, width, strike, dip
     return "\n".join(out) + "\n"
 
 
+@dataclass
+class AntiplaneDeck:
+    """The input deck of fortran/2d/antiplane/src/ratestate.f90 (init,
+    958-1230): shear modulus, interval, vertical segments (Vpl, x2, x3,
+    width, dip) with friction, observed patches, observation points (name,
+    x2, x3)."""
+    wdir: str
+    mu: float
+    interval: float
+    x: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    width: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    dip: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    friction: dict = field(default_factory=dict)
+    observation_states: list = field(default_factory=list)
+    observation_points: list = field(default_factory=list)   # (name, x1 = 0, x2, x3)
+
+    @property
+    def n(self) -> int:
+        return self.width.size
+
+    def friction_arrays(self):
+        """Engine.set_friction order (the rake is unused in 2-D)."""
+        keys = ("Vpl", "rake", "mu0", "sig", "a", "b", "L", "Vo", "damping")
+        fr = dict(self.friction, rake=[0.0] * self.n)
+        return tuple(np.asarray(fr[k], dtype=np.float64) for k in keys)
+
+
+def parse_antiplane(text: str) -> AntiplaneDeck:
+    """Parse a 2-D antiplane deck; raises DeckError where the reference stops
+    (fortran/2d/antiplane/src/ratestate.f90:1003-1230)."""
+    L = _Lines(text)
+    wdir = L.next("output directory").strip()
+    mu = _real(L.values("shear modulus", 1)[0])
+    if 0 > mu:
+        raise DeckError("shear modulus must be positive")
+    interval = _real(L.values("time interval", 1)[0])
+    if interval <= 0.0:
+        raise DeckError("simulation time must be positive")
+    d = AntiplaneDeck(wdir=wdir, mu=mu, interval=interval)
+    ns = _int(L.values("number of patches", 1)[0])
+    fr = {k: [] for k in ("Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping")}
+    if ns > 0:
+        rows = []
+        for k in range(1, ns + 1):
+            t = L.values(f"patch {k}", 6)
+            if _int(t[0]) != k:
+                raise DeckError("invalid patch definition: unexpected index")
+            v = [_real(x) for x in t[1:6]]
+            if v[3] <= 0.0:
+                raise DeckError("patch width must be positive")
+            rows.append(v)
+        r = np.array(rows)
+        d.x = np.stack([np.zeros(ns), r[:, 1], r[:, 2]], 1)
+        d.width, d.dip = r[:, 3].copy(), r[:, 4] * DEG2RAD
+        fr["Vpl"] = list(r[:, 0])
+        dummy = _int(L.values("number of frictional patches", 1)[0])
+        if dummy != ns:
+            raise DeckError("all patches require frictional properties")
+        for k in range(1, ns + 1):
+            t = L.values(f"friction row {k}", 8)
+            if _int(t[0]) != k:
+                raise DeckError("invalid friction property definition: unexpected index")
+            for key, tok in zip(("mu0", "sig", "a", "b", "L", "Vo", "damping"), t[1:8]):
+                fr[key].append(_real(tok))
+    d.friction = fr
+    d.observation_states = _indices(L, "observation patch")
+    npnt = _int(L.values("number of observation points", 1)[0])
+    for k in range(1, npnt + 1):
+        t = L.values(f"observation point {k}", 4)
+        if _int(t[0]) != k:
+            raise DeckError("observation points: unexpected index")
+        # the along-strike coordinate is zero in 2-D (ratestate.f90:1193-1194)
+        d.observation_points.append((t[1][:10], 0.0, _real(t[2]), _real(t[3])))
+    # ratestate.f90:1211-1217 tests .AND.: a deck without patches but with a
+    # positive interval is not refused there; it has nothing to integrate here
+    if d.n == 0:
+        raise DeckError("nothing to do")
+    for j in d.observation_states:
+        if not 1 <= j <= d.n:
+            raise DeckError(f"observation patch {j} is not an element")
+    return d
+
+
+def format_antiplane(wdir, model, interval, observe=(), points=()) -> str:
+    """A 2-D antiplane deck of an AntiplaneModel (configs) in the layout
+    parse_antiplane() reads."""
+    R2D = 1.0 / DEG2RAD
+    f = lambda v: repr(float(v))  # noqa: E731
+    out = ["# output directory", str(wdir), "# shear modulus", f(model.mu), "# time interval",
+           f(interval), "# number of patches", str(model.n), "# n Vpl x2 x3 width dip"]
+    for k in range(model.n):
+        out.append(" ".join([str(k + 1)] + [f(v) for v in (
+            model.Vpl[k], model.x[k][1], model.x[k][2], model.width[k], model.dip[k] * R2D)]))
+    out += ["# number of frictional patches", str(model.n)]
+    for k in range(model.n):
+        out.append(" ".join([str(k + 1)] + [f(v) for v in (
+            model.mu0[k], model.sig[k], model.a[k], model.b[k], model.L[k], model.Vo[k],
+            model.damping[k])]))
+    out += ["# number of observation patches", str(len(observe))]
+    out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
+    out += ["# number of observation points", str(len(points))]
+    out += [f"{k + 1} {n} {f(x2)} {f(x3)}" for k, (n, x2, x3) in enumerate(points)]
+    return "\n".join(out) + "\n"
+
+
 def format_viscous(wdir, lambda_, mu, interval, fault=None, vol=None, maxwell=True,
                    thermal=True, observe=(), observe_vol=(), points=(), events=1) -> str:
     """An input deck of viscouscycles.f90 in the layout parse_viscous() reads:

@@ -22,6 +22,10 @@ of iterations.  It writes what the reference writes each iteration
                                 (exportGeometry, ratestate.f90:374-399);
   stdout                        every 50 iterations: iteration, time, step, vMax.
 
+A 2-D antiplane deck (unicycle_amd.antiplane, fortran/2d/antiplane/src/
+ratestate.f90:254-276) runs the same loop with 4 state entries per patch,
+one displacement column in opts-<name>.dat and no VTP file.
+
 A viscouscycles deck (unicycle_amd.viscouscycles, viscouscycles.f90:413-434)
 runs the same loop with strain volumes and adds: volume-%08d.dat per observed
 volume (12 state entries and their derivatives), the fault and flow parts of
@@ -44,10 +48,17 @@ DPATCH = 7    # state entries per patch (STATE_VECTOR_DGF_PATCH of macros.f90)
 DVOLUME = 12  # state entries per strain volume (STATE_VECTOR_DGF_VOLUME)
 
 
-def setup(d: deckmod.Deck, device: int = 0, precision: int = PREC_F64) -> Engine:
+def setup(d, device: int = 0, precision: int = PREC_F64) -> Engine:
     """Mesh, friction (and for a viscouscycles deck the volumes and their
     rheology), buildG and buildO on the GPU."""
     eng = Engine(device)
+    if isinstance(d, deckmod.AntiplaneDeck):   # fortran/2d/antiplane
+        eng.set_antiplane(d.mu, d.x, d.width, d.dip)
+        eng.set_friction(*d.friction_arrays())
+        eng.buildG(precision)
+        if d.observation_points:
+            eng.buildO([p[1:] for p in d.observation_points])
+        return eng
     eng.set_elastic(d.lambda_, d.mu)
     if d.n_rect:
         eng.set_rectangles(d.rect_x, d.rect_length, d.rect_width, d.rect_strike, d.rect_dip)
@@ -98,10 +109,13 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     (iterations, final time, final state)."""
     fmt = deckmod.es_field
     viscous = isinstance(d, deckmod.ViscousDeck)
+    ap = isinstance(d, deckmod.AntiplaneDeck)
+    dpatch = 4 if ap else DPATCH   # STATE_VECTOR_DGF_PATCH of the 2-D macros.f90: 4
     eng = setup(d, device, precision)
-    # initStateVector: ratestate.f90:548-610, viscouscycles.f90:710-820
+    # initStateVector: ratestate.f90:548-610, viscouscycles.f90:710-820, 2-D
+    # antiplane ratestate.f90:568-605
     y = eng.initStateVector(1 if viscous else 0)
-    npatch = d.n_rect + d.n_tri
+    npatch = d.n if ap else d.n_rect + d.n_tri
     nvol = d.n_vol if viscous else 0
     if nvol and not d.n_maxwell:
         # no Maxwell rows: the reference's initial stress reads unset
@@ -110,16 +124,19 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
             y[DPATCH * npatch + DVOLUME * v + 6:DPATCH * npatch + DVOLUME * (v + 1)] = 0.0
     time, dt_next, dt_done = 0.0, 1.0e-3, 0.0     # ratestate.f90:166-167
     obs_vol = d.observation_volumes if viscous else []
-    idx = [DPATCH * (j - 1) + k for j in d.observation_states for k in range(DPATCH)]
+    idx = [dpatch * (j - 1) + k for j in d.observation_states for k in range(dpatch)]
     idx += [DPATCH * npatch + DVOLUME * (j - 1) + k for j in obs_vol for k in range(DVOLUME)]
+    ncol = 1 if ap else 3   # displacement components per point
     idx = np.array(idx, dtype=np.int64)
     files, pfiles = [], []
     if write_files:
         os.makedirs(d.wdir, exist_ok=True)
-        # exportGeometry (ratestate.f90:374-399, viscouscycles.f90:473-515)
-        with open(os.path.join(d.wdir, "patch-rectangular.vtp"), "w") as f:
-            f.write(vtp.rectangles_vtp(d.rect_x, d.rect_length, d.rect_width, d.rect_strike,
-                                       d.rect_dip, rect_friction(d)))
+        # exportGeometry (ratestate.f90:374-399, viscouscycles.f90:473-515;
+        # the 2-D program writes none)
+        if not ap:
+            with open(os.path.join(d.wdir, "patch-rectangular.vtp"), "w") as f:
+                f.write(vtp.rectangles_vtp(d.rect_x, d.rect_length, d.rect_width, d.rect_strike,
+                                           d.rect_dip, rect_friction(d)))
         if viscous:
             with open(os.path.join(d.wdir, "volume.vtp"), "w") as f:
                 f.write(vtp.volumes_vtp(d.vol_x, d.vol_length, d.vol_thickness, d.vol_width,
@@ -129,7 +146,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
         ftime.write(fmt(0.0, 19, 12, 2) + fmt(dt_next, 19, 12, 2) + "\n")
         # observed elements, patches then volumes (ratestate.f90:273-288,
         # viscouscycles.f90:370-392)
-        files = [(open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w"), DPATCH)
+        files = [(open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w"), dpatch)
                  for j in d.observation_states]
         files += [(open(os.path.join(d.wdir, f"volume-{j:08d}.dat"), "w"), DVOLUME)
                   for j in obs_vol]
@@ -161,9 +178,10 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
                                 + "".join(fmt(v, 20, 12, 3) for v in ys)
                                 + "".join(fmt(v, 20, 12, 3) for v in ds) + "\n")
                     for j, f in enumerate(pfiles):
-                        cols = [res.disp[k, 3 * j:3 * j + 3]]
+                        c0, c1 = ncol * j, ncol * (j + 1)
+                        cols = [res.disp[k, c0:c1]]
                         if viscous:
-                            cols += [res.disp_f[k, 3 * j:3 * j + 3], res.disp_l[k, 3 * j:3 * j + 3]]
+                            cols += [res.disp_f[k, c0:c1], res.disp_l[k, c0:c1]]
                         f.write(fmt(t_start, 19, 12, 2)
                                 + "".join(fmt(v, 19, 12, 2) for c in cols for v in c) + "\n")
                 if i % 50 == 0:
@@ -190,12 +208,13 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     return it, time, y
 
 
-def main(argv=None, viscous: bool = False) -> int:
-    name = "viscouscycles" if viscous else "ratestate"
-    media = "viscoelastic media" if viscous else "media"
+def main(argv=None, viscous: bool = False, antiplane: bool = False) -> int:
+    name = "viscouscycles" if viscous else ("antiplane" if antiplane else "ratestate")
+    media = ("three-dimensional viscoelastic media" if viscous else
+             "antiplane strain" if antiplane else "three-dimensional media")
     ap = argparse.ArgumentParser(prog=f"unicycle_amd.{name}",
-                                 description="quasi-dynamic earthquake simulation in three-"
-                                             f"dimensional {media} (GPU)")
+                                 description=f"quasi-dynamic earthquake simulation in {media} "
+                                             "(GPU)")
     ap.add_argument("-e", "--epsilon", type=float, default=1.0e-6)
     ap.add_argument("-m", "--maximum-step", type=float, default=1.7976931348623158e308)
     ap.add_argument("-i", "--maximum-iterations", type=int, default=1000000)
@@ -210,19 +229,25 @@ def main(argv=None, viscous: bool = False) -> int:
         return 0
     text = open(a.deck).read() if a.deck else sys.stdin.read()
     try:
-        d = deckmod.parse_viscous(text) if viscous else deckmod.parse(text)
+        if antiplane:
+            d = deckmod.parse_antiplane(text)
+        else:
+            d = deckmod.parse_viscous(text) if viscous else deckmod.parse(text)
         if viscous and d.n_vol:
             rheology(d)
     except deckmod.DeckError as exc:
         print(f"input error: {exc}", file=sys.stderr)
         return 1
-    print(f"# {name.upper()}")
-    print(f"# quasi-dynamic earthquake simulation in three-dimensional {media}")
+    print("# RATESTATE" if antiplane else f"# {name.upper()}")
+    print(f"# quasi-dynamic earthquake simulation in {media}")
     print(f"# numerical accuracy: {deckmod.es_field(a.epsilon, 11, 4, 2)}")
     print(f"# maximum iterations: {a.maximum_iterations:11d}")
     print(f"# output directory\n  {d.wdir}")
-    print(f"# {d.n_rect} rectangular and {d.n_tri} triangular patches, "
-          f"{len(d.observation_states)} observed")
+    if antiplane:
+        print(f"# {d.n} patches, {len(d.observation_states)} observed")
+    else:
+        print(f"# {d.n_rect} rectangular and {d.n_tri} triangular patches, "
+              f"{len(d.observation_states)} observed")
     if viscous:
         print(f"# {d.n_vol} strain volumes, {len(d.observation_volumes)} observed")
     print(f"# {len(d.observation_points)} observation points")
