@@ -297,3 +297,30 @@ def test_antiplane_deck_round_trip():
     for got, want in zip(d.friction_arrays(), m.friction()):
         np.testing.assert_allclose(got, want, rtol=1e-15)
     assert d.observation_states == [1, 40] and d.observation_points == [("A", 0.0, 1e3, 0.0)]
+
+
+# ---------------------------------------------------------------------- 0-D --
+PDECK = """# fortran/0d deck
+out/p
+# shear modulus
+30e3
+# time interval
+3.15e8
+# Vpl radius
+1d-9 2000
+# mu0 sig a b L Vo G/(2Vs)
+0.6 100 0.01 0.014 0.01 1e-6 5
+"""
+
+
+def test_parse_point_deck():
+    d = deck.parse_point(PDECK)
+    assert (d.wdir, d.mu, d.interval, d.Vpl, d.radius) == ("out/p", 30e3, 3.15e8, 1e-9, 2000.0)
+    assert (d.mu0, d.sig, d.a, d.b, d.L, d.Vo, d.damping) == (0.6, 100, 0.01, 0.014, 0.01, 1e-6, 5)
+    from unicycle_amd import springslider
+    assert springslider.stiffness(d) == -7.0 * springslider.PI / 16.0 * 30e3 / 2000.0
+    for old, new, needle in (("30e3\n# time", "-1\n# time", "shear modulus"),
+                             ("3.15e8", "-1", "simulation time"),
+                             ("0.01 1e-6 5", "0.01 1e-6", "7 values")):
+        with pytest.raises(deck.DeckError, match=needle):
+            deck.parse_point(PDECK.replace(old, new, 1))

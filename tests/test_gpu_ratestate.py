@@ -225,3 +225,45 @@ def test_antiplane_deck_run_matches_oracle(tmp_path, engine_factory):
                                    atol=1e-9 * np.abs(want).max() + 1e-300)
     assert not os.path.exists(os.path.join(wdir, "patch-rectangular.vtp"))
     assert np.abs(y - yref).max() <= 1e-6 * np.abs(yref).max()
+
+
+def test_springslider_deck_run(tmp_path, engine_factory):
+    """fortran/0d deck through the driver: the step sequence of the oracle's
+    one-patch model with the circular-crack stiffness (0d ratestate.f90:
+    230-277 is the 2-D antiplane odefun with K = -7 pi mu / (16 r)), and the
+    slip against scipy's DOP853 on the scalar ODE through the first event."""
+    from scipy.integrate import solve_ivp
+    from unicycle_amd import springslider
+    wdir = str(tmp_path / "p")
+    text = (f"{wdir}\n30e3\n{30 * configs.YEAR!r}\n1e-9 2000\n"
+            "0.6 100 0.01 0.014 0.01 1e-6 5\n")
+    d = deck.parse_point(text)
+    log = io.StringIO()
+    it, time, y = springslider.run(d, out=log, chunk=300)
+    k = springslider.stiffness(d)
+    sysm = orc.APSystem([np.array([v]) for v in (d.Vpl, d.mu0, d.sig, d.a, d.b, d.L, d.Vo,
+                                                 d.damping)], np.array([[k]]))
+    y0 = sysm.init_state()
+    yref, st, tl, dl, vl, rc = sysm.integrate(y0, d.interval, max_steps=10**6, log_cap=10**6)
+    assert it == st.steps and time == pytest.approx(st.time, rel=1e-9)
+    assert vl.max() > 1e-3   # the first event is at about 23 years
+    tt = _read(os.path.join(wdir, "time.txt"), [19, 19])
+    np.testing.assert_allclose(tt[1:, 0], np.concatenate([[0.0], tl[:-1]]), rtol=1e-6, atol=0)
+    rows = _read(os.path.join(wdir, "patch.dat"), [19] + [21] * 8)
+    assert rows.shape == (it, 9)
+    np.testing.assert_array_equal(rows[:, 0], tt[1:, 0])
+    np.testing.assert_allclose(rows[0, 1:5], y0, rtol=1e-12)
+    LN10 = np.log(10.0)
+
+    def f(t, s):
+        V = 10.0 ** s[3]
+        dth = (10.0 ** (-s[2]) - V / d.L) / LN10
+        dk = k * (V - d.Vpl)
+        dv = (dk - d.b * d.sig * LN10 * dth) / (d.a * d.sig + d.damping * V) / LN10
+        return [V, dk - d.damping * V * LN10 * dv, dth, dv]
+
+    ref = solve_ivp(f, [0.0, time], y0, method="DOP853", rtol=1e-12, atol=1e-14)
+    # the global error of RK45 at epsilon 1e-6 over 30 years and an event
+    # (the DOP853 solution is six orders tighter): 3.6e-5 here
+    assert y[0] == pytest.approx(ref.y[0, -1], rel=1e-4)
+    assert np.abs(y - yref).max() <= 1e-6 * np.abs(yref).max()

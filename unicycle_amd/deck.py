@@ -519,6 +519,41 @@ def format_antiplane(wdir, model, interval, observe=(), points=()) -> str:
     return "\n".join(out) + "\n"
 
 
+@dataclass
+class PointDeck:
+    """The input deck of fortran/0d/src/ratestate.f90 (init, 292-476): one
+    circular patch (a spring-slider): output directory, shear modulus,
+    interval, (Vpl, radius), (mu0, sig, a, b, L, Vo, G/(2Vs))."""
+    wdir: str
+    mu: float
+    interval: float
+    Vpl: float
+    radius: float
+    mu0: float
+    sig: float
+    a: float
+    b: float
+    L: float
+    Vo: float
+    damping: float
+
+
+def parse_point(text: str) -> PointDeck:
+    """Parse a 0-D deck; raises DeckError where the reference stops
+    (fortran/0d/src/ratestate.f90:386-475)."""
+    L = _Lines(text)
+    wdir = L.next("output directory").strip()
+    mu = _real(L.values("shear modulus", 1)[0])
+    if 0 > mu:
+        raise DeckError("shear modulus must be positive")
+    interval = _real(L.values("time interval", 1)[0])
+    if interval <= 0.0:
+        raise DeckError("simulation time must be positive")
+    g = [_real(t) for t in L.values("Vpl and radius", 2)[:2]]
+    f = [_real(t) for t in L.values("friction properties", 7)[:7]]
+    return PointDeck(wdir, mu, interval, g[0], g[1], *f)
+
+
 def format_viscous(wdir, lambda_, mu, interval, fault=None, vol=None, maxwell=True,
                    thermal=True, observe=(), observe_vol=(), points=(), events=1) -> str:
     """An input deck of viscouscycles.f90 in the layout parse_viscous() reads:
