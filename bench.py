@@ -103,6 +103,20 @@ def host_cores() -> int:
     return max(1, min(n, 16))
 
 
+def host_info() -> dict:
+    """The host the CPU baseline ran on: CPU model and the cores visible."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "cores_visible": os.cpu_count(), "cores_used": host_cores()}
+
+
 def _progress(tag):
     def f(st):
         log(f"[{tag}] attempts {st.attempts} steps {st.steps} t_min {st.min_time:.4g} s "
@@ -493,7 +507,8 @@ def cpu_baseline_ensemble(cpu_inputs, att64, sim_years, budget_s=20.0):
         log(f"cpu group {gi}: {rate:.2f} attempts/s")
     return {"value": sim_years / total, "unit": "sim-yr/s", "cores": cores, "kind": "port",
             "sample": "instance 0 of each group from its initial state, " + "; ".join(samples)
-                      + "; costed with every instance's GPU attempt count"}
+                      + "; costed with every instance's GPU attempt count",
+            "host": host_info()}
 
 
 def bench_plain(args):
