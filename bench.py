@@ -19,18 +19,21 @@ Reported:
   value         sum over instances of the simulated time reached per wall
                 second, for the fastest batched variant that meets the
                 binary64 bar on this run (slip error <= 1e-6 at a fixed
-                pre-event time, first-event times within 0.1 %): split bf16
-                with learned step-size proposals, split bf16, mixed, or
-                binary64 (which meets it by construction).
+                pre-event time, first-event times within 0.1 %): bf16+i8
+                with or without learned step-size proposals, split bf16,
+                mixed, or binary64 (which meets it by construction).
   batched_f64   binary64 K on v_mfma_f64_16x16x4f64; per-instance arithmetic
                 and step decisions are those of the plain path.
   plain_fp64    the single-instance device path (streaming binary64 GEMV per
                 instance), measured per attempt on instance 0 of every group
                 and costed over every instance's attempt count of the run.
   vs_plain_fp64 value / plain_fp64 on the same work.
-  mixed, split_bf16, split_bf16_learned
-                binary64 step-start matvec (mixed) or split bf16 throughout,
-                split-bf16 stage increments; the last with the step-size
+  mixed, split_bf16, bf16_i8, bf16_i8_learned
+                binary64 step-start matvec (mixed) or the reduced K
+                throughout, reduced stage increments: split bf16 (4 bytes per
+                K entry) or bf16+i8 (3 bytes: bf16 hi and an 8-bit code of
+                the residual, decoded in registers); the last with the
+                step-size
                 network (trained on instances drawn apart from the bench's:
                 plain binary64 histories, closed-loop rounds, output offset):
                 throughput on the same attempt budget, slip error against
@@ -39,7 +42,8 @@ Reported:
                 instance's first event).
   roofline      the stage GEMM of the headline variant (dominant kernel),
                 timed with HIP events on the library stream: HBM-bound for
-                split bf16, MFMA-bound for binary64.
+                the reduced formats, MFMA-bound for binary64; traffic from
+                the FETCH_SIZE / WRITE_SIZE counters of the same GEMM.
   learned_step_proposal
                 the 8-64-64-1 step-size network trained on the plain binary64
                 integrator's step histories of 16 instances of the 1024-patch
@@ -81,8 +85,9 @@ FP64_MFMA_PEAK = 78.6e12   # FLOP/s, dense binary64 MFMA
 INF_T = 1e30               # no end time: runs are bounded by attempts
 # HBM bytes per launch of the dominant kernels from rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE runs, scaled by a 2 GiB copy (profiles/copy_calibration_2GiB.md,
-# profiles/stage_gemm_split_bf16_ensemble256.md); same workload as this bench
-MEASURED_TRAFFIC = {"split_bf16_stage_gemm": 2.269e9}
+# profiles/stage_gemm_split_bf16_ensemble256.md,
+# profiles/stage_gemm_bf16_i8_ensemble256.md); same workload as this bench
+MEASURED_TRAFFIC = {"split_bf16": 2.269e9, "bf16_i8": 1.733e9}
 # FP64 flop per (receiver, source) pair of the assembly kernels from
 # rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / pairs on the same
 # models (profiles/assembly_greens_functions.md)
@@ -217,7 +222,7 @@ def train_step_net(eng, args):
 
 
 def bench_ensemble(args):
-    from unicycle_amd import Engine, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, configs
+    from unicycle_amd import Engine, PREC_BF16_I8, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, configs
     R = args.attempts
     eng = Engine(0)
     t0 = time.perf_counter()
@@ -250,11 +255,12 @@ def bench_ensemble(args):
     ens.close()
 
     reduced = {}
-    variants = [("mixed", PREC_MIXED, None), ("split_bf16", PREC_SPLIT_BF16, None)]
+    variants = [("mixed", PREC_MIXED, None), ("split_bf16", PREC_SPLIT_BF16, None),
+                ("bf16_i8", PREC_BF16_I8, None)]
     net = None
     if not args.no_learned:
         net, net_info = train_step_net(eng, args)
-        variants.append(("split_bf16_learned", PREC_SPLIT_BF16, net))
+        variants.append(("bf16_i8_learned", PREC_BF16_I8, net))
     for name, prec, mlp in variants:
         ensr, _ = configs.ensemble_setup(eng, prec)
         if mlp is not None:
@@ -282,10 +288,12 @@ def bench_ensemble(args):
             "rejected_fraction": 1.0 - str_.steps / max(1, str_.attempts)}
     reduced["mixed"]["dtype"] = "binary64 K for t_0 = K v_0, split-bf16 increments, f64 state"
     reduced["split_bf16"]["dtype"] = "split-bf16 K (t_0 and increments), f64 state"
+    reduced["bf16_i8"]["dtype"] = ("bf16+i8 K (3 bytes per entry; t_0 and increments), "
+                                   "split-bf16 V, f64 state")
     if net is not None:
-        reduced["split_bf16_learned"]["dtype"] = ("split-bf16 K (t_0 and increments), f64 state, "
-                                                  "learned step-size proposals")
-        reduced["split_bf16_learned"]["training"] = net_info
+        reduced["bf16_i8_learned"]["dtype"] = reduced["bf16_i8"]["dtype"] + \
+            ", learned step-size proposals"
+        reduced["bf16_i8_learned"]["training"] = net_info
         net.close()
 
     # plain binary64 device path: instance 0 of each group, R attempts
@@ -327,8 +335,10 @@ def bench_ensemble(args):
             ev.get("rel_err_max", 0.0) <= 1e-3
         r["meets_bar"] = bool(ok)
         if ok and r["value"] > head["value"]:
+            fmt = "bf16_i8" if name.startswith("bf16_i8") else "split_bf16"
+            kdesc = "bf16+i8 K (3 B/entry)" if fmt == "bf16_i8" else "split-bf16 K"
             head = {"name": name, "value": r["value"], "ms_per_step": 1e3 * r["wall_s"],
-                    "dtype": "f64 state, split-bf16 K (fp32 MFMA accumulate)"
+                    "dtype": f"f64 state, {kdesc} (fp32 MFMA accumulate)"
                              + (" + binary64 step-start matvec" if name == "mixed" else "")
                              + (" + learned step proposals (fp32 MLP)" if "learned" in name
                                 else ""),
@@ -336,8 +346,9 @@ def bench_ensemble(args):
             gb = r["stage_gemm_GBps"]
             roof = {"bound": "hbm", "achieved": gb, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": gb * 1e9 / HBM_PEAK,
-                    "traffic": MEASURED_TRAFFIC["split_bf16_stage_gemm"],
-                    "kernel": "ucy::gemm_bf16_kernel<variant> (split-bf16 MFMA stage GEMM, all groups)",
+                    "traffic": MEASURED_TRAFFIC[fmt],
+                    "kernel": f"ucy::gemm_bf16_kernel<variant, {str(fmt == 'bf16_i8').lower()}> "
+                              f"({kdesc} MFMA stage GEMM, all groups)",
                     "avg_launch_us": r["stage_gemm_us"],
                     "bytes_per_launch": r["stage_gemm_GBps"] * 1e9 * r["stage_gemm_us"] * 1e-6,
                     "useful_TFLOPs": r["stage_gemm_useful_TFLOPs"]}
@@ -674,8 +685,9 @@ def bench_gemm(args):
     """The all-groups stage GEMM of one precision alone, launched directly
     (no graph): autotuning launches, then 20 timed launches.  For profiler
     and counter runs."""
-    from unicycle_amd import Engine, PREC_F64, PREC_SPLIT_BF16, configs
-    prec = {"f64": PREC_F64, "split_bf16": PREC_SPLIT_BF16}[args.precision]
+    from unicycle_amd import Engine, PREC_BF16_I8, PREC_F64, PREC_SPLIT_BF16, configs
+    prec = {"f64": PREC_F64, "split_bf16": PREC_SPLIT_BF16,
+            "bf16_i8": PREC_BF16_I8}[args.precision]
     eng = Engine(0)
     ens, info = configs.ensemble_setup(eng, prec)
     t, b = ens.time_gemm(20)
@@ -732,7 +744,7 @@ def main():
                     help="horizon of the first-event comparison")
     ap.add_argument("--workload", default="ensemble",
                     choices=["ensemble", "plain", "gemm", "rhs", "copy", "assembly"])
-    ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16"],
+    ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16", "bf16_i8"],
                     help="stage-GEMM precision of --workload gemm")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")

@@ -46,6 +46,11 @@ extern "C" {
 /* ensembles only: binary64 K for the step-start matvec t_0 = K v_0 and the
  * split-bf16 pair for the stage increments t_i = t_0 + K (v_i - v_0) */
 #define UCY_PREC_MIXED 2
+/* ensembles only: as UCY_PREC_SPLIT_BF16 with K stored in 3 bytes per entry,
+ * the bf16 hi and an 8-bit code of the residual in steps of 1/128 of hi's
+ * ulp (about 2^-16 relative), decoded to the bf16 lo operand in registers;
+ * 25 % fewer stage-GEMM bytes */
+#define UCY_PREC_BF16_I8 3
 
 typedef struct ucy_ctx ucy_ctx;
 

@@ -30,7 +30,7 @@ struct HostGroup {
 
 struct ucy_ensemble {
     ucy_ctx *ctx = nullptr;
-    int mode = 0;  // 0 split-bf16, 1 binary64
+    int mode = 0;  // 0 split-bf16, 1 binary64, 2 mixed, 3 bf16+i8
     std::vector<HostGroup> hg;
     EnsTable tab{};
     EnsTable *d_tab = nullptr;
@@ -84,11 +84,14 @@ extern "C" ucy_ensemble *ucy_ensemble_create(ucy_ctx *ctx, int precision)
 {
     if (!ctx) return nullptr;
     if (precision != UCY_PREC_SPLIT_BF16 && precision != UCY_PREC_F64 &&
-        precision != UCY_PREC_MIXED)
+        precision != UCY_PREC_MIXED && precision != UCY_PREC_BF16_I8)
         return nullptr;
     ucy_ensemble *E = new ucy_ensemble();
     E->ctx = ctx;
-    E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0 : precision == UCY_PREC_F64 ? 1 : 2;
+    E->mode = precision == UCY_PREC_SPLIT_BF16 ? 0
+              : precision == UCY_PREC_F64      ? 1
+              : precision == UCY_PREC_MIXED    ? 2
+                                               : 3;
     ctx->ensembles.push_back(E);
     return E;
 }
@@ -284,13 +287,19 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             EHIP(hipFree(dR));
             EHIP(hipFree(dS));
         }
-        // stage-GEMM bytes per launch: split bf16 for modes 0 and 2
+        // stage-GEMM bytes per launch: split bf16 (4 bytes per entry) for
+        // modes 0 and 2, bf16+i8 (3 bytes) for mode 3
         if (G.mode != 1) {
+            const double kbytes = G.mode == 3 ? 3.0 : 4.0;
             uint16_t *tiles = nullptr;
             double *rs = nullptr;
-            if (ealloc(E, &tiles, (size_t)G.mp * G.np * 2)) return UCY_ERR_HIP;
+            if (ealloc(E, &tiles, (size_t)G.mp * G.np * (G.mode == 3 ? 3 : 4) / 2))
+                return UCY_ERR_HIP;
             if (ealloc(E, &rs, G.mp)) return UCY_ERR_HIP;
-            ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
+            if (G.mode == 3)
+                ens_launch_split3_K(c->stream, K64, G.m, G.n, G.mp, G.np, (uint8_t *)tiles, rs);
+            else
+                ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
             EHIP(hipGetLastError());
             G.Ktiles = tiles;
             G.rscale = rs;
@@ -298,9 +307,9 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             if (ealloc(E, &G.Tpart, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.v0, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.t0, (size_t)G.mp * kEnsCols)) return UCY_ERR_HIP;
-            bytes += 4.0 * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
+            bytes += kbytes * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
         }
-        if (G.mode != 0) {
+        if (G.mode == 1 || G.mode == 2) {
             double *tiles = nullptr;
             if (ealloc(E, &tiles, (size_t)G.mp * G.np)) return UCY_ERR_HIP;
             ens_launch_tile_K64(c->stream, K64, G.m, G.n, G.mp, G.np, tiles);
@@ -400,14 +409,15 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
                 continue;
             }
             if (upload_table(E)) return UCY_ERR_HIP;
-            ens_launch_gemm(c->stream, E->d_tab, T, 2);
+            // 2 warm launches, 15 timed (single launches vary by ~10 %)
+            for (int r = 0; r < 2; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
             EHIP(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 5; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
+            for (int r = 0; r < 15; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
             EHIP(hipEventRecord(e1, c->stream));
             EHIP(hipEventSynchronize(e1));
             float ms = 0;
             EHIP(hipEventElapsedTime(&ms, e0, e1));
-            const double t = 1e-3 * ms / 5;
+            const double t = 1e-3 * ms / 15;
             E->tune_seconds.push_back(t);
             if (t < best_t) {
                 best_t = t;

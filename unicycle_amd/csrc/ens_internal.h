@@ -33,8 +33,9 @@ struct EnsGroup {
     int nkb;          // np / 16
     int ksplit;       // split-K factor of the GEMM
     int kb_per_split; // k-blocks per split
-    int mode;         // 0 split-bf16, 1 binary64, 2 mixed (see below)
-    const uint16_t *Ktiles;  // split-bf16 tiles (modes 0, 2)
+    int mode;         // 0 split-bf16, 1 binary64, 2 mixed (see below), 3 bf16+i8
+                      // (as 0 with the 3-byte K tiles)
+    const uint16_t *Ktiles;  // split-bf16 tiles (modes 0, 2), bf16+i8 tile bytes (mode 3)
     const double *rscale;    // row scales (modes 0, 2)
     const double *K64;       // binary64 K tiles (modes 1, 2)
     uint16_t *Vtiles;        // V split-bf16 tiles (modes 0, 2)
@@ -93,6 +94,8 @@ int ens_gemm_variant_lds(int variant);
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,
                         double *rscale);
+void ens_launch_split3_K(hipStream_t st, const double *K, int m, int n, int mp, int np,
+                         uint8_t *tiles, double *rscale);
 void ens_launch_tile_K64(hipStream_t st, const double *K, int m, int n, int mp, int np,
                          double *tiles);
 // stage 1 = the step-start matvec (skipped when no instance of a group starts

@@ -14,6 +14,11 @@
 //       offset  (((rb*nkb + kb)*2 + part)*64 + lane)*8 + j
 //   V tile (kb, cb, part)   k 16*kb.., columns 32*cb..
 //       offset  (((kb*2 + cb)*2 + part)*64 + lane)*8 + j
+// The bf16+i8 K (3 bytes per entry) keeps the hi tile and replaces the lo
+// tile by 8-bit codes relative to hi's exponent, decoded to the same bf16 lo
+// operand in registers (split3_kernel, decode_lo8):
+//   K tile (rb, kb)         1536 bytes at ((rb*nkb + kb)*1536):
+//       hi  bytes 16*lane + 2*j,   lo code  byte 1024 + 8*lane + j
 #include <hip/hip_runtime.h>
 #include "ens_internal.h"
 
@@ -22,6 +27,7 @@ namespace ucy {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 __constant__ const double kELg10 = 2.302585092994045684;
 
@@ -39,8 +45,10 @@ __device__ __forceinline__ uint16_t bf16_rne(double x, double &xr)
         const float f = (float)xr;
         return (uint16_t)(__float_as_uint(f) >> 16);
     }
+    // 45 dropped bits: add half their range less one, plus the kept lsb
+    // (ties to even), and truncate
     const unsigned long long lsb = (u >> 45) & 1ull;
-    u += 0x00001FFFFFFFFFFFull + lsb;
+    u += 0x00000FFFFFFFFFFFull + lsb;
     u &= ~0x00001FFFFFFFFFFFull;
     xr = __longlong_as_double((long long)u);
     const float f = (float)xr;  // exact: 8 significant bits, binary32 range
@@ -145,6 +153,83 @@ void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, i
                        mp, np, rscale, tiles);
 }
 
+// K -> bf16+i8 tiles with the same exact row scales.  For x = K / s, hi =
+// bf16(x) and the residual r = x - hi (exact in binary64) is coded in steps
+// of 2^(E-142), E the biased exponent of hi, i.e. 1/128 of hi's ulp:
+// q = rint(r / 2^(E-142)) lies in [-128, 128] (|r| <= ulp/2).  Where it
+// would be +128, hi moves one ulp up and the code is taken again (then in
+// [-128, -64]), so every entry keeps |x - (hi + q 2^(E-142))| <= 2^(E-143),
+// 2^-16 (1 + 2^-8) relative.  Entries with E < 32 (|x| < 2^-95, |x| <= 1 by
+// the row scale) are flushed to zero: their products lie far below the fp32
+// accumulation's resolution, and the flush keeps the decoder's scale
+// finite.
+__global__ void split3_kernel(const double *__restrict__ K, int m, int n, int mp, int np,
+                              const double *__restrict__ rscale, uint8_t *__restrict__ tiles)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)mp * np) return;
+    const int r = (int)(i / np), c = (int)(i % np);
+    const double x = (r < m && c < n) ? K[(size_t)r * n + c] / rscale[r] : 0.0;
+    double h;
+    uint16_t hi = bf16_rne(x, h);
+    int E = (hi >> 7) & 0xFF;
+    int q = 0;
+    if (E < 32) {
+        hi = 0;
+    } else {
+        double t = rint(ldexp(x - h, 142 - E));
+        if (t > 127.0) {
+            hi = bf16_rne(h + ldexp(1.0, E - 134), h);   // exact: the next bf16 up
+            E = (hi >> 7) & 0xFF;
+            t = rint(ldexp(x - h, 142 - E));
+        }
+        q = (int)fmin(fmax(t, -128.0), 127.0);
+    }
+    const int nkb = np >> 4;
+    const int rb = r >> 5, kb = c >> 4, w = c & 15;
+    const int lane = (r & 31) + 32 * (w >> 3), j = w & 7;
+    uint8_t *tile = tiles + ((size_t)rb * nkb + kb) * 1536;
+    reinterpret_cast<uint16_t *>(tile)[lane * 8 + j] = hi;
+    tile[1024 + lane * 8 + j] = (uint8_t)(int8_t)q;
+}
+
+void ens_launch_split3_K(hipStream_t st, const double *K, int m, int n, int mp, int np,
+                         uint8_t *tiles, double *rscale)
+{
+    hipLaunchKernelGGL(rowscale_kernel, dim3((mp + 3) / 4), dim3(256), 0, st, K, m, n, mp, rscale);
+    const long long tot = (long long)mp * np;
+    hipLaunchKernelGGL(split3_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, K, m,
+                       n, mp, np, rscale, tiles);
+}
+
+// bf16 lo operand of 8 bf16+i8 entries from their hi words (2 bf16 each, low
+// half first) and lo codes (one byte each): lo_j = q_j 2^(E_j - 142).  The
+// scale 2^(E-142) has the binary32 bits (E << 23) - (15 << 23), finite for
+// E = 0 (where q = 0) since the packer leaves no 0 < E < 32.  q_j 2^(E_j-142)
+// has at most 8 significant bits, so its top 16 bits are the exact bf16.
+// (Scaling hi by 2^15 instead would save the subtraction but keeps a second
+// copy of hi live: past 256 VGPRs, half the waves.)
+__device__ __forceinline__ bf16x8 decode_lo8(const u32x4 &h, const u32x2 &q)
+{
+    u32x4 out;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const unsigned H = h[i];
+        const unsigned Q = q[i >> 1] >> (16 * (i & 1));
+        const float s0 = __uint_as_float(((H << 16) & 0x7F800000u) - 0x07800000u);
+        const float s1 = __uint_as_float((H & 0x7F800000u) - 0x07800000u);
+        const float l0 = (float)(int)(signed char)(Q & 0xFFu) * s0;
+        const float l1 = (float)(int)(signed char)((Q >> 8) & 0xFFu) * s1;
+        // bytes 2, 3 of l0 then bytes 2, 3 of l1
+        out[i] = __builtin_amdgcn_perm(__float_as_uint(l1), __float_as_uint(l0), 0x07060302u);
+    }
+    return __builtin_bit_cast(bf16x8, out);
+}
+
+// bytes of one K tile (rb, kb): split bf16 hi + lo, or bf16+i8
+template <bool I8>
+constexpr size_t kTileBytes = I8 ? 1536 : 2048;
+
 // single-instance K -> packed split-bf16 pairs, row-major with the row
 // stride of K: word = hi << 16 | lo of K / rs[row]
 __global__ void __launch_bounds__(256) split_rowmajor_kernel(const double *__restrict__ K,
@@ -184,7 +269,7 @@ __device__ __forceinline__ int group_of(const int *prefix, int ng, int blk)
 // k-split.  RB = 4 halves the V-tile reads per K byte against RB = 2 at
 // twice the accumulators; the autotuner picks.
 // ---------------------------------------------------------------------------
-template <int RB, bool PIPE>
+template <int RB, bool PIPE, bool I8>
 __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int sp, int lane)
 {
     const int kb0 = sp * G.kb_per_split;
@@ -200,31 +285,41 @@ __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int s
     // tile (rb, kb): hi then lo, 64 lanes x 16 bytes each.  Pipelining is
     // by hand: loops holding MFMA builtins are not runtime-unrolled by the
     // compiler.
-    const u32x4 *A = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)rb0 * G.nkb * 2) * 64 + lane;
-    const size_t rstride = (size_t)G.nkb * 128;
+    const char *A = reinterpret_cast<const char *>(G.Ktiles) + (size_t)rb0 * G.nkb * kTileBytes<I8>;
+    const size_t rstride = (size_t)G.nkb * kTileBytes<I8>;
     const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
+    // per 32-row block: hi, and the bf16 lo (split) or the lo codes (I8)
     u32x4 ca[RB][2], cb[4], na[RB][2], nb[4];
-    auto load = [&](int kb, u32x4 (&ra)[RB][2], u32x4 (&rbv)[4]) {
+    u32x2 cq[RB], nq[RB];
+    auto load = [&](int kb, u32x4 (&ra)[RB][2], u32x2 (&rq)[RB], u32x4 (&rbv)[4]) {
 #pragma unroll
         for (int a = 0; a < RB; a++) {
-            ra[a][0] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128);
-            ra[a][1] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128 + 64);
+            const char *t = A + a * rstride + (size_t)kb * kTileBytes<I8>;
+            ra[a][0] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + lane);
+            if constexpr (I8)
+                rq[a] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+            else
+                ra[a][1] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) rbv[q] = Bv[(size_t)kb * 256 + 64 * q];
     };
-    if (kb0 < kb1) load(kb0, ca, cb);
+    if (kb0 < kb1) load(kb0, ca, cq, cb);
     for (int kb = kb0; kb < kb1; kb++) {
         // PIPE: the next k-block's loads go out before this one's MFMAs
         // (more bytes in flight per wave, more registers: fewer waves)
-        if (!PIPE && kb > kb0) load(kb, ca, cb);
-        if (PIPE && kb + 1 < kb1) load(kb + 1, na, nb);
+        if (!PIPE && kb > kb0) load(kb, ca, cq, cb);
+        if (PIPE && kb + 1 < kb1) load(kb + 1, na, nq, nb);
         const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, cb[0]), __builtin_bit_cast(bf16x8, cb[1])},
                                 {__builtin_bit_cast(bf16x8, cb[2]), __builtin_bit_cast(bf16x8, cb[3])}};
 #pragma unroll
         for (int a = 0; a < RB; a++) {
             const bf16x8 Ah = __builtin_bit_cast(bf16x8, ca[a][0]);
-            const bf16x8 Al = __builtin_bit_cast(bf16x8, ca[a][1]);
+            bf16x8 Al;
+            if constexpr (I8)
+                Al = decode_lo8(ca[a][0], cq[a]);
+            else
+                Al = __builtin_bit_cast(bf16x8, ca[a][1]);
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
@@ -236,7 +331,10 @@ __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int s
 #pragma unroll
             for (int a = 0; a < RB; a++) {
                 ca[a][0] = na[a][0];
-                ca[a][1] = na[a][1];
+                if constexpr (I8)
+                    cq[a] = nq[a];
+                else
+                    ca[a][1] = na[a][1];
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) cb[q] = nb[q];
@@ -264,14 +362,14 @@ __host__ __device__ __forceinline__ int bf16_row_items(int mp, int rb)
     return ((mp >> 5) + rb - 1) / rb;
 }
 
-template <int RB, bool PIPE>
+template <int RB, bool PIPE, bool I8>
 __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
 {
     const int nrb = G.mp >> 5;
     const int nri = bf16_row_items(G.mp, RB);
     const int rb0 = (item % nri) * RB, sp = item / nri;
-    if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE>(G, rb0, sp, lane);
-    else gemm_tile_bf16<2, PIPE>(G, rb0, sp, lane);
+    if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE, I8>(G, rb0, sp, lane);
+    else gemm_tile_bf16<2, PIPE, I8>(G, rb0, sp, lane);
 }
 
 // split-bf16 GEMM variants of the autotuner: K rows per wave, pipelined
@@ -290,7 +388,7 @@ constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
 // from L2 once per block instead of once per wave.  Waves past the last row
 // item only help with the staging.  The last row item of a group may hold
 // two 32-row blocks (mp is a multiple of 64).
-template <int RB, bool PIPE, int CK>
+template <int RB, bool PIPE, int CK, bool I8>
 __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, int sp, int wave,
                                                     int lane, u32x4 *lds)
 {
@@ -309,8 +407,8 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
-    const u32x4 *A = reinterpret_cast<const u32x4 *>(G.Ktiles) + ((size_t)(valid ? rb0 : 0) * G.nkb * 2) * 64 + lane;
-    const size_t rstride = (size_t)G.nkb * 128;
+    const char *A = reinterpret_cast<const char *>(G.Ktiles) + (size_t)(valid ? rb0 : 0) * G.nkb * kTileBytes<I8>;
+    const size_t rstride = (size_t)G.nkb * kTileBytes<I8>;
     const u32x4 *Vg = reinterpret_cast<const u32x4 *>(G.Vtiles);
     const u32x4 zero = {0u, 0u, 0u, 0u};
     // chunk 0 of V into buffer 0
@@ -321,15 +419,20 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
     }
     __syncthreads();
     u32x4 ca[RB][2], na[RB][2], stg[CK];
-    auto loadK = [&](int kb, u32x4 (&ra)[RB][2]) {
+    u32x2 cq[RB], nq[RB];
+    auto loadK = [&](int kb, u32x4 (&ra)[RB][2], u32x2 (&rq)[RB]) {
 #pragma unroll
         for (int a = 0; a < RB; a++)
             if (a < nvr) {
-                ra[a][0] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128);
-                ra[a][1] = __builtin_nontemporal_load(A + a * rstride + (size_t)kb * 128 + 64);
+                const char *t = A + a * rstride + (size_t)kb * kTileBytes<I8>;
+                ra[a][0] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + lane);
+                if constexpr (I8)
+                    rq[a] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+                else
+                    ra[a][1] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
             }
     };
-    if (valid) loadK(kb0, ca);
+    if (valid) loadK(kb0, ca, cq);
     int buf = 0;
     for (int kb = kb0; kb < kb1; kb++) {
         const int k = (kb - kb0) % CK;
@@ -341,8 +444,8 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
                 stg[u] = kbn < kb1 ? Vg[(size_t)kbn * 256 + (i & 255)] : zero;
             }
         }
-        if (PIPE && valid && kb + 1 < kb1) loadK(kb + 1, na);
-        if (!PIPE && valid && kb > kb0) loadK(kb, ca);
+        if (PIPE && valid && kb + 1 < kb1) loadK(kb + 1, na, nq);
+        if (!PIPE && valid && kb > kb0) loadK(kb, ca, cq);
         if (valid) {
             const u32x4 *Bl = lds + buf * (CK * 256) + k * 256 + lane;
             const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, Bl[0]), __builtin_bit_cast(bf16x8, Bl[64])},
@@ -351,7 +454,11 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
             for (int a = 0; a < RB; a++) {
                 if (a >= nvr) continue;
                 const bf16x8 Ah = __builtin_bit_cast(bf16x8, ca[a][0]);
-                const bf16x8 Al = __builtin_bit_cast(bf16x8, ca[a][1]);
+                bf16x8 Al;
+                if constexpr (I8)
+                    Al = decode_lo8(ca[a][0], cq[a]);
+                else
+                    Al = __builtin_bit_cast(bf16x8, ca[a][1]);
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
@@ -363,7 +470,10 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
 #pragma unroll
                 for (int a = 0; a < RB; a++) {
                     ca[a][0] = na[a][0];
-                    ca[a][1] = na[a][1];
+                    if constexpr (I8)
+                        cq[a] = nq[a];
+                    else
+                        ca[a][1] = na[a][1];
                 }
             }
         }
@@ -446,7 +556,7 @@ __host__ __device__ __forceinline__ bool gemm_f64_path(int mode, int stage)
 
 // stage 1 = the step-start matvec (skipped when every instance of the group
 // retries: t_0 stands), 2..6 the stage matvecs, 0 a standalone matvec
-template <int V>
+template <int V, bool I8>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restrict__ tab, int stage)
 {
     constexpr int RB = kBf16Variants[V].rb;
@@ -460,12 +570,13 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restri
         const int nrg = (bf16_row_items(G.mp, RB) + 3) / 4;
         const int local = blk - tab->blk_gemm[g];
         const int wave = threadIdx.x >> 6;
-        gemm_block_bf16_lds<RB, kBf16Variants[V].pipe, CK>(G, 4 * (local % nrg) + wave, local / nrg,
-                                                           wave, threadIdx.x & 63, lds);
+        gemm_block_bf16_lds<RB, kBf16Variants[V].pipe, CK, I8>(G, 4 * (local % nrg) + wave,
+                                                               local / nrg, wave, threadIdx.x & 63,
+                                                               lds);
     } else {
         const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
         if (item >= bf16_row_items(G.mp, RB) * G.ksplit) return;
-        gemm_wave_bf16<RB, kBf16Variants[V].pipe>(G, item, threadIdx.x & 63);
+        gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8>(G, item, threadIdx.x & 63);
     }
 }
 
@@ -490,26 +601,28 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(const EnsTable *__restric
     gemm_wave_f64(G, item, threadIdx.x & 63);
 }
 
+template <bool I8>
+static void launch_gemm_bf16(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
+{
+    const dim3 grid(htab.blk_gemm[htab.ngroups]);
+    switch (htab.variant) {
+    case 0: hipLaunchKernelGGL((gemm_bf16_kernel<0, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 1: hipLaunchKernelGGL((gemm_bf16_kernel<1, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 2: hipLaunchKernelGGL((gemm_bf16_kernel<2, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 3: hipLaunchKernelGGL((gemm_bf16_kernel<3, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<4, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    }
+}
+
 void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
 {
     if (gemm_f64_path(htab.g[0].mode, stage))
         hipLaunchKernelGGL(gemm_f64_kernel, dim3(htab.blk_gemm64[htab.ngroups]), dim3(256), 0, st,
                            dtab, stage);
-    else if (htab.variant == 0)
-        hipLaunchKernelGGL(gemm_bf16_kernel<0>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
-                           st, dtab, stage);
-    else if (htab.variant == 1)
-        hipLaunchKernelGGL(gemm_bf16_kernel<1>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
-                           st, dtab, stage);
-    else if (htab.variant == 2)
-        hipLaunchKernelGGL(gemm_bf16_kernel<2>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
-                           st, dtab, stage);
-    else if (htab.variant == 3)
-        hipLaunchKernelGGL(gemm_bf16_kernel<3>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
-                           st, dtab, stage);
+    else if (htab.g[0].mode == 3)
+        launch_gemm_bf16<true>(st, dtab, htab, stage);
     else
-        hipLaunchKernelGGL(gemm_bf16_kernel<4>, dim3(htab.blk_gemm[htab.ngroups]), dim3(256), 0,
-                           st, dtab, stage);
+        launch_gemm_bf16<false>(st, dtab, htab, stage);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,8 +643,8 @@ __device__ __forceinline__ void put_split(const EnsGroup &G, int c, int inst, do
 
 // pre-pass (ratestate.f90:683-702) of element e of instance inst at state X;
 // writes the slip rates to out[0..1] and the GEMM operand entries: binary64
-// v (mode 1); at the step start (base) v_0, kept, as split bf16 (mode 0) or
-// binary64 (mode 2); at a stage the split-bf16 increment v - v_0
+// v (mode 1); at the step start (base) v_0, kept, as split bf16 (modes 0, 3)
+// or binary64 (mode 2); at a stage the split-bf16 increment v - v_0
 __device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X, double *out,
                                               int e, int inst, bool base)
 {

@@ -21,6 +21,7 @@ from . import _lib
 PREC_F64 = 0
 PREC_SPLIT_BF16 = 1
 PREC_MIXED = 2        # ensembles: binary64 step-start matvec, split-bf16 increments
+PREC_BF16_I8 = 3      # ensembles: split bf16 with K in 3 bytes (bf16 hi + 8-bit lo code)
 
 
 class UcyError(RuntimeError):

@@ -8,7 +8,10 @@ Instances are grouped by geometry; a group shares one kernel K.  Precisions:
                    t_i = t_0 + K (v_i - v_0) over the step start, so the
                    embedded error estimate (whose weights sum to zero) does
                    not see the constant part of the rounding;
-  PREC_MIXED       as PREC_SPLIT_BF16 with t_0 = K v_0 in binary64.
+  PREC_MIXED       as PREC_SPLIT_BF16 with t_0 = K v_0 in binary64;
+  PREC_BF16_I8     as PREC_SPLIT_BF16 with K in 3 bytes per entry: the bf16
+                   hi and an 8-bit code of the residual in steps of 1/128 of
+                   hi's ulp, decoded to the bf16 lo operand in registers.
 State, time, step control and error norms stay binary64 per instance.
 """
 from __future__ import annotations
@@ -20,7 +23,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .engine import Engine, UcyError, PREC_F64, PREC_SPLIT_BF16, PREC_MIXED, _p  # noqa: F401
+from .engine import (Engine, UcyError, PREC_F64, PREC_SPLIT_BF16, PREC_MIXED,  # noqa: F401
+                     PREC_BF16_I8, _p)
 
 
 @dataclass
