@@ -821,13 +821,20 @@ __device__ __forceinline__ double ens_stage_value(int stage, double h, double y0
 }
 
 // stage input `stage` (2..6) of element e: y + h sum_j a_ij k_j
-// (ode45.f90:133-161) and its odefun pre-pass
+// (ode45.f90:133-161) and its odefun pre-pass.  Only the components the
+// right-hand side reads are formed: traction 2, 3 (rake), log10 theta 5 and
+// log10 V 6.  Slip (0, 1) and normal traction (4) of a stage input feed
+// nothing (the derivatives do not depend on them; the fifth-order solution
+// and the error are built from the stored derivatives), and skipping them
+// saves 3/7 of this loop's traffic.
 __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, int stage)
 {
     if (stage == 2 && e == 0) G.errmax[inst] = 0.0;
     const double h = G.h[inst];
-    for (int k = 0; k < 7; k++) {
-        const size_t i = sidx(G, k, e, inst);
+    constexpr int kRhsComps[4] = {2, 3, 5, 6};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const size_t i = sidx(G, kRhsComps[c], e, inst);
         const double k1 = G.dydt[i];
         const double k2 = stage > 2 ? G.ak[2][i] : 0.0;
         const double k3 = stage > 3 ? G.ak[3][i] : 0.0;
