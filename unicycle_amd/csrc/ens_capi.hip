@@ -324,7 +324,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         if (ealloc(E, &G.any_rhs, 1)) return UCY_ERR_HIP;
         // ---- state and control
         const size_t S7 = (size_t)7 * G.N * kEnsCols;
-        double **vecs[] = {&G.y, &G.dydt, &G.yscal, &G.ytmp, &G.ak[2], &G.ak[3], &G.ak[4],
+        double **vecs[] = {&G.y, &G.dydt, &G.ytmp, &G.ak[2], &G.ak[3], &G.ak[4],
                            &G.ak[5], &G.ak[6], &G.ycand};
         for (double **v : vecs)
             if (ealloc(E, v, S7)) return UCY_ERR_HIP;
@@ -354,7 +354,8 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
                 EHIP(hipMemcpyAsync(ds[k], hs[k]->data(), sizeof(double) * F, hipMemcpyHostToDevice, c->stream));
             EHIP(hipStreamSynchronize(c->stream));
         }
-        double **cv[] = {&G.time, &G.h, &G.hnext, &G.hdid, &G.errmax, &G.vmax, &G.t_event};
+        double **cv[] = {&G.time, &G.h, &G.hnext, &G.hdid, &G.errmax, &G.vmax, &G.t_event,
+                         &G.hstart};
         for (double **v : cv)
             if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
         int **ci[] = {&G.need_rhs, &G.accepted, &G.done, &G.status};

@@ -49,10 +49,11 @@ struct EnsGroup {
     double *t0;              // [mp][64] tractions at the step start
     int *any_rhs;            // some instance of the group starts a step
     // state (SoA, 7 components)
-    double *y, *dydt, *yscal, *ytmp, *ak[7], *ycand;   // the embedded error is only reduced
+    double *y, *dydt, *ytmp, *ak[7], *ycand;   // the embedded error is only reduced
     EnsFriction fr;
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;
+    double *hstart;          // h of the step's first attempt: the dt_try of yscal
     double *t_event;         // time of the first step starting with vMax > event_vmax (-1: none)
     double *hist;            // [5][64] proposal history: log10 vMax, h, log10 errmax of the
                              // last accepted step (h = 0: none yet), rejections of this step,

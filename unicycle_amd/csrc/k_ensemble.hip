@@ -758,7 +758,7 @@ __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restri
 }
 
 // post-pass (ratestate.f90:747-771) writing stage derivative `stage`
-// (1 = step start, then also yscal of ratestate.f90:315)
+// (1 = step start, which also records the step's dt_try for yscal)
 __device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, int stage)
 {
     const double *X = stage == 1 ? G.y : G.ytmp;
@@ -779,13 +779,7 @@ __device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, in
     out[sidx(G, 4, e, inst)] = t[2];
     out[sidx(G, 5, e, inst)] = d5;
     out[sidx(G, 6, e, inst)] = d6;
-    if (stage == 1) {
-        const double h = G.h[inst];
-        for (int k = 0; k < 7; k++) {
-            const size_t i = sidx(G, k, e, inst);
-            G.yscal[i] = fabs(G.y[i]) + fabs(h * G.dydt[i]) + 1.0e-30;
-        }
-    }
+    if (stage == 1 && e == 0) G.hstart[inst] = G.h[inst];
 }
 
 __device__ __forceinline__ double ens_stage_value(int stage, double h, double y0, double k1,
@@ -859,11 +853,16 @@ __global__ void __launch_bounds__(256) ens_post_stage_kernel(const EnsTable *__r
 }
 
 // fifth-order solution and embedded error of element e (ode45.f90:162-172);
-// returns max_k |yerr/yscal| (NaN propagates)
+// returns max_k |yerr/yscal| (NaN propagates).  yscal (ratestate.f90:315)
+// is formed here from the y and dydt this pass reads anyway, with the dt_try
+// of the step's first attempt (retries keep it, as the reference's RKQSm
+// keeps its yscal argument): the same arithmetic as a stored yscal, without
+// its 7-component write and read.
 __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst)
 {
     double emax = 0.0;
     const double h = G.h[inst];
+    const double hs = G.hstart[inst];
     const double DC1 = 37.0 / 378.0 - 2825.0 / 27648.0, DC3 = 250.0 / 621.0 - 18575.0 / 48384.0,
                  DC4 = 125.0 / 594.0 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0,
                  DC6 = 512.0 / 1771.0 - 0.25;
@@ -882,7 +881,8 @@ __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst
         er = er + DC5 * k5;
         er = er + DC6 * k6;
         er = h * er;
-        const double r = fabs(er / G.yscal[i]);
+        const double yscal = fabs(G.y[i]) + fabs(hs * k1) + 1.0e-30;
+        const double r = fabs(er / yscal);
         emax = (r > emax || r != r) ? r : emax;
     }
     return emax;
