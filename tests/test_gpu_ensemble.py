@@ -68,7 +68,7 @@ def test_ensemble_matvec(engine_factory, precision):
     # every GEMM variant of the autotuner, at split depths that give one and
     # several k-splits (the groups have 8 and 16 k-blocks)
     tilings = [(kb, 0) for kb in (4, 64)] if precision == PREC_F64 else \
-        [(kb, v) for v in range(5) for kb in (4, 64)]
+        [(kb, v) for v in range(6) for kb in (4, 64)]
     for kb, v in tilings:
         ens.set_tiling(kb, v)
         for gi, K in enumerate(Ks):

@@ -87,8 +87,10 @@ struct EnsTable {
 // split-bf16 stage-GEMM variants of the autotuner: 0 = 64 rows per wave,
 // 1 = 64 rows with the next k-block's loads issued ahead, 2 = 128 rows, ahead,
 // 3 = 128 rows with V staged in LDS per block in chunks of 4 k-blocks,
-// 4 = 64 rows, ahead, V staged in chunks of 8 k-blocks
-constexpr int kGemmVariants = 5;
+// 4 = 64 rows, ahead, V staged in chunks of 8 k-blocks, 5 = 128 rows (200
+// VGPRs; 128 rows ahead with V staged in chunks of 2 needs 264-272, one wave
+// per SIMD, and measured 461 us against 265 us: not kept)
+constexpr int kGemmVariants = 6;
 int ens_gemm_variant_rows(int variant);
 int ens_gemm_variant_lds(int variant);
 

@@ -380,7 +380,7 @@ struct Bf16Variant {
     int lds_chunk;
 };
 constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
-    {2, false, 0}, {2, true, 0}, {4, true, 0}, {4, false, 4}, {2, true, 8}};
+    {2, false, 0}, {2, true, 0}, {4, true, 0}, {4, false, 4}, {2, true, 8}, {4, false, 0}};
 
 // LDS variants: the 4 waves of a block take 4 consecutive row items of one
 // k-split; the block stages the split's V tiles through LDS in chunks of CK
@@ -610,7 +610,8 @@ static void launch_gemm_bf16(hipStream_t st, const EnsTable *dtab, const EnsTabl
     case 1: hipLaunchKernelGGL((gemm_bf16_kernel<1, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 2: hipLaunchKernelGGL((gemm_bf16_kernel<2, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 3: hipLaunchKernelGGL((gemm_bf16_kernel<3, I8>), grid, dim3(256), 0, st, dtab, stage); break;
-    default: hipLaunchKernelGGL((gemm_bf16_kernel<4, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 4: hipLaunchKernelGGL((gemm_bf16_kernel<4, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<5, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     }
 }
 
