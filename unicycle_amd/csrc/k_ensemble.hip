@@ -156,13 +156,13 @@ void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, i
 // K -> bf16+i8 tiles with the same exact row scales.  For x = K / s, hi =
 // bf16(x) and the residual r = x - hi (exact in binary64) is coded in steps
 // of 2^(E-142), E the biased exponent of hi, i.e. 1/128 of hi's ulp:
-// q = rint(r / 2^(E-142)) lies in [-128, 128] (|r| <= ulp/2).  Where it
-// would be +128, hi moves one ulp up and the code is taken again (then in
-// [-128, -64]), so every entry keeps |x - (hi + q 2^(E-142))| <= 2^(E-143),
-// 2^-16 (1 + 2^-8) relative.  Entries with E < 32 (|x| < 2^-95, |x| <= 1 by
-// the row scale) are flushed to zero: their products lie far below the fp32
-// accumulation's resolution, and the flush keeps the decoder's scale
-// finite.
+// q = rint(r / 2^(E-142)) lies in [-128, 128] (|r| <= ulp/2), and the byte
+// stored is -q (the decoder's scale carries a minus sign).  Where q would
+// be -128, hi moves one ulp down and q is taken again (then in [64, 128]),
+// so every entry keeps |x - (hi + q 2^(E-142))| <= 2^(E-143), 2^-16
+// (1 + 2^-8) relative.  Entries with E < 32 (|x| < 2^-95, |x| <= 1 by the
+// row scale) are flushed to zero: their products lie far below the fp32
+// accumulation's resolution.
 __global__ void split3_kernel(const double *__restrict__ K, int m, int n, int mp, int np,
                               const double *__restrict__ rscale, uint8_t *__restrict__ tiles)
 {
@@ -178,12 +178,12 @@ __global__ void split3_kernel(const double *__restrict__ K, int m, int n, int mp
         hi = 0;
     } else {
         double t = rint(ldexp(x - h, 142 - E));
-        if (t > 127.0) {
-            hi = bf16_rne(h + ldexp(1.0, E - 134), h);   // exact: the next bf16 up
+        if (t < -127.0) {
+            hi = bf16_rne(h - ldexp(1.0, E - 134), h);   // exact: the next bf16 down
             E = (hi >> 7) & 0xFF;
             t = rint(ldexp(x - h, 142 - E));
         }
-        q = (int)fmin(fmax(t, -128.0), 127.0);
+        q = -(int)fmin(fmax(t, -127.0), 128.0);
     }
     const int nkb = np >> 4;
     const int rb = r >> 5, kb = c >> 4, w = c & 15;
@@ -203,21 +203,24 @@ void ens_launch_split3_K(hipStream_t st, const double *K, int m, int n, int mp, 
 }
 
 // bf16 lo operand of 8 bf16+i8 entries from their hi words (2 bf16 each, low
-// half first) and lo codes (one byte each): lo_j = q_j 2^(E_j - 142).  The
-// scale 2^(E-142) has the binary32 bits (E << 23) - (15 << 23), finite for
-// E = 0 (where q = 0) since the packer leaves no 0 < E < 32.  q_j 2^(E_j-142)
-// has at most 8 significant bits, so its top 16 bits are the exact bf16.
-// (Scaling hi by 2^15 instead would save the subtraction but keeps a second
-// copy of hi live: past 256 VGPRs, half the waves.)
+// half first) and lo bytes c_j = -q_j: lo_j = c_j (-2^(E_j - 142)).  Both
+// halves' scales come from one mask and one add: per 16-bit half, E << 7
+// plus 0x7880 is (E - 15) << 7 with the sign bit set for E >= 15, i.e. the
+// top half of the binary32 -2^(E-142) (no carry between the halves: E <=
+// 127 keeps each half below 0xB801); for E = 0 (a zero hi, code 0) it is
+// +2^114, finite.  The packer leaves no 0 < E < 32.  c_j times the scale has
+// at most 8 significant bits, so its top 16 bits are the exact bf16.
+// (Scaling hi by 2^15 instead would save the adjust but keeps a second copy
+// of hi live: past 256 VGPRs, half the waves.)
 __device__ __forceinline__ bf16x8 decode_lo8(const u32x4 &h, const u32x2 &q)
 {
     u32x4 out;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const unsigned H = h[i];
+        const unsigned M = (h[i] & 0x7F807F80u) + 0x78807880u;
         const unsigned Q = q[i >> 1] >> (16 * (i & 1));
-        const float s0 = __uint_as_float(((H << 16) & 0x7F800000u) - 0x07800000u);
-        const float s1 = __uint_as_float((H & 0x7F800000u) - 0x07800000u);
+        const float s0 = __uint_as_float(M << 16);
+        const float s1 = __uint_as_float(M & 0xFFFF0000u);
         const float l0 = (float)(int)(signed char)(Q & 0xFFu) * s0;
         const float l1 = (float)(int)(signed char)((Q >> 8) & 0xFFu) * s1;
         // bytes 2, 3 of l0 then bytes 2, 3 of l1
