@@ -919,66 +919,121 @@ __global__ void __launch_bounds__(256) ens_post_finish_kernel(const EnsTable *__
 }
 
 // per-instance RKQSm step control + main-loop time update (ode45.f90:70-105,
-// ratestate.f90:317-326); one thread per instance
-__global__ void ens_control_kernel(EnsTable *tab)
+// ratestate.f90:317-326).  One block of 256 threads per group: wave 0 runs
+// the rule, one lane per instance.  With a step-size network, the four waves
+// evaluate it together for the lanes that ask: parameters staged in LDS,
+// layer 1 and the tanh of layer 2 split by hidden unit (16 per wave), the
+// output summed by wave 0 over the 64 units in order.  Every value is formed
+// in the order of mlp_eval, so the proposal is the one a single thread
+// computes; the split takes the network's ~5,000 dependent FMAs and global
+// loads per lane off the attempt's critical path.
+__global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
 {
+    __shared__ float sP[kMlpParams];
+    __shared__ float sX[kMlpIn][64];
+    __shared__ float sH[kMlpH][64];
+    __shared__ float sT[kMlpH][64];
     const int blk = blockIdx.x;
     const int g = group_of(tab->blk_inst, tab->ngroups, blk);
     const EnsGroup &G = tab->g[g];
-    const int inst = (blk - tab->blk_inst[g]) * 64 + threadIdx.x;
-    if (inst >= G.ninst || G.done[inst]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inst = (blk - tab->blk_inst[g]) * 64 + lane;
     const double SAFETY = 0.7, PGROW = -0.2, PSHRNK = -0.15, ERRCON = 1.89e-4;
-    G.attempts[inst] += 1;
-    const double raw = G.errmax[inst];
-    G.accepted[inst] = 0;
-    double h = G.h[inst];
     double *hist = G.hist;
-    if (!(raw == raw) || raw > 1.7976931348623157e308) {
-        // an attempt whose error norm is not finite (an overflow inside the
-        // stages) is rejected with the largest shrink of the rule, x0.1
-        // (ode45.f90:81 would accept it: NaN > 1 is false); the fourth such
-        // attempt in a row fails the instance
-        const double nf = hist[4 * kEnsCols + inst] + 1.0;
-        hist[4 * kEnsCols + inst] = nf;
-        hist[3 * kEnsCols + inst] += 1.0;
-        h *= 0.1;
-        G.h[inst] = h;
-        G.need_rhs[inst] = 0;
-        if (nf >= 4.0 || 0.0 + h == 0.0) {
-            G.status[inst] = 2;
-            G.done[inst] = 1;
-            atomicAdd(tab->all_done, 1);
-        }
-        return;
-    }
-    hist[4 * kEnsCols + inst] = 0.0;
-    const double errmax = raw / tab->epsilon;
-    if (errmax > 1.0) {
-        h = h * SAFETY * pow(errmax, PSHRNK);
-        G.h[inst] = h;
-        G.need_rhs[inst] = 0;
-        hist[3 * kEnsCols + inst] += 1.0;
-        if (0.0 + h == 0.0) {
-            G.status[inst] = 1;
-            G.done[inst] = 1;
-            atomicAdd(tab->all_done, 1);
-        }
-    } else {
-        double hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
-        if (hnext > tab->max_step) hnext = tab->max_step;
-        if (tab->mlp && errmax > ERRCON) {
-            // learned proposal of the next step where the error estimate is
-            // informative (below ERRCON the classic rule already takes the
-            // largest growth, x10); the embedded error estimate still judges
-            // the proposal and a rejection falls back to the classic shrink
-            const bool first = hist[1 * kEnsCols + inst] == 0.0;
-            float f[kMlpIn];
-            mlp_features(G.vmax[inst], hist[inst], h, hist[1 * kEnsCols + inst], errmax,
-                         hist[2 * kEnsCols + inst], hist[3 * kEnsCols + inst], hnext, first, f);
-            const MlpNorm &nm = tab->mlp_norm;
+    bool capcheck = false, finish = false, want = false;
+    double h = 0.0, hnext = 0.0, errmax = 0.0;
+    // ---- the rule (wave 0); an accepted step waits for the proposal
+    if (wave == 0 && inst < G.ninst && !G.done[inst]) {
+        G.attempts[inst] += 1;
+        const double raw = G.errmax[inst];
+        G.accepted[inst] = 0;
+        h = G.h[inst];
+        if (!(raw == raw) || raw > 1.7976931348623157e308) {
+            // an attempt whose error norm is not finite (an overflow inside
+            // the stages) is rejected with the largest shrink of the rule,
+            // x0.1 (ode45.f90:81 would accept it: NaN > 1 is false); the
+            // fourth such attempt in a row fails the instance
+            const double nf = hist[4 * kEnsCols + inst] + 1.0;
+            hist[4 * kEnsCols + inst] = nf;
+            hist[3 * kEnsCols + inst] += 1.0;
+            h *= 0.1;
+            G.h[inst] = h;
+            G.need_rhs[inst] = 0;
+            if (nf >= 4.0 || 0.0 + h == 0.0) {
+                G.status[inst] = 2;
+                G.done[inst] = 1;
+                atomicAdd(tab->all_done, 1);
+            }
+        } else {
+            capcheck = true;
+            hist[4 * kEnsCols + inst] = 0.0;
+            errmax = raw / tab->epsilon;
+            if (errmax > 1.0) {
+                h = h * SAFETY * pow(errmax, PSHRNK);
+                G.h[inst] = h;
+                G.need_rhs[inst] = 0;
+                hist[3 * kEnsCols + inst] += 1.0;
+                if (0.0 + h == 0.0) {
+                    G.status[inst] = 1;
+                    G.done[inst] = 1;
+                    atomicAdd(tab->all_done, 1);
+                }
+            } else {
+                finish = true;
+                hnext = errmax > ERRCON ? SAFETY * h * pow(errmax, PGROW) : 10.0 * h;
+                if (hnext > tab->max_step) hnext = tab->max_step;
+                // learned proposal of the next step where the error estimate
+                // is informative (below ERRCON the classic rule already takes
+                // the largest growth, x10); the embedded error estimate still
+                // judges the proposal and a rejection falls back to the
+                // classic shrink
+                want = tab->mlp && errmax > ERRCON;
+                if (want) {
+                    const bool first = hist[1 * kEnsCols + inst] == 0.0;
+                    float f[kMlpIn];
+                    mlp_features(G.vmax[inst], hist[inst], h, hist[1 * kEnsCols + inst], errmax,
+                                 hist[2 * kEnsCols + inst], hist[3 * kEnsCols + inst], hnext, first,
+                                 f);
+                    const MlpNorm &nm = tab->mlp_norm;
 #pragma unroll
-            for (int i = 0; i < kMlpIn; i++) f[i] = (f[i] - nm.xmean[i]) / nm.xscale[i];
-            const double lp = (double)(mlp_eval(tab->mlp, f) * nm.yscale + nm.ymean);
+                    for (int i = 0; i < kMlpIn; i++) sX[i][lane] = (f[i] - nm.xmean[i]) / nm.xscale[i];
+                }
+            }
+        }
+    }
+    if (wave == 0 && !want) {
+#pragma unroll
+        for (int i = 0; i < kMlpIn; i++) sX[i][lane] = 0.0f;
+    }
+    // ---- the network (all waves, every thread reaches the barriers)
+    if (tab->mlp && __syncthreads_or(want)) {
+        for (int i = threadIdx.x; i < kMlpParams; i += 256) sP[i] = tab->mlp[i];
+        __syncthreads();
+        for (int jj = 0; jj < 16; jj++) {
+            const int j = 16 * wave + jj;
+            float a = sP[kMlpOffB1 + j];
+#pragma unroll
+            for (int i = 0; i < kMlpIn; i++) a = fmaf(sP[j * kMlpIn + i], sX[i][lane], a);
+            sH[j][lane] = tanhf(a);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = 16 * wave + kk;
+            float a = sP[kMlpOffB2 + k];
+            const float *w = sP + kMlpOffW2 + k * kMlpH;
+#pragma unroll 16
+            for (int j = 0; j < kMlpH; j++) a = fmaf(w[j], sH[j][lane], a);
+            sT[k][lane] = tanhf(a);
+        }
+        __syncthreads();
+    }
+    // ---- the accepted step (wave 0)
+    if (finish) {
+        if (want) {
+            float y = sP[kMlpOffB3];
+            for (int k = 0; k < kMlpH; k++) y = fmaf(sP[kMlpOffW3 + k], sT[k][lane], y);
+            const MlpNorm &nm = tab->mlp_norm;
+            const double lp = (double)(y * nm.yscale + nm.ymean);
             // within a factor 4 of the classic proposal and the classic
             // growth limit: the network corrects the rule, it does not
             // extrapolate past it
@@ -1019,7 +1074,7 @@ __global__ void ens_control_kernel(EnsTable *tab)
             atomicAdd(tab->all_done, 1);
         }
     }
-    if (!G.done[inst] && G.attempts[inst] >= tab->max_attempts) {
+    if (capcheck && !G.done[inst] && G.attempts[inst] >= tab->max_attempts) {
         G.done[inst] = 1;
         atomicAdd(tab->all_done, 1);
     }
@@ -1070,7 +1125,7 @@ void ens_launch_post_finish(hipStream_t st, const EnsTable *dtab, const EnsTable
 }
 void ens_launch_control(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
 {
-    hipLaunchKernelGGL(ens_control_kernel, dim3(h.blk_inst[h.ngroups]), dim3(64), 0, st,
+    hipLaunchKernelGGL(ens_control_kernel, dim3(h.blk_inst[h.ngroups]), dim3(256), 0, st,
                        const_cast<EnsTable *>(dtab));
 }
 void ens_launch_commit(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
