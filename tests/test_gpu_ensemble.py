@@ -89,12 +89,17 @@ def test_ensemble_matvec(engine_factory, precision):
 @pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_BF16_I8])
 def test_stored_kernel_format(engine_factory, precision):
     """Every entry of the stored K, read back through the stage GEMM with
-    unit slip-rate vectors (V = 1 has no lo part, the three products and
-    their fp32 sum hi + lo are exact, the row scale is a power of two),
+    unit slip-rate vectors (V = 1 has no lo part, the products are exact,
+    the row scale is a power of two, and the fp32 sum hi + lo is exact
+    except for split pairs with lo far below hi),
     against the binary64 K: both formats within 2^-16 relative (with the
     bf16+i8 flush of entries below 2^-94 of the row maximum), bf16+i8
     beyond 2^-17 somewhere (the 8-bit code is what is read), on three
-    tiling variants."""
+    tiling variants; and equal to the numpy restatement of the format
+    applied to the device's binary64 K (the plain engine's, built by the
+    same assembly kernel from the same inputs): bf16+i8 bit for bit,
+    split bf16 within one binary32 ulp (the MFMA accumulator's own
+    rounding of hi + lo where lo lies more than 16 binades below hi)."""
     eng = engine_factory()
     model = configs.thrust_fault(nx=8, nz=8, ellipse=None)
     draws, y0 = _instances(model, 64, seed=2)
@@ -113,6 +118,47 @@ def test_stored_kernel_format(engine_factory, precision):
             (kb, v, (err / np.abs(K)).max())
         if precision == PREC_BF16_I8:
             assert (err / np.where(K == 0, 1, np.abs(K))).max() > 2.0 ** -17
+    plain = _plain_engine(engine_factory, model, model.friction())
+    Kd = plain.get_kernel()
+    want = _stored_restatement(Kd, precision)
+    if precision == PREC_BF16_I8:
+        mismatch = int(np.count_nonzero(Kt != want))
+        assert mismatch == 0, f"{mismatch} of {Kt.size} stored entries differ from the restatement"
+    else:
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(Kt - want) <= ulp)
+
+
+def _bf16_rne(x):
+    """binary64 -> 8 significant bits, ties to even (as a binary64 value)."""
+    m, e = np.frexp(x)
+    return np.ldexp(np.rint(np.ldexp(m, 8)), e - 8)
+
+
+def _stored_restatement(K, precision):
+    """The stored K of each format, restated in numpy from its definition:
+    power-of-two row scale s with max |K_row| / s in [0.5, 1); x = K / s;
+    hi = RNE bf16 of x; split bf16 adds lo = RNE bf16 of x - hi; bf16+i8
+    adds q 2^(E-142), q = rint((x - hi) / 2^(E-142)), E the binary32 biased
+    exponent of hi, hi moved one ulp down where q would be -128, and
+    entries with E < 32 flushed to zero.  As read back through the GEMM:
+    the fp32 accumulator holds hi + lo, exact for bf16+i8 (16 bits apart at
+    most) and rounded to binary32 for a split pair whose lo lies more than
+    16 binades below hi."""
+    mx = np.abs(K).max(axis=1, keepdims=True)
+    s = np.where(mx > 0, np.ldexp(1.0, np.frexp(mx)[1]), 1.0)
+    x = K / s
+    h = _bf16_rne(x)
+    if precision == PREC_SPLIT_BF16:
+        return s * (h + _bf16_rne(x - h)).astype(np.float32).astype(np.float64)
+    E = np.where(h != 0, np.frexp(h)[1] + 126, 0)
+    t = np.rint(np.ldexp(x - h, 142 - E))
+    bump = t < -127
+    h = np.where(bump, h - np.ldexp(1.0, E - 134), h)
+    E = np.where(h != 0, np.frexp(h)[1] + 126, 0)
+    t = np.rint(np.ldexp(x - h, 142 - E))
+    kept = np.where(E < 32, 0.0, h + np.ldexp(t, E - 142))
+    return s * kept
 
 
 def _plain_engine(engine_factory, model, fr):
