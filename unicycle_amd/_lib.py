@@ -151,7 +151,7 @@ _SIGS = {
 }
 
 EXPORTED = tuple(_SIGS)
-GEMM_TUNE_ENTRIES = 25   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
+GEMM_TUNE_ENTRIES = 42   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:

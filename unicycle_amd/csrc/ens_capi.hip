@@ -388,15 +388,19 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     E->tune_seconds.clear();
     int best = 0, best_v = 0;
     double best_t = 1e30;
-    const int cands[] = {32, 64, 128, 256, 512};
+    // split depths (k-blocks of 16 per split); the table below is
+    // UCY_GEMM_TUNE_ENTRIES = kGemmVariants x kDepths entries
+    constexpr int kDepths = 7;
+    const int cands[kDepths] = {32, 64, 96, 128, 192, 256, 512};
+    static_assert(kGemmVariants * kDepths == UCY_GEMM_TUNE_ENTRIES, "tuning table size");
     {
         hipEvent_t e0, e1;
         EHIP(hipEventCreate(&e0));
         EHIP(hipEventCreate(&e1));
         // (split-bf16 variant, k-blocks per split); the binary64 GEMM has one
         // variant
-        for (int ci = 0; ci < kGemmVariants * 5; ci++) {
-            const int kb = cands[ci % 5], v = ci / 5;
+        for (int ci = 0; ci < kGemmVariants * kDepths; ci++) {
+            const int kb = cands[ci % kDepths], v = ci / kDepths;
             if (v > 0 && E->mode == 1) {
                 E->tune_seconds.push_back(-1.0);
                 continue;

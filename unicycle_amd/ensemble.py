@@ -185,11 +185,12 @@ class Ensemble:
         return T
 
     def tuning(self):
-        """(k-blocks per split, split-bf16 GEMM variant, seconds per launch
-        of the candidates: k-blocks 32..512 of variant 0, 1, ...).  Variants:
-        0 = 64 K rows per wave, 1 = 64 rows with loads issued a k-block
-        ahead, 2 = 128 rows, ahead, 3 = 128 rows with V staged in LDS in
-        chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged in chunks of 8."""
+        """(k-blocks per split, bf16 GEMM variant, seconds per launch of the
+        candidates: k-blocks 32, 64, 96, 128, 192, 256, 512 of variant 0, then
+        1, ...).  Variants: 0 = 64 K rows per wave, 1 = 64 rows with loads
+        issued a k-block ahead, 2 = 128 rows, ahead, 3 = 128 rows with V
+        staged in LDS in chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged
+        in chunks of 8, 5 = 128 rows."""
         kb, var = C.c_int(), C.c_int()
         t = np.full(_lib.GEMM_TUNE_ENTRIES, -1.0)
         self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
