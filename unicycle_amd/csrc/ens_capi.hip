@@ -331,26 +331,25 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         G.ak[0] = G.ak[1] = G.dydt;
         // friction SoA [e][inst]
         const size_t F = (size_t)G.N * kEnsCols;
-        double **fv[] = {&G.fr.a, &G.fr.b, &G.fr.sig, &G.fr.L, &G.fr.damping, &G.fr.vplc, &G.fr.vpls};
+        double **fv[] = {&G.fr.asig, &G.fr.bsig, &G.fr.L, &G.fr.damping, &G.fr.vplc, &G.fr.vpls};
         for (double **v : fv)
             if (ealloc(E, v, F)) return UCY_ERR_HIP;
         {
-            std::vector<double> fa(F, 0.0), fb(F, 0.0), fs(F, 1.0), fl(F, 1.0), fd(F, 0.0),
-                fc(F, 0.0), fsn(F, 0.0);
+            std::vector<double> fas(F, 0.0), fbs(F, 0.0), fl(F, 1.0), fd(F, 0.0), fc(F, 0.0),
+                fsn(F, 0.0);
             for (int i = 0; i < H.ninst; i++)
                 for (int e = 0; e < H.N; e++) {
                     const size_t src = (size_t)i * H.N + e, dst = (size_t)e * kEnsCols + i;
-                    fa[dst] = H.a[src];
-                    fb[dst] = H.b[src];
-                    fs[dst] = H.sig[src];
+                    fas[dst] = H.a[src] * H.sig[src];
+                    fbs[dst] = H.b[src] * H.sig[src];
                     fl[dst] = H.Lc[src];
                     fd[dst] = H.damping[src];
                     fc[dst] = H.Vpl[src] * cos(H.rake[src]);
                     fsn[dst] = H.Vpl[src] * sin(H.rake[src]);
                 }
-            const std::vector<double> *hs[] = {&fa, &fb, &fs, &fl, &fd, &fc, &fsn};
-            double *ds[] = {G.fr.a, G.fr.b, G.fr.sig, G.fr.L, G.fr.damping, G.fr.vplc, G.fr.vpls};
-            for (int k = 0; k < 7; k++)
+            const std::vector<double> *hs[] = {&fas, &fbs, &fl, &fd, &fc, &fsn};
+            double *ds[] = {G.fr.asig, G.fr.bsig, G.fr.L, G.fr.damping, G.fr.vplc, G.fr.vpls};
+            for (int k = 0; k < 6; k++)
                 EHIP(hipMemcpyAsync(ds[k], hs[k]->data(), sizeof(double) * F, hipMemcpyHostToDevice, c->stream));
             EHIP(hipStreamSynchronize(c->stream));
         }

@@ -22,7 +22,9 @@ constexpr int kEnsCols = 64;     // instance columns per group (padded)
 constexpr int kMaxGroups = 8;
 
 struct EnsFriction {             // per (element, instance), SoA arrays [e*64+inst]
-    double *a, *b, *sig, *L, *damping, *vplc, *vpls;
+    // a sig and b sig as products (the right-hand side uses a, b and sig
+    // only there, multiplied first: same roundings, one read fewer)
+    double *asig, *bsig, *L, *damping, *vplc, *vpls;
 };
 
 struct EnsGroup {

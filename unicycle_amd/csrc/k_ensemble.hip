@@ -775,8 +775,8 @@ __device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, in
     const double cr = cos(rake), sr = sin(rake);
     const double d5 = (exp(-X[sidx(G, 5, e, inst)] * kELg10) - velocity / G.fr.L[f]) / kELg10;
     const double dtau = t[0] * cr + t[1] * sr;
-    const double d6 = (dtau - G.fr.b[f] * G.fr.sig[f] * d5 * kELg10)
-                      / (G.fr.a[f] * G.fr.sig[f] + G.fr.damping[f] * velocity) / kELg10;
+    const double d6 = (dtau - G.fr.bsig[f] * d5 * kELg10)
+                      / (G.fr.asig[f] + G.fr.damping[f] * velocity) / kELg10;
     const double correction = G.fr.damping[f] * velocity * d6 * kELg10;
     out[sidx(G, 2, e, inst)] = t[0] - correction * cr;
     out[sidx(G, 3, e, inst)] = t[1] - correction * sr;
