@@ -9,59 +9,43 @@ geometry groups of 64 (1024, 2048, 4096, 8192 Okada rectangles of 2 km, the
 states.  Each group shares one K; every RK45 stage of every instance is one
 batched matvec per group inside a captured hipGraph.
 
-A "step" is one pass of all 256 instances through a fixed amount of work:
---attempts RK45 attempts per instance from the same seeded initial states
-(with 2 km cells the nucleation size is ~0.5 km, so seismic events take
-thousands of 0.01-1 s steps; a fixed amount of work, not a fixed simulated
-window, keeps the bench bounded and compares the paths on identical work).
+A "step" is --attempts RK45 attempts of every instance.  The headline
+variant runs --warmup untimed steps and then --steps timed steps, all
+continuing ONE run from the seeded initial states, so the timed window
+starts after the start-up growth of the step size; ms_per_step is the mean
+wall time of the timed steps (each bracketed by device synchronisation) and
+value is the simulated time the 256 instances advance in them, summed, per
+wall second.
 
-Reported:
-  value         sum over instances of the simulated time reached per wall
-                second, for the fastest batched variant that meets the
-                binary64 bar on this run (slip error <= 1e-6 at a fixed
-                pre-event time, first-event times within 0.1 %): bf16+i8
-                with or without learned step-size proposals, split bf16,
-                mixed, or binary64 (which meets it by construction).
-  batched_f64   binary64 K on v_mfma_f64_16x16x4f64; per-instance arithmetic
-                and step decisions are those of the plain path.
-  plain_fp64    the single-instance device path (streaming binary64 GEMV per
-                instance), measured per attempt on instance 0 of every group
-                and costed over every instance's attempt count of the run.
-  vs_plain_fp64 value / plain_fp64 on the same work.
-  mixed, split_bf16, bf16_i8, bf16_i8_learned
-                binary64 step-start matvec (mixed) or the reduced K
-                throughout, reduced stage increments: split bf16 (4 bytes per
-                K entry) or bf16+i8 (3 bytes: bf16 hi and an 8-bit code of
-                the residual, decoded in registers); the last with the
-                step-size
-                network (trained on instances drawn apart from the bench's:
-                plain binary64 histories, closed-loop rounds, output offset):
-                throughput on the same attempt budget, slip error against
-                binary64 at a fixed pre-event time (runs clamped to land on
-                it), first-event times against binary64 (runs stopped at each
-                instance's first event).
-  roofline      the stage GEMM of the headline variant (dominant kernel),
-                timed with HIP events on the library stream: HBM-bound for
-                the reduced formats, MFMA-bound for binary64; traffic from
-                the FETCH_SIZE / WRITE_SIZE counters of the same GEMM.
+Sections, in order (the first five form the line; the rest run only while
+the --budget wall clock allows, and a failing section becomes an "errors"
+entry instead of a lost line):
+  batched_f64   binary64 K on v_mfma_f64_16x16x4f64, --attempts attempts
+                from the initial states, timed once; it is the precision
+                reference: slip at a fixed pre-event time (every instance
+                clamped to land on it) and first-event times.
+  training      the 8-64-64-1 step-size network (hand-written GPU kernels),
+                trained on instances drawn apart from the bench's.
+  headline      --headline (default bf16+i8 K with the learned proposals):
+                the timed steps, its stage-GEMM timing (HIP events on the
+                library stream) and its slip and first-event errors against
+                binary64.  value is reported only if it meets the bar (slip
+                <= 1e-6, events within 0.1 %, none in only one run);
+                otherwise value falls back to batched binary64.
+  plain_fp64    the single-instance binary64 device path (streaming GEMV),
+                seconds per attempt of instance 0 of every group, costed over
+                every instance's attempts of the headline window.
+  cpu_baseline  the oracle's C restatement (OpenMP over the host cores) per
+                attempt on instance 0 of every group over a bounded sample,
+                costed the same way.
+  bf16_i8, split_bf16, mixed
+                the other precisions on the reference's schedule, with their
+                errors against binary64.
   learned_step_proposal
-                the 8-64-64-1 step-size network trained on the plain binary64
-                integrator's step histories of 16 instances of the 1024-patch
-                group, refined by two closed-loop rounds on the same
-                instances and an output offset, against the classic
-                controller on the 48 held-out instances over a fixed
-                simulated window: RHS evaluations per simulated year,
-                rejected fraction, slip and first-event-time differences, and
-                the (lockstep-bound) sim-yr/s against the plain binary64 path.
-  cpu_baseline  the oracle's C restatement (OpenMP over the host cores): RK45
-                attempts/s on instance 0 of every group over a bounded sample,
-                costed over every instance's attempt count of the GPU run.
-  assembly      Green's-function assembly (buildG) on the device: pairs/s of
-                the Okada rectangles (2048-patch thrust), the Nikkhoo-Walter
-                triangles (20,000-triangle curved megathrust, 19.2 GB K) and
-                the strain volumes (4096 patches + 8192 cuboids, 28.2 GB K),
-                against the FP64 vector peak, with the oracle's buildG on the
-                host cores beside it.
+                the network against the classic controller on 48 held-out
+                instances over a fixed simulated window.
+  assembly      Green's-function assembly (buildG) on the device for the
+                three source types at the single-instance model sizes.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--attempts R]
 Prints exactly one JSON line on stdout; progress goes to stderr.
@@ -129,12 +113,48 @@ def _progress(tag):
     return f
 
 
+class Budget:
+    """Wall-clock budget of the whole bench (seconds since the process
+    started): sections past the core run only while their estimate fits."""
+
+    def __init__(self, seconds: float):
+        self.t0 = time.perf_counter()
+        self.limit = float(seconds)
+
+    def elapsed(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def allows(self, estimate: float) -> bool:
+        return self.elapsed() + estimate <= self.limit
+
+
+def section(out: dict, name: str, budget: Budget, estimate: float, fn):
+    """Run fn() as an optional section of the JSON line: skipped when its
+    estimate does not fit the wall budget, an exception becomes an error
+    entry instead of a lost line."""
+    timing = out.setdefault("section_seconds", {})
+    if not budget.allows(estimate):
+        out.setdefault("skipped", {})[name] = (f"wall budget: {budget.elapsed():.0f} s used of "
+                                               f"{budget.limit:.0f}, estimate {estimate:.0f} s")
+        log(f"section {name} skipped ({out['skipped'][name]})")
+        return None
+    t0 = time.perf_counter()
+    try:
+        return fn()
+    except Exception as e:  # recorded in the line, the bench goes on
+        out.setdefault("errors", {})[name] = f"{type(e).__name__}: {e}"
+        log(f"section {name} failed: {type(e).__name__}: {e}")
+        return None
+    finally:
+        timing[name] = round(time.perf_counter() - t0, 1)
+
+
 def run_attempts(ens, R, tag):
     """R attempts per instance from the initial states (no end time)."""
     ens.reset()
     ens.set_stop_at_end(False)
     ens.set_events(1e-3, False)
-    return ens.run(INF_T, max_attempts=R, chunk_attempts=max(1, R // 4), progress=_progress(tag))
+    return ens.run(INF_T, max_attempts=R, chunk_attempts=max(1, R // 2), progress=_progress(tag))
 
 
 def slip_at(ens, T, tag):
@@ -144,6 +164,9 @@ def slip_at(ens, T, tag):
     ens.set_events(1e-3, False)
     st = ens.run(T, max_attempts=10**6, chunk_attempts=2000, progress=_progress(tag))
     ens.set_stop_at_end(False)
+    if st.n_failed or st.min_time < T:   # no slip figure: the bar then fails
+        log(f"{tag}: {st.n_failed} failed, t_min {st.min_time:.4g} < {T:.4g}")
+        return st, None
     return st, [ens.state(g)[0] for g in range(len(ens.groups))]
 
 
@@ -157,6 +180,8 @@ def first_events(ens, T, tag):
 
 
 def slip_error(ya, yb):
+    if ya is None or yb is None:
+        return None
     num = den = 0.0
     for a, b in zip(ya, yb):
         sa = np.hypot(a[:, 0::7], a[:, 1::7])
@@ -167,14 +192,31 @@ def slip_error(ya, yb):
 
 
 def event_error(ta, tb):
-    both = (ta > 0) & (tb > 0)
-    if not both.any():
-        return {"instances_with_event": 0}
-    r = np.abs(tb[both] - ta[both]) / ta[both]
-    return {"instances_with_event": int(both.sum()),
-            "event_only_in_one": int(((ta > 0) ^ (tb > 0)).sum()),
-            "rel_err_median": float(np.median(r)), "rel_err_max": float(r.max()),
-            "within_0.1pct": float(np.mean(r <= 1e-3))}
+    """First-event times tb against ta (-1: no event by the horizon).  Events
+    found by one run only are counted before anything else, so a run that
+    loses (or invents) every event cannot pass for one that matches."""
+    a_ev, b_ev = ta > 0, tb > 0
+    both = a_ev & b_ev
+    out = {"instances_with_event": int(both.sum()),
+           "event_only_in_one": int((a_ev ^ b_ev).sum())}
+    if both.any():
+        r = np.abs(tb[both] - ta[both]) / ta[both]
+        out.update({"rel_err_median": float(np.median(r)), "rel_err_max": float(r.max()),
+                    "within_0.1pct": float(np.mean(r <= 1e-3))})
+    return out
+
+
+def meets_bar(slip_err, ev) -> bool:
+    """The headline bar: slip error <= 1e-6 at the pre-event time, first-event
+    times within 0.1 %, no event in only one of the two runs.  A missing
+    figure fails."""
+    if slip_err is None or not np.isfinite(slip_err) or slip_err > 1e-6:
+        return False
+    if ev is None or ev.get("event_only_in_one", 1) != 0:
+        return False
+    if ev.get("instances_with_event", 0) > 0 and not ev.get("rel_err_max", np.inf) <= 1e-3:
+        return False
+    return True
 
 
 TRAIN_SEED = 200   # draws of the network's training instances (the bench's are 100 + group)
@@ -185,8 +227,7 @@ def train_step_net(eng, args):
     the bench's instances: the plain binary64 integrator's step histories of
     16 instances of the 1024-patch geometry (draws seeded TRAIN_SEED), then
     two closed-loop rounds and the output offset on an ensemble of 16
-    training instances of each of the four geometries in split bf16 (the
-    precision the network then serves)."""
+    training instances of each of the four geometries in split bf16."""
     from unicycle_amd import Engine, PREC_SPLIT_BF16, configs, stepnet
     t0 = time.perf_counter()
     nx, nz = configs.ENSEMBLE_GROUPS[0]
@@ -221,180 +262,168 @@ def train_step_net(eng, args):
     return net, info
 
 
-def bench_ensemble(args):
-    from unicycle_amd import Engine, PREC_BF16_I8, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, configs
+VARIANTS = {  # name: (precision constant, learned proposals, dtype text)
+    "bf16_i8_learned": ("PREC_BF16_I8", True,
+                        "bf16+i8 K (3 bytes per entry), split-bf16 V, fp32 MFMA accumulate, "
+                        "f64 state, learned step-size proposals (fp32 MLP)"),
+    "bf16_i8": ("PREC_BF16_I8", False,
+                "bf16+i8 K (3 bytes per entry), split-bf16 V, fp32 MFMA accumulate, f64 state"),
+    "split_bf16": ("PREC_SPLIT_BF16", False,
+                   "split-bf16 K (hi+lo, 4 bytes per entry), fp32 MFMA accumulate, f64 state"),
+    "mixed": ("PREC_MIXED", False,
+              "binary64 K for t_0 = K v_0, split-bf16 increments, f64 state"),
+}
+
+
+def _prec(name):
+    import unicycle_amd
+    return getattr(unicycle_amd, VARIANTS[name][0])
+
+
+def group_attempts(ens):
+    return [ens.state(g)[4].astype(np.int64) for g in range(len(ens.groups))]
+
+
+def bench_reference(args, eng, shapes_out):
+    """Batched binary64 (the precision reference of every reduced variant):
+    R attempts per instance from the initial states, timed once; then the
+    slip at the pre-event time and the first-event times."""
+    from unicycle_amd import PREC_F64, configs
     R = args.attempts
-    eng = Engine(0)
     t0 = time.perf_counter()
     ens, info = configs.ensemble_setup(eng, PREC_F64)
-    t_setup = time.perf_counter() - t0
-    shapes = [(3 * m.n, 2 * m.n, y0.shape[0]) for m, d, y0 in info]
-    n_inst = sum(s[2] for s in shapes)
-    log(f"ensemble setup+build {t_setup:.1f} s; groups {[(m, n) for m, n, _ in shapes]}")
-    for w in range(args.warmup):
-        run_attempts(ens, min(R, 200), f"warmup {w}")
-    st, t_start = None, time.perf_counter()
-    for k in range(args.steps):
-        st = run_attempts(ens, R, f"f64 step {k}")
-    t_total = time.perf_counter() - t_start
+    setup_s = time.perf_counter() - t0
+    shapes_out.extend((3 * m.n, 2 * m.n, y0.shape[0]) for m, d, y0 in info)
+    log(f"ensemble setup+build {setup_s:.1f} s; groups {[(m, n) for m, n, _ in shapes_out]}")
+    t0 = time.perf_counter()
+    st = run_attempts(ens, R, "f64")
+    wall = time.perf_counter() - t0
     if st.n_failed:
-        raise RuntimeError(f"{st.n_failed} instances failed")
-    value = st.inst_time_sum / configs.YEAR / (t_total / args.steps)
-    st64 = [ens.state(g) for g in range(len(info))]
-    att64 = [s[4] for s in st64]
-    # dominant kernel: the stage GEMM over all groups, HIP events on its stream
+        raise RuntimeError(f"{st.n_failed} binary64 instances failed")
     t_gemm, gemm_bytes = ens.time_gemm(20)
-    flops = sum(2.0 * m * n * 64 for m, n, _ in shapes)
     kb, _, _ = ens.tuning()
-    tiling = {"f64": {"kb_per_split": kb}}
-    # precision references: slip at a fixed pre-event time, first-event times
-    Tslip = args.slip_years * configs.YEAR
-    Tevt = args.event_years * configs.YEAR
-    _, ys64 = slip_at(ens, Tslip, "f64 slip window")
-    te64 = first_events(ens, Tevt, "f64 first events")
+    _, ys = slip_at(ens, args.slip_years * configs.YEAR, "f64 slip window")
+    if ys is None:
+        raise RuntimeError("binary64 ensemble did not reach the slip window")
+    te = first_events(ens, args.event_years * configs.YEAR, "f64 first events")
     ens.close()
+    return {"info": info, "ys": ys, "te": te, "st": st, "wall": wall, "t_gemm": t_gemm,
+            "gemm_bytes": gemm_bytes, "kb": kb, "setup_s": setup_s}
 
-    reduced = {}
-    variants = [("mixed", PREC_MIXED, None), ("split_bf16", PREC_SPLIT_BF16, None),
-                ("bf16_i8", PREC_BF16_I8, None)]
-    net = None
-    if not args.no_learned:
-        net, net_info = train_step_net(eng, args)
-        variants.append(("bf16_i8_learned", PREC_BF16_I8, net))
-    for name, prec, mlp in variants:
-        ensr, _ = configs.ensemble_setup(eng, prec)
-        if mlp is not None:
-            mlp.attach(ensr)
-        if args.warmup:
-            run_attempts(ensr, min(R, 200), f"{name} warmup")
+
+def bench_headline(args, eng, name, net, ref):
+    """The headline variant: warmup W and then K timed steps of R attempts
+    per instance each, continuing one run from the seeded initial states
+    (step k covers attempts (W+k)R+1 .. (W+k+1)R of every instance), then its
+    GEMM timing and its slip and first-event errors against binary64."""
+    from unicycle_amd import configs
+    R, W, K = args.attempts, args.warmup, args.steps
+    ens, _ = configs.ensemble_setup(eng, _prec(name))
+    if net is not None:
+        net.attach(ens)
+    ens.reset()
+    ens.set_stop_at_end(False)
+    ens.set_events(1e-3, False)
+    cap = 0
+    for w in range(W):
+        cap += R
+        _progress(f"{name} warmup {w}")(ens.run(INF_T, max_attempts=cap))
+    att0 = group_attempts(ens)
+    time0 = [ens.state(g)[1] for g in range(len(ens.groups))]
+    s0 = float(sum(t.sum() for t in time0))
+    walls, st = [], None
+    for k in range(K):
+        cap += R
         t0 = time.perf_counter()
-        str_ = run_attempts(ensr, R, name)
-        wall_r = time.perf_counter() - t0
-        t_g, b_g = ensr.time_gemm(20)
-        kb_r, var_r, tt = ensr.tuning()
-        tiling[name] = {"kb_per_split": kb_r, "variant": var_r,
-                        "us_per_launch": [round(1e6 * x, 1) for x in tt]}
-        _, ysr = slip_at(ensr, Tslip, f"{name} slip window")
-        ter = first_events(ensr, Tevt, f"{name} first events")
-        ensr.close()
-        reduced[name] = {
-            "value": str_.inst_time_sum / configs.YEAR / wall_r, "unit": "sim-yr/s",
-            "attempts": int(str_.attempts), "wall_s": wall_r,
-            "slip_error_vs_f64": slip_error(ys64, ysr), "slip_window_s": Tslip,
-            "first_event_time_vs_f64": event_error(te64, ter),
+        st = ens.run(INF_T, max_attempts=cap)
+        walls.append(time.perf_counter() - t0)
+        log(f"[{name} step {k}] {walls[-1]:.3f} s; attempts {st.attempts} t_min {st.min_time:.4g} s "
+            f"t_max {st.max_time:.4g} s failed {st.n_failed}")
+    if st.n_failed:
+        raise RuntimeError(f"{st.n_failed} instances of {name} failed")
+    att1 = group_attempts(ens)
+    d_att = [a1 - a0 for a0, a1 in zip(att0, att1)]
+    sim_s = st.inst_time_sum - s0
+    steps_total, attempts_total = int(st.steps), int(st.attempts)
+    t_g, b_g = ens.time_gemm(20)
+    kb, var, tt = ens.tuning()
+    # per-phase split of an attempt (launched one by one, HIP events), from
+    # where the timed window ended
+    phases = ens.time_phases(20)
+    Tslip, Tevt = args.slip_years * configs.YEAR, args.event_years * configs.YEAR
+    _, ys = slip_at(ens, Tslip, f"{name} slip window")
+    te = first_events(ens, Tevt, f"{name} first events")
+    ens.close()
+    serr = slip_error(ref["ys"], ys)
+    ev = event_error(ref["te"], te)
+    wall = float(np.sum(walls))
+    return {"name": name, "value": sim_s / configs.YEAR / wall, "walls": walls, "wall": wall,
+            "sim_years": sim_s / configs.YEAR, "d_att": d_att,
+            "attempts": int(sum(a.sum() for a in d_att)), "steps_total": steps_total,
+            "attempts_total": attempts_total,
+            "t_window": [float(min(t.min() for t in time0)), float(st.min_time),
+                         float(st.max_time)],
+            "t_gemm": t_g, "b_gemm": b_g, "phases_us": {k: round(1e6 * v, 1) for k, v in phases.items()},
+            "tiling": {"kb_per_split": kb, "variant": var,
+                       "us_per_launch": [round(1e6 * x, 1) for x in tt]},
+            "slip_error_vs_f64": serr, "first_event_time_vs_f64": ev,
+            "meets_bar": meets_bar(serr, ev)}
+
+
+def bench_variant(args, eng, name, net, ref, flops):
+    """A secondary variant on the reference's schedule: R attempts per
+    instance from the initial states, timed once; GEMM timing; slip and
+    first-event errors against binary64."""
+    from unicycle_amd import configs
+    R = args.attempts
+    ens, _ = configs.ensemble_setup(eng, _prec(name))
+    if net is not None:
+        net.attach(ens)
+    t0 = time.perf_counter()
+    st = run_attempts(ens, R, name)
+    wall = time.perf_counter() - t0
+    t_g, b_g = ens.time_gemm(20)
+    kb, var, tt = ens.tuning()
+    _, ys = slip_at(ens, args.slip_years * configs.YEAR, f"{name} slip window")
+    te = first_events(ens, args.event_years * configs.YEAR, f"{name} first events")
+    ens.close()
+    serr = slip_error(ref["ys"], ys)
+    ev = event_error(ref["te"], te)
+    return {"value_from_t0": st.inst_time_sum / configs.YEAR / wall, "unit": "sim-yr/s",
+            "window": f"{R} attempts per instance from the initial states",
+            "us_per_lockstep_attempt": 1e6 * wall / R,
+            "speed_vs_f64_per_attempt": (ref["wall"] / R) / (wall / R),
+            "rejected_fraction": 1.0 - st.steps / max(1, st.attempts),
             "stage_gemm_us": 1e6 * t_g, "stage_gemm_GBps": b_g / t_g / 1e9,
             "stage_gemm_hbm_frac": b_g / t_g / HBM_PEAK,
             "stage_gemm_useful_TFLOPs": flops / t_g / 1e12,
-            "rejected_fraction": 1.0 - str_.steps / max(1, str_.attempts)}
-    reduced["mixed"]["dtype"] = "binary64 K for t_0 = K v_0, split-bf16 increments, f64 state"
-    reduced["split_bf16"]["dtype"] = "split-bf16 K (t_0 and increments), f64 state"
-    reduced["bf16_i8"]["dtype"] = ("bf16+i8 K (3 bytes per entry; t_0 and increments), "
-                                   "split-bf16 V, f64 state")
-    if net is not None:
-        reduced["bf16_i8_learned"]["dtype"] = reduced["bf16_i8"]["dtype"] + \
-            ", learned step-size proposals"
-        reduced["bf16_i8_learned"]["training"] = net_info
-        net.close()
+            "tiling": {"kb_per_split": kb, "variant": var},
+            "slip_error_vs_f64": serr, "slip_window_s": args.slip_years * configs.YEAR,
+            "first_event_time_vs_f64": ev, "meets_bar": meets_bar(serr, ev),
+            "dtype": VARIANTS[name][2]}
 
-    # plain binary64 device path: instance 0 of each group, R attempts
-    plain_time, plain_detail, cpu_inputs = 0.0, [], []
+
+def plain_per_attempt(args, info):
+    """The single-instance binary64 device path (streaming GEMV per
+    instance): seconds per RK45 attempt on instance 0 of every group from its
+    initial state."""
+    from unicycle_amd import Engine, configs
+    per, detail, cpu_inputs = [], [], []
     for gi, (m, d, y0) in enumerate(info):
         p = Engine(0)
         p.set_elastic(m.lambda_, m.mu)
         p.set_rectangles(*m.geometry())
         p.set_friction(*configs.instance_friction(m, d, 0))
         p.buildG()
-        steps0 = int(st64[gi][3][0])
-        res = p.integrate(y0[0], INF_T, max_steps=steps0)
-        per_att = res.wall_seconds / res.attempts
-        plain_time += per_att * float(np.sum(att64[gi]))
-        plain_detail.append({"patches": m.n, "attempts_inst0": int(res.attempts),
-                             "ensemble_attempts_inst0": int(att64[gi][0]),
-                             "us_per_attempt": 1e6 * per_att})
-        log(f"plain group {gi}: {res.attempts} attempts, {1e6 * per_att:.0f} us/attempt")
+        res = p.integrate(y0[0], INF_T, max_steps=args.plain_steps)
+        per.append(res.wall_seconds / res.attempts)
+        detail.append({"patches": m.n, "attempts": int(res.attempts),
+                       "us_per_attempt": 1e6 * per[-1]})
+        log(f"plain group {gi}: {res.attempts} attempts, {1e6 * per[-1]:.0f} us/attempt")
         if not args.no_cpu:
             cpu_inputs.append((gi, p.get_kernel(), m, d, y0[0]))
         p.close()
-    sim_years = st.inst_time_sum / configs.YEAR
-    value_plain = sim_years / plain_time
-
-    # headline: the fastest batched variant whose measured slip error (<= 1e-6
-    # at the pre-event time) and first-event times (within 0.1 %) meet the bar;
-    # binary64 meets it by construction
-    head = {"name": "f64", "value": value, "ms_per_step": 1e3 * t_total / args.steps,
-            "dtype": "f64", "desc": "binary64 K on MFMA"}
-    roof = {"bound": "mfma", "achieved": flops / t_gemm / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
-            "unit": "TFLOP/s", "frac": flops / t_gemm / FP64_MFMA_PEAK, "traffic": None,
-            "kernel": "ucy::gemm_f64_kernel (binary64 MFMA stage GEMM, all groups)",
-            "avg_launch_us": 1e6 * t_gemm, "flops_per_launch": flops,
-            "bytes_per_launch": gemm_bytes, "hbm_GBps": gemm_bytes / t_gemm / 1e9}
-    for name in reduced:
-        r = reduced[name]
-        ev = r["first_event_time_vs_f64"]
-        ok = r["slip_error_vs_f64"] <= 1e-6 and ev.get("event_only_in_one", 0) == 0 and \
-            ev.get("rel_err_max", 0.0) <= 1e-3
-        r["meets_bar"] = bool(ok)
-        if ok and r["value"] > head["value"]:
-            fmt = "bf16_i8" if name.startswith("bf16_i8") else "split_bf16"
-            kdesc = "bf16+i8 K (3 B/entry)" if fmt == "bf16_i8" else "split-bf16 K"
-            head = {"name": name, "value": r["value"], "ms_per_step": 1e3 * r["wall_s"],
-                    "dtype": f"f64 state, {kdesc} (fp32 MFMA accumulate)"
-                             + (" + binary64 step-start matvec" if name == "mixed" else "")
-                             + (" + learned step proposals (fp32 MLP)" if "learned" in name
-                                else ""),
-                    "desc": r["dtype"]}
-            gb = r["stage_gemm_GBps"]
-            roof = {"bound": "hbm", "achieved": gb, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": gb * 1e9 / HBM_PEAK,
-                    "traffic": MEASURED_TRAFFIC[fmt],
-                    "kernel": f"ucy::gemm_bf16_kernel<variant, {str(fmt == 'bf16_i8').lower()}> "
-                              f"({kdesc} MFMA stage GEMM, all groups)",
-                    "avg_launch_us": r["stage_gemm_us"],
-                    "bytes_per_launch": r["stage_gemm_GBps"] * 1e9 * r["stage_gemm_us"] * 1e-6,
-                    "useful_TFLOPs": r["stage_gemm_useful_TFLOPs"]}
-
-    out = {
-        "metric": METRIC,
-        "value": head["value"],
-        "unit": "sim-yr/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": head["dtype"],
-        "data": "synthetic (seeded friction draws and initial slip-rate perturbations)",
-        "config": {"workload": "batched ensemble: 4 groups x 64 instances of 1024/2048/4096/8192 "
-                               "2-km Okada patches (30 deg thrust), "
-                               f"{R} RK45 attempts per instance from the initial state, "
-                               f"{head['desc']}, eps 1e-6",
-                   "headline_variant": head["name"],
-                   "instances": n_inst, "groups": [[m, n] for m, n, _ in shapes],
-                   "attempts_per_instance": R, "gemm_tiling": tiling},
-        "vs_plain_fp64": head["value"] / value_plain,
-        "batched_f64": {"value": value, "unit": "sim-yr/s", "vs_plain_fp64": value / value_plain,
-                        "ms_per_step": 1e3 * t_total / args.steps},
-        "plain_fp64": {"value": value_plain, "unit": "sim-yr/s",
-                       "method": "single-instance device path, per-attempt cost of instance 0 "
-                                 "of each group times every instance's attempts",
-                       "groups": plain_detail},
-        **reduced,
-        "roofline": roof,
-        "f64_gemm_roofline": {"bound": "mfma", "achieved": flops / t_gemm / 1e12,
-                              "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s",
-                              "frac": flops / t_gemm / FP64_MFMA_PEAK,
-                              "avg_launch_us": 1e6 * t_gemm, "hbm_GBps": gemm_bytes / t_gemm / 1e9},
-        "detail": {"attempts": int(st.attempts), "accepted_steps": int(st.steps),
-                   "replays": int(st.replays), "simulated_years_total": sim_years,
-                   "min_time_s": st.min_time, "max_time_s": st.max_time,
-                   "rhs_per_s": 6.0 * st.attempts / (t_total / args.steps),
-                   "instances_with_event_by": {"years": args.event_years,
-                                               "count": int((te64 > 0).sum())},
-                   "setup_s": t_setup},
-    }
-    return out, (cpu_inputs, att64, sim_years)
+    return per, detail, cpu_inputs
 
 
 def bench_learned(args, plain_us_per_attempt):
@@ -731,23 +760,172 @@ def bench_copy(args):
     return {"workload": "copy", "bytes_read_per_launch": 8 * n, "bytes_written_per_launch": 8 * n}
 
 
+def bench_ensemble(args):
+    """The headline line: batched binary64 once (the precision reference),
+    the step-size network, the headline variant's timed steps, the plain
+    binary64 device path and the oracle on the host cores per attempt; then,
+    while the wall budget lasts, the other variants, the learned-proposal
+    comparison and the assembly figures."""
+    from unicycle_amd import Engine, configs
+    budget = Budget(args.budget)
+    R, W, K = args.attempts, args.warmup, args.steps
+    out = {"metric": METRIC, "value": None, "unit": "sim-yr/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": None,
+           "data": "synthetic (seeded friction draws and initial slip-rate perturbations)",
+           "config": {"workload": "batched ensemble: 4 geometry groups x 64 instances of "
+                                  "1024/2048/4096/8192 2-km Okada patches (30 deg thrust); "
+                                  f"a step = {R} RK45 attempts per instance continuing one run "
+                                  f"from the seeded initial states ({W} untimed, then {K} timed "
+                                  "steps); eps 1e-6",
+                      "attempts_per_step": R}}
+    eng = Engine(0)
+    shapes = []
+    ref = section(out, "batched_f64", budget, 0.0, lambda: bench_reference(args, eng, shapes))
+    if ref is None:
+        return out
+    flops = sum(2.0 * m * n * 64 for m, n, _ in shapes)
+    out["config"].update({"instances": sum(s[2] for s in shapes),
+                          "groups": [[m, n] for m, n, _ in shapes]})
+    name, net, net_info = args.headline, None, None
+    if VARIANTS[name][1]:
+        r = section(out, "step_net_training", budget, 0.0, lambda: train_step_net(eng, args))
+        if r is None:
+            name = "bf16_i8"
+        else:
+            net, net_info = r
+    head = section(out, "headline", budget, 0.0,
+                   lambda: bench_headline(args, eng, name, net, ref))
+    pl = section(out, "plain_fp64", budget, 0.0, lambda: plain_per_attempt(args, ref["info"]))
+
+    f64_costed = None
+    if head is not None:
+        # batched binary64 costed per lockstep attempt over the headline's window
+        f64_costed = head["sim_years"] / (K * ref["wall"])
+        out["config"]["headline_variant"] = name
+        out["config"]["gemm_tiling"] = head["tiling"]
+        out["config"]["window_s"] = {"t_min_start": head["t_window"][0],
+                                     "t_min_end": head["t_window"][1],
+                                     "t_max_end": head["t_window"][2]}
+        hd = {"value": head["value"], "unit": "sim-yr/s", "meets_bar": head["meets_bar"],
+              "slip_error_vs_f64": head["slip_error_vs_f64"],
+              "slip_window_s": args.slip_years * configs.YEAR,
+              "first_event_time_vs_f64": head["first_event_time_vs_f64"],
+              "event_horizon_s": args.event_years * configs.YEAR,
+              "step_walls_s": [round(w, 4) for w in head["walls"]],
+              "simulated_years_timed": head["sim_years"], "attempts_timed": head["attempts"],
+              "rhs_per_s": 6.0 * head["attempts"] / head["wall"],
+              "attempt_phases_us": head["phases_us"],
+              "us_per_lockstep_attempt": 1e6 * head["wall"] / (K * R),
+              "rejected_fraction_since_t0": 1.0 - head["steps_total"] / max(
+                  1, head["attempts_total"]),
+              "dtype": VARIANTS[name][2]}
+        if net_info is not None:
+            hd["training"] = net_info
+        out[name] = hd
+        if head["meets_bar"]:
+            out["value"] = head["value"]
+            out["ms_per_step"] = 1e3 * float(np.mean(head["walls"]))
+            out["dtype"] = "bf16+i8 K, fp32 accumulate, f64 state"
+            if name == "split_bf16":
+                out["dtype"] = "split-bf16 K, fp32 accumulate, f64 state"
+            gb = head["b_gemm"] / head["t_gemm"] / 1e9
+            fmt = "bf16_i8" if name.startswith("bf16_i8") else "split_bf16"
+            out["roofline"] = {
+                "bound": "hbm", "achieved": gb, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": gb * 1e9 / HBM_PEAK, "traffic": MEASURED_TRAFFIC.get(fmt),
+                "kernel": f"ucy::gemm_bf16_kernel<{head['tiling']['variant']}, "
+                          f"{str(fmt == 'bf16_i8').lower()}> (stage GEMM, all groups)",
+                "avg_launch_us": 1e6 * head["t_gemm"], "bytes_per_launch": head["b_gemm"],
+                "useful_TFLOPs": flops / head["t_gemm"] / 1e12,
+                "timing": "HIP events on the library stream, 20 launches"}
+        else:
+            out.setdefault("errors", {})["headline"] = (
+                f"{name} misses the bar (slip {head['slip_error_vs_f64']}, events "
+                f"{head['first_event_time_vs_f64']}); value is batched binary64")
+            out["value"] = f64_costed
+            out["ms_per_step"] = 1e3 * ref["wall"]
+            out["dtype"] = "f64"
+            out["config"]["headline_variant"] = "f64"
+    out["batched_f64"] = {
+        "value_from_t0": ref["st"].inst_time_sum / configs.YEAR / ref["wall"],
+        "window_from_t0": f"{R} attempts per instance from the initial states, timed once",
+        "us_per_lockstep_attempt": 1e6 * ref["wall"] / R,
+        "value_on_headline_window": f64_costed, "unit": "sim-yr/s",
+        "setup_s": ref["setup_s"], "kb_per_split": ref["kb"]}
+    out["f64_gemm_roofline"] = {
+        "bound": "mfma", "achieved": flops / ref["t_gemm"] / 1e12,
+        "peak": FP64_MFMA_PEAK / 1e12, "unit": "TFLOP/s", "frac": flops / ref["t_gemm"] / FP64_MFMA_PEAK,
+        "avg_launch_us": 1e6 * ref["t_gemm"], "hbm_GBps": ref["gemm_bytes"] / ref["t_gemm"] / 1e9}
+    cpu_inputs = []
+    if pl is not None and head is not None:
+        per, detail, cpu_inputs = pl
+        plain_wall = sum(p * float(a.sum()) for p, a in zip(per, head["d_att"]))
+        value_plain = head["sim_years"] / plain_wall
+        out["plain_fp64"] = {"value": value_plain, "unit": "sim-yr/s",
+                             "method": "single-instance binary64 device path (streaming GEMV), "
+                                       "seconds per RK45 attempt of instance 0 of each group "
+                                       "times every instance's attempts in the headline window",
+                             "groups": detail}
+        out["vs_plain_fp64"] = (out["value"] or 0.0) / value_plain
+        out["batched_f64"]["vs_plain_fp64"] = f64_costed / value_plain
+    if not args.no_cpu and cpu_inputs and head is not None:
+        cb = section(out, "cpu_baseline", budget, 0.0,
+                     lambda: cpu_baseline_ensemble(cpu_inputs, head["d_att"], head["sim_years"]))
+        if cb is not None:
+            out["cpu_baseline"] = cb
+    # ---- optional sections under the wall budget
+    bar_s = out["section_seconds"].get("headline", 60.0) - (head["wall"] if head else 0.0)
+    per_variant = max(20.0, bar_s) + 4.0 * R * 2.5e-3 + 10.0
+    extra = ["bf16_i8", "split_bf16", "mixed"] if not args.no_variants else []
+    for v in extra:
+        if v == name:
+            continue
+        r = section(out, v, budget, per_variant * (2.0 if v == "mixed" else 1.0),
+                    lambda v=v: bench_variant(args, eng, v, None, ref, flops))
+        if r is not None:
+            out[v] = r
+    if net is not None:
+        net.close()
+    if not args.no_learned and pl is not None:
+        r = section(out, "learned_step_proposal", budget, 150.0,
+                    lambda: bench_learned(args, 1e6 * pl[0][0]))
+        if r is not None:
+            out["learned_step_proposal"] = r
+    if not args.no_assembly:
+        r = section(out, "assembly", budget, 100.0,
+                    lambda: bench_assembly(args, cpu=not args.no_cpu))
+        if r is not None:
+            out["assembly"] = r
+    out["section_seconds"]["total"] = round(budget.elapsed(), 1)
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--attempts", type=int, default=2000,
+    ap.add_argument("--steps", type=int, default=5, help="timed steps of the headline variant")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed steps before them")
+    ap.add_argument("--attempts", type=int, default=1000,
                     help="RK45 attempts per instance in one step")
+    ap.add_argument("--headline", default="bf16_i8_learned", choices=sorted(VARIANTS))
+    ap.add_argument("--budget", type=float, default=420.0,
+                    help="wall seconds after which optional sections are skipped")
     ap.add_argument("--slip-years", type=float, default=0.05,
                     help="pre-event time of the slip-error comparison")
     ap.add_argument("--event-years", type=float, default=0.5,
                     help="horizon of the first-event comparison")
+    ap.add_argument("--plain-steps", type=int, default=400,
+                    help="accepted steps of the plain path's per-attempt timing")
     ap.add_argument("--workload", default="ensemble",
                     choices=["ensemble", "plain", "gemm", "rhs", "copy", "assembly"])
     ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16", "bf16_i8"],
                     help="stage-GEMM precision of --workload gemm")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="no learned-proposal section (the headline keeps its network)")
+    ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-assembly", action="store_true")
     ap.add_argument("--learned-years", type=float, default=0.25,
                     help="simulated window of the learned-proposal comparison")
@@ -764,14 +942,7 @@ def main():
               "assembly": lambda a: bench_assembly(a, cpu=not a.no_cpu)}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
-    out, (cpu_inputs, att64, sim_years) = bench_ensemble(args)
-    if not args.no_learned:
-        out["learned_step_proposal"] = bench_learned(
-            args, out["plain_fp64"]["groups"][0]["us_per_attempt"])
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_ensemble(cpu_inputs, att64, sim_years)
-    if not args.no_assembly:
-        out["assembly"] = bench_assembly(args, cpu=not args.no_cpu)
+    out = bench_ensemble(args)
     print(json.dumps(out), flush=True)
 
 

@@ -384,6 +384,14 @@ int ucy_ensemble_tuning(const ucy_ensemble *ens, int *kb_per_split, int *variant
 /* HIP-event timing of the all-groups stage GEMM (seconds per launch) and its
  * algorithmic HBM bytes per launch */
 int ucy_ensemble_time_gemm(ucy_ensemble *ens, int iters, double *avg_seconds, double *bytes);
+/* HIP-event timing of the phases of an RK45 attempt, launched one by one
+ * (not from the captured graph): phase_seconds[UCY_ENS_PHASES] receives the
+ * summed seconds of each phase over `attempts` attempts, in execution order:
+ * start, gemm 1, post 1, gemm 2, post 2, gemm 3, post 3, gemm 4, post 4,
+ * gemm 5, post 5, gemm 6, finish, control.  Every instance advances by
+ * `attempts` attempts (no end time), as run() would advance it. */
+#define UCY_ENS_PHASES 14
+int ucy_ensemble_time_phases(ucy_ensemble *ens, int attempts, double *phase_seconds);
 
 /* ---- learned RK45 step-size proposal (no counterpart in the reference) --- */
 /* An 8 -> 64 -> 64 -> 1 tanh perceptron (binary32, 4801 parameters) that

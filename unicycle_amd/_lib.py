@@ -148,10 +148,13 @@ _SIGS = {
     "ucy_ensemble_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), _pd]),
     "ucy_ensemble_set_tiling": (_i, [_vp, _i, _i]),
     "ucy_ensemble_time_gemm": (_i, [_vp, _i, _pd, _pd]),
+    "ucy_ensemble_time_phases": (_i, [_vp, _i, _pd]),
 }
 
 EXPORTED = tuple(_SIGS)
 GEMM_TUNE_ENTRIES = 42   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
+ENS_PHASES = ("start", "gemm 1", "post 1", "gemm 2", "post 2", "gemm 3", "post 3", "gemm 4",
+              "post 4", "gemm 5", "post 5", "gemm 6", "finish", "control")  # UCY_ENS_PHASES
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:

@@ -48,6 +48,7 @@ struct ucy_ensemble {
     bool use_mlp = false;
     std::vector<double> tune_seconds;
     double gemm_bytes = 0;
+    int *h_done = nullptr;   // pinned host copy of the finished-instance count
 };
 
 static int efail(ucy_ensemble *E, int code, const char *fmt, ...)
@@ -105,6 +106,8 @@ static void release(ucy_ensemble *E)
     for (double *p : E->slogs)
         if (p) (void)hipFree(p);
     E->slogs.clear();
+    if (E->h_done) (void)hipHostFree(E->h_done);
+    E->h_done = nullptr;
     E->d_tab = nullptr;
     E->built = false;
 }
@@ -382,6 +385,7 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     EHIP(hipMalloc(&dt, sizeof(EnsTable)));
     E->allocs.push_back(dt);
     E->d_tab = (EnsTable *)dt;
+    EHIP(hipHostMalloc((void **)&E->h_done, sizeof(int), hipHostMallocDefault));
     E->gemm_bytes = bytes;
     // ---- tune the split-K tiling on the device: time the all-groups GEMM
     E->tune_seconds.clear();
@@ -496,8 +500,7 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     EHIP(hipMemsetAsync(E->tab.all_done, 0, sizeof(int), c->stream));
     ens_launch_rearm(c->stream, E->d_tab, E->tab);
     EHIP(hipGetLastError());
-    int *h_done = nullptr;
-    EHIP(hipHostMalloc((void **)&h_done, sizeof(int), hipHostMallocDefault));
+    int *h_done = E->h_done;   // allocated with the ensemble, freed by release()
     EHIP(hipMemcpyAsync(h_done, E->tab.all_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     EHIP(hipStreamSynchronize(c->stream));
     auto t0 = std::chrono::steady_clock::now();
@@ -515,7 +518,6 @@ extern "C" int ucy_ensemble_run(ucy_ensemble *E, double t_end, int64_t max_attem
     ens_launch_commit(c->stream, E->d_tab, E->tab);
     EHIP(hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
-    (void)hipHostFree(h_done);
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->min_time = 1e300;
@@ -749,6 +751,69 @@ extern "C" int ucy_ensemble_time_gemm(ucy_ensemble *E, int iters, double *avg_se
     if (avg_seconds) *avg_seconds = 1e-3 * ms / iters;
     if (bytes) *bytes = E->gemm_bytes;
     return UCY_OK;
+}
+
+extern "C" int ucy_ensemble_time_phases(ucy_ensemble *E, int attempts, double *phase_seconds)
+{
+    if (!E || attempts <= 0 || !phase_seconds) return UCY_ERR_ARG;
+    if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
+    ucy_ctx *c = E->ctx;
+    EHIP(hipSetDevice(c->device));
+    // an unbounded run: no end time, no attempt cap, no event stop
+    E->tab.t_end = 1e300;
+    E->tab.max_attempts = (long long)1 << 62;
+    E->tab.stop_at_end = 0;
+    E->tab.stop_at_event = 0;
+    E->tab.event_vmax = E->event_vmax;
+    E->tab.mlp = E->use_mlp ? E->d_mlp : nullptr;
+    E->tab.mlp_norm = E->mlp_norm;
+    if (upload_table(E)) return UCY_ERR_HIP;
+    EHIP(hipMemsetAsync(E->tab.all_done, 0, sizeof(int), c->stream));
+    ens_launch_rearm(c->stream, E->d_tab, E->tab);
+    EHIP(hipGetLastError());
+    hipEvent_t ev[UCY_ENS_PHASES + 1];
+    for (int i = 0; i <= UCY_ENS_PHASES; i++) {
+        hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; j++) (void)hipEventDestroy(ev[j]);
+            return efail(E, UCY_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
+    for (int i = 0; i < UCY_ENS_PHASES; i++) phase_seconds[i] = 0.0;
+    int rc = UCY_OK;
+    for (int a = 0; a < attempts && rc == UCY_OK; a++) {
+        int p = 0;
+        (void)hipEventRecord(ev[p++], c->stream);
+        ens_launch_start(c->stream, E->d_tab, E->tab);
+        (void)hipEventRecord(ev[p++], c->stream);
+        for (int s = 1; s <= 5; s++) {
+            ens_launch_gemm(c->stream, E->d_tab, E->tab, s);
+            (void)hipEventRecord(ev[p++], c->stream);
+            ens_launch_post_stage(c->stream, E->d_tab, E->tab, s);
+            (void)hipEventRecord(ev[p++], c->stream);
+        }
+        ens_launch_gemm(c->stream, E->d_tab, E->tab, 6);
+        (void)hipEventRecord(ev[p++], c->stream);
+        ens_launch_post_finish(c->stream, E->d_tab, E->tab);
+        (void)hipEventRecord(ev[p++], c->stream);
+        ens_launch_control(c->stream, E->d_tab, E->tab);
+        (void)hipEventRecord(ev[p++], c->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventSynchronize(ev[UCY_ENS_PHASES]);
+        for (int i = 0; i < UCY_ENS_PHASES && e == hipSuccess; i++) {
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            phase_seconds[i] += 1e-3 * ms;
+        }
+        if (e != hipSuccess) rc = efail(E, UCY_ERR_HIP, "phase timing: %s", hipGetErrorString(e));
+    }
+    if (rc == UCY_OK) {
+        ens_launch_commit(c->stream, E->d_tab, E->tab);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = efail(E, UCY_ERR_HIP, "commit: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i <= UCY_ENS_PHASES; i++) (void)hipEventDestroy(ev[i]);
+    return rc;
 }
 
 namespace ucy {

@@ -204,3 +204,11 @@ class Ensemble:
         s, b = C.c_double(), C.c_double()
         self._chk(self._L.ucy_ensemble_time_gemm(self._h, int(iters), C.byref(s), C.byref(b)))
         return s.value, b.value
+
+    def time_phases(self, attempts: int = 20) -> dict:
+        """Seconds per attempt of each phase of an RK45 attempt (kernels
+        launched one by one, HIP events between them).  Advances every
+        instance by `attempts` attempts."""
+        t = np.zeros(len(_lib.ENS_PHASES))
+        self._chk(self._L.ucy_ensemble_time_phases(self._h, int(attempts), _p(t)))
+        return {k: float(v) / attempts for k, v in zip(_lib.ENS_PHASES, t)}
