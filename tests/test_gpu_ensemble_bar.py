@@ -1,0 +1,153 @@
+"""The headline bar on an uneven multi-group ensemble, through the C ABI.
+
+Three geometry groups of different sizes and instance counts (8 x 4, 16 x 8
+and 16 x 16 two-km Okada patches with 64, 37 and 13 instances) run in one
+ensemble, so every launch of the attempt graph covers groups of different
+shapes and partly filled instance columns.
+
+- Binary64: selected instances of every group against the oracle's own
+  integration (orc.System.integrate, the reference loop of
+  ratestate.f90:307-328 with RKQSm/RKCKm of ode45.f90:51-202): the same
+  accept/reject sequence (steps and attempts), the same step sizes to 1e-6
+  and the state to 1e-6 relative, over a window that ends before any first
+  event.  Only the summation order of the matvec differs.
+- bf16+i8 K, with and without the learned step-size proposals: the bar of
+  the headline figure against the binary64 ensemble, i.e. slip within 1e-6
+  (relative, all instances) at a pre-event time every instance lands on
+  exactly, first-event times (step-start vMax above 1e-3 m/s) within 0.1 %,
+  and no instance with an event in only one of the two runs.
+"""
+import numpy as np
+import pytest
+
+import orc
+from unicycle_amd import configs, stepnet
+from unicycle_amd.ensemble import Ensemble, PREC_BF16_I8, PREC_F64
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = (((8, 4), 64), ((16, 8), 37), ((16, 16), 13))
+T_SLIP = 0.05 * configs.YEAR     # pre-event time of the slip comparison
+T_EVENT = 0.5 * configs.YEAR     # horizon of the first-event comparison
+MAX_ATTEMPTS = 400000
+
+
+def _group(nxz, n_inst, seed):
+    model = configs.thrust_fault(nx=nxz[0], nz=nxz[1], ellipse=None)
+    draws = configs.ensemble_instances(model, n_inst, seed)
+    zeros = np.zeros((3 * model.n, 2 * model.n))
+    y0 = []
+    for i in range(n_inst):
+        s = orc.System(configs.instance_friction(model, draws, i), zeros)
+        y0.append(configs.perturbed_initial_state(model, s.init_state(), int(draws["seeds"][i])))
+    return model, draws, np.array(y0)
+
+
+@pytest.fixture(scope="module")
+def groups():
+    return [_group(nxz, n, 300 + g) for g, (nxz, n) in enumerate(GROUPS)]
+
+
+def _ensemble(eng, precision, groups, net=None):
+    ens = Ensemble(eng, precision)
+    for model, d, y0 in groups:
+        ens.add_group(model, d["a"], d["b"], d["sig"], d["L"], y0)
+    ens.build()
+    if net is not None:
+        net.attach(ens)
+    return ens
+
+
+def _slip(ens):
+    ens.reset()
+    ens.set_stop_at_end(True)
+    ens.set_events(1e-3, False)
+    st = ens.run(T_SLIP, max_attempts=MAX_ATTEMPTS)
+    ens.set_stop_at_end(False)
+    assert st.n_failed == 0 and st.min_time >= T_SLIP
+    ys = [ens.state(g)[0] for g in range(len(ens.groups))]
+    return np.concatenate([np.hypot(y[:, 0::7], y[:, 1::7]).ravel() for y in ys]), ys
+
+
+def _events(ens):
+    ens.reset()
+    ens.set_events(1e-3, True)
+    st = ens.run(T_EVENT, max_attempts=MAX_ATTEMPTS)
+    ens.set_events(1e-3, False)
+    assert st.n_failed == 0
+    return np.concatenate([ens.first_events(g) for g in range(len(ens.groups))])
+
+
+def test_f64_ensemble_matches_the_oracle_per_instance(engine_factory, groups):
+    eng = engine_factory()
+    ens = _ensemble(eng, PREC_F64, groups)
+    cap = 5000
+    ens.set_step_log(cap)
+    ens.reset()
+    ens.set_stop_at_end(False)        # the reference loop steps past the interval
+    st = ens.run(T_SLIP, max_attempts=MAX_ATTEMPTS)
+    assert st.n_failed == 0 and st.min_time >= T_SLIP
+    checked = 0
+    for g, (model, d, y0) in enumerate(groups):
+        y, t, dt, steps, att = ens.state(g)
+        assert np.max(y[:, 6::7]) < -3.0, "the window holds an event"
+        logs = ens.step_log(g)
+        K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
+        n_inst = y0.shape[0]
+        for i in sorted({0, n_inst // 2, n_inst - 1}):
+            sysm = orc.System(configs.instance_friction(model, d, i), K, nthreads=8)
+            y_ref, so, tl, dl, rc = sysm.integrate(y0[i], T_SLIP, log_cap=cap)
+            assert rc == 0
+            assert steps[i] == so.steps and att[i] == so.attempts, \
+                f"group {g} instance {i}: steps {steps[i]}/{so.steps} attempts {att[i]}/{so.attempts}"
+            assert steps[i] <= cap
+            np.testing.assert_allclose(logs[i]["dt"], dl, rtol=1e-6)
+            np.testing.assert_allclose(logs[i]["t"], tl, rtol=1e-6)
+            assert t[i] == pytest.approx(so.time, rel=1e-6)
+            np.testing.assert_allclose(y[i], y_ref, rtol=1e-6, atol=1e-9 * np.abs(y_ref).max())
+            checked += 1
+    assert checked == 9
+    ens.close()
+
+
+def _step_net(eng, groups):
+    """A step-size network trained on the plain binary64 integrator's
+    histories of four instances of the 16 x 8 group (hand-written GPU
+    training kernels)."""
+    from unicycle_amd import Engine
+    model, d, y0 = groups[1]
+    ic = Engine(0)
+    X, T = stepnet.training_data(ic, model, d, range(4), 1500, y0)
+    ic.close()
+    net = stepnet.StepNet(eng, seed=0)
+    net.fit_normalization(X, T)
+    net.train(X, T, epochs=40, batch=4096, lr=2e-3, seed=1)
+    return net
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_bf16_i8_ensemble_meets_the_headline_bar(engine_factory, groups, learned):
+    eng = engine_factory()
+    ref = _ensemble(eng, PREC_F64, groups)
+    s64, ys64 = _slip(ref)
+    for y in ys64:
+        assert np.max(y[:, 6::7]) < -3.0, "the slip window holds an event"
+    te64 = _events(ref)
+    ref.close()
+    net = _step_net(eng, groups) if learned else None
+    ens = _ensemble(eng, PREC_BF16_I8, groups, net)
+    sr, _ = _slip(ens)
+    ter = _events(ens)
+    ens.close()
+    if net is not None:
+        net.close()
+    err = np.linalg.norm(sr - s64) / np.linalg.norm(s64)
+    both = (te64 > 0) & (ter > 0)
+    only_one = int(((te64 > 0) ^ (ter > 0)).sum())
+    rel = np.abs(ter[both] - te64[both]) / te64[both] if both.any() else np.zeros(0)
+    print(f"bf16+i8{' + network' if learned else ''}: slip error {err:.3e} at {T_SLIP:.3g} s; "
+          f"{int(both.sum())} first events, max rel {rel.max() if rel.size else 0:.3e}, "
+          f"{only_one} in one run only")
+    assert err <= 1e-6
+    assert only_one == 0
+    assert rel.size == 0 or rel.max() <= 1e-3

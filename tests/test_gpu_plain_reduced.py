@@ -19,9 +19,9 @@ from unicycle_amd import PREC_F64, PREC_MIXED, PREC_SPLIT_BF16, configs
 pytestmark = pytest.mark.gpu
 
 
-def _model():
+def _model(npower=1.0):
     fault = configs.thrust_fault(nx=8, nz=4, ellipse=None)
-    vol = configs.volume_grid(4, 4, 2, size=4000.0, top=20e3, origin=(0.0, 0.0))
+    vol = configs.volume_grid(4, 4, 2, size=4000.0, top=20e3, origin=(0.0, 0.0), npower=npower)
     return fault, vol
 
 
@@ -63,9 +63,12 @@ def _slip_at(eng, res, T, n):
     return np.hypot(ss, sd)
 
 
-@pytest.mark.parametrize("prec", [PREC_SPLIT_BF16, PREC_MIXED])
-def test_reduced_integrate_tracks_f64(engine_factory, prec):
-    fault, vol = _model()
+@pytest.mark.parametrize("prec,npower", [(PREC_SPLIT_BF16, 1.0), (PREC_MIXED, 1.0),
+                                         (PREC_SPLIT_BF16, 3.0), (PREC_MIXED, 3.0)])
+def test_reduced_integrate_tracks_f64(engine_factory, prec, npower):
+    """Maxwell (n = 1) and power-law (n = 3) volumes: the reduced K against
+    binary64 on the same run."""
+    fault, vol = _model(npower)
     T = 5 * configs.YEAR
     hmax = T / 4000
     out = {}
@@ -79,6 +82,7 @@ def test_reduced_integrate_tracks_f64(engine_factory, prec):
         out[p] = (res, _slip_at(eng, res, T, fault.n))
     (r64, s64), (rr, sr) = out[PREC_F64], out[prec]
     err = np.linalg.norm(sr - s64) / np.linalg.norm(s64)
-    print(f"prec {prec}: attempts {rr.attempts} vs {r64.attempts}, slip error {err:.3e}")
+    print(f"prec {prec} n {npower}: attempts {rr.attempts} vs {r64.attempts}, "
+          f"slip error {err:.3e}")
     assert rr.attempts <= 1.2 * r64.attempts + 20
     assert err < (1e-6 if prec == PREC_MIXED else 1e-5)

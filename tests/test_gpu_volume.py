@@ -76,9 +76,12 @@ def test_strainvolume_general_eigenstrain(engine_factory):
     assert np.linalg.norm(t - o) <= 1e-11 * np.linalg.norm(o)
 
 
-def _viscous_model(nx=6, nz=3, vn=(3, 3, 2)):
+def _viscous_model(nx=6, nz=3, vn=(3, 3, 2), npower=1.0):
+    """npower = 1: Maxwell; npower = 3: power law with gammadot0 from a
+    seeded lognormal about the Maxwell value (the viscoelastic model's two
+    rheologies)."""
     fault = configs.thrust_fault(nx=nx, nz=nz, ellipse=None)
-    vol = configs.volume_grid(*vn, size=4000.0, top=20e3, origin=(0.0, 0.0))
+    vol = configs.volume_grid(*vn, size=4000.0, top=20e3, origin=(0.0, 0.0), npower=npower)
     return fault, vol
 
 
@@ -112,11 +115,14 @@ def test_buildG_fault_and_volumes_matches_oracle(engine_factory):
     assert np.all(np.abs(K[:, vc] - Kref[:, vc]) <= 1e-9 * rowscale)
 
 
-def test_viscous_rhs_and_steps_match_oracle(engine_factory):
-    """Fault + Maxwell volumes: initial state (viscouscycles.f90:710-803), the
-    volume branch of odefun (933-1080) and 100 device-resident adaptive steps
-    against the oracle."""
-    fault, vol = _viscous_model()
+@pytest.mark.parametrize("npower", [1.0, 3.0])
+def test_viscous_rhs_and_steps_match_oracle(engine_factory, npower):
+    """Fault + Maxwell (n = 1) or power-law (n = 3) volumes: initial state
+    (viscouscycles.f90:710-803), the volume branch of odefun (933-1080,
+    sII^(n-1) at :963-974) and 100 device-resident adaptive steps against
+    the oracle."""
+    fault, vol = _viscous_model(npower=npower)
+    assert np.all(vol.npower == npower)
     eng = _engine(engine_factory, fault, vol)
     K = orc.buildG(orc.Mesh(fault.lambda_, fault.mu, rect=fault.geometry(), vol=vol.geometry()),
                    nthreads=8)

@@ -411,10 +411,180 @@ def test_rkqs_controller_branches():
     assert hnext == 3e-6
 
 
+def _scaled_error_case(lam=-2.0, h=0.05):
+    """y' = lam y from y = 1: the embedded error of the (pinned) Cash-Karp
+    step at h, so that a chosen yscal fixes errmax/epsilon."""
+    f = lambda y: lam * y
+    y = np.array([1.0])
+    d = f(y)
+    yerr = orc.rkck(1, y, d, h, f)[1]
+    assert abs(yerr[0]) > 0
+    return f, y, d, abs(yerr[0])
+
+
+@pytest.mark.parametrize("ratio", [0.5, 0.03, 3.0e-4])
+def test_rkqs_growth_by_value(ratio):
+    """Accepted step with errmax/epsilon = r: the next step is
+    0.7 h r^-0.2 when r > 1.89e-4 (ode45.f90:92-94; SAFETY 0.7, PGROW -0.2,
+    ERRCON 1.89e-4 of Numerical Recipes 2e p. 719), written here from
+    those published constants."""
+    eps, h = 1e-6, 0.05
+    f, y, d, e = _scaled_error_case(h=h)
+    yscal = np.array([e / (ratio * eps)])
+    r = e / yscal[0] / eps            # the ratio as the controller forms it
+    rc, t, y1, hdid, hnext, att = orc.rkqs(1, 0.0, y, d, yscal, h, f, epsilon=eps)
+    assert rc == 0 and att == 1 and hdid == h and t == h
+    assert hnext == pytest.approx(0.7 * h * r ** -0.2, rel=1e-14)
+
+
+def test_rkqs_tiny_error_grows_tenfold():
+    """errmax/epsilon below ERRCON = 1.89e-4: hnext = 10 h (ode45.f90:95)."""
+    eps, h = 1e-6, 0.05
+    f, y, d, e = _scaled_error_case(h=h)
+    for ratio in (1.0e-4, 1.0e-9):
+        rc, t, y1, hdid, hnext, att = orc.rkqs(1, 0.0, y, d, np.array([e / (ratio * eps)]), h,
+                                               f, epsilon=eps)
+        assert rc == 0 and att == 1 and hnext == 10.0 * h
+
+
+@pytest.mark.parametrize("ratio", [8.0, 40.0])
+def test_rkqs_shrink_by_value(ratio):
+    """Rejected attempt with errmax/epsilon = r > 1: the retry takes
+    h1 = 0.7 h r^-0.15 (ode45.f90:81-82, PSHRNK -0.15) with the same yscal;
+    its own error ratio r1 (from the pinned Cash-Karp step at h1) then
+    decides acceptance and the growth 0.7 h1 r1^-0.2."""
+    eps, h = 1e-6, 0.05
+    f, y, d, e = _scaled_error_case(h=h)
+    yscal = np.array([e / (ratio * eps)])
+    r = e / yscal[0] / eps
+    h1 = h * 0.7 * r ** -0.15
+    e1 = abs(orc.rkck(1, y, d, h1, f)[1][0])
+    r1 = e1 / yscal[0] / eps
+    assert r1 <= 1.0          # the case is built so that the first retry passes
+    rc, t, y1, hdid, hnext, att = orc.rkqs(1, 0.0, y, d, yscal, h, f, epsilon=eps)
+    assert rc == 0 and att == 2
+    assert hdid == pytest.approx(h1, rel=1e-15)
+    assert hnext == pytest.approx(0.7 * hdid * r1 ** -0.2, rel=1e-14)
+    assert y1[0] == pytest.approx(math.exp(-2.0 * hdid), rel=1e-6)
+
+
 def test_rkqs_underflow_is_reported():
+    """A step that cannot shrink below the resolution of t: the reference
+    stops with 'stepsize underflow in RKQS' (ode45.f90:86-90); the oracle
+    returns -1 and leaves y unchanged."""
+    f = lambda y: -2.0 * y
+    y = np.array([1.0])
+    rc, t, y1, hdid, hnext, att = orc.rkqs(1, 1.0e20, y, f(y), np.array([1e-300]), 1.0, f)
+    assert rc == -1 and t == 1.0e20 and y1[0] == 1.0 and att >= 1
+
+
+def test_rkqs_nan_error_is_accepted_as_in_the_reference():
+    """A NaN error norm is not greater than 1 (ode45.f90:81), so the
+    reference accepts the attempt, and NaN > ERRCON is false, so hnext =
+    10 h.  The oracle keeps that behaviour; the device ensemble rejects such
+    attempts instead (a documented deviation)."""
     f = lambda y: np.array([np.nan])
-    rc, *_ = orc.rkqs(1, 0.0, np.array([1.0]), np.array([1.0]), np.array([1.0]), 1.0, f)
-    assert rc == -1 or rc == 0   # NaN errmax is ignored by MAXVAL semantics
+    rc, t, y1, hdid, hnext, att = orc.rkqs(1, 0.0, np.array([1.0]), np.array([1.0]),
+                                           np.array([1.0]), 1.0, f)
+    assert rc == 0 and att == 1 and hdid == 1.0 and hnext == 10.0
+
+
+# ------------------------------------------------- volumes in odefun -------
+def _volume_system(npower, edot_bg, gammadot0=2.5e-3, Q=0.0, nx=2, ny=1, nz=1):
+    """Volumes only (no patches): the oracle's viscouscycles system."""
+    vm = configs.volume_grid(nx=nx, ny=ny, nz=nz, size=4000.0, npower=npower)
+    n = vm.n
+    K = orc.buildG(orc.Mesh(30e3, 30e3, vol=vm.geometry()))
+    empty = [np.zeros(0) for _ in range(9)]
+    eb = np.tile(np.asarray(edot_bg, float), (n, 1))
+    rheo = (gammadot0 * np.ones(n), npower * np.ones(n), eb, np.ones(n), Q * np.ones(n),
+            8.314 * np.ones(n), 873.0 * np.ones(n))
+    return orc.System(empty, K, rheo=rheo, ic_variant=1), K, n
+
+
+def _tensor(c):
+    """Symmetric 3x3 tensor of the six components (11, 12, 13, 22, 23, 33)."""
+    return np.array([[c[0], c[1], c[2]], [c[1], c[3], c[4]], [c[2], c[4], c[5]]])
+
+
+def _components(T):
+    return np.array([T[0, 0], T[0, 1], T[0, 2], T[1, 1], T[1, 2], T[2, 2]])
+
+
+# a trace-free background strain rate with every component set
+EDOT_DEV = np.array([2.0e-15, 1.0e-15, -0.5e-15, -3.0e-15, 0.7e-15, 1.0e-15])
+
+
+@pytest.mark.parametrize("npower", [1.0, 3.0])
+def test_volume_initial_state_is_stationary(npower):
+    """With Gm = 1 and Q = 0 the initial stress (viscouscycles.f90:769-803)
+    is the stress that flows at the background rate under the power law of
+    the right-hand side (:963-974), for any exponent n: the strain rate is
+    the background rate, v = e - e_bg vanishes and so does the stress rate
+    K v.  Holds only if the initial-stress invariant, the deviatoric split,
+    the factor 2 of the off-diagonal terms of sII, the exponents 1/n and
+    n - 1 and the v = e - e_bg wiring all agree."""
+    s, K, n = _volume_system(npower, EDOT_DEV)
+    y = s.init_state()
+    dydt, vmax, v, t = s.odefun(y)
+    q = dydt.reshape(n, 12)
+    scale = np.abs(EDOT_DEV).max()
+    assert np.allclose(q[:, :6], EDOT_DEV[None, :], rtol=0, atol=1e-12 * scale)
+    assert np.abs(v).max() <= 1e-12 * scale
+    assert np.abs(q[:, 6:]).max() <= 1e-12 * scale * np.abs(K).sum(axis=1).max()
+    assert np.all(y.reshape(n, 12)[:, :6] == 0.0)
+
+
+@pytest.mark.parametrize("npower", [1.0, 3.0])
+def test_volume_initial_stress_follows_the_flow_law(npower):
+    """The initial stress (viscouscycles.f90:769-803) is deviatoric, coaxial
+    with the background rate, and its second invariant J2^(1/2) satisfies
+    gammadot0 exp(-Q/(R T)) sII^n = Gm^n |e_bg| (the Arrhenius factor enters
+    the initial state only, not the right-hand side, :783 vs :967)."""
+    Q = 1.2e5
+    s, K, n = _volume_system(npower, EDOT_DEV, Q=Q)
+    y = s.init_state().reshape(n, 12)
+    S = _tensor(y[0, 6:])
+    assert abs(np.trace(S)) <= 1e-14 * np.abs(S).max()
+    Ed = _tensor(EDOT_DEV)
+    eII = math.sqrt(0.5 * np.sum(Ed * Ed))
+    sII = math.sqrt(0.5 * np.sum(S * S))
+    assert np.allclose(S / sII, Ed / eII, rtol=0, atol=1e-14)
+    assert 2.5e-3 * math.exp(-Q / (8.314 * 873.0)) * sII ** npower == pytest.approx(eII, rel=1e-12)
+
+
+@pytest.mark.parametrize("which", ["pure_shear", "uniaxial", "general"])
+def test_volume_rhs_hand_values_power_law(which):
+    """One volume at n = 3: strain rate e = gammadot0 sII^(n-1) s' with the
+    deviatoric stress s' = sigma - tr(sigma)/3 I and sII = sqrt(s':s'/2), in
+    tensor form (numpy), against closed forms where they exist: pure shear
+    sigma12 = tau gives sII = tau, e12 = g tau^3; uniaxial sigma11 = S gives
+    sII = S/sqrt(3), e11 = g (S^2/3) (2S/3).  The stress rate is the kernel
+    row times v = e - e_bg (viscouscycles.f90:1072-1080)."""
+    g0, npow = 2.5e-3, 3.0
+    s, K, n = _volume_system(npow, EDOT_DEV, gammadot0=g0, nx=1)
+    assert n == 1 and K.shape == (6, 6)
+    sig = {"pure_shear": [0.0, 7.0, 0.0, 0.0, 0.0, 0.0],
+           "uniaxial": [9.0, 0.0, 0.0, 0.0, 0.0, 0.0],
+           "general": [4.0, -1.5, 2.0, -6.0, 0.5, 3.0]}[which]
+    y = np.concatenate([np.zeros(6), sig])
+    dydt, vmax, v, t = s.odefun(y)
+    Sg = _tensor(sig)
+    dev = Sg - np.trace(Sg) / 3.0 * np.eye(3)
+    sII = math.sqrt(0.5 * np.sum(dev * dev))
+    e = g0 * sII ** (npow - 1.0) * _components(dev)
+    if which == "pure_shear":
+        assert sII == pytest.approx(7.0, rel=1e-15)
+        assert e[1] == pytest.approx(g0 * 7.0 ** 3, rel=1e-14)
+    if which == "uniaxial":
+        assert sII == pytest.approx(9.0 / math.sqrt(3.0), rel=1e-15)
+        assert e[0] == pytest.approx(g0 * 27.0 * 6.0, rel=1e-14)
+        assert e[3] == pytest.approx(-g0 * 27.0 * 3.0, rel=1e-14)
+    np.testing.assert_allclose(dydt[:6], e, rtol=1e-13, atol=1e-16 * np.abs(e).max())
+    np.testing.assert_allclose(v, e - EDOT_DEV, rtol=1e-13, atol=1e-16 * np.abs(e).max())
+    want = K @ (e - EDOT_DEV)
+    np.testing.assert_allclose(dydt[6:], want, rtol=1e-12,
+                               atol=1e-14 * np.abs(K).sum(axis=1).max() * np.abs(e).max())
 
 
 # ---------------------------------------------------------- strain volumes --

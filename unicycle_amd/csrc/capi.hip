@@ -274,6 +274,21 @@ extern "C" int ucy_reference_system_strainvolume(int ns, const double *x, const 
 // ---------------------------------------------------------------------------
 // model definition
 // ---------------------------------------------------------------------------
+static int upload_params(ucy_ctx *c);
+
+// set-up calls place the friction and rheology on the device as soon as the
+// model is complete (every patch has friction, every volume a rheology), so
+// that entry points which may not allocate find them there
+static int upload_if_complete(ucy_ctx *c)
+{
+    const int np = c->nrect + c->ntri;
+    if (np + c->nvol == 0) return UCY_OK;
+    if (np && (!c->have_friction || (int)c->Vpl.size() != np)) return UCY_OK;
+    if (c->nvol && (!c->have_rheology || (int)c->gd0.size() != c->nvol)) return UCY_OK;
+    HIPCHK(hipSetDevice(c->device));
+    return upload_params(c);
+}
+
 extern "C" int ucy_set_elastic(ucy_ctx *c, double lambda, double mu)
 {
     if (!c) return UCY_ERR_ARG;
@@ -301,7 +316,7 @@ extern "C" int ucy_set_rectangles(ucy_ctx *c, int ns, const double *x, const dou
     c->rstrike.assign(strike, strike + ns);
     c->rdip.assign(dip, dip + ns);
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 // the 2-D antiplane model (fortran/2d/antiplane): ns segments with upper
@@ -327,7 +342,7 @@ extern "C" int ucy_set_antiplane(ucy_ctx *c, double mu, int ns, const double *x,
     c->rdip.assign(dip, dip + ns);
     c->ntri = c->nvol = c->nve = 0;
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 // antiplane.f90:81-109
@@ -366,7 +381,7 @@ extern "C" int ucy_set_triangles(ucy_ctx *c, int ns, const int *i1, const int *i
     c->i3.assign(i3, i3 + ns);
     c->vert.assign(v, v + 3 * (size_t)nVe);
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 extern "C" int ucy_set_volumes(ucy_ctx *c, int ns, const double *x, const double *length,
@@ -385,7 +400,7 @@ extern "C" int ucy_set_volumes(ucy_ctx *c, int ns, const double *x, const double
     c->vW.assign(width, width + ns);
     c->vstrike.assign(strike, strike + ns);
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 extern "C" int ucy_set_friction(ucy_ctx *c, int np, const double *Vpl, const double *rake,
@@ -405,7 +420,7 @@ extern "C" int ucy_set_friction(ucy_ctx *c, int np, const double *Vpl, const dou
     c->damping.assign(damping, damping + np);
     c->have_friction = true;
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 extern "C" int ucy_set_rheology(ucy_ctx *c, int nv, const double *gammadot0,
@@ -422,7 +437,7 @@ extern "C" int ucy_set_rheology(ucy_ctx *c, int nv, const double *gammadot0,
     c->To.assign(To, To + nv);
     c->have_rheology = true;
     c->params_dirty = true;
-    return UCY_OK;
+    return upload_if_complete(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -878,10 +893,26 @@ static int upload_params(ucy_ctx *c)
         H[i].npower = c->npow[i];
         for (int k = 0; k < 6; k++) H[i].ebg[k] = c->ebg[6 * i + k];
     }
-    if (c->d_fr) HIPCHK(hipFree(c->d_fr));
-    if (c->d_rh) HIPCHK(hipFree(c->d_rh));
-    HIPCHK(hipMalloc((void **)&c->d_fr, sizeof(Friction) * (np + 1)));
-    HIPCHK(hipMalloc((void **)&c->d_rh, sizeof(Rheology) * (c->nvol + 1)));
+    // the device arrays are reused while they are large enough; a captured
+    // attempt holds their addresses, so a reallocation drops it
+    if ((c->fr_cap < np + 1 || c->rh_cap < c->nvol + 1) && c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    if (c->fr_cap < np + 1) {
+        if (c->d_fr) HIPCHK(hipFree(c->d_fr));
+        c->d_fr = nullptr;
+        c->fr_cap = 0;
+        HIPCHK(hipMalloc((void **)&c->d_fr, sizeof(Friction) * (np + 1)));
+        c->fr_cap = np + 1;
+    }
+    if (c->rh_cap < c->nvol + 1) {
+        if (c->d_rh) HIPCHK(hipFree(c->d_rh));
+        c->d_rh = nullptr;
+        c->rh_cap = 0;
+        HIPCHK(hipMalloc((void **)&c->d_rh, sizeof(Rheology) * (c->nvol + 1)));
+        c->rh_cap = c->nvol + 1;
+    }
     HIPCHK(hipMemcpyAsync(c->d_fr, F.data(), sizeof(Friction) * (np + 1), hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipMemcpyAsync(c->d_rh, H.data(), sizeof(Rheology) * (c->nvol + 1),

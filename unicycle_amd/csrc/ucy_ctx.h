@@ -60,6 +60,7 @@ struct ucy_ctx {
     // device per-element data
     Friction *d_fr = nullptr;
     Rheology *d_rh = nullptr;
+    int fr_cap = 0, rh_cap = 0;   // entries the two device arrays hold
     bool params_dirty = true;
     // work buffers
     double *work = nullptr;
