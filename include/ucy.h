@@ -235,6 +235,21 @@ int ucy_init_state(ucy_ctx *ctx, int variant, double *y);
 /* odefun(n,time,y,dydt) of ratestate.f90:624-778 (+ viscouscycles.f90:933-1080);
  * vmax receives the maximum slip velocity (may be NULL). */
 int ucy_odefun(ucy_ctx *ctx, int n, double time, const double *y, double *dydt, double *vmax);
+/* No counterpart in the reference: odefun for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and arithmetic as ucy_odefun, on device
+ * buffers: y_dev (n entries) in, dydt_dev (n) out, vmax_dev (1 entry, may be
+ * NULL) out.  It allocates nothing: the slip-rate vector and the vMax
+ * accumulator live in a static 1 MiB device array per GPU, so the kernel may
+ * have at most 131040 columns (the n of ucy_kernel_shape); larger models get
+ * UCY_ERR_LIMIT.  The kernel must be built and friction/rheology set for
+ * every element (the set-up calls place them on the device).  stream is a
+ * hipStream_t: NULL runs on the context's stream and waits for the result;
+ * otherwise the call only enqueues on that stream.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other contexts or
+ * streams) race. */
+int ucy_odefun_dev_static(ucy_ctx *ctx, int n, const double *y_dev, double *dydt_dev,
+                          double *vmax_dev, void *stream);
 /* RKCKm(n,t,yin,dydt,h,yout,yerr) of ode45.f90:116-202 */
 int ucy_rkck(ucy_ctx *ctx, int n, double t, const double *yin, const double *dydt, double h,
              double *yout, double *yerr);

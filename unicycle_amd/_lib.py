@@ -113,6 +113,7 @@ _SIGS = {
     "ucy_matvec_reduced": (_i, [_vp, _pd, _pd]),
     "ucy_init_state": (_i, [_vp, _i, _pd]),
     "ucy_odefun": (_i, [_vp, _i, _d, _pd, _pd, _pd]),
+    "ucy_odefun_dev_static": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "ucy_rkck": (_i, [_vp, _i, _d, _pd, _pd, _d, _pd, _pd]),
     "ucy_rkqs": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _d, _pd, _pd, _d, _d]),
     "ucy_integrate": (_i, [_vp, _pd, _d, _d, _i64, _d, _d, _pd, _pd, _pd, _i64,

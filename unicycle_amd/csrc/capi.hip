@@ -10,6 +10,8 @@
 // compiler does not fuse sin/cos pairs into sincos, whose results differ in
 // the last bit from separate sin and cos calls.
 #include <hip/hip_runtime.h>
+#include <atomic>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -185,6 +187,34 @@ static void refsys_volume(int ns, const double *x, const double *L, const double
 // ---------------------------------------------------------------------------
 extern "C" const char *ucy_version(void) { return "ucy-mi355x 0.1 (gfx950)"; }
 
+// ---------------------------------------------------------------------------
+// Static scratch of ucy_odefun_dev_static: a file-scope device array (each
+// GPU holds its own copy until the process ends).  Host code never uses the
+// array's name as an address: ucy_create resolves it once per device with
+// hipGetSymbolAddress and every later use goes through that pointer.
+// Layout (doubles): [0] the vMax accumulator, padded to 256 bytes, then the
+// slip-rate vector v of up to kStaticMaxCols entries.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticBytes = size_t(1) << 20;                 // 1 MiB per device
+constexpr int64_t kStaticLead = 32;                              // doubles before v
+constexpr int64_t kStaticMaxCols = (int64_t)(kStaticBytes / sizeof(double)) - kStaticLead;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kStaticBytes / sizeof(double)];
+static std::atomic<double *> g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// with `device` current: the device address of this GPU's scratch
+static bool resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return false;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device].load()) return true;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return false;
+    g_scratch[device].store(static_cast<double *>(p));
+    return true;
+}
+
 extern "C" ucy_ctx *ucy_create(int device, int flags)
 {
     (void)flags;
@@ -195,6 +225,11 @@ extern "C" ucy_ctx *ucy_create(int device, int flags)
         return nullptr;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    if (!resolve_scratch(device)) {
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return nullptr;
     }
@@ -1102,6 +1137,42 @@ extern "C" int ucy_odefun(ucy_ctx *c, int n, double time, const double *y, doubl
     HIPCHK(hipMemcpyAsync(dydt, b.ak[2], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     if (vmax) HIPCHK(hipMemcpyAsync(vmax, d_vmax, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return UCY_OK;
+}
+
+// clears the vMax accumulator and the padding column of v (a kernel, not
+// hipMemsetAsync, so that the call can be captured into a graph)
+__global__ void static_prep_kernel(double *vmax, double *vpad, int npad)
+{
+    if (threadIdx.x == 0) *vmax = 0.0;
+    if ((int)threadIdx.x < npad) vpad[threadIdx.x] = 0.0;
+}
+
+extern "C" int ucy_odefun_dev_static(ucy_ctx *c, int n, const double *y_dev, double *dydt_dev,
+                                     double *vmax_dev, void *stream)
+{
+    if (!c || !y_dev || !dydt_dev) return UCY_ERR_ARG;
+    if (!c->K) return fail(c, UCY_ERR_STATE, "kernel not built");
+    const PlainDims d = dims(c);
+    if (n != d.dim) return fail(c, UCY_ERR_ARG, "state length %d, expected %d", n, d.dim);
+    if (c->params_dirty || !c->d_fr || !c->d_rh)
+        return fail(c, UCY_ERR_STATE,
+                    "friction/rheology not on the device: set them for every element first");
+    if (c->n > kStaticMaxCols)
+        return fail(c, UCY_ERR_LIMIT, "%lld kernel columns exceed the static scratch (%lld)",
+                    (long long)c->n, (long long)kStaticMaxCols);
+    double *base = g_scratch[c->device].load();
+    if (!base) return fail(c, UCY_ERR_STATE, "no static scratch on device %d", c->device);
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    double *vm = vmax_dev ? vmax_dev : base;
+    double *v = base + kStaticLead;
+    const int npad = (int)(c->n - c->nl);  // 0, or the antiplane model's padding column
+    hipLaunchKernelGGL(static_prep_kernel, dim3(1), dim3(64), 0, st, vm, v + c->nl, npad);
+    launch_prepass(st, d, c->d_fr, c->d_rh, y_dev, dydt_dev, v, vm);
+    launch_gemv_post(st, d, c->K, c->d_fr, y_dev, v, dydt_dev, nullptr);
+    HIPCHK(hipGetLastError());
+    if (!stream) HIPCHK(hipStreamSynchronize(st));
     return UCY_OK;
 }
 
