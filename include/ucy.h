@@ -386,11 +386,13 @@ int ucy_ensemble_matvec(ucy_ensemble *ens, int group, const double *V, double *T
  * the bf16 kernel variant (0: 64 K rows per wave; 1: 64 rows with the next
  * k-block's loads issued ahead; 2: 128 rows, ahead; 3: 128 rows with the V
  * tiles staged in LDS per block in chunks of 4 k-blocks; 4: 64 rows, ahead,
- * V staged in chunks of 8 k-blocks; 5: 128 rows), and its table of seconds
+ * V staged in chunks of 8 k-blocks; 5: 128 rows; 6: 64 rows, 7: 128 rows,
+ * with every load a direct global->LDS load, K per wave and V per block,
+ * 4 (6) or 3 (7) k-blocks deep, bf16+i8 only), and its table of seconds
  * per launch (UCY_GEMM_TUNE_ENTRIES entries: k-blocks 32, 64, 96, 128, 192,
  * 256, 512 of variant 0, then of 1, ...; -1 = not admissible); pointers may
  * be NULL */
-#define UCY_GEMM_TUNE_ENTRIES 42
+#define UCY_GEMM_TUNE_ENTRIES 56
 /* override the autotuner: k-blocks per split and split-bf16 variant (as in
  * ucy_ensemble_tuning; binary64 ensembles take variant 0 only) */
 int ucy_ensemble_set_tiling(ucy_ensemble *ens, int kb_per_split, int variant);

@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 import orc
-from unicycle_amd import configs
+from unicycle_amd import UcyError, configs
 from unicycle_amd.ensemble import Ensemble, PREC_BF16_I8, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16
 
 pytestmark = pytest.mark.gpu
@@ -67,8 +67,12 @@ def test_ensemble_matvec(engine_factory, precision):
     Ks = [orc.buildG(orc.Mesh(m.lambda_, m.mu, rect=m.geometry()), nthreads=8) for m in groups]
     # every GEMM variant of the autotuner, at split depths that give one and
     # several k-splits (the groups have 8 and 16 k-blocks)
-    tilings = [(kb, 0) for kb in (4, 64)] if precision == PREC_F64 else \
-        [(kb, v) for v in range(6) for kb in (4, 64)]
+    nvar = {PREC_F64: 1, PREC_BF16_I8: 8}.get(precision, 6)   # 6, 7: bf16+i8 only
+    tilings = [(kb, v) for v in range(nvar) for kb in (4, 64)]
+    if precision in (PREC_SPLIT_BF16, PREC_MIXED):
+        for v in (6, 7):
+            with pytest.raises(UcyError):
+                ens.set_tiling(64, v)
     for kb, v in tilings:
         ens.set_tiling(kb, v)
         for gi, K in enumerate(Ks):
@@ -110,7 +114,8 @@ def test_stored_kernel_format(engine_factory, precision):
     n = K.shape[1]
     rowmax = np.abs(K).max(axis=1, keepdims=True)
     bound = 2.0 ** -16
-    for kb, v in ((4, 0), (64, 3), (4, 4)):
+    tilings = ((4, 0), (64, 3), (4, 4)) + (((4, 6), (64, 7)) if precision == PREC_BF16_I8 else ())
+    for kb, v in tilings:
         ens.set_tiling(kb, v)
         Kt = np.concatenate([ens.matvec(0, np.eye(n)[:, j:j + 64]) for j in range(0, n, 64)], axis=1)
         err = np.abs(Kt - K)

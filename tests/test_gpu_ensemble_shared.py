@@ -1,0 +1,99 @@
+"""Instances of one geometry beyond 64: the library stores them as
+consecutive groups of 64 instance columns ("slices") that share one K, and
+the stage GEMM streams that K once for all the slices of the family (the
+waves of a block take the slices of one row item and k-split).
+
+- matvec: every slice against K V in binary64, within the bound of each
+  precision (test_gpu_ensemble.py states the bounds), on every tiling
+  variant the shared form admits; the LDS variants, which stage one V per
+  block, are refused.
+- stepping: a family of two slices runs the same arithmetic as the same
+  instances added as two separate groups (each with its own copy of K) on
+  the same tiling, so every instance's state, time and counters agree bit
+  for bit.
+"""
+import numpy as np
+import pytest
+
+import orc
+from unicycle_amd import UcyError, configs
+from unicycle_amd.ensemble import Ensemble, PREC_BF16_I8, PREC_F64, PREC_MIXED, PREC_SPLIT_BF16
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(model, n_inst, seed):
+    draws = configs.ensemble_instances(model, n_inst, seed)
+    zeros = np.zeros((3 * model.n, 2 * model.n))
+    y0 = []
+    for i in range(n_inst):
+        s = orc.System(configs.instance_friction(model, draws, i), zeros)
+        y0.append(configs.perturbed_initial_state(model, s.init_state(), int(draws["seeds"][i])))
+    return draws, np.array(y0)
+
+
+def _add(ens, model, d, y0, sl=slice(None)):
+    return ens.add_group(model, d["a"][sl], d["b"][sl], d["sig"][sl], d["L"][sl], y0[sl])
+
+
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_F64, PREC_MIXED, PREC_BF16_I8])
+def test_shared_k_matvec(engine_factory, precision):
+    eng = engine_factory()
+    ma = configs.thrust_fault(nx=8, nz=8, ellipse=None)
+    mb = configs.thrust_fault(nx=16, nz=8, ellipse=None)
+    da, ya = _instances(ma, 150, seed=1)
+    db, yb = _instances(mb, 37, seed=2)
+    ens = Ensemble(eng, precision)
+    assert _add(ens, ma, da, ya) == 0
+    assert _add(ens, mb, db, yb) == 3
+    assert ens.groups == [(ma.n, 64), (ma.n, 64), (ma.n, 22), (mb.n, 37)]
+    assert ens.families == [[0, 1, 2], [3]]
+    ens.build()
+    Ka = orc.buildG(orc.Mesh(ma.lambda_, ma.mu, rect=ma.geometry()), nthreads=8)
+    Kb = orc.buildG(orc.Mesh(mb.lambda_, mb.mu, rect=mb.geometry()), nthreads=8)
+    Ks = [Ka, Ka, Ka, Kb]
+    rng = np.random.default_rng(0)
+    variants = [0] if precision == PREC_F64 else [0, 1, 2, 5]
+    for kb, v in [(kb, v) for v in variants for kb in (4, 64)]:
+        ens.set_tiling(kb, v)
+        for g, K in enumerate(Ks):
+            n_inst = ens.groups[g][1]
+            V = rng.standard_normal((K.shape[1], n_inst)) * \
+                10.0 ** rng.uniform(-12, 0, (K.shape[1], n_inst))
+            T = ens.matvec(g, V)
+            want = K @ V
+            scale = np.abs(K) @ np.abs(V)
+            err = np.abs(T - want)
+            bound = 1e-13 if precision == PREC_F64 else 2.0 ** -13
+            assert np.all(err <= bound * scale), (kb, v, g, (err / scale).max())
+    if precision != PREC_F64:
+        for v in (3, 4) + ((6, 7) if precision == PREC_BF16_I8 else ()):
+            with pytest.raises(UcyError):
+                ens.set_tiling(64, v)
+
+
+@pytest.mark.parametrize("precision", [PREC_F64, PREC_BF16_I8])
+def test_shared_k_stepping_equals_separate_groups(engine_factory, precision):
+    eng = engine_factory()
+    model = configs.thrust_fault(nx=8, nz=4, ellipse=None)
+    d, y0 = _instances(model, 100, seed=7)
+    runs = []
+    for shared in (True, False):
+        ens = Ensemble(eng, precision)
+        if shared:
+            _add(ens, model, d, y0)
+            assert ens.families == [[0, 1]]
+        else:
+            _add(ens, model, d, y0, slice(0, 64))
+            _add(ens, model, d, y0, slice(64, 100))
+            assert ens.families == [[0], [1]]
+        ens.build()
+        ens.set_tiling(4, 0)       # the same split-K sums in both
+        ens.reset()
+        st = ens.run(1e30, max_attempts=1500)
+        assert st.n_failed == 0
+        runs.append([ens.state(g) for g in range(2)])
+        ens.close()
+    for g in range(2):
+        for a, b in zip(runs[0][g], runs[1][g]):
+            assert np.array_equal(a, b)

@@ -22,6 +22,7 @@ using namespace ucy;
 
 struct HostGroup {
     int N = 0, ninst = 0;
+    int share = -1;   // a slice of the K family headed by group `share` (-1: a head)
     double lambda = 0, mu = 0;
     std::vector<double> x, L, W, strike, dip;
     // per instance-major [inst][e]
@@ -132,31 +133,42 @@ extern "C" int ucy_ensemble_add_group(ucy_ensemble *E, int np, const double *x, 
                                       const double *y0)
 {
     if (!E || np <= 0 || n_inst <= 0) return UCY_ERR_ARG;
-    if (n_inst > kEnsCols) return efail(E, UCY_ERR_LIMIT, "at most %d instances per group", kEnsCols);
-    if ((int)E->hg.size() >= kMaxGroups) return efail(E, UCY_ERR_LIMIT, "at most %d groups", kMaxGroups);
+    if (n_inst > kEnsCols * kMaxSlices)
+        return efail(E, UCY_ERR_LIMIT, "at most %d instances per geometry", kEnsCols * kMaxSlices);
+    // more than 64 instances: consecutive groups of 64 (the last one partly
+    // filled) that share the first one's K
+    const int slices = (n_inst + kEnsCols - 1) / kEnsCols;
+    if ((int)E->hg.size() + slices > kMaxGroups)
+        return efail(E, UCY_ERR_LIMIT, "at most %d groups of 64 instances", kMaxGroups);
     if (!(mu > 0)) return efail(E, UCY_ERR_ARG, "shear modulus must be positive");
-    HostGroup G;
-    G.N = np;
-    G.ninst = n_inst;
-    G.lambda = lambda;
-    G.mu = mu;
-    G.x.assign(x, x + 3 * (size_t)np);
-    G.L.assign(length, length + np);
-    G.W.assign(width, width + np);
-    G.strike.assign(strike, strike + np);
-    G.dip.assign(dip, dip + np);
-    const size_t ne = (size_t)np * n_inst;
-    G.Vpl.assign(Vpl, Vpl + ne);
-    G.rake.assign(rake, rake + ne);
-    G.a.assign(a, a + ne);
-    G.b.assign(b, b + ne);
-    G.sig.assign(sig, sig + ne);
-    G.Lc.assign(L, L + ne);
-    G.damping.assign(damping, damping + ne);
-    G.y0.assign(y0, y0 + 7 * ne);
-    E->hg.push_back(std::move(G));
+    const int head = (int)E->hg.size();
+    for (int sl = 0; sl < slices; sl++) {
+        HostGroup G;
+        G.N = np;
+        G.ninst = std::min(kEnsCols, n_inst - kEnsCols * sl);
+        G.share = sl ? head : -1;
+        G.lambda = lambda;
+        G.mu = mu;
+        if (sl == 0) {   // the geometry is only read for the head's K
+            G.x.assign(x, x + 3 * (size_t)np);
+            G.L.assign(length, length + np);
+            G.W.assign(width, width + np);
+            G.strike.assign(strike, strike + np);
+            G.dip.assign(dip, dip + np);
+        }
+        const size_t o = (size_t)np * kEnsCols * sl, ne = (size_t)np * G.ninst;
+        G.Vpl.assign(Vpl + o, Vpl + o + ne);
+        G.rake.assign(rake + o, rake + o + ne);
+        G.a.assign(a + o, a + o + ne);
+        G.b.assign(b + o, b + o + ne);
+        G.sig.assign(sig + o, sig + o + ne);
+        G.Lc.assign(L + o, L + o + ne);
+        G.damping.assign(damping + o, damping + o + ne);
+        G.y0.assign(y0 + 7 * o, y0 + 7 * (o + ne));
+        E->hg.push_back(std::move(G));
+    }
     release(E);
-    return (int)E->hg.size() - 1;
+    return head;
 }
 
 // upload y0 (reference layout per instance) into the SoA state and reset control
@@ -208,19 +220,29 @@ static void set_split(EnsTable &T, int kbps, int variant)
         G.ksplit = (G.nkb + k - 1) / k;
         G.kb_per_split = (G.nkb + G.ksplit - 1) / G.ksplit;
     }
+    // GEMM blocks per K family: work items (row item, k-split, slice), 4
+    // waves per block
     int acc = 0, acc64 = 0;
-    for (int gi = 0; gi < T.ngroups; gi++) {
-        const EnsGroup &G = T.g[gi];
-        T.blk_gemm[gi] = acc;
-        T.blk_gemm64[gi] = acc64;
+    for (int f = 0; f < T.nfam; f++) {
+        const EnsGroup &G = T.g[T.fam_head[f]];
+        const int S = T.fam_slices[f];
+        T.blk_gemm[f] = acc;
+        T.blk_gemm64[f] = acc64;
         const int nri = (G.mp / 32 + rb - 1) / rb;
-        // LDS variants: one block per (4 consecutive row items, k-split)
+        // LDS variants (one slice only): one block per (4 consecutive row items, k-split)
         acc += ens_gemm_variant_lds(variant) ? ((nri + 3) / 4) * G.ksplit
-                                             : (nri * G.ksplit + 3) / 4;
-        acc64 += ((G.mp / 64) * G.ksplit + 3) / 4;
+                                             : (nri * G.ksplit * S + 3) / 4;
+        acc64 += ((G.mp / 64) * G.ksplit * S + 3) / 4;
     }
-    T.blk_gemm[T.ngroups] = acc;
-    T.blk_gemm64[T.ngroups] = acc64;
+    T.blk_gemm[T.nfam] = acc;
+    T.blk_gemm64[T.nfam] = acc64;
+}
+
+static bool has_shared_families(const EnsTable &T)
+{
+    for (int f = 0; f < T.nfam; f++)
+        if (T.fam_slices[f] > 1) return true;
+    return false;
 }
 
 static int upload_table(ucy_ensemble *E)
@@ -231,6 +253,8 @@ static int upload_table(ucy_ensemble *E)
 }
 
 static const int kMaxSplit = 16;
+// the direct global->LDS variants (6, 7) of the bf16+i8 stage GEMM
+static const bool kDmaVariantsOn = true;
 
 extern "C" int ucy_ensemble_build(ucy_ensemble *E)
 {
@@ -256,10 +280,15 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         G.nkb = G.np / 16;
         G.mode = E->mode;
         total += H.ninst;
-        // ---- K on the device (greens.f90:73-275 for rectangle receivers)
+        // ---- K on the device (greens.f90:73-275 for rectangle receivers),
+        // once per K family: its other slices take the head's tiles
+        const bool head = H.share < 0;
         double *K64 = nullptr;
-        EHIP(hipMalloc((void **)&K64, sizeof(double) * (size_t)G.m * G.n));
-        {
+        if (head) {
+            EHIP(hipMalloc((void **)&K64, sizeof(double) * (size_t)G.m * G.n));
+            E->allocs.push_back(K64);   // freed below, or by release() on an error
+        }
+        if (head) {
             std::vector<double> sv(3 * (size_t)H.N), dv(3 * (size_t)H.N), nv(3 * (size_t)H.N),
                 xc(3 * (size_t)H.N);
             refsys_okada(H.N, H.x.data(), H.L.data(), H.W.data(), H.strike.data(), H.dip.data(),
@@ -293,37 +322,51 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         // stage-GEMM bytes per launch: split bf16 (4 bytes per entry) for
         // modes 0 and 2, bf16+i8 (3 bytes) for mode 3
         if (G.mode != 1) {
-            const double kbytes = G.mode == 3 ? 3.0 : 4.0;
-            uint16_t *tiles = nullptr;
-            double *rs = nullptr;
-            if (ealloc(E, &tiles, (size_t)G.mp * G.np * (G.mode == 3 ? 3 : 4) / 2))
-                return UCY_ERR_HIP;
-            if (ealloc(E, &rs, G.mp)) return UCY_ERR_HIP;
-            if (G.mode == 3)
-                ens_launch_split3_K(c->stream, K64, G.m, G.n, G.mp, G.np, (uint8_t *)tiles, rs);
-            else
-                ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
-            EHIP(hipGetLastError());
-            G.Ktiles = tiles;
-            G.rscale = rs;
+            if (head) {
+                const double kbytes = G.mode == 3 ? 3.0 : 4.0;
+                uint16_t *tiles = nullptr;
+                double *rs = nullptr;
+                if (ealloc(E, &tiles, (size_t)G.mp * G.np * (G.mode == 3 ? 3 : 4) / 2))
+                    return UCY_ERR_HIP;
+                if (ealloc(E, &rs, G.mp)) return UCY_ERR_HIP;
+                if (G.mode == 3)
+                    ens_launch_split3_K(c->stream, K64, G.m, G.n, G.mp, G.np, (uint8_t *)tiles, rs);
+                else
+                    ens_launch_split_K(c->stream, K64, G.m, G.n, G.mp, G.np, tiles, rs);
+                EHIP(hipGetLastError());
+                G.Ktiles = tiles;
+                G.rscale = rs;
+                bytes += kbytes * G.mp * G.np + 8.0 * G.mp;
+            } else {
+                G.Ktiles = T.g[H.share].Ktiles;
+                G.rscale = T.g[H.share].rscale;
+            }
             if (ealloc(E, &G.Vtiles, (size_t)G.np * kEnsCols * 2)) return UCY_ERR_HIP;
             if (ealloc(E, &G.Tpart, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.v0, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.t0, (size_t)G.mp * kEnsCols)) return UCY_ERR_HIP;
-            bytes += kbytes * G.mp * G.np + 8.0 * G.mp + 4.0 * G.np * kEnsCols;
+            bytes += 4.0 * G.np * kEnsCols;
         }
         if (G.mode == 1 || G.mode == 2) {
-            double *tiles = nullptr;
-            if (ealloc(E, &tiles, (size_t)G.mp * G.np)) return UCY_ERR_HIP;
-            ens_launch_tile_K64(c->stream, K64, G.m, G.n, G.mp, G.np, tiles);
-            EHIP(hipGetLastError());
-            G.K64 = tiles;
+            if (head) {
+                double *tiles = nullptr;
+                if (ealloc(E, &tiles, (size_t)G.mp * G.np)) return UCY_ERR_HIP;
+                ens_launch_tile_K64(c->stream, K64, G.m, G.n, G.mp, G.np, tiles);
+                EHIP(hipGetLastError());
+                G.K64 = tiles;
+                if (G.mode == 1) bytes += 8.0 * G.mp * G.np;
+            } else {
+                G.K64 = T.g[H.share].K64;
+            }
             if (ealloc(E, &G.V64, (size_t)G.np * kEnsCols)) return UCY_ERR_HIP;
             if (ealloc(E, &G.T64, (size_t)kMaxSplit * G.mp * kEnsCols)) return UCY_ERR_HIP;
-            if (G.mode == 1) bytes += 8.0 * G.mp * G.np + 8.0 * G.np * kEnsCols;
+            if (G.mode == 1) bytes += 8.0 * G.np * kEnsCols;
         }
         EHIP(hipStreamSynchronize(c->stream));
-        EHIP(hipFree(K64));
+        if (K64) {
+            E->allocs.erase(std::find(E->allocs.begin(), E->allocs.end(), (void *)K64));
+            EHIP(hipFree(K64));
+        }
         if (ealloc(E, &G.any_rhs, 1)) return UCY_ERR_HIP;
         // ---- state and control
         const size_t S7 = (size_t)7 * G.N * kEnsCols;
@@ -375,6 +418,17 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     }
     T.blk_elem[T.ngroups] = eacc;
     T.blk_inst[T.ngroups] = iacc;
+    // K families: a head and the slices that follow it
+    T.nfam = 0;
+    for (int gi = 0; gi < T.ngroups; gi++) {
+        if (E->hg[gi].share < 0) {
+            T.fam_head[T.nfam] = gi;
+            T.fam_slices[T.nfam] = 1;
+            T.nfam++;
+        } else {
+            T.fam_slices[T.nfam - 1]++;
+        }
+    }
     T.total_inst = total;
     T.t_end = 0;
     T.epsilon = 1e-6;
@@ -404,7 +458,10 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         // variant
         for (int ci = 0; ci < kGemmVariants * kDepths; ci++) {
             const int kb = cands[ci % kDepths], v = ci / kDepths;
-            if (v > 0 && E->mode == 1) {
+            // binary64 has one variant; the LDS variants stage one V per
+            // block and so serve families of one slice only
+            if ((v > 0 && E->mode == 1) || (ens_gemm_variant_lds(v) && has_shared_families(T)) ||
+                (ens_gemm_variant_dma(v) && (E->mode != 3 || !kDmaVariantsOn))) {
                 E->tune_seconds.push_back(-1.0);
                 continue;
             }
@@ -417,10 +474,17 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
                 continue;
             }
             if (upload_table(E)) return UCY_ERR_HIP;
-            // 2 warm launches, 15 timed (single launches vary by ~10 %)
+            // 2 warm launches, 15 timed (single launches vary by ~10 %); a
+            // candidate whose launch fails is not admissible
             for (int r = 0; r < 2; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
+            if (hipGetLastError() != hipSuccess) {
+                EHIP(hipStreamSynchronize(c->stream));
+                E->tune_seconds.push_back(-1.0);
+                continue;
+            }
             EHIP(hipEventRecord(e0, c->stream));
             for (int r = 0; r < 15; r++) ens_launch_gemm(c->stream, E->d_tab, T, 2);
+            EHIP(hipGetLastError());
             EHIP(hipEventRecord(e1, c->stream));
             EHIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -702,6 +766,13 @@ extern "C" int ucy_ensemble_set_tiling(ucy_ensemble *E, int kb_per_split, int va
     if (!E->built) return efail(E, UCY_ERR_STATE, "ensemble not built");
     if (E->mode == 1 && variant != 0)
         return efail(E, UCY_ERR_ARG, "the binary64 GEMM has one variant");
+    if (ens_gemm_variant_lds(variant) && has_shared_families(E->tab))
+        return efail(E, UCY_ERR_ARG, "variant %d stages one V per block: not with shared K",
+                     variant);
+    if (ens_gemm_variant_dma(variant) && E->mode != 3)
+        return efail(E, UCY_ERR_ARG, "variant %d streams bf16+i8 K tiles only", variant);
+    if (ens_gemm_variant_dma(variant) && !kDmaVariantsOn)
+        return efail(E, UCY_ERR_ARG, "variant %d is switched off", variant);
     EnsTable T = E->tab;
     set_split(T, kb_per_split, variant);
     for (int gi = 0; gi < T.ngroups; gi++)

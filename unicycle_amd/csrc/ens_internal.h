@@ -19,7 +19,11 @@
 namespace ucy {
 
 constexpr int kEnsCols = 64;     // instance columns per group (padded)
-constexpr int kMaxGroups = 8;
+constexpr int kMaxGroups = 16;
+// a geometry with more than 64 instances is stored as up to kMaxSlices
+// consecutive groups ("slices") that share one K: the stage GEMM streams the
+// family's K once for all its slices
+constexpr int kMaxSlices = 4;
 
 struct EnsFriction {             // per (element, instance), SoA arrays [e*64+inst]
     // a sig and b sig as products (the right-hand side uses a, b and sig
@@ -70,8 +74,13 @@ struct EnsGroup {
 struct EnsTable {
     int ngroups;
     EnsGroup g[kMaxGroups];
-    int blk_gemm[kMaxGroups + 1];   // prefix sums of split-bf16 GEMM blocks per group
-    int blk_gemm64[kMaxGroups + 1]; // prefix sums of binary64 GEMM blocks per group
+    // K families: family f is groups fam_head[f] .. fam_head[f] + fam_slices[f] - 1,
+    // which share the head's K tiles (and row scales)
+    int nfam;
+    int fam_head[kMaxGroups];
+    int fam_slices[kMaxGroups];
+    int blk_gemm[kMaxGroups + 1];   // prefix sums of split-bf16 GEMM blocks per family
+    int blk_gemm64[kMaxGroups + 1]; // prefix sums of binary64 GEMM blocks per family
     int variant;                    // split-bf16 GEMM variant (rows per wave, pipelining)
     int blk_elem[kMaxGroups + 1];   // prefix sums of elementwise blocks
     int blk_inst[kMaxGroups + 1];   // prefix sums of per-instance blocks
@@ -92,9 +101,13 @@ struct EnsTable {
 // 4 = 64 rows, ahead, V staged in chunks of 8 k-blocks, 5 = 128 rows (200
 // VGPRs; 128 rows ahead with V staged in chunks of 2 needs 264-272, one wave
 // per SIMD, and measured 461 us against 265 us: not kept)
-constexpr int kGemmVariants = 6;
+// 6 = 64 rows per wave, 7 = 128 rows: every load is a direct global->LDS
+// load, each wave's K tiles in its own LDS ring and V in a ring shared by the
+// block, 4 (6) or 3 (7) k-blocks deep (bf16+i8 only)
+constexpr int kGemmVariants = 8;
 int ens_gemm_variant_rows(int variant);
 int ens_gemm_variant_lds(int variant);
+int ens_gemm_variant_dma(int variant);
 
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,

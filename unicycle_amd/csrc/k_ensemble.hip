@@ -20,6 +20,7 @@
 //   K tile (rb, kb)         1536 bytes at ((rb*nkb + kb)*1536):
 //       hi  bytes 16*lane + 2*j,   lo code  byte 1024 + 8*lane + j
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "ens_internal.h"
 
 namespace ucy {
@@ -272,7 +273,30 @@ __device__ __forceinline__ int group_of(const int *prefix, int ng, int blk)
 // k-split.  RB = 4 halves the V-tile reads per K byte against RB = 2 at
 // twice the accumulators; the autotuner picks.
 // ---------------------------------------------------------------------------
-template <int RB, bool PIPE, bool I8>
+// K loads: non-temporal when the wave is the only reader of its K tiles
+// (NT); plain when the slices of a K family read the same tiles in the
+// waves of one block, so that the first read serves the others from cache
+// Loads through the global address space: a load through a generic (flat)
+// pointer may reach LDS, its counter then retires out of order, and the
+// compiler can only wait for all outstanding loads at once, which drains
+// every load issued ahead.
+template <typename T>
+using gptr = __attribute__((address_space(1))) const T *;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const T *p)
+{
+    return (gptr<T>)p;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ T kload(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(as_global(p));
+    else return *as_global(p);
+}
+
+template <int RB, bool PIPE, bool I8, bool NT>
 __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int sp, int lane)
 {
     const int kb0 = sp * G.kb_per_split;
@@ -298,14 +322,14 @@ __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int s
 #pragma unroll
         for (int a = 0; a < RB; a++) {
             const char *t = A + a * rstride + (size_t)kb * kTileBytes<I8>;
-            ra[a][0] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + lane);
+            ra[a][0] = kload<NT>(reinterpret_cast<const u32x4 *>(t) + lane);
             if constexpr (I8)
-                rq[a] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+                rq[a] = kload<NT>(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
             else
-                ra[a][1] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
+                ra[a][1] = kload<NT>(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) rbv[q] = Bv[(size_t)kb * 256 + 64 * q];
+        for (int q = 0; q < 4; q++) rbv[q] = *as_global(Bv + (size_t)kb * 256 + 64 * q);
     };
     if (kb0 < kb1) load(kb0, ca, cq, cb);
     for (int kb = kb0; kb < kb1; kb++) {
@@ -365,14 +389,200 @@ __host__ __device__ __forceinline__ int bf16_row_items(int mp, int rb)
     return ((mp >> 5) + rb - 1) / rb;
 }
 
-template <int RB, bool PIPE, bool I8>
+// ---------------------------------------------------------------------------
+// Variants 6 and 7: every load of the k-loop is a direct global->LDS load
+// (global_load_lds; the 16- and 4-byte forms land lane l's bytes at 16 l and
+// 4 l, which keeps a tile's layout; the 12-byte form lands lane l at 16 l as
+// well, so it would not).  The 4 waves of a block take 4 consecutive row items of
+// one k-split (as the V-staging variants); each wave streams its own K tiles
+// into a private LDS ring and one quarter of the shared V tile into a
+// block-wide ring, D k-blocks deep.  Per k-block: wait for this wave's loads
+// of the k-block (the counter retires in issue order), a barrier without a
+// memory fence (every quarter of V has landed, and every wave has finished
+// reading the slot about to be refilled), issue the loads D - 1 k-blocks
+// ahead, read the slot from LDS, multiply.  The LDS reads are one asm block
+// that also waits for its own results: the compiler does not order a plain
+// LDS read after the DMA that fills it, and would otherwise wait for every
+// outstanding load (the whole ring) before it.
+// ---------------------------------------------------------------------------
+
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// One ring slot: RB bf16+i8 K tiles (hi 16 bytes and codes 8 bytes per lane)
+// from the wave's ring at kh = kring + 16 lane, kq = kring + 1024 + 8 lane,
+// and the 4 x 16 bytes of V per lane from vv = vring + 16 lane; KO and VO
+// are the slot's byte offsets
+template <int RB, int KO, int VO>
+__device__ __forceinline__ void lds_read_slot(unsigned kh, unsigned kq, unsigned vv,
+                                              u32x4 (&h)[RB], u32x2 (&q)[RB], u32x4 (&v)[4])
+{
+    static_assert(KO + (RB - 1) * 1536 + 1024 < 65536 && VO + 3072 < 65536, "ds offsets");
+    if constexpr (RB == 4)
+        asm volatile(
+            "ds_read_b128 %0, %12 offset:%15\n"
+            "ds_read_b128 %1, %12 offset:%16\n"
+            "ds_read_b128 %2, %12 offset:%17\n"
+            "ds_read_b128 %3, %12 offset:%18\n"
+            "ds_read_b64 %4, %13 offset:%15\n"
+            "ds_read_b64 %5, %13 offset:%16\n"
+            "ds_read_b64 %6, %13 offset:%17\n"
+            "ds_read_b64 %7, %13 offset:%18\n"
+            "ds_read_b128 %8, %14 offset:%19\n"
+            "ds_read_b128 %9, %14 offset:%20\n"
+            "ds_read_b128 %10, %14 offset:%21\n"
+            "ds_read_b128 %11, %14 offset:%22\n"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(h[0]), "=v"(h[1]), "=v"(h[2]), "=v"(h[3]), "=v"(q[0]), "=v"(q[1]), "=v"(q[2]),
+              "=v"(q[3]), "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+            : "v"(kh), "v"(kq), "v"(vv), "n"(KO), "n"(KO + 1536), "n"(KO + 3072),
+              "n"(KO + 4608), "n"(VO), "n"(VO + 1024), "n"(VO + 2048), "n"(VO + 3072)
+            : "memory");
+    else
+        asm volatile(
+            "ds_read_b128 %0, %8 offset:%11\n"
+            "ds_read_b128 %1, %8 offset:%12\n"
+            "ds_read_b64 %2, %9 offset:%11\n"
+            "ds_read_b64 %3, %9 offset:%12\n"
+            "ds_read_b128 %4, %10 offset:%13\n"
+            "ds_read_b128 %5, %10 offset:%14\n"
+            "ds_read_b128 %6, %10 offset:%15\n"
+            "ds_read_b128 %7, %10 offset:%16\n"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(h[0]), "=v"(h[1]), "=v"(q[0]), "=v"(q[1]), "=v"(v[0]), "=v"(v[1]), "=v"(v[2]),
+              "=v"(v[3])
+            : "v"(kh), "v"(kq), "v"(vv), "n"(KO), "n"(KO + 1536), "n"(VO), "n"(VO + 1024),
+              "n"(VO + 2048), "n"(VO + 3072)
+            : "memory");
+}
+
+template <int RB, int D>
+__device__ __forceinline__ void gemm_block_bf16_dma(const EnsGroup &G, int ri, int sp, int wave,
+                                                    int lane, char *kring, char *vring)
+{
+    constexpr int TB = 1536;
+    constexpr int PER = 3 * RB + 1;   // loads per k-block and wave: 3 per K tile, 1 V quarter
+    static_assert(D >= 2 && (D - 2) * PER < 64, "ring depth");
+    const int nrb = G.mp >> 5;
+    const int nri = bf16_row_items(G.mp, RB);
+    const bool valid = ri < nri;
+    const int rb0 = (valid ? ri : 0) * RB;
+    const int nvr = valid ? min(RB, nrb - rb0) : 0;   // RB, or 2 on the tail item
+    const int kb0 = sp * G.kb_per_split;
+    const int nk = min(kb0 + G.kb_per_split, G.nkb) - kb0;
+    floatx16 acc[RB][2];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const char *Kt = reinterpret_cast<const char *>(G.Ktiles);
+    const char *Vt = reinterpret_cast<const char *>(G.Vtiles);
+    typedef __attribute__((address_space(3))) void lds_void;
+    // the loads of k-block kb0 + j (clamped to the last one; waves past the
+    // last row item and rows past the last block load valid tiles they do
+    // not use, so every wave issues PER loads per k-block) into ring slot `slot`
+    auto issue = [&](int j, int slot) {
+        const int kb = kb0 + min(j, nk - 1);
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const char *t = Kt + ((size_t)min(rb0 + a, nrb - 1) * G.nkb + kb) * TB;
+            char *l = kring + (slot * RB + a) * TB;
+            // the tile's 1024 bytes of hi (64 lanes x 16), then its 512 bytes
+            // of codes as two loads of 64 lanes x 4.  All lanes take part in
+            // every DMA: the LDS base is a wave-uniform register (M0), and a
+            // load under a lane mask has been merged by the compiler with a
+            // neighbouring one of another base, the mask's lanes then
+            // writing at the wrong base
+            __builtin_amdgcn_global_load_lds((const void *)(t + 16 * lane), (lds_void *)l, 16, 0,
+                                             0);
+            __builtin_amdgcn_global_load_lds((const void *)(t + 1024 + 4 * lane),
+                                             (lds_void *)(l + 1024), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(t + 1280 + 4 * lane),
+                                             (lds_void *)(l + 1280), 4, 0, 0);
+        }
+        __builtin_amdgcn_global_load_lds((const void *)(Vt + (size_t)kb * 4096 + wave * 1024 + lane * 16),
+                                         (lds_void *)(vring + slot * 4096 + wave * 1024), 16, 0, 0);
+    };
+    const unsigned kl = (unsigned)(uintptr_t)(__attribute__((address_space(3))) char *)kring;
+    const unsigned vl = (unsigned)(uintptr_t)(__attribute__((address_space(3))) char *)vring;
+    const unsigned kh = kl + 16 * lane, kq = kl + 1024 + 8 * lane, vv = vl + 16 * lane;
+    auto step = [&](int j, auto slot_c) {
+        constexpr int slot = decltype(slot_c)::value;
+        wait_vm<(D - 2) * PER>();          // this wave's loads of k-block j
+        __builtin_amdgcn_s_barrier();       // all quarters of V j; slot j - 1 read by all
+        issue(j + D - 1, (slot + D - 1) % D);
+        if (!valid) return;
+        u32x4 hh[RB], vv4[4];
+        u32x2 qq[RB];
+        lds_read_slot<RB, slot * RB * TB, slot * 4096>(kh, kq, vv, hh, qq, vv4);
+        // past the range V counts as 0: the products are exact zeros
+        const bool live = j < nk;
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < 4; q++) vv4[q] = live ? vv4[q] : zero;
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, vv4[0]), __builtin_bit_cast(bf16x8, vv4[1])},
+                                {__builtin_bit_cast(bf16x8, vv4[2]), __builtin_bit_cast(bf16x8, vv4[3])}};
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            if (a >= nvr) continue;
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, hh[a]);
+            const bf16x8 Al = decode_lo8(hh[a], qq[a]);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
+            }
+        }
+    };
+    if (nk > 0) {
+#pragma unroll
+        for (int s = 0; s < D - 1; s++) issue(s, s);
+        for (int j = 0; j < nk; j += D)
+            static_for<D>([&](auto s_c) { step(j + decltype(s_c)::value, s_c); });
+        wait_vm<0>();   // no LDS write may land after the block has left
+    }
+    if (!valid) return;
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < RB; a++) {
+        if (a >= nvr) continue;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+    }
+}
+
+
+template <int RB, bool PIPE, bool I8, bool NT>
 __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
 {
     const int nrb = G.mp >> 5;
     const int nri = bf16_row_items(G.mp, RB);
     const int rb0 = (item % nri) * RB, sp = item / nri;
-    if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE, I8>(G, rb0, sp, lane);
-    else gemm_tile_bf16<2, PIPE, I8>(G, rb0, sp, lane);
+    if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE, I8, NT>(G, rb0, sp, lane);
+    else gemm_tile_bf16<2, PIPE, I8, NT>(G, rb0, sp, lane);
 }
 
 // split-bf16 GEMM variants of the autotuner: K rows per wave, pipelined
@@ -381,9 +591,12 @@ struct Bf16Variant {
     int rb;
     bool pipe;
     int lds_chunk;
+    int dma;        // > 0: K tiles through the wave's LDS ring this many k-blocks deep
+                    // (gemm_tile_bf16_dma, bf16+i8 only)
 };
 constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
-    {2, false, 0}, {2, true, 0}, {4, true, 0}, {4, false, 4}, {2, true, 8}, {4, false, 0}};
+    {2, false, 0, 0}, {2, true, 0, 0}, {4, true, 0, 0}, {4, false, 4, 0}, {2, true, 8, 0},
+    {4, false, 0, 0}, {2, false, 0, 4}, {4, false, 0, 3}};
 
 // LDS variants: the 4 waves of a block take 4 consecutive row items of one
 // k-split; the block stages the split's V tiles through LDS in chunks of CK
@@ -418,7 +631,7 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
 #pragma unroll
     for (int u = 0; u < CK; u++) {
         const int i = tid + 256 * u, kb = kb0 + (i >> 8);
-        lds[i] = kb < kb1 ? Vg[(size_t)kb * 256 + (i & 255)] : zero;
+        lds[i] = kb < kb1 ? *as_global(Vg + (size_t)kb * 256 + (i & 255)) : zero;
     }
     __syncthreads();
     u32x4 ca[RB][2], na[RB][2], stg[CK];
@@ -428,11 +641,11 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
         for (int a = 0; a < RB; a++)
             if (a < nvr) {
                 const char *t = A + a * rstride + (size_t)kb * kTileBytes<I8>;
-                ra[a][0] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + lane);
+                ra[a][0] = kload<true>(reinterpret_cast<const u32x4 *>(t) + lane);
                 if constexpr (I8)
-                    rq[a] = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+                    rq[a] = kload<true>(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
                 else
-                    ra[a][1] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
+                    ra[a][1] = kload<true>(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
             }
     };
     if (valid) loadK(kb0, ca, cq);
@@ -444,7 +657,7 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
 #pragma unroll
             for (int u = 0; u < CK; u++) {
                 const int i = tid + 256 * u, kbn = next + (i >> 8);
-                stg[u] = kbn < kb1 ? Vg[(size_t)kbn * 256 + (i & 255)] : zero;
+                stg[u] = kbn < kb1 ? *as_global(Vg + (size_t)kbn * 256 + (i & 255)) : zero;
             }
         }
         if (PIPE && valid && kb + 1 < kb1) loadK(kb + 1, na, nq);
@@ -509,6 +722,7 @@ __device__ __forceinline__ void gemm_block_bf16_lds(const EnsGroup &G, int ri, i
 // binary64 batched matvec on v_mfma_f64_16x16x4f64: one wavefront = 64 rows
 // x 64 columns over one k-split (4 x 4 tiles of 16 x 16)
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+template <bool NT>
 __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int lane)
 {
     const int nrq = G.mp >> 6;
@@ -527,9 +741,9 @@ __device__ __forceinline__ void gemm_wave_f64(const EnsGroup &G, int item, int l
     for (int kt = kt0; kt < kt1; kt++) {
         double av[4], bv[4];
 #pragma unroll
-        for (int a = 0; a < 4; a++) av[a] = __builtin_nontemporal_load(A + ((size_t)a * nkt + kt) * 64);
+        for (int a = 0; a < 4; a++) av[a] = kload<NT>(A + ((size_t)a * nkt + kt) * 64);
 #pragma unroll
-        for (int c = 0; c < 4; c++) bv[c] = B[((size_t)kt * 4 + c) * 64];
+        for (int c = 0; c < 4; c++) bv[c] = *as_global(B + ((size_t)kt * 4 + c) * 64);
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -565,27 +779,59 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restri
     constexpr int RB = kBf16Variants[V].rb;
     constexpr int CK = kBf16Variants[V].lds_chunk;
     const int blk = blockIdx.x;
-    const int g = group_of(tab->blk_gemm, tab->ngroups, blk);
-    const EnsGroup &G = tab->g[g];
-    if (stage == 1 && !*G.any_rhs) return;   // block-uniform
+    const int f = group_of(tab->blk_gemm, tab->nfam, blk);
+    const int head = tab->fam_head[f], S = tab->fam_slices[f];
     if constexpr (CK > 0) {
+        // one slice per family (the tuner admits the LDS variants only then)
+        const EnsGroup &G = tab->g[head];
+        if (stage == 1 && !*G.any_rhs) return;   // block-uniform
         __shared__ u32x4 lds[2 * (CK > 0 ? CK : 1) * 256];
         const int nrg = (bf16_row_items(G.mp, RB) + 3) / 4;
-        const int local = blk - tab->blk_gemm[g];
+        const int local = blk - tab->blk_gemm[f];
         const int wave = threadIdx.x >> 6;
         gemm_block_bf16_lds<RB, kBf16Variants[V].pipe, CK, I8>(G, 4 * (local % nrg) + wave,
                                                                local / nrg, wave, threadIdx.x & 63,
                                                                lds);
+    } else if constexpr (kBf16Variants[V].dma > 0) {
+        // one slice per family (block-shaped, as the V-staging variants);
+        // bf16+i8 K only (the tuner admits it only then)
+        if constexpr (I8) {
+            constexpr int DD = kBf16Variants[V].dma;
+            const EnsGroup &G = tab->g[head];
+            if (stage == 1 && !*G.any_rhs) return;   // block-uniform
+            __shared__ __attribute__((aligned(16))) char kring[4 * DD * RB * 1536];
+            __shared__ __attribute__((aligned(16))) char vring[DD * 4096];
+            const int nrg = (bf16_row_items(G.mp, RB) + 3) / 4;
+            const int local = blk - tab->blk_gemm[f];
+            const int wave = threadIdx.x >> 6;
+            gemm_block_bf16_dma<RB, DD>(G, 4 * (local % nrg) + wave, local / nrg, wave,
+                                        threadIdx.x & 63, kring + wave * (DD * RB * 1536),
+                                        vring);
+        }
     } else {
-        const int item = (blk - tab->blk_gemm[g]) * 4 + (threadIdx.x >> 6);
-        if (item >= bf16_row_items(G.mp, RB) * G.ksplit) return;
-        gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8>(G, item, threadIdx.x & 63);
+        // work items (row item, k-split, slice), slice fastest: the waves of
+        // a block take the slices of one row item and k-split, so the
+        // family's K tiles leave HBM once for all of them
+        const int item = (blk - tab->blk_gemm[f]) * 4 + (threadIdx.x >> 6);
+        const EnsGroup &H = tab->g[head];
+        if (item >= bf16_row_items(H.mp, RB) * H.ksplit * S) return;
+        const EnsGroup &G = tab->g[head + item % S];
+        if (stage == 1 && !*G.any_rhs) return;
+        if (S == 1) gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8, true>(G, item, threadIdx.x & 63);
+        else gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8, false>(G, item / S, threadIdx.x & 63);
     }
 }
 
+int ens_gemm_variant_dma(int v)
+{
+    return kBf16Variants[v].dma > 0;
+}
+
+// block-shaped variants: V shared by the 4 waves of a block (one k-split
+// per block, one slice per K family)
 int ens_gemm_variant_lds(int v)
 {
-    return kBf16Variants[v].lds_chunk;
+    return kBf16Variants[v].lds_chunk > 0 || kBf16Variants[v].dma > 0;
 }
 
 int ens_gemm_variant_rows(int v)
@@ -596,32 +842,37 @@ int ens_gemm_variant_rows(int v)
 __global__ void __launch_bounds__(256) gemm_f64_kernel(const EnsTable *__restrict__ tab, int stage)
 {
     const int blk = blockIdx.x;
-    const int g = group_of(tab->blk_gemm64, tab->ngroups, blk);
-    const EnsGroup &G = tab->g[g];
-    const int item = (blk - tab->blk_gemm64[g]) * 4 + (threadIdx.x >> 6);
-    if (item >= (G.mp >> 6) * G.ksplit) return;
+    const int f = group_of(tab->blk_gemm64, tab->nfam, blk);
+    const int head = tab->fam_head[f], S = tab->fam_slices[f];
+    const EnsGroup &H = tab->g[head];
+    const int item = (blk - tab->blk_gemm64[f]) * 4 + (threadIdx.x >> 6);
+    if (item >= (H.mp >> 6) * H.ksplit * S) return;
+    const EnsGroup &G = tab->g[head + item % S];
     if (stage == 1 && !*G.any_rhs) return;
-    gemm_wave_f64(G, item, threadIdx.x & 63);
+    if (S == 1) gemm_wave_f64<true>(G, item, threadIdx.x & 63);
+    else gemm_wave_f64<false>(G, item / S, threadIdx.x & 63);
 }
 
 template <bool I8>
 static void launch_gemm_bf16(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
 {
-    const dim3 grid(htab.blk_gemm[htab.ngroups]);
+    const dim3 grid(htab.blk_gemm[htab.nfam]);
     switch (htab.variant) {
     case 0: hipLaunchKernelGGL((gemm_bf16_kernel<0, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 1: hipLaunchKernelGGL((gemm_bf16_kernel<1, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 2: hipLaunchKernelGGL((gemm_bf16_kernel<2, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 3: hipLaunchKernelGGL((gemm_bf16_kernel<3, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 4: hipLaunchKernelGGL((gemm_bf16_kernel<4, I8>), grid, dim3(256), 0, st, dtab, stage); break;
-    default: hipLaunchKernelGGL((gemm_bf16_kernel<5, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 5: hipLaunchKernelGGL((gemm_bf16_kernel<5, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 6: hipLaunchKernelGGL((gemm_bf16_kernel<6, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<7, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     }
 }
 
 void ens_launch_gemm(hipStream_t st, const EnsTable *dtab, const EnsTable &htab, int stage)
 {
     if (gemm_f64_path(htab.g[0].mode, stage))
-        hipLaunchKernelGGL(gemm_f64_kernel, dim3(htab.blk_gemm64[htab.ngroups]), dim3(256), 0, st,
+        hipLaunchKernelGGL(gemm_f64_kernel, dim3(htab.blk_gemm64[htab.nfam]), dim3(256), 0, st,
                            dtab, stage);
     else if (htab.g[0].mode == 3)
         launch_gemm_bf16<true>(st, dtab, htab, stage);

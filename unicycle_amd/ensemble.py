@@ -48,6 +48,7 @@ class Ensemble:
         if not self._h:
             raise UcyError(-1, "cannot create ensemble")
         self.groups = []
+        self.families = []     # group indices sharing one K, per add_group call
         self._log_cap = {}
         engine._ensembles.append(self)
 
@@ -70,7 +71,11 @@ class Ensemble:
 
     def add_group(self, model, a, b, sig, L, y0):
         """model: configs.FaultModel geometry and the shared Vpl/rake/damping;
-        a, b, sig, L: (n_inst, N) per-instance friction; y0: (n_inst, 7N)."""
+        a, b, sig, L: (n_inst, N) per-instance friction; y0: (n_inst, 7N).
+        Up to 256 instances: beyond 64 the library stores the geometry as
+        consecutive groups of 64 instances (slices, the last one partly
+        filled) that share one K, and each slice appears here as a group of
+        its own.  Returns the index of the first slice."""
         a = np.ascontiguousarray(a, dtype=np.float64)
         n_inst, N = a.shape
         rep = lambda v: np.ascontiguousarray(np.broadcast_to(v, (n_inst, N)), dtype=np.float64)
@@ -82,7 +87,9 @@ class Ensemble:
                                             *[_p(v) for v in arrs])
         if st < 0:
             self._chk(st)
-        self.groups.append((N, n_inst))
+        for k in range(0, n_inst, 64):
+            self.groups.append((N, min(64, n_inst - k)))
+        self.families.append(list(range(st, len(self.groups))))
         return st
 
     def build(self):
@@ -190,7 +197,9 @@ class Ensemble:
         1, ...).  Variants: 0 = 64 K rows per wave, 1 = 64 rows with loads
         issued a k-block ahead, 2 = 128 rows, ahead, 3 = 128 rows with V
         staged in LDS in chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged
-        in chunks of 8, 5 = 128 rows."""
+        in chunks of 8, 5 = 128 rows, 6 = 64 rows and 7 = 128 rows with
+        every load a direct global->LDS load (K per wave, V per block), 4 or
+        3 k-blocks deep (6, 7: bf16+i8 only)."""
         kb, var = C.c_int(), C.c_int()
         t = np.full(_lib.GEMM_TUNE_ENTRIES, -1.0)
         self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
