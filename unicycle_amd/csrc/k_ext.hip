@@ -157,8 +157,8 @@ sv_rows_kernel(double G, double nu, const SvSource *__restrict__ src, int ns,
     const Receiver R = rcv[r];
     const SvSource S = src[s];
     double st[6][6];
-    svj::stress6<VD>(R.xc[0], R.xc[1], R.xc[2], S.q[0], S.q[1], S.q[2], S.L, S.T, S.W, S.strike, G,
-                     nu, SeedVD(), st);
+    svj::stress6<VD, SeedVD, true>(R.xc[0], R.xc[1], R.xc[2], S.q[0], S.q[1], S.q[2], S.L, S.T,
+                                   S.W, S.strike, G, nu, SeedVD(), st);
     constexpr int NC = KIND == 0 ? 3 : 6;
     double o[6][NC];
     const double *sv = R.sv, *dv = R.dv, *nv = R.nv;
