@@ -158,6 +158,23 @@ int ucy_set_volumes(ucy_ctx *ctx, int ns, const double *x, const double *length,
  * ucy_rkqs and ucy_integrate(_ex) run this model. */
 int ucy_set_antiplane(ucy_ctx *ctx, double mu, int ns, const double *x, const double *width,
                       const double *dip);
+/* Strain volumes of the 2-D antiplane viscouscycles model
+ * (fortran/2d/antiplane/src/viscouscycles.f90 there; deck section
+ * :1404-1474): nv volumes with upper centre x(3,nv) (x(2), x(3) used),
+ * thickness (horizontal), width (depth extent) and dip (radians; the frame
+ * uses it, the Green's functions ignore it, antiplane.f90:479-515).  Call
+ * after ucy_set_antiplane, which clears them.  The state gains 4 entries per
+ * volume (e12, e13, s12, s13); K becomes (ns + 2 nv) square: after the
+ * segments' rows and columns, two stress rows (s12, s13 in the volume's
+ * frame) and two strain columns (e12, e13) per volume (greens.f90:93-248).
+ * ucy_set_rheology supplies the power law per volume with the background
+ * strain rate in edot_bg[6 i + 1] (e12) and [6 i + 2] (e13):
+ * eII = gammadot0 (sII/Gm)^(n-1) / Gm exp(-Q/(R To)) (viscouscycles.f90:
+ * 846-877), the initial stress from the same law at the background rate
+ * (:740-762).  Displacement kernels gain the volumes' (e12, e13) columns
+ * (computeDisplacementStrainVolumeAntiplane, antiplane.f90:213-255). */
+int ucy_set_antiplane_volumes(ucy_ctx *ctx, int nv, const double *x, const double *thickness,
+                              const double *width, const double *dip);
 /* computeTractionKernelsAntiplane (antiplane.f90:352-379) */
 int ucy_antiplane_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
                                    const double dv[3], const double nv[3], int ns,

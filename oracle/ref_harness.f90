@@ -206,4 +206,55 @@ CONTAINS
     CALL computeReferenceSystemAntiplane(ns,x,W,dip,sv,dv,nv,xc)
   END SUBROUTINE
 
+  ! antiplane.f90:479 computeStressStrainVolumeAntiplane
+  SUBROUTINE ref_stress_sv_antiplane(x2,x3,q2,q3,T,W,dip,e12,e13,G,s12,s13) &
+       BIND(C,NAME="ref_stress_sv_antiplane")
+    USE antiplane
+    REAL(C_DOUBLE), VALUE :: x2,x3,q2,q3,T,W,dip,e12,e13,G
+    REAL(C_DOUBLE), INTENT(OUT) :: s12,s13
+    CALL computeStressStrainVolumeAntiplane(x2,x3,q2,q3,T,W,dip,e12,e13,G,s12,s13)
+  END SUBROUTINE
+
+  ! antiplane.f90:213 computeDisplacementStrainVolumeAntiplane
+  SUBROUTINE ref_displacement_sv_antiplane(x2,x3,q2,q3,T,W,dip,e12,e13,u1) &
+       BIND(C,NAME="ref_displacement_sv_antiplane")
+    USE antiplane
+    REAL(C_DOUBLE), VALUE :: x2,x3,q2,q3,T,W,dip,e12,e13
+    REAL(C_DOUBLE), INTENT(OUT) :: u1
+    CALL computeDisplacementStrainVolumeAntiplane(x2,x3,q2,q3,T,W,dip,e12,e13,u1)
+  END SUBROUTINE
+
+  ! antiplane.f90:543 computeTractionKernelsStrainVolumeAntiplane
+  SUBROUTINE ref_traction_sv_antiplane(x,sv,dv,nv,ns,y,T,W,dip,e12,e13,G,ts) &
+       BIND(C,NAME="ref_traction_sv_antiplane")
+    USE antiplane
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),T(ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: e12,e13,G
+    REAL(C_DOUBLE), INTENT(OUT) :: ts(ns)
+    CALL computeTractionKernelsStrainVolumeAntiplane(x,sv,dv,nv,ns,y,T,W,dip,e12,e13,G,ts)
+  END SUBROUTINE
+
+  ! antiplane.f90:600 computeStressKernelsStrainVolumeAntiplane
+  SUBROUTINE ref_stress_kernels_sv_antiplane(x,sv,dv,nv,ns,y,T,W,dip,e12,e13,G,s12,s13) &
+       BIND(C,NAME="ref_stress_kernels_sv_antiplane")
+    USE antiplane
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),T(ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: e12,e13,G
+    REAL(C_DOUBLE), INTENT(OUT) :: s12(ns),s13(ns)
+    CALL computeStressKernelsStrainVolumeAntiplane(x,sv,dv,nv,ns,y,T,W,dip,e12,e13,G,s12,s13)
+  END SUBROUTINE
+
+  ! antiplane.f90:401 computeStressKernelsAntiplane
+  SUBROUTINE ref_stress_kernels_antiplane(x,sv,dv,nv,ns,y,W,dip,G,s12,s13) &
+       BIND(C,NAME="ref_stress_kernels_antiplane")
+    USE antiplane
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: G
+    REAL(C_DOUBLE), INTENT(OUT) :: s12(ns),s13(ns)
+    CALL computeStressKernelsAntiplane(x,sv,dv,nv,ns,y,W,dip,G,s12,s13)
+  END SUBROUTINE
+
 END MODULE ucy_ref_harness

@@ -168,10 +168,31 @@ void orc_buildG_antiplane(int ns, const double *x, const double *W, const double
                           double *K, int nthreads);
 void orc_buildO_antiplane(int ns, const double *x, const double *W, const double *dip, int npts,
                           const double *pts, double *O);
+/* strain volumes of the 2-D antiplane viscouscycles model
+ * (fortran/2d/antiplane/src/antiplane.f90:213-255, 479-635): s12, s13 of the
+ * primed eigenstrain (e12p, e13p) of a volume of thickness T and width W
+ * whose top is at (q2, q3); the displacement it derives from */
+void orc_stress_sv_antiplane(double x2, double x3, double q2, double q3, double T, double W,
+                             double dip, double e12p, double e13p, double G, double *s12,
+                             double *s13);
+double orc_displacement_sv_antiplane(double x2, double x3, double q2, double q3, double T,
+                                     double W, double dip, double e12p, double e13p);
+/* buildG of patches and volumes (greens.f90:93-248 of fortran/2d/antiplane):
+ * m = n = np + 2 nv; rows: one traction per patch, then (s12, s13) per volume
+ * in its frame; columns: one slip per patch, then (e12, e13) per volume */
+void orc_buildG_antiplane_volumes(int np, const double *px, const double *pW,
+                                  const double *pdip, int nv, const double *vx,
+                                  const double *vT, const double *vW, const double *vdip,
+                                  double mu, double *K, int nthreads);
 typedef struct {
     int n;                 /* patches; state 4 per patch: slip, traction, log10 theta, log10 V */
     const double *Vpl, *mu0, *sig, *a, *b, *L, *Vo, *damping;
-    const double *K;       /* n x n row-major */
+    const double *K;       /* (n + 2 nvol) x (n + 2 nvol) row-major */
+    /* volumes (viscouscycles.f90 of fortran/2d/antiplane); state 4 per volume:
+     * e12, e13, s12, s13; background strain rate (e12, e13); power law
+     * gammadot0 (sII/Gm)^(n-1)/Gm exp(-Q/(R To)) */
+    int nvol;
+    const double *gammadot0, *npower, *e12, *e13, *Gm, *Q, *R, *To;
 } orc_ap_system;
 void orc_ap_init_state(const orc_ap_system *s, double *y);
 double orc_ap_odefun(const orc_ap_system *s, const double *y, double *dydt, double *vwork,

@@ -18,6 +18,12 @@ The fixtures are inputs and the reference's outputs only:
                         computeDisplacementVerticalStrainVolume (215-268)
   antiplane.npz         computeStressAntiplane / computeTractionKernels-
                         Antiplane (fortran/2d/antiplane/src/antiplane.f90)
+  antiplane_volumes.npz computeStressStrainVolumeAntiplane, computeDisplacement-
+                        StrainVolumeAntiplane, the traction and stress kernels
+                        of volumes and segments (antiplane.f90:213-255,
+                        401-635) and buildG of a small mesh of segments and
+                        volumes assembled from those kernels in the layout of
+                        greens.f90:93-248 (fortran/2d/antiplane)
   displacement.npz      computeDisplacementOkada92 (okada92.f90:230-298) and
                         computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367,
                         DispInHalfSpace.c:42-152) at random points, at the
@@ -310,11 +316,82 @@ def antiplane(rng, n=400, ns=120):
                         y=y, W=W, dip=dip, receivers=np.array(rcv), traction=np.array(ts))
 
 
+def _ap_frames(x, W, dip):
+    n = W.size
+    o = [np.zeros((n, 3)) for _ in range(4)]
+    REF.ref_refsys_antiplane(n, P(np.ascontiguousarray(x)), P(W), P(dip), *[P(a) for a in o])
+    return o
+
+
+def antiplane_volumes(rng, n=400):
+    """Strain volumes of the 2-D antiplane model: the stress and displacement
+    of unit and random eigenstrains at random points (a quarter at the
+    surface, a quarter inside a volume), and buildG of 7 dipping segments and
+    5 volumes assembled from the reference's kernel routines."""
+    G = 30e3
+    q = np.stack([rng.uniform(-2e4, 2e4, n), rng.uniform(0, 1e4, n), rng.uniform(200, 5e3, n),
+                  rng.uniform(200, 5e3, n)], 1)            # q2, q3, T, W
+    x = np.stack([rng.uniform(-2e4, 2e4, n), rng.uniform(0, 2e4, n)], 1)
+    x[::4, 1] = 0.0
+    inside = np.arange(1, n, 4)
+    x[inside, 0] = q[inside, 0] + rng.uniform(-0.45, 0.45, inside.size) * q[inside, 2]
+    x[inside, 1] = q[inside, 1] + rng.uniform(0.05, 0.95, inside.size) * q[inside, 3]
+    e = rng.standard_normal((n, 2))
+    e[::3] = [1.0, 0.0]
+    e[1::3] = [0.0, 1.0]
+    dip = np.full(n, np.pi / 2)
+    inp = np.concatenate([x, q, dip[:, None], e], 1)      # x2 x3 q2 q3 T W dip e12 e13
+    s, u = np.zeros((n, 2)), np.zeros(n)
+    for i, r in enumerate(inp):
+        a, b, c = D(), D(), D()
+        REF.ref_stress_sv_antiplane(*[D(v) for v in r], D(G), C.byref(a), C.byref(b))
+        REF.ref_displacement_sv_antiplane(*[D(v) for v in r], C.byref(c))
+        s[i] = a.value, b.value
+        u[i] = c.value
+    # a mesh: segments (x, W, dip) and volumes (x, T, W, dip = 90 degrees)
+    nps, nv = 7, 5
+    px = np.stack([np.zeros(nps), rng.uniform(-1e4, 1e4, nps), rng.uniform(0, 5e3, nps)], 1)
+    pW, pdip = rng.uniform(500, 3e3, nps), rng.uniform(0.3, 1.5, nps)
+    vx = np.stack([np.zeros(nv), rng.uniform(-1e4, 1e4, nv), rng.uniform(1e4, 2e4, nv)], 1)
+    vT, vW, vdip = rng.uniform(1e3, 4e3, nv), rng.uniform(1e3, 4e3, nv), np.full(nv, np.pi / 2)
+    pf, vf = _ap_frames(px, pW, pdip), _ap_frames(vx, vW, vdip)
+    m = nps + 2 * nv
+    K = np.zeros((m, m))
+    pxc, vxc = np.ascontiguousarray(px), np.ascontiguousarray(vx)
+    for r in range(nps + nv):
+        sv, dv, nvv, xc = [f[r] for f in pf] if r < nps else [f[r - nps] for f in vf]
+        fr = [P(np.ascontiguousarray(a)) for a in (xc, sv, dv, nvv)]
+        if r < nps:      # tractionRows: segments, then (e12, e13) per volume
+            ts = np.zeros(nps)
+            REF.ref_traction_antiplane(*fr, nps, P(pxc), P(pW), P(pdip), D(G), P(ts))
+            K[r, :nps] = ts
+            for j in range(2):
+                tv = np.zeros(nv)
+                REF.ref_traction_sv_antiplane(*fr, nv, P(vxc), P(vT), P(vW), P(vdip),
+                                              D(float(j == 0)), D(float(j == 1)), D(G), P(tv))
+                K[r, nps + j::2] = tv
+        else:            # stressRows: s12 row, then s13 row
+            row = nps + 2 * (r - nps)
+            a, b = np.zeros(nps), np.zeros(nps)
+            REF.ref_stress_kernels_antiplane(*fr, nps, P(pxc), P(pW), P(pdip), D(G), P(a), P(b))
+            K[row, :nps], K[row + 1, :nps] = a, b
+            for j in range(2):
+                a, b = np.zeros(nv), np.zeros(nv)
+                REF.ref_stress_kernels_sv_antiplane(*fr, nv, P(vxc), P(vT), P(vW), P(vdip),
+                                                    D(float(j == 0)), D(float(j == 1)), D(G),
+                                                    P(a), P(b))
+                K[row, nps + j::2], K[row + 1, nps + j::2] = a, b
+    np.savez_compressed(os.path.join(HERE, "antiplane_volumes.npz"), inputs=inp, stress=s,
+                        displacement=u, px=px, pW=pW, pdip=pdip, vx=vx, vT=vT, vW=vW, vdip=vdip,
+                        G=G, K=K)
+
+
 FIXTURES = {
     "okada_dc3d": (20260101, [dc3d_cases, okada_kernels, nikkhoo_strain, nikkhoo_kernels,
                               strainvolume_kernels]),
     "displacement": (20261202, [displacement]),
     "antiplane": (20261203, [antiplane]),
+    "antiplane_volumes": (20261204, [antiplane_volumes]),
 }
 
 if __name__ == "__main__":

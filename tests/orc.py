@@ -143,9 +143,13 @@ def slip_vector(y, np_, nvol):
     return np.concatenate(u)
 
 
+AP_RHEO = ("gammadot0", "npower", "e12", "e13", "Gm", "Q", "R", "To")
+
+
 class OrcAPSystem(C.Structure):
     _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in (
-        "Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")]
+        "Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")] + [
+        ("nvol", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in AP_RHEO]
 
 
 def buildG_antiplane(x, W, dip, mu, nthreads=1):
@@ -154,6 +158,34 @@ def buildG_antiplane(x, W, dip, mu, nthreads=1):
     K = np.empty((ns, ns))
     oracle().orc_buildG_antiplane(ns, p(x), p(W), p(dip), C.c_double(mu), p(K), nthreads)
     return K
+
+
+def buildG_antiplane_volumes(x, W, dip, vx, vT, vW, vdip, mu, nthreads=1):
+    """buildG of the 2-D viscouscycles model: patches and strain volumes."""
+    x, W, dip = f64(x), f64(W), f64(dip)
+    vx, vT, vW, vdip = f64(vx), f64(vT), f64(vW), f64(vdip)
+    np_, nv = W.size, vW.size
+    n = np_ + 2 * nv
+    K = np.empty((n, n))
+    oracle().orc_buildG_antiplane_volumes(np_, p(x), p(W), p(dip), nv, p(vx), p(vT), p(vW),
+                                          p(vdip), C.c_double(mu), p(K), nthreads)
+    return K
+
+
+def stress_sv_antiplane(x2, x3, q2, q3, T, W, dip, e12, e13, G):
+    s12, s13 = C.c_double(), C.c_double()
+    d = C.c_double
+    oracle().orc_stress_sv_antiplane(d(x2), d(x3), d(q2), d(q3), d(T), d(W), d(dip), d(e12),
+                                     d(e13), d(G), C.byref(s12), C.byref(s13))
+    return s12.value, s13.value
+
+
+def displacement_sv_antiplane(x2, x3, q2, q3, T, W, dip, e12, e13):
+    lib = oracle()
+    lib.orc_displacement_sv_antiplane.restype = C.c_double
+    d = C.c_double
+    return lib.orc_displacement_sv_antiplane(d(x2), d(x3), d(q2), d(q3), d(T), d(W), d(dip),
+                                             d(e12), d(e13))
 
 
 def buildO_antiplane(x, W, dip, points):
@@ -165,25 +197,38 @@ def buildO_antiplane(x, W, dip, points):
 
 
 class APSystem:
-    """2-D antiplane rate-and-state (fortran/2d/antiplane/src/ratestate.f90)."""
+    """2-D antiplane rate-and-state (fortran/2d/antiplane/src/ratestate.f90),
+    with strain volumes when rheo (the AP_RHEO arrays, one entry per volume)
+    is given (viscouscycles.f90 there); K is then (n + 2 nvol) square."""
 
-    def __init__(self, fric, K):
+    def __init__(self, fric, K, rheo=None):
         self.keep = [f64(a) for a in fric] + [f64(K)]
         s = OrcAPSystem()
         s.n = self.keep[0].size
         for k, a in zip(("Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K"), self.keep):
             setattr(s, k, p(a))
+        s.nvol = 0
+        if rheo is not None:
+            arrs = [f64(a) for a in rheo]
+            self.keep += arrs
+            s.nvol = arrs[0].size
+            for k, a in zip(AP_RHEO, arrs):
+                setattr(s, k, p(a))
+        assert self.keep[8].shape == (s.n + 2 * s.nvol,) * 2
         self.c = s
         self.n = s.n
+        self.nvol = s.nvol
+        self.dim = 4 * (s.n + s.nvol)
 
     def init_state(self):
-        y = np.empty(4 * self.n)
+        y = np.empty(self.dim)
         oracle().orc_ap_init_state(C.byref(self.c), p(y))
         return y
 
     def odefun(self, y):
         y = f64(y)
-        dydt, v, t = np.empty(4 * self.n), np.empty(self.n), np.empty(self.n)
+        nc = self.n + 2 * self.nvol
+        dydt, v, t = np.empty(self.dim), np.empty(nc), np.empty(nc)
         lib = oracle()
         lib.orc_ap_odefun.restype = C.c_double
         vmax = lib.orc_ap_odefun(C.byref(self.c), p(y), p(dydt), p(v), p(t))

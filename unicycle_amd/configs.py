@@ -359,6 +359,72 @@ class AntiplaneModel:
         return (self.Vpl, self.mu0, self.sig, self.a, self.b, self.L, self.Vo, self.damping)
 
 
+@dataclass
+class AntiplaneViscousModel:
+    """The 2-D antiplane viscouscycles model (fortran/2d/antiplane/src/
+    viscouscycles.f90): a fault of segments (AntiplaneModel) above a grid of
+    strain volumes with a power-law (Maxwell for n = 1) rheology."""
+    fault: AntiplaneModel
+    vx: np.ndarray          # (nv, 3) upper centres; x2, x3 used
+    thickness: np.ndarray   # horizontal extent
+    vwidth: np.ndarray      # depth extent
+    vdip: np.ndarray        # radians
+    e12: np.ndarray         # background strain rate (1/s)
+    e13: np.ndarray
+    Gm: np.ndarray
+    gammadot0: np.ndarray
+    npower: np.ndarray
+    Q: np.ndarray
+    R: np.ndarray
+    To: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.fault.n
+
+    @property
+    def nvol(self) -> int:
+        return self.thickness.size
+
+    def edot_bg(self):
+        """Engine.set_rheology's background strain rate (6, nv): e12 and e13 in
+        entries 1 and 2 of each volume."""
+        e = np.zeros((self.nvol, 6))
+        e[:, 1], e[:, 2] = self.e12, self.e13
+        return e
+
+    def rheology(self):
+        """Engine.set_rheology order."""
+        return (self.gammadot0, self.npower, self.edot_bg(), self.Gm, self.Q, self.R, self.To)
+
+    def oracle_rheology(self):
+        """orc.APSystem's rheo order."""
+        return (self.gammadot0, self.npower, self.e12, self.e13, self.Gm, self.Q, self.R, self.To)
+
+
+def antiplane_viscous(n: int = 100, n2: int = 10, n3: int = 10, depth: float = 25e3,
+                      gap: float = 10e3, vsize: float = 2e3, npower: float = 1.0,
+                      gammadot0: float = 1e-12, Gm: float = 1.0, Q: float = 0.0,
+                      To: float = 1200.0, e12: float = 1e-15, e13: float = 0.0,
+                      **fault) -> AntiplaneViscousModel:
+    """The layout of the reference's viscous1 tutorial
+    (fortran/2d/antiplane/examples/tutorials/viscous1.sh) at a chosen size:
+    n segments over [0, depth) and, below a gap, an n2 x n3 grid of volumes of
+    vsize x vsize centred on x2 = 0 with e12 = 1e-15/s, Gm = 1,
+    gammadot0 = 1e-12, n = 1, Q = 0 and R = 8.314 by default."""
+    f = antiplane_fault(n=n, depth=depth, vw_top=5e3, vw_bottom=15e3, **fault)
+    j2, j3 = np.meshgrid(np.arange(1, n2 + 1), np.arange(1, n3 + 1))
+    nv = n2 * n3
+    vx = np.zeros((nv, 3))
+    vx[:, 1] = ((j2 - n2 / 2) * vsize).ravel()
+    vx[:, 2] = (depth + gap + (j3 - 1) * vsize).ravel()
+    one = np.ones(nv)
+    return AntiplaneViscousModel(fault=f, vx=vx, thickness=vsize * one, vwidth=vsize * one,
+                                 vdip=np.pi / 2 * one, e12=e12 * one, e13=e13 * one,
+                                 Gm=Gm * one, gammadot0=gammadot0 * one, npower=npower * one,
+                                 Q=Q * one, R=8.314 * one, To=To * one)
+
+
 def antiplane_fault(n: int = 400, depth: float = 40e3, vw_top: float = 2e3,
                     vw_bottom: float = 15e3, mu: float = 30e3, vs: float = 3000.0,
                     L: float = 0.01, sig: float = 100.0, Vpl: float = 1e-9) -> AntiplaneModel:

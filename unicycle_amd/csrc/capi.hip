@@ -438,6 +438,33 @@ extern "C" int ucy_set_volumes(ucy_ctx *c, int ns, const double *x, const double
     return upload_if_complete(c);
 }
 
+// strain volumes of the 2-D antiplane model (fortran/2d/antiplane/src/
+// viscouscycles.f90:1404-1474): upper centre x(3,nv) (x(2), x(3) used),
+// thickness (horizontal), width (depth extent) and dip (radians, used by the
+// frame only); after ucy_set_antiplane, which clears them
+extern "C" int ucy_set_antiplane_volumes(ucy_ctx *c, int nv, const double *x,
+                                         const double *thickness, const double *width,
+                                         const double *dip)
+{
+    if (!c || nv < 0 || (nv && (!x || !thickness || !width || !dip))) return UCY_ERR_ARG;
+    if (c->model != 1) return fail(c, UCY_ERR_STATE, "no antiplane model: call ucy_set_antiplane first");
+    for (int i = 0; i < nv; i++) {
+        if (x[3 * i + 2] < 0)  // viscouscycles.f90 (2-D antiplane):1455-1460
+            return fail(c, UCY_ERR_GEOMETRY, "volume %d: depth must be positive", i + 1);
+        if (!(thickness[i] > 0.0) || !(width[i] > 0.0))  // :1469-1473
+            return fail(c, UCY_ERR_ARG, "volume %d: dimensions must be positive", i + 1);
+    }
+    c->nvol = nv;
+    c->vx.assign(x, x + 3 * (size_t)nv);
+    c->vT.assign(thickness, thickness + nv);
+    c->vW.assign(width, width + nv);
+    c->vdip.assign(dip, dip + nv);
+    c->vL.assign(nv, 0.0);
+    c->vstrike.assign(nv, 0.0);
+    c->params_dirty = true;
+    return upload_if_complete(c);
+}
+
 extern "C" int ucy_set_friction(ucy_ctx *c, int np, const double *Vpl, const double *rake,
                                 const double *mu0, const double *sig, const double *a,
                                 const double *b, const double *L, const double *Vo,
@@ -515,8 +542,9 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     const int np = c->nrect + c->ntri, ne = np + c->nvol;
     if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
     const bool ap = c->model == 1;
-    const int64_t m = ap ? np : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
-    const int64_t n = ap ? np + (np & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t nap = np + 2 * (int64_t)c->nvol;  // antiplane rows and columns
+    const int64_t m = ap ? nap : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t n = ap ? nap + (nap & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
     c->K = nullptr;
@@ -529,27 +557,43 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     c->m = m;
     c->n = n;
-    c->nl = ap ? np : n;
+    c->nl = ap ? nap : n;
     std::vector<Receiver> R;
     if (ap) {  // buildG of fortran/2d/antiplane (greens.f90:93-248)
+        // receivers: the patches, then the volumes (frames of
+        // computeReferenceSystemAntiplane for both, viscouscycles.f90:1085-1107);
+        // sources: (q2, q3, W) per patch, then (q2, q3, W, T) per volume
+        const int nv = c->nvol, ne = np + nv;
+        std::vector<Receiver> RV;
         refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R);
-        std::vector<double> S(3 * (size_t)np);
+        refsys_antiplane(nv, c->vx.data(), c->vW.data(), c->vdip.data(), RV);
+        R.insert(R.end(), RV.begin(), RV.end());
+        std::vector<double> S(3 * (size_t)np + 4 * (size_t)nv + 1);
         for (int i = 0; i < np; i++) {
             S[3 * i] = c->rx[3 * i + 1];
             S[3 * i + 1] = c->rx[3 * i + 2];
             S[3 * i + 2] = c->rW[i];
         }
+        double *SV = S.data() + 3 * (size_t)np;
+        for (int i = 0; i < nv; i++) {
+            SV[4 * i] = c->vx[3 * i + 1];
+            SV[4 * i + 1] = c->vx[3 * i + 2];
+            SV[4 * i + 2] = c->vW[i];
+            SV[4 * i + 3] = c->vT[i];
+        }
         Receiver *d_R = nullptr;
         double *d_S = nullptr;
-        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * np));
-        HIPCHK(hipMalloc((void **)&d_S, sizeof(double) * 3 * np));
-        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * np, hipMemcpyHostToDevice,
+        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * ne));
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(double) * S.size()));
+        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * ne, hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
                               c->stream));
         auto t0 = std::chrono::steady_clock::now();
-        if (n != np) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
-        launch_ap_rows(c->stream, d_S, np, d_R, np, c->mu, c->K, n, nullptr);
+        if (n != nap) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
+        if (np) launch_ap_rows(c->stream, d_S, np, d_R, np, c->mu, c->K, n, nullptr);
+        if (nv) launch_ap_volume_blocks(c->stream, d_S, np, d_S + 3 * (size_t)np, nv, d_R, c->mu,
+                                        c->K, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         c->build_seconds[0] =
@@ -665,7 +709,7 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
     if (!c || !K || m <= 0 || n <= 0) return UCY_ERR_ARG;
     const int np = c->nrect + c->ntri;
     const bool shape_ok = c->model == 1
-        ? (m == np && n == np)
+        ? (m == np + 2 * (int64_t)c->nvol && n == m)
         : (m == 3 * (int64_t)np + 6 * (int64_t)c->nvol && n == 2 * (int64_t)np + 6 * (int64_t)c->nvol);
     if (!shape_ok) return fail(c, UCY_ERR_ARG, "kernel shape does not match the mesh");
     HIPCHK(hipSetDevice(c->device));
@@ -747,28 +791,39 @@ extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
     if (npts == 0) return UCY_OK;
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
     if (c->model == 1) {  // fortran/2d/antiplane/src/greens.f90:275-383
-        const int ns = c->nrect;
-        if (ns == 0) return fail(c, UCY_ERR_STATE, "no elements");
-        std::vector<double> S(3 * (size_t)ns);
+        // columns: one slip per segment, then e12, e13 per volume
+        const int ns = c->nrect, nv = c->nvol;
+        const int64_t on = ns + 2 * (int64_t)nv;
+        if (on == 0) return fail(c, UCY_ERR_STATE, "no elements");
+        std::vector<double> S(3 * (size_t)ns + 4 * (size_t)nv + 1);
         for (int i = 0; i < ns; i++) {
             S[3 * i] = c->rx[3 * i + 1];
             S[3 * i + 1] = c->rx[3 * i + 2];
             S[3 * i + 2] = c->rW[i];
         }
+        for (int i = 0; i < nv; i++) {
+            double *q = S.data() + 3 * (size_t)ns + 4 * i;
+            q[0] = c->vx[3 * i + 1];
+            q[1] = c->vx[3 * i + 2];
+            q[2] = c->vW[i];
+            q[3] = c->vT[i];
+        }
         double *buf = nullptr, *O = nullptr;
-        HIPCHK(hipMalloc((void **)&buf, sizeof(double) * 3 * ((size_t)ns + npts)));
-        HIPCHK(hipMalloc((void **)&O, sizeof(double) * (size_t)npts * ns));
-        HIPCHK(hipMemcpyAsync(buf, S.data(), sizeof(double) * 3 * ns, hipMemcpyHostToDevice,
+        HIPCHK(hipMalloc((void **)&buf, sizeof(double) * (S.size() + 3 * (size_t)npts)));
+        HIPCHK(hipMalloc((void **)&O, sizeof(double) * (size_t)npts * on));
+        double *pts = buf + S.size();
+        HIPCHK(hipMemcpyAsync(buf, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(hipMemcpyAsync(buf + 3 * ns, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(pts, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice,
                               c->stream));
-        launch_ap_disp(c->stream, buf, ns, buf + 3 * ns, npts, O, ns);
+        if (ns) launch_ap_disp(c->stream, buf, ns, pts, npts, O, on);
+        if (nv) launch_ap_volume_disp(c->stream, buf + 3 * (size_t)ns, nv, pts, npts, O + ns, on);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(buf));
         c->O = O;
         c->o_rows = npts;
-        c->o_n = ns;
+        c->o_n = on;
         return UCY_OK;
     }
     const int np = c->nrect + c->ntri;
@@ -826,7 +881,7 @@ extern "C" int ucy_displacement_shape(const ucy_ctx *c, int64_t *rows, int64_t *
 static int check_disp(ucy_ctx *c)
 {
     if (!c->O) return fail(c, UCY_ERR_STATE, "displacement kernels not built");
-    const int64_t want = c->model == 1 ? (int64_t)c->nrect
+    const int64_t want = c->model == 1 ? (int64_t)c->nrect + 2 * (int64_t)c->nvol
                                        : 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol;
     if (c->o_n != want)
         return fail(c, UCY_ERR_STATE, "displacement kernels built for another mesh");
@@ -849,7 +904,7 @@ extern "C" int ucy_get_displacement_kernel(ucy_ctx *c, double *O, double *Of, do
     HIPCHK(hipMemcpyAsync(h, c->O, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     const bool ap = c->model == 1;
-    const int64_t nf = ap ? c->o_n : 2 * (int64_t)(c->nrect + c->ntri);
+    const int64_t nf = ap ? c->nrect : 2 * (int64_t)(c->nrect + c->ntri);
     for (int64_t r = 0; r < c->o_rows; r++) {
         const bool vertical = !ap && r % 3 == 2;
         for (int64_t j = 0; j < c->o_n; j++) {
@@ -882,7 +937,7 @@ extern "C" int ucy_displacement(ucy_ctx *c, const double *y, double *d, double *
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    const int64_t dim = c->model == 1 ? 4 * (int64_t)c->nrect
+    const int64_t dim = c->model == 1 ? 4 * (int64_t)(c->nrect + c->nvol)
                                       : 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
     const int64_t R = c->o_rows;
     double *buf = nullptr;
@@ -927,6 +982,8 @@ static int upload_params(ucy_ctx *c)
         H[i].gammadot0 = c->gd0[i];
         H[i].npower = c->npow[i];
         for (int k = 0; k < 6; k++) H[i].ebg[k] = c->ebg[6 * i + k];
+        H[i].Gm = c->Gm[i];
+        H[i].arr = exp(-c->Qa[i] / (c->Rg[i] * c->To[i]));
     }
     // the device arrays are reused while they are large enough; a captured
     // attempt holds their addresses, so a reallocation drops it
@@ -963,7 +1020,7 @@ static PlainDims dims(const ucy_ctx *c)
     d.np = c->nrect + c->ntri;
     d.nv = c->nvol;
     d.ap = c->model == 1;
-    d.dim = d.ap ? 4 * d.np : 7 * d.np + 12 * d.nv;
+    d.dim = d.ap ? 4 * (d.np + d.nv) : 7 * d.np + 12 * d.nv;
     d.m = (int)c->m;
     d.n = (int)c->n;
     return d;
@@ -1088,6 +1145,22 @@ extern "C" int ucy_init_state(ucy_ctx *c, int variant, double *y)
                    - c->damping[i] * c->Vpl[i];
             p[2] = log(c->Lc[i] / c->Vpl[i]) / lg10;
             p[3] = log(c->Vpl[i] * 0.98) / lg10;
+        }
+        // volumes, viscouscycles.f90 (2-D antiplane):740-762: the stress
+        // that flows at the background rate (e12, e13 in ebg[1], ebg[2])
+        for (int i = 0; i < c->nvol; i++) {
+            double *q = y + 4 * (np + i);
+            const double e12 = c->ebg[6 * i + 1], e13 = c->ebg[6 * i + 2];
+            const double eps = sqrt(e12 * e12 + e13 * e13);
+            if (0.0 < eps && 0.0 < c->gd0[i]) {
+                const double tau = c->Gm[i] * pow(eps / c->gd0[i] * exp(c->Qa[i] / (c->Rg[i] * c->To[i])),
+                                                  1.0 / c->npow[i]);
+                q[2] = tau * e12 / eps;
+                q[3] = tau * e13 / eps;
+            } else {
+                q[2] = q[3] = 0.0;
+            }
+            q[0] = q[1] = 0.0;
         }
         return UCY_OK;
     }

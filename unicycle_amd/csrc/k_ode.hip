@@ -88,6 +88,24 @@ __device__ __forceinline__ void ap_postpass(const double *p, const Friction &f, 
     dp[1] = dtau - f.damping * velocity * dp[3] * kLg10;
 }
 
+// 2-D antiplane volume (fortran/2d/antiplane/src/viscouscycles.f90:846-877):
+// state [e12, e13, s12, s13]; strain rate eII s with
+// eII = gammadot0 (sII/Gm)^(n-1) / Gm exp(-Q/(R To)), v = that less the
+// background rate; returns the strain-rate norm (eMax of :869)
+__device__ __forceinline__ double ap_volume_prepass(const double *q, const Rheology &r, double *dq,
+                                                    double *vq)
+{
+    const double s12 = q[2], s13 = q[3];
+    const double sII = sqrt(s12 * s12 + s13 * s13);
+    const double eII = r.gammadot0 * pow(sII / r.Gm, r.npower - 1.0) / r.Gm * r.arr;
+    const double e12 = eII * s12, e13 = eII * s13;
+    dq[0] = e12;
+    dq[1] = e13;
+    vq[0] = e12 - r.ebg[1];
+    vq[1] = e13 - r.ebg[2];
+    return sqrt(e12 * e12 + e13 * e13);
+}
+
 __device__ __forceinline__ void volume_prepass(const double *q, const Rheology &r, double *dq,
                                                double *vq)
 {
@@ -131,12 +149,12 @@ __device__ __forceinline__ int elem_state_off(const PlainDims &d, int e)
 }
 __device__ __forceinline__ int elem_v_off(const PlainDims &d, int e)
 {
-    if (d.ap) return e;
+    if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 2 * e : 2 * d.np + 6 * (e - d.np);
 }
 __device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
 {
-    if (d.ap) return e;
+    if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 3 * e : 3 * d.np + 6 * (e - d.np);
 }
 // state entries and slip-rate entries of element e
@@ -146,7 +164,7 @@ __device__ __forceinline__ int elem_nstate(const PlainDims &d, int e)
 }
 __device__ __forceinline__ int elem_nv(const PlainDims &d, int e)
 {
-    return d.ap ? 1 : (e < d.np ? 2 : 6);
+    return d.ap ? (e < d.np ? 1 : 2) : (e < d.np ? 2 : 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +327,10 @@ __global__ void prepass_kernel(PlainDims d, const Friction *__restrict__ fr,
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (d.ap) {
+    if (d.ap && e >= d.np) {
+        dydt[so + 2] = dydt[so + 3] = 0.0;
+        ap_volume_prepass(y + so, rh[e - d.np], dydt + so, v + vo);
+    } else if (d.ap) {
         for (int k = 1; k < 4; k++) dydt[so + k] = 0.0;
         const double vel = ap_prepass(y + so, fr[e], dydt + so, v + vo);
         if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
@@ -364,7 +385,20 @@ __device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const KOperan
 {
     const int so = elem_state_off(d, e);
     const int64_t ro = elem_row_off(d, e);
-    if (d.ap) {
+    if (d.ap && e >= d.np) {
+        double acc[2];
+        elem_rows<2, MODE>(K, ro, d.n, v, t0, lane, acc);
+        if (lane == 0) {
+            if (t_out) { t_out[ro] = acc[0]; t_out[ro + 1] = acc[1]; }
+            dydt[so + 2] = acc[0];  // viscouscycles.f90 (2-D antiplane):939-942
+            dydt[so + 3] = acc[1];
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 4; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    } else if (d.ap) {
         double acc[1];
         elem_rows<1, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
@@ -475,7 +509,11 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
     }
     // ctl->vmax was cleared by the controller that accepted the previous step
     if (ctl->done || !ctl->need_rhs) return;
-    if (d.ap) {
+    if (d.ap && e >= d.np) {
+        // eMax of the step-start odefun (viscouscycles.f90 (2-D antiplane):869)
+        const double en = ap_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
+    } else if (d.ap) {
         const double vel = ap_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
     } else if (e < d.np) {
@@ -551,7 +589,8 @@ __device__ __forceinline__ void stage_elem(const PlainDims &d, const Friction *_
                                 b.ak[5][i]);
     }
     double *out = b.ak[stage];
-    if (d.ap) ap_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    if (d.ap && e >= d.np) ap_volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
+    else if (d.ap) ap_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
     if (b.v0) {
@@ -753,14 +792,14 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, 
     const int64_t r = blockIdx.x;
     const bool vertical = !ap && (r % 3) == 2;
     const double *Or = O + r * n;
-    // the 2-D antiplane model: one slip per patch (state entry 0 of 4), one
-    // displacement component per point, no volumes
-    const int64_t nf = ap ? n : 2 * (int64_t)np, vbase = 7 * (int64_t)np;
+    // the 2-D antiplane model: one slip per patch (state entry 0 of 4), then
+    // e12, e13 per volume (entries 0, 1 of 4); one displacement component per point
+    const int64_t nf = ap ? np : 2 * (int64_t)np, vbase = 7 * (int64_t)np;
     double a = 0.0, af = 0.0, al = 0.0;
     for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         double u;
         if (ap) {
-            u = y[4 * j];
+            u = j < np ? y[4 * j] : y[4 * np + 4 * ((j - np) >> 1) + ((j - np) & 1)];
         } else if (j < nf) {
             u = y[7 * (j >> 1) + (j & 1)];
         } else {

@@ -32,9 +32,10 @@ struct ucy_ctx {
     int ntri = 0, nve = 0;
     std::vector<int> i1, i2, i3;
     std::vector<double> vert;
-    // volumes
+    // volumes (the 2-D antiplane model's: upper centre x(3,nv), thickness vT,
+    // width vW, dip vdip in radians)
     int nvol = 0;
-    std::vector<double> vx, vL, vT, vW, vstrike;
+    std::vector<double> vx, vL, vT, vW, vstrike, vdip;
     // friction / rheology (element order)
     std::vector<double> Vpl, rake, mu0, sig, a, b, Lc, Vo, damping;
     std::vector<double> gd0, npow, ebg, Gm, Qa, Rg, To;

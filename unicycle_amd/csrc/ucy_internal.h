@@ -19,10 +19,13 @@ struct Friction {
     double a, b, sig, L, damping, vplc, vpls, pad;
 };
 
-// Per-volume rheology for the RHS (viscouscycles.f90:963-995).
+// Per-volume rheology for the RHS (viscouscycles.f90:963-995).  The 2-D
+// antiplane volumes (fortran/2d/antiplane/src/viscouscycles.f90:846-877) use
+// ebg[1], ebg[2] (e12, e13), Gm and arr = exp(-Q/(R To)), formed on the host.
 struct Rheology {
     double gammadot0, npower;
     double ebg[6];
+    double Gm, arr;
 };
 
 // Device-side control block of the graph-captured integrator.
@@ -53,10 +56,12 @@ struct Control {
 struct PlainDims {
     int np;      // patches
     int nv;      // volumes
-    int dim;     // 7*np + 12*nv (4*np for the antiplane model)
+    int dim;     // 7*np + 12*nv (4*(np + nv) for the antiplane model)
     int m, n;    // K rows, columns
-    int ap;      // 1: 2-D antiplane patches (fortran/2d/antiplane: 4 state entries,
-                 // one slip rate and one traction row per patch, no volumes)
+    int ap;      // 1: the 2-D antiplane model (fortran/2d/antiplane: 4 state
+                 // entries per element; per patch one slip rate and one
+                 // traction row, per volume two strain rates e12, e13 and two
+                 // stress rows s12, s13)
 };
 
 // ---- assembly launchers (k_assemble.hip) ----
@@ -78,6 +83,14 @@ void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *r
 // antiplane buildO: row k = point k (x(3,npts), x2 and x3 used), column s
 void launch_ap_disp(hipStream_t st, const double *src, int ns, const double *pts, int npts,
                     double *O, int64_t ldo);
+// antiplane buildG blocks that involve strain volumes: every receiver (np
+// patches, then nv volumes) against the volume sources vol[4q..4q+3] =
+// (q2, q3, W, T), and the volume receivers against the segments seg
+void launch_ap_volume_blocks(hipStream_t st, const double *seg, int np, const double *vol, int nv,
+                             const Receiver *rcv, double G, double *K, int64_t ldk);
+// antiplane buildO volume columns: O[k][2q + j], unit e12 (j = 0), e13 (j = 1)
+void launch_ap_volume_disp(hipStream_t st, const double *vol, int nv, const double *pts, int npts,
+                           double *O, int64_t ldo);
 
 // ---- GEMV / RHS launchers (k_ode.hip) ----
 void launch_matvec(hipStream_t st, const double *K, int m, int n, const double *v, double *t);

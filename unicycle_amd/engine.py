@@ -131,6 +131,18 @@ class Engine:
         self.nrect, self.ntri, self.nvol = x.shape[0], 0, 0
         self._update_np()
 
+    def set_antiplane_volumes(self, x, thickness, width, dip):
+        """Strain volumes of the 2-D antiplane viscouscycles model
+        (fortran/2d/antiplane/src/viscouscycles.f90): upper centres x (nv, 3;
+        x2, x3 used), thickness, width, dip (radians); after set_antiplane.
+        State: 4 entries per volume (e12, e13, s12, s13); set_rheology takes
+        the background e12, e13 in entries 1 and 2 of each edot_bg row."""
+        x = _f64(x).reshape(-1, 3)
+        self._apv = [x, _f64(thickness), _f64(width), _f64(dip)]
+        self._chk(self._L.ucy_set_antiplane_volumes(self._h, x.shape[0],
+                                                    *[_p(a) for a in self._apv]))
+        self.nvol = x.shape[0]
+
     def set_rectangles(self, x, length, width, strike, dip):
         x = _f64(x).reshape(-1, 3)
         ns = x.shape[0]
@@ -205,7 +217,7 @@ class Engine:
 
     @property
     def dim(self) -> int:
-        return 4 * self.np if self.antiplane else 7 * self.np + 12 * self.nvol
+        return 4 * (self.np + self.nvol) if self.antiplane else 7 * self.np + 12 * self.nvol
 
     def get_kernel(self) -> np.ndarray:
         m, n = self.shape
