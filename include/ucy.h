@@ -175,6 +175,32 @@ int ucy_set_antiplane(ucy_ctx *ctx, double mu, int ns, const double *x, const do
  * (computeDisplacementStrainVolumeAntiplane, antiplane.f90:213-255). */
 int ucy_set_antiplane_volumes(ucy_ctx *ctx, int nv, const double *x, const double *thickness,
                               const double *width, const double *dip);
+/* The 2-D plane-strain model of fortran/2d/planestrain (ratestate.f90 there):
+ * ns segments (upper corner x(3,ns), x(2) and x(3) used; width; dip in
+ * radians) slipping along dip in a half space of Lame parameters lambda, mu.
+ * Replaces any other elements.  A segment is a line dislocation evaluated as
+ * the reference does, as a shear zone 1e-4 of its width thick
+ * (planestrain.f90:105-164).  The state has 5 entries per patch (dip slip,
+ * dip traction, normal traction, log10 theta, log10 V); K is 2 ns x ns (rows
+ * 2r, 2r+1: dip and normal traction of receiver r; column s: dip slip of
+ * source s, greens.f90:66-226).  ucy_set_friction takes the same arrays (rake
+ * unused), ucy_build_kernel builds K, ucy_build_displacement gives two rows
+ * per point (east, down; greens.f90:253-367), and ucy_init_state
+ * (ratestate.f90:585-627), ucy_odefun (:637-753), ucy_rkck, ucy_rkqs and
+ * ucy_integrate(_ex) run it.  The parameter checks of the stress routine
+ * (Poisson's ratio in [-1, 0.5], depths >= 0) give UCY_ERR_GEOMETRY. */
+int ucy_set_planestrain(ucy_ctx *ctx, double lambda, double mu, int ns, const double *x,
+                        const double *width, const double *dip);
+/* the plane-strain deck's tau0 per patch: a positive value replaces the
+ * initial dip traction (ratestate.f90 of 2d/planestrain:604-609); reset by
+ * ucy_set_planestrain */
+int ucy_set_initial_traction(ucy_ctx *ctx, int np, const double *tau0);
+/* computeTractionKernelsPlaneStrain (planestrain.f90:266-291): dip and normal
+ * traction on one receiver (x, sv, dv, nv) of unit dip slip on ns sources */
+int ucy_planestrain_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
+                                     const double dv[3], const double nv[3], int ns,
+                                     const double *y, const double *W, const double *dip,
+                                     double G, double lambda, double *td, double *tn);
 /* computeTractionKernelsAntiplane (antiplane.f90:352-379) */
 int ucy_antiplane_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],
                                    const double dv[3], const double nv[3], int ns,

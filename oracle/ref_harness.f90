@@ -257,4 +257,60 @@ CONTAINS
     CALL computeStressKernelsAntiplane(x,sv,dv,nv,ns,y,W,dip,G,s12,s13)
   END SUBROUTINE
 
+  ! fortran/2d/planestrain/src/computeStressStrainVolumePlaneStrain.f90:53
+  SUBROUTINE ref_stress_sv_planestrain(r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam,s22,s23,s33) &
+       BIND(C,NAME="ref_stress_sv_planestrain")
+    REAL(C_DOUBLE), VALUE :: r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: s22,s23,s33
+    EXTERNAL :: computeStressStrainVolumePlaneStrain
+    CALL computeStressStrainVolumePlaneStrain(r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam,s22,s23,s33)
+  END SUBROUTINE
+
+  ! computeDisplacementStrainVolumePlaneStrain.f90:54
+  SUBROUTINE ref_displacement_sv_planestrain(r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam,u2,u3) &
+       BIND(C,NAME="ref_displacement_sv_planestrain")
+    REAL(C_DOUBLE), VALUE :: r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: u2,u3
+    EXTERNAL :: computeDisplacementStrainVolumePlaneStrain
+    CALL computeDisplacementStrainVolumePlaneStrain(r2,x3,q2,q3,T,W,dip,e22,e23,e33,G,lam,u2,u3)
+  END SUBROUTINE
+
+  ! planestrain.f90:105 computeStressPlaneStrain
+  SUBROUTINE ref_stress_planestrain(x2,x3,q2,q3,W,dip,s,G,lam,s22,s23,s33) &
+       BIND(C,NAME="ref_stress_planestrain")
+    USE planestrain
+    REAL(C_DOUBLE), VALUE :: x2,x3,q2,q3,W,dip,s,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: s22,s23,s33
+    CALL computeStressPlaneStrain(x2,x3,q2,q3,W,dip,s,G,lam,s22,s23,s33)
+  END SUBROUTINE
+
+  ! planestrain.f90:146 computeDisplacementPlaneStrain
+  SUBROUTINE ref_displacement_planestrain(x2,x3,q2,q3,W,dip,s,G,lam,u2,u3) &
+       BIND(C,NAME="ref_displacement_planestrain")
+    USE planestrain
+    REAL(C_DOUBLE), VALUE :: x2,x3,q2,q3,W,dip,s,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: u2,u3
+    CALL computeDisplacementPlaneStrain(x2,x3,q2,q3,W,dip,s,G,lam,u2,u3)
+  END SUBROUTINE
+
+  ! planestrain.f90:266 computeTractionKernelsPlaneStrain
+  SUBROUTINE ref_traction_planestrain(x,sv,dv,nv,ns,y,W,dip,G,lam,td,tn) &
+       BIND(C,NAME="ref_traction_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: td(ns),tn(ns)
+    CALL computeTractionKernelsPlaneStrain(x,sv,dv,nv,ns,y,W,dip,G,lam,td,tn)
+  END SUBROUTINE
+
+  ! planestrain.f90:55 computeReferenceSystemPlaneStrain
+  SUBROUTINE ref_refsys_planestrain(ns,x,W,dip,sv,dv,nv,xc) BIND(C,NAME="ref_refsys_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), INTENT(OUT) :: sv(3,ns),dv(3,ns),nv(3,ns),xc(3,ns)
+    CALL computeReferenceSystemPlaneStrain(ns,x,W,dip,sv,dv,nv,xc)
+  END SUBROUTINE
+
 END MODULE ucy_ref_harness

@@ -194,6 +194,47 @@ typedef struct {
     int nvol;
     const double *gammadot0, *npower, *e12, *e13, *Gm, *Q, *R, *To;
 } orc_ap_system;
+/* ---- 2-D plane strain (fortran/2d/planestrain/src) --------------------- */
+/* stress and displacement of a dipping strain volume (computeStressStrain-
+ * VolumePlaneStrain.f90, computeDisplacementStrainVolumePlaneStrain.f90):
+ * observation point (r2, x3), zone top (q2, q3), thickness T, width W, dip
+ * (radians), strain e22', e23', e33' in the zone's frame.  0, or -2 / -3 / -4
+ * for the reference's STOPs (Poisson's ratio, x3 < 0, q3 < 0). */
+int orc_stress_sv_planestrain(double r2, double x3, double q2, double q3, double T, double W,
+                              double dip, double e22p, double e23p, double e33p, double G,
+                              double lambda, double *s22, double *s23, double *s33);
+int orc_displacement_sv_planestrain(double r2, double x3, double q2, double q3, double T,
+                                    double W, double dip, double e22p, double e23p,
+                                    double e33p, double G, double lambda, double *u2,
+                                    double *u3);
+/* line dislocation with dip slip s (planestrain.f90:105-164) */
+int orc_stress_planestrain(double x2, double x3, double q2, double q3, double W, double dip,
+                           double s, double G, double lambda, double *s22, double *s23,
+                           double *s33);
+int orc_displacement_planestrain(double x2, double x3, double q2, double q3, double W,
+                                 double dip, double s, double G, double lambda, double *u2,
+                                 double *u3);
+int orc_traction_kernels_planestrain(const double x[3], const double sv[3], const double dv[3],
+                                     const double nv[3], int ns, const double *y,
+                                     const double *W, const double *dip, double G,
+                                     double lambda, double *td, double *tn);
+/* K (2 ns x ns) row-major: rows 2r, 2r+1 = dip and normal traction of
+ * receiver r, column s = dip slip of source s */
+int orc_buildG_planestrain(int ns, const double *x, const double *W, const double *dip,
+                           double G, double lambda, double *K, int nthreads);
+typedef struct {
+    int n;   /* patches; state 5 per patch: dip slip, dip traction, normal
+              * traction, log10 theta, log10 V */
+    const double *Vpl, *tau0, *mu0, *sig, *a, *b, *L, *Vo, *damping;  /* tau0 may be NULL */
+    const double *K;       /* 2n x n row-major */
+} orc_ps_system;
+void orc_ps_init_state(const orc_ps_system *s, double *y);
+double orc_ps_odefun(const orc_ps_system *s, const double *y, double *dydt, double *vwork,
+                     double *twork);
+long orc_ps_integrate(const orc_ps_system *s, double *y, double interval, double dt0,
+                      long max_steps, double epsilon, double max_step, double *t_log,
+                      double *dt_log, double *vmax_log, long log_cap, orc_run_stats *st);
+
 void orc_ap_init_state(const orc_ap_system *s, double *y);
 double orc_ap_odefun(const orc_ap_system *s, const double *y, double *dydt, double *vwork,
                      double *twork);

@@ -24,6 +24,10 @@ The fixtures are inputs and the reference's outputs only:
                         401-635) and buildG of a small mesh of segments and
                         volumes assembled from those kernels in the layout of
                         greens.f90:93-248 (fortran/2d/antiplane)
+  planestrain.npz       computeStressStrainVolumePlaneStrain, computeDisplacement-
+                        StrainVolumePlaneStrain, computeStressPlaneStrain,
+                        computeDisplacementPlaneStrain and computeTraction-
+                        KernelsPlaneStrain (fortran/2d/planestrain/src)
   displacement.npz      computeDisplacementOkada92 (okada92.f90:230-298) and
                         computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367,
                         DispInHalfSpace.c:42-152) at random points, at the
@@ -386,12 +390,75 @@ def antiplane_volumes(rng, n=400):
                         G=G, K=K)
 
 
+def planestrain(rng, n=400, nf=300, ns=60):
+    """The plane-strain routines on random inputs: dipping strain volumes
+    with random strain (a quarter of the points at the surface, a quarter
+    inside the zone), line dislocations, and the traction kernels of 4
+    dipping receivers on ns random segments."""
+    G, lam = 30e3, 20e3
+    inp = []
+    for i in range(n):
+        q2, q3 = rng.uniform(-1e4, 1e4), rng.uniform(0, 1e4)
+        T, W = rng.uniform(100, 5e3, 2)
+        dip = rng.uniform(0.1, np.pi - 0.1)
+        if i % 4 == 1:     # inside: a point of the zone's interior
+            a, b = rng.uniform(-0.45, 0.45) * T, rng.uniform(0.05, 0.95) * W
+            r2 = q2 + a * np.sin(dip) + b * np.cos(dip)
+            x3 = q3 - a * np.cos(dip) + b * np.sin(dip)
+        else:
+            r2, x3 = q2 + rng.uniform(-2, 2) * W, q3 + rng.uniform(-1, 2) * W
+        x3 = 0.0 if i % 4 == 0 or x3 < 0 else x3
+        inp.append([r2, x3, q2, q3, T, W, dip, *rng.standard_normal(3)])
+    inp = np.array(inp)
+    s, u = np.zeros((n, 3)), np.zeros((n, 2))
+    for i, r in enumerate(inp):
+        o = [D() for _ in range(5)]
+        REF.ref_stress_sv_planestrain(*[D(v) for v in r], D(G), D(lam), *[C.byref(x) for x in o[:3]])
+        REF.ref_displacement_sv_planestrain(*[D(v) for v in r], D(G), D(lam), *[C.byref(x) for x in o[3:]])
+        s[i] = [x.value for x in o[:3]]
+        u[i] = [x.value for x in o[3:]]
+    finp = []
+    for i in range(nf):
+        q2, q3 = rng.uniform(-1e4, 1e4), rng.uniform(0, 1e4)
+        W, dip = rng.uniform(100, 5e3), rng.uniform(0.1, np.pi - 0.1)
+        x3 = 0.0 if i % 4 == 0 else max(0.0, q3 + rng.uniform(-1, 2) * W)
+        finp.append([q2 + rng.uniform(-3, 3) * W, x3, q2, q3, W, dip, rng.uniform(-2, 2)])
+    finp = np.array(finp)
+    fs, fu = np.zeros((nf, 3)), np.zeros((nf, 2))
+    for i, r in enumerate(finp):
+        o = [D() for _ in range(5)]
+        REF.ref_stress_planestrain(*[D(v) for v in r], D(G), D(lam), *[C.byref(x) for x in o[:3]])
+        REF.ref_displacement_planestrain(*[D(v) for v in r], D(G), D(lam), *[C.byref(x) for x in o[3:]])
+        fs[i] = [x.value for x in o[:3]]
+        fu[i] = [x.value for x in o[3:]]
+    y = np.stack([np.zeros(ns), rng.uniform(-2e4, 2e4, ns), rng.uniform(0, 1e4, ns)], 1)
+    W, dip = rng.uniform(100, 5e3, ns), rng.uniform(0.1, np.pi - 0.1, ns)
+    rcv, td, tn = [], [], []
+    for _ in range(4):
+        d = rng.uniform(0.1, np.pi - 0.1)
+        x = np.array([0.0, rng.uniform(-1e4, 1e4), rng.uniform(100, 1e4)])
+        sv = np.array([1.0, 0.0, 0.0])
+        dv = np.array([0.0, -np.cos(d), -np.sin(d)])
+        nv = np.array([-0.0, np.sin(d), -np.cos(d)])
+        a, b = np.zeros(ns), np.zeros(ns)
+        REF.ref_traction_planestrain(P(x), P(sv), P(dv), P(nv), ns, P(np.ascontiguousarray(y)),
+                                     P(W), P(dip), D(G), D(lam), P(a), P(b))
+        rcv.append(np.concatenate([x, sv, dv, nv]))
+        td.append(a)
+        tn.append(b)
+    np.savez_compressed(os.path.join(HERE, "planestrain.npz"), G=G, lam=lam, inputs=inp,
+                        stress=s, displacement=u, fault_inputs=finp, fault_stress=fs,
+                        fault_displacement=fu, y=y, W=W, dip=dip, receivers=np.array(rcv),
+                        td=np.array(td), tn=np.array(tn))
+
+
 FIXTURES = {
     "okada_dc3d": (20260101, [dc3d_cases, okada_kernels, nikkhoo_strain, nikkhoo_kernels,
                               strainvolume_kernels]),
     "displacement": (20261202, [displacement]),
     "antiplane": (20261203, [antiplane]),
     "antiplane_volumes": (20261204, [antiplane_volumes]),
+    "planestrain": (20261205, [planestrain]),
 }
 
 if __name__ == "__main__":

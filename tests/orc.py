@@ -196,6 +196,97 @@ def buildO_antiplane(x, W, dip, points):
     return O
 
 
+class OrcPSSystem(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in (
+        "Vpl", "tau0", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")]
+
+
+def buildG_planestrain(x, W, dip, G, lam, nthreads=1):
+    """buildG of the plane-strain model: (2 ns x ns), rows dip / normal traction."""
+    x, W, dip = f64(x), f64(W), f64(dip)
+    ns = W.size
+    K = np.empty((2 * ns, ns))
+    rc = oracle().orc_buildG_planestrain(ns, p(x), p(W), p(dip), C.c_double(G), C.c_double(lam),
+                                         p(K), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_buildG_planestrain failed {rc}")
+    return K
+
+
+def _ps_call(fn, args, nout):
+    outs = [C.c_double() for _ in range(nout)]
+    rc = getattr(oracle(), fn)(*[C.c_double(a) for a in args], *[C.byref(o) for o in outs])
+    if rc:
+        raise RuntimeError(f"{fn} failed {rc}")
+    return tuple(o.value for o in outs)
+
+
+def stress_sv_planestrain(r2, x3, q2, q3, T, W, dip, e22, e23, e33, G, lam):
+    return _ps_call("orc_stress_sv_planestrain", (r2, x3, q2, q3, T, W, dip, e22, e23, e33, G, lam), 3)
+
+
+def displacement_sv_planestrain(r2, x3, q2, q3, T, W, dip, e22, e23, e33, G, lam):
+    return _ps_call("orc_displacement_sv_planestrain",
+                    (r2, x3, q2, q3, T, W, dip, e22, e23, e33, G, lam), 2)
+
+
+def stress_planestrain(x2, x3, q2, q3, W, dip, s, G, lam):
+    return _ps_call("orc_stress_planestrain", (x2, x3, q2, q3, W, dip, s, G, lam), 3)
+
+
+def displacement_planestrain(x2, x3, q2, q3, W, dip, s, G, lam):
+    return _ps_call("orc_displacement_planestrain", (x2, x3, q2, q3, W, dip, s, G, lam), 2)
+
+
+class PSSystem:
+    """2-D plane-strain rate-and-state (fortran/2d/planestrain/src/ratestate.f90);
+    tau0 (may be None) is the deck's initial-traction option."""
+
+    def __init__(self, fric, K, tau0=None):
+        arrs = [f64(a) for a in fric]
+        n = arrs[0].size
+        t0 = f64(tau0) if tau0 is not None else None
+        self.keep = arrs + [f64(K)] + ([t0] if t0 is not None else [])
+        s = OrcPSSystem()
+        s.n = n
+        for k, a in zip(("Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping"), arrs):
+            setattr(s, k, p(a))
+        s.K = p(self.keep[8])
+        if t0 is not None:
+            s.tau0 = p(t0)
+        assert self.keep[8].shape == (2 * n, n)
+        self.c = s
+        self.n = n
+        self.dim = 5 * n
+
+    def init_state(self):
+        y = np.empty(self.dim)
+        oracle().orc_ps_init_state(C.byref(self.c), p(y))
+        return y
+
+    def odefun(self, y):
+        y = f64(y)
+        dydt, v, t = np.empty(self.dim), np.empty(self.n), np.empty(2 * self.n)
+        lib = oracle()
+        lib.orc_ps_odefun.restype = C.c_double
+        vmax = lib.orc_ps_odefun(C.byref(self.c), p(y), p(dydt), p(v), p(t))
+        return dydt, vmax
+
+    def integrate(self, y0, interval, dt0=1e-3, max_steps=10**6, epsilon=1e-6,
+                  max_step=1.7976931348623158e308, log_cap=0):
+        y = f64(y0).copy()
+        logs = [np.empty(max(log_cap, 1)) for _ in range(3)]
+        st = OrcStats()
+        lib = oracle()
+        lib.orc_ps_integrate.restype = C.c_long
+        rc = lib.orc_ps_integrate(C.byref(self.c), p(y), C.c_double(interval), C.c_double(dt0),
+                                  C.c_long(max_steps), C.c_double(epsilon),
+                                  C.c_double(max_step), p(logs[0]), p(logs[1]), p(logs[2]),
+                                  C.c_long(log_cap), C.byref(st))
+        n = min(st.steps, log_cap)
+        return y, st, logs[0][:n], logs[1][:n], logs[2][:n], rc
+
+
 class APSystem:
     """2-D antiplane rate-and-state (fortran/2d/antiplane/src/ratestate.f90),
     with strain volumes when rheo (the AP_RHEO arrays, one entry per volume)

@@ -94,6 +94,10 @@ _SIGS = {
     "ucy_set_triangles": (_i, [_vp, _i, _pi, _pi, _pi, _i, _pd]),
     "ucy_set_antiplane": (_i, [_vp, _d, _i, _pd, _pd, _pd]),
     "ucy_set_antiplane_volumes": (_i, [_vp, _i, _pd, _pd, _pd, _pd]),
+    "ucy_set_planestrain": (_i, [_vp, _d, _d, _i, _pd, _pd, _pd]),
+    "ucy_set_initial_traction": (_i, [_vp, _i, _pd]),
+    "ucy_planestrain_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _d, _d,
+                                              _pd, _pd]),
     "ucy_antiplane_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _d, _pd]),
     "ucy_set_volumes": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd]),
     "ucy_set_friction": (_i, [_vp, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),

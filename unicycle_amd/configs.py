@@ -360,6 +360,35 @@ class AntiplaneModel:
 
 
 @dataclass
+class PlaneStrainModel(AntiplaneModel):
+    """Dipping thrust of the 2-D plane-strain model (fortran/2d/planestrain):
+    n segments of width W down dip from the surface; the Lame parameter
+    lambda_ beside the antiplane model's fields."""
+    lambda_: float = 30e3
+
+
+def planestrain_fault(n: int = 200, dip_deg: float = 30.0, length: float = 60e3,
+                      vw_top: float = 5e3, vw_bottom: float = 15e3, mu: float = 30e3,
+                      lambda_: float = 30e3, vs: float = 3000.0, L: float = 0.01,
+                      sig: float = 100.0, Vpl: float = 1e-9) -> PlaneStrainModel:
+    """A thrust dipping dip_deg from the surface, length m down dip, velocity
+    weakening (b = 0.014 against a = 0.010) between the depths vw_top and
+    vw_bottom of the segments' centres."""
+    W = length / n
+    dip = np.deg2rad(dip_deg)
+    k = np.arange(n)
+    x = np.zeros((n, 3))
+    x[:, 1] = k * W * np.cos(dip)
+    x[:, 2] = k * W * np.sin(dip)
+    one = np.ones(n)
+    mid = x[:, 2] + W / 2 * np.sin(dip)
+    b = np.where((mid > vw_top) & (mid < vw_bottom), 0.014, 0.006)
+    return PlaneStrainModel(mu=mu, x=x, width=W * one, dip=dip * one, Vpl=Vpl * one,
+                            mu0=0.6 * one, sig=sig * one, a=0.010 * one, b=b, L=L * one,
+                            Vo=1e-6 * one, damping=mu / (2.0 * vs) * one, lambda_=lambda_)
+
+
+@dataclass
 class AntiplaneViscousModel:
     """The 2-D antiplane viscouscycles model (fortran/2d/antiplane/src/
     viscouscycles.f90): a fault of segments (AntiplaneModel) above a grid of

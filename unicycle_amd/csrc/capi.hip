@@ -465,6 +465,72 @@ extern "C" int ucy_set_antiplane_volumes(ucy_ctx *c, int nv, const double *x,
     return upload_if_complete(c);
 }
 
+// the 2-D plane-strain model (fortran/2d/planestrain): ns segments with
+// upper corner x(3,ns) (x(2), x(3) used), width and dip (radians); replaces
+// any other elements
+extern "C" int ucy_set_planestrain(ucy_ctx *c, double lambda, double mu, int ns, const double *x,
+                                   const double *width, const double *dip)
+{
+    if (!c || ns < 0 || (ns && (!x || !width || !dip))) return UCY_ERR_ARG;
+    if (0 > mu) return fail(c, UCY_ERR_ARG, "shear modulus must be positive");  // ratestate.f90:1046-1051
+    for (int i = 0; i < ns; i++)
+        if (!(width[i] > 0.0))  // ratestate.f90 (2-D plane strain):1110-1114
+            return fail(c, UCY_ERR_ARG, "patch %d: width must be positive", i + 1);
+    c->model = 2;
+    c->mu = mu;
+    c->lambda = lambda;
+    c->have_elastic = true;
+    c->nrect = ns;
+    c->rx.assign(x, x + 3 * (size_t)ns);
+    c->rW.assign(width, width + ns);
+    c->rL.assign(width, width + ns);
+    c->rstrike.assign(ns, 0.0);
+    c->rdip.assign(dip, dip + ns);
+    c->ntri = c->nvol = c->nve = 0;
+    c->tau0.clear();
+    c->params_dirty = true;
+    return upload_if_complete(c);
+}
+
+extern "C" int ucy_set_initial_traction(ucy_ctx *c, int np, const double *tau0)
+{
+    if (!c || np < 0 || (np && !tau0)) return UCY_ERR_ARG;
+    c->tau0.assign(tau0, tau0 + np);
+    return UCY_OK;
+}
+
+// plane-strain sources: top, width, sin / cos of 1..4 times the dip
+static std::vector<PsSource> ps_sources(int ns, const double *x, const double *W,
+                                        const double *dip)
+{
+    std::vector<PsSource> S(ns);
+    for (int i = 0; i < ns; i++) {
+        S[i].q2 = x[3 * i + 1];
+        S[i].q3 = x[3 * i + 2];
+        S[i].W = W[i];
+        for (int k = 0; k < 4; k++) {
+            S[i].sk[k] = sin((k + 1.0) * dip[i]);
+            S[i].ck[k] = cos((k + 1.0) * dip[i]);
+        }
+    }
+    return S;
+}
+
+// the parameter checks of computeStressStrainVolumePlaneStrain.f90:75-86
+static int ps_check(ucy_ctx *c, double nu, int nr, const double *x3, int xstride, int ns,
+                    const double *q3, int qstride)
+{
+    if (-1.0 > nu || 0.5 < nu)
+        return fail(c, UCY_ERR_GEOMETRY, "Poisson's ratio %g outside [-1, 0.5]", nu);
+    for (int i = 0; i < nr; i++)
+        if (0 > x3[(size_t)xstride * i])
+            return fail(c, UCY_ERR_GEOMETRY, "observation depth (x3) must be positive");
+    for (int i = 0; i < ns; i++)
+        if (0 > q3[(size_t)qstride * i])
+            return fail(c, UCY_ERR_GEOMETRY, "source depth (q3) must be positive");
+    return 0;
+}
+
 extern "C" int ucy_set_friction(ucy_ctx *c, int np, const double *Vpl, const double *rake,
                                 const double *mu0, const double *sig, const double *a,
                                 const double *b, const double *L, const double *Vo,
@@ -541,10 +607,19 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipSetDevice(c->device));
     const int np = c->nrect + c->ntri, ne = np + c->nvol;
     if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
-    const bool ap = c->model == 1;
+    const bool ap = c->model == 1, ps = c->model == 2;
     const int64_t nap = np + 2 * (int64_t)c->nvol;  // antiplane rows and columns
-    const int64_t m = ap ? nap : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
-    const int64_t n = ap ? nap + (nap & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t m = ap ? nap : ps ? 2 * (int64_t)np : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t n = ap ? nap + (nap & 1)
+                    : ps ? np + (np & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+    if (ps) {
+        std::vector<Receiver> R0;
+        refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R0);
+        const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
+        const int rc = ps_check(c, nu, np, R0.empty() ? nullptr : &R0[0].xc[2],
+                                (int)(sizeof(Receiver) / sizeof(double)), np, c->rx.data() + 2, 3);
+        if (rc) return rc;
+    }
     free_work(c);
     if (c->K) HIPCHK(hipFree(c->K));
     c->K = nullptr;
@@ -557,8 +632,41 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     c->m = m;
     c->n = n;
-    c->nl = ap ? nap : n;
+    c->nl = ap ? nap : ps ? np : n;
     std::vector<Receiver> R;
+    if (ps) {  // buildG of fortran/2d/planestrain (greens.f90:66-226)
+        refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R);
+        const std::vector<PsSource> S = ps_sources(np, c->rx.data(), c->rW.data(), c->rdip.data());
+        Receiver *d_R = nullptr;
+        PsSource *d_S = nullptr;
+        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * np));
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(PsSource) * np));
+        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * np, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * np, hipMemcpyHostToDevice,
+                              c->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        if (n != np) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
+        const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
+        launch_ps_rows(c->stream, d_S, np, d_R, np, c->mu, c->lambda, nu, c->K, n, nullptr,
+                       nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->build_seconds[0] =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        c->build_seconds[1] = c->build_seconds[2] = 0.0;
+        HIPCHK(hipFree(d_R));
+        HIPCHK(hipFree(d_S));
+        if (precision != UCY_PREC_F64) {
+            HIPCHK(hipMalloc((void **)&c->Kp, (size_t)(m * n) * sizeof(uint32_t)));
+            HIPCHK(hipMalloc((void **)&c->Krs, (size_t)m * sizeof(double)));
+            launch_split_rowmajor(c->stream, c->K, m, n, c->Krs, c->Kp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->kprec = precision;
+        }
+        return UCY_OK;
+    }
     if (ap) {  // buildG of fortran/2d/antiplane (greens.f90:93-248)
         // receivers: the patches, then the volumes (frames of
         // computeReferenceSystemAntiplane for both, viscouscycles.f90:1085-1107);
@@ -708,7 +816,8 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
 {
     if (!c || !K || m <= 0 || n <= 0) return UCY_ERR_ARG;
     const int np = c->nrect + c->ntri;
-    const bool shape_ok = c->model == 1
+    const bool shape_ok = c->model == 2 ? (m == 2 * (int64_t)np && n == np)
+        : c->model == 1
         ? (m == np + 2 * (int64_t)c->nvol && n == m)
         : (m == 3 * (int64_t)np + 6 * (int64_t)c->nvol && n == 2 * (int64_t)np + 6 * (int64_t)c->nvol);
     if (!shape_ok) return fail(c, UCY_ERR_ARG, "kernel shape does not match the mesh");
@@ -721,7 +830,7 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
     if (c->Krs) HIPCHK(hipFree(c->Krs));
     c->Krs = nullptr;
     c->kprec = UCY_PREC_F64;
-    const int64_t ld = c->model == 1 ? n + (n & 1) : n;
+    const int64_t ld = c->model ? n + (n & 1) : n;
     if (ld % 2) return fail(c, UCY_ERR_STATE, "odd column count");
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * ld) * sizeof(double)));
     if (ld != n) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * ld) * sizeof(double), c->stream));
@@ -774,6 +883,45 @@ extern "C" int ucy_antiplane_traction_kernels(ucy_ctx *c, const double x[3], con
     return UCY_OK;
 }
 
+// computeTractionKernelsPlaneStrain (fortran/2d/planestrain/src/
+// planestrain.f90:266-291): one receiver, ns sources y(3,ns), W, dip
+extern "C" int ucy_planestrain_traction_kernels(ucy_ctx *c, const double x[3], const double sv[3],
+                                                const double dv[3], const double nv[3], int ns,
+                                                const double *y, const double *W,
+                                                const double *dip, double G, double lambda,
+                                                double *td, double *tn)
+{
+    if (!c || ns < 0) return UCY_ERR_ARG;
+    if (ns == 0) return UCY_OK;
+    const double nu = lambda / 2.0 / (lambda + G);
+    const int rc = ps_check(c, nu, 1, x + 2, 1, ns, y + 2, 3);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    Receiver R;
+    for (int k = 0; k < 3; k++) {
+        R.xc[k] = x[k];
+        R.sv[k] = sv[k];
+        R.dv[k] = dv[k];
+        R.nv[k] = nv[k];
+    }
+    const std::vector<PsSource> S = ps_sources(ns, y, W, dip);
+    char *buf = nullptr;
+    const size_t bytes = sizeof(Receiver) + sizeof(PsSource) * (size_t)ns + sizeof(double) * 2 * (size_t)ns;
+    HIPCHK(hipMalloc((void **)&buf, bytes));
+    Receiver *d_R = (Receiver *)buf;
+    PsSource *d_S = (PsSource *)(buf + sizeof(Receiver));
+    double *d_o = (double *)(buf + sizeof(Receiver) + sizeof(PsSource) * (size_t)ns);
+    HIPCHK(hipMemcpyAsync(d_R, &R, sizeof(Receiver), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * ns, hipMemcpyHostToDevice, c->stream));
+    launch_ps_rows(c->stream, d_S, ns, d_R, 1, G, lambda, nu, nullptr, 0, d_o, d_o + ns);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(td, d_o, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(tn, d_o + ns, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(buf));
+    return UCY_OK;
+}
+
 // ---------------------------------------------------------------------------
 // observation points: buildO (greens.f90:302-462) and exportPoints
 // ---------------------------------------------------------------------------
@@ -790,6 +938,32 @@ extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
     c->o_rows = c->o_n = 0;
     if (npts == 0) return UCY_OK;
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
+    if (c->model == 2) {  // fortran/2d/planestrain/src/greens.f90:253-367
+        const int ns = c->nrect;
+        if (ns == 0) return fail(c, UCY_ERR_STATE, "no elements");
+        const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
+        int rc = ps_check(c, nu, npts, x + 2, 3, ns, c->rx.data() + 2, 3);
+        if (rc) return rc;
+        const std::vector<PsSource> S = ps_sources(ns, c->rx.data(), c->rW.data(), c->rdip.data());
+        char *buf = nullptr;
+        double *O = nullptr;
+        HIPCHK(hipMalloc((void **)&buf, sizeof(PsSource) * ns + sizeof(double) * 3 * (size_t)npts));
+        HIPCHK(hipMalloc((void **)&O, sizeof(double) * 2 * (size_t)npts * ns));
+        PsSource *d_S = (PsSource *)buf;
+        double *pts = (double *)(buf + sizeof(PsSource) * ns);
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * ns, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(pts, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice,
+                              c->stream));
+        launch_ps_disp(c->stream, d_S, ns, pts, npts, c->mu, c->lambda, nu, O, ns);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(buf));
+        c->O = O;
+        c->o_rows = 2 * (int64_t)npts;
+        c->o_n = ns;
+        return UCY_OK;
+    }
     if (c->model == 1) {  // fortran/2d/antiplane/src/greens.f90:275-383
         // columns: one slip per segment, then e12, e13 per volume
         const int ns = c->nrect, nv = c->nvol;
@@ -881,7 +1055,8 @@ extern "C" int ucy_displacement_shape(const ucy_ctx *c, int64_t *rows, int64_t *
 static int check_disp(ucy_ctx *c)
 {
     if (!c->O) return fail(c, UCY_ERR_STATE, "displacement kernels not built");
-    const int64_t want = c->model == 1 ? (int64_t)c->nrect + 2 * (int64_t)c->nvol
+    const int64_t want = c->model == 2 ? (int64_t)c->nrect
+                       : c->model == 1 ? (int64_t)c->nrect + 2 * (int64_t)c->nvol
                                        : 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol;
     if (c->o_n != want)
         return fail(c, UCY_ERR_STATE, "displacement kernels built for another mesh");
@@ -903,10 +1078,10 @@ extern "C" int ucy_get_displacement_kernel(ucy_ctx *c, double *O, double *Of, do
     }
     HIPCHK(hipMemcpyAsync(h, c->O, sizeof(double) * cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    const bool ap = c->model == 1;
-    const int64_t nf = ap ? c->nrect : 2 * (int64_t)(c->nrect + c->ntri);
+    const bool two_d = c->model != 0;
+    const int64_t nf = two_d ? c->nrect : 2 * (int64_t)(c->nrect + c->ntri);
     for (int64_t r = 0; r < c->o_rows; r++) {
-        const bool vertical = !ap && r % 3 == 2;
+        const bool vertical = !two_d && r % 3 == 2;
         for (int64_t j = 0; j < c->o_n; j++) {
             const double v = h[r * c->o_n + j];
             const bool f = j < nf || vertical;
@@ -924,7 +1099,7 @@ extern "C" int ucy_displacement_dev(ucy_ctx *c, const double *y, double *d, doub
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    launch_disp_gemv(c->stream, c->O, c->o_rows, c->o_n, c->nrect + c->ntri, c->model == 1, y, d,
+    launch_disp_gemv(c->stream, c->O, c->o_rows, c->o_n, c->nrect + c->ntri, c->model, y, d,
                      df, dl, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -937,14 +1112,15 @@ extern "C" int ucy_displacement(ucy_ctx *c, const double *y, double *d, double *
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    const int64_t dim = c->model == 1 ? 4 * (int64_t)(c->nrect + c->nvol)
+    const int64_t dim = c->model == 2 ? 5 * (int64_t)c->nrect
+                      : c->model == 1 ? 4 * (int64_t)(c->nrect + c->nvol)
                                       : 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
     const int64_t R = c->o_rows;
     double *buf = nullptr;
     HIPCHK(hipMalloc((void **)&buf, sizeof(double) * (size_t)(dim + 3 * R)));
     double *dy = buf, *dd = buf + dim, *ddf = dd + R, *ddl = ddf + R;
     HIPCHK(hipMemcpyAsync(dy, y, sizeof(double) * dim, hipMemcpyHostToDevice, c->stream));
-    launch_disp_gemv(c->stream, c->O, R, c->o_n, c->nrect + c->ntri, c->model == 1, dy, dd,
+    launch_disp_gemv(c->stream, c->O, R, c->o_n, c->nrect + c->ntri, c->model, dy, dd,
                      df ? ddf : nullptr, dl ? ddl : nullptr, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(d, dd, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
@@ -973,9 +1149,10 @@ static int upload_params(ucy_ctx *c)
         F[i].sig = c->sig[i];
         F[i].L = c->Lc[i];
         F[i].damping = c->damping[i];
-        // antiplane: v = V - Vpl (ratestate.f90 2-D:676); 3-D: ratestate.f90:701-702
-        F[i].vplc = c->model == 1 ? c->Vpl[i] : c->Vpl[i] * cos(c->rake[i]);
-        F[i].vpls = c->model == 1 ? 0.0 : c->Vpl[i] * sin(c->rake[i]);
+        // 2-D: v = V - Vpl (ratestate.f90 of 2d/antiplane:676, 2d/planestrain:710);
+        // 3-D: ratestate.f90:701-702
+        F[i].vplc = c->model ? c->Vpl[i] : c->Vpl[i] * cos(c->rake[i]);
+        F[i].vpls = c->model ? 0.0 : c->Vpl[i] * sin(c->rake[i]);
     }
     std::vector<Rheology> H(c->nvol + 1);
     for (int i = 0; i < c->nvol; i++) {
@@ -1020,7 +1197,8 @@ static PlainDims dims(const ucy_ctx *c)
     d.np = c->nrect + c->ntri;
     d.nv = c->nvol;
     d.ap = c->model == 1;
-    d.dim = d.ap ? 4 * (d.np + d.nv) : 7 * d.np + 12 * d.nv;
+    d.ps = c->model == 2;
+    d.dim = d.ps ? 5 * d.np : d.ap ? 4 * (d.np + d.nv) : 7 * d.np + 12 * d.nv;
     d.m = (int)c->m;
     d.n = (int)c->n;
     return d;
@@ -1137,6 +1315,20 @@ extern "C" int ucy_init_state(ucy_ctx *c, int variant, double *y)
     if (np && (int)c->Vpl.size() != np) return fail(c, UCY_ERR_STATE, "friction not set");
     if (c->nvol && (int)c->gd0.size() != c->nvol) return fail(c, UCY_ERR_STATE, "rheology not set");
     const double lg10 = 2.302585092994045684;
+    if (c->model == 2) {  // ratestate.f90 (2-D plane strain):585-627
+        for (int i = 0; i < np; i++) {
+            double *p = y + 5 * i;
+            const bool t0 = i < (int)c->tau0.size() && 0 < c->tau0[i];
+            p[0] = 0.0;
+            p[1] = t0 ? c->tau0[i]
+                      : c->sig[i] * (c->mu0[i] + (c->a[i] - c->b[i]) * log(c->Vpl[i] / c->Vo[i]))
+                            - c->damping[i] * c->Vpl[i];
+            p[2] = 0.0;
+            p[3] = log(c->Lc[i] / c->Vpl[i]) / lg10;
+            p[4] = log(c->Vpl[i] * 0.98) / lg10;
+        }
+        return UCY_OK;
+    }
     if (c->model == 1) {  // ratestate.f90 (2-D antiplane):568-605
         for (int i = 0; i < np; i++) {
             double *p = y + 4 * i;
@@ -1338,7 +1530,7 @@ static int capture_attempt(ucy_ctx *c)
     launch_controller(c->stream, b, c->d_ctl);
     if (b.n_obs > 0) launch_obs_log(c->stream, b, c->d_ctl);
     if (b.disp)
-        launch_disp_gemv(c->stream, b.O, b.o_rows, b.o_n, d.np, d.ap, b.y, b.disp, b.disp_f,
+        launch_disp_gemv(c->stream, b.O, b.o_rows, b.o_n, d.np, c->model, b.y, b.disp, b.disp_f,
                          b.disp_l, c->d_ctl);
     HIPCHK(hipStreamEndCapture(c->stream, &g));
     hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);

@@ -88,6 +88,32 @@ __device__ __forceinline__ void ap_postpass(const double *p, const Friction &f, 
     dp[1] = dtau - f.damping * velocity * dp[3] * kLg10;
 }
 
+// 2-D plane-strain patch (fortran/2d/planestrain/src/ratestate.f90:692-712):
+// dip-slip rate V, v = V - Vpl (vplc holds Vpl)
+__device__ __forceinline__ double ps_prepass(const double *p, const Friction &f, double *dp,
+                                             double *vp)
+{
+    const double velocity = exp(p[4] * kLg10);
+    dp[0] = velocity;
+    vp[0] = velocity - f.vplc;
+    return velocity;
+}
+
+// ratestate.f90 (2-D plane strain):723-750: the dip traction drives the
+// friction law at the constant normal stress sig; the normal traction rate is
+// integrated as is
+__device__ __forceinline__ void ps_postpass(const double *p, const Friction &f, const double *t,
+                                            double *dp)
+{
+    const double velocity = exp(p[4] * kLg10);
+    dp[3] = (exp(-p[3] * kLg10) - velocity / f.L) / kLg10;
+    const double dtau = t[0];
+    const double sigma = f.sig;
+    dp[4] = (dtau - f.b * sigma * dp[3] * kLg10) / (f.a * sigma + f.damping * velocity) / kLg10;
+    dp[1] = dtau - f.damping * velocity * dp[4] * kLg10;
+    dp[2] = t[1];
+}
+
 // 2-D antiplane volume (fortran/2d/antiplane/src/viscouscycles.f90:846-877):
 // state [e12, e13, s12, s13]; strain rate eII s with
 // eII = gammadot0 (sII/Gm)^(n-1) / Gm exp(-Q/(R To)), v = that less the
@@ -144,27 +170,30 @@ __device__ __forceinline__ void patch_postpass(const double *p, const Friction &
 
 __device__ __forceinline__ int elem_state_off(const PlainDims &d, int e)
 {
+    if (d.ps) return 5 * e;
     if (d.ap) return 4 * e;
     return e < d.np ? 7 * e : 7 * d.np + 12 * (e - d.np);
 }
 __device__ __forceinline__ int elem_v_off(const PlainDims &d, int e)
 {
+    if (d.ps) return e;
     if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 2 * e : 2 * d.np + 6 * (e - d.np);
 }
 __device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
 {
+    if (d.ps) return 2 * e;
     if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 3 * e : 3 * d.np + 6 * (e - d.np);
 }
 // state entries and slip-rate entries of element e
 __device__ __forceinline__ int elem_nstate(const PlainDims &d, int e)
 {
-    return d.ap ? 4 : (e < d.np ? 7 : 12);
+    return d.ps ? 5 : d.ap ? 4 : (e < d.np ? 7 : 12);
 }
 __device__ __forceinline__ int elem_nv(const PlainDims &d, int e)
 {
-    return d.ap ? (e < d.np ? 1 : 2) : (e < d.np ? 2 : 6);
+    return d.ps ? 1 : d.ap ? (e < d.np ? 1 : 2) : (e < d.np ? 2 : 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -327,7 +356,11 @@ __global__ void prepass_kernel(PlainDims d, const Friction *__restrict__ fr,
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (d.ap && e >= d.np) {
+    if (d.ps) {
+        for (int k = 1; k < 5; k++) dydt[so + k] = 0.0;
+        const double vel = ps_prepass(y + so, fr[e], dydt + so, v + vo);
+        if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
+    } else if (d.ap && e >= d.np) {
         dydt[so + 2] = dydt[so + 3] = 0.0;
         ap_volume_prepass(y + so, rh[e - d.np], dydt + so, v + vo);
     } else if (d.ap) {
@@ -385,7 +418,19 @@ __device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const KOperan
 {
     const int so = elem_state_off(d, e);
     const int64_t ro = elem_row_off(d, e);
-    if (d.ap && e >= d.np) {
+    if (d.ps) {
+        double acc[2];
+        elem_rows<2, MODE>(K, ro, d.n, v, t0, lane, acc);
+        if (lane == 0) {
+            if (t_out) { t_out[ro] = acc[0]; t_out[ro + 1] = acc[1]; }
+            ps_postpass(y + so, fr[e], acc, dydt + so);
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 5; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    } else if (d.ap && e >= d.np) {
         double acc[2];
         elem_rows<2, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
@@ -509,7 +554,10 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
     }
     // ctl->vmax was cleared by the controller that accepted the previous step
     if (ctl->done || !ctl->need_rhs) return;
-    if (d.ap && e >= d.np) {
+    if (d.ps) {
+        const double vel = ps_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
+    } else if (d.ap && e >= d.np) {
         // eMax of the step-start odefun (viscouscycles.f90 (2-D antiplane):869)
         const double en = ap_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
@@ -589,7 +637,8 @@ __device__ __forceinline__ void stage_elem(const PlainDims &d, const Friction *_
                                 b.ak[5][i]);
     }
     double *out = b.ak[stage];
-    if (d.ap && e >= d.np) ap_volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
+    if (d.ps) ps_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    else if (d.ap && e >= d.np) ap_volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
     else if (d.ap) ap_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
@@ -778,7 +827,7 @@ __global__ void obs_log_kernel(RkBuffers b, Control *ctl)
 // as buildO splits O into Of and Ol (greens.f90:388-445): fault columns and
 // the volumes' third component go to df, the volumes' first two to dl.
 __global__ void __launch_bounds__(256)
-disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, int ap,
+disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, int model,
                  const double *__restrict__ y, double *d, double *df, double *dl,
                  const Control *ctl)
 {
@@ -790,15 +839,20 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, 
         off = row * rows;
     }
     const int64_t r = blockIdx.x;
-    const bool vertical = !ap && (r % 3) == 2;
+    // model 0: the 3-D model (3 rows per point), 1: antiplane (1 row),
+    // 2: plane strain (2 rows: east, down; one dip slip per patch)
+    const bool ap = model == 1, ps = model == 2;
+    const bool vertical = model == 0 && (r % 3) == 2;
     const double *Or = O + r * n;
     // the 2-D antiplane model: one slip per patch (state entry 0 of 4), then
     // e12, e13 per volume (entries 0, 1 of 4); one displacement component per point
-    const int64_t nf = ap ? np : 2 * (int64_t)np, vbase = 7 * (int64_t)np;
+    const int64_t nf = (ap || ps) ? np : 2 * (int64_t)np, vbase = 7 * (int64_t)np;
     double a = 0.0, af = 0.0, al = 0.0;
     for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         double u;
-        if (ap) {
+        if (ps) {
+            u = y[5 * j];
+        } else if (ap) {
             u = j < np ? y[4 * j] : y[4 * np + 4 * ((j - np) >> 1) + ((j - np) & 1)];
         } else if (j < nf) {
             u = y[7 * (j >> 1) + (j & 1)];
@@ -838,12 +892,12 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, 
     }
 }
 
-void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int ap,
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int model,
                       const double *y, double *d, double *df, double *dl, const Control *ctl)
 {
     if (rows <= 0) return;
     hipLaunchKernelGGL(disp_gemv_kernel, dim3((unsigned)rows), dim3(256), 0, st, O, rows, n, np,
-                       ap, y, d, df, dl, ctl);
+                       model, y, d, df, dl, ctl);
 }
 
 void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl)

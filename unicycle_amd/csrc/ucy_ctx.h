@@ -22,9 +22,12 @@ struct ucy_ctx {
     // elastic moduli
     double lambda = 0, mu = 0;
     bool have_elastic = false;
-    // 0: the 3-D model; 1: the 2-D antiplane model (its segments use the
-    // rectangle arrays: x, width, dip)
+    // 0: the 3-D model; 1: the 2-D antiplane model; 2: the 2-D plane-strain
+    // model (the 2-D segments use the rectangle arrays: x, width, dip)
     int model = 0;
+    // plane strain: initial dip traction per patch where positive (the deck's
+    // tau0, fortran/2d/planestrain/src/ratestate.f90:604-609); empty: none
+    std::vector<double> tau0;
     // rectangles
     int nrect = 0;
     std::vector<double> rx, rL, rW, rstrike, rdip;

@@ -62,6 +62,9 @@ struct PlainDims {
                  // entries per element; per patch one slip rate and one
                  // traction row, per volume two strain rates e12, e13 and two
                  // stress rows s12, s13)
+    int ps;      // 1: the 2-D plane-strain model (fortran/2d/planestrain: 5
+                 // state entries per patch, one dip-slip rate, two traction
+                 // rows: dip and normal)
 };
 
 // ---- assembly launchers (k_assemble.hip) ----
@@ -83,6 +86,20 @@ void launch_ap_rows(hipStream_t st, const double *src, int ns, const Receiver *r
 // antiplane buildO: row k = point k (x(3,npts), x2 and x3 used), column s
 void launch_ap_disp(hipStream_t st, const double *src, int ns, const double *pts, int npts,
                     double *O, int64_t ldo);
+// plane-strain sources (fortran/2d/planestrain): top (q2, q3), width W and the
+// sines and cosines of 1..4 times the dip, formed on the host
+struct PsSource {
+    double q2, q3, W;
+    double sk[4], ck[4];
+};
+// plane-strain buildG rows 2r, 2r+1 (dip, normal traction) for receivers
+// [0, nr) from ns sources; td, tn (one receiver) may replace K
+void launch_ps_rows(hipStream_t st, const PsSource *src, int ns, const Receiver *rcv, int nr,
+                    double G, double lam, double nu, double *K, int64_t ldk, double *td,
+                    double *tn);
+// plane-strain buildO rows 2k, 2k+1 (east, down) for points [0, npts)
+void launch_ps_disp(hipStream_t st, const PsSource *src, int ns, const double *pts, int npts,
+                    double G, double lam, double nu, double *O, int64_t ldo);
 // antiplane buildG blocks that involve strain volumes: every receiver (np
 // patches, then nv volumes) against the volume sources vol[4q..4q+3] =
 // (q2, q3, W, T), and the volume receivers against the segments seg
@@ -147,7 +164,7 @@ void launch_obs_log(hipStream_t st, const RkBuffers &b, Control *ctl);
 // strain (6 per volume) entries of y (ap: the slip, entry 0 of 4 per patch); df, dl (may be null): the parts of buildO's
 // Of and Ol.  With ctl: written at log row log_count - 1 of the arrays, only in
 // the attempt that accepted a step.
-void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int ap,
+void launch_disp_gemv(hipStream_t st, const double *O, int64_t rows, int64_t n, int np, int model,
                       const double *y, double *d, double *df, double *dl, const Control *ctl);
 // standalone Cash-Karp combination kernels for the ucy_rkck entry point
 void launch_rk_combine(hipStream_t st, int n, int stage, double h, const double *yin,

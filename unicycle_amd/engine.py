@@ -86,6 +86,7 @@ class Engine:
         self.np = 0
         self.nvol = 0
         self.antiplane = False
+        self.planestrain = False
         self._ensembles = []
 
     # -- plumbing ---------------------------------------------------------
@@ -128,8 +129,40 @@ class Engine:
         self._chk(self._L.ucy_set_antiplane(self._h, self.mu, x.shape[0],
                                             *[_p(a) for a in self._ap]))
         self.antiplane = True
+        self.planestrain = False
         self.nrect, self.ntri, self.nvol = x.shape[0], 0, 0
         self._update_np()
+
+    def set_planestrain(self, lambda_, mu, x, width, dip):
+        """The 2-D plane-strain model (fortran/2d/planestrain): segments with
+        upper corners x (ns, 3; x2, x3 used), width, dip (radians), slipping
+        along dip in a half space of Lame parameters lambda_, mu.  State: 5
+        entries per patch (dip slip, dip and normal traction, log10 theta,
+        log10 V)."""
+        x = _f64(x).reshape(-1, 3)
+        self._ps = [x, _f64(width), _f64(dip)]
+        self.lambda_, self.mu = float(lambda_), float(mu)
+        self._chk(self._L.ucy_set_planestrain(self._h, self.lambda_, self.mu, x.shape[0],
+                                              *[_p(a) for a in self._ps]))
+        self.antiplane, self.planestrain = False, True
+        self.nrect, self.ntri, self.nvol = x.shape[0], 0, 0
+        self._update_np()
+
+    def set_initial_traction(self, tau0):
+        """The plane-strain deck's tau0: a positive entry replaces the
+        patch's initial dip traction (ratestate.f90 of 2d/planestrain:604-609)."""
+        self._tau0 = _f64(tau0)
+        self._chk(self._L.ucy_set_initial_traction(self._h, self._tau0.size, _p(self._tau0)))
+
+    def computeTractionKernelsPlaneStrain(self, x, sv, dv, nv, ns, y, W, dip, G, lambda_):
+        """planestrain.f90:266-291: (td, tn) on one receiver from unit dip slip
+        on ns sources."""
+        td, tn = np.empty(ns), np.empty(ns)
+        self._chk(self._L.ucy_planestrain_traction_kernels(
+            self._h, _p(_f64(x)), _p(_f64(sv)), _p(_f64(dv)), _p(_f64(nv)), ns,
+            _p(_f64(y).reshape(-1)), _p(_f64(W)), _p(_f64(dip)), float(G), float(lambda_),
+            _p(td), _p(tn)))
+        return td, tn
 
     def set_antiplane_volumes(self, x, thickness, width, dip):
         """Strain volumes of the 2-D antiplane viscouscycles model
@@ -146,7 +179,7 @@ class Engine:
     def set_rectangles(self, x, length, width, strike, dip):
         x = _f64(x).reshape(-1, 3)
         ns = x.shape[0]
-        self.antiplane = False
+        self.antiplane = self.planestrain = False
         self._rect = [x, _f64(length), _f64(width), _f64(strike), _f64(dip)]
         self.nrect = ns
         self._chk(self._L.ucy_set_rectangles(self._h, ns, *[_p(a) for a in self._rect]))
@@ -157,8 +190,8 @@ class Engine:
         idx = [np.ascontiguousarray(np.asarray(a, dtype=np.int32)) for a in (i1, i2, i3)]
         v = _f64(vertices).reshape(-1, 3)
         self._tri = idx + [v]
-        if self.antiplane:
-            self.nrect, self.antiplane = 0, False
+        if self.antiplane or self.planestrain:
+            self.nrect, self.antiplane, self.planestrain = 0, False, False
         self.ntri = idx[0].size
         self._chk(self._L.ucy_set_triangles(self._h, self.ntri, _pi(idx[0]), _pi(idx[1]),
                                             _pi(idx[2]), v.shape[0], _p(v)))
@@ -167,8 +200,8 @@ class Engine:
     def set_volumes(self, x, length, thickness, width, strike):
         x = _f64(x).reshape(-1, 3)
         self._vol = [x, _f64(length), _f64(thickness), _f64(width), _f64(strike)]
-        if self.antiplane:
-            self.nrect, self.antiplane = 0, False
+        if self.antiplane or self.planestrain:
+            self.nrect, self.antiplane, self.planestrain = 0, False, False
             self._update_np()
         self.nvol = x.shape[0]
         self._chk(self._L.ucy_set_volumes(self._h, self.nvol, *[_p(a) for a in self._vol]))
@@ -217,6 +250,8 @@ class Engine:
 
     @property
     def dim(self) -> int:
+        if self.planestrain:
+            return 5 * self.np
         return 4 * (self.np + self.nvol) if self.antiplane else 7 * self.np + 12 * self.nvol
 
     def get_kernel(self) -> np.ndarray:
