@@ -67,10 +67,11 @@ def test_ensemble_matvec(engine_factory, precision):
     Ks = [orc.buildG(orc.Mesh(m.lambda_, m.mu, rect=m.geometry()), nthreads=8) for m in groups]
     # every GEMM variant of the autotuner, at split depths that give one and
     # several k-splits (the groups have 8 and 16 k-blocks)
-    nvar = {PREC_F64: 1, PREC_BF16_I8: 8}.get(precision, 6)   # 6, 7: bf16+i8 only
-    tilings = [(kb, v) for v in range(nvar) for kb in (4, 64)]
+    variants = {PREC_F64: [0], PREC_BF16_I8: list(range(10))}.get(
+        precision, [0, 1, 2, 3, 4, 5, 9])   # 6-8: bf16+i8 only
+    tilings = [(kb, v) for v in variants for kb in (4, 64)]
     if precision in (PREC_SPLIT_BF16, PREC_MIXED):
-        for v in (6, 7):
+        for v in (6, 7, 8):
             with pytest.raises(UcyError):
                 ens.set_tiling(64, v)
     for kb, v in tilings:
@@ -114,7 +115,8 @@ def test_stored_kernel_format(engine_factory, precision):
     n = K.shape[1]
     rowmax = np.abs(K).max(axis=1, keepdims=True)
     bound = 2.0 ** -16
-    tilings = ((4, 0), (64, 3), (4, 4)) + (((4, 6), (64, 7)) if precision == PREC_BF16_I8 else ())
+    tilings = ((4, 0), (64, 3), (4, 4), (4, 9)) + (((4, 6), (64, 7), (4, 8))
+                                                   if precision == PREC_BF16_I8 else ())
     for kb, v in tilings:
         ens.set_tiling(kb, v)
         Kt = np.concatenate([ens.matvec(0, np.eye(n)[:, j:j + 64]) for j in range(0, n, 64)], axis=1)

@@ -5,8 +5,9 @@ waves of a block take the slices of one row item and k-split).
 
 - matvec: every slice against K V in binary64, within the bound of each
   precision (test_gpu_ensemble.py states the bounds), on every tiling
-  variant the shared form admits; the LDS variants, which stage one V per
-  block, are refused.
+  variant the shared form admits (with bf16+i8 K, the wave-shaped
+  direct-to-LDS variants too); the variants that stage one V per block are
+  refused.
 - stepping: a family of two slices runs the same arithmetic as the same
   instances added as two separate groups (each with its own copy of K) on
   the same tiling, so every instance's state, time and counters agree bit
@@ -53,7 +54,7 @@ def test_shared_k_matvec(engine_factory, precision):
     Kb = orc.buildG(orc.Mesh(mb.lambda_, mb.mu, rect=mb.geometry()), nthreads=8)
     Ks = [Ka, Ka, Ka, Kb]
     rng = np.random.default_rng(0)
-    variants = [0] if precision == PREC_F64 else [0, 1, 2, 5]
+    variants = {PREC_F64: [0], PREC_BF16_I8: [0, 1, 2, 5, 8, 9]}.get(precision, [0, 1, 2, 5, 9])
     for kb, v in [(kb, v) for v in variants for kb in (4, 64)]:
         ens.set_tiling(kb, v)
         for g, K in enumerate(Ks):

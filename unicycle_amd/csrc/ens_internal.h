@@ -104,7 +104,11 @@ struct EnsTable {
 // 6 = 64 rows per wave, 7 = 128 rows: every load is a direct global->LDS
 // load, each wave's K tiles in its own LDS ring and V in a ring shared by the
 // block, 4 (6) or 3 (7) k-blocks deep (bf16+i8 only)
-constexpr int kGemmVariants = 8;
+// 8 = 128 rows, wave-shaped as 0-5, K tiles by direct global->LDS loads
+// into a ring private to the wave and V into a register ring, 3 k-blocks
+// deep, no barrier (bf16+i8 only); 9 = 128 rows in two register sets used
+// in turn, the loads of one k-block out while the other is multiplied
+constexpr int kGemmVariants = 10;
 int ens_gemm_variant_rows(int variant);
 int ens_gemm_variant_lds(int variant);
 int ens_gemm_variant_dma(int variant);

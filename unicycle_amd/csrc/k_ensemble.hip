@@ -382,6 +382,99 @@ __device__ __forceinline__ void gemm_tile_bf16(const EnsGroup &G, int rb0, int s
             }
 }
 
+// The same tile with two register sets used in turn (variant 9): the loop
+// takes k-blocks in pairs, loads the odd one into set 1 before multiplying
+// set 0 and the next even one into set 0 before multiplying set 1, so every
+// wait covers only the older set (a copy from a "next" set to a "current"
+// one would wait for the loads just issued).  Loads are unconditional: a
+// k-block past the range loads the last one again and multiplies a zero V.
+template <int RB, bool I8, bool NT>
+__device__ __forceinline__ void gemm_tile_bf16_2set(const EnsGroup &G, int rb0, int sp, int lane)
+{
+    const int kb0 = sp * G.kb_per_split;
+    const int kb1 = min(kb0 + G.kb_per_split, G.nkb);
+    floatx16 acc[RB][2];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const char *A = reinterpret_cast<const char *>(G.Ktiles) + (size_t)rb0 * G.nkb * kTileBytes<I8>;
+    const size_t rstride = (size_t)G.nkb * kTileBytes<I8>;
+    const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
+    u32x4 ca[2][RB][2], cb[2][4];
+    u32x2 cq[2][RB];
+    auto load = [&](int kb, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        kb = min(kb, kb1 - 1);
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const char *t = A + a * rstride + (size_t)kb * kTileBytes<I8>;
+            ca[s][a][0] = kload<NT>(reinterpret_cast<const u32x4 *>(t) + lane);
+            if constexpr (I8)
+                cq[s][a] = kload<NT>(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+            else
+                ca[s][a][1] = kload<NT>(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) cb[s][q] = *as_global(Bv + (size_t)kb * 256 + 64 * q);
+    };
+    auto mult = [&](bool live, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+        u32x4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = live ? cb[s][q] : zero;
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, v[0]), __builtin_bit_cast(bf16x8, v[1])},
+                                {__builtin_bit_cast(bf16x8, v[2]), __builtin_bit_cast(bf16x8, v[3])}};
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, ca[s][a][0]);
+            bf16x8 Al;
+            if constexpr (I8)
+                Al = decode_lo8(ca[s][a][0], cq[s][a]);
+            else
+                Al = __builtin_bit_cast(bf16x8, ca[s][a][1]);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
+            }
+        }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    // the scheduler keeps the four phases in order (it would otherwise
+    // hoist both sets' loads and interleave the sets' products, so that
+    // every iteration waits for all its loads)
+    if (kb0 < kb1) {
+        load(kb0, S0{});
+        for (int kb = kb0; kb < kb1; kb += 2) {
+            load(kb + 1, S1{});
+            __builtin_amdgcn_sched_barrier(0);
+            mult(true, S0{});
+            __builtin_amdgcn_sched_barrier(0);
+            load(kb + 2, S0{});
+            __builtin_amdgcn_sched_barrier(0);
+            mult(kb + 1 < kb1, S1{});
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+}
+
 // row items of RB 32-row blocks: mp is a multiple of 64, so the last item of
 // RB = 4 may hold two blocks
 __host__ __device__ __forceinline__ int bf16_row_items(int mp, int rb)
@@ -574,13 +667,159 @@ __device__ __forceinline__ void gemm_block_bf16_dma(const EnsGroup &G, int ri, i
     }
 }
 
+// The K tiles of one ring slot (no V): RB x (hi 16 bytes, codes 8 bytes) per
+// lane, waiting for its own reads as lds_read_slot does
+template <int RB, int KO>
+__device__ __forceinline__ void lds_read_k(unsigned kh, unsigned kq, u32x4 (&h)[RB], u32x2 (&q)[RB])
+{
+    static_assert(KO + (RB - 1) * 1536 + 1024 < 65536, "ds offsets");
+    if constexpr (RB == 4)
+        asm volatile(
+            "ds_read_b128 %0, %8 offset:%10\n"
+            "ds_read_b128 %1, %8 offset:%11\n"
+            "ds_read_b128 %2, %8 offset:%12\n"
+            "ds_read_b128 %3, %8 offset:%13\n"
+            "ds_read_b64 %4, %9 offset:%10\n"
+            "ds_read_b64 %5, %9 offset:%11\n"
+            "ds_read_b64 %6, %9 offset:%12\n"
+            "ds_read_b64 %7, %9 offset:%13\n"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(h[0]), "=v"(h[1]), "=v"(h[2]), "=v"(h[3]), "=v"(q[0]), "=v"(q[1]), "=v"(q[2]),
+              "=v"(q[3])
+            : "v"(kh), "v"(kq), "n"(KO), "n"(KO + 1536), "n"(KO + 3072), "n"(KO + 4608)
+            : "memory");
+    else
+        asm volatile(
+            "ds_read_b128 %0, %4 offset:%6\n"
+            "ds_read_b128 %1, %4 offset:%7\n"
+            "ds_read_b64 %2, %5 offset:%6\n"
+            "ds_read_b64 %3, %5 offset:%7\n"
+            "s_waitcnt lgkmcnt(0)"
+            : "=v"(h[0]), "=v"(h[1]), "=v"(q[0]), "=v"(q[1])
+            : "v"(kh), "v"(kq), "n"(KO), "n"(KO + 1536)
+            : "memory");
+}
 
-template <int RB, bool PIPE, bool I8, bool NT>
+// Variants 8 and 9: the wave-shaped work items of variants 0-5 (one row
+// item, k-split and slice per wave, no barrier), with the wave's K tiles
+// streamed D k-blocks ahead by direct global->LDS loads into a ring private
+// to the wave, and the V tile of the same k-blocks loaded into a register
+// ring of the same depth.  K in flight per wave is (D - 1) k-blocks without
+// their registers.  Per k-block: wait for this wave's loads of the k-block
+// (retired in issue order: the loads of the D - 2 later k-blocks may still
+// be out), issue the k-block D - 1 ahead into the slot read at the previous
+// step (its reads waited for themselves), read the slot, multiply.  NT: the
+// K loads are non-temporal (one slice per family; slices of a shared K find
+// its tiles in L2).
+template <int RB, int D, bool NT>
+__device__ __forceinline__ void gemm_wave_bf16_dmav(const EnsGroup &G, int item, int lane,
+                                                    char *kring)
+{
+    constexpr int TB = 1536;
+    constexpr int PER = 3 * RB + 4;   // loads per k-block: 3 per K tile, 4 of V
+    static_assert(D >= 2 && (D - 2) * PER < 64, "ring depth");
+    const int nrb = G.mp >> 5;
+    const int nri = bf16_row_items(G.mp, RB);
+    const int rb0 = (item % nri) * RB, sp = item / nri;
+    const int nvr = min(RB, nrb - rb0);   // RB, or 2 on the tail item
+    const int kb0 = sp * G.kb_per_split;
+    const int nk = min(kb0 + G.kb_per_split, G.nkb) - kb0;
+    floatx16 acc[RB][2];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const char *Kt = reinterpret_cast<const char *>(G.Ktiles);
+    const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
+    typedef __attribute__((address_space(3))) void lds_void;
+    u32x4 vr[D][4];
+    // the loads of k-block kb0 + j (clamped to the last one; rows past the
+    // last block load a valid tile they do not use, so every k-block is PER
+    // loads) into slot `slot`
+    auto issue = [&](int j, auto slot_c) {
+        constexpr int slot = decltype(slot_c)::value;
+        const int kb = kb0 + min(j, nk - 1);
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const char *t = Kt + ((size_t)min(rb0 + a, nrb - 1) * G.nkb + kb) * TB;
+            char *l = kring + (slot * RB + a) * TB;
+            // full-wave loads only (the LDS base is wave-uniform); cache
+            // policy 2 = nt
+            __builtin_amdgcn_global_load_lds((const void *)(t + 16 * lane), (lds_void *)l, 16, 0,
+                                             NT ? 2 : 0);
+            __builtin_amdgcn_global_load_lds((const void *)(t + 1024 + 4 * lane),
+                                             (lds_void *)(l + 1024), 4, 0, NT ? 2 : 0);
+            __builtin_amdgcn_global_load_lds((const void *)(t + 1280 + 4 * lane),
+                                             (lds_void *)(l + 1280), 4, 0, NT ? 2 : 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) vr[slot][q] = *as_global(Bv + (size_t)kb * 256 + 64 * q);
+    };
+    const unsigned kl = (unsigned)(uintptr_t)(__attribute__((address_space(3))) char *)kring;
+    const unsigned kh = kl + 16 * lane, kq = kl + 1024 + 8 * lane;
+    auto step = [&](int j, auto slot_c) {
+        constexpr int slot = decltype(slot_c)::value;
+        wait_vm<(D - 2) * PER>();          // this wave's loads of k-block j
+        issue(j + D - 1, std::integral_constant<int, (slot + D - 1) % D>{});
+        u32x4 hh[RB];
+        u32x2 qq[RB];
+        lds_read_k<RB, slot * RB * TB>(kh, kq, hh, qq);
+        // past the range V counts as 0: the products are exact zeros
+        const bool live = j < nk;
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+        u32x4 v4[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v4[q] = live ? vr[slot][q] : zero;
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, v4[0]), __builtin_bit_cast(bf16x8, v4[1])},
+                                {__builtin_bit_cast(bf16x8, v4[2]), __builtin_bit_cast(bf16x8, v4[3])}};
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            if (a >= nvr) continue;
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, hh[a]);
+            const bf16x8 Al = decode_lo8(hh[a], qq[a]);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
+            }
+        }
+    };
+    if (nk > 0) {
+        static_for<D - 1>([&](auto s_c) { issue(decltype(s_c)::value, s_c); });
+        for (int j = 0; j < nk; j += D)
+            static_for<D>([&](auto s_c) { step(j + decltype(s_c)::value, s_c); });
+        wait_vm<0>();   // no LDS write may land after the wave has left
+    }
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < RB; a++) {
+        if (a >= nvr) continue;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+    }
+}
+
+
+template <int RB, bool PIPE, bool TWO, bool I8, bool NT>
 __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
 {
     const int nrb = G.mp >> 5;
     const int nri = bf16_row_items(G.mp, RB);
     const int rb0 = (item % nri) * RB, sp = item / nri;
+    if constexpr (TWO) {
+        if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16_2set<RB, I8, NT>(G, rb0, sp, lane);
+        else gemm_tile_bf16_2set<2, I8, NT>(G, rb0, sp, lane);
+        return;
+    }
     if (RB == 2 || rb0 + RB <= nrb) gemm_tile_bf16<RB, PIPE, I8, NT>(G, rb0, sp, lane);
     else gemm_tile_bf16<2, PIPE, I8, NT>(G, rb0, sp, lane);
 }
@@ -592,11 +831,17 @@ struct Bf16Variant {
     bool pipe;
     int lds_chunk;
     int dma;        // > 0: K tiles through the wave's LDS ring this many k-blocks deep
-                    // (gemm_tile_bf16_dma, bf16+i8 only)
+                    // (bf16+i8 only)
+    bool vreg;      // with dma: V in a register ring of the same depth, wave-shaped
+                    // (gemm_wave_bf16_dmav); else V in a ring shared by the block
+                    // (gemm_block_bf16_dma)
+    bool two_sets;  // two register sets in turn (gemm_tile_bf16_2set)
 };
 constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
-    {2, false, 0, 0}, {2, true, 0, 0}, {4, true, 0, 0}, {4, false, 4, 0}, {2, true, 8, 0},
-    {4, false, 0, 0}, {2, false, 0, 4}, {4, false, 0, 3}};
+    {2, false, 0, 0, false, false}, {2, true, 0, 0, false, false}, {4, true, 0, 0, false, false},
+    {4, false, 4, 0, false, false}, {2, true, 8, 0, false, false}, {4, false, 0, 0, false, false},
+    {2, false, 0, 4, false, false}, {4, false, 0, 3, false, false}, {4, false, 0, 3, true, false},
+    {4, false, 0, 0, false, true}};
 
 // LDS variants: the 4 waves of a block take 4 consecutive row items of one
 // k-split; the block stages the split's V tiles through LDS in chunks of CK
@@ -792,6 +1037,23 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restri
         gemm_block_bf16_lds<RB, kBf16Variants[V].pipe, CK, I8>(G, 4 * (local % nrg) + wave,
                                                                local / nrg, wave, threadIdx.x & 63,
                                                                lds);
+    } else if constexpr (kBf16Variants[V].dma > 0 && kBf16Variants[V].vreg) {
+        // wave-shaped as below, each wave with its own K ring; bf16+i8 K
+        // only (the tuner admits it only then)
+        if constexpr (I8) {
+            constexpr int RB = kBf16Variants[V].rb, DD = kBf16Variants[V].dma;
+            __shared__ __attribute__((aligned(16))) char kring[4 * DD * RB * 1536];
+            // wave-uniform, stated so the tile addresses stay scalar
+            const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int item = (blk - tab->blk_gemm[f]) * 4 + wave;
+            const EnsGroup &H = tab->g[head];
+            if (item >= bf16_row_items(H.mp, RB) * H.ksplit * S) return;
+            const EnsGroup &G = tab->g[head + item % S];
+            if (stage == 1 && !*G.any_rhs) return;
+            char *ring = kring + wave * (DD * RB * 1536);
+            if (S == 1) gemm_wave_bf16_dmav<RB, DD, true>(G, item, threadIdx.x & 63, ring);
+            else gemm_wave_bf16_dmav<RB, DD, false>(G, item / S, threadIdx.x & 63, ring);
+        }
     } else if constexpr (kBf16Variants[V].dma > 0) {
         // one slice per family (block-shaped, as the V-staging variants);
         // bf16+i8 K only (the tuner admits it only then)
@@ -812,13 +1074,16 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restri
         // work items (row item, k-split, slice), slice fastest: the waves of
         // a block take the slices of one row item and k-split, so the
         // family's K tiles leave HBM once for all of them
-        const int item = (blk - tab->blk_gemm[f]) * 4 + (threadIdx.x >> 6);
+        // wave-uniform, stated so that the group's fields come by scalar
+        // loads and the tile addresses stay scalar
+        const int item = (blk - tab->blk_gemm[f]) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         const EnsGroup &H = tab->g[head];
         if (item >= bf16_row_items(H.mp, RB) * H.ksplit * S) return;
         const EnsGroup &G = tab->g[head + item % S];
         if (stage == 1 && !*G.any_rhs) return;
-        if (S == 1) gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8, true>(G, item, threadIdx.x & 63);
-        else gemm_wave_bf16<RB, kBf16Variants[V].pipe, I8, false>(G, item / S, threadIdx.x & 63);
+        constexpr bool P = kBf16Variants[V].pipe, TW = kBf16Variants[V].two_sets;
+        if (S == 1) gemm_wave_bf16<RB, P, TW, I8, true>(G, item, threadIdx.x & 63);
+        else gemm_wave_bf16<RB, P, TW, I8, false>(G, item / S, threadIdx.x & 63);
     }
 }
 
@@ -831,7 +1096,7 @@ int ens_gemm_variant_dma(int v)
 // per block, one slice per K family)
 int ens_gemm_variant_lds(int v)
 {
-    return kBf16Variants[v].lds_chunk > 0 || kBf16Variants[v].dma > 0;
+    return kBf16Variants[v].lds_chunk > 0 || (kBf16Variants[v].dma > 0 && !kBf16Variants[v].vreg);
 }
 
 int ens_gemm_variant_rows(int v)
@@ -845,7 +1110,7 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(const EnsTable *__restric
     const int f = group_of(tab->blk_gemm64, tab->nfam, blk);
     const int head = tab->fam_head[f], S = tab->fam_slices[f];
     const EnsGroup &H = tab->g[head];
-    const int item = (blk - tab->blk_gemm64[f]) * 4 + (threadIdx.x >> 6);
+    const int item = (blk - tab->blk_gemm64[f]) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (item >= (H.mp >> 6) * H.ksplit * S) return;
     const EnsGroup &G = tab->g[head + item % S];
     if (stage == 1 && !*G.any_rhs) return;
@@ -865,7 +1130,9 @@ static void launch_gemm_bf16(hipStream_t st, const EnsTable *dtab, const EnsTabl
     case 4: hipLaunchKernelGGL((gemm_bf16_kernel<4, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 5: hipLaunchKernelGGL((gemm_bf16_kernel<5, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 6: hipLaunchKernelGGL((gemm_bf16_kernel<6, I8>), grid, dim3(256), 0, st, dtab, stage); break;
-    default: hipLaunchKernelGGL((gemm_bf16_kernel<7, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 7: hipLaunchKernelGGL((gemm_bf16_kernel<7, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 8: hipLaunchKernelGGL((gemm_bf16_kernel<8, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<9, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     }
 }
 

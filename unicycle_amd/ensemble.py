@@ -199,7 +199,10 @@ class Ensemble:
         staged in LDS in chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged
         in chunks of 8, 5 = 128 rows, 6 = 64 rows and 7 = 128 rows with
         every load a direct global->LDS load (K per wave, V per block), 4 or
-        3 k-blocks deep (6, 7: bf16+i8 only)."""
+        3 k-blocks deep, 8 = 128 rows with K per wave by direct
+        global->LDS loads and V in registers, 3 k-blocks deep, no barrier
+        (6-8: bf16+i8 only), 9 = 128 rows in two register sets used in
+        turn."""
         kb, var = C.c_int(), C.c_int()
         t = np.full(_lib.GEMM_TUNE_ENTRIES, -1.0)
         self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
