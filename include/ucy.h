@@ -195,6 +195,33 @@ int ucy_set_planestrain(ucy_ctx *ctx, double lambda, double mu, int ns, const do
  * initial dip traction (ratestate.f90 of 2d/planestrain:604-609); reset by
  * ucy_set_planestrain */
 int ucy_set_initial_traction(ucy_ctx *ctx, int np, const double *tau0);
+/* Strain volumes of the plane-strain viscouscycles program
+ * (fortran/2d/planestrain/src/viscouscycles.f90): nv volumes with top
+ * x(3,nv) (x(2), x(3) used), thickness, width and dip (radians), after
+ * ucy_set_planestrain (which clears them).  Depths below zero give
+ * UCY_ERR_GEOMETRY and dimensions not positive UCY_ERR_ARG (:1559-1576).
+ * The state gains 6 entries per volume after the patches (e22, e23, e33,
+ * s22, s23, s33; macros.f90:73-79); K gains three stress rows per volume
+ * (s22', s23', s33' in the volume's frame) and three strain columns per
+ * volume, each a strain of 1e-3 (greens.f90:66-226), so that K is
+ * (2 np + 3 nv) x (np + 3 nv).  As greens.f90:212-216 passes them, the
+ * volume rows of patch sources take the moduli exchanged (G = lambda,
+ * lambda = mu).  ucy_set_rheology supplies the nonlinear Maxwell law per
+ * volume, with the background strain rate in edot_bg[6 i + 3] (e22),
+ * [6 i + 4] (e23), [6 i + 5] (e33) and Gm unused: eII = gammadot0
+ * sII^(n-1) exp(-Q/(R To)) on the 2-D deviatoric stress, v = (e - e_bg) 1e3
+ * (viscouscycles.f90:929-967); the initial stress flows at the background
+ * rate under the same law (:777-810).  Displacement kernels gain three
+ * columns per volume, unit e22' and e23' strains and a zero column for e33'
+ * (the reference's eye there has a zero third row, greens.f90:270-273,
+ * 334-344). */
+int ucy_set_planestrain_volumes(ucy_ctx *ctx, int nv, const double *x, const double *thickness,
+                                const double *width, const double *dip);
+/* the plane-strain volumes' initial stress s0(3,nv) (s22, s23, s33): a
+ * volume whose three values are not all zero starts from them instead of
+ * the background-rate stress (viscouscycles.f90 of 2d/planestrain:779-810);
+ * reset by ucy_set_planestrain and ucy_set_planestrain_volumes */
+int ucy_set_initial_volume_stress(ucy_ctx *ctx, int nv, const double *s0);
 /* computeTractionKernelsPlaneStrain (planestrain.f90:266-291): dip and normal
  * traction on one receiver (x, sv, dv, nv) of unit dip slip on ns sources */
 int ucy_planestrain_traction_kernels(ucy_ctx *ctx, const double x[3], const double sv[3],

@@ -15,7 +15,14 @@
  *     slip column per source);
  *   ratestate.f90:585-627 (initStateVector, with the tau0 option) and
  *     :637-753 (odefun); the main loop is ratestate.f90:266-295 with RKQSm
- *     of ode45.f90:51-108, the 3-D loop's arithmetic (orc_rkqs).
+ *     of ode45.f90:51-108, the 3-D loop's arithmetic (orc_rkqs);
+ *   with strain volumes, the viscouscycles program of the same directory:
+ *     planestrain.f90:314-475 (volume-frame stress kernels, traction
+ *     kernels of volumes), greens.f90:66-226 (all four blocks, unit strains
+ *     of 1e-3), viscouscycles.f90:714-837 (initStateVector: the volume
+ *     stress from the power law at the background rate, or s0) and
+ *     :841-1065 (odefun: eII = gammadot0 sII^(n-1) exp(-Q/(R To)) with the
+ *     2-D deviatoric stress, v = (e - e_bg) 1e3).
  *
  * The displacement integrals are shared with the device kernel
  * (unicycle_amd/csrc/ps_jint.h, templated on the number type); here they are
@@ -212,6 +219,138 @@ extern "C" int orc_buildG_planestrain(int ns, const double *x, const double *W, 
     return rc;
 }
 
+/* rotation into the receiver's frame of planestrain.f90:332-340 (and
+ * :462-470): s22' = n.s.n, s23' = -n.s.d, s33' = d.s.d, in the reference's
+ * operation order */
+static void ps_volume_frame(const double nv[3], const double dv[3], double s22, double s23,
+                            double s33, double *o22, double *o23, double *o33)
+{
+    *o22 = (nv[1] * s22 + nv[2] * s23) * nv[1] + (nv[1] * s23 + nv[2] * s33) * nv[2];
+    *o23 = -(nv[1] * s22 + nv[2] * s23) * dv[1] - (nv[1] * s23 + nv[2] * s33) * dv[2];
+    *o33 = -(-dv[1] * s22 - dv[2] * s23) * dv[1] - (-dv[1] * s23 - dv[2] * s33) * dv[2];
+}
+
+/* planestrain.f90:314-348 */
+extern "C" int orc_stress_kernels_planestrain(const double x[3], const double sv[3],
+                                              const double dv[3], const double nv[3], int ns,
+                                              const double *y, const double *W,
+                                              const double *dip, double G, double lambda,
+                                              double *s22, double *s23, double *s33)
+{
+    (void)sv;
+    for (int i = 0; i < ns; i++) {
+        double a, b, c;
+        const int rc = orc_stress_planestrain(x[1], x[2], y[3 * i + 1], y[3 * i + 2], W[i], dip[i],
+                                              1.0, G, lambda, &a, &b, &c);
+        if (rc) return rc;
+        ps_volume_frame(nv, dv, a, b, c, s22 + i, s23 + i, s33 + i);
+    }
+    return 0;
+}
+
+/* planestrain.f90:375-410 */
+extern "C" int orc_traction_kernels_sv_planestrain(const double x[3], const double sv[3],
+                                                   const double dv[3], const double nv[3],
+                                                   int ns, const double *y, const double *T,
+                                                   const double *W, const double *dip,
+                                                   double e22p, double e23p, double e33p,
+                                                   double G, double lambda, double *td,
+                                                   double *tn)
+{
+    (void)sv;
+    for (int i = 0; i < ns; i++) {
+        double s22, s23, s33;
+        const int rc = orc_stress_sv_planestrain(x[1], x[2], y[3 * i + 1], y[3 * i + 2], T[i], W[i],
+                                                 dip[i], e22p, e23p, e33p, G, lambda, &s22, &s23,
+                                                 &s33);
+        if (rc) return rc;
+        td[i] = (nv[1] * s22 + nv[2] * s23) * dv[1] + (nv[1] * s23 + nv[2] * s33) * dv[2];
+        tn[i] = (nv[1] * s22 + nv[2] * s23) * nv[1] + (nv[1] * s23 + nv[2] * s33) * nv[2];
+    }
+    return 0;
+}
+
+/* planestrain.f90:435-475 */
+extern "C" int orc_stress_kernels_sv_planestrain(const double x[3], const double sv[3],
+                                                 const double dv[3], const double nv[3], int ns,
+                                                 const double *y, const double *T,
+                                                 const double *W, const double *dip, double e22p,
+                                                 double e23p, double e33p, double G,
+                                                 double lambda, double *s22, double *s23,
+                                                 double *s33)
+{
+    (void)sv;
+    for (int i = 0; i < ns; i++) {
+        double a, b, c;
+        const int rc = orc_stress_sv_planestrain(x[1], x[2], y[3 * i + 1], y[3 * i + 2], T[i], W[i],
+                                                 dip[i], e22p, e23p, e33p, G, lambda, &a, &b, &c);
+        if (rc) return rc;
+        ps_volume_frame(nv, dv, a, b, c, s22 + i, s23 + i, s33 + i);
+    }
+    return 0;
+}
+
+/* greens.f90:66-226 of fortran/2d/planestrain with strain volumes */
+extern "C" int orc_buildG_planestrain_volumes(int np, const double *x, const double *W,
+                                              const double *dip, int nv, const double *vx,
+                                              const double *vT, const double *vW,
+                                              const double *vdip, double G, double lambda,
+                                              double *K, int nthreads)
+{
+    const int ne = np + nv;
+    const size_t n = (size_t)np + 3 * (size_t)nv;
+    double *f = (double *)std::malloc(sizeof(double) * 12 * (size_t)(ne + 1));
+    double *sv = f, *dv = f + 3 * (ne + 1), *nvv = f + 6 * (ne + 1), *xc = f + 9 * (ne + 1);
+    orc_refsys_antiplane(np, x, W, dip, sv, dv, nvv, xc);
+    orc_refsys_antiplane(nv, vx, vW, vdip, sv + 3 * np, dv + 3 * np, nvv + 3 * np, xc + 3 * np);
+    int rc = 0;
+    (void)nthreads;
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1) reduction(min : rc)
+    {
+        double *w = (double *)std::malloc(sizeof(double) * 3 * (size_t)(nv + 1));
+#pragma omp for schedule(dynamic, 4)
+        for (int r = 0; r < ne; r++) {
+            const double *X = xc + 3 * r, *S = sv + 3 * r, *D = dv + 3 * r, *N = nvv + 3 * r;
+            int e = 0;
+            if (r < np) {
+                double *rd = K + (size_t)(2 * r) * n, *rn = rd + n;
+                e = orc_traction_kernels_planestrain(X, S, D, N, np, x, W, dip, G, lambda, rd, rn);
+                for (int j = 0; j < 3 && !e; j++) {
+                    const double u[3] = {j == 0 ? 1e-3 : 0.0, j == 1 ? 1e-3 : 0.0,
+                                         j == 2 ? 1e-3 : 0.0};
+                    e = orc_traction_kernels_sv_planestrain(X, S, D, N, nv, vx, vT, vW, vdip, u[0],
+                                                            u[1], u[2], G, lambda, w, w + nv);
+                    for (int i = 0; i < nv && !e; i++) {
+                        rd[np + 3 * i + j] = w[i];
+                        rn[np + 3 * i + j] = w[nv + i];
+                    }
+                }
+            } else {
+                double *r0 = K + (size_t)(2 * np + 3 * (r - np)) * n, *r1 = r0 + n, *r2 = r1 + n;
+                // the moduli in the order stressRows passes them (mu, lambda)
+                // into the (lambda, G) of computeStressKernelsPlaneStrain
+                e = orc_stress_kernels_planestrain(X, S, D, N, np, x, W, dip, lambda, G, r0, r1, r2);
+                for (int j = 0; j < 3 && !e; j++) {
+                    const double u[3] = {j == 0 ? 1e-3 : 0.0, j == 1 ? 1e-3 : 0.0,
+                                         j == 2 ? 1e-3 : 0.0};
+                    e = orc_stress_kernels_sv_planestrain(X, S, D, N, nv, vx, vT, vW, vdip, u[0],
+                                                          u[1], u[2], G, lambda, w, w + nv,
+                                                          w + 2 * (size_t)nv);
+                    for (int i = 0; i < nv && !e; i++) {
+                        r0[np + 3 * i + j] = w[i];
+                        r1[np + 3 * i + j] = w[nv + i];
+                        r2[np + 3 * i + j] = w[2 * nv + i];
+                    }
+                }
+            }
+            if (e < rc) rc = e;
+        }
+        std::free(w);
+    }
+    std::free(f);
+    return rc;
+}
+
 /* ratestate.f90 (2-D plane strain):585-627 */
 extern "C" void orc_ps_init_state(const orc_ps_system *s, double *y)
 {
@@ -228,6 +367,36 @@ extern "C" void orc_ps_init_state(const orc_ps_system *s, double *y)
         p[3] = std::log(s->L[i] / s->Vpl[i]) / lg10;
         p[4] = std::log(s->Vpl[i] * 0.98) / lg10;
     }
+    /* viscouscycles.f90 (2-D plane strain):777-821 */
+    for (int i = 0; i < s->nv; i++) {
+        double *q = y + 5 * (size_t)s->n + 6 * (size_t)i;
+        const double *e = s->ebg + 3 * i;
+        double s22, s23, s33;
+        const double *z = s->s0 ? s->s0 + 3 * i : nullptr;
+        if (!z || (0.0 == z[0] && 0.0 == z[1] && 0.0 == z[2])) {
+            const double ekk = e[0] + e[2];
+            const double e22p = e[0] - ekk / 2, e33p = e[2] - ekk / 2;
+            const double eps = std::sqrt((e22p * e22p + 2.0 * (e[1] * e[1]) + e33p * e33p) / 2.0);
+            if (0.0 < eps && 0.0 < s->gammadot0[i]) {
+                const double tau = std::pow(eps / s->gammadot0[i] *
+                                                std::exp(s->Q[i] / (s->R[i] * s->To[i])),
+                                            1.0 / s->npower[i]);
+                s22 = tau * e22p / eps;
+                s23 = tau * e[1] / eps;
+                s33 = tau * e33p / eps;
+            } else {
+                s22 = s23 = s33 = 0.0;
+            }
+        } else {
+            s22 = z[0];
+            s23 = z[1];
+            s33 = z[2];
+        }
+        q[0] = q[1] = q[2] = 0.0;
+        q[3] = s22;
+        q[4] = s23;
+        q[5] = s33;
+    }
 }
 
 /* odefun (ratestate.f90:637-753): returns vMax */
@@ -235,16 +404,41 @@ extern "C" double orc_ps_odefun(const orc_ps_system *s, const double *y, double 
                                 double *vwork, double *twork)
 {
     const double lg10 = 2.302585092994045684;  /* LOG(1.d1) */
-    const int n = s->n;
+    const int n = s->n, nv = s->nv;
     double vmax = 0.0;
-    std::memset(dydt, 0, sizeof(double) * 5 * (size_t)n);
+    std::memset(dydt, 0, sizeof(double) * (5 * (size_t)n + 6 * (size_t)nv));
     for (int i = 0; i < n; i++) {
         const double velocity = std::exp(y[5 * i + 4] * lg10);
         vmax = velocity > vmax ? velocity : vmax;
         dydt[5 * i] = velocity;
         vwork[i] = velocity - s->Vpl[i];
     }
-    orc_matvec(2 * n, n, s->K, vwork, twork, 1);
+    /* viscouscycles.f90 (2-D plane strain):930-967 */
+    for (int i = 0; i < nv; i++) {
+        const double *q = y + 5 * (size_t)n + 6 * (size_t)i;
+        double *dq = dydt + 5 * (size_t)n + 6 * (size_t)i;
+        const double p = (q[3] + q[5]) / 2.0;
+        const double s22p = q[3] - p, s33p = q[5] - p, s23 = q[4];
+        const double sII = std::sqrt((s22p * s22p + 2.0 * (s23 * s23) + s33p * s33p) / 2.0);
+        const double eII = s->gammadot0[i] * std::pow(sII, s->npower[i] - 1.0)
+                           * std::exp(-s->Q[i] / (s->R[i] * s->To[i]));
+        const double e22 = eII * s22p, e23 = eII * s23, e33 = eII * s33p;
+        dq[0] = e22;
+        dq[1] = e23;
+        dq[2] = e33;
+        double *v = vwork + n + 3 * (size_t)i;
+        v[0] = (e22 - s->ebg[3 * i]) * 1e3;
+        v[1] = (e23 - s->ebg[3 * i + 1]) * 1e3;
+        v[2] = (e33 - s->ebg[3 * i + 2]) * 1e3;
+    }
+    orc_matvec(2 * n + 3 * nv, n + 3 * nv, s->K, vwork, twork, 1);
+    for (int i = 0; i < nv; i++) {  /* :1048-1053 */
+        double *dq = dydt + 5 * (size_t)n + 6 * (size_t)i;
+        const double *t = twork + 2 * (size_t)n + 3 * (size_t)i;
+        dq[3] = t[0];
+        dq[4] = t[1];
+        dq[5] = t[2];
+    }
     for (int i = 0; i < n; i++) {
         const double *p = y + 5 * i;
         double *dp = dydt + 5 * i;
@@ -280,10 +474,10 @@ extern "C" long orc_ps_integrate(const orc_ps_system *s, double *y, double inter
                                  double *dt_log, double *vmax_log, long log_cap,
                                  orc_run_stats *st)
 {
-    const int dim = 5 * s->n;
-    double *work = (double *)std::malloc(sizeof(double) * (size_t)(11 * dim + 3 * s->n));
+    const int dim = 5 * s->n + 6 * s->nv, nvv = s->n + 3 * s->nv;
+    double *work = (double *)std::malloc(sizeof(double) * (size_t)(11 * dim + 3 * nvv));
     double *dydt = work, *yscal = work + dim, *rk = work + 2 * dim;
-    ps_ctx c = {s, work + 11 * dim, work + 11 * dim + s->n, 0.0, 0};
+    ps_ctx c = {s, work + 11 * dim, work + 11 * dim + nvv, 0.0, 0};
     double time = 0.0, dt_next = dt0;
     long steps = 0, attempts = 0;
     while (steps < max_steps) {

@@ -304,6 +304,54 @@ CONTAINS
     CALL computeTractionKernelsPlaneStrain(x,sv,dv,nv,ns,y,W,dip,G,lam,td,tn)
   END SUBROUTINE
 
+  ! planestrain.f90:314 computeStressKernelsPlaneStrain (its own argument
+  ! order: lambda before G)
+  SUBROUTINE ref_stress_kernels_planestrain(x,sv,dv,nv,ns,y,W,dip,lam,G,s22,s23,s33) &
+       BIND(C,NAME="ref_stress_kernels_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: lam,G
+    REAL(C_DOUBLE), INTENT(OUT) :: s22(ns),s23(ns),s33(ns)
+    CALL computeStressKernelsPlaneStrain(x,sv,dv,nv,ns,y,W,dip,lam,G,s22,s23,s33)
+  END SUBROUTINE
+
+  ! planestrain.f90:375 computeTractionKernelsStrainVolumePlaneStrain
+  SUBROUTINE ref_traction_sv_planestrain(x,sv,dv,nv,ns,y,T,W,dip,e22,e23,e33,G,lam,td,tn) &
+       BIND(C,NAME="ref_traction_sv_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),T(ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: e22,e23,e33,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: td(ns),tn(ns)
+    CALL computeTractionKernelsStrainVolumePlaneStrain(x,sv,dv,nv,ns,y,T,W,dip,e22,e23,e33, &
+         G,lam,td,tn)
+  END SUBROUTINE
+
+  ! planestrain.f90:435 computeStressKernelsStrainVolumePlaneStrain
+  SUBROUTINE ref_stress_kernels_sv_planestrain(x,sv,dv,nv,ns,y,T,W,dip,e22,e23,e33,G,lam, &
+       s22,s23,s33) BIND(C,NAME="ref_stress_kernels_sv_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),sv(3),dv(3),nv(3),y(3,ns),T(ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: e22,e23,e33,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: s22(ns),s23(ns),s33(ns)
+    CALL computeStressKernelsStrainVolumePlaneStrain(x,sv,dv,nv,ns,y,T,W,dip,e22,e23,e33, &
+         G,lam,s22,s23,s33)
+  END SUBROUTINE
+
+  ! planestrain.f90:221 computeDisplacementKernelsStrainVolumePlaneStrain
+  SUBROUTINE ref_displacement_kernels_sv_planestrain(x,ns,y,T,W,dip,e22,e23,e33,G,lam,u2,u3) &
+       BIND(C,NAME="ref_displacement_kernels_sv_planestrain")
+    USE planestrain
+    INTEGER(C_INT), VALUE :: ns
+    REAL(C_DOUBLE), INTENT(IN) :: x(3),y(3,ns),T(ns),W(ns),dip(ns)
+    REAL(C_DOUBLE), VALUE :: e22,e23,e33,G,lam
+    REAL(C_DOUBLE), INTENT(OUT) :: u2(ns),u3(ns)
+    CALL computeDisplacementKernelsStrainVolumePlaneStrain(x,ns,y,T,W,dip,e22,e23,e33, &
+         G,lam,u2,u3)
+  END SUBROUTINE
+
   ! planestrain.f90:55 computeReferenceSystemPlaneStrain
   SUBROUTINE ref_refsys_planestrain(ns,x,W,dip,sv,dv,nv,xc) BIND(C,NAME="ref_refsys_planestrain")
     USE planestrain

@@ -222,11 +222,46 @@ int orc_traction_kernels_planestrain(const double x[3], const double sv[3], cons
  * receiver r, column s = dip slip of source s */
 int orc_buildG_planestrain(int ns, const double *x, const double *W, const double *dip,
                            double G, double lambda, double *K, int nthreads);
+/* the volume-frame stress kernels (planestrain.f90:314-348, 435-475;
+ * s22' = n.s.n, s23' = -n.s.d, s33' = d.s.d) and the traction kernels of
+ * strain volumes (:375-410), vectorised over ns sources as the reference's */
+int orc_stress_kernels_planestrain(const double x[3], const double sv[3], const double dv[3],
+                                   const double nv[3], int ns, const double *y,
+                                   const double *W, const double *dip, double G,
+                                   double lambda, double *s22, double *s23, double *s33);
+int orc_traction_kernels_sv_planestrain(const double x[3], const double sv[3],
+                                        const double dv[3], const double nv[3], int ns,
+                                        const double *y, const double *T, const double *W,
+                                        const double *dip, double e22p, double e23p,
+                                        double e33p, double G, double lambda, double *td,
+                                        double *tn);
+int orc_stress_kernels_sv_planestrain(const double x[3], const double sv[3], const double dv[3],
+                                      const double nv[3], int ns, const double *y,
+                                      const double *T, const double *W, const double *dip,
+                                      double e22p, double e23p, double e33p, double G,
+                                      double lambda, double *s22, double *s23, double *s33);
+/* buildG of the plane-strain viscouscycles program (greens.f90:66-226):
+ * receivers np patches (rows td, tn) then nv volumes (rows s22', s23',
+ * s33'); sources np dip slips then nv x 3 strains of 1e-3 (e22', e23',
+ * e33'); K (2 np + 3 nv) x (np + 3 nv) row-major.  The volume rows of patch
+ * sources take G and lambda swapped, as stressRows passes them (greens.f90:
+ * 212-216 against the argument order of planestrain.f90:314). */
+int orc_buildG_planestrain_volumes(int np, const double *x, const double *W, const double *dip,
+                                   int nv, const double *vx, const double *vT, const double *vW,
+                                   const double *vdip, double G, double lambda, double *K,
+                                   int nthreads);
 typedef struct {
     int n;   /* patches; state 5 per patch: dip slip, dip traction, normal
               * traction, log10 theta, log10 V */
     const double *Vpl, *tau0, *mu0, *sig, *a, *b, *L, *Vo, *damping;  /* tau0 may be NULL */
-    const double *K;       /* 2n x n row-major */
+    const double *K;       /* (2n + 3nv) x (n + 3nv) row-major */
+    /* strain volumes (viscouscycles.f90 of fortran/2d/planestrain): state 6
+     * per volume after the patches: e22, e23, e33, s22, s23, s33; the
+     * nonlinear Maxwell rheology gammadot0, n, Q, R, To; the background
+     * strain rate e_bg (3 per volume: e22, e23, e33); s0 (3 per volume,
+     * may be NULL) the initial stress, used where not all zero */
+    int nv;
+    const double *gammadot0, *npower, *Q, *R, *To, *ebg, *s0;
 } orc_ps_system;
 void orc_ps_init_state(const orc_ps_system *s, double *y);
 double orc_ps_odefun(const orc_ps_system *s, const double *y, double *dydt, double *vwork,

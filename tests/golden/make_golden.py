@@ -28,6 +28,13 @@ The fixtures are inputs and the reference's outputs only:
                         StrainVolumePlaneStrain, computeStressPlaneStrain,
                         computeDisplacementPlaneStrain and computeTraction-
                         KernelsPlaneStrain (fortran/2d/planestrain/src)
+  planestrain_volumes.npz  the traction and stress kernels of plane-strain
+                        strain volumes and the volume-frame stress kernels of
+                        segments (planestrain.f90:314-475), the volumes'
+                        displacement kernels with buildO's unit strains
+                        (:221-264), and buildG of a small mesh of segments and
+                        volumes assembled from those kernels in the layout of
+                        greens.f90:66-226 (fortran/2d/planestrain)
   displacement.npz      computeDisplacementOkada92 (okada92.f90:230-298) and
                         computeDisplacementNikkhoo15 (nikkhoo15.f90:324-367,
                         DispInHalfSpace.c:42-152) at random points, at the
@@ -452,6 +459,111 @@ def planestrain(rng, n=400, nf=300, ns=60):
                         td=np.array(td), tn=np.array(tn))
 
 
+def _ps_frames(x, W, dip):
+    n = W.size
+    o = [np.zeros((n, 3)) for _ in range(4)]
+    REF.ref_refsys_planestrain(n, P(np.ascontiguousarray(x)), P(W), P(dip), *[P(a) for a in o])
+    return o
+
+
+def planestrain_volumes(rng, nrcv=4, ns=40, npts=12):
+    """Strain volumes of the plane-strain viscouscycles program: the volume
+    kernel routines of planestrain.f90:221-475 (traction and volume-frame
+    stress kernels of volumes, volume-frame stress kernels of segments,
+    displacement kernels of volumes with the unit strains of buildO) on
+    random receivers and sources, and buildG of 6 segments and 5 volumes
+    assembled from those routines in the layout of greens.f90:66-226, each
+    called with the arguments greens.f90 passes (G = 30e3, lambda = 20e3)."""
+    G, lam = 30e3, 20e3
+    y = np.stack([np.zeros(ns), rng.uniform(-2e4, 2e4, ns), rng.uniform(0, 1e4, ns)], 1)
+    T, W = rng.uniform(200, 5e3, ns), rng.uniform(200, 5e3, ns)
+    dip = rng.uniform(0.1, np.pi - 0.1, ns)
+    yc = np.ascontiguousarray(y)
+    rcv, out = [], {k: [] for k in ("td", "tn", "vs", "fs")}
+    for i in range(nrcv):
+        d = rng.uniform(0.1, np.pi - 0.1)
+        x = np.array([0.0, rng.uniform(-1e4, 1e4), rng.uniform(100, 1e4)])
+        if i == 1:      # inside source 0's zone
+            a, b = 0.2 * T[0], 0.4 * W[0]
+            x[1] = y[0, 1] + a * np.sin(dip[0]) + b * np.cos(dip[0])
+            x[2] = y[0, 2] - a * np.cos(dip[0]) + b * np.sin(dip[0])
+        if i == 2:
+            x[2] = 0.0
+        fr = [x, np.array([1.0, 0.0, 0.0]), np.array([0.0, -np.cos(d), -np.sin(d)]),
+              np.array([-0.0, np.sin(d), -np.cos(d)])]
+        rcv.append(np.concatenate(fr))
+        fp = [P(np.ascontiguousarray(a)) for a in fr]
+        e = rng.standard_normal(3)
+        a, b = np.zeros(ns), np.zeros(ns)
+        REF.ref_traction_sv_planestrain(*fp, ns, P(yc), P(T), P(W), P(dip), *[D(v) for v in e],
+                                        D(G), D(lam), P(a), P(b))
+        out["td"].append(a)
+        out["tn"].append(b)
+        o = [np.zeros(ns) for _ in range(3)]
+        REF.ref_stress_kernels_sv_planestrain(*fp, ns, P(yc), P(T), P(W), P(dip),
+                                              *[D(v) for v in e], D(G), D(lam), *[P(a) for a in o])
+        out["vs"].append(np.concatenate([[*e], *o]))
+        o = [np.zeros(ns) for _ in range(3)]
+        REF.ref_stress_kernels_planestrain(*fp, ns, P(yc), P(W), P(dip), D(lam), D(G),
+                                           *[P(a) for a in o])
+        out["fs"].append(np.concatenate(o))
+    pts = np.stack([np.zeros(npts), rng.uniform(-2e4, 2e4, npts),
+                    np.where(np.arange(npts) % 2 == 0, 0.0, rng.uniform(0, 1e4, npts))], 1)
+    disp = []
+    for q in pts:
+        for eye in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 0.0)):
+            u2, u3 = np.zeros(ns), np.zeros(ns)
+            REF.ref_displacement_kernels_sv_planestrain(P(np.ascontiguousarray(q)), ns, P(yc), P(T),
+                                                        P(W), P(dip), *[D(v) for v in eye], D(G),
+                                                        D(lam), P(u2), P(u3))
+            disp.append(np.concatenate([u2, u3]))
+    # the mesh: 6 dipping segments and 5 volumes; K rows td, tn per segment,
+    # then s22', s23', s33' per volume; columns the dip slips, then e22',
+    # e23', e33' (strains of 1e-3) per volume
+    nps, nv = 6, 5
+    px = np.stack([np.zeros(nps), rng.uniform(-1e4, 1e4, nps), rng.uniform(0, 5e3, nps)], 1)
+    pW, pdip = rng.uniform(500, 3e3, nps), rng.uniform(0.3, 1.5, nps)
+    vx = np.stack([np.zeros(nv), rng.uniform(-1e4, 1e4, nv), rng.uniform(8e3, 2e4, nv)], 1)
+    vT, vW = rng.uniform(1e3, 4e3, nv), rng.uniform(1e3, 4e3, nv)
+    vdip = rng.uniform(0.3, np.pi - 0.3, nv)
+    pf, vf = _ps_frames(px, pW, pdip), _ps_frames(vx, vW, vdip)
+    K = np.zeros((2 * nps + 3 * nv, nps + 3 * nv))
+    pxc, vxc = np.ascontiguousarray(px), np.ascontiguousarray(vx)
+    for r in range(nps + nv):
+        sv, dv, nvv, xc = [f[r] for f in pf] if r < nps else [f[r - nps] for f in vf]
+        fr = [P(np.ascontiguousarray(a)) for a in (xc, sv, dv, nvv)]
+        if r < nps:      # tractionRows
+            a, b = np.zeros(nps), np.zeros(nps)
+            REF.ref_traction_planestrain(*fr, nps, P(pxc), P(pW), P(pdip), D(G), D(lam), P(a), P(b))
+            K[2 * r, :nps], K[2 * r + 1, :nps] = a, b
+            for j in range(3):
+                a, b = np.zeros(nv), np.zeros(nv)
+                u = [1e-3 * float(j == k) for k in range(3)]
+                REF.ref_traction_sv_planestrain(*fr, nv, P(vxc), P(vT), P(vW), P(vdip),
+                                                *[D(v) for v in u], D(G), D(lam), P(a), P(b))
+                K[2 * r, nps + j::3], K[2 * r + 1, nps + j::3] = a, b
+        else:            # stressRows: (mu, lambda) into (lambda, G), as greens.f90 passes them
+            row = 2 * nps + 3 * (r - nps)
+            o = [np.zeros(nps) for _ in range(3)]
+            REF.ref_stress_kernels_planestrain(*fr, nps, P(pxc), P(pW), P(pdip), D(G), D(lam),
+                                               *[P(a) for a in o])
+            for c in range(3):
+                K[row + c, :nps] = o[c]
+            for j in range(3):
+                o = [np.zeros(nv) for _ in range(3)]
+                u = [1e-3 * float(j == k) for k in range(3)]
+                REF.ref_stress_kernels_sv_planestrain(*fr, nv, P(vxc), P(vT), P(vW), P(vdip),
+                                                      *[D(v) for v in u], D(G), D(lam),
+                                                      *[P(a) for a in o])
+                for c in range(3):
+                    K[row + c, nps + j::3] = o[c]
+    np.savez_compressed(os.path.join(HERE, "planestrain_volumes.npz"), G=G, lam=lam, y=y, T=T,
+                        W=W, dip=dip, receivers=np.array(rcv), td=np.array(out["td"]),
+                        tn=np.array(out["tn"]), vstress=np.array(out["vs"]),
+                        fstress=np.array(out["fs"]), points=pts, disp=np.array(disp), px=px,
+                        pW=pW, pdip=pdip, vx=vx, vT=vT, vW=vW, vdip=vdip, K=K)
+
+
 FIXTURES = {
     "okada_dc3d": (20260101, [dc3d_cases, okada_kernels, nikkhoo_strain, nikkhoo_kernels,
                               strainvolume_kernels]),
@@ -459,6 +571,7 @@ FIXTURES = {
     "antiplane": (20261203, [antiplane]),
     "antiplane_volumes": (20261204, [antiplane_volumes]),
     "planestrain": (20261205, [planestrain]),
+    "planestrain_volumes": (20261206, [planestrain_volumes]),
 }
 
 if __name__ == "__main__":

@@ -198,7 +198,44 @@ def buildO_antiplane(x, W, dip, points):
 
 class OrcPSSystem(C.Structure):
     _fields_ = [("n", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in (
-        "Vpl", "tau0", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")]
+        "Vpl", "tau0", "mu0", "sig", "a", "b", "L", "Vo", "damping", "K")] + \
+        [("nv", C.c_int)] + [(k, C.POINTER(C.c_double)) for k in (
+            "gammadot0", "npower", "Q", "R", "To", "ebg", "s0")]
+
+
+# the plane-strain volumes' rheology arrays, one entry per volume (ebg: 3
+# per volume, the background e22, e23, e33)
+PS_RHEO = ("gammadot0", "npower", "Q", "R", "To", "ebg")
+
+
+def buildG_planestrain_volumes(x, W, dip, vx, vT, vW, vdip, G, lam, nthreads=1):
+    """buildG of the plane-strain viscouscycles program: (2 np + 3 nv) x
+    (np + 3 nv); rows td, tn per patch then s22', s23', s33' per volume;
+    columns the dip slips then 3 unit strains (1e-3) per volume."""
+    x, W, dip, vx, vT, vW, vdip = (f64(a) for a in (x, W, dip, vx, vT, vW, vdip))
+    np_, nv = W.size, vW.size
+    K = np.empty((2 * np_ + 3 * nv, np_ + 3 * nv))
+    rc = oracle().orc_buildG_planestrain_volumes(np_, p(x), p(W), p(dip), nv, p(vx), p(vT), p(vW),
+                                                 p(vdip), C.c_double(G), C.c_double(lam), p(K),
+                                                 nthreads)
+    if rc:
+        raise RuntimeError(f"orc_buildG_planestrain_volumes failed {rc}")
+    return K
+
+
+def ps_kernel_call(fn, x, sv, dv, nv, y, arrays, scalars, nout, lib=None):
+    """A vectorised plane-strain kernel routine (the oracle's orc_* or the
+    reference build's ref_*): receiver frame, sources y (3, ns) and per-source
+    arrays, then scalars; returns nout arrays of ns values."""
+    y = f64(y)
+    ns = y.size // 3
+    outs = [np.empty(ns) for _ in range(nout)]
+    args = [p(f64(a)) for a in (x, sv, dv, nv)] + [ns, p(y)] + [p(f64(a)) for a in arrays] + \
+        [C.c_double(v) for v in scalars] + [p(o) for o in outs]
+    rc = getattr(lib or oracle(), fn)(*args)
+    if lib is None and rc:          # the reference's routines are subroutines
+        raise RuntimeError(f"{fn} failed {rc}")
+    return outs
 
 
 def buildG_planestrain(x, W, dip, G, lam, nthreads=1):
@@ -240,9 +277,12 @@ def displacement_planestrain(x2, x3, q2, q3, W, dip, s, G, lam):
 
 class PSSystem:
     """2-D plane-strain rate-and-state (fortran/2d/planestrain/src/ratestate.f90);
-    tau0 (may be None) is the deck's initial-traction option."""
+    tau0 (may be None) is the deck's initial-traction option.  With rheo (the
+    PS_RHEO arrays) the strain volumes of that directory's viscouscycles
+    program follow the patches; s0 (3 per volume, may be None) is their
+    initial-stress option."""
 
-    def __init__(self, fric, K, tau0=None):
+    def __init__(self, fric, K, tau0=None, rheo=None, s0=None):
         arrs = [f64(a) for a in fric]
         n = arrs[0].size
         t0 = f64(tau0) if tau0 is not None else None
@@ -254,10 +294,25 @@ class PSSystem:
         s.K = p(self.keep[8])
         if t0 is not None:
             s.tau0 = p(t0)
-        assert self.keep[8].shape == (2 * n, n)
+        nv = 0
+        if rheo is not None:
+            ra = [f64(a) for a in rheo]
+            nv = ra[0].size
+            assert ra[5].size == 3 * nv
+            self.keep += ra
+            for k, a in zip(PS_RHEO, ra):
+                setattr(s, k, p(a))
+            if s0 is not None:
+                z = f64(s0)
+                assert z.size == 3 * nv
+                self.keep.append(z)
+                s.s0 = p(z)
+        s.nv = nv
+        assert self.keep[8].shape == (2 * n + 3 * nv, n + 3 * nv)
         self.c = s
         self.n = n
-        self.dim = 5 * n
+        self.nv = nv
+        self.dim = 5 * n + 6 * nv
 
     def init_state(self):
         y = np.empty(self.dim)
@@ -266,7 +321,8 @@ class PSSystem:
 
     def odefun(self, y):
         y = f64(y)
-        dydt, v, t = np.empty(self.dim), np.empty(self.n), np.empty(2 * self.n)
+        dydt, v = np.empty(self.dim), np.empty(self.n + 3 * self.nv)
+        t = np.empty(2 * self.n + 3 * self.nv)
         lib = oracle()
         lib.orc_ps_odefun.restype = C.c_double
         vmax = lib.orc_ps_odefun(C.byref(self.c), p(y), p(dydt), p(v), p(t))

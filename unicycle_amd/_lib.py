@@ -96,6 +96,8 @@ _SIGS = {
     "ucy_set_antiplane_volumes": (_i, [_vp, _i, _pd, _pd, _pd, _pd]),
     "ucy_set_planestrain": (_i, [_vp, _d, _d, _i, _pd, _pd, _pd]),
     "ucy_set_initial_traction": (_i, [_vp, _i, _pd]),
+    "ucy_set_planestrain_volumes": (_i, [_vp, _i, _pd, _pd, _pd, _pd]),
+    "ucy_set_initial_volume_stress": (_i, [_vp, _i, _pd]),
     "ucy_planestrain_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _d, _d,
                                               _pd, _pd]),
     "ucy_antiplane_traction_kernels": (_i, [_vp, _pd, _pd, _pd, _pd, _i, _pd, _pd, _pd, _d, _pd]),

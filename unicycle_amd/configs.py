@@ -389,6 +389,75 @@ def planestrain_fault(n: int = 200, dip_deg: float = 30.0, length: float = 60e3,
 
 
 @dataclass
+class PlaneStrainViscousModel:
+    """The plane-strain viscouscycles model (fortran/2d/planestrain/src/
+    viscouscycles.f90): a dipping fault of segments (PlaneStrainModel) and a
+    grid of strain volumes with the nonlinear Maxwell rheology
+    eII = gammadot0 sII^(n-1) exp(-Q/(R To))."""
+    fault: PlaneStrainModel
+    vx: np.ndarray          # (nv, 3) tops; x2, x3 used
+    thickness: np.ndarray
+    vwidth: np.ndarray      # extent down dip
+    vdip: np.ndarray        # radians
+    e22: np.ndarray         # background strain rate (1/s)
+    e23: np.ndarray
+    e33: np.ndarray
+    gammadot0: np.ndarray
+    npower: np.ndarray
+    Q: np.ndarray
+    R: np.ndarray
+    To: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return self.fault.n
+
+    @property
+    def nvol(self) -> int:
+        return self.thickness.size
+
+    def edot_bg(self):
+        """Engine.set_rheology's background strain rate (6, nv): e22, e23,
+        e33 in entries 3, 4, 5 of each volume."""
+        e = np.zeros((self.nvol, 6))
+        e[:, 3], e[:, 4], e[:, 5] = self.e22, self.e23, self.e33
+        return e
+
+    def rheology(self):
+        """Engine.set_rheology order (Gm unused by this model)."""
+        return (self.gammadot0, self.npower, self.edot_bg(), np.ones(self.nvol), self.Q, self.R,
+                self.To)
+
+    def oracle_rheology(self):
+        """orc.PSSystem's rheo order."""
+        return (self.gammadot0, self.npower, self.Q, self.R, self.To,
+                np.stack([self.e22, self.e23, self.e33], 1).ravel())
+
+
+def planestrain_viscous(n: int = 60, n2: int = 8, n3: int = 4, vsize: float = 4e3,
+                        depth: float = 20e3, npower: float = 1.0, gammadot0: float = 1e-12,
+                        Q: float = 0.0, To: float = 1200.0, e22: float = -1e-15,
+                        e23: float = 0.0, e33: float = 1e-15, **fault) -> PlaneStrainViscousModel:
+    """A thrust of n segments (planestrain_fault, 30 deg over 60 km down dip
+    by default) above an n2 x n3 grid of vsize x vsize volumes (dip 90 deg:
+    width down, thickness across) from depth, centred on x2 = 10 km (clear
+    of the fault, which reaches that depth at x2 = 35 km); pure
+    shear e22 = -e33 of 1e-15/s in the background, n = 1, gammadot0 = 1e-12,
+    Q = 0 and R = 8.314 by default."""
+    f = planestrain_fault(n=n, **fault)
+    j2, j3 = np.meshgrid(np.arange(n2), np.arange(n3))
+    nv = n2 * n3
+    vx = np.zeros((nv, 3))
+    vx[:, 1] = (10e3 + (j2 - (n2 - 1) / 2) * vsize).ravel()
+    vx[:, 2] = (depth + j3 * vsize).ravel()
+    one = np.ones(nv)
+    return PlaneStrainViscousModel(fault=f, vx=vx, thickness=vsize * one, vwidth=vsize * one,
+                                   vdip=np.pi / 2 * one, e22=e22 * one, e23=e23 * one,
+                                   e33=e33 * one, gammadot0=gammadot0 * one,
+                                   npower=npower * one, Q=Q * one, R=8.314 * one, To=To * one)
+
+
+@dataclass
 class AntiplaneViscousModel:
     """The 2-D antiplane viscouscycles model (fortran/2d/antiplane/src/
     viscouscycles.f90): a fault of segments (AntiplaneModel) above a grid of

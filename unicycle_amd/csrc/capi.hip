@@ -10,6 +10,7 @@
 // compiler does not fuse sin/cos pairs into sincos, whose results differ in
 // the last bit from separate sin and cos calls.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <chrono>
@@ -488,8 +489,46 @@ extern "C" int ucy_set_planestrain(ucy_ctx *c, double lambda, double mu, int ns,
     c->rdip.assign(dip, dip + ns);
     c->ntri = c->nvol = c->nve = 0;
     c->tau0.clear();
+    c->s0.clear();
     c->params_dirty = true;
     return upload_if_complete(c);
+}
+
+// strain volumes of the 2-D plane-strain model (fortran/2d/planestrain/src/
+// viscouscycles.f90:1535-1578): top x(3,nv) (x(2), x(3) used), thickness,
+// width and dip (radians); after ucy_set_planestrain, which clears them
+extern "C" int ucy_set_planestrain_volumes(ucy_ctx *c, int nv, const double *x,
+                                           const double *thickness, const double *width,
+                                           const double *dip)
+{
+    if (!c || nv < 0 || (nv && (!x || !thickness || !width || !dip))) return UCY_ERR_ARG;
+    if (c->model != 2)
+        return fail(c, UCY_ERR_STATE, "no plane-strain model: call ucy_set_planestrain first");
+    for (int i = 0; i < nv; i++) {
+        if (0 > x[3 * i + 2])  // :1559-1564
+            return fail(c, UCY_ERR_GEOMETRY, "volume %d: depth must be positive", i + 1);
+        if (!(std::min(width[i], thickness[i]) > 0.0))  // :1572-1576
+            return fail(c, UCY_ERR_ARG, "volume %d: strain volume dimension must be positive",
+                        i + 1);
+    }
+    c->nvol = nv;
+    c->vx.assign(x, x + 3 * (size_t)nv);
+    c->vT.assign(thickness, thickness + nv);
+    c->vW.assign(width, width + nv);
+    c->vdip.assign(dip, dip + nv);
+    c->vL.assign(nv, 0.0);
+    c->vstrike.assign(nv, 0.0);
+    c->s0.clear();
+    c->params_dirty = true;
+    return upload_if_complete(c);
+}
+
+extern "C" int ucy_set_initial_volume_stress(ucy_ctx *c, int nv, const double *s0)
+{
+    if (!c || nv < 0 || (nv && !s0)) return UCY_ERR_ARG;
+    if (c->model != 2) return fail(c, UCY_ERR_STATE, "initial volume stress: plane-strain model only");
+    c->s0.assign(s0, s0 + 3 * (size_t)nv);
+    return UCY_OK;
 }
 
 extern "C" int ucy_set_initial_traction(ucy_ctx *c, int np, const double *tau0)
@@ -609,15 +648,26 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
     const bool ap = c->model == 1, ps = c->model == 2;
     const int64_t nap = np + 2 * (int64_t)c->nvol;  // antiplane rows and columns
-    const int64_t m = ap ? nap : ps ? 2 * (int64_t)np : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
+    const int64_t nps = np + 3 * (int64_t)c->nvol;  // plane-strain columns
+    const int64_t m = ap ? nap
+                    : ps ? 2 * (int64_t)np + 3 * (int64_t)c->nvol
+                         : 3 * (int64_t)np + 6 * (int64_t)c->nvol;
     const int64_t n = ap ? nap + (nap & 1)
-                    : ps ? np + (np & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
+                    : ps ? nps + (nps & 1) : 2 * (int64_t)np + 6 * (int64_t)c->nvol;
     if (ps) {
-        std::vector<Receiver> R0;
+        // receivers (patches, then volumes) and sources as the kernels see
+        // them; the volume rows of patch sources take G and lambda exchanged
+        std::vector<Receiver> R0, RV0;
         refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R0);
+        refsys_antiplane(c->nvol, c->vx.data(), c->vW.data(), c->vdip.data(), RV0);
+        R0.insert(R0.end(), RV0.begin(), RV0.end());
+        const int stride = (int)(sizeof(Receiver) / sizeof(double));
         const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
-        const int rc = ps_check(c, nu, np, R0.empty() ? nullptr : &R0[0].xc[2],
-                                (int)(sizeof(Receiver) / sizeof(double)), np, c->rx.data() + 2, 3);
+        int rc = ps_check(c, nu, np + c->nvol, R0.empty() ? nullptr : &R0[0].xc[2], stride, np,
+                          c->rx.data() + 2, 3);
+        if (!rc) rc = ps_check(c, nu, 0, nullptr, 1, c->nvol, c->vx.data() + 2, 3);
+        if (!rc && c->nvol && np)
+            rc = ps_check(c, c->mu / 2.0 / (c->mu + c->lambda), 0, nullptr, 1, 0, nullptr, 1);
         if (rc) return rc;
     }
     free_work(c);
@@ -632,24 +682,41 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
     HIPCHK(hipMalloc((void **)&c->K, (size_t)(m * n) * sizeof(double)));
     c->m = m;
     c->n = n;
-    c->nl = ap ? nap : ps ? np : n;
+    c->nl = ap ? nap : ps ? nps : n;
     std::vector<Receiver> R;
     if (ps) {  // buildG of fortran/2d/planestrain (greens.f90:66-226)
+        // receivers: the patches, then the volumes (the same frames,
+        // planestrain.f90:55-83 for both, viscouscycles.f90:1187-1213);
+        // sources: the patches, then the volumes with their thicknesses
+        const int nv = c->nvol, ne = np + nv;
+        std::vector<Receiver> RV;
         refsys_antiplane(np, c->rx.data(), c->rW.data(), c->rdip.data(), R);
-        const std::vector<PsSource> S = ps_sources(np, c->rx.data(), c->rW.data(), c->rdip.data());
+        refsys_antiplane(nv, c->vx.data(), c->vW.data(), c->vdip.data(), RV);
+        R.insert(R.end(), RV.begin(), RV.end());
+        std::vector<PsSource> S = ps_sources(np, c->rx.data(), c->rW.data(), c->rdip.data());
+        const std::vector<PsSource> SV = ps_sources(nv, c->vx.data(), c->vW.data(), c->vdip.data());
+        S.insert(S.end(), SV.begin(), SV.end());
         Receiver *d_R = nullptr;
         PsSource *d_S = nullptr;
-        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * np));
-        HIPCHK(hipMalloc((void **)&d_S, sizeof(PsSource) * np));
-        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * np, hipMemcpyHostToDevice,
+        double *d_T = nullptr;
+        HIPCHK(hipMalloc((void **)&d_R, sizeof(Receiver) * ne));
+        HIPCHK(hipMalloc((void **)&d_S, sizeof(PsSource) * ne));
+        HIPCHK(hipMalloc((void **)&d_T, sizeof(double) * (nv + 1)));
+        HIPCHK(hipMemcpyAsync(d_R, R.data(), sizeof(Receiver) * ne, hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * np, hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * ne, hipMemcpyHostToDevice,
                               c->stream));
+        if (nv)
+            HIPCHK(hipMemcpyAsync(d_T, c->vT.data(), sizeof(double) * nv, hipMemcpyHostToDevice,
+                                  c->stream));
         auto t0 = std::chrono::steady_clock::now();
-        if (n != np) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
+        if (n != nps) HIPCHK(hipMemsetAsync(c->K, 0, (size_t)(m * n) * sizeof(double), c->stream));
         const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
-        launch_ps_rows(c->stream, d_S, np, d_R, np, c->mu, c->lambda, nu, c->K, n, nullptr,
-                       nullptr);
+        if (np)
+            launch_ps_rows(c->stream, d_S, np, d_R, np, c->mu, c->lambda, nu, c->K, n, nullptr,
+                           nullptr);
+        if (nv)
+            launch_ps_blocks(c->stream, d_S, np, d_S + np, d_T, nv, d_R, c->mu, c->lambda, c->K, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         c->build_seconds[0] =
@@ -657,6 +724,7 @@ extern "C" int ucy_build_kernel(ucy_ctx *c, int precision)
         c->build_seconds[1] = c->build_seconds[2] = 0.0;
         HIPCHK(hipFree(d_R));
         HIPCHK(hipFree(d_S));
+        HIPCHK(hipFree(d_T));
         if (precision != UCY_PREC_F64) {
             HIPCHK(hipMalloc((void **)&c->Kp, (size_t)(m * n) * sizeof(uint32_t)));
             HIPCHK(hipMalloc((void **)&c->Krs, (size_t)m * sizeof(double)));
@@ -816,7 +884,8 @@ extern "C" int ucy_set_kernel(ucy_ctx *c, const double *K, int64_t m, int64_t n)
 {
     if (!c || !K || m <= 0 || n <= 0) return UCY_ERR_ARG;
     const int np = c->nrect + c->ntri;
-    const bool shape_ok = c->model == 2 ? (m == 2 * (int64_t)np && n == np)
+    const bool shape_ok = c->model == 2
+        ? (m == 2 * (int64_t)np + 3 * (int64_t)c->nvol && n == np + 3 * (int64_t)c->nvol)
         : c->model == 1
         ? (m == np + 2 * (int64_t)c->nvol && n == m)
         : (m == 3 * (int64_t)np + 6 * (int64_t)c->nvol && n == 2 * (int64_t)np + 6 * (int64_t)c->nvol);
@@ -939,29 +1008,39 @@ extern "C" int ucy_build_displacement(ucy_ctx *c, int npts, const double *x)
     if (npts == 0) return UCY_OK;
     if (!c->have_elastic) return fail(c, UCY_ERR_STATE, "elastic moduli not set");
     if (c->model == 2) {  // fortran/2d/planestrain/src/greens.f90:253-367
-        const int ns = c->nrect;
-        if (ns == 0) return fail(c, UCY_ERR_STATE, "no elements");
+        // columns: one dip slip per patch, then e22, e23, e33 per volume
+        const int ns = c->nrect, nv = c->nvol, ne = ns + nv;
+        const int64_t on = ns + 3 * (int64_t)nv;
+        if (ne == 0) return fail(c, UCY_ERR_STATE, "no elements");
         const double nu = c->lambda / 2.0 / (c->lambda + c->mu);
         int rc = ps_check(c, nu, npts, x + 2, 3, ns, c->rx.data() + 2, 3);
+        if (!rc) rc = ps_check(c, nu, 0, nullptr, 1, nv, c->vx.data() + 2, 3);
         if (rc) return rc;
-        const std::vector<PsSource> S = ps_sources(ns, c->rx.data(), c->rW.data(), c->rdip.data());
+        std::vector<PsSource> S = ps_sources(ns, c->rx.data(), c->rW.data(), c->rdip.data());
+        const std::vector<PsSource> SV = ps_sources(nv, c->vx.data(), c->vW.data(), c->vdip.data());
+        S.insert(S.end(), SV.begin(), SV.end());
         char *buf = nullptr;
         double *O = nullptr;
-        HIPCHK(hipMalloc((void **)&buf, sizeof(PsSource) * ns + sizeof(double) * 3 * (size_t)npts));
-        HIPCHK(hipMalloc((void **)&O, sizeof(double) * 2 * (size_t)npts * ns));
+        const size_t sb = sizeof(PsSource) * ne, tb = sizeof(double) * (nv + 1);
+        HIPCHK(hipMalloc((void **)&buf, sb + tb + sizeof(double) * 3 * (size_t)npts));
+        HIPCHK(hipMalloc((void **)&O, sizeof(double) * 2 * (size_t)npts * on));
         PsSource *d_S = (PsSource *)buf;
-        double *pts = (double *)(buf + sizeof(PsSource) * ns);
-        HIPCHK(hipMemcpyAsync(d_S, S.data(), sizeof(PsSource) * ns, hipMemcpyHostToDevice,
-                              c->stream));
+        double *d_T = (double *)(buf + sb);
+        double *pts = (double *)(buf + sb + tb);
+        HIPCHK(hipMemcpyAsync(d_S, S.data(), sb, hipMemcpyHostToDevice, c->stream));
+        if (nv)
+            HIPCHK(hipMemcpyAsync(d_T, c->vT.data(), sizeof(double) * nv, hipMemcpyHostToDevice,
+                                  c->stream));
         HIPCHK(hipMemcpyAsync(pts, x, sizeof(double) * 3 * npts, hipMemcpyHostToDevice,
                               c->stream));
-        launch_ps_disp(c->stream, d_S, ns, pts, npts, c->mu, c->lambda, nu, O, ns);
+        launch_ps_disp(c->stream, d_S, ns, pts, npts, c->mu, c->lambda, nu, O, on);
+        launch_ps_volume_disp(c->stream, d_S + ns, d_T, nv, pts, npts, c->mu, c->lambda, O, on, ns);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(buf));
         c->O = O;
         c->o_rows = 2 * (int64_t)npts;
-        c->o_n = ns;
+        c->o_n = on;
         return UCY_OK;
     }
     if (c->model == 1) {  // fortran/2d/antiplane/src/greens.f90:275-383
@@ -1055,7 +1134,7 @@ extern "C" int ucy_displacement_shape(const ucy_ctx *c, int64_t *rows, int64_t *
 static int check_disp(ucy_ctx *c)
 {
     if (!c->O) return fail(c, UCY_ERR_STATE, "displacement kernels not built");
-    const int64_t want = c->model == 2 ? (int64_t)c->nrect
+    const int64_t want = c->model == 2 ? (int64_t)c->nrect + 3 * (int64_t)c->nvol
                        : c->model == 1 ? (int64_t)c->nrect + 2 * (int64_t)c->nvol
                                        : 2 * (int64_t)(c->nrect + c->ntri) + 6 * (int64_t)c->nvol;
     if (c->o_n != want)
@@ -1112,7 +1191,7 @@ extern "C" int ucy_displacement(ucy_ctx *c, const double *y, double *d, double *
     int rc = check_disp(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->device));
-    const int64_t dim = c->model == 2 ? 5 * (int64_t)c->nrect
+    const int64_t dim = c->model == 2 ? 5 * (int64_t)c->nrect + 6 * (int64_t)c->nvol
                       : c->model == 1 ? 4 * (int64_t)(c->nrect + c->nvol)
                                       : 7 * (int64_t)(c->nrect + c->ntri) + 12 * (int64_t)c->nvol;
     const int64_t R = c->o_rows;
@@ -1198,7 +1277,7 @@ static PlainDims dims(const ucy_ctx *c)
     d.nv = c->nvol;
     d.ap = c->model == 1;
     d.ps = c->model == 2;
-    d.dim = d.ps ? 5 * d.np : d.ap ? 4 * (d.np + d.nv) : 7 * d.np + 12 * d.nv;
+    d.dim = d.ps ? 5 * d.np + 6 * d.nv : d.ap ? 4 * (d.np + d.nv) : 7 * d.np + 12 * d.nv;
     d.m = (int)c->m;
     d.n = (int)c->n;
     return d;
@@ -1326,6 +1405,37 @@ extern "C" int ucy_init_state(ucy_ctx *c, int variant, double *y)
             p[2] = 0.0;
             p[3] = log(c->Lc[i] / c->Vpl[i]) / lg10;
             p[4] = log(c->Vpl[i] * 0.98) / lg10;
+        }
+        // volumes, viscouscycles.f90 (2-D plane strain):777-821: s0 where
+        // not all zero, else the stress that flows at the background rate
+        // (e22, e23, e33 in ebg[3..5]) under the power law with Arrhenius
+        for (int i = 0; i < c->nvol; i++) {
+            double *q = y + 5 * (size_t)np + 6 * (size_t)i;
+            const double e22 = c->ebg[6 * i + 3], e23 = c->ebg[6 * i + 4], e33 = c->ebg[6 * i + 5];
+            const bool given = 3 * (size_t)(i + 1) <= c->s0.size() &&
+                               !(0.0 == c->s0[3 * i] && 0.0 == c->s0[3 * i + 1] &&
+                                 0.0 == c->s0[3 * i + 2]);
+            double s22 = 0.0, s23 = 0.0, s33 = 0.0;
+            if (given) {
+                s22 = c->s0[3 * i];
+                s23 = c->s0[3 * i + 1];
+                s33 = c->s0[3 * i + 2];
+            } else {
+                const double ekk = e22 + e33;
+                const double e22p = e22 - ekk / 2, e33p = e33 - ekk / 2;
+                const double eps = sqrt((e22p * e22p + 2.0 * (e23 * e23) + e33p * e33p) / 2.0);
+                if (0.0 < eps && 0.0 < c->gd0[i]) {
+                    const double tau =
+                        pow(eps / c->gd0[i] * exp(c->Qa[i] / (c->Rg[i] * c->To[i])), 1.0 / c->npow[i]);
+                    s22 = tau * e22p / eps;
+                    s23 = tau * e23 / eps;
+                    s33 = tau * e33p / eps;
+                }
+            }
+            q[0] = q[1] = q[2] = 0.0;
+            q[3] = s22;
+            q[4] = s23;
+            q[5] = s33;
         }
         return UCY_OK;
     }

@@ -132,6 +132,30 @@ __device__ __forceinline__ double ap_volume_prepass(const double *q, const Rheol
     return sqrt(e12 * e12 + e13 * e13);
 }
 
+// 2-D plane-strain volume (fortran/2d/planestrain/src/viscouscycles.f90:
+// 929-967): state [e22, e23, e33, s22, s23, s33]; with the 2-D deviatoric
+// stress s' (p = (s22 + s33) / 2), strain rate eII s' with
+// eII = gammadot0 sII^(n-1) exp(-Q/(R To)); v = (that - the background
+// rate) 1e3, as K's volume columns are strains of 1e-3; returns the
+// strain-rate norm (eMax of :962)
+__device__ __forceinline__ double ps_volume_prepass(const double *q, const Rheology &r, double *dq,
+                                                    double *vq)
+{
+    const double s22 = q[3], s23 = q[4], s33 = q[5];
+    const double p = (s22 + s33) / 2.0;
+    const double s22p = s22 - p, s33p = s33 - p;
+    const double sII = sqrt((s22p * s22p + 2.0 * (s23 * s23) + s33p * s33p) / 2.0);
+    const double eII = r.gammadot0 * pow(sII, r.npower - 1.0) * r.arr;
+    const double e22 = eII * s22p, e23 = eII * s23, e33 = eII * s33p;
+    dq[0] = e22;
+    dq[1] = e23;
+    dq[2] = e33;
+    vq[0] = (e22 - r.ebg[3]) * 1e3;
+    vq[1] = (e23 - r.ebg[4]) * 1e3;
+    vq[2] = (e33 - r.ebg[5]) * 1e3;
+    return sqrt((e22 * e22 + 2.0 * (e23 * e23) + e33 * e33) / 2.0);
+}
+
 __device__ __forceinline__ void volume_prepass(const double *q, const Rheology &r, double *dq,
                                                double *vq)
 {
@@ -170,30 +194,30 @@ __device__ __forceinline__ void patch_postpass(const double *p, const Friction &
 
 __device__ __forceinline__ int elem_state_off(const PlainDims &d, int e)
 {
-    if (d.ps) return 5 * e;
+    if (d.ps) return e < d.np ? 5 * e : 5 * d.np + 6 * (e - d.np);
     if (d.ap) return 4 * e;
     return e < d.np ? 7 * e : 7 * d.np + 12 * (e - d.np);
 }
 __device__ __forceinline__ int elem_v_off(const PlainDims &d, int e)
 {
-    if (d.ps) return e;
+    if (d.ps) return e < d.np ? e : d.np + 3 * (e - d.np);
     if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 2 * e : 2 * d.np + 6 * (e - d.np);
 }
 __device__ __forceinline__ int elem_row_off(const PlainDims &d, int e)
 {
-    if (d.ps) return 2 * e;
+    if (d.ps) return e < d.np ? 2 * e : 2 * d.np + 3 * (e - d.np);
     if (d.ap) return e < d.np ? e : d.np + 2 * (e - d.np);
     return e < d.np ? 3 * e : 3 * d.np + 6 * (e - d.np);
 }
 // state entries and slip-rate entries of element e
 __device__ __forceinline__ int elem_nstate(const PlainDims &d, int e)
 {
-    return d.ps ? 5 : d.ap ? 4 : (e < d.np ? 7 : 12);
+    return d.ps ? (e < d.np ? 5 : 6) : d.ap ? 4 : (e < d.np ? 7 : 12);
 }
 __device__ __forceinline__ int elem_nv(const PlainDims &d, int e)
 {
-    return d.ps ? 1 : d.ap ? (e < d.np ? 1 : 2) : (e < d.np ? 2 : 6);
+    return d.ps ? (e < d.np ? 1 : 3) : d.ap ? (e < d.np ? 1 : 2) : (e < d.np ? 2 : 6);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,7 +380,10 @@ __global__ void prepass_kernel(PlainDims d, const Friction *__restrict__ fr,
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.np + d.nv) return;
     const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (d.ps) {
+    if (d.ps && e >= d.np) {
+        dydt[so + 3] = dydt[so + 4] = dydt[so + 5] = 0.0;
+        ps_volume_prepass(y + so, rh[e - d.np], dydt + so, v + vo);
+    } else if (d.ps) {
         for (int k = 1; k < 5; k++) dydt[so + k] = 0.0;
         const double vel = ps_prepass(y + so, fr[e], dydt + so, v + vo);
         if (vmax) atomicMax(reinterpret_cast<unsigned long long *>(vmax), dbits(vel));
@@ -418,7 +445,21 @@ __device__ __forceinline__ void elem_gemv_post(const PlainDims &d, const KOperan
 {
     const int so = elem_state_off(d, e);
     const int64_t ro = elem_row_off(d, e);
-    if (d.ps) {
+    if (d.ps && e >= d.np) {
+        double acc[3];
+        elem_rows<3, MODE>(K, ro, d.n, v, t0, lane, acc);
+        if (lane == 0) {
+            for (int k = 0; k < 3; k++) {
+                if (t_out) t_out[ro + k] = acc[k];
+                dydt[so + 3 + k] = acc[k];  // viscouscycles.f90 (2-D plane strain):1048-1053
+            }
+            if (yscal) {
+                const double h = *yscal_h;
+                for (int k = 0; k < 6; k++)
+                    yscal[so + k] = fabs(y[so + k]) + fabs(h * dydt[so + k]) + 1.0e-30;
+            }
+        }
+    } else if (d.ps) {
         double acc[2];
         elem_rows<2, MODE>(K, ro, d.n, v, t0, lane, acc);
         if (lane == 0) {
@@ -554,7 +595,11 @@ __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
     }
     // ctl->vmax was cleared by the controller that accepted the previous step
     if (ctl->done || !ctl->need_rhs) return;
-    if (d.ps) {
+    if (d.ps && e >= d.np) {
+        // eMax of the step-start odefun (viscouscycles.f90 (2-D plane strain):962)
+        const double en = ps_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
+    } else if (d.ps) {
         const double vel = ps_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
         atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
     } else if (d.ap && e >= d.np) {
@@ -637,7 +682,8 @@ __device__ __forceinline__ void stage_elem(const PlainDims &d, const Friction *_
                                 b.ak[5][i]);
     }
     double *out = b.ak[stage];
-    if (d.ps) ps_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
+    if (d.ps && e >= d.np) ps_volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
+    else if (d.ps) ps_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else if (d.ap && e >= d.np) ap_volume_prepass(b.ytmp + so, rh[e - d.np], out + so, vout + vo);
     else if (d.ap) ap_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
     else if (e < d.np) patch_prepass(b.ytmp + so, fr[e], out + so, vout + vo);
@@ -851,7 +897,10 @@ disp_gemv_kernel(const double *__restrict__ O, int64_t rows, int64_t n, int np, 
     for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
         double u;
         if (ps) {
-            u = y[5 * j];
+            // plane strain: one dip slip per patch (entry 0 of 5), then e22,
+            // e23, e33 per volume (entries 0..2 of 6); viscouscycles.f90
+            // (2-D plane strain):546-566
+            u = j < np ? y[5 * j] : y[5 * (int64_t)np + 6 * ((j - np) / 3) + (j - np) % 3];
         } else if (ap) {
             u = j < np ? y[4 * j] : y[4 * np + 4 * ((j - np) >> 1) + ((j - np) & 1)];
         } else if (j < nf) {

@@ -28,6 +28,10 @@ struct ucy_ctx {
     // plane strain: initial dip traction per patch where positive (the deck's
     // tau0, fortran/2d/planestrain/src/ratestate.f90:604-609); empty: none
     std::vector<double> tau0;
+    // initial stress s22, s23, s33 of each plane-strain volume (used where
+    // not all zero; fortran/2d/planestrain/src/viscouscycles.f90:779-810);
+    // empty: none
+    std::vector<double> s0;
     // rectangles
     int nrect = 0;
     std::vector<double> rx, rL, rW, rstrike, rdip;
@@ -35,8 +39,8 @@ struct ucy_ctx {
     int ntri = 0, nve = 0;
     std::vector<int> i1, i2, i3;
     std::vector<double> vert;
-    // volumes (the 2-D antiplane model's: upper centre x(3,nv), thickness vT,
-    // width vW, dip vdip in radians)
+    // volumes (the 2-D models': upper centre x(3,nv), thickness vT, width
+    // vW, dip vdip in radians)
     int nvol = 0;
     std::vector<double> vx, vL, vT, vW, vstrike, vdip;
     // friction / rheology (element order)

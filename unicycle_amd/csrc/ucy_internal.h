@@ -64,7 +64,8 @@ struct PlainDims {
                  // stress rows s12, s13)
     int ps;      // 1: the 2-D plane-strain model (fortran/2d/planestrain: 5
                  // state entries per patch, one dip-slip rate, two traction
-                 // rows: dip and normal)
+                 // rows: dip and normal; per volume 6 state entries e22, e23,
+                 // e33, s22, s23, s33, three strain rates and three stress rows)
 };
 
 // ---- assembly launchers (k_assemble.hip) ----
@@ -100,6 +101,17 @@ void launch_ps_rows(hipStream_t st, const PsSource *src, int ns, const Receiver 
 // plane-strain buildO rows 2k, 2k+1 (east, down) for points [0, npts)
 void launch_ps_disp(hipStream_t st, const PsSource *src, int ns, const double *pts, int npts,
                     double G, double lam, double nu, double *O, int64_t ldo);
+// the plane-strain buildG blocks that involve strain volumes (receivers: np
+// patches then nv volumes in rcv; volume sources vsrc with thicknesses vT):
+// patch rows <- volume columns, volume rows <- patch columns (G and lambda
+// exchanged, as the reference passes them), volume rows <- volume columns
+void launch_ps_blocks(hipStream_t st, const PsSource *psrc, int np, const PsSource *vsrc,
+                      const double *vT, int nv, const Receiver *rcv, double G, double lam,
+                      double *K, int64_t ldk);
+// plane-strain buildO columns col0 + 3v + j of strain volumes
+void launch_ps_volume_disp(hipStream_t st, const PsSource *vsrc, const double *vT, int nv,
+                           const double *pts, int npts, double G, double lam, double *O,
+                           int64_t ldo, int64_t col0);
 // antiplane buildG blocks that involve strain volumes: every receiver (np
 // patches, then nv volumes) against the volume sources vol[4q..4q+3] =
 // (q2, q3, W, T), and the volume receivers against the segments seg

@@ -154,6 +154,27 @@ class Engine:
         self._tau0 = _f64(tau0)
         self._chk(self._L.ucy_set_initial_traction(self._h, self._tau0.size, _p(self._tau0)))
 
+    def set_planestrain_volumes(self, x, thickness, width, dip):
+        """Strain volumes of the plane-strain viscouscycles program
+        (fortran/2d/planestrain/src/viscouscycles.f90): tops x (nv, 3; x2, x3
+        used), thickness, width, dip (radians); after set_planestrain.
+        State: 6 entries per volume after the patches (e22, e23, e33, s22,
+        s23, s33); set_rheology takes the background e22, e23, e33 in
+        entries 3, 4, 5 of each edot_bg row (Gm unused)."""
+        x = _f64(x).reshape(-1, 3)
+        self._psv = [x, _f64(thickness), _f64(width), _f64(dip)]
+        self._chk(self._L.ucy_set_planestrain_volumes(self._h, x.shape[0],
+                                                      *[_p(a) for a in self._psv]))
+        self.nvol = x.shape[0]
+
+    def set_initial_volume_stress(self, s0):
+        """The plane-strain volumes' s0 (nv, 3: s22, s23, s33): a volume
+        whose three values are not all zero starts from them
+        (viscouscycles.f90 of 2d/planestrain:779-810)."""
+        self._s0 = _f64(s0).reshape(-1)
+        self._chk(self._L.ucy_set_initial_volume_stress(self._h, self._s0.size // 3,
+                                                        _p(self._s0)))
+
     def computeTractionKernelsPlaneStrain(self, x, sv, dv, nv, ns, y, W, dip, G, lambda_):
         """planestrain.f90:266-291: (td, tn) on one receiver from unit dip slip
         on ns sources."""
@@ -251,7 +272,7 @@ class Engine:
     @property
     def dim(self) -> int:
         if self.planestrain:
-            return 5 * self.np
+            return 5 * self.np + 6 * self.nvol
         return 4 * (self.np + self.nvol) if self.antiplane else 7 * self.np + 12 * self.nvol
 
     def get_kernel(self) -> np.ndarray:
