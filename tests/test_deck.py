@@ -324,3 +324,153 @@ def test_parse_point_deck():
                              ("0.01 1e-6 5", "0.01 1e-6", "7 values")):
         with pytest.raises(deck.DeckError, match=needle):
             deck.parse_point(PDECK.replace(old, new, 1))
+
+
+PSDECK = """# output directory
+out/ps
+# elastic moduli (mu, lambda)
+30d3, 20e3
+# time interval
+3.15e9
+# number of patches
+2
+# n Vpl x2 x3 width dip
+1 1e-9 0 0 1000 30
+2 1d-9, 866.0254, 500, 1000, 30
+# number of frictional patches
+2
+# n tau0 mu0 sig a b L Vo G/(2Vs)
+1 0 0.6 100 0.01 0.014 0.01 1e-6 5
+2 42.5 0.6 100 0.01 0.006 0.01 1e-6 5
+# number of observation patches
+1
+1 2
+# number of observation points
+1
+1 GPS01 5000 0
+"""
+
+PSVDECK = PSDECK.replace("""# number of observation patches
+1
+1 2
+""", """# number of strain volumes
+2
+# n e22 e23 e33 x2 x3 width thickness dip
+1 -1e-15 0 1e-15 -5e3 2e4 4e3 4e3 90
+2 -1d-15, 2e-16, 1d-15, -1e3, 2e4, 4e3, 4e3, 60
+# number of nonlinear Maxwell strain volumes
+2
+# n s22 s23 s33 gammadot0m n Q R
+1 0 0 0 1e-12 1 0 8.314
+2 0.5 -0.25 0 1e-12 3 1.2e5 8.314
+# number of thermal strain volumes
+2
+1 3.3 1200
+2 3.3 1300
+# number of observation patches
+1
+1 2
+# number of observation volumes
+1
+1 2
+""")
+
+APVDECK = APDECK.replace("""# number of observation patches
+1
+1 2
+""", """# number of strain volumes
+1
+# n e12 e13 x2 x3 width thickness dip
+1 1e-15 0 0 2e4 4e3 4e3 90
+# number of nonlinear Maxwell strain volumes
+1
+# n Gm gammadot0m n Q R
+1 1.0 1e-12 1 0 8.314
+# number of thermal strain volumes
+0
+# number of observation patches
+1
+1 2
+# number of observation volumes
+1
+1 1
+""")
+
+
+def test_parse_planestrain_deck():
+    d = deck.parse_planestrain(PSDECK)
+    assert isinstance(d, deck.PlaneStrainDeck) and d.planestrain and not d.viscous
+    assert d.mu == 30e3 and d.lambda_ == 20e3 and d.n == 2
+    np.testing.assert_array_equal(d.x, [[0, 0, 0], [0, 866.0254, 500]])
+    assert d.dip[1] == 30 * deck.DEG2RAD
+    np.testing.assert_array_equal(d.tau0, [0.0, 42.5])
+    Vpl, rake, mu0, sig, a, b, L, Vo, eta = d.friction_arrays()
+    np.testing.assert_array_equal(b, [0.014, 0.006])
+    assert d.observation_points == [("GPS01", 0.0, 5000.0, 0.0)]
+
+
+def test_parse_planestrain_viscous_deck():
+    d = deck.parse_planestrain_viscous(PSVDECK)
+    assert d.viscous and d.planestrain and d.n_vol == 2
+    np.testing.assert_array_equal(d.vol_ebg[:, 3:], [[-1e-15, 0, 1e-15], [-1e-15, 2e-16, 1e-15]])
+    assert not d.vol_ebg[:, :3].any()
+    np.testing.assert_array_equal(d.vol_x, [[0, -5e3, 2e4], [0, -1e3, 2e4]])
+    assert d.vol_dip[1] == 60 * deck.DEG2RAD          # any dip in plane strain
+    np.testing.assert_array_equal(d.s0, [[0, 0, 0], [0.5, -0.25, 0]])
+    assert d.maxwell["npower"] == [1.0, 3.0] and d.maxwell["Q"] == [0.0, 1.2e5]
+    assert d.thermal["To"] == [1200.0, 1300.0]
+    assert d.observation_states == [2] and d.observation_volumes == [2]
+
+
+def test_parse_antiplane_viscous_deck():
+    d = deck.parse_antiplane_viscous(APVDECK)
+    assert d.viscous and not d.planestrain and d.n_vol == 1
+    np.testing.assert_array_equal(d.vol_ebg[0], [0, 1e-15, 0, 0, 0, 0])
+    assert d.maxwell["Gm"] == [1.0] and not d.thermal
+    assert d.observation_volumes == [1]
+
+
+@pytest.mark.parametrize("text,old,new,needle", [
+    ("ps", "30d3, 20e3", "-1, 20e3", "shear modulus"),
+    ("ps", "1 1e-9 0 0 1000 30", "1 1e-9 0 0 0 30", "width must be positive"),
+    ("ps", "2 42.5 0.6", "3 42.5 0.6", "unexpected index"),
+    ("psv", "1 -1e-15 0 1e-15 -5e3 2e4", "1 -1e-15 0 1e-15 -5e3 -2e4", "depth must be positive"),
+    ("psv", "-1e3, 2e4, 4e3, 4e3, 60", "-1e3, 2e4, 4e3, 0, 60", "dimension must be positive"),
+    ("psv", "# number of nonlinear Maxwell strain volumes\n2",
+     "# number of nonlinear Maxwell strain volumes\n1", "none or all"),
+    ("psv", "1 0 0 0 1e-12 1 0 8.314", "1 0 0 0 1e-12 1 0 0", "R must be positive"),
+    ("psv", "2 3.3 1300", "2 3.3 0", "To must be positive"),
+    ("psv", "# number of observation volumes\n1\n1 2", "# number of observation volumes\n1\n1 3",
+     "not a volume"),
+    ("apv", "4e3 4e3 90", "4e3 4e3 60", "vertical"),
+    ("apv", "1 1.0 1e-12 1 0 8.314", "1 0 1e-12 1 0 8.314", "sig0 must be positive"),
+])
+def test_2d_viscous_deck_errors(text, old, new, needle):
+    src, parse = {"ps": (PSDECK, deck.parse_planestrain),
+                  "psv": (PSVDECK, deck.parse_planestrain_viscous),
+                  "apv": (APVDECK, deck.parse_antiplane_viscous)}[text]
+    assert old in src
+    with pytest.raises(deck.DeckError, match=needle):
+        parse(src.replace(old, new, 1))
+
+
+def test_2d_decks_round_trip():
+    m = configs.planestrain_viscous(n=7, n2=2, n3=2)
+    s0 = np.arange(12.0).reshape(4, 3) * 1e-3
+    d = deck.parse_planestrain_viscous(deck.format_2d(
+        "w", m.fault, 1e9, vol=m, s0=s0, tau0=np.full(7, 3.0), observe=[1, 7], observe_vol=[4],
+        points=[("A", 1e3, 0.0)]))
+    np.testing.assert_array_equal(d.x, m.fault.x)
+    np.testing.assert_allclose(d.dip, m.fault.dip, rtol=1e-15)
+    np.testing.assert_array_equal(d.vol_x, m.vx)
+    np.testing.assert_array_equal(d.vol_ebg, m.edot_bg())
+    np.testing.assert_array_equal(d.s0, s0)
+    np.testing.assert_array_equal(d.tau0, np.full(7, 3.0))
+    assert (d.mu, d.lambda_) == (m.fault.mu, m.fault.lambda_)
+    for k, key in enumerate(("gammadot0", "npower", "Q", "R")):
+        np.testing.assert_array_equal(d.maxwell[key], getattr(m, key))
+    np.testing.assert_array_equal(d.thermal["To"], m.To)
+    ma = configs.antiplane_viscous(n=6, n2=2, n3=1)
+    da = deck.parse_antiplane_viscous(deck.format_2d("w", ma.fault, 1e9, vol=ma))
+    np.testing.assert_array_equal(da.vol_ebg, ma.edot_bg())
+    np.testing.assert_array_equal(da.maxwell["Gm"], ma.Gm)

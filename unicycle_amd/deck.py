@@ -36,6 +36,7 @@ from __future__ import annotations
 
 import shlex
 from dataclasses import dataclass, field
+from typing import ClassVar
 
 import numpy as np
 
@@ -419,7 +420,8 @@ class AntiplaneDeck:
     """The input deck of fortran/2d/antiplane/src/ratestate.f90 (init,
     958-1230): shear modulus, interval, vertical segments (Vpl, x2, x3,
     width, dip) with friction, observed patches, observation points (name,
-    x2, x3)."""
+    x2, x3).  The subclasses are the decks of the other 2-D programs; the
+    volume fields are used by the viscouscycles ones."""
     wdir: str
     mu: float
     interval: float
@@ -429,10 +431,30 @@ class AntiplaneDeck:
     friction: dict = field(default_factory=dict)
     observation_states: list = field(default_factory=list)
     observation_points: list = field(default_factory=list)   # (name, x1 = 0, x2, x3)
+    # strain volumes: tops (x2, x3 used), width, thickness, dip; the
+    # background strain rate in the 3-D component order (e11, e12, e13, e22,
+    # e23, e33: the antiplane programs fill e12, e13, the plane-strain ones
+    # e22, e23, e33); Maxwell rows (Gm, gammadot0, npower, Q, R) and thermal
+    # rows (rhoc, To), each for none or all volumes
+    vol_x: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    vol_width: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_thickness: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_dip: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    vol_ebg: np.ndarray = field(default_factory=lambda: np.zeros((0, 6)))
+    n_maxwell: int = 0
+    maxwell: dict = field(default_factory=dict)
+    thermal: dict = field(default_factory=dict)
+    observation_volumes: list = field(default_factory=list)   # 1-based volume indices
+    planestrain: ClassVar[bool] = False
+    viscous: ClassVar[bool] = False
 
     @property
     def n(self) -> int:
         return self.width.size
+
+    @property
+    def n_vol(self) -> int:
+        return self.vol_width.size
 
     def friction_arrays(self):
         """Engine.set_friction order (the rake is unused in 2-D)."""
@@ -441,20 +463,137 @@ class AntiplaneDeck:
         return tuple(np.asarray(fr[k], dtype=np.float64) for k in keys)
 
 
-def parse_antiplane(text: str) -> AntiplaneDeck:
-    """Parse a 2-D antiplane deck; raises DeckError where the reference stops
-    (fortran/2d/antiplane/src/ratestate.f90:1003-1230)."""
+@dataclass
+class AntiplaneViscousDeck(AntiplaneDeck):
+    """The deck of fortran/2d/antiplane/src/viscouscycles.f90 (init,
+    1124-1760): the ratestate deck with strain volumes (e12, e13, x2, x3,
+    width, thickness, dip = 90), their Maxwell rows (Gm, gammadot0, n, Q, R)
+    and thermal rows, and observed volumes after the observed patches."""
+    viscous: ClassVar[bool] = True
+
+
+@dataclass
+class PlaneStrainDeck(AntiplaneDeck):
+    """The deck of fortran/2d/planestrain/src/ratestate.f90 (init, 895-1374):
+    elastic moduli (mu, lambda), interval, dipping segments (Vpl, x2, x3,
+    width, dip) with friction rows that begin with tau0, observed patches,
+    observation points."""
+    lambda_: float = 0.0
+    tau0: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    planestrain: ClassVar[bool] = True
+
+
+@dataclass
+class PlaneStrainViscousDeck(PlaneStrainDeck):
+    """The deck of fortran/2d/planestrain/src/viscouscycles.f90 (init,
+    1231-2040): the plane-strain ratestate deck with strain volumes (e22,
+    e23, e33, x2, x3, width, thickness, dip), their nonlinear Maxwell rows
+    (s22, s23, s33 of the initial stress, gammadot0, n, Q, R) and thermal
+    rows, and observed volumes after the observed patches."""
+    s0: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    viscous: ClassVar[bool] = True
+
+
+def _volumes_2d(L: _Lines, d: AntiplaneDeck):
+    """strain volumes, Maxwell rows and thermal rows of the 2-D viscouscycles
+    decks (fortran/2d/antiplane/src/viscouscycles.f90:1410-1590,
+    fortran/2d/planestrain/src/viscouscycles.f90:1522-1688)"""
+    ps = d.planestrain
+    nv = _int(L.values("number of strain volumes", 1)[0])
+    if nv <= 0:
+        return
+    ne = 3 if ps else 2                      # background strain-rate components
+    rows = []
+    for k in range(1, nv + 1):
+        t = L.values(f"strain volume {k}", 6 + ne)
+        v = [_real(x) for x in t[1:6 + ne]]
+        if not ps and 90.0 != v[6]:          # antiplane viscouscycles.f90:1457-1462
+            raise DeckError("strain volume must be vertical (dip=90)")
+        if 0 > v[ne + 1]:
+            raise DeckError("depth must be positive")
+        if _int(t[0]) != k:
+            raise DeckError("invalid strain volume definition: unexpected index")
+        if min(v[ne + 2], v[ne + 3]) <= 0.0:
+            raise DeckError("strain volume dimension must be positive")
+        rows.append(v)
+    r = np.array(rows)
+    d.vol_ebg = np.zeros((nv, 6))
+    if ps:
+        d.vol_ebg[:, 3:6] = r[:, 0:3]
+    else:
+        d.vol_ebg[:, 1:3] = r[:, 0:2]
+    d.vol_x = np.stack([np.zeros(nv), r[:, ne], r[:, ne + 1]], 1)
+    d.vol_width, d.vol_thickness = r[:, ne + 2].copy(), r[:, ne + 3].copy()
+    d.vol_dip = r[:, ne + 4] * DEG2RAD
+    d.n_maxwell = _int(L.values("number of nonlinear Maxwell strain volumes", 1)[0])
+    if d.n_maxwell != 0:
+        if d.n_maxwell != nv:
+            raise DeckError("nonlinear Maxwell properties are for none or all strain volumes")
+        keys = ("gammadot0", "npower", "Q", "R")
+        d.maxwell = {k: [] for k in ("Gm",) + keys}
+        s0 = []
+        for k in range(1, nv + 1):
+            if ps:                           # i s22 s23 s33 gammadot0 n Q R
+                t = L.values(f"Maxwell row {k}", 8)
+                v = [_real(x) for x in t[1:8]]
+                s0.append(v[0:3])
+                gm, rest = 1.0, v[3:7]
+            else:                            # i Gm gammadot0 n Q R
+                t = L.values(f"Maxwell row {k}", 6)
+                v = [_real(x) for x in t[1:6]]
+                gm, rest = v[0], v[1:5]
+                if 0 >= gm:
+                    raise DeckError("sig0 must be positive")
+            if 0 >= rest[3]:
+                raise DeckError("R must be positive")
+            if _int(t[0]) != k:
+                raise DeckError("invalid property definition for strain volume: unexpected index")
+            d.maxwell["Gm"].append(gm)
+            for key, x in zip(keys, rest):
+                d.maxwell[key].append(x)
+        if ps:
+            d.s0 = np.array(s0)
+    nth = _int(L.values("number of thermal strain volumes", 1)[0])
+    if nth != 0:
+        if nth != nv:
+            raise DeckError("thermal properties are for none or all strain volumes")
+        d.thermal = {"rhoc": [], "To": []}
+        for k in range(1, nv + 1):
+            t = L.values(f"thermal row {k}", 3)
+            rhoc, To = _real(t[1]), _real(t[2])
+            if 0 >= To:
+                raise DeckError("To must be positive")
+            if _int(t[0]) != k:
+                raise DeckError("invalid property definition for strain volume: unexpected index")
+            d.thermal["rhoc"].append(rhoc)
+            d.thermal["To"].append(To)
+
+
+def _parse_2d(text: str, cls):
+    """A deck of one of the 2-D programs; raises DeckError where the
+    reference stops (the `init` routines of fortran/2d/antiplane/src/
+    ratestate.f90:1003-1230, viscouscycles.f90:1250-1760 and of
+    fortran/2d/planestrain/src/ratestate.f90:1041-1240, viscouscycles.f90:
+    1386-1790)."""
     L = _Lines(text)
+    ps = cls.planestrain
     wdir = L.next("output directory").strip()
-    mu = _real(L.values("shear modulus", 1)[0])
+    if ps:
+        t = L.values("elastic moduli", 2)
+        mu, lam = _real(t[0]), _real(t[1])  # mu first in the plane-strain decks
+    else:
+        mu = _real(L.values("shear modulus", 1)[0])
     if 0 > mu:
         raise DeckError("shear modulus must be positive")
     interval = _real(L.values("time interval", 1)[0])
     if interval <= 0.0:
         raise DeckError("simulation time must be positive")
-    d = AntiplaneDeck(wdir=wdir, mu=mu, interval=interval)
+    d = cls(wdir=wdir, mu=mu, interval=interval)
+    if ps:
+        d.lambda_ = lam
     ns = _int(L.values("number of patches", 1)[0])
-    fr = {k: [] for k in ("Vpl", "mu0", "sig", "a", "b", "L", "Vo", "damping")}
+    fkeys = (("tau0",) if ps else ()) + ("mu0", "sig", "a", "b", "L", "Vo", "damping")
+    fr = {k: [] for k in ("Vpl",) + fkeys}
     if ns > 0:
         rows = []
         for k in range(1, ns + 1):
@@ -473,13 +612,19 @@ def parse_antiplane(text: str) -> AntiplaneDeck:
         if dummy != ns:
             raise DeckError("all patches require frictional properties")
         for k in range(1, ns + 1):
-            t = L.values(f"friction row {k}", 8)
+            t = L.values(f"friction row {k}", 1 + len(fkeys))
             if _int(t[0]) != k:
                 raise DeckError("invalid friction property definition: unexpected index")
-            for key, tok in zip(("mu0", "sig", "a", "b", "L", "Vo", "damping"), t[1:8]):
+            for key, tok in zip(fkeys, t[1:1 + len(fkeys)]):
                 fr[key].append(_real(tok))
+    if ps:
+        d.tau0 = np.asarray(fr.pop("tau0"), dtype=np.float64)
     d.friction = fr
+    if cls.viscous:
+        _volumes_2d(L, d)
     d.observation_states = _indices(L, "observation patch")
+    if cls.viscous:
+        d.observation_volumes = _indices(L, "observation volume")
     npnt = _int(L.values("number of observation points", 1)[0])
     for k in range(1, npnt + 1):
         t = L.values(f"observation point {k}", 4)
@@ -487,14 +632,37 @@ def parse_antiplane(text: str) -> AntiplaneDeck:
             raise DeckError("observation points: unexpected index")
         # the along-strike coordinate is zero in 2-D (ratestate.f90:1193-1194)
         d.observation_points.append((t[1][:10], 0.0, _real(t[2]), _real(t[3])))
-    # ratestate.f90:1211-1217 tests .AND.: a deck without patches but with a
-    # positive interval is not refused there; it has nothing to integrate here
-    if d.n == 0:
+    # the references test (ns = 0 .AND. interval <= 0), which the interval
+    # check excludes; a deck with no element has nothing to integrate here
+    if d.n + d.n_vol == 0:
         raise DeckError("nothing to do")
     for j in d.observation_states:
         if not 1 <= j <= d.n:
             raise DeckError(f"observation patch {j} is not an element")
+    for j in d.observation_volumes:
+        if not 1 <= j <= d.n_vol:
+            raise DeckError(f"observation volume {j} is not a volume")
     return d
+
+
+def parse_antiplane(text: str) -> AntiplaneDeck:
+    """Parse a 2-D antiplane deck (fortran/2d/antiplane/src/ratestate.f90)."""
+    return _parse_2d(text, AntiplaneDeck)
+
+
+def parse_antiplane_viscous(text: str) -> AntiplaneViscousDeck:
+    """Parse a 2-D antiplane viscouscycles deck."""
+    return _parse_2d(text, AntiplaneViscousDeck)
+
+
+def parse_planestrain(text: str) -> PlaneStrainDeck:
+    """Parse a 2-D plane-strain ratestate deck."""
+    return _parse_2d(text, PlaneStrainDeck)
+
+
+def parse_planestrain_viscous(text: str) -> PlaneStrainViscousDeck:
+    """Parse a 2-D plane-strain viscouscycles deck."""
+    return _parse_2d(text, PlaneStrainViscousDeck)
 
 
 def format_antiplane(wdir, model, interval, observe=(), points=()) -> str:
@@ -514,6 +682,56 @@ def format_antiplane(wdir, model, interval, observe=(), points=()) -> str:
             model.damping[k])]))
     out += ["# number of observation patches", str(len(observe))]
     out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
+    out += ["# number of observation points", str(len(points))]
+    out += [f"{k + 1} {n} {f(x2)} {f(x3)}" for k, (n, x2, x3) in enumerate(points)]
+    return "\n".join(out) + "\n"
+
+
+def format_2d(wdir, fault, interval, observe=(), points=(), vol=None, observe_vol=(),
+              tau0=None, s0=None, maxwell=True, thermal=True) -> str:
+    """A deck of one of the 2-D programs in the layout _parse_2d() reads:
+    fault an AntiplaneModel or a PlaneStrainModel (configs: the plane-strain
+    decks carry mu, lambda and tau0 per patch), vol an AntiplaneViscousModel
+    or PlaneStrainViscousModel for the viscouscycles decks (None: a
+    ratestate deck)."""
+    R2D = 1.0 / DEG2RAD
+    f = lambda v: repr(float(v))  # noqa: E731
+    ps = hasattr(fault, "lambda_")
+    out = ["# output directory", str(wdir)]
+    out += (["# elastic moduli", f"{f(fault.mu)} {f(fault.lambda_)}"] if ps
+            else ["# shear modulus", f(fault.mu)])
+    out += ["# time interval", f(interval), "# number of patches", str(fault.n),
+            "# n Vpl x2 x3 width dip"]
+    for k in range(fault.n):
+        out.append(" ".join([str(k + 1)] + [f(v) for v in (
+            fault.Vpl[k], fault.x[k][1], fault.x[k][2], fault.width[k], fault.dip[k] * R2D)]))
+    if fault.n:
+        out += ["# number of frictional patches", str(fault.n)]
+    for k in range(fault.n):
+        t0 = [] if not ps else [0.0 if tau0 is None else tau0[k]]
+        out.append(" ".join([str(k + 1)] + [f(v) for v in t0 + [
+            fault.mu0[k], fault.sig[k], fault.a[k], fault.b[k], fault.L[k], fault.Vo[k],
+            fault.damping[k]]]))
+    if vol is not None:
+        nv = vol.nvol
+        out += ["# number of strain volumes", str(nv)]
+        for k in range(nv):
+            e = [vol.e22[k], vol.e23[k], vol.e33[k]] if ps else [vol.e12[k], vol.e13[k]]
+            out.append(" ".join([str(k + 1)] + [f(v) for v in e + [
+                vol.vx[k][1], vol.vx[k][2], vol.vwidth[k], vol.thickness[k], vol.vdip[k] * R2D]]))
+        out += ["# number of nonlinear Maxwell strain volumes", str(nv if maxwell else 0)]
+        for k in range(nv if maxwell else 0):
+            head = (list(s0[k]) if s0 is not None else [0.0, 0.0, 0.0]) if ps else [vol.Gm[k]]
+            out.append(" ".join([str(k + 1)] + [f(v) for v in head + [
+                vol.gammadot0[k], vol.npower[k], vol.Q[k], vol.R[k]]]))
+        out += ["# number of thermal strain volumes", str(nv if thermal else 0)]
+        for k in range(nv if thermal else 0):
+            out.append(f"{k + 1} {f(3.3)} {f(vol.To[k])}")
+    out += ["# number of observation patches", str(len(observe))]
+    out += [f"{k + 1} {j}" for k, j in enumerate(observe)]
+    if vol is not None:
+        out += ["# number of observation volumes", str(len(observe_vol))]
+        out += [f"{k + 1} {j}" for k, j in enumerate(observe_vol)]
     out += ["# number of observation points", str(len(points))]
     out += [f"{k + 1} {n} {f(x2)} {f(x3)}" for k, (n, x2, x3) in enumerate(points)]
     return "\n".join(out) + "\n"

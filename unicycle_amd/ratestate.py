@@ -24,7 +24,14 @@ of iterations.  It writes what the reference writes each iteration
 
 A 2-D antiplane deck (unicycle_amd.antiplane, fortran/2d/antiplane/src/
 ratestate.f90:254-276) runs the same loop with 4 state entries per patch,
-one displacement column in opts-<name>.dat and no VTP file.
+one displacement column in opts-<name>.dat and no VTP file; a 2-D
+plane-strain deck (unicycle_amd.planestrain, fortran/2d/planestrain/src/
+ratestate.f90:285-305) with 5 state entries per patch, each value of a
+patch row after a blank (its format's X), two displacement columns (east,
+down).  The 2-D viscouscycles decks (unicycle_amd.antiplane_viscouscycles,
+unicycle_amd.planestrain_viscouscycles) add volume-%08d.dat (4 or 6 state
+entries and their derivatives), d, df and dl in opts-<name>.dat and eMax
+in the progress line, as the 3-D one.
 
 A viscouscycles deck (unicycle_amd.viscouscycles, viscouscycles.f90:413-434)
 runs the same loop with strain volumes and adds: volume-%08d.dat per observed
@@ -52,9 +59,23 @@ def setup(d, device: int = 0, precision: int = PREC_F64) -> Engine:
     """Mesh, friction (and for a viscouscycles deck the volumes and their
     rheology), buildG and buildO on the GPU."""
     eng = Engine(device)
-    if isinstance(d, deckmod.AntiplaneDeck):   # fortran/2d/antiplane
-        eng.set_antiplane(d.mu, d.x, d.width, d.dip)
-        eng.set_friction(*d.friction_arrays())
+    if isinstance(d, deckmod.AntiplaneDeck):   # fortran/2d
+        if d.planestrain:
+            eng.set_planestrain(d.lambda_, d.mu, d.x, d.width, d.dip)
+            if np.any(d.tau0 > 0):
+                eng.set_initial_traction(d.tau0)
+            if d.viscous and d.n_vol:
+                eng.set_planestrain_volumes(d.vol_x, d.vol_thickness, d.vol_width, d.vol_dip)
+        else:
+            eng.set_antiplane(d.mu, d.x, d.width, d.dip)
+            if d.viscous and d.n_vol:
+                eng.set_antiplane_volumes(d.vol_x, d.vol_thickness, d.vol_width, d.vol_dip)
+        if d.viscous and d.n_vol:
+            eng.set_rheology(*rheology(d))
+            if d.planestrain and d.s0.size and np.any(d.s0 != 0.0):
+                eng.set_initial_volume_stress(d.s0)
+        if d.n:
+            eng.set_friction(*d.friction_arrays())
         eng.buildG(precision)
         if d.observation_points:
             eng.buildO([p[1:] for p in d.observation_points])
@@ -75,10 +96,10 @@ def setup(d, device: int = 0, precision: int = PREC_F64) -> Engine:
     return eng
 
 
-def rheology(d: deckmod.ViscousDeck):
-    """Engine.set_rheology arguments of a viscouscycles deck: the Maxwell rows
-    (gammadot0, n, Gm, Q, R), the volumes' loading strain rates and the thermal
-    To.  Without Maxwell rows the volumes do not flow (viscouscycles.f90:963:
+def rheology(d):
+    """Engine.set_rheology arguments of a viscouscycles deck (3-D or 2-D): the
+    Maxwell rows (gammadot0, n, Gm, Q, R), the volumes' loading strain rates
+    and the thermal To.  Without Maxwell rows the volumes do not flow (viscouscycles.f90:963:
     the strain rate stays zero); without thermal rows the Arrhenius factor
     needs Q = 0 (the reference would read an unset temperature)."""
     nv = d.n_vol
@@ -108,16 +129,21 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     """Run the deck (ratestate, or viscouscycles for a ViscousDeck); returns
     (iterations, final time, final state)."""
     fmt = deckmod.es_field
-    viscous = isinstance(d, deckmod.ViscousDeck)
-    ap = isinstance(d, deckmod.AntiplaneDeck)
-    dpatch = 4 if ap else DPATCH   # STATE_VECTOR_DGF_PATCH of the 2-D macros.f90: 4
+    viscous3d = isinstance(d, deckmod.ViscousDeck)
+    ap = isinstance(d, deckmod.AntiplaneDeck)          # the 2-D programs
+    ps = ap and d.planestrain
+    viscous = viscous3d or (ap and d.viscous)
+    # STATE_VECTOR_DGF_PATCH / _VOLUME of the macros.f90 of each program
+    dpatch = 5 if ps else 4 if ap else DPATCH
+    dvol = 6 if ps else 4 if ap else DVOLUME
     eng = setup(d, device, precision)
     # initStateVector: ratestate.f90:548-610, viscouscycles.f90:710-820, 2-D
-    # antiplane ratestate.f90:568-605
+    # antiplane ratestate.f90:568-605, viscouscycles.f90:697-770, 2-D plane
+    # strain ratestate.f90:585-627, viscouscycles.f90:714-837
     y = eng.initStateVector(1 if viscous else 0)
     npatch = d.n if ap else d.n_rect + d.n_tri
     nvol = d.n_vol if viscous else 0
-    if nvol and not d.n_maxwell:
+    if viscous3d and nvol and not d.n_maxwell:
         # no Maxwell rows: the reference's initial stress reads unset
         # properties; here the volumes start unstressed
         for v in range(nvol):
@@ -125,8 +151,11 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     time, dt_next, dt_done = 0.0, 1.0e-3, 0.0     # ratestate.f90:166-167
     obs_vol = d.observation_volumes if viscous else []
     idx = [dpatch * (j - 1) + k for j in d.observation_states for k in range(dpatch)]
-    idx += [DPATCH * npatch + DVOLUME * (j - 1) + k for j in obs_vol for k in range(DVOLUME)]
-    ncol = 1 if ap else 3   # displacement components per point
+    idx += [dpatch * npatch + dvol * (j - 1) + k for j in obs_vol for k in range(dvol)]
+    ncol = 2 if ps else 1 if ap else 3   # displacement components per point
+    # the plane-strain ratestate program writes each value of a patch row
+    # after a blank ("X," in its format, ratestate.f90:543-547)
+    sep = " " if ps and not viscous else ""
     idx = np.array(idx, dtype=np.int64)
     files, pfiles = [], []
     if write_files:
@@ -137,7 +166,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
             with open(os.path.join(d.wdir, "patch-rectangular.vtp"), "w") as f:
                 f.write(vtp.rectangles_vtp(d.rect_x, d.rect_length, d.rect_width, d.rect_strike,
                                            d.rect_dip, rect_friction(d)))
-        if viscous:
+        if viscous3d:
             with open(os.path.join(d.wdir, "volume.vtp"), "w") as f:
                 f.write(vtp.volumes_vtp(d.vol_x, d.vol_length, d.vol_thickness, d.vol_width,
                                         d.vol_strike, d.vol_dip))
@@ -148,7 +177,7 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
         # viscouscycles.f90:370-392)
         files = [(open(os.path.join(d.wdir, f"patch-{j:08d}.dat"), "w"), dpatch)
                  for j in d.observation_states]
-        files += [(open(os.path.join(d.wdir, f"volume-{j:08d}.dat"), "w"), DVOLUME)
+        files += [(open(os.path.join(d.wdir, f"volume-{j:08d}.dat"), "w"), dvol)
                   for j in obs_vol]
         # ratestate.f90:291-304
         pfiles = [open(os.path.join(d.wdir, f"opts-{p[0].strip()}.dat"), "w")
@@ -175,8 +204,8 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
                         ds = res.obs_dydt[k, o:o + dgf]
                         o += dgf
                         f.write(fmt(t_start, 19, 12, 2)
-                                + "".join(fmt(v, 20, 12, 3) for v in ys)
-                                + "".join(fmt(v, 20, 12, 3) for v in ds) + "\n")
+                                + "".join(sep + fmt(v, 20, 12, 3) for v in ys)
+                                + "".join(sep + fmt(v, 20, 12, 3) for v in ds) + "\n")
                     for j, f in enumerate(pfiles):
                         c0, c1 = ncol * j, ncol * (j + 1)
                         cols = [res.disp[k, c0:c1]]
@@ -208,10 +237,16 @@ def run(d: deckmod.Deck, epsilon: float = 1.0e-6, max_step: float = 1.7976931348
     return it, time, y
 
 
-def main(argv=None, viscous: bool = False, antiplane: bool = False) -> int:
-    name = "viscouscycles" if viscous else ("antiplane" if antiplane else "ratestate")
-    media = ("three-dimensional viscoelastic media" if viscous else
-             "antiplane strain" if antiplane else "three-dimensional media")
+def main(argv=None, viscous: bool = False, antiplane: bool = False,
+         planestrain: bool = False) -> int:
+    two_d = antiplane or planestrain
+    name = ("antiplane" if antiplane else "planestrain" if planestrain else "")
+    name = (name + "_viscouscycles" if viscous else name) if two_d else (
+        "viscouscycles" if viscous else "ratestate")
+    media = ("antiplane strain" if antiplane else "plane strain" if planestrain
+             else "three-dimensional media")
+    if viscous:
+        media = "viscoelastic media, " + media if two_d else "three-dimensional viscoelastic media"
     ap = argparse.ArgumentParser(prog=f"unicycle_amd.{name}",
                                  description=f"quasi-dynamic earthquake simulation in {media} "
                                              "(GPU)")
@@ -230,7 +265,9 @@ def main(argv=None, viscous: bool = False, antiplane: bool = False) -> int:
     text = open(a.deck).read() if a.deck else sys.stdin.read()
     try:
         if antiplane:
-            d = deckmod.parse_antiplane(text)
+            d = (deckmod.parse_antiplane_viscous if viscous else deckmod.parse_antiplane)(text)
+        elif planestrain:
+            d = (deckmod.parse_planestrain_viscous if viscous else deckmod.parse_planestrain)(text)
         else:
             d = deckmod.parse_viscous(text) if viscous else deckmod.parse(text)
         if viscous and d.n_vol:
@@ -238,12 +275,12 @@ def main(argv=None, viscous: bool = False, antiplane: bool = False) -> int:
     except deckmod.DeckError as exc:
         print(f"input error: {exc}", file=sys.stderr)
         return 1
-    print("# RATESTATE" if antiplane else f"# {name.upper()}")
+    print("# VISCOUSCYCLES" if viscous else "# RATESTATE")
     print(f"# quasi-dynamic earthquake simulation in {media}")
     print(f"# numerical accuracy: {deckmod.es_field(a.epsilon, 11, 4, 2)}")
     print(f"# maximum iterations: {a.maximum_iterations:11d}")
     print(f"# output directory\n  {d.wdir}")
-    if antiplane:
+    if two_d:
         print(f"# {d.n} patches, {len(d.observation_states)} observed")
     else:
         print(f"# {d.n_rect} rectangular and {d.n_tri} triangular patches, "
