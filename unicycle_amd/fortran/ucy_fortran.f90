@@ -13,6 +13,12 @@
 !!   computeTractionKernelsOkada92(...) okada92.f90:368-377 (and the other
 !!                                      Green's-function routines)
 !!   buildO(in,layout,O,Of,Ol)          greens.f90:302 (O stays on the GPU)
+!!   computeReferenceSystemOkada92 / Nikkhoo15 / VerticalStrainVolume
+!!                                      okada92.f90:59, nikkhoo15.f90:50,
+!!                                      strainvolume.f90:135
+!! and ucy_setup_mesh / ucy_setup_friction / ucy_setup_rheology / buildG_gpu
+!! take the mesh of any of the 3-D programs (rectangles, triangles, strain
+!! volumes with their rheology) in the reference's array shapes.
 !!   exportPoints()                     ratestate.f90:414-486 (d = O u)
 !! A nonzero status from the library becomes the reference's STOP with the
 !! library's message (the library itself never aborts).
@@ -105,6 +111,34 @@ MODULE ucy_fortran
                                      R(*), To(*)
        INTEGER(C_INT) :: ucy_set_rheology
      END FUNCTION ucy_set_rheology
+
+     FUNCTION ucy_reference_system_okada92(ns, x, L, W, strike, dip, sv, dv, nv, xc) &
+          BIND(C, NAME="ucy_reference_system_okada92")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: ns
+       REAL(C_DOUBLE), INTENT(IN) :: x(3,*), L(*), W(*), strike(*), dip(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: sv(3,*), dv(3,*), nv(3,*), xc(3,*)
+       INTEGER(C_INT) :: ucy_reference_system_okada92
+     END FUNCTION ucy_reference_system_okada92
+
+     FUNCTION ucy_reference_system_nikkhoo15(nVe, v, ns, i1, i2, i3, sv, dv, nv, xc) &
+          BIND(C, NAME="ucy_reference_system_nikkhoo15")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: nVe, ns
+       REAL(C_DOUBLE), INTENT(IN) :: v(3,*)
+       INTEGER(C_INT), INTENT(IN) :: i1(*), i2(*), i3(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: sv(3,*), dv(3,*), nv(3,*), xc(3,*)
+       INTEGER(C_INT) :: ucy_reference_system_nikkhoo15
+     END FUNCTION ucy_reference_system_nikkhoo15
+
+     FUNCTION ucy_reference_system_strainvolume(ns, x, L, W, strike, sv, dv, nv, xc) &
+          BIND(C, NAME="ucy_reference_system_strainvolume")
+       IMPORT :: C_INT, C_DOUBLE
+       INTEGER(C_INT), VALUE :: ns
+       REAL(C_DOUBLE), INTENT(IN) :: x(3,*), L(*), W(*), strike(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: sv(3,*), dv(3,*), nv(3,*), xc(3,*)
+       INTEGER(C_INT) :: ucy_reference_system_strainvolume
+     END FUNCTION ucy_reference_system_strainvolume
 
      FUNCTION ucy_build_kernel(ctx, precision) BIND(C, NAME="ucy_build_kernel")
        IMPORT :: C_PTR, C_INT
@@ -239,6 +273,95 @@ CONTAINS
                    damping), "ucy_set_friction")
     CALL ucy_check(ucy_build_kernel(ucy_ctx, UCY_PREC_F64), "buildG")
   END SUBROUTINE ucy_setup_rectangles
+
+  !> the elements of any 3-D program (ratestate.f90:1128-1396, viscouscycles.f90:
+  !! 1730-1820): rectangles x(3,nRect) (upper corners), length, width,
+  !! strike, dip; triangles i1, i2, i3 (1-based) on vertices(3,nVe); vertical
+  !! strain volumes vx(3,nVol) (upper corners), length, thickness, width,
+  !! strike; angles in radians.  A count of zero skips its arrays.
+  SUBROUTINE ucy_setup_mesh(lambda, mu, nRect, x, length, width, strike, dip, &
+                            nTri, i1, i2, i3, nVe, vertices, &
+                            nVol, vx, vlength, vthickness, vwidth, vstrike)
+    REAL*8, INTENT(IN) :: lambda, mu
+    INTEGER, INTENT(IN) :: nRect, nTri, nVe, nVol
+    REAL*8, DIMENSION(3,*), INTENT(IN) :: x, vertices, vx
+    REAL*8, DIMENSION(*), INTENT(IN) :: length, width, strike, dip
+    INTEGER, DIMENSION(*), INTENT(IN) :: i1, i2, i3
+    REAL*8, DIMENSION(*), INTENT(IN) :: vlength, vthickness, vwidth, vstrike
+    INTEGER(C_INT), ALLOCATABLE :: j1(:), j2(:), j3(:)
+    CALL ucy_check(ucy_set_elastic(ucy_ctx, lambda, mu), "ucy_set_elastic")
+    IF (0 .LT. nRect) &
+       CALL ucy_check(ucy_set_rectangles(ucy_ctx, INT(nRect, C_INT), x, length, width, strike, &
+                      dip), "ucy_set_rectangles")
+    IF (0 .LT. nTri) THEN
+       ALLOCATE(j1(nTri), j2(nTri), j3(nTri))
+       j1 = INT(i1(1:nTri), C_INT)
+       j2 = INT(i2(1:nTri), C_INT)
+       j3 = INT(i3(1:nTri), C_INT)
+       CALL ucy_check(ucy_set_triangles(ucy_ctx, INT(nTri, C_INT), j1, j2, j3, &
+                      INT(nVe, C_INT), vertices), "ucy_set_triangles")
+       DEALLOCATE(j1, j2, j3)
+    END IF
+    IF (0 .LT. nVol) &
+       CALL ucy_check(ucy_set_volumes(ucy_ctx, INT(nVol, C_INT), vx, vlength, vthickness, vwidth, &
+                      vstrike), "ucy_set_volumes")
+  END SUBROUTINE ucy_setup_mesh
+
+  !> friction of every patch, rectangles then triangles (the decks' friction
+  !! sections, ratestate.f90:1180-1216 and 1326-1362)
+  SUBROUTINE ucy_setup_friction(np, Vpl, rake, mu0, sig, a, b, L, Vo, damping)
+    INTEGER, INTENT(IN) :: np
+    REAL*8, DIMENSION(np), INTENT(IN) :: Vpl, rake, mu0, sig, a, b, L, Vo, damping
+    CALL ucy_check(ucy_set_friction(ucy_ctx, INT(np, C_INT), Vpl, rake, mu0, sig, a, b, L, Vo, &
+                   damping), "ucy_set_friction")
+  END SUBROUTINE ucy_setup_friction
+
+  !> the volumes' nonlinear Maxwell and thermal properties (viscouscycles.f90:
+  !! 1821-1930) with the loading strain rate e(6,nv) (e11, e12, e13, e22, e23, e33)
+  SUBROUTINE ucy_setup_rheology(nv, gammadot0, npower, e, Gm, Q, R, To)
+    INTEGER, INTENT(IN) :: nv
+    REAL*8, DIMENSION(nv), INTENT(IN) :: gammadot0, npower, Gm, Q, R, To
+    REAL*8, DIMENSION(6,nv), INTENT(IN) :: e
+    CALL ucy_check(ucy_set_rheology(ucy_ctx, INT(nv, C_INT), gammadot0, npower, e, Gm, Q, R, &
+                   To), "ucy_set_rheology")
+  END SUBROUTINE ucy_setup_rheology
+
+  !> buildG (greens.f90:73-275): K assembled and kept on the GPU
+  SUBROUTINE buildG_gpu(precision)
+    INTEGER, INTENT(IN) :: precision
+    CALL ucy_check(ucy_build_kernel(ucy_ctx, INT(precision, C_INT)), "buildG")
+  END SUBROUTINE buildG_gpu
+
+  !> computeReferenceSystemOkada92 (okada92.f90:59-87)
+  SUBROUTINE computeReferenceSystemOkada92_gpu(ns, x, L, W, strike, dip, sv, dv, nv, xc)
+    INTEGER, INTENT(IN) :: ns
+    REAL*8, DIMENSION(3,ns), INTENT(IN) :: x
+    REAL*8, DIMENSION(ns), INTENT(IN) :: L, W, strike, dip
+    REAL*8, DIMENSION(3,ns), INTENT(OUT) :: sv, dv, nv, xc
+    CALL ucy_check(ucy_reference_system_okada92(INT(ns, C_INT), x, L, W, strike, dip, sv, dv, nv, &
+                   xc), "computeReferenceSystemOkada92")
+  END SUBROUTINE computeReferenceSystemOkada92_gpu
+
+  !> computeReferenceSystemNikkhoo15 (nikkhoo15.f90:50-101, stuart97.f90:44-95)
+  SUBROUTINE computeReferenceSystemNikkhoo15_gpu(nVe, v, ns, i1, i2, i3, sv, dv, nv, xc)
+    INTEGER, INTENT(IN) :: nVe, ns
+    REAL*8, DIMENSION(3,nVe), INTENT(IN) :: v
+    INTEGER, DIMENSION(ns), INTENT(IN) :: i1, i2, i3
+    REAL*8, DIMENSION(3,ns), INTENT(OUT) :: sv, dv, nv, xc
+    CALL ucy_check(ucy_reference_system_nikkhoo15(INT(nVe, C_INT), v, INT(ns, C_INT), &
+                   INT(i1, C_INT), INT(i2, C_INT), INT(i3, C_INT), sv, dv, nv, xc), &
+                   "computeReferenceSystemNikkhoo15")
+  END SUBROUTINE computeReferenceSystemNikkhoo15_gpu
+
+  !> computeReferenceSystemVerticalStrainVolume (strainvolume.f90:135-163)
+  SUBROUTINE computeReferenceSystemVerticalStrainVolume_gpu(ns, x, L, W, strike, sv, dv, nv, xc)
+    INTEGER, INTENT(IN) :: ns
+    REAL*8, DIMENSION(3,ns), INTENT(IN) :: x
+    REAL*8, DIMENSION(ns), INTENT(IN) :: L, W, strike
+    REAL*8, DIMENSION(3,ns), INTENT(OUT) :: sv, dv, nv, xc
+    CALL ucy_check(ucy_reference_system_strainvolume(INT(ns, C_INT), x, L, W, strike, sv, dv, nv, &
+                   xc), "computeReferenceSystemVerticalStrainVolume")
+  END SUBROUTINE computeReferenceSystemVerticalStrainVolume_gpu
 
   !> initStateVector (ratestate.f90:548-610; variant 1: viscouscycles.f90:710-820)
   SUBROUTINE initStateVector_gpu(n, y, variant)
