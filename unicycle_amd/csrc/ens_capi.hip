@@ -197,6 +197,9 @@ static int load_initial(ucy_ensemble *E, double dt0)
         std::vector<double> hz(5 * kEnsCols, 0.0);
         EHIP(hipMemcpyAsync(G.hist, hz.data(), sizeof(double) * hz.size(), hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.errmax, zero.data(), sizeof(double) * kEnsCols, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> zrep(kMaxRep * kEnsCols, 0.0);
+        EHIP(hipMemcpyAsync(G.vmax_rep, zrep.data(), sizeof(double) * zrep.size(), hipMemcpyHostToDevice, c->stream));
+        EHIP(hipMemcpyAsync(G.errmax_rep, zrep.data(), sizeof(double) * zrep.size(), hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.need_rhs, one.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.accepted, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
         EHIP(hipMemcpyAsync(G.done, izero.data(), sizeof(int) * kEnsCols, hipMemcpyHostToDevice, c->stream));
@@ -407,6 +410,8 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
         for (int **v : ci)
             if (ealloc(E, v, kEnsCols)) return UCY_ERR_HIP;
         if (ealloc(E, &G.hist, 5 * kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.vmax_rep, kMaxRep * kEnsCols)) return UCY_ERR_HIP;
+        if (ealloc(E, &G.errmax_rep, kMaxRep * kEnsCols)) return UCY_ERR_HIP;
         G.slog = nullptr;
         G.slog_cap = 0;
         if (ealloc(E, &G.steps, kEnsCols)) return UCY_ERR_HIP;

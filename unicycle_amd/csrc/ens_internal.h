@@ -24,6 +24,7 @@ constexpr int kMaxGroups = 16;
 // consecutive groups ("slices") that share one K: the stage GEMM streams the
 // family's K once for all its slices
 constexpr int kMaxSlices = 4;
+constexpr int kMaxRep = 16;      // replicas of the per-instance maxima
 
 struct EnsFriction {             // per (element, instance), SoA arrays [e*64+inst]
     // a sig and b sig as products (the right-hand side uses a, b and sig
@@ -59,6 +60,11 @@ struct EnsGroup {
     EnsFriction fr;
     // per-instance control
     double *time, *h, *hnext, *hdid, *errmax, *vmax;
+    // [kMaxRep][64] partial maxima of the step-start vMax (start kernel) and
+    // of the attempt's error norm (finish kernel), one atomic per block into
+    // replica blockIdx % kMaxRep; the control kernel folds them into vmax,
+    // errmax (one address per instance would serialise thousands of atomics)
+    double *vmax_rep, *errmax_rep;
     double *hstart;          // h of the step's first attempt: the dt_try of yscal
     double *t_event;         // time of the first step starting with vMax > event_vmax (-1: none)
     double *hist;            // [5][64] proposal history: log10 vMax, h, log10 errmax of the

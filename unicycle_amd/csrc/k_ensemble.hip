@@ -1259,23 +1259,25 @@ __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restri
     const bool live = elem_coords(tab, g, e, inst);
     const EnsGroup &G = tab->g[g];
     double vel = 0.0;
+    bool need = false;
     if (live) {
         commit_elem(G, e, inst);
-        if (!G.done[inst] && G.need_rhs[inst]) {
-            *G.any_rhs = 1;
-            vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
-        }
+        need = !G.done[inst] && G.need_rhs[inst];
+        if (need) vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
     }
     // vMax per instance: the 4 elements of this block share instance
-    // columns, so reduce across the waves first (one atomic per block and
-    // instance; slip rates are positive, so their bit patterns order as they do)
+    // columns, so reduce across the waves first, then one atomic per block
+    // and instance into the block's replica (slip rates are positive, so
+    // their bit patterns order as they do)
     __shared__ unsigned long long red[256];
     red[threadIdx.x] = (unsigned long long)__double_as_longlong(vel);
-    __syncthreads();
+    if (__syncthreads_or(need) && threadIdx.x == 0) *G.any_rhs = 1;
     if (threadIdx.x < 64) {
         unsigned long long b = red[threadIdx.x];
         for (int w = 1; w < 4; w++) b = red[threadIdx.x + 64 * w] > b ? red[threadIdx.x + 64 * w] : b;
-        if (inst < G.ninst && b) atomicMax(reinterpret_cast<unsigned long long *>(&G.vmax[inst]), b);
+        if (inst < G.ninst && b)
+            atomicMax(reinterpret_cast<unsigned long long *>(
+                          &G.vmax_rep[(blockIdx.x % kMaxRep) * kEnsCols + inst]), b);
     }
 }
 
@@ -1345,7 +1347,8 @@ __device__ __forceinline__ double ens_stage_value(int stage, double h, double y0
 // saves 3/7 of this loop's traffic.
 __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, int stage)
 {
-    if (stage == 2 && e == 0) G.errmax[inst] = 0.0;
+    if (stage == 2 && e == 0)
+        for (int r = 0; r < kMaxRep; r++) G.errmax_rep[r * kEnsCols + inst] = 0.0;
     const double h = G.h[inst];
     constexpr int kRhsComps[4] = {2, 3, 5, 6};
 #pragma unroll
@@ -1432,7 +1435,8 @@ __global__ void __launch_bounds__(256) ens_post_finish_kernel(const EnsTable *__
         unsigned long long b = red[threadIdx.x];
         for (int w = 1; w < 4; w++) b = red[threadIdx.x + 64 * w] > b ? red[threadIdx.x + 64 * w] : b;
         if (inst < G.ninst && b)
-            atomicMax(reinterpret_cast<unsigned long long *>(&G.errmax[inst]), b);
+            atomicMax(reinterpret_cast<unsigned long long *>(
+                          &G.errmax_rep[(blockIdx.x % kMaxRep) * kEnsCols + inst]), b);
     }
 }
 
@@ -1463,6 +1467,20 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
     // ---- the rule (wave 0); an accepted step waits for the proposal
     if (wave == 0 && inst < G.ninst && !G.done[inst]) {
         G.attempts[inst] += 1;
+        // fold the replicas of the step-start vMax and the error norm (bit
+        // patterns: the maximum of non-negative values, NaN above +inf)
+        unsigned long long bv = 0ull, be = 0ull;
+#pragma unroll 4
+        for (int r = 0; r < kMaxRep; r++) {
+            const unsigned long long a = (unsigned long long)__double_as_longlong(
+                                             G.vmax_rep[r * kEnsCols + inst]),
+                                     c = (unsigned long long)__double_as_longlong(
+                                             G.errmax_rep[r * kEnsCols + inst]);
+            bv = a > bv ? a : bv;
+            be = c > be ? c : be;
+        }
+        G.vmax[inst] = __longlong_as_double((long long)bv);
+        G.errmax[inst] = __longlong_as_double((long long)be);
         const double raw = G.errmax[inst];
         G.accepted[inst] = 0;
         h = G.h[inst];
@@ -1587,6 +1605,7 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
         const bool event = G.t_event[inst] < 0.0 && G.vmax[inst] > tab->event_vmax;
         if (event) G.t_event[inst] = G.time[inst] - h;
         G.vmax[inst] = 0.0;
+        for (int r = 0; r < kMaxRep; r++) G.vmax_rep[r * kEnsCols + inst] = 0.0;
         if (tab->t_end <= G.time[inst] || (event && tab->stop_at_event)) {
             G.done[inst] = 1;
             atomicAdd(tab->all_done, 1);

@@ -582,48 +582,66 @@ void launch_gemv_post(hipStream_t st, const PlainDims &d, const double *K, const
 // v2) because a wave writes the next stage's entries of its element while
 // other waves still read the current ones.  A run ends with one commit.
 // ---------------------------------------------------------------------------
+// max of the bit patterns of a wavefront's non-negative values (NaN bits
+// order above +inf), every lane active
+__device__ __forceinline__ unsigned long long wave_max_bits(unsigned long long b)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long c = (unsigned long long)__shfl_xor((long long)b, o, 64);
+        b = c > b ? c : b;
+    }
+    return b;
+}
+
 __global__ void start_kernel(PlainDims d, const Friction *__restrict__ fr,
                              const Rheology *__restrict__ rh, RkBuffers b, Control *ctl)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e == 0) ctl->obs_pending = 0;
-    if (e >= d.np + d.nv) return;
-    const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
-    if (ctl->accepted) {
-        const int ns = elem_nstate(d, e);
-        for (int k = 0; k < ns; k++) b.y[so + k] = b.ycand[so + k];
+    const bool live = e < d.np + d.nv;
+    // the step-start vMax and eMax, reduced over the wavefront and then
+    // folded in with one atomic per wave (one per element would serialise
+    // on the two addresses)
+    double vel = 0.0, en = 0.0;
+    if (live) {
+        const int so = elem_state_off(d, e), vo = elem_v_off(d, e);
+        if (ctl->accepted) {
+            const int ns = elem_nstate(d, e);
+            for (int k = 0; k < ns; k++) b.y[so + k] = b.ycand[so + k];
+        }
+        // ctl->vmax was cleared by the controller that accepted the previous step
+        if (!ctl->done && ctl->need_rhs) {
+            if (d.ps && e >= d.np) {
+                // eMax of the step-start odefun (viscouscycles.f90 (2-D plane strain):962)
+                en = ps_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+            } else if (d.ps) {
+                vel = ps_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+            } else if (d.ap && e >= d.np) {
+                // eMax of the step-start odefun (viscouscycles.f90 (2-D antiplane):869)
+                en = ap_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+            } else if (d.ap) {
+                vel = ap_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+            } else if (e < d.np) {
+                // vMax of the step-start odefun (ratestate.f90:693), as exported every step
+                vel = patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
+            } else {
+                volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
+                // eMax of the step-start odefun (viscouscycles.f90:986), printed by export
+                const double *r = b.dydt + so;
+                en = sqrt((r[0] * r[0] + 2.0 * (r[1] * r[1]) + 2.0 * (r[2] * r[2]) +
+                           r[3] * r[3] + 2.0 * (r[4] * r[4]) + r[5] * r[5]) / 2.0);
+            }
+            if (b.v0) {
+                const int nvk = elem_nv(d, e);
+                for (int k = 0; k < nvk; k++) b.v0[vo + k] = b.v[vo + k];
+            }
+        }
     }
-    // ctl->vmax was cleared by the controller that accepted the previous step
-    if (ctl->done || !ctl->need_rhs) return;
-    if (d.ps && e >= d.np) {
-        // eMax of the step-start odefun (viscouscycles.f90 (2-D plane strain):962)
-        const double en = ps_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
-    } else if (d.ps) {
-        const double vel = ps_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
-    } else if (d.ap && e >= d.np) {
-        // eMax of the step-start odefun (viscouscycles.f90 (2-D antiplane):869)
-        const double en = ap_volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
-    } else if (d.ap) {
-        const double vel = ap_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
-    } else if (e < d.np) {
-        // vMax of the step-start odefun (ratestate.f90:693), as exported every step
-        const double vel = patch_prepass(b.y + so, fr[e], b.dydt + so, b.v + vo);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), dbits(vel));
-    } else {
-        volume_prepass(b.y + so, rh[e - d.np], b.dydt + so, b.v + vo);
-        // eMax of the step-start odefun (viscouscycles.f90:986), printed by export
-        const double *r = b.dydt + so;
-        const double en = sqrt((r[0] * r[0] + 2.0 * (r[1] * r[1]) + 2.0 * (r[2] * r[2]) +
-                                r[3] * r[3] + 2.0 * (r[4] * r[4]) + r[5] * r[5]) / 2.0);
-        atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), dbits(en));
-    }
-    if (b.v0) {
-        const int nvk = elem_nv(d, e);
-        for (int k = 0; k < nvk; k++) b.v0[vo + k] = b.v[vo + k];
+    const unsigned long long bv = wave_max_bits(dbits(vel)), be = wave_max_bits(dbits(en));
+    if ((threadIdx.x & 63) == 0) {
+        if (bv) atomicMax(reinterpret_cast<unsigned long long *>(&ctl->vmax), bv);
+        if (be) atomicMax(reinterpret_cast<unsigned long long *>(&ctl->emax), be);
     }
 }
 
