@@ -460,12 +460,13 @@ int ucy_ensemble_matvec(ucy_ensemble *ens, int group, const double *V, double *T
  * with every load a direct global->LDS load, K per wave and V per block,
  * 4 (6) or 3 (7) k-blocks deep; 8: 128 rows, K tiles per wave by direct
  * global->LDS loads and V in registers, 3 k-blocks deep, no barrier; 6-8
- * bf16+i8 only; 9: 128 rows in two register sets used in turn), and its
- * table of seconds
+ * bf16+i8 only; 9: 128 rows in two register sets used in turn; 10: 128
+ * rows per block of 4 waves, one K-family slice per wave, each K tile loaded
+ * and decoded once per block into LDS), and its table of seconds
  * per launch (UCY_GEMM_TUNE_ENTRIES entries: k-blocks 32, 64, 96, 128, 192,
  * 256, 512 of variant 0, then of 1, ...; -1 = not admissible); pointers may
  * be NULL */
-#define UCY_GEMM_TUNE_ENTRIES 70
+#define UCY_GEMM_TUNE_ENTRIES 77
 /* override the autotuner: k-blocks per split and split-bf16 variant (as in
  * ucy_ensemble_tuning; binary64 ensembles take variant 0 only) */
 int ucy_ensemble_set_tiling(ucy_ensemble *ens, int kb_per_split, int variant);

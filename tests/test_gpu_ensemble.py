@@ -67,8 +67,8 @@ def test_ensemble_matvec(engine_factory, precision):
     Ks = [orc.buildG(orc.Mesh(m.lambda_, m.mu, rect=m.geometry()), nthreads=8) for m in groups]
     # every GEMM variant of the autotuner, at split depths that give one and
     # several k-splits (the groups have 8 and 16 k-blocks)
-    variants = {PREC_F64: [0], PREC_BF16_I8: list(range(10))}.get(
-        precision, [0, 1, 2, 3, 4, 5, 9])   # 6-8: bf16+i8 only
+    variants = {PREC_F64: [0], PREC_BF16_I8: list(range(11))}.get(
+        precision, [0, 1, 2, 3, 4, 5, 9, 10])   # 6-8: bf16+i8 only
     tilings = [(kb, v) for v in variants for kb in (4, 64)]
     if precision in (PREC_SPLIT_BF16, PREC_MIXED):
         for v in (6, 7, 8):

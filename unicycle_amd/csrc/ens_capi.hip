@@ -232,9 +232,12 @@ static void set_split(EnsTable &T, int kbps, int variant)
         T.blk_gemm[f] = acc;
         T.blk_gemm64[f] = acc64;
         const int nri = (G.mp / 32 + rb - 1) / rb;
-        // LDS variants (one slice only): one block per (4 consecutive row items, k-split)
-        acc += ens_gemm_variant_lds(variant) ? ((nri + 3) / 4) * G.ksplit
-                                             : (nri * G.ksplit * S + 3) / 4;
+        // LDS variants (one slice only): one block per (4 consecutive row
+        // items, k-split); K-sharing variant: one block per (row item,
+        // k-split), its waves the slices
+        acc += ens_gemm_variant_lds(variant)      ? ((nri + 3) / 4) * G.ksplit
+               : ens_gemm_variant_kshare(variant) ? nri * G.ksplit
+                                                  : (nri * G.ksplit * S + 3) / 4;
         acc64 += ((G.mp / 64) * G.ksplit * S + 3) / 4;
     }
     T.blk_gemm[T.nfam] = acc;

@@ -113,11 +113,14 @@ struct EnsTable {
 // 8 = 128 rows, wave-shaped as 0-5, K tiles by direct global->LDS loads
 // into a ring private to the wave and V into a register ring, 3 k-blocks
 // deep, no barrier (bf16+i8 only); 9 = 128 rows in two register sets used
-// in turn, the loads of one k-block out while the other is multiplied
-constexpr int kGemmVariants = 10;
+// in turn, the loads of one k-block out while the other is multiplied;
+// 10 = 128 rows, block-shaped for K families: one slice per wave, each K
+// tile loaded and decoded once per block into LDS for the block's slices
+constexpr int kGemmVariants = 11;
 int ens_gemm_variant_rows(int variant);
 int ens_gemm_variant_lds(int variant);
 int ens_gemm_variant_dma(int variant);
+int ens_gemm_variant_kshare(int variant);
 
 // launchers (k_ensemble.hip)
 void ens_launch_split_K(hipStream_t st, const double *K, int m, int n, int mp, int np, uint16_t *tiles,

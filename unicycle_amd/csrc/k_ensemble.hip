@@ -809,6 +809,171 @@ __device__ __forceinline__ void gemm_wave_bf16_dmav(const EnsGroup &G, int item,
 }
 
 
+// Variant 10, for K families of several slices (more than 64 instances of
+// one geometry): the 4 waves of a block take one row item (4 32-row blocks)
+// and one k-split, wave w multiplying slice w.  Per k-block, wave w loads
+// 32-row K tile w, decodes it (the lo codes of bf16+i8) and writes its hi
+// and lo operands to LDS; after a barrier every wave multiplies all four
+// tiles with its own slice's V.  The K tile leaves memory and is decoded once
+// per block instead of once per slice: with 4 slices the GEMM carries 4x the
+// products per K byte and is bound by the matrix cores, whose issue the
+// other variants share with 4x the decode work.
+// Pipeline: LDS holds two k-blocks (2 x 4 tiles x hi, lo x 1 KiB); the raw
+// tiles of the two k-blocks after them are in flight in two register sets,
+// and V one k-block pair ahead in two more, so that every wait covers loads
+// issued a k-block pair earlier; one barrier per k-block.  A wave without a
+// slice (S < 4) or whose slice skips the matvec still loads, decodes and
+// meets every barrier.
+template <bool I8>
+__device__ __forceinline__ void gemm_block_bf16_kshare(const EnsTable *__restrict__ tab, int head,
+                                                       int S, int ri, int sp, int wave, int lane,
+                                                       int stage, u32x4 *lds)
+{
+    constexpr int RB = 4;
+    const EnsGroup &H = tab->g[head];
+    const int nrb = H.mp >> 5;
+    const int rb0 = ri * RB;
+    const int nvr = min(RB, nrb - rb0);   // 4, or 2 on the tail item
+    const int kb0 = sp * H.kb_per_split;
+    const int kb1 = min(kb0 + H.kb_per_split, H.nkb);
+    // this wave's slice, and whether its product is stored.  Every wave
+    // multiplies (a wave without a slice reads slice 0's V): the products
+    // stay free of branches, which would move the accumulators between
+    // register files at every join.  A block none of whose slices needs the
+    // matvec returns before the first barrier.
+    const bool mine = wave < S;
+    const EnsGroup &G = tab->g[head + (mine ? wave : 0)];
+    const bool active = mine && !(stage == 1 && !*G.any_rhs);
+    if (stage == 1) {
+        bool any = false;
+        for (int w = 0; w < S; w++) any = any || *tab->g[head + w].any_rhs;
+        if (!any) return;   // block-uniform
+    }
+    const bool dec = wave < nvr;   // this wave's K tile exists (else zeros)
+    floatx16 acc[RB][2];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc[a][b][i] = 0.0f;
+    const char *A = reinterpret_cast<const char *>(H.Ktiles) +
+                    (size_t)(rb0 + (dec ? wave : 0)) * H.nkb * kTileBytes<I8>;
+    const u32x4 *Bv = reinterpret_cast<const u32x4 *>(G.Vtiles) + lane;
+    u32x4 rh[2], rl[2], bv[2][4];
+    u32x2 rq[2];
+    auto loadk = [&](int kb, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        const char *t = A + (size_t)min(kb, kb1 - 1) * kTileBytes<I8>;
+        rh[s] = kload<true>(reinterpret_cast<const u32x4 *>(t) + lane);
+        if constexpr (I8)
+            rq[s] = kload<true>(reinterpret_cast<const u32x2 *>(t + 1024) + lane);
+        else
+            rl[s] = kload<true>(reinterpret_cast<const u32x4 *>(t) + 64 + lane);
+    };
+    auto loadv = [&](int kb, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        const u32x4 *v = Bv + (size_t)min(kb, kb1 - 1) * 256;
+#pragma unroll
+        for (int q = 0; q < 4; q++) bv[s][q] = *as_global(v + 64 * q);
+    };
+    // LDS: [buffer][tile][hi, lo][lane]
+    auto put = [&](int buf, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        u32x4 *d = lds + ((buf * RB + wave) * 2) * 64 + lane;
+        d[0] = rh[s];
+        if constexpr (I8)
+            d[64] = __builtin_bit_cast(u32x4, decode_lo8(rh[s], rq[s]));
+        else
+            d[64] = rl[s];
+    };
+    // a k-block past the range multiplies a zero V
+    auto mult = [&](int buf, bool live, auto set_c) {
+        constexpr int s = decltype(set_c)::value;
+        const u32x4 zero = {0u, 0u, 0u, 0u};
+        u32x4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = live ? bv[s][q] : zero;
+        const bf16x8 B[2][2] = {{__builtin_bit_cast(bf16x8, v[0]), __builtin_bit_cast(bf16x8, v[1])},
+                                {__builtin_bit_cast(bf16x8, v[2]), __builtin_bit_cast(bf16x8, v[3])}};
+#pragma unroll
+        for (int a = 0; a < RB; a++) {
+            const u32x4 *t = lds + ((buf * RB + a) * 2) * 64 + lane;
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, t[0]);
+            const bf16x8 Al = __builtin_bit_cast(bf16x8, t[64]);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][1], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, B[b][0], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, B[b][0], acc[a][b], 0, 0, 0);
+            }
+        }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    if (kb0 < kb1) {
+        // a wave without a K tile (tail item) leaves zeros in its LDS slots
+        if (!dec) {
+            const u32x4 zero = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 4; i++) lds[(((i >> 1) * RB + wave) * 2 + (i & 1)) * 64 + lane] = zero;
+        }
+        // LDS buffers 0, 1 <- kb0, kb0 + 1; then, in the order the loop
+        // issues them (so that the waits the compiler derives at the loop
+        // head hold for every iteration): V set 0 <- kb0, raw set 0 <-
+        // kb0 + 2, V set 1 <- kb0 + 1, raw set 1 <- kb0 + 3 (past the
+        // range: the last k-block again)
+        loadk(kb0, S0{});
+        loadk(kb0 + 1, S1{});
+        if (dec) {
+            put(0, S0{});
+            put(1, S1{});
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        loadv(kb0, S0{});
+        __builtin_amdgcn_sched_barrier(0);
+        loadk(kb0 + 2, S0{});
+        __builtin_amdgcn_sched_barrier(0);
+        loadv(kb0 + 1, S1{});
+        __builtin_amdgcn_sched_barrier(0);
+        loadk(kb0 + 3, S1{});
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        for (int kb = kb0; kb < kb1; kb += 2) {
+            // buffer 0 holds kb, buffer 1 kb + 1, raw set 0 is kb + 2 and
+            // set 1 kb + 3, V set 0 is kb and set 1 kb + 1
+            __builtin_amdgcn_sched_barrier(0);
+            mult(0, true, S0{});
+            __builtin_amdgcn_sched_barrier(0);
+            loadv(kb + 2, S0{});
+            __syncthreads();
+            if (dec) put(0, S0{});
+            loadk(kb + 4, S0{});
+            __builtin_amdgcn_sched_barrier(0);
+            mult(1, kb + 1 < kb1, S1{});
+            __builtin_amdgcn_sched_barrier(0);
+            loadv(kb + 3, S1{});
+            __syncthreads();
+            if (dec) put(1, S1{});
+            loadk(kb + 5, S1{});
+        }
+    }
+    if (!active) return;
+    float *T = G.Tpart + (size_t)sp * G.mp * kEnsCols;
+#pragma unroll
+    for (int a = 0; a < RB; a++) {
+        if (a >= nvr) break;
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = 32 * (rb0 + a) + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int col = 32 * b + (lane & 31);
+                T[(size_t)row * kEnsCols + col] = acc[a][b][i];
+            }
+    }
+}
+
 template <int RB, bool PIPE, bool TWO, bool I8, bool NT>
 __device__ __forceinline__ void gemm_wave_bf16(const EnsGroup &G, int item, int lane)
 {
@@ -836,12 +1001,16 @@ struct Bf16Variant {
                     // (gemm_wave_bf16_dmav); else V in a ring shared by the block
                     // (gemm_block_bf16_dma)
     bool two_sets;  // two register sets in turn (gemm_tile_bf16_2set)
+    bool kshare;    // block-shaped, one slice per wave, K tiles decoded once
+                    // per block into LDS (gemm_block_bf16_kshare)
 };
 constexpr Bf16Variant kBf16Variants[kGemmVariants] = {
-    {2, false, 0, 0, false, false}, {2, true, 0, 0, false, false}, {4, true, 0, 0, false, false},
-    {4, false, 4, 0, false, false}, {2, true, 8, 0, false, false}, {4, false, 0, 0, false, false},
-    {2, false, 0, 4, false, false}, {4, false, 0, 3, false, false}, {4, false, 0, 3, true, false},
-    {4, false, 0, 0, false, true}};
+    {2, false, 0, 0, false, false, false}, {2, true, 0, 0, false, false, false},
+    {4, true, 0, 0, false, false, false},  {4, false, 4, 0, false, false, false},
+    {2, true, 8, 0, false, false, false},  {4, false, 0, 0, false, false, false},
+    {2, false, 0, 4, false, false, false}, {4, false, 0, 3, false, false, false},
+    {4, false, 0, 3, true, false, false},  {4, false, 0, 0, false, true, false},
+    {4, false, 0, 0, false, false, true}};
 
 // LDS variants: the 4 waves of a block take 4 consecutive row items of one
 // k-split; the block stages the split's V tiles through LDS in chunks of CK
@@ -1026,7 +1195,16 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const EnsTable *__restri
     const int blk = blockIdx.x;
     const int f = group_of(tab->blk_gemm, tab->nfam, blk);
     const int head = tab->fam_head[f], S = tab->fam_slices[f];
-    if constexpr (CK > 0) {
+    if constexpr (kBf16Variants[V].kshare) {
+        // one block per (row item, k-split), one slice per wave
+        __shared__ u32x4 lds[2 * 4 * 2 * 64];
+        const EnsGroup &H = tab->g[head];
+        const int nri = bf16_row_items(H.mp, 4);
+        const int local = blk - tab->blk_gemm[f];
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        gemm_block_bf16_kshare<I8>(tab, head, S, local % nri, local / nri, wave, threadIdx.x & 63,
+                                   stage, lds);
+    } else if constexpr (CK > 0) {
         // one slice per family (the tuner admits the LDS variants only then)
         const EnsGroup &G = tab->g[head];
         if (stage == 1 && !*G.any_rhs) return;   // block-uniform
@@ -1099,6 +1277,11 @@ int ens_gemm_variant_lds(int v)
     return kBf16Variants[v].lds_chunk > 0 || (kBf16Variants[v].dma > 0 && !kBf16Variants[v].vreg);
 }
 
+int ens_gemm_variant_kshare(int v)
+{
+    return kBf16Variants[v].kshare;
+}
+
 int ens_gemm_variant_rows(int v)
 {
     return 32 * kBf16Variants[v].rb;
@@ -1132,7 +1315,8 @@ static void launch_gemm_bf16(hipStream_t st, const EnsTable *dtab, const EnsTabl
     case 6: hipLaunchKernelGGL((gemm_bf16_kernel<6, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 7: hipLaunchKernelGGL((gemm_bf16_kernel<7, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     case 8: hipLaunchKernelGGL((gemm_bf16_kernel<8, I8>), grid, dim3(256), 0, st, dtab, stage); break;
-    default: hipLaunchKernelGGL((gemm_bf16_kernel<9, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    case 9: hipLaunchKernelGGL((gemm_bf16_kernel<9, I8>), grid, dim3(256), 0, st, dtab, stage); break;
+    default: hipLaunchKernelGGL((gemm_bf16_kernel<10, I8>), grid, dim3(256), 0, st, dtab, stage); break;
     }
 }
 

@@ -202,7 +202,8 @@ class Ensemble:
         3 k-blocks deep, 8 = 128 rows with K per wave by direct
         global->LDS loads and V in registers, 3 k-blocks deep, no barrier
         (6-8: bf16+i8 only), 9 = 128 rows in two register sets used in
-        turn."""
+        turn, 10 = 128 rows per block of 4 waves, one K-family slice per
+        wave, each K tile loaded and decoded once per block into LDS."""
         kb, var = C.c_int(), C.c_int()
         t = np.full(_lib.GEMM_TUNE_ENTRIES, -1.0)
         self._chk(self._L.ucy_ensemble_tuning(self._h, C.byref(kb), C.byref(var), _p(t)))
