@@ -828,6 +828,26 @@ def bench_gemm(args):
                        "us_per_launch": [round(1e6 * x, 1) for x in tt]}}
 
 
+def bench_target_gemm(args):
+    """The stage GEMM of the 256-instance, 4096-patch target alone (one K
+    family of four slices, bf16+i8), launched directly (no graph):
+    autotuning launches, then 20 timed launches.  For profiler and counter
+    runs."""
+    from unicycle_amd import Engine, PREC_BF16_I8
+    eng = Engine(0)
+    ens, m = target_setup(eng, PREC_BF16_I8, shared=True)
+    t, b = ens.time_gemm(20)
+    kb, var, tt = ens.tuning()
+    flops = 2.0 * 3 * (3 * m.n) * (2 * m.n) * 256
+    ens.close()
+    return {"workload": "stage GEMM, 256 instances of one 4096-patch geometry (bf16+i8, one "
+                        "K family of 4 slices)", "avg_launch_us": 1e6 * t,
+            "bytes_per_launch": b, "GBps": b / t / 1e9,
+            "executed_TFLOPs": flops / t / 1e12, "executed_bf16_frac": flops / t / 2.5e15,
+            "tuning": {"kb_per_split": kb, "variant": var,
+                       "us_per_launch": [round(1e6 * x, 1) for x in tt]}}
+
+
 def bench_rhs(args):
     """The plain path's fused matvec + odefun post-pass kernel alone on the
     2048-patch thrust, launched directly (no graph), 100 timed launches."""
@@ -1050,7 +1070,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=400,
                     help="accepted steps of the plain path's per-attempt timing")
     ap.add_argument("--workload", default="ensemble",
-                    choices=["ensemble", "plain", "gemm", "rhs", "copy", "assembly", "attempt"])
+                    choices=["ensemble", "plain", "gemm", "target_gemm", "rhs", "copy",
+                             "assembly", "attempt"])
     ap.add_argument("--precision", default="split_bf16", choices=["f64", "split_bf16", "bf16_i8"],
                     help="stage-GEMM precision of --workload gemm")
     ap.add_argument("--no-cpu", action="store_true")
@@ -1072,7 +1093,8 @@ def main():
               f"multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "ensemble":
-        fn = {"plain": bench_plain, "gemm": bench_gemm, "rhs": bench_rhs,
+        fn = {"plain": bench_plain, "gemm": bench_gemm, "target_gemm": bench_target_gemm,
+              "rhs": bench_rhs,
               "copy": bench_copy, "attempt": bench_attempt,
               "assembly": lambda a: bench_assembly(a, cpu=not a.no_cpu)}[args.workload]
         print(json.dumps(fn(args)), flush=True)

@@ -463,10 +463,12 @@ int ucy_ensemble_matvec(ucy_ensemble *ens, int group, const double *V, double *T
  * bf16+i8 only; 9: 128 rows in two register sets used in turn; 10: 128
  * rows per block of 4 waves, one K-family slice per wave, each K tile loaded
  * and decoded once per block into LDS), and its table of seconds
- * per launch (UCY_GEMM_TUNE_ENTRIES entries: k-blocks 32, 64, 96, 128, 192,
- * 256, 512 of variant 0, then of 1, ...; -1 = not admissible); pointers may
- * be NULL */
-#define UCY_GEMM_TUNE_ENTRIES 77
+ * per launch (UCY_GEMM_TUNE_ENTRIES entries: k-blocks 32, 48, 64, 80, 96,
+ * 104, 128, 160, 192, 256, 512 of variant 0, then of 1, ...; -1 = not
+ * admissible); pointers may be NULL.  The tuner takes the candidate with the
+ * least launch time plus the post-passes' read of its split partial
+ * products charged at 4 TB/s. */
+#define UCY_GEMM_TUNE_ENTRIES 121
 /* override the autotuner: k-blocks per split and split-bf16 variant (as in
  * ucy_ensemble_tuning; binary64 ensembles take variant 0 only) */
 int ucy_ensemble_set_tiling(ucy_ensemble *ens, int kb_per_split, int variant);

@@ -160,7 +160,7 @@ _SIGS = {
 }
 
 EXPORTED = tuple(_SIGS)
-GEMM_TUNE_ENTRIES = 77   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
+GEMM_TUNE_ENTRIES = 121   # UCY_GEMM_TUNE_ENTRIES of include/ucy.h
 ENS_PHASES = ("start", "gemm 1", "post 1", "gemm 2", "post 2", "gemm 3", "post 3", "gemm 4",
               "post 4", "gemm 5", "post 5", "gemm 6", "finish", "control")  # UCY_ENS_PHASES
 

@@ -455,8 +455,10 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
     double best_t = 1e30;
     // split depths (k-blocks of 16 per split); the table below is
     // UCY_GEMM_TUNE_ENTRIES = kGemmVariants x kDepths entries
-    constexpr int kDepths = 7;
-    const int cands[kDepths] = {32, 64, 96, 128, 192, 256, 512};
+    // (48, 80 and 104 give the odd split counts: 5 splits of a 512-k-block
+    // K at 104, where 4 and 6 leave a second, half-empty round of blocks)
+    constexpr int kDepths = 11;
+    const int cands[kDepths] = {32, 48, 64, 80, 96, 104, 128, 160, 192, 256, 512};
     static_assert(kGemmVariants * kDepths == UCY_GEMM_TUNE_ENTRIES, "tuning table size");
     {
         hipEvent_t e0, e1;
@@ -499,8 +501,17 @@ extern "C" int ucy_ensemble_build(ucy_ensemble *E)
             EHIP(hipEventElapsedTime(&ms, e0, e1));
             const double t = 1e-3 * ms / 15;
             E->tune_seconds.push_back(t);
-            if (t < best_t) {
-                best_t = t;
+            // the choice also charges the post-pass's read of the partial
+            // products, one per split, at 4 TB/s (what the attempt measures:
+            // a deeper split's extra partials cost the post kernels about
+            // that), so that a split that only moves time from the GEMM into
+            // the post-pass is not taken
+            double pbytes = 0.0;
+            for (int gi = 0; gi < T.ngroups; gi++)
+                pbytes += (double)T.g[gi].ksplit * T.g[gi].mp * kEnsCols * (E->mode == 1 ? 8 : 4);
+            const double teff = t + pbytes / 4.0e12;
+            if (teff < best_t) {
+                best_t = teff;
                 best = kb;
                 best_v = v;
             }

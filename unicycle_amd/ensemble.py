@@ -193,8 +193,8 @@ class Ensemble:
 
     def tuning(self):
         """(k-blocks per split, bf16 GEMM variant, seconds per launch of the
-        candidates: k-blocks 32, 64, 96, 128, 192, 256, 512 of variant 0, then
-        1, ...).  Variants: 0 = 64 K rows per wave, 1 = 64 rows with loads
+        candidates: k-blocks 32, 48, 64, 80, 96, 104, 128, 160, 192, 256, 512
+        of variant 0, then 1, ...).  Variants: 0 = 64 K rows per wave, 1 = 64 rows with loads
         issued a k-block ahead, 2 = 128 rows, ahead, 3 = 128 rows with V
         staged in LDS in chunks of 4 k-blocks, 4 = 64 rows, ahead, V staged
         in chunks of 8, 5 = 128 rows, 6 = 64 rows and 7 = 128 rows with
