@@ -89,7 +89,11 @@ def test_shared_k_stepping_equals_separate_groups(engine_factory, precision):
     model = configs.thrust_fault(nx=8, nz=4, ellipse=None)
     d, y0 = _instances(model, 100, seed=7)
     runs = []
-    for shared in (True, False):
+    # (shared K, GEMM variant): the same split-K sums in every run; in
+    # bf16+i8 also the K-sharing variant 10 on the family, whose products
+    # run in the wave-shaped variants' order
+    cases = [(True, 0), (False, 0)] + ([(True, 10)] if precision == PREC_BF16_I8 else [])
+    for shared, variant in cases:
         ens = Ensemble(eng, precision)
         if shared:
             _add(ens, model, d, y0)
@@ -99,12 +103,13 @@ def test_shared_k_stepping_equals_separate_groups(engine_factory, precision):
             _add(ens, model, d, y0, slice(64, 100))
             assert ens.families == [[0], [1]]
         ens.build()
-        ens.set_tiling(4, 0)       # the same split-K sums in both
+        ens.set_tiling(4, variant)
         ens.reset()
         st = ens.run(1e30, max_attempts=1500)
         assert st.n_failed == 0
         runs.append([ens.state(g) for g in range(2)])
         ens.close()
-    for g in range(2):
-        for a, b in zip(runs[0][g], runs[1][g]):
-            assert np.array_equal(a, b)
+    for other in runs[1:]:
+        for g in range(2):
+            for a, b in zip(runs[0][g], other[g]):
+                assert np.array_equal(a, b)
