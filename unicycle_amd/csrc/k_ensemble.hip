@@ -1467,7 +1467,11 @@ __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restri
 
 // post-pass (ratestate.f90:747-771) writing stage derivative `stage`
 // (1 = step start, which also records the step's dt_try for yscal)
-__device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, int stage)
+// keep: the derivative's components 2-6 also go to keep[2..6], for the same
+// thread's next pass; store = false leaves memory alone (stage 6, whose
+// only reader is the same thread's finish_elem)
+__device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, int stage,
+                                          double *keep = nullptr, bool store = true)
 {
     const double *X = stage == 1 ? G.y : G.ytmp;
     double *out = G.ak[stage];
@@ -1482,11 +1486,21 @@ __device__ __forceinline__ void post_elem(const EnsGroup &G, int e, int inst, in
     const double d6 = (dtau - G.fr.bsig[f] * d5 * kELg10)
                       / (G.fr.asig[f] + G.fr.damping[f] * velocity) / kELg10;
     const double correction = G.fr.damping[f] * velocity * d6 * kELg10;
-    out[sidx(G, 2, e, inst)] = t[0] - correction * cr;
-    out[sidx(G, 3, e, inst)] = t[1] - correction * sr;
-    out[sidx(G, 4, e, inst)] = t[2];
-    out[sidx(G, 5, e, inst)] = d5;
-    out[sidx(G, 6, e, inst)] = d6;
+    const double d2 = t[0] - correction * cr, d3 = t[1] - correction * sr;
+    if (keep) {
+        keep[2] = d2;
+        keep[3] = d3;
+        keep[4] = t[2];
+        keep[5] = d5;
+        keep[6] = d6;
+    }
+    if (store) {
+        out[sidx(G, 2, e, inst)] = d2;
+        out[sidx(G, 3, e, inst)] = d3;
+        out[sidx(G, 4, e, inst)] = t[2];
+        out[sidx(G, 5, e, inst)] = d5;
+        out[sidx(G, 6, e, inst)] = d6;
+    }
     if (stage == 1 && e == 0) G.hstart[inst] = G.h[inst];
 }
 
@@ -1529,7 +1543,10 @@ __device__ __forceinline__ double ens_stage_value(int stage, double h, double y0
 // nothing (the derivatives do not depend on them; the fifth-order solution
 // and the error are built from the stored derivatives), and skipping them
 // saves 3/7 of this loop's traffic.
-__device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, int stage)
+// klast: components of k_(stage-1), the derivative the same thread's
+// post-pass has just formed (read from registers, not back from memory)
+__device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, int stage,
+                                           const double *klast)
 {
     if (stage == 2 && e == 0)
         for (int r = 0; r < kMaxRep; r++) G.errmax_rep[r * kEnsCols + inst] = 0.0;
@@ -1538,27 +1555,39 @@ __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, i
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const size_t i = sidx(G, kRhsComps[c], e, inst);
-        const double k1 = G.dydt[i];
-        const double k2 = stage > 2 ? G.ak[2][i] : 0.0;
-        const double k3 = stage > 3 ? G.ak[3][i] : 0.0;
-        const double k4 = stage > 4 ? G.ak[4][i] : 0.0;
-        const double k5 = stage > 5 ? G.ak[5][i] : 0.0;
+        const double kl = klast[kRhsComps[c]];
+        const double k1 = stage == 2 ? kl : G.dydt[i];
+        const double k2 = stage > 2 ? (stage == 3 ? kl : G.ak[2][i]) : 0.0;
+        const double k3 = stage > 3 ? (stage == 4 ? kl : G.ak[3][i]) : 0.0;
+        const double k4 = stage > 4 ? (stage == 5 ? kl : G.ak[4][i]) : 0.0;
+        const double k5 = stage > 5 ? kl : 0.0;
         G.ytmp[i] = ens_stage_value(stage, h, G.y[i], k1, k2, k3, k4, k5);
     }
     ens_prepass(G, G.ytmp, G.ak[stage], e, inst, false);
 }
 
 // after the GEMM of stage s (1..5): the post-pass of s, then the input and
-// pre-pass of stage s+1 (one launch per stage instead of two)
-__global__ void __launch_bounds__(256) ens_post_stage_kernel(const EnsTable *__restrict__ tab, int stage)
+// pre-pass of stage s+1 (one launch per stage instead of two).  k_s goes to
+// the stage input from registers; the kernel is held to 64 VGPRs (8 waves
+// per SIMD), which its loads need more than the compiler's deeper hoisting
+// (at 78 VGPRs the same code ran 8-10 % slower)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+ens_post_stage_kernel(const EnsTable *__restrict__ tab, int stage)
 {
     int g, e, inst;
     if (!elem_coords(tab, g, e, inst)) return;
     const EnsGroup &G = tab->g[g];
     if (G.done[inst]) return;
     // a retried attempt keeps the step-start derivative (ratestate.f90:313)
-    if (stage != 1 || G.need_rhs[inst]) post_elem(G, e, inst, stage);
-    stage_elem(G, e, inst, stage + 1);
+    double kv[7];
+    if (stage != 1 || G.need_rhs[inst]) {
+        post_elem(G, e, inst, stage, kv);
+    } else {
+        constexpr int kRhsComps[4] = {2, 3, 5, 6};
+#pragma unroll
+        for (int c = 0; c < 4; c++) kv[kRhsComps[c]] = G.dydt[sidx(G, kRhsComps[c], e, inst)];
+    }
+    stage_elem(G, e, inst, stage + 1, kv);
 }
 
 // fifth-order solution and embedded error of element e (ode45.f90:162-172);
@@ -1567,7 +1596,9 @@ __global__ void __launch_bounds__(256) ens_post_stage_kernel(const EnsTable *__r
 // of the step's first attempt (retries keep it, as the reference's RKQSm
 // keeps its yscal argument): the same arithmetic as a stored yscal, without
 // its 7-component write and read.
-__device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst)
+// k6: stage 6's derivative components 2-6 from the post-pass (its slip
+// rates, components 0 and 1, come from the stage-6 pre-pass in memory)
+__device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst, const double *k6v)
 {
     double emax = 0.0;
     const double h = G.h[inst];
@@ -1575,10 +1606,11 @@ __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst
     const double DC1 = 37.0 / 378.0 - 2825.0 / 27648.0, DC3 = 250.0 / 621.0 - 18575.0 / 48384.0,
                  DC4 = 125.0 / 594.0 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0,
                  DC6 = 512.0 / 1771.0 - 0.25;
+#pragma unroll
     for (int k = 0; k < 7; k++) {
         const size_t i = sidx(G, k, e, inst);
         const double k1 = G.dydt[i], k3 = G.ak[3][i], k4 = G.ak[4][i], k5 = G.ak[5][i],
-                     k6 = G.ak[6][i];
+                     k6 = k >= 2 ? k6v[k] : G.ak[6][i];
         double yr = (37.0 / 378.0) * k1;
         yr = yr + (250.0 / 621.0) * k3;
         yr = yr + (125.0 / 594.0) * k4;
@@ -1598,8 +1630,10 @@ __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst
 }
 
 // after the GEMM of stage 6: its post-pass, the fifth-order solution, the
-// embedded error and the per-instance max |yerr/yscal|
-__global__ void __launch_bounds__(256) ens_post_finish_kernel(const EnsTable *__restrict__ tab)
+// embedded error and the per-instance max |yerr/yscal|.  k_6's components
+// 2-6 stay in registers (nothing else reads them) and are not stored; held
+// to 64 VGPRs as the post-stage kernel (96 unheld: 115 against 105 µs)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) ens_post_finish_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
     const bool live = elem_coords(tab, g, e, inst);
@@ -1608,8 +1642,9 @@ __global__ void __launch_bounds__(256) ens_post_finish_kernel(const EnsTable *__
     if (blockIdx.x == tab->blk_elem[g] && threadIdx.x == 0) *G.any_rhs = 0;
     double emax = 0.0;
     if (live && !G.done[inst]) {
-        post_elem(G, e, inst, 6);
-        emax = finish_elem(G, e, inst);
+        double k6[7];
+        post_elem(G, e, inst, 6, k6, false);
+        emax = finish_elem(G, e, inst, k6);
     }
     // the 4 elements of this block share instance columns: reduce across waves
     __shared__ unsigned long long red[256];
