@@ -1570,7 +1570,7 @@ __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, i
 // pre-pass of stage s+1 (one launch per stage instead of two).  k_s goes to
 // the stage input from registers; the kernel is held to 64 VGPRs (8 waves
 // per SIMD), which its loads need more than the compiler's deeper hoisting
-// (at 78 VGPRs the same code ran 8-10 % slower)
+// (a form of it left at 78 VGPRs ran 8-10 % slower)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 ens_post_stage_kernel(const EnsTable *__restrict__ tab, int stage)
 {
@@ -1633,7 +1633,8 @@ __device__ __forceinline__ double finish_elem(const EnsGroup &G, int e, int inst
 // embedded error and the per-instance max |yerr/yscal|.  k_6's components
 // 2-6 stay in registers (nothing else reads them) and are not stored; held
 // to 64 VGPRs as the post-stage kernel (96 unheld: 115 against 105 µs)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) ens_post_finish_kernel(const EnsTable *__restrict__ tab)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+ens_post_finish_kernel(const EnsTable *__restrict__ tab)
 {
     int g, e, inst;
     const bool live = elem_coords(tab, g, e, inst);
