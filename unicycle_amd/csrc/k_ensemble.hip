@@ -1351,12 +1351,14 @@ __device__ __forceinline__ void put_split(const EnsGroup &G, int c, int inst, do
 // writes the slip rates to out[0..1] and the GEMM operand entries: binary64
 // v (mode 1); at the step start (base) v_0, kept, as split bf16 (modes 0, 3)
 // or binary64 (mode 2); at a stage the split-bf16 increment v - v_0
-__device__ __forceinline__ double ens_prepass(const EnsGroup &G, const double *X, double *out,
-                                              int e, int inst, bool base)
+// x2, x3, x6: the state's traction components 2, 3 and log10 V, passed in
+// registers by callers that have just formed them
+__device__ __forceinline__ double ens_prepass(const EnsGroup &G, double x2, double x3, double x6,
+                                              double *out, int e, int inst, bool base)
 {
     const size_t f = (size_t)e * kEnsCols + inst;
-    const double rake = atan2(X[sidx(G, 3, e, inst)], X[sidx(G, 2, e, inst)]);
-    const double velocity = exp(X[sidx(G, 6, e, inst)] * kELg10);
+    const double rake = atan2(x3, x2);
+    const double velocity = exp(x6 * kELg10);
     const double vs = velocity * cos(rake), vd = velocity * sin(rake);
     out[sidx(G, 0, e, inst)] = vs;
     out[sidx(G, 1, e, inst)] = vd;
@@ -1428,13 +1430,17 @@ __device__ __forceinline__ bool elem_coords(const EnsTable *tab, int &g, int &e,
 // ode45.f90:100-101), then the odefun pre-pass at y for instances that start
 // a new step.  The commit is idempotent (a finished instance keeps its
 // accepted flag and its ycand), and the run ends with one commit launch.
-__device__ __forceinline__ void commit_elem(const EnsGroup &G, int e, int inst)
+// returns whether it committed; then yv holds the committed state
+__device__ __forceinline__ bool commit_elem(const EnsGroup &G, int e, int inst, double yv[7])
 {
-    if (!G.accepted[inst]) return;
+    if (!G.accepted[inst]) return false;
+#pragma unroll
     for (int k = 0; k < 7; k++) {
         const size_t i = sidx(G, k, e, inst);
-        G.y[i] = G.ycand[i];
+        yv[k] = G.ycand[i];
+        G.y[i] = yv[k];
     }
+    return true;
 }
 
 __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restrict__ tab)
@@ -1445,9 +1451,17 @@ __global__ void __launch_bounds__(256) ens_start_kernel(const EnsTable *__restri
     double vel = 0.0;
     bool need = false;
     if (live) {
-        commit_elem(G, e, inst);
+        double yv[7];
+        const bool fresh = commit_elem(G, e, inst, yv);
         need = !G.done[inst] && G.need_rhs[inst];
-        if (need) vel = ens_prepass(G, G.y, G.dydt, e, inst, true);
+        if (need) {
+            if (!fresh) {
+                yv[2] = G.y[sidx(G, 2, e, inst)];
+                yv[3] = G.y[sidx(G, 3, e, inst)];
+                yv[6] = G.y[sidx(G, 6, e, inst)];
+            }
+            vel = ens_prepass(G, yv[2], yv[3], yv[6], G.dydt, e, inst, true);
+        }
     }
     // vMax per instance: the 4 elements of this block share instance
     // columns, so reduce across the waves first, then one atomic per block
@@ -1552,6 +1566,7 @@ __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, i
         for (int r = 0; r < kMaxRep; r++) G.errmax_rep[r * kEnsCols + inst] = 0.0;
     const double h = G.h[inst];
     constexpr int kRhsComps[4] = {2, 3, 5, 6};
+    double yt[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const size_t i = sidx(G, kRhsComps[c], e, inst);
@@ -1561,9 +1576,10 @@ __device__ __forceinline__ void stage_elem(const EnsGroup &G, int e, int inst, i
         const double k3 = stage > 3 ? (stage == 4 ? kl : G.ak[3][i]) : 0.0;
         const double k4 = stage > 4 ? (stage == 5 ? kl : G.ak[4][i]) : 0.0;
         const double k5 = stage > 5 ? kl : 0.0;
-        G.ytmp[i] = ens_stage_value(stage, h, G.y[i], k1, k2, k3, k4, k5);
+        yt[c] = ens_stage_value(stage, h, G.y[i], k1, k2, k3, k4, k5);
+        G.ytmp[i] = yt[c];
     }
-    ens_prepass(G, G.ytmp, G.ak[stage], e, inst, false);
+    ens_prepass(G, yt[0], yt[1], yt[3], G.ak[stage], e, inst, false);
 }
 
 // after the GEMM of stage s (1..5): the post-pass of s, then the input and
@@ -1864,7 +1880,8 @@ __global__ void __launch_bounds__(256) ens_commit_kernel(const EnsTable *__restr
 {
     int g, e, inst;
     if (!elem_coords(tab, g, e, inst)) return;
-    commit_elem(tab->g[g], e, inst);
+    double yv[7];
+    commit_elem(tab->g[g], e, inst, yv);
 }
 
 void ens_launch_start(hipStream_t st, const EnsTable *dtab, const EnsTable &h)
