@@ -1022,10 +1022,10 @@ def bench_ensemble(args):
             out["cpu_baseline"] = cb
     # ---- optional sections under the wall budget
     if not args.no_target and pl is not None:
-        r = section(out, "north_star_target", budget, 90.0,
+        r = section(out, "target_ensemble", budget, 90.0,
                     lambda: bench_target(args, eng, net, 1e6 * pl[0][2]))
         if r is not None:
-            out["north_star_target"] = r
+            out["target_ensemble"] = r
     bar_s = out["section_seconds"].get("headline", 60.0) - (head["wall"] if head else 0.0)
     per_variant = max(20.0, bar_s) + 4.0 * R * 2.5e-3 + 10.0
     extra = ["bf16_i8", "split_bf16", "mixed"] if not args.no_variants else []
@@ -1089,7 +1089,7 @@ def main():
                     help="plain-integrator steps per training instance")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X and has no "
+        print(f"bench.py measures one GPU: the engine runs on a single MI355X and has no "
               f"multi-GPU path (--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "ensemble":
