@@ -8,6 +8,8 @@ waves of a block take the slices of one row item and k-split).
   variant the shared form admits (with bf16+i8 K, the wave-shaped
   direct-to-LDS variants too); the variants that stage one V per block are
   refused.
+- four slices: the K-sharing variant with every wave busy, bit for bit
+  against the wave-shaped variant.
 - stepping: a family of two slices runs the same arithmetic as the same
   instances added as two separate groups (each with its own copy of K) on
   the same tiling, so every instance's state, time and counters agree bit
@@ -113,3 +115,34 @@ def test_shared_k_stepping_equals_separate_groups(engine_factory, precision):
         for g in range(2):
             for a, b in zip(runs[0][g], other[g]):
                 assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_BF16_I8])
+def test_kshare_four_slices(engine_factory, precision):
+    """A full family of four slices (256 instances): every wave of the
+    K-sharing variant multiplies (none idle, as with fewer slices), its
+    product equals the wave-shaped 128-row variant's bit for bit, slice by
+    slice, at one and at several k-splits (a 16 x 8 thrust: 16 k-blocks;
+    384 rows, three row items, the last of two 32-row blocks), and K V
+    within the precision's bound."""
+    eng = engine_factory()
+    model = configs.thrust_fault(nx=16, nz=8, ellipse=None)
+    d, y0 = _instances(model, 256, seed=11)
+    ens = Ensemble(eng, precision)
+    assert _add(ens, model, d, y0) == 0
+    assert ens.families == [[0, 1, 2, 3]]
+    ens.build()
+    K = orc.buildG(orc.Mesh(model.lambda_, model.mu, rect=model.geometry()), nthreads=8)
+    rng = np.random.default_rng(3)
+    V = [rng.standard_normal((K.shape[1], 64)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], 64))
+         for _ in range(4)]
+    for kb in (4, 64):
+        got = {}
+        for v in (5, 10):
+            ens.set_tiling(kb, v)
+            got[v] = [ens.matvec(g, V[g]) for g in range(4)]
+        for g in range(4):
+            assert np.array_equal(got[10][g], got[5][g]), (kb, g)
+            err = np.abs(got[10][g] - K @ V[g])
+            assert np.all(err <= 2.0 ** -13 * (np.abs(K) @ np.abs(V[g]))), (kb, g)
+    ens.close()
