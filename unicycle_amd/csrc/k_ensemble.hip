@@ -1687,10 +1687,13 @@ ens_post_finish_kernel(const EnsTable *__restrict__ tab)
 // loads per lane off the attempt's critical path.
 __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
 {
-    __shared__ float sP[kMlpParams];
+    // parameters, inputs [input][instance], hidden layers [instance][unit]
+    // (rows padded by 4 floats: a lane's 16-byte reads of its own row fall
+    // in distinct banks across 16 lanes)
+    __shared__ __attribute__((aligned(16))) float sP[kMlpParams + 3];
     __shared__ float sX[kMlpIn][64];
-    __shared__ float sH[kMlpH][64];
-    __shared__ float sT[kMlpH][64];
+    __shared__ __attribute__((aligned(16))) float sH[64][kMlpH + 4];
+    __shared__ __attribute__((aligned(16))) float sT[64][kMlpH + 4];
     const int blk = blockIdx.x;
     const int g = group_of(tab->blk_inst, tab->ngroups, blk);
     const EnsGroup &G = tab->g[g];
@@ -1699,6 +1702,24 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
     const double SAFETY = 0.7, PGROW = -0.2, PSHRNK = -0.15, ERRCON = 1.89e-4;
     double *hist = G.hist;
     bool capcheck = false, finish = false, want = false;
+    // the network's parameters into LDS, all of a thread's loads issued at
+    // once and before the rule, whose latency they overlap (a strided copy
+    // loop would wait for each load in turn); the barrier before the
+    // network makes them visible
+    if (tab->mlp) {
+        constexpr int kPer = (kMlpParams + 255) / 256;
+        float w[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const int i = threadIdx.x + 256 * r;
+            w[r] = i < kMlpParams ? tab->mlp[i] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; r++) {
+            const int i = threadIdx.x + 256 * r;
+            if (i < kMlpParams) sP[i] = w[r];
+        }
+    }
     double h = 0.0, hnext = 0.0, errmax = 0.0;
     // ---- the rule (wave 0); an accepted step waits for the proposal
     if (wave == 0 && inst < G.ninst && !G.done[inst]) {
@@ -1778,24 +1799,43 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
         for (int i = 0; i < kMlpIn; i++) sX[i][lane] = 0.0f;
     }
     // ---- the network (all waves, every thread reaches the barriers)
+    // (each thread holds its instance's inputs and hidden row in registers
+    // and reads the weights, the same for every lane, 4 at a time; every
+    // sum runs in mlp_eval's order)
     if (tab->mlp && __syncthreads_or(want)) {
-        for (int i = threadIdx.x; i < kMlpParams; i += 256) sP[i] = tab->mlp[i];
-        __syncthreads();
+        float x[kMlpIn];
+#pragma unroll
+        for (int i = 0; i < kMlpIn; i++) x[i] = sX[i][lane];
         for (int jj = 0; jj < 16; jj++) {
             const int j = 16 * wave + jj;
             float a = sP[kMlpOffB1 + j];
 #pragma unroll
-            for (int i = 0; i < kMlpIn; i++) a = fmaf(sP[j * kMlpIn + i], sX[i][lane], a);
-            sH[j][lane] = tanhf(a);
+            for (int i = 0; i < kMlpIn; i++) a = fmaf(sP[j * kMlpIn + i], x[i], a);
+            sH[lane][j] = tanhf(a);
         }
         __syncthreads();
+        float hv[kMlpH];
+#pragma unroll
+        for (int j4 = 0; j4 < kMlpH / 4; j4++) {
+            const float4 t = reinterpret_cast<const float4 *>(&sH[lane][0])[j4];
+            hv[4 * j4] = t.x;
+            hv[4 * j4 + 1] = t.y;
+            hv[4 * j4 + 2] = t.z;
+            hv[4 * j4 + 3] = t.w;
+        }
         for (int kk = 0; kk < 16; kk++) {
             const int k = 16 * wave + kk;
             float a = sP[kMlpOffB2 + k];
-            const float *w = sP + kMlpOffW2 + k * kMlpH;
-#pragma unroll 16
-            for (int j = 0; j < kMlpH; j++) a = fmaf(w[j], sH[j][lane], a);
-            sT[k][lane] = tanhf(a);
+            const float4 *w4 = reinterpret_cast<const float4 *>(sP + kMlpOffW2 + k * kMlpH);
+#pragma unroll
+            for (int j4 = 0; j4 < kMlpH / 4; j4++) {
+                const float4 w = w4[j4];
+                a = fmaf(w.x, hv[4 * j4], a);
+                a = fmaf(w.y, hv[4 * j4 + 1], a);
+                a = fmaf(w.z, hv[4 * j4 + 2], a);
+                a = fmaf(w.w, hv[4 * j4 + 3], a);
+            }
+            sT[lane][k] = tanhf(a);
         }
         __syncthreads();
     }
@@ -1803,7 +1843,16 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
     if (finish) {
         if (want) {
             float y = sP[kMlpOffB3];
-            for (int k = 0; k < kMlpH; k++) y = fmaf(sP[kMlpOffW3 + k], sT[k][lane], y);
+            const float4 *t4 = reinterpret_cast<const float4 *>(&sT[lane][0]);
+            const float4 *w4 = reinterpret_cast<const float4 *>(sP + kMlpOffW3);
+#pragma unroll
+            for (int k4 = 0; k4 < kMlpH / 4; k4++) {
+                const float4 t = t4[k4], w = w4[k4];
+                y = fmaf(w.x, t.x, y);
+                y = fmaf(w.y, t.y, y);
+                y = fmaf(w.z, t.z, y);
+                y = fmaf(w.w, t.w, y);
+            }
             const MlpNorm &nm = tab->mlp_norm;
             const double lp = (double)(y * nm.yscale + nm.ymean);
             // within a factor 4 of the classic proposal and the classic
