@@ -1727,7 +1727,7 @@ __global__ void __launch_bounds__(256) ens_control_kernel(EnsTable *tab)
         // fold the replicas of the step-start vMax and the error norm (bit
         // patterns: the maximum of non-negative values, NaN above +inf)
         unsigned long long bv = 0ull, be = 0ull;
-#pragma unroll 4
+#pragma unroll
         for (int r = 0; r < kMaxRep; r++) {
             const unsigned long long a = (unsigned long long)__double_as_longlong(
                                              G.vmax_rep[r * kEnsCols + inst]),
