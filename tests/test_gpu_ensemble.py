@@ -92,6 +92,40 @@ def test_ensemble_matvec(engine_factory, precision):
 
 
 @pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_BF16_I8])
+def test_matvec_largest_group(engine_factory, precision):
+    """The stage GEMM at the uneven mix's largest group (8192 two-km patches,
+    K of 24576 x 16384, 1024 k-blocks of 16), on the tuner's choice and on
+    the wave-shaped 128-row variant at 128 k-blocks per split (8 splits),
+    against K V in binary64 with the K the plain path
+    assembles on the device from the same inputs: within 2^-13 sum |K||V| per
+    row for every column, with slip-rate deviations spanning twelve decades."""
+    eng = engine_factory()
+    ens, info = configs.ensemble_setup(eng, precision, groups=[(128, 64)])
+    model = info[0][0]
+    assert model.n == 8192
+    plain = engine_factory()
+    plain.set_elastic(model.lambda_, model.mu)
+    plain.set_rectangles(*model.geometry())
+    plain.set_friction(*model.friction())
+    plain.buildG()
+    K = plain.get_kernel()
+    assert K.shape == (3 * 8192, 2 * 8192)
+    rng = np.random.default_rng(8)
+    V = rng.standard_normal((K.shape[1], 64)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], 64))
+    want = K @ V
+    scale = np.abs(K) @ np.abs(V)
+    del K
+    kb0, v0, _ = ens.tuning()
+    for kb, v in [(kb0, v0), (128, 5)]:
+        ens.set_tiling(kb, v)
+        T = ens.matvec(0, V)
+        err = np.abs(T - want)
+        assert np.all(err <= 2.0 ** -13 * scale), (kb, v, (err / scale).max())
+        assert (err / scale).max() > 1e-9   # really reduced precision
+    ens.close()
+
+
+@pytest.mark.parametrize("precision", [PREC_SPLIT_BF16, PREC_BF16_I8])
 def test_stored_kernel_format(engine_factory, precision):
     """Every entry of the stored K, read back through the stage GEMM with
     unit slip-rate vectors (V = 1 has no lo part, the products are exact,
