@@ -146,3 +146,41 @@ def test_kshare_four_slices(engine_factory, precision):
             err = np.abs(got[10][g] - K @ V[g])
             assert np.all(err <= 2.0 ** -13 * (np.abs(K) @ np.abs(V[g]))), (kb, g)
     ens.close()
+
+
+def test_kshare_target_size(engine_factory):
+    """The 256-instance target's shape: one 64 x 64 thrust (4096 patches, K of
+    12288 x 8192) as a family of four slices in bf16+i8.  The K-sharing
+    variant at the tuner's depth for it (104 k-blocks: 5 splits) and at 64
+    (8 splits) equals the wave-shaped 128-row variant bit for bit, every
+    slice, and each slice's product is within 2^-13 sum |K||V| per row of
+    K V in binary64 with the K the plain path assembles from the same
+    inputs."""
+    eng = engine_factory()
+    model = configs.thrust_fault(nx=64, nz=64, ellipse=None)
+    plain = engine_factory()
+    plain.set_elastic(model.lambda_, model.mu)
+    plain.set_rectangles(*model.geometry())
+    plain.set_friction(*model.friction())
+    y_base = plain.initStateVector()
+    plain.buildG()
+    K = plain.get_kernel()
+    d = configs.ensemble_instances(model, 256, seed=12)
+    ens = Ensemble(eng, PREC_BF16_I8)
+    assert ens.add_group(model, d["a"], d["b"], d["sig"], d["L"], np.tile(y_base, (256, 1))) == 0
+    assert ens.families == [[0, 1, 2, 3]]
+    ens.build()
+    rng = np.random.default_rng(13)
+    V = [rng.standard_normal((K.shape[1], 64)) * 10.0 ** rng.uniform(-12, 0, (K.shape[1], 64))
+         for _ in range(4)]
+    want = [K @ v for v in V]
+    scale = [np.abs(K) @ np.abs(v) for v in V]
+    for kb in (104, 64):
+        got = {}
+        for v in (5, 10):
+            ens.set_tiling(kb, v)
+            got[v] = [ens.matvec(g, V[g]) for g in range(4)]
+        for g in range(4):
+            assert np.array_equal(got[10][g], got[5][g]), (kb, g)
+            assert np.all(np.abs(got[10][g] - want[g]) <= 2.0 ** -13 * scale[g]), (kb, g)
+    ens.close()
